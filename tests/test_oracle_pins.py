@@ -1,0 +1,237 @@
+"""Pins of the CPU FP64 oracle against the reference's own trace, closed forms,
+brute force and independent library routines (numpy FFT, numpy.linalg)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# --- the reference's shipped 15-step trace (standard_output.txt:88-104) -----------------
+@pytest.mark.parametrize("dealias", [0, 1])
+def test_trace_matches_reference_stdout(dealias):
+    fx = json.load(open(os.path.join(GOLDEN, "standard_output_trace.json")))
+    o = orc.Oracle(nx=32, ny=32, nz=17, xlen=2 * np.pi, ylen=np.pi, re=500.0, dt=0.002, u_centerline=1.5,
+                   flow_control_method=1, target_pressure_gradient=0.006, use_dealias_23_rule=dealias,
+                   enable_perturbations=1, perturbation_kind=0, perturbation_amplitude=0.01)
+    o.init_fields()
+    for row in fx["rows"]:
+        o.step(1)
+        d = o.diagnostics()
+        assert d["step"] == row["step"]
+        got = {k: "%.4E" % d[f] for k, f in (("umax", "umax"), ("vmax", "vmax"), ("wmax", "wmax"),
+                                              ("E", "energy"), ("dPdx", "pressure_gradient"))}
+        want = {k: "%.4E" % float(row[k]) for k in got}
+        assert got == want, f"step {row['step']}"
+
+
+def test_initial_perturbation_scale_matches_reference():
+    # standard_output.txt:45: "Scaled from max= 2.0000E+00"; :53: max |u'|, |v'| = 1e-2, 5e-3
+    u, v, w = orc.initial_field(nx=32, ny=32, nz=17, xlen=2 * np.pi, ylen=np.pi, u_centerline=0.0,
+                                enable_perturbations=1, perturbation_amplitude=0.01)
+    assert "%.4E" % np.abs(u).max() == "1.0000E-02"
+    assert "%.4E" % np.abs(v).max() == "5.0000E-03"
+    assert np.all(w == 0.0)
+
+
+# --- LGL nodes, weights, differentiation matrix (lgl_module.f90) -------------------------
+def test_lgl_closed_forms():
+    x, w = orc.lgl(3)
+    np.testing.assert_allclose(x, [-1, 0, 1], atol=1e-16)
+    np.testing.assert_allclose(w, [1 / 3, 4 / 3, 1 / 3], rtol=1e-15)
+    x, w = orc.lgl(5)
+    r = math.sqrt(3 / 7)
+    np.testing.assert_allclose(x, [-1, -r, 0, r, 1], atol=1e-15)
+    np.testing.assert_allclose(w, [1 / 10, 49 / 90, 32 / 45, 49 / 90, 1 / 10], rtol=1e-14)
+
+
+@pytest.mark.parametrize("n", [17, 33, 65, 129])
+def test_lgl_quadrature_exact(n):
+    x, w = orc.lgl(n)
+    assert abs(w.sum() - 2.0) < 1e-13
+    for p in range(0, 2 * n - 2, 2):                 # exact for degree <= 2n-3
+        assert abs(np.dot(w, x ** p) - 2.0 / (p + 1)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [17, 33])
+def test_d1_exact_on_polynomials(n):
+    x, _ = orc.lgl(n)
+    D = orc.diff_matrix(n)
+    for p in range(1, n):
+        got = D @ (x ** p)
+        want = p * x ** (p - 1)
+        assert np.max(np.abs(got - want)) < 1e-10 * n * n * p, p
+    # d2/dz2 of 1.5(1-z^2) = -3 (debug_flow_analysis.f90:59-67)
+    np.testing.assert_allclose(D @ (D @ (1.5 * (1 - x ** 2))), -3.0, atol=1e-9)
+
+
+@pytest.mark.parametrize("n", [17, 33, 49, 57, 65, 129])
+def test_d1_denominator_threshold(n):
+    # lgl_module.f90:254: D(i,j) = 0 when |prod_{k!=j}(x_j - x_k)| <= 1e-15.  Brute force the
+    # products independently and compare which entries survive.
+    x, _ = orc.lgl(n)
+    D = orc.diff_matrix(n)
+    prods = np.array([np.prod([x[j] - x[k] for k in range(n) if k != j]) for j in range(n)])
+    for j in range(n):
+        col = np.delete(D[:, j], j)
+        if abs(prods[j]) > 1e-15:
+            assert np.all(col != 0.0)
+        else:
+            assert np.all(col == 0.0)
+    if n >= 65:                                      # the reference's D1 is diagonal here
+        assert np.count_nonzero(D - np.diag(np.diag(D))) == 0
+
+
+# --- wavenumbers and mask ----------------------------------------------------------------
+def test_wavenumbers_closed_form():
+    nx, ny, xlen, ylen = 32, 24, 4 * np.pi, 2 * np.pi
+    xw, yw = orc.wavenumbers(nx, ny, xlen, ylen)
+    np.testing.assert_array_equal(xw, np.arange(nx // 2 + 1) * (2 * np.pi / xlen))
+    k = np.array([j if j <= ny // 2 else j - ny for j in range(ny)], dtype=float)
+    np.testing.assert_allclose(yw, k * 2.0 * np.pi / ylen, rtol=1e-15)
+    assert yw[ny // 2] > 0                            # Nyquist carries +ny/2
+
+
+@pytest.mark.parametrize("nx,kxm", [(32, 10), (64, 21), (96, 32), (128, 42), (192, 64)])
+def test_dealias_mask_bounds(nx, kxm):
+    ny = nx
+    m = orc.dealias_mask(nx, ny)
+    kx = np.arange(nx // 2 + 1)[None, :]
+    ky = np.array([j if j <= ny // 2 else j - ny for j in range(ny)])[:, None]
+    want = (kx <= kxm) & (np.abs(ky) <= kxm)
+    np.testing.assert_array_equal(m.astype(bool), want)
+
+
+# --- 2D transforms (FFTW conventions) ----------------------------------------------------
+def _dft_r2c_brute(f):
+    ny, nx = f.shape
+    X = np.zeros((ny, nx // 2 + 1), dtype=complex)
+    y = np.arange(ny)[:, None]; x = np.arange(nx)[None, :]
+    for ky in range(ny):
+        for kx in range(nx // 2 + 1):
+            X[ky, kx] = np.sum(f * np.exp(-2j * np.pi * (kx * x / nx + ky * y / ny)))
+    return X
+
+
+def _c2r_formula(X, nx):
+    # full complex inverse along y, then half-complex inverse along x using Re of kx=0, nx/2
+    ny = X.shape[0]
+    g = np.fft.ifft(X, axis=0) * ny
+    c = np.full(nx // 2 + 1, 2.0); c[0] = 1.0; c[nx // 2] = 1.0
+    xs = np.arange(nx)
+    out = np.zeros((ny, nx))
+    for k in range(nx // 2 + 1):
+        out += c[k] * np.real(g[:, k][:, None] * np.exp(2j * np.pi * k * xs[None, :] / nx))
+    return out / (nx * ny)
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 6), (6, 10), (12, 4)])
+def test_fft_r2c_brute_force(nx, ny):
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal((2, ny, nx))
+    X = orc.fft_r2c(f)
+    for p in range(2):
+        np.testing.assert_allclose(X[p], _dft_r2c_brute(f[p]), atol=1e-12)
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (96, 64), (128, 128), (192, 96)])
+def test_fft_vs_numpy(nx, ny):
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((3, ny, nx))
+    X = orc.fft_r2c(f)
+    np.testing.assert_allclose(X, np.fft.rfft2(f), atol=1e-11 * math.sqrt(nx * ny))
+    back = orc.fft_c2r(X, nx)
+    assert np.max(np.abs(back - f)) < 1e-13
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 6), (32, 32)])
+def test_c2r_non_hermitian_semantics(nx, ny):
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((1, ny, nx // 2 + 1)) + 1j * rng.standard_normal((1, ny, nx // 2 + 1))
+    np.testing.assert_allclose(orc.fft_c2r(X, nx)[0], _c2r_formula(X[0], nx), atol=1e-14)
+
+
+def test_fft_roundtrip_known_answer():
+    # test_fft_roundtrip (DNS_pressure_BC_3D.f90:2035-2058): sin(2 alpha x) cos(3 beta y), err < 1e-12
+    nx, ny, xlen, ylen = 32, 32, 2 * np.pi, np.pi
+    x = np.arange(nx) * xlen / nx; y = np.arange(ny) * ylen / ny
+    f = np.sin(2 * (2 * np.pi / xlen) * x)[None, :] * np.cos(3 * (2 * np.pi / ylen) * y)[:, None]
+    back = orc.fft_c2r(orc.fft_r2c(f[None]), nx)[0]
+    assert np.max(np.abs(back - f)) < 1e-12
+
+
+# --- whole-step invariants ---------------------------------------------------------------
+def test_poiseuille_fixed_point():
+    # U = uc(1-z^2), G = 2 uc/Re is a steady solution; E = 4 uc^2/15 = 0.6 for uc = 1.5
+    uc, re = 1.5, 500.0
+    o = orc.Oracle(nx=16, ny=16, nz=17, re=re, u_centerline=uc, target_pressure_gradient=2 * uc / re,
+                   enable_perturbations=0, use_dealias_23_rule=1)
+    o.init_fields()
+    u0 = o.get_fields()[0]
+    o.step(20)
+    u, v, w, p = o.get_fields()
+    assert np.max(np.abs(u - u0)) < 1e-11
+    assert np.max(np.abs(v)) < 1e-12 and np.max(np.abs(w)) < 1e-12
+    assert abs(o.diagnostics()["energy"] - 0.6) < 1e-12
+
+
+def test_cached_factors_bitwise_equal_refactoring():
+    kw = dict(nx=16, ny=16, nz=17, re=180.0, flow_control_method=2, use_dealias_23_rule=1,
+              perturbation_kind=1, seed=7, noise_kmax=3, perturbation_amplitude=0.05)
+    a = orc.Oracle(refactor_every_step=0, **kw); a.init_fields(); a.step(4)
+    b = orc.Oracle(refactor_every_step=1, **kw); b.init_fields(); b.step(4)
+    for fa, fb in zip(a.get_fields(), b.get_fields()):
+        np.testing.assert_array_equal(fa, fb)
+
+
+def test_helmholtz_double_lifting_against_numpy_solve():
+    # Independent restatement of one viscous stage with numpy.fft + numpy.linalg.solve, with
+    # nonzero wall values so the double lifting of :1494-1502 is exercised.
+    nx, ny, nz, re, dt = 8, 8, 9, 100.0, 0.01
+    o = orc.Oracle(nx=nx, ny=ny, nz=nz, re=re, dt=dt, flow_control_method=1, target_pressure_gradient=0.02,
+                   use_dealias_23_rule=0, perturbation_kind=1, seed=3, noise_kmax=2, perturbation_amplitude=0.1)
+    o.init_fields()
+    rng = np.random.default_rng(5)
+    gpx = rng.standard_normal((2, ny, nx)) * 0.1; gpy = rng.standard_normal((2, ny, nx)) * 0.1
+    o.set_wall_gradients(gpx, gpy)
+    u, v, w, _ = o.get_fields()
+    o.compute_source_terms()
+    su = o.source_terms()[0]
+    o.viscous_step()
+    ustar = o.star_fields()[0]
+    z, wz = orc.lgl(nz)
+    D = orc.diff_matrix(nz)
+    f = 0.5 * dt / re
+    kx = np.fft.rfftfreq(nx, 1.0 / nx) * (2 * np.pi / (2 * np.pi))
+    ky = np.array([j if j <= ny // 2 else j - ny for j in range(ny)], float) * 2.0 * np.pi / np.pi
+    uh = np.fft.rfft2(u); sh = np.fft.rfft2(su)
+    lap = np.einsum("km,myx->kyx", D @ D, uh) - (kx[None, None, :] ** 2 + ky[None, :, None] ** 2) * uh
+    rh = uh + dt * sh + f * lap
+    b1 = np.fft.rfft2(-dt * gpx[0]); b2 = np.fft.rfft2(-dt * gpx[1])
+    A = D.T @ np.diag(wz) @ D
+    out = np.zeros_like(uh)
+    for j in range(ny):
+        for i in range(nx // 2 + 1):
+            H = f * A + np.diag((1 + f * (kx[i] ** 2 + ky[j] ** 2)) * wz)
+            I = slice(1, nz - 1)
+            rhs = wz[I] * rh[I, j, i] - 2 * H[I, 0] * b1[j, i] - 2 * H[I, nz - 1] * b2[j, i]
+            out[I, j, i] = np.linalg.solve(H[I, I], rhs)
+            out[0, j, i] = b1[j, i]; out[nz - 1, j, i] = b2[j, i]
+    want = np.fft.irfft2(out, s=(ny, nx))
+    np.testing.assert_allclose(ustar, want, atol=1e-12 * np.abs(want).max())
+
+
+def test_seeded_noise_is_solenoidal_and_scaled():
+    kw = dict(nx=32, ny=32, nz=33, xlen=4 * np.pi, ylen=2 * np.pi, u_centerline=0.0, perturbation_kind=1,
+              seed=12345, noise_kmax=8, perturbation_amplitude=0.05)
+    u, v, w = orc.initial_field(**kw)
+    assert abs(np.sqrt(u ** 2 + v ** 2).max() - 0.05) < 1e-15
+    o = orc.Oracle(**kw); o.init_fields()
+    assert o.max_divergence() < 1e-13
+    u2, _, _ = orc.initial_field(**kw)
+    np.testing.assert_array_equal(u, u2)
