@@ -1,0 +1,400 @@
+// setup.cpp -- host-side grid constants, LGL operators and per-mode solve bases.
+#include "setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dns {
+
+static const double kPi = 3.141592653589793115997963468544185161590576171875;  // 4*atan(1)
+
+// ---------------------------------------------------------------------------------------
+// LGL (lgl_module.f90:30-156, 211-263)
+// ---------------------------------------------------------------------------------------
+static void legendre_p(int n, double x, double& p) {
+    if (n == 0) { p = 1.0; return; }
+    if (n == 1) { p = x; return; }
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; k++) {
+        double p2 = (double(2 * k - 1) * x * p1 - double(k - 1) * p0) / double(k);
+        p0 = p1; p1 = p2;
+    }
+    p = p1;
+}
+
+static void legendre_pdd(int n, double x, double& p, double& dp, double& d2p) {
+    if (n == 0) { p = 1.0; dp = 0.0; d2p = 0.0; return; }
+    if (n == 1) { p = x; dp = 1.0; d2p = 0.0; return; }
+    double p0 = 1.0, dp0 = 0.0, e0 = 0.0, p1 = x, dp1 = 1.0, e1 = 0.0;
+    for (int k = 2; k <= n; k++) {
+        double c = double(2 * k - 1), d = double(k - 1), kk = double(k);
+        double p2 = (c * x * p1 - d * p0) / kk;
+        double dp2 = (c * (x * dp1 + p1) - d * dp0) / kk;
+        double e2 = (c * (x * e1 + 2.0 * dp1) - d * e0) / kk;
+        p0 = p1; p1 = p2; dp0 = dp1; dp1 = dp2; e0 = e1; e1 = e2;
+    }
+    p = p1; dp = dp1; d2p = e1;
+}
+
+void lgl_nodes_weights(int n, std::vector<double>& x, std::vector<double>& w) {
+    x.assign(n, 0.0); w.assign(n, 0.0);
+    if (n == 1) { w[0] = 2.0; return; }
+    if (n == 2) { x[0] = -1.0; x[1] = 1.0; w[0] = w[1] = 1.0; return; }
+    x[0] = -1.0; x[n - 1] = 1.0;
+    w[0] = 2.0 / double(n * (n - 1)); w[n - 1] = w[0];
+    for (int i = 1; i < n - 1; i++) {
+        double xi = -std::cos(kPi * double(i) / double(n - 1));
+        for (int it = 0; it < 100; it++) {
+            double xo = xi, p, dp, d2p;
+            legendre_pdd(n - 1, xi, p, dp, d2p);
+            xi = xo - dp / d2p;
+            if (std::fabs(xi - xo) < 1.0e-15) break;
+        }
+        x[i] = xi;
+        double p;
+        legendre_p(n - 1, xi, p);
+        w[i] = 2.0 / (double(n * (n - 1)) * p * p);
+    }
+}
+
+void differentiation_matrix(int n, const std::vector<double>& x, std::vector<double>& d) {
+    d.assign(size_t(n) * n, 0.0);
+    if (n == 1) return;
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) {
+            if (i == j) {
+                double s = 0.0;
+                for (int k = 0; k < n; k++) if (k != i) s += 1.0 / (x[i] - x[k]);
+                d[i + size_t(n) * i] = s;
+            } else {
+                double p1 = 1.0, p2 = 1.0;
+                for (int k = 0; k < n; k++) {
+                    if (k != i && k != j) p1 *= (x[i] - x[k]);
+                    if (k != j) p2 *= (x[j] - x[k]);
+                }
+                d[i + size_t(n) * j] = (std::fabs(p2) > 1.0e-15) ? p1 / p2 : 0.0;
+            }
+        }
+    }
+}
+
+void wavenumbers(int nx, int ny, double xlen, double ylen, std::vector<double>& xw, std::vector<double>& yw) {
+    double alpha = 2.0 * kPi / xlen;
+    xw.resize(nx / 2 + 1); yw.resize(ny);
+    for (int i = 0; i <= nx / 2; i++) xw[i] = double(i) * alpha;
+    for (int j = 0; j < ny; j++) {
+        if (j + 1 <= ny / 2 + 1) yw[j] = double(j) * 2.0 * kPi / ylen;
+        else yw[j] = double(j - ny) * 2.0 * kPi / ylen;
+    }
+}
+
+void dealias_mask(int nx, int ny, std::vector<unsigned char>& m) {
+    int nxh = nx / 2 + 1, kxm = (nx / 2) * 2 / 3, kym = (ny / 2) * 2 / 3;
+    m.assign(size_t(nxh) * ny, 0);
+    for (int j = 0; j < ny; j++) {
+        int ky = (j + 1 <= ny / 2 + 1) ? j : std::abs(j - ny);
+        for (int i = 0; i < nxh; i++) m[i + size_t(nxh) * j] = (i <= kxm && ky <= kym) ? 1 : 0;
+    }
+}
+
+SwizzledA swizzle_a(const std::vector<double>& M, int rows, int cols) {
+    SwizzledA s;
+    s.rows = rows; s.cols = cols;
+    s.rb = (rows + 15) / 16; s.ks = (cols + 3) / 4;
+    s.data.assign(size_t(s.rb) * s.ks * 64, 0.0);
+    for (int rb = 0; rb < s.rb; rb++)
+        for (int ks = 0; ks < s.ks; ks++)
+            for (int lane = 0; lane < 64; lane++) {
+                int r = rb * 16 + (lane & 15), c = 4 * ks + (lane >> 4);
+                if (r < rows && c < cols) s.data[(size_t(rb) * s.ks + ks) * 64 + lane] = M[size_t(r) * cols + c];
+            }
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------
+// symmetric eigen decomposition (cyclic Jacobi in long double)
+// ---------------------------------------------------------------------------------------
+typedef long double ld;
+
+static void jacobi_eig(int n, std::vector<ld>& A, std::vector<ld>& V, std::vector<ld>& lam) {
+    V.assign(size_t(n) * n, 0.0L);
+    for (int i = 0; i < n; i++) V[size_t(i) * n + i] = 1.0L;
+    ld scale = 0.0L;
+    for (size_t i = 0; i < A.size(); i++) scale = std::max(scale, std::fabs(A[i]));
+    for (int sweep = 0; sweep < 100; sweep++) {
+        ld off = 0.0L;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) off = std::max(off, std::fabs(A[size_t(p) * n + q]));
+        if (off <= scale * 1e-30L || off == 0.0L) break;
+        for (int p = 0; p < n; p++) {
+            for (int q = p + 1; q < n; q++) {
+                ld apq = A[size_t(p) * n + q];
+                if (apq == 0.0L) continue;
+                ld app = A[size_t(p) * n + p], aqq = A[size_t(q) * n + q];
+                ld th = (aqq - app) / (2.0L * apq);
+                ld t = (th >= 0 ? 1.0L : -1.0L) / (std::fabs(th) + std::sqrt(th * th + 1.0L));
+                ld c = 1.0L / std::sqrt(t * t + 1.0L), s = t * c;
+                for (int k = 0; k < n; k++) {            // A <- A J (columns p, q)
+                    ld akp = A[size_t(k) * n + p], akq = A[size_t(k) * n + q];
+                    A[size_t(k) * n + p] = c * akp - s * akq;
+                    A[size_t(k) * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {            // A <- J^T A (rows p, q)
+                    ld apk = A[size_t(p) * n + k], aqk = A[size_t(q) * n + k];
+                    A[size_t(p) * n + k] = c * apk - s * aqk;
+                    A[size_t(q) * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {            // V <- V J
+                    ld vkp = V[size_t(k) * n + p], vkq = V[size_t(k) * n + q];
+                    V[size_t(k) * n + p] = c * vkp - s * vkq;
+                    V[size_t(k) * n + q] = s * vkp + c * vkq;
+                }
+            }
+        }
+    }
+    lam.resize(n);
+    for (int i = 0; i < n; i++) lam[i] = A[size_t(i) * n + i];
+}
+
+// inverse of a dense matrix (row-major) by Gauss-Jordan with partial pivoting, long double
+static bool invert(int n, std::vector<ld> a, std::vector<ld>& inv) {
+    inv.assign(size_t(n) * n, 0.0L);
+    for (int i = 0; i < n; i++) inv[size_t(i) * n + i] = 1.0L;
+    for (int c = 0; c < n; c++) {
+        int p = c;
+        for (int r = c + 1; r < n; r++) if (std::fabs(a[size_t(r) * n + c]) > std::fabs(a[size_t(p) * n + c])) p = r;
+        if (a[size_t(p) * n + c] == 0.0L) return false;
+        if (p != c)
+            for (int k = 0; k < n; k++) { std::swap(a[size_t(p) * n + k], a[size_t(c) * n + k]); std::swap(inv[size_t(p) * n + k], inv[size_t(c) * n + k]); }
+        ld piv = a[size_t(c) * n + c];
+        for (int k = 0; k < n; k++) { a[size_t(c) * n + k] /= piv; inv[size_t(c) * n + k] /= piv; }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            ld f = a[size_t(r) * n + c];
+            if (f == 0.0L) continue;
+            for (int k = 0; k < n; k++) { a[size_t(r) * n + k] -= f * a[size_t(c) * n + k]; inv[size_t(r) * n + k] -= f * inv[size_t(c) * n + k]; }
+        }
+    }
+    return true;
+}
+
+bool build_operators(int nz, double dt, double re, double theta, bool cn, Operators& op, std::string& err) {
+    if (nz < 3) { err = "nz must be >= 3"; return false; }
+    op.nz = nz; op.m = nz - 2;
+    const int m = op.m;
+    lgl_nodes_weights(nz, op.z, op.wz);
+    differentiation_matrix(nz, op.z, op.d1);
+    // d2 = matmul(d1, d1) (DNS_pressure_BC_3D.f90:801)
+    op.d2.assign(size_t(nz) * nz, 0.0);
+    for (int j = 0; j < nz; j++)
+        for (int i = 0; i < nz; i++) {
+            double s = 0.0;
+            for (int k = 0; k < nz; k++) s += op.d1[i + size_t(nz) * k] * op.d1[k + size_t(nz) * j];
+            op.d2[i + size_t(nz) * j] = s;
+        }
+    // stiffness A(k,n) = sum_m d1(m,k) d1(m,n) w_m (DNS_pressure_BC_3D.f90:1464-1471, 1688)
+    op.stiff.assign(size_t(nz) * nz, 0.0);
+    for (int n = 0; n < nz; n++)
+        for (int k = 0; k < nz; k++) {
+            double s = 0.0;
+            for (int q = 0; q < nz; q++) s += op.d1[q + size_t(nz) * k] * op.d1[q + size_t(nz) * n] * op.wz[q];
+            op.stiff[k + size_t(nz) * n] = s;
+        }
+    auto A = [&](int r, int c) -> ld { return (ld)op.stiff[r + size_t(nz) * c]; };
+    auto D1 = [&](int r, int c) -> ld { return (ld)op.d1[r + size_t(nz) * c]; };
+    auto D2 = [&](int r, int c) -> ld { return (ld)op.d2[r + size_t(nz) * c]; };
+    std::vector<ld> sw(nz), isw(nz);
+    for (int k = 0; k < nz; k++) { sw[k] = std::sqrt((ld)op.wz[k]); isw[k] = 1.0L / sw[k]; }
+
+    // D1 (row-major) for the z-derivative contraction
+    {
+        std::vector<double> rm(size_t(nz) * nz);
+        for (int r = 0; r < nz; r++) for (int c = 0; c < nz; c++) rm[size_t(r) * nz + c] = op.d1[r + size_t(nz) * c];
+        op.D1 = swizzle_a(rm, nz, nz);
+    }
+
+    // ---- Helmholtz ----
+    const ld f = (ld)theta * (ld)dt / (ld)re;
+    op.f = double(f);
+    {
+        std::vector<ld> At(size_t(m) * m), Q, lam;
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) At[size_t(i) * m + j] = isw[i + 1] * A(i + 1, j + 1) * isw[j + 1];
+        std::vector<ld> A0 = At;
+        jacobi_eig(m, At, Q, lam);
+        // reconstruction check
+        ld e = 0, s = 0;
+        for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) {
+            ld r = 0; for (int k = 0; k < m; k++) r += Q[size_t(i) * m + k] * lam[k] * Q[size_t(j) * m + k];
+            e = std::max(e, std::fabs(r - A0[size_t(i) * m + j])); s = std::max(s, std::fabs(A0[size_t(i) * m + j]));
+        }
+        op.helm_recon_err = double(s > 0 ? e / s : e);
+        const int K = m + nz;
+        std::vector<double> F(size_t(m) * K, 0.0), B(size_t(m) * m, 0.0);
+        op.hc1.assign(m, 0.0); op.hc2.assign(m, 0.0); op.hl.assign(m, 0.0);
+        for (int j = 0; j < m; j++) {
+            // T(j, i) = Q(i, j) sqrt(w_i)
+            for (int i = 0; i < m; i++) F[size_t(j) * K + i] = double(Q[size_t(i) * m + j] * sw[i + 1]);
+            if (cn) {
+                for (int c = 0; c < nz; c++) {          // f * (T D2_I)(j, c)
+                    ld acc = 0;
+                    for (int i = 0; i < m; i++) acc += Q[size_t(i) * m + j] * sw[i + 1] * D2(i + 1, c);
+                    F[size_t(j) * K + m + c] = double(f * acc);
+                }
+            }
+            ld c1 = 0, c2 = 0;
+            for (int i = 0; i < m; i++) {
+                c1 += Q[size_t(i) * m + j] * isw[i + 1] * A(i + 1, 0);
+                c2 += Q[size_t(i) * m + j] * isw[i + 1] * A(i + 1, nz - 1);
+            }
+            op.hc1[j] = double(-2.0L * f * c1);
+            op.hc2[j] = double(-2.0L * f * c2);
+            op.hl[j] = double(f * lam[j] + 1.0L);
+            for (int i = 0; i < m; i++) B[size_t(i) * m + j] = double(isw[i + 1] * Q[size_t(i) * m + j]);
+        }
+        op.hF = swizzle_a(F, m, K);
+        op.hB = swizzle_a(B, m, m);
+    }
+
+    // ---- Poisson ----
+    {
+        std::vector<ld> Af(size_t(nz) * nz), Q, lam;
+        for (int i = 0; i < nz; i++)
+            for (int j = 0; j < nz; j++) Af[size_t(i) * nz + j] = isw[i] * A(i, j) * isw[j];
+        std::vector<ld> A0 = Af;
+        jacobi_eig(nz, Af, Q, lam);
+        ld e = 0, s = 0;
+        for (int i = 0; i < nz; i++) for (int j = 0; j < nz; j++) {
+            ld r = 0; for (int k = 0; k < nz; k++) r += Q[size_t(i) * nz + k] * lam[k] * Q[size_t(j) * nz + k];
+            e = std::max(e, std::fabs(r - A0[size_t(i) * nz + j])); s = std::max(s, std::fabs(A0[size_t(i) * nz + j]));
+        }
+        op.pois_recon_err = double(s > 0 ? e / s : e);
+        const int K = 2 * nz;
+        std::vector<double> F(size_t(nz) * K, 0.0), B(size_t(nz) * nz, 0.0), DB(size_t(nz) * nz, 0.0);
+        op.pl.assign(nz, 0.0);
+        for (int j = 0; j < nz; j++) {
+            for (int i = 0; i < nz; i++) F[size_t(j) * K + i] = double(Q[size_t(i) * nz + j] * sw[i]);
+            for (int c = 0; c < nz; c++) {
+                ld acc = 0;
+                for (int i = 0; i < nz; i++) acc += Q[size_t(i) * nz + j] * sw[i] * D1(i, c);
+                F[size_t(j) * K + nz + c] = double(acc);
+            }
+            op.pl[j] = double(lam[j]);
+        }
+        for (int i = 0; i < nz; i++)
+            for (int j = 0; j < nz; j++) {
+                B[size_t(i) * nz + j] = double(isw[i] * Q[size_t(i) * nz + j]);
+                ld acc = 0;
+                for (int k = 0; k < nz; k++) acc += D1(i, k) * isw[k] * Q[size_t(k) * nz + j];
+                DB[size_t(i) * nz + j] = double(acc);
+            }
+        op.pF = swizzle_a(F, nz, K);
+        op.pB = swizzle_a(B, nz, nz);
+        op.pDB = swizzle_a(DB, nz, nz);
+        // mode (0,0): M0 = -(A_{1:,1:})^{-1}
+        std::vector<ld> a1(size_t(nz - 1) * (nz - 1)), inv;
+        for (int i = 0; i < nz - 1; i++) for (int j = 0; j < nz - 1; j++) a1[size_t(i) * (nz - 1) + j] = A(i + 1, j + 1);
+        if (!invert(nz - 1, a1, inv)) { err = "singular mode-(0,0) Poisson operator"; return false; }
+        op.m0.resize(inv.size());
+        for (size_t i = 0; i < inv.size(); i++) op.m0[i] = double(-inv[i]);
+    }
+    return true;
+}
+
+void twiddles(int n, std::vector<double>& tw) {
+    tw.resize(size_t(2) * n);
+    for (int k = 0; k < n; k++) {
+        // reduce by symmetry so that the angle argument stays in [0, pi/4]
+        ld ang = -2.0L * 3.14159265358979323846264338327950288L * (ld)k / (ld)n;
+        tw[2 * k] = double(std::cos(ang));
+        tw[2 * k + 1] = double(std::sin(ang));
+    }
+}
+
+bool radix_factors(int n, std::vector<int>& f) {
+    f.clear();
+    int m = n;
+    while (m % 4 == 0) { f.push_back(4); m /= 4; }
+    while (m % 2 == 0) { f.push_back(2); m /= 2; }
+    while (m % 3 == 0) { f.push_back(3); m /= 3; }
+    return m == 1;
+}
+
+// ---------------------------------------------------------------------------------------
+// initial fields
+// ---------------------------------------------------------------------------------------
+static uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static double u01(uint64_t& s) { return double(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+void initial_field(const InitSpec& p, std::vector<double>& u, std::vector<double>& v, std::vector<double>& w) {
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const size_t npl = size_t(nx) * ny, N = npl * nz;
+    std::vector<double> z, wz;
+    lgl_nodes_weights(nz, z, wz);
+    u.assign(N, 0.0); v.assign(N, 0.0); w.assign(N, 0.0);
+    for (int k = 0; k < nz; k++)
+        for (size_t q = 0; q < npl; q++) u[q + npl * k] = p.u_centerline * (1.0 - z[k] * z[k]);
+    if (p.enable_perturbations) {
+        std::vector<double> up(N, 0.0), vp(N, 0.0);
+        if (p.kind == 0) {
+            double kx = 2.0 * kPi * double(p.mode_x) / p.xlen;
+            double ky = 2.0 * kPi * double(p.mode_y) / p.ylen;
+            for (int k = 0; k < nz; k++) {
+                double wd = 1.0 - z[k] * z[k];
+                for (int j = 0; j < ny; j++) {
+                    double y = double(j) * p.ylen / double(ny);
+                    for (int i = 0; i < nx; i++) {
+                        double x = double(i) * p.xlen / double(nx);
+                        size_t n = size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * k);
+                        up[n] = -(ky * std::sin(kx * x) * std::cos(ky * y) * wd);
+                        vp[n] = kx * std::cos(kx * x) * std::sin(ky * y) * wd;
+                    }
+                }
+            }
+        } else {
+            uint64_t st = p.seed;
+            std::vector<double> u2(npl, 0.0), v2(npl, 0.0);
+            for (int mx = 0; mx <= p.kmax; mx++)
+                for (int my = 0; my <= p.kmax; my++) {
+                    if (mx == 0 && my == 0) continue;
+                    double ph1 = 2.0 * kPi * u01(st), ph2 = 2.0 * kPi * u01(st);
+                    double a = 1.0 / double(mx * mx + my * my);
+                    double kx = 2.0 * kPi * double(mx) / p.xlen, ky = 2.0 * kPi * double(my) / p.ylen;
+                    for (int j = 0; j < ny; j++) {
+                        double y = double(j) * p.ylen / double(ny);
+                        double sy = std::sin(ky * y + ph2), cy = std::cos(ky * y + ph2);
+                        for (int i = 0; i < nx; i++) {
+                            double x = double(i) * p.xlen / double(nx);
+                            double sx = std::sin(kx * x + ph1), cx = std::cos(kx * x + ph1);
+                            u2[i + size_t(nx) * j] += -(a * ky * sx * cy);
+                            v2[i + size_t(nx) * j] += a * kx * cx * sy;
+                        }
+                    }
+                }
+            for (int k = 0; k < nz; k++) {
+                double wd = 1.0 - z[k] * z[k];
+                for (size_t q = 0; q < npl; q++) { up[q + npl * k] = u2[q] * wd; vp[q + npl * k] = v2[q] * wd; }
+            }
+        }
+        double cm = 0.0;
+        for (size_t n = 0; n < N; n++) { double s = std::sqrt(up[n] * up[n] + vp[n] * vp[n] + 0.0 * 0.0); if (s > cm) cm = s; }
+        if (cm > 1.0e-12) {
+            double sf = p.amplitude / cm;
+            for (size_t n = 0; n < N; n++) { up[n] = up[n] * sf; vp[n] = vp[n] * sf; }
+        }
+        for (size_t q = 0; q < npl; q++) { up[q] = 0.0; vp[q] = 0.0; up[npl * (nz - 1) + q] = 0.0; vp[npl * (nz - 1) + q] = 0.0; }
+        for (size_t n = 0; n < N; n++) { u[n] = u[n] + up[n]; v[n] = v[n] + vp[n]; }
+    }
+    for (size_t q = 0; q < npl; q++) {
+        u[q] = v[q] = w[q] = 0.0;
+        u[npl * (nz - 1) + q] = v[npl * (nz - 1) + q] = w[npl * (nz - 1) + q] = 0.0;
+    }
+}
+
+}  // namespace dns

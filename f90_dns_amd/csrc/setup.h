@@ -1,0 +1,81 @@
+// setup.h -- host-side grid constants, LGL operators and per-mode solve bases.
+//
+// Everything here runs once per context on the host, in FP64 (the eigen bases in
+// 80-bit long double, rounded once), and is uploaded to HBM.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace dns {
+
+// LGL nodes and weights, Newton on P'_{n-1} from -cos(pi (i-1)/(n-1)), tolerance 1e-15,
+// at most 100 iterations (lgl_module.f90:30-80).
+void lgl_nodes_weights(int n, std::vector<double>& x, std::vector<double>& w);
+
+// Lagrange differentiation matrix, column-major D[i + n*j] = D(i,j), with the
+// reference's rule that an entry is zero when |prod_{k!=j}(x_j - x_k)| <= 1e-15
+// (lgl_module.f90:211-263).  For n >= 65 every off-diagonal entry is zero.
+void differentiation_matrix(int n, const std::vector<double>& x, std::vector<double>& d);
+
+// Wavenumbers of setup_spectral_3d (DNS_pressure_BC_3D.f90:769-796).
+void wavenumbers(int nx, int ny, double xlen, double ylen, std::vector<double>& xw, std::vector<double>& yw);
+
+// 2/3 dealias mask of apply_dealias_truncation (DNS_pressure_BC_3D.f90:1163-1193), 1 = kept.
+void dealias_mask(int nx, int ny, std::vector<unsigned char>& m);
+
+// Matrix in the lane order of v_mfma_f64_16x16x4f64's A operand: for row block rb and
+// k-step s, entry [(rb*KS + s)*64 + lane] = M(rb*16 + lane%16, 4*s + lane/16), zero outside
+// the rows x cols of M.  M is given row-major (rows x cols).
+struct SwizzledA {
+    int rows = 0, cols = 0, rb = 0, ks = 0;
+    std::vector<double> data;
+};
+SwizzledA swizzle_a(const std::vector<double>& rowmajor, int rows, int cols);
+
+// Operators of one grid and run (nz, dt, Re, theta):
+//  Helmholtz (interior unknowns I = 1..nz-2, m = nz-2), with Q L Q^T = W_I^{-1/2} A_II W_I^{-1/2}:
+//    y = T (alpha u_I + dt S_I) + f*(T D2_I) u + c1*bc1 + c2*bc2,  z = y / (hl + f k^2),
+//    x_I = B z, x_0 = bc1, x_{nz-1} = bc2,
+//    T = Q^T W_I^{1/2}, B = W_I^{-1/2} Q, c_{1,2} = -2 f Q^T W_I^{-1/2} A_{I,{0,nz-1}}, hl = f lambda + 1.
+//  Poisson (all nz unknowns), with Qf Lf Qf^T = W^{-1/2} A W^{-1/2}:
+//    y = Tf div + (Tf D1) w,  z = -y / (dt (lf + k^2)),  phi = Bf z,  D1 phi = (D1 Bf) z,
+//    Tf = Qf^T W^{1/2}, Bf = W^{-1/2} Qf.  Mode (0,0): phi_0 = 0, phi_{1:} = M0 b_{1:} with
+//    M0 = -(A_{1:,1:})^{-1} and b = w * (D1 w*) / dt.
+struct Operators {
+    int nz = 0, m = 0;
+    double f = 0.0;              // theta dt / Re
+    std::vector<double> z, wz, d1, d2, stiff;     // d1, d2, stiff column-major nz x nz
+    SwizzledA D1;                // nz x nz
+    SwizzledA hF;                // m x (m + nz): [T | f T D2_I]
+    SwizzledA hB;                // m x m
+    std::vector<double> hc1, hc2, hl;   // length m
+    SwizzledA pF;                // nz x 2nz: [Tf | Tf D1]
+    SwizzledA pB;                // nz x nz: Bf
+    SwizzledA pDB;               // nz x nz: D1 Bf
+    std::vector<double> pl;      // length nz
+    std::vector<double> m0;      // (nz-1) x (nz-1) row-major
+    double helm_recon_err = 0, pois_recon_err = 0;   // max |reconstruction - operator|, relative
+};
+bool build_operators(int nz, double dt, double re, double theta, bool crank_nicolson, Operators& op,
+                     std::string& err);
+
+// exp(-2 pi i k / n), k = 0..n-1, as interleaved (re, im)
+void twiddles(int n, std::vector<double>& tw);
+
+// Radix factors of n (4s, then 2, then 3); false if n has a prime factor > 3.
+bool radix_factors(int n, std::vector<int>& f);
+
+struct InitSpec {
+    int nx, ny, nz;
+    double xlen, ylen, u_centerline;
+    int enable_perturbations, kind, mode_x, mode_y;
+    double amplitude;
+    uint64_t seed;
+    int kmax;
+};
+// initialize_fields_3d + apply_solenoidal_perturbations (DNS_pressure_BC_3D.f90:909-962,3219-3265;
+// perturbation_analytical_simple.f90:24-155); kind 1 = seeded stream-function noise.
+void initial_field(const InitSpec& s, std::vector<double>& u, std::vector<double>& v, std::vector<double>& w);
+
+}  // namespace dns

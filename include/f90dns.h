@@ -1,0 +1,141 @@
+/* f90dns.h -- C ABI of the MI355X (gfx950) channel-DNS time step.
+ *
+ * Drop-in boundary for the 3D fractional step of DNS_pressure_BC_3D.f90.  The
+ * reference's stage subroutines are argument-less internal procedures of
+ * `program dns_3d` working on host-associated globals, and its FFT boundary is
+ * the Fortran module fftw3_dns_module.f90.  This header replaces both with plain
+ * C entry points (no C++ or torch types): a Fortran driver binds them with
+ * ISO_C_BINDING, any other host language with its own FFI.
+ *
+ * Conventions
+ *  - Physical fields are double, Fortran order f(i,j,k) at [i + nx*(j + ny*k)],
+ *    i = x (periodic, fastest), j = y (periodic), k = z (LGL, k = 0 at z = -1).
+ *  - Spectral planes are complex(nx/2+1, ny) stored as interleaved (re, im) double
+ *    pairs, kx fastest, FFTW's r2c layout.
+ *  - Every function returns 0 on success or a DNS_E* code; dns_last_error() gives
+ *    the message of the last failure on that context (the reference stops the
+ *    program instead: DNS_pressure_BC_3D.f90:837-875,1506,1715-1718).
+ *  - All device work of a context runs in order on the context's HIP stream;
+ *    functions that return host data synchronise that stream.
+ */
+#ifndef F90DNS_H
+#define F90DNS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DNS_OK = 0,
+    DNS_EINVAL = 1,       /* bad parameter or shape */
+    DNS_ENOMEM = 2,       /* device or host allocation failed */
+    DNS_EDEVICE = 3,      /* HIP runtime error or no gfx950 device */
+    DNS_ENONFINITE = 4,   /* NaN/Inf found in a field */
+    DNS_ESOLVER = 5,      /* per-mode operator could not be factorised */
+    DNS_EUNSUPPORTED = 6  /* option not implemented on this path */
+};
+
+enum { DNS_PREC_FP64 = 0, DNS_PREC_MIXED = 1 };
+
+/* Run parameters: the namelist variables of read_input_3d that the 3D step reads
+ * (DNS_pressure_BC_3D.f90:559-568), plus the precision of the device path. */
+typedef struct dns_params {
+    int nx, ny, nz;                     /* &grid nx_input, ny_input, nz_input */
+    double xlen, ylen, re, dt;          /* &simulation xlen, ylen, re; &time_control dt */
+    int use_crank_nicolson;             /* &simulation: theta = 0.5 if nonzero, else 1 */
+    int use_dealias_23_rule;            /* &computational */
+    int use_weak_form_divergence;       /* &pressure_solver: must be 0 (DNS_EUNSUPPORTED) */
+    int flow_control_method;            /* &flow_control: 1 = constant G, 2 = PI on U_bulk */
+    double target_pressure_gradient;
+    double target_bulk_velocity;
+    double controller_gain;
+    int controller_update_freq;
+    int istart;                         /* &time_control istart: the PI controller's step argument */
+    double u_centerline;                /* &simulation */
+    int enable_perturbations;           /* &perturbations */
+    int perturbation_kind;              /* 0 = analytical (mode_x, mode_y), 1 = seeded noise */
+    int analytical_mode_x, analytical_mode_y;
+    double perturbation_amplitude;
+    unsigned long long seed;            /* seeded noise: splitmix64 seed */
+    int noise_kmax;                     /* seeded noise: modes 0 <= mx, my <= noise_kmax */
+    int precision;                      /* DNS_PREC_FP64 or DNS_PREC_MIXED */
+} dns_params;
+
+/* Diagnostics of output_diagnostics_3d / check_divergence_3d
+ * (DNS_pressure_BC_3D.f90:2094-2146, 2171-2190). */
+typedef struct dns_diag {
+    double umax, vmax, wmax;            /* max |u|, |v|, |w| */
+    double energy;                      /* LGL-weighted kinetic energy per unit volume */
+    double pressure_gradient;           /* current G (current_pressure_gradient) */
+    double bulk_velocity;               /* U_bulk of the current u */
+    double avg_pressure_gradient;       /* running average of G (method 2) */
+    double bulk_error_integral;         /* PI state */
+    double bulk_error_prev;
+    double time;
+    int step;
+} dns_diag;
+
+typedef struct dns_ctx dns_ctx;
+
+/* Context: grid constants, LGL operators, per-mode solve bases and the resident
+ * fields.  Replaces setup_grid_3d, setup_spectral_3d, allocate_arrays_3d
+ * (DNS_pressure_BC_3D.f90:728-888). */
+int dns_create(const dns_params* p, dns_ctx** out);
+int dns_destroy(dns_ctx* ctx);
+const char* dns_last_error(const dns_ctx* ctx);   /* ctx may be NULL: last creation error */
+
+/* initialize_fields_3d + apply_solenoidal_perturbations (DNS_pressure_BC_3D.f90:909-962,
+ * 3219-3265): Poiseuille profile, perturbation, wall values; p = 0, time = 0. */
+int dns_init_fields(dns_ctx* ctx);
+/* Host physical fields in/out (any pointer may be NULL to skip that field). */
+int dns_set_fields(dns_ctx* ctx, const double* u, const double* v, const double* w, const double* p);
+int dns_get_fields(dns_ctx* ctx, double* u, double* v, double* w, double* p);
+/* Device-pointer variants (physical fields already in HBM). */
+int dns_set_fields_device(dns_ctx* ctx, const double* u, const double* v, const double* w, const double* p);
+int dns_get_fields_device(dns_ctx* ctx, double* u, double* v, double* w, double* p);
+/* PI controller / forcing state (current_pressure_gradient, bulk_velocity_error_integral,
+ * bulk_velocity_error_prev, pressure_gradient_sum, pressure_gradient_count). */
+int dns_set_controller(dns_ctx* ctx, double G, double integral, double prev, double gsum, int gcount);
+
+/* The five stage subroutines of the time loop, same names and order as
+ * DNS_pressure_BC_3D.f90:314-326.  dns_begin_step does `time = time + dt` (:296). */
+int dns_begin_step(dns_ctx* ctx);
+int dns_compute_source_terms(dns_ctx* ctx);       /* :1032-1148 */
+int dns_viscous_step(dns_ctx* ctx);               /* :1285-1332 */
+int dns_solve_pressure(dns_ctx* ctx);             /* :1631-1733 */
+int dns_pressure_correction(dns_ctx* ctx);        /* :1903-1929 */
+int dns_apply_boundary_conditions(dns_ctx* ctx);  /* :1953-1972 */
+/* nsteps whole steps (begin + five stages), pressure stages fused; asynchronous. */
+int dns_step(dns_ctx* ctx, int nsteps);
+/* Same as dns_step, the step replayed from a captured HIP graph. */
+int dns_step_graph(dns_ctx* ctx, int nsteps);
+int dns_synchronize(dns_ctx* ctx);
+
+int dns_diagnostics(dns_ctx* ctx, dns_diag* d);
+int dns_check_divergence(dns_ctx* ctx, double* max_div);
+
+/* Per-step device time of the last dns_step / dns_step_graph call, from HIP events
+ * recorded on the context stream: min, p50, p99, mean in milliseconds. */
+int dns_step_timing(dns_ctx* ctx, double* out4);
+/* Name of the HIP stream handle (hipStream_t) the context runs on. */
+void* dns_stream(dns_ctx* ctx);
+
+/* ---- FFT module (fftw3_dns_module.f90:19-152) --------------------------------
+ * Plane transforms real(nx,ny) <-> complex(nx/2+1,ny): forward unnormalised, e^{-i};
+ * backward e^{+i} then divided by nx*ny, with FFTW's c2r handling of non-Hermitian
+ * input.  Host-array variants copy in and out; _device variants take HBM pointers. */
+typedef struct dns_fft_plan dns_fft_plan;
+int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out);  /* setup_fftw3_plans_dns */
+int dns_fft_plan_destroy(dns_fft_plan* plan);                                /* destroy_fftw3_plans_dns */
+int dns_fft_forward(dns_fft_plan* plan, const double* real_in, double* complex_out, int nplanes);
+int dns_fft_backward(dns_fft_plan* plan, const double* complex_in, double* real_out, int nplanes);
+int dns_fft_forward_device(dns_fft_plan* plan, const double* real_in, double* complex_out, int nplanes, void* stream);
+int dns_fft_backward_device(dns_fft_plan* plan, const double* complex_in, double* real_out, int nplanes, void* stream);
+
+/* Library version string. */
+const char* dns_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
