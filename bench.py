@@ -1,0 +1,161 @@
+"""Benchmark of the 3D channel-DNS time step on one MI355X.
+
+Metric: grid-point steps per second (nx*ny*nz x full fractional steps / wall second) of the
+production shape 128x128x129 (Lx = 4 pi, Ly = 2 pi, Re = 180, dt = 0.002, PI control on the
+bulk velocity, 2/3 dealiasing, seeded stream-function noise of amplitude 0.05), FP64, state
+resident in HBM, steps replayed from a captured HIP graph.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+WORKLOADS = {
+    "channel_128x128x129_fp64": dict(
+        nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+        use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,
+        target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, controller_update_freq=7, istart=0,
+        u_centerline=1.0, enable_perturbations=1, perturbation_kind=1, seed=2024, noise_kmax=8,
+        perturbation_amplitude=0.05),
+    "channel_32x32x33_fp64": dict(
+        nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+        use_dealias_23_rule=1, flow_control_method=1, target_pressure_gradient=2.0 / 180.0,
+        u_centerline=1.0, enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1,
+        analytical_mode_y=1, perturbation_amplitude=0.01),
+}
+
+
+def kernel_work(kw):
+    """Algorithmic HBM bytes and flops per launch of each step kernel (FP64).
+
+    Spectral field: 16 B x (nx/2+1) ny nz; physical field: 8 B x nx ny nz.  Each kernel is
+    charged one read of its inputs and one write of its outputs; flops count the useful
+    multiply-adds of the wall-normal contractions (2 per MAC), none for transforms.
+    """
+    nx, ny, nz = kw["nx"], kw["ny"], kw["nz"]
+    H2 = 2 * (nx // 2 + 1) * ny
+    S = 8.0 * H2 * nz            # bytes of one spectral field
+    P = 8.0 * nx * ny * nz       # bytes of one physical field
+    m = nz - 2
+    return {
+        "zderiv_vort": dict(bytes=5 * S, flops=2 * 2.0 * nz * nz * H2),
+        "c2r_nl": dict(bytes=5 * S + 6 * P, flops=0.0),
+        "r2c_cross": dict(bytes=6 * P + 3 * S, flops=0.0),
+        "flow_control": dict(bytes=16.0 * nz, flops=0.0),
+        "helmholtz": dict(bytes=9 * S, flops=3 * 2.0 * (m * (m + nz) + m * m) * H2),
+        "poisson_fused": dict(bytes=8 * S, flops=2.0 * (nz * 2 * nz + 2 * nz * nz) * H2),
+        "poisson_mode0": dict(bytes=0.0, flops=0.0),
+    }
+
+
+def cpu_baseline(kw, steps):
+    """The oracle (C port of the reference algorithm as written: per-mode matrix rebuild and
+    LU every step, serial mode loops, OpenMP over planes/columns) on the host cores."""
+    from oracle import oracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = orc.Oracle(refactor_every_step=1, **kw)
+    o.init_fields()
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    n = kw["nx"] * kw["ny"] * kw["nz"]
+    return {"value": n * steps / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} step(s) of the same {kw['nx']}x{kw['ny']}x{kw['nz']} workload, FP64, "
+                      f"per-mode rebuild + LU every step as the reference, {threads} OpenMP threads, "
+                      f"{dt:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="channel_128x128x129_fp64", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of graph replay")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the ensemble/serving "
+              "path would shard whole members across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    import f90_dns_amd as dns
+    kw = WORKLOADS[args.workload]
+    sim = dns.Simulation(**kw)
+    sim.init_fields()
+    graph = not args.no_graph
+    print(f"[bench] {args.workload}: warmup {args.warmup}", file=sys.stderr, flush=True)
+    sim.step(args.warmup, graph=graph)
+    sim.synchronize()
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    sim.step(args.steps, graph=graph)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    lat = sim.step_timing()
+    npts = kw["nx"] * kw["ny"] * kw["nz"]
+    value = npts * args.steps / wall
+    diag = sim.diagnostics()
+    if not np.isfinite(diag["energy"]):
+        raise SystemExit("non-finite energy after the timed steps")
+
+    # per-kernel device time (events between launches, same stream) for the roofline entry
+    prof = sim.profile_step(args.profile_steps) if args.profile_steps > 0 else {}
+    work = kernel_work(kw)
+    roof = None
+    if prof:
+        name = max(prof, key=prof.get)
+        ms = prof[name]
+        wk = work[name]
+        if wk["flops"] > 0:
+            ach = wk["flops"] / (ms * 1e-3) / 1e12
+            roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "ms": ms,
+                    "hbm_gbs": wk["bytes"] / (ms * 1e-3) / 1e9}
+        else:
+            ach = wk["bytes"] / (ms * 1e-3) / 1e9
+            roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": None, "ms": ms}
+        roof["kernels_ms"] = prof
+    print("[bench] kernel ms: " + json.dumps(prof), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline and args.cpu_steps > 0:
+        print(f"[bench] cpu baseline: {args.cpu_steps} oracle step(s)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(kw, args.cpu_steps)
+
+    out = {
+        "metric": "grid-point RK4 steps/sec (one full fractional step of DNS_pressure_BC_3D.f90)",
+        "value": value,
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded stream-function noise initial field)",
+        "config": {"workload": args.workload, "nx": kw["nx"], "ny": kw["ny"], "nz": kw["nz"],
+                   "precision": "fp64", "graph": graph, "parallelism": "single GPU"},
+        "step_latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "min": lat["min_ms"], "mean": lat["mean_ms"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,284 @@
+"""f90_dns_amd -- MI355X (gfx950) 3D channel-DNS time step behind a C ABI.
+
+This module is the Python host binding of include/f90dns.h (ctypes, no torch types).  It
+mirrors the reference's run configuration (the namelist variables of
+DNS_pressure_BC_3D.f90:559-568) and its stage subroutines (:314-326).  Every entry point
+runs on the GPU through libf90dns.so; if the library or a gfx950 device is missing, the
+calls raise -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libf90dns.so")
+
+DNS_PREC_FP64 = 0
+DNS_PREC_MIXED = 1
+_ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "ENONFINITE", 5: "ESOLVER", 6: "EUNSUPPORTED"}
+
+
+class DnsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class DnsParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("xlen", ctypes.c_double), ("ylen", ctypes.c_double), ("re", ctypes.c_double), ("dt", ctypes.c_double),
+        ("use_crank_nicolson", ctypes.c_int), ("use_dealias_23_rule", ctypes.c_int),
+        ("use_weak_form_divergence", ctypes.c_int), ("flow_control_method", ctypes.c_int),
+        ("target_pressure_gradient", ctypes.c_double), ("target_bulk_velocity", ctypes.c_double),
+        ("controller_gain", ctypes.c_double), ("controller_update_freq", ctypes.c_int), ("istart", ctypes.c_int),
+        ("u_centerline", ctypes.c_double),
+        ("enable_perturbations", ctypes.c_int), ("perturbation_kind", ctypes.c_int),
+        ("analytical_mode_x", ctypes.c_int), ("analytical_mode_y", ctypes.c_int),
+        ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+class DnsDiag(ctypes.Structure):
+    _fields_ = [
+        ("umax", ctypes.c_double), ("vmax", ctypes.c_double), ("wmax", ctypes.c_double), ("energy", ctypes.c_double),
+        ("pressure_gradient", ctypes.c_double), ("bulk_velocity", ctypes.c_double),
+        ("avg_pressure_gradient", ctypes.c_double), ("bulk_error_integral", ctypes.c_double),
+        ("bulk_error_prev", ctypes.c_double), ("time", ctypes.c_double), ("step", ctypes.c_int),
+    ]
+
+
+# reference defaults (read_input_3d, DNS_pressure_BC_3D.f90:571-603) where the 3D step reads them
+DEFAULTS = dict(
+    nx=128, ny=32, nz=33, xlen=2 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.01,
+    use_crank_nicolson=1, use_dealias_23_rule=0, use_weak_form_divergence=0, flow_control_method=1,
+    target_pressure_gradient=0.03, target_bulk_velocity=1.0, controller_gain=0.15, controller_update_freq=7,
+    istart=0, u_centerline=1.5, enable_perturbations=0, perturbation_kind=0, analytical_mode_x=1,
+    analytical_mode_y=1, perturbation_amplitude=0.01, seed=0, noise_kmax=8, precision=DNS_PREC_FP64,
+)
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Load libf90dns.so (built in-tree by f90_dns_amd/build.py); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DnsError(3, f"{LIB_PATH} is not built; run python -m f90_dns_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        L.dns_create.argtypes = [ctypes.POINTER(DnsParams), ctypes.POINTER(_vp)]
+        L.dns_destroy.argtypes = [_vp]
+        L.dns_last_error.argtypes = [_vp]
+        L.dns_last_error.restype = ctypes.c_char_p
+        L.dns_version.restype = ctypes.c_char_p
+        for n in ("dns_init_fields", "dns_begin_step", "dns_compute_source_terms", "dns_viscous_step",
+                  "dns_solve_pressure", "dns_pressure_correction", "dns_apply_boundary_conditions",
+                  "dns_synchronize"):
+            getattr(L, n).argtypes = [_vp]
+        for n in ("dns_set_fields", "dns_get_fields", "dns_set_fields_device", "dns_get_fields_device"):
+            getattr(L, n).argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.dns_set_controller.argtypes = [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int]
+        L.dns_step.argtypes = [_vp, ctypes.c_int]
+        L.dns_step_graph.argtypes = [_vp, ctypes.c_int]
+        L.dns_diagnostics.argtypes = [_vp, ctypes.POINTER(DnsDiag)]
+        L.dns_check_divergence.argtypes = [_vp, _dp]
+        L.dns_step_timing.argtypes = [_vp, _dp]
+        L.dns_stream.argtypes = [_vp]
+        L.dns_stream.restype = _vp
+        L.dns_fft_plan_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.dns_fft_plan_destroy.argtypes = [_vp]
+        for n in ("dns_fft_forward", "dns_fft_backward"):
+            getattr(L, n).argtypes = [_vp, _vp, _vp, ctypes.c_int]
+        for n in ("dns_fft_forward_device", "dns_fft_backward_device"):
+            getattr(L, n).argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.dns_debug_spectral.argtypes = [_vp, ctypes.c_int, _vp]
+        L.dns_debug_zderiv.argtypes = [_vp, _vp, _vp]
+        L.dns_debug_operator_error.argtypes = [_vp, _dp]
+        L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
+        L.dns_profile_kernel_name.argtypes = [ctypes.c_int]
+        L.dns_profile_kernel_name.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def make_params(**kw):
+    d = dict(DEFAULTS)
+    unknown = set(kw) - set(d)
+    if unknown:
+        raise KeyError(f"unknown parameters {sorted(unknown)}")
+    d.update(kw)
+    p = DnsParams()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+def _arr(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != int(np.prod(shape)):
+        raise ValueError(f"expected {shape} values, got {a.shape}")
+    return a
+
+
+class Simulation:
+    """One resident simulation (a dns_ctx).  Physical fields are numpy (nz, ny, nx) arrays
+    in the reference's Fortran order f(i,j,k)."""
+
+    def __init__(self, **kw):
+        self.params = make_params(**kw)
+        self.nx, self.ny, self.nz = self.params.nx, self.params.ny, self.params.nz
+        h = _vp()
+        L = lib()
+        rc = L.dns_create(ctypes.byref(self.params), ctypes.byref(h))
+        if rc != 0:
+            raise DnsError(rc, L.dns_last_error(None).decode())
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DnsError(rc, lib().dns_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dns_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def spectral_shape(self):
+        return (self.nz, self.ny, self.nx // 2 + 1)
+
+    def init_fields(self):
+        self._check(lib().dns_init_fields(self._h))
+
+    def set_fields(self, u=None, v=None, w=None, p=None):
+        arrs = [None if a is None else _arr(a, self.shape) for a in (u, v, w, p)]
+        self._check(lib().dns_set_fields(self._h, *[None if a is None else a.ctypes.data for a in arrs]))
+
+    def get_fields(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        self._check(lib().dns_get_fields(self._h, *[a.ctypes.data for a in out]))
+        return out
+
+    def set_controller(self, G, integral=0.0, prev=0.0, gsum=0.0, gcount=0):
+        self._check(lib().dns_set_controller(self._h, G, integral, prev, gsum, gcount))
+
+    # the stage subroutines, DNS_pressure_BC_3D.f90:314-326
+    def begin_step(self):
+        self._check(lib().dns_begin_step(self._h))
+
+    def compute_source_terms(self):
+        self._check(lib().dns_compute_source_terms(self._h))
+
+    def viscous_step(self):
+        self._check(lib().dns_viscous_step(self._h))
+
+    def solve_pressure(self):
+        self._check(lib().dns_solve_pressure(self._h))
+
+    def pressure_correction(self):
+        self._check(lib().dns_pressure_correction(self._h))
+
+    def apply_boundary_conditions(self):
+        self._check(lib().dns_apply_boundary_conditions(self._h))
+
+    def step(self, n=1, graph=False):
+        L = lib()
+        self._check((L.dns_step_graph if graph else L.dns_step)(self._h, n))
+
+    def synchronize(self):
+        self._check(lib().dns_synchronize(self._h))
+
+    def step_timing(self):
+        out = (ctypes.c_double * 4)()
+        self._check(lib().dns_step_timing(self._h, out))
+        return dict(min_ms=out[0], p50_ms=out[1], p99_ms=out[2], mean_ms=out[3])
+
+    def diagnostics(self):
+        d = DnsDiag()
+        self._check(lib().dns_diagnostics(self._h, ctypes.byref(d)))
+        return {f: getattr(d, f) for f, _ in DnsDiag._fields_}
+
+    def max_divergence(self):
+        v = ctypes.c_double()
+        self._check(lib().dns_check_divergence(self._h, ctypes.byref(v)))
+        return v.value
+
+    def spectral(self, which):
+        """Copy a resident spectral buffer (see include/f90dns_debug.h) as complex (nz, ny, nx/2+1)."""
+        out = np.empty(self.spectral_shape, dtype=np.complex128)
+        self._check(lib().dns_debug_spectral(self._h, which, out.ctypes.data))
+        return out
+
+    def zderiv(self, spec):
+        a = np.ascontiguousarray(spec, dtype=np.complex128).reshape(self.spectral_shape)
+        out = np.empty_like(a)
+        self._check(lib().dns_debug_zderiv(self._h, a.ctypes.data, out.ctypes.data))
+        return out
+
+    def profile_step(self, nsteps):
+        """Mean per-kernel device time (ms) over nsteps eager steps, events between launches."""
+        out = (ctypes.c_double * 7)()
+        self._check(lib().dns_profile_step(self._h, nsteps, out))
+        return {lib().dns_profile_kernel_name(i).decode(): out[i] for i in range(7)}
+
+    def operator_error(self):
+        out = (ctypes.c_double * 2)()
+        self._check(lib().dns_debug_operator_error(self._h, out))
+        return out[0], out[1]
+
+
+class FFTPlan:
+    """The FFT module of the reference (fftw3_dns_module.f90:19-152) on the GPU."""
+
+    def __init__(self, nx, ny, precision=DNS_PREC_FP64):
+        self.nx, self.ny = nx, ny
+        h = _vp()
+        rc = lib().dns_fft_plan_create(nx, ny, precision, ctypes.byref(h))
+        if rc != 0:
+            raise DnsError(rc, lib().dns_last_error(None).decode())
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().dns_fft_plan_destroy(self._h)
+            self._h = None
+
+    def forward(self, f):
+        """f: (nplanes, ny, nx) real -> (nplanes, ny, nx/2+1) complex, unnormalised."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        npl = f.shape[0]
+        out = np.empty((npl, self.ny, self.nx // 2 + 1), dtype=np.complex128)
+        rc = lib().dns_fft_forward(self._h, f.ctypes.data, out.ctypes.data, npl)
+        if rc:
+            raise DnsError(rc, "dns_fft_forward")
+        return out
+
+    def backward(self, F):
+        """F: (nplanes, ny, nx/2+1) complex -> (nplanes, ny, nx) real, divided by nx*ny."""
+        F = np.ascontiguousarray(F, dtype=np.complex128)
+        npl = F.shape[0]
+        out = np.empty((npl, self.ny, self.nx))
+        rc = lib().dns_fft_backward(self._h, F.ctypes.data, out.ctypes.data, npl)
+        if rc:
+            raise DnsError(rc, "dns_fft_backward")
+        return out

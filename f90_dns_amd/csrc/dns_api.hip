@@ -1,0 +1,789 @@
+// dns_api.hip -- the C ABI of include/f90dns.h: context set-up, resident fields and the
+// stage sequence of the 3D fractional step on one MI355X.
+//
+// State lives in HBM in spectral form: u_hat, v_hat, w_hat, p_hat (complex (nx/2+1, ny, nz)).
+// That is exact with respect to the reference, which keeps physical fields and transforms
+// them forward at the start of every stage: a real field transformed forward gives a
+// Hermitian spectrum, and every operator of the step maps Hermitian spectra to Hermitian
+// spectra except the i*k derivatives, whose c2r/r2c round trips the kernels apply as
+// effective wavenumbers (x Nyquist column -> 0; y Nyquist row of the kx = 0, nx/2 columns -> 0).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/f90dns.h"
+#include "../../include/f90dns_debug.h"
+#include "kernels.h"
+#include "setup.h"
+
+using namespace dns;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct dns_ctx {
+    dns_params p;
+    int nx, ny, nz, nxh, H2, npl;
+    size_t N, NS;
+    double theta, ybar;
+    int rb, rbi;
+    hipStream_t stream = nullptr;
+    FFTGeom g;
+    Operators op;
+    // constants
+    double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *d1rm = nullptr, *m0 = nullptr;
+    unsigned char* mask = nullptr;
+    double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr;
+    double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
+    // fields
+    double *uh = nullptr, *vh = nullptr, *wh = nullptr, *ph = nullptr;
+    double *us = nullptr, *vs = nullptr, *ws = nullptr;
+    double *ox = nullptr, *oy = nullptr, *sh[3] = {nullptr, nullptr, nullptr};
+    double *phys[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *phi = nullptr, *dphi = nullptr, *tmp = nullptr;
+    FlowState* flow = nullptr;
+    double *red_partial = nullptr, *red_result = nullptr;
+    int red_blocks = 512;
+    std::vector<void*> allocs;
+    double time = 0.0;
+    int step = 0;
+    std::string err;
+    // timing
+    std::vector<hipEvent_t> ev;
+    std::vector<double> step_ms;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+static int fail(dns_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg; else g_create_error = msg;
+    return code;
+}
+#define HIPCHECK(c, call)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail((c), DNS_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+template <class T>
+static int dalloc(dns_ctx* c, T** ptr, size_t n) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return fail(c, DNS_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *ptr = static_cast<T*>(q);
+    return 0;
+}
+template <class T>
+static int upload(dns_ctx* c, T** ptr, const std::vector<T>& h) {
+    int r = dalloc(c, ptr, h.size());
+    if (r) return r;
+    HIPCHECK(c, hipMemcpy(*ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+static bool check_gfx950(std::string& why) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { why = "no HIP device"; return false; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) { why = "hipGetDeviceProperties failed"; return false; }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        why = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
+        return false;
+    }
+    return true;
+}
+
+static bool make_factors(int n, Factors& f) {
+    std::vector<int> v;
+    if (!radix_factors(n, v) || v.size() > 14) return false;
+    f.n = n; f.nf = int(v.size());
+    for (size_t i = 0; i < v.size(); i++) f.f[i] = v[i];
+    return true;
+}
+
+static int destroy_impl(dns_ctx* c) {
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    for (void* q : c->allocs) (void)hipFree(q);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" {
+
+const char* dns_version(void) { return "f90_dns_amd 0.1 (gfx950, FP64 spectral-resident step)"; }
+
+const char* dns_last_error(const dns_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+int dns_create(const dns_params* p, dns_ctx** out) {
+    if (!p || !out) return fail(nullptr, DNS_EINVAL, "null argument");
+    *out = nullptr;
+    if (p->nx < 4 || p->ny < 4 || (p->nx & 1) || (p->ny & 1)) return fail(nullptr, DNS_EINVAL, "nx and ny must be even and >= 4");
+    if (p->nz < 5 || p->nz > 144) return fail(nullptr, DNS_EINVAL, "nz must be in [5, 144]");
+    if (!(p->dt > 0) || !(p->re > 0) || !(p->xlen > 0) || !(p->ylen > 0)) return fail(nullptr, DNS_EINVAL, "dt, re, xlen, ylen must be positive");
+    if (p->use_weak_form_divergence) return fail(nullptr, DNS_EUNSUPPORTED, "weak-form pressure RHS is not implemented");
+    if (p->precision != DNS_PREC_FP64) return fail(nullptr, DNS_EUNSUPPORTED, "only DNS_PREC_FP64 is implemented");
+    if (p->flow_control_method != 1 && p->flow_control_method != 2)
+        return fail(nullptr, DNS_EINVAL, "flow_control_method must be 1 or 2");
+    if (p->flow_control_method == 2 && (p->target_bulk_velocity <= 0 || p->controller_gain <= 0 || p->controller_update_freq <= 0))
+        return fail(nullptr, DNS_EINVAL, "method 2 needs positive target_bulk_velocity, controller_gain, controller_update_freq");
+    std::string why;
+    if (!check_gfx950(why)) return fail(nullptr, DNS_EDEVICE, why);
+
+    dns_ctx* c = new dns_ctx();
+    c->p = *p;
+    c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nxh = p->nx / 2 + 1;
+    c->H2 = 2 * c->nxh * c->ny; c->npl = c->nx * c->ny;
+    c->N = size_t(c->npl) * c->nz; c->NS = size_t(c->H2) * c->nz;
+    c->theta = p->use_crank_nicolson ? 0.5 : 1.0;
+    c->ybar = 2.0;
+    c->rb = (c->nz + 15) / 16;
+    c->rbi = (c->nz - 2 + 15) / 16;
+    auto bail = [&](int code) { int r = code; destroy_impl(c); return r; };
+
+    if (!fft_geom(c->nx, c->ny, c->g)) {
+        g_create_error = "plane too large for the LDS-resident FP64 transform (nx/2+1)*ny must be <= 8320";
+        return bail(DNS_EUNSUPPORTED);
+    }
+    if (!make_factors(c->ny, c->g.fy) || !make_factors(c->nx / 2, c->g.fx)) {
+        g_create_error = "nx/2 and ny must be of the form 2^a 3^b";
+        return bail(DNS_EINVAL);
+    }
+    if (fft_prepare(c->g) != hipSuccess) {
+        g_create_error = "hipFuncSetAttribute failed for the plane kernels";
+        return bail(DNS_EDEVICE);
+    }
+    std::string err;
+    if (!build_operators(c->nz, p->dt, p->re, c->theta, p->use_crank_nicolson != 0, c->op, err)) {
+        g_create_error = err;
+        return bail(DNS_ESOLVER);
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        return bail(DNS_EDEVICE);
+    }
+    int r = 0;
+    std::vector<double> tw, xw, yw;
+    std::vector<unsigned char> mask;
+    twiddles(c->nx, tw);
+    if ((r = upload(c, &c->twx, tw))) { g_create_error = c->err; return bail(r); }
+    twiddles(c->ny, tw);
+    if ((r = upload(c, &c->twy, tw))) { g_create_error = c->err; return bail(r); }
+    c->g.twx = reinterpret_cast<const double2*>(c->twx);
+    c->g.twy = reinterpret_cast<const double2*>(c->twy);
+    wavenumbers(c->nx, c->ny, p->xlen, p->ylen, xw, yw);
+    dealias_mask(c->nx, c->ny, mask);
+    std::vector<double> d1rm(size_t(c->nz) * c->nz);
+    for (int i = 0; i < c->nz; i++)
+        for (int j = 0; j < c->nz; j++) d1rm[size_t(i) * c->nz + j] = c->op.d1[i + size_t(c->nz) * j];
+    const Operators& o = c->op;
+    if ((r = upload(c, &c->xw, xw)) || (r = upload(c, &c->yw, yw)) || (r = upload(c, &c->mask, mask)) ||
+        (r = upload(c, &c->wz, o.wz)) || (r = upload(c, &c->d1rm, d1rm)) || (r = upload(c, &c->m0, o.m0)) ||
+        (r = upload(c, &c->D1sw, o.D1.data)) || (r = upload(c, &c->hF, o.hF.data)) || (r = upload(c, &c->hB, o.hB.data)) ||
+        (r = upload(c, &c->hc1, o.hc1)) || (r = upload(c, &c->hc2, o.hc2)) || (r = upload(c, &c->hl, o.hl)) ||
+        (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) || (r = upload(c, &c->pDB, o.pDB.data)) ||
+        (r = upload(c, &c->pl, o.pl))) {
+        g_create_error = c->err;
+        return bail(r);
+    }
+    double** specs[] = {&c->uh, &c->vh, &c->wh, &c->ph, &c->us, &c->vs, &c->ws, &c->ox, &c->oy,
+                        &c->sh[0], &c->sh[1], &c->sh[2], &c->phi, &c->dphi, &c->tmp};
+    for (double** q : specs)
+        if ((r = dalloc(c, q, c->NS))) { g_create_error = c->err; return bail(r); }
+    for (int f = 0; f < 6; f++)
+        if ((r = dalloc(c, &c->phys[f], c->N))) { g_create_error = c->err; return bail(r); }
+    if ((r = dalloc(c, &c->flow, 1)) || (r = dalloc(c, &c->red_partial, size_t(c->red_blocks) * 5)) ||
+        (r = dalloc(c, &c->red_result, 5))) {
+        g_create_error = c->err;
+        return bail(r);
+    }
+    for (double** q : specs) (void)hipMemset(*q, 0, c->NS * sizeof(double));
+    FlowState fs{};
+    fs.G = p->target_pressure_gradient;
+    fs.gavg = p->target_pressure_gradient;
+    if (hipMemcpy(c->flow, &fs, sizeof(fs), hipMemcpyHostToDevice) != hipSuccess) {
+        g_create_error = "upload of the flow-control state failed";
+        return bail(DNS_EDEVICE);
+    }
+    *out = c;
+    return 0;
+}
+
+int dns_destroy(dns_ctx* c) { return destroy_impl(c); }
+
+void* dns_stream(dns_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+static FFTArgs fft_args(dns_ctx* c) {
+    FFTArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.xw = c->xw; a.yw = c->yw;
+    return a;
+}
+
+// forward transform of physical fields (device pointers, nf <= 4) into spectral fields
+static int r2c_fields(dns_ctx* c, const double* const* src, double* const* dst, int nf) {
+    FFTArgs a = fft_args(c);
+    for (int f = 0; f < nf; f++) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
+    a.sstride = c->npl; a.dstride = c->H2;
+    HIPCHECK(c, launch_r2c(c->g, R2C_PLAIN, a, c->nz, nf, c->stream));
+    return 0;
+}
+static int c2r_fields(dns_ctx* c, const double* const* src, double* const* dst, int nf) {
+    FFTArgs a = fft_args(c);
+    for (int f = 0; f < nf; f++) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
+    a.sstride = c->H2; a.dstride = c->npl;
+    HIPCHECK(c, launch_c2r(c->g, C2R_PLAIN, a, c->nz, nf, c->stream));
+    return 0;
+}
+
+int dns_set_fields_device(dns_ctx* c, const double* u, const double* v, const double* w, const double* p) {
+    if (!c) return DNS_EINVAL;
+    const double* in[4] = {u, v, w, p};
+    double* outs[4] = {c->uh, c->vh, c->wh, c->ph};
+    for (int f = 0; f < 4; f++) {
+        if (!in[f]) continue;
+        int r = r2c_fields(c, &in[f], &outs[f], 1);
+        if (r) return r;
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_set_fields(dns_ctx* c, const double* u, const double* v, const double* w, const double* p) {
+    if (!c) return DNS_EINVAL;
+    const double* in[4] = {u, v, w, p};
+    double* outs[4] = {c->uh, c->vh, c->wh, c->ph};
+    for (int f = 0; f < 4; f++) {
+        if (!in[f]) continue;
+        HIPCHECK(c, hipMemcpyAsync(c->phys[0], in[f], c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        const double* s = c->phys[0];
+        int r = r2c_fields(c, &s, &outs[f], 1);
+        if (r) return r;
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_get_fields_device(dns_ctx* c, double* u, double* v, double* w, double* p) {
+    if (!c) return DNS_EINVAL;
+    double* outs[4] = {u, v, w, p};
+    const double* in[4] = {c->uh, c->vh, c->wh, c->ph};
+    for (int f = 0; f < 4; f++) {
+        if (!outs[f]) continue;
+        int r = c2r_fields(c, &in[f], &outs[f], 1);
+        if (r) return r;
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_get_fields(dns_ctx* c, double* u, double* v, double* w, double* p) {
+    if (!c) return DNS_EINVAL;
+    double* outs[4] = {u, v, w, p};
+    const double* in[4] = {c->uh, c->vh, c->wh, c->ph};
+    int r = c2r_fields(c, in, c->phys, 4);
+    if (r) return r;
+    for (int f = 0; f < 4; f++)
+        if (outs[f]) HIPCHECK(c, hipMemcpyAsync(outs[f], c->phys[f], c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_init_fields(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    InitSpec s{c->nx, c->ny, c->nz, c->p.xlen, c->p.ylen, c->p.u_centerline, c->p.enable_perturbations,
+               c->p.perturbation_kind, c->p.analytical_mode_x, c->p.analytical_mode_y, c->p.perturbation_amplitude,
+               (uint64_t)c->p.seed, c->p.noise_kmax};
+    std::vector<double> u, v, w;
+    initial_field(s, u, v, w);
+    std::vector<double> zero(c->N, 0.0);
+    int r = dns_set_fields(c, u.data(), v.data(), w.data(), zero.data());
+    c->time = 0.0; c->step = 0;
+    return r;
+}
+
+int dns_set_controller(dns_ctx* c, double G, double integral, double prev, double gsum, int gcount) {
+    if (!c) return DNS_EINVAL;
+    FlowState fs{};
+    fs.G = G; fs.err_int = integral; fs.err_prev = prev; fs.gsum = gsum; fs.gcount = gcount;
+    fs.gavg = gcount > 0 ? gsum / double(gcount) : G;
+    HIPCHECK(c, hipMemcpyAsync(c->flow, &fs, sizeof(fs), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---- stages -----------------------------------------------------------------------------
+int dns_begin_step(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    c->step += 1;
+    c->time = c->time + c->p.dt;
+    return 0;
+}
+
+static FlowArgs flow_args(dns_ctx* c) {
+    FlowArgs fa{};
+    fa.nz = c->nz; fa.method = c->p.flow_control_method; fa.freq = c->p.controller_update_freq; fa.istart = c->p.istart;
+    fa.target_G = c->p.target_pressure_gradient; fa.target_Ub = c->p.target_bulk_velocity; fa.gain = c->p.controller_gain;
+    fa.re = c->p.re; fa.dt = c->p.dt; fa.nxny = double(c->nx * c->ny);
+    double ws = 0.0;
+    for (int k = 0; k < c->nz; k++) ws += c->op.wz[k];
+    fa.wsum = ws;
+    fa.uh = c->uh; fa.H2 = c->H2; fa.wz = c->wz; fa.st = c->flow;
+    return fa;
+}
+
+int dns_compute_source_terms(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    ZArgs z{};
+    z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
+    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.in[0] = c->uh; z.in[1] = c->vh; z.in[2] = c->wh;
+    z.out[0] = c->ox; z.out[1] = c->oy;
+    z.xw = c->xw; z.yw = c->yw;
+    HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->stream));
+    FFTArgs a = fft_args(c);
+    a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->wh; a.src[3] = c->ox; a.src[4] = c->oy;
+    for (int f = 0; f < 6; f++) a.dst[f] = c->phys[f];
+    a.sstride = c->H2; a.dstride = c->npl;
+    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz, 6, c->stream));
+    FFTArgs b = fft_args(c);
+    for (int f = 0; f < 6; f++) b.src[f] = c->phys[f];
+    for (int f = 0; f < 3; f++) b.dst[f] = c->sh[f];
+    b.mask = c->p.use_dealias_23_rule ? c->mask : nullptr;
+    b.sstride = c->npl; b.dstride = c->H2;
+    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz, 3, c->stream));
+    FlowArgs fa = flow_args(c);
+    HIPCHECK(c, launch_flow_control(fa, c->stream));
+    return 0;
+}
+
+int dns_viscous_step(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    HelmArgs h{};
+    h.nz = c->nz; h.m = c->nz - 2; h.H2 = c->H2; h.nxh = c->nxh; h.ny = c->ny; h.nx = c->nx;
+    h.Fsw = c->hF; h.FKS = c->op.hF.ks; h.Bsw = c->hB; h.BKS = c->op.hB.ks;
+    h.c1 = c->hc1; h.c2 = c->hc2; h.hl = c->hl;
+    h.f = c->op.f; h.dt = c->p.dt; h.nxny = double(c->nx * c->ny);
+    h.cn = c->p.use_crank_nicolson ? 1 : 0;
+    h.xw = c->xw; h.yw = c->yw;
+    h.uh[0] = c->uh; h.uh[1] = c->vh; h.uh[2] = c->wh;
+    for (int f = 0; f < 3; f++) h.sh[f] = c->sh[f];
+    h.out[0] = c->us; h.out[1] = c->vs; h.out[2] = c->ws;
+    h.ph = c->ph;
+    h.G = &c->flow->G;
+    HIPCHECK(c, launch_helmholtz(c->rbi, h, c->stream));
+    return 0;
+}
+
+static PoisArgs poisson_args(dns_ctx* c, int fused) {
+    PoisArgs a{};
+    a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
+    a.Fsw = c->pF; a.FKS = c->op.pF.ks; a.Bsw = c->pB; a.DBsw = c->pDB; a.BKS = c->op.pB.ks;
+    a.pl = c->pl; a.dt = c->p.dt; a.fused = fused; a.xw = c->xw; a.yw = c->yw;
+    a.us = c->us; a.vs = c->vs; a.ws = c->ws;
+    a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph;
+    a.phi = c->phi; a.dphi = c->dphi;
+    return a;
+}
+
+static Mode0Args mode0_args(dns_ctx* c, int fused) {
+    Mode0Args m{};
+    m.nz = c->nz; m.H2 = c->H2; m.d1 = c->d1rm; m.m0 = c->m0; m.wz = c->wz; m.dt = c->p.dt; m.fused = fused;
+    m.us = c->us; m.vs = c->vs; m.ws = c->ws;
+    m.u = c->uh; m.v = c->vh; m.w = c->wh; m.p = c->ph;
+    m.phi = c->phi; m.dphi = c->dphi;
+    return m;
+}
+
+static int pressure(dns_ctx* c, int fused) {
+    PoisArgs a = poisson_args(c, fused);
+    HIPCHECK(c, launch_poisson(c->rb, a, c->stream));
+    Mode0Args m = mode0_args(c, fused);
+    HIPCHECK(c, launch_poisson_mode0(m, c->stream));
+    return 0;
+}
+
+int dns_solve_pressure(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    return pressure(c, 0);
+}
+
+int dns_pressure_correction(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    CorrArgs a{};
+    a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
+    a.xw = c->xw; a.yw = c->yw;
+    a.us = c->us; a.vs = c->vs; a.ws = c->ws; a.phi = c->phi; a.dphi = c->dphi;
+    a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph; a.dt = c->p.dt;
+    HIPCHECK(c, launch_correction(a, c->stream));
+    return 0;
+}
+
+int dns_apply_boundary_conditions(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    HIPCHECK(c, launch_zero_walls(c->nz, c->H2, c->uh, c->vh, c->wh, c->stream));
+    return 0;
+}
+
+static int one_step_fused(dns_ctx* c) {
+    int r;
+    if ((r = dns_compute_source_terms(c))) return r;
+    if ((r = dns_viscous_step(c))) return r;
+    return pressure(c, 1);
+}
+
+static int ensure_events(dns_ctx* c, int n) {
+    while ((int)c->ev.size() < n) {
+        hipEvent_t e;
+        HIPCHECK(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    return 0;
+}
+
+static int collect_timing(dns_ctx* c, int nsteps) {
+    HIPCHECK(c, hipEventSynchronize(c->ev[nsteps]));
+    c->step_ms.resize(nsteps);
+    for (int s = 0; s < nsteps; s++) {
+        float ms = 0.f;
+        HIPCHECK(c, hipEventElapsedTime(&ms, c->ev[s], c->ev[s + 1]));
+        c->step_ms[s] = ms;
+    }
+    return 0;
+}
+
+int dns_step(dns_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return DNS_EINVAL;
+    int r = ensure_events(c, nsteps + 1);
+    if (r) return r;
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int s = 0; s < nsteps; s++) {
+        dns_begin_step(c);
+        if ((r = one_step_fused(c))) return r;
+        HIPCHECK(c, hipEventRecord(c->ev[s + 1], c->stream));
+    }
+    c->step_ms.clear();
+    c->step_ms.push_back(-double(nsteps));   // marker: timing not yet collected
+    return 0;
+}
+
+int dns_step_graph(dns_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) return DNS_EINVAL;
+    int r;
+    if (!c->gexec) {
+        // one eager step first would change the state; capture without executing instead
+        HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        r = one_step_fused(c);
+        hipGraph_t gph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &gph);
+        if (r) return r;
+        if (e != hipSuccess) return fail(c, DNS_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        c->graph = gph;
+        HIPCHECK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    }
+    if ((r = ensure_events(c, nsteps + 1))) return r;
+    HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+    for (int s = 0; s < nsteps; s++) {
+        dns_begin_step(c);
+        HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[s + 1], c->stream));
+    }
+    c->step_ms.clear();
+    c->step_ms.push_back(-double(nsteps));
+    return 0;
+}
+
+int dns_synchronize(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_step_timing(dns_ctx* c, double* out4) {
+    if (!c || !out4) return DNS_EINVAL;
+    if (!c->step_ms.empty() && c->step_ms[0] < 0) {
+        int n = int(-c->step_ms[0]);
+        int r = collect_timing(c, n);
+        if (r) return r;
+    }
+    if (c->step_ms.empty()) { out4[0] = out4[1] = out4[2] = out4[3] = 0.0; return 0; }
+    std::vector<double> t = c->step_ms;
+    std::sort(t.begin(), t.end());
+    auto pct = [&](double q) {
+        double pos = q * (t.size() - 1);
+        size_t lo = size_t(std::floor(pos)), hi = std::min(t.size() - 1, lo + 1);
+        return t[lo] + (pos - lo) * (t[hi] - t[lo]);
+    };
+    double sum = 0.0;
+    for (double x : t) sum += x;
+    out4[0] = t.front(); out4[1] = pct(0.5); out4[2] = pct(0.99); out4[3] = sum / t.size();
+    return 0;
+}
+
+// ---- diagnostics --------------------------------------------------------------------------
+static int reduce(dns_ctx* c, const double* u, const double* v, const double* w, const double* d, double* res) {
+    ReduceArgs a{};
+    a.npl = c->npl; a.nz = c->nz; a.u = u; a.v = v; a.w = w; a.d = d; a.wz = c->wz;
+    a.partial = c->red_partial; a.result = c->red_result; a.nblocks = c->red_blocks;
+    HIPCHECK(c, launch_reduce(a, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(res, c->red_result, 5 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_diagnostics(dns_ctx* c, dns_diag* d) {
+    if (!c || !d) return DNS_EINVAL;
+    const double* in[3] = {c->uh, c->vh, c->wh};
+    int r = c2r_fields(c, in, c->phys, 3);
+    if (r) return r;
+    double res[5];
+    if ((r = reduce(c, c->phys[0], c->phys[1], c->phys[2], nullptr, res))) return r;
+    std::vector<double> u00(c->nz);
+    HIPCHECK(c, hipMemcpy2DAsync(u00.data(), sizeof(double), c->uh, size_t(c->H2) * sizeof(double), sizeof(double),
+                                 c->nz, hipMemcpyDeviceToHost, c->stream));
+    FlowState fs;
+    HIPCHECK(c, hipMemcpyAsync(&fs, c->flow, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const double xlen = c->p.xlen, ylen = c->p.ylen, ybar = c->ybar;
+    const double fac = (xlen / double(c->nx)) * (ylen / double(c->ny)) * (ybar / 2.0);
+    d->umax = res[0]; d->vmax = res[1]; d->wmax = res[2];
+    d->energy = res[3] * fac / (xlen * ylen * ybar);
+    double vol = 0.0, ws = 0.0;
+    for (int k = 0; k < c->nz; k++) { vol += u00[k] * c->op.wz[k]; ws += c->op.wz[k]; }
+    d->bulk_velocity = vol / (double(c->nx * c->ny) * ws);
+    d->pressure_gradient = fs.G;
+    d->avg_pressure_gradient = fs.gavg;
+    d->bulk_error_integral = fs.err_int;
+    d->bulk_error_prev = fs.err_prev;
+    d->time = c->time;
+    d->step = c->step;
+    return 0;
+}
+
+int dns_check_divergence(dns_ctx* c, double* max_div) {
+    if (!c || !max_div) return DNS_EINVAL;
+    ZArgs z{};
+    z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
+    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.in[0] = c->wh; z.out[0] = c->tmp; z.xw = c->xw; z.yw = c->yw;
+    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, c->stream));
+    FFTArgs a = fft_args(c);
+    a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->tmp; a.dst[0] = c->phys[0];
+    a.sstride = c->H2; a.dstride = c->npl;
+    HIPCHECK(c, launch_c2r(c->g, C2R_DIV, a, c->nz, 1, c->stream));
+    double res[5];
+    int r = reduce(c, nullptr, nullptr, nullptr, c->phys[0], res);
+    if (r) return r;
+    *max_div = res[4];
+    return 0;
+}
+
+// ---- debug hooks (include/f90dns_debug.h) -------------------------------------------------
+int dns_debug_spectral(dns_ctx* c, int which, double* out) {
+    if (!c || !out) return DNS_EINVAL;
+    const double* src[] = {c->uh, c->vh, c->wh, c->ph, c->us, c->vs, c->ws, c->sh[0], c->sh[1], c->sh[2],
+                           c->ox, c->oy, c->phi, c->dphi};
+    if (which < 0 || which >= int(sizeof(src) / sizeof(src[0]))) return fail(c, DNS_EINVAL, "bad buffer index");
+    HIPCHECK(c, hipMemcpyAsync(out, src[which], c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_debug_zderiv(dns_ctx* c, const double* spec_in, double* spec_out) {
+    if (!c || !spec_in || !spec_out) return DNS_EINVAL;
+    HIPCHECK(c, hipMemcpyAsync(c->tmp, spec_in, c->NS * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    ZArgs z{};
+    z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
+    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.in[0] = c->tmp; z.out[0] = c->phi; z.xw = c->xw; z.yw = c->yw;
+    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(spec_out, c->phi, c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// One eager fused step per iteration with an event between every kernel launch; out[i] is
+// the mean duration (ms) of kernel i in the order of dns_profile_kernel_name.
+static const char* kKernelNames[] = {"zderiv_vort", "c2r_nl", "r2c_cross", "flow_control", "helmholtz",
+                                     "poisson_fused", "poisson_mode0"};
+const char* dns_profile_kernel_name(int i) {
+    return (i >= 0 && i < int(sizeof(kKernelNames) / sizeof(kKernelNames[0]))) ? kKernelNames[i] : nullptr;
+}
+
+int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
+    if (!c || !out || nsteps <= 0) return DNS_EINVAL;
+    const int NK = 7;
+    int r = ensure_events(c, NK + 1);
+    if (r) return r;
+    std::vector<double> acc(NK, 0.0);
+    for (int s = 0; s < nsteps; s++) {
+        dns_begin_step(c);
+        HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
+        ZArgs z{};
+        z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
+        z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+        z.in[0] = c->uh; z.in[1] = c->vh; z.in[2] = c->wh; z.out[0] = c->ox; z.out[1] = c->oy;
+        z.xw = c->xw; z.yw = c->yw;
+        HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
+        FFTArgs a = fft_args(c);
+        a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->wh; a.src[3] = c->ox; a.src[4] = c->oy;
+        for (int f = 0; f < 6; f++) a.dst[f] = c->phys[f];
+        a.sstride = c->H2; a.dstride = c->npl;
+        HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz, 6, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
+        FFTArgs b = fft_args(c);
+        for (int f = 0; f < 6; f++) b.src[f] = c->phys[f];
+        for (int f = 0; f < 3; f++) b.dst[f] = c->sh[f];
+        b.mask = c->p.use_dealias_23_rule ? c->mask : nullptr;
+        b.sstride = c->npl; b.dstride = c->H2;
+        HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz, 3, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
+        FlowArgs fa = flow_args(c);
+        HIPCHECK(c, launch_flow_control(fa, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
+        if ((r = dns_viscous_step(c))) return r;
+        HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
+        PoisArgs pa = poisson_args(c, 1);
+        HIPCHECK(c, launch_poisson(c->rb, pa, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
+        Mode0Args ma = mode0_args(c, 1);
+        HIPCHECK(c, launch_poisson_mode0(ma, c->stream));
+        HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));
+        HIPCHECK(c, hipEventSynchronize(c->ev[7]));
+        for (int i = 0; i < NK; i++) {
+            float ms = 0.f;
+            HIPCHECK(c, hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+            acc[i] += ms;
+        }
+    }
+    for (int i = 0; i < NK; i++) out[i] = acc[i] / nsteps;
+    return 0;
+}
+
+int dns_debug_operator_error(dns_ctx* c, double* out2) {
+    if (!c || !out2) return DNS_EINVAL;
+    out2[0] = c->op.helm_recon_err;
+    out2[1] = c->op.pois_recon_err;
+    return 0;
+}
+
+// ---- FFT module -----------------------------------------------------------------------------
+}  // extern "C"
+
+struct dns_fft_plan {
+    FFTGeom g;
+    double *twx = nullptr, *twy = nullptr;
+    double *din = nullptr, *dout = nullptr;
+    size_t cap_in = 0, cap_out = 0;
+    hipStream_t stream = nullptr;
+};
+
+static int plan_buf(double** p, size_t& cap, size_t n) {
+    if (cap >= n) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr; cap = 0;
+    if (hipMalloc(p, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
+    cap = n;
+    return 0;
+}
+
+extern "C" {
+
+int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
+    if (!out) return DNS_EINVAL;
+    *out = nullptr;
+    if (precision != DNS_PREC_FP64) { g_create_error = "only FP64 plans are implemented"; return DNS_EUNSUPPORTED; }
+    if (nx < 4 || ny < 4 || (nx & 1) || (ny & 1)) { g_create_error = "nx, ny must be even and >= 4"; return DNS_EINVAL; }
+    std::string why;
+    if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
+    dns_fft_plan* p = new dns_fft_plan();
+    if (!fft_geom(nx, ny, p->g) || !make_factors(ny, p->g.fy) || !make_factors(nx / 2, p->g.fx)) {
+        delete p;
+        g_create_error = "unsupported plane size";
+        return DNS_EUNSUPPORTED;
+    }
+    std::vector<double> tw;
+    twiddles(nx, tw);
+    if (hipMalloc(&p->twx, tw.size() * sizeof(double)) != hipSuccess) { delete p; return DNS_ENOMEM; }
+    (void)hipMemcpy(p->twx, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    twiddles(ny, tw);
+    if (hipMalloc(&p->twy, tw.size() * sizeof(double)) != hipSuccess) { (void)hipFree(p->twx); delete p; return DNS_ENOMEM; }
+    (void)hipMemcpy(p->twy, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    p->g.twx = reinterpret_cast<const double2*>(p->twx);
+    p->g.twy = reinterpret_cast<const double2*>(p->twy);
+    (void)fft_prepare(p->g);
+    (void)hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    *out = p;
+    return 0;
+}
+
+int dns_fft_plan_destroy(dns_fft_plan* p) {
+    if (!p) return 0;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(p->twx); (void)hipFree(p->twy);
+    if (p->din) (void)hipFree(p->din);
+    if (p->dout) (void)hipFree(p->dout);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+int dns_fft_forward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    FFTArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.src[0] = in; a.dst[0] = out;
+    a.sstride = (long long)p->g.nx * p->g.ny; a.dstride = 2LL * p->g.nxh * p->g.ny;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    return launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, s) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    FFTArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.src[0] = in; a.dst[0] = out;
+    a.sstride = 2LL * p->g.nxh * p->g.ny; a.dstride = (long long)p->g.nx * p->g.ny;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, s) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_forward(dns_fft_plan* p, const double* in, double* out, int nplanes) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    size_t nr = size_t(p->g.nx) * p->g.ny * nplanes, nc = size_t(2) * p->g.nxh * p->g.ny * nplanes;
+    if (plan_buf(&p->din, p->cap_in, nr) || plan_buf(&p->dout, p->cap_out, nc)) return DNS_ENOMEM;
+    if (hipMemcpyAsync(p->din, in, nr * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) return DNS_EDEVICE;
+    int r = dns_fft_forward_device(p, p->din, p->dout, nplanes, p->stream);
+    if (r) return r;
+    if (hipMemcpyAsync(out, p->dout, nc * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) return DNS_EDEVICE;
+    return hipStreamSynchronize(p->stream) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_backward(dns_fft_plan* p, const double* in, double* out, int nplanes) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    size_t nr = size_t(p->g.nx) * p->g.ny * nplanes, nc = size_t(2) * p->g.nxh * p->g.ny * nplanes;
+    if (plan_buf(&p->din, p->cap_in, nc) || plan_buf(&p->dout, p->cap_out, nr)) return DNS_ENOMEM;
+    if (hipMemcpyAsync(p->din, in, nc * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) return DNS_EDEVICE;
+    int r = dns_fft_backward_device(p, p->din, p->dout, nplanes, p->stream);
+    if (r) return r;
+    if (hipMemcpyAsync(out, p->dout, nr * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) return DNS_EDEVICE;
+    return hipStreamSynchronize(p->stream) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// kernels.h -- argument blocks and host launchers of the device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace dns {
+
+// ---- plane FFT kernels ------------------------------------------------------------------
+enum C2ROp { C2R_PLAIN = 0, C2R_NL = 1, C2R_DIV = 2 };
+enum R2COp { R2C_PLAIN = 0, R2C_CROSS = 1 };
+
+struct FFTGeom {
+    int nx, ny, nxh, M;          // M = nx/2
+    Factors fy, fx;              // factors of ny and of M
+    const double2* twx;          // exp(-2 pi i k/nx), k < nx
+    const double2* twy;          // exp(-2 pi i k/ny), k < ny
+    int cls;                     // plane-size class (0, 1, 2), -1 = unsupported
+    size_t lds_bytes;
+};
+
+struct FFTArgs {
+    int nx, ny, nxh, M;
+    Factors fy, fx;
+    const double2* twx;
+    const double2* twy;
+    const double* xw;
+    const double* yw;
+    const unsigned char* mask;   // r2c: dealias mask (null = none)
+    const double* src[6];
+    double* dst[6];
+    long long sstride, dstride;  // doubles between consecutive planes of one field
+};
+
+bool fft_geom(int nx, int ny, FFTGeom& g);
+// raise the dynamic-LDS limit of the plane kernels of g's size class (once per process)
+hipError_t fft_prepare(const FFTGeom& g);
+// c2r: grid (nplanes, nfields); r2c likewise.  Return hipSuccess or the launch error.
+hipError_t launch_c2r(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, hipStream_t s);
+hipError_t launch_r2c(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, hipStream_t s);
+
+// ---- wall-normal contractions and per-mode solves ---------------------------------------
+struct ZArgs {
+    int nz, H2, nxh, ny, nx;     // H2 = 2 * (nx/2+1) * ny real columns per plane
+    const double* Asw; int KS;   // swizzled operator (nz rows)
+    const double* in[3];
+    double* out[3];
+    const double* xw; const double* yw;
+};
+enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
+hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s);
+
+struct HelmArgs {
+    int nz, m, H2, nxh, ny, nx;
+    const double* Fsw; int FKS;  // m x (m + nz) [T | f T D2]  (KS covers m cols when not CN)
+    const double* Bsw; int BKS;  // m x m
+    const double* c1; const double* c2; const double* hl;
+    double f, dt, nxny;
+    int cn;
+    const double* xw; const double* yw;
+    const double* uh[3]; const double* sh[3]; double* out[3];
+    const double* ph;
+    const double* G;             // device scalar: current pressure gradient
+};
+hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s);
+
+struct PoisArgs {
+    int nz, H2, nxh, ny, nx;
+    const double* Fsw; int FKS;  // nz x 2nz
+    const double* Bsw; const double* DBsw; int BKS;
+    const double* pl;
+    double dt;
+    int fused;                   // 1: apply correction + walls; 0: write phi, D1 phi only
+    const double* xw; const double* yw;
+    const double* us; const double* vs; const double* ws;   // u*, v*, w* spectral
+    double* u; double* v; double* w; double* p;             // fused outputs
+    double* phi; double* dphi;                              // stage outputs
+};
+hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s);
+
+struct Mode0Args {
+    int nz, H2;
+    const double* d1;            // row-major nz x nz
+    const double* m0;            // row-major (nz-1)^2
+    const double* wz;
+    double dt;
+    int fused;
+    const double* us; const double* vs; const double* ws;
+    double* u; double* v; double* w; double* p;
+    double* phi; double* dphi;
+};
+hipError_t launch_poisson_mode0(const Mode0Args& a, hipStream_t s);
+
+struct CorrArgs {
+    int nz, H2, nxh, ny, nx;
+    const double* xw; const double* yw;
+    const double* us; const double* vs; const double* ws;
+    const double* phi; const double* dphi;
+    double* u; double* v; double* w; double* p;
+    double dt;
+};
+hipError_t launch_correction(const CorrArgs& a, hipStream_t s);
+hipError_t launch_zero_walls(int nz, int H2, double* u, double* v, double* w, hipStream_t s);
+
+// PI / constant forcing state, device resident (update_pressure_gradient_3d)
+struct FlowState {
+    double G, Ub, err, err_int, err_prev, gsum, gavg;
+    int gcount, pad;
+};
+struct FlowArgs {
+    int nz, method, freq, istart;
+    double target_G, target_Ub, gain, re, dt, nxny, wsum;
+    const double* uh;            // u-hat: reads Re u(0,0,k) at stride H2
+    long long H2;
+    const double* wz;
+    FlowState* st;
+};
+hipError_t launch_flow_control(const FlowArgs& a, hipStream_t s);
+
+// reductions over physical fields: max|u|,|v|,|w|, sum 0.5|u|^2 w_k, max|d| (any may be null)
+struct ReduceArgs {
+    long long npl; int nz;
+    const double* u; const double* v; const double* w; const double* d;
+    const double* wz;
+    double* partial;             // nblocks * 5
+    double* result;              // 5
+    int nblocks;
+};
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+
+}  // namespace dns

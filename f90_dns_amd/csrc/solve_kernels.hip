@@ -1,0 +1,470 @@
+// solve_kernels.hip -- wall-normal (z) work on spectral columns: the LGL D1 contraction,
+// the Crank-Nicolson Helmholtz solves and the pressure Poisson solve + projection, each as
+// FP64 MFMA contractions over 16-column tiles per wavefront, plus the small kernels of the
+// step (mode (0,0) Poisson, correction, wall values, PI forcing, reductions).
+//
+// Spectral fields are complex(nx/2+1, ny, nz), plane-major; seen as real matrices they are
+// nz rows x H2 = 2 (nx/2+1) ny columns, column c = 2*mode + part (part 0 = Re, 1 = Im).
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace dns {
+
+struct ColInfo {
+    int c, mode, part, i, j;
+    bool valid;
+};
+__device__ __forceinline__ ColInfo col_info(int H2, int nxh) {
+    ColInfo ci;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    ci.c = blockIdx.x * 64 + wave * 16 + (lane & 15);
+    ci.valid = ci.c < H2;
+    ci.mode = ci.c >> 1;
+    ci.part = ci.c & 1;
+    ci.j = ci.mode / nxh;
+    ci.i = ci.mode - ci.j * nxh;
+    return ci;
+}
+
+// Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
+// the x Nyquist column gives an anti-Hermitian column (zero after the round trip), and so
+// does the y Nyquist row in the kx = 0 and kx = nx/2 columns.
+__device__ __forceinline__ double kx_eff(const double* xw, int i, int nx) { return i == nx / 2 ? 0.0 : xw[i]; }
+__device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx, int ny) {
+    return (j == ny / 2 && (i == 0 || i == nx / 2)) ? 0.0 : yw[j];
+}
+
+// -------------------------------------------------------------------------------------
+// z derivative: out = D1 in  (compute_derivatives_3d dz, DNS_pressure_BC_3D.f90:2348-2358),
+// or the vorticity components that need it: omega_x = i ky w - D1 v, omega_y = D1 u - i kx w
+// (DNS_pressure_BC_3D.f90:1077-1079).
+// -------------------------------------------------------------------------------------
+template <int RB, int MODE>
+__global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
+    const ColInfo ci = col_info(a.H2, a.nxh);
+    const int lane = threadIdx.x & 63;
+    const int H2 = a.H2, nz = a.nz;
+    const int nfield = MODE == Z_VORT ? 2 : 1;
+    for (int q = 0; q < nfield; q++) {
+        const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : blockIdx.y;   // vort: v first, then u
+        const double* in = a.in[f];
+        d4 acc[RB];
+        zero_acc(acc);
+        mfma_tile<RB>(acc, a.Asw, a.KS, [&](int k) -> double {
+            return (ci.valid && k < nz) ? in[size_t(k) * H2 + ci.c] : 0.0;
+        });
+        if (!ci.valid) continue;
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = rb * 16 + 4 * (lane >> 4) + r;
+                if (row >= nz) continue;
+                const double y = acc[rb][r];
+                if (MODE == Z_PLAIN) {
+                    a.out[f][size_t(row) * H2 + ci.c] = y;
+                } else {
+                    const double2 w = reinterpret_cast<const double2*>(a.in[2] + size_t(row) * H2)[ci.mode];
+                    if (f == 1) {        // omega_x = i ky w - D1 v
+                        const double ky = a.yw[ci.j];
+                        a.out[0][size_t(row) * H2 + ci.c] = ci.part == 0 ? (-ky * w.y - y) : (ky * w.x - y);
+                    } else {             // omega_y = D1 u - i kx w
+                        const double kx = a.xw[ci.i];
+                        a.out[1][size_t(row) * H2 + ci.c] = ci.part == 0 ? (y + kx * w.y) : (y - kx * w.x);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// Helmholtz, one velocity component per grid.y (solve_viscous_helmholtz,
+// DNS_pressure_BC_3D.f90:1425-1521, with the explicit half of Crank-Nicolson from
+// add_explicit_diffusion :1355-1381 and the forcing of :1107-1148 folded in):
+//   interior y = T (alpha u_I + dt S'_I) + f (T D2_I) u + c1 bc1 + c2 bc2,
+//   z = y / (hl + f k^2), u*_I = B z, u*_0 = bc1, u*_{nz-1} = bc2,
+// alpha = 1 - f k^2 (CN) or 1, S' = S + G nx ny at mode (0,0) of the u component, and
+// bc = -dt (i k_eff p_hat) at the walls (the Kim & Moin values -dt grad p, :1325-1328).
+// -------------------------------------------------------------------------------------
+template <int RBI>
+__global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
+    __shared__ double zs[4][RBI * 16][16];
+    const ColInfo ci = col_info(a.H2, a.nxh);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
+    const int comp = blockIdx.y;
+    const int H2 = a.H2, nz = a.nz, m = a.m;
+    const double* uh = a.uh[comp];
+    const double* sh = a.sh[comp];
+    double k2 = 0.0, bc1 = 0.0, bc2 = 0.0, force = 0.0;
+    if (ci.valid) {
+        const double kx = a.xw[ci.i], ky = a.yw[ci.j];
+        k2 = kx * kx + ky * ky;
+        if (comp < 2) {
+            const double ke = comp == 0 ? kx_eff(a.xw, ci.i, a.nx) : ky_eff(a.yw, ci.i, ci.j, a.nx, a.ny);
+            const double2 p0 = reinterpret_cast<const double2*>(a.ph)[ci.mode];
+            const double2 p1 = reinterpret_cast<const double2*>(a.ph + size_t(nz - 1) * H2)[ci.mode];
+            // -dt * (i ke p)
+            bc1 = ci.part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x);
+            bc2 = ci.part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x);
+        }
+        if (comp == 0 && ci.c == 0) force = (*a.G) * a.nxny;
+    }
+    const double alpha = a.cn ? 1.0 - a.f * k2 : 1.0;
+    d4 acc[RBI];
+    zero_acc(acc);
+    mfma_tile<RBI>(acc, a.Fsw, a.FKS, [&](int k) -> double {
+        if (!ci.valid) return 0.0;
+        if (k < m) {
+            const size_t q = size_t(k + 1) * H2 + ci.c;
+            return alpha * uh[q] + a.dt * (sh[q] + force);
+        }
+        if (k < m + nz) return uh[size_t(k - m) * H2 + ci.c];
+        return 0.0;
+    });
+#pragma unroll
+    for (int rb = 0; rb < RBI; rb++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            double z = 0.0;
+            if (row < m && ci.valid) z = (acc[rb][r] + a.c1[row] * bc1 + a.c2[row] * bc2) / (a.hl[row] + a.f * k2);
+            zs[wave][row][col] = z;
+        }
+    }
+    __syncthreads();
+    zero_acc(acc);
+    mfma_tile<RBI>(acc, a.Bsw, a.BKS, [&](int k) -> double { return k < RBI * 16 ? zs[wave][k][col] : 0.0; });
+    if (!ci.valid) return;
+    double* out = a.out[comp];
+#pragma unroll
+    for (int rb = 0; rb < RBI; rb++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            if (row < m) out[size_t(row + 1) * H2 + ci.c] = acc[rb][r];
+        }
+    }
+    if ((lane >> 4) == 0) {
+        out[ci.c] = bc1;
+        out[size_t(nz - 1) * H2 + ci.c] = bc2;
+    }
+}
+
+// -------------------------------------------------------------------------------------
+// Pressure (solve_pressure_3d_standard :1654-1733 with compute_divergence_3d :1858-1879)
+// and, when fused, the projection + wall values (pressure_correction_3d :1903-1929,
+// apply_boundary_conditions_3d :1953-1972):
+//   y = Tf div + (Tf D1) w*,  div = i kx_eff u* + i ky_eff v*,
+//   z = -y / (dt (lf + k^2)), phi = Bf z, D1 phi = (D1 Bf) z,
+//   u = u* - dt i kx_eff phi, v = v* - dt i ky_eff phi, w = w* - dt D1 phi, p += phi.
+// Mode (0,0) is left to k_poisson_mode0.
+// -------------------------------------------------------------------------------------
+template <int RB>
+__global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
+    __shared__ double zs[4][RB * 16][16];
+    const ColInfo ci = col_info(a.H2, a.nxh);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
+    const int H2 = a.H2, nz = a.nz;
+    double k2 = 0.0, kxe = 0.0, kye = 0.0;
+    if (ci.valid) {
+        const double kx = a.xw[ci.i], ky = a.yw[ci.j];
+        k2 = kx * kx + ky * ky;
+        kxe = kx_eff(a.xw, ci.i, a.nx);
+        kye = ky_eff(a.yw, ci.i, ci.j, a.nx, a.ny);
+    }
+    const bool live = ci.valid && ci.mode != 0;
+    d4 acc[RB];
+    zero_acc(acc);
+    mfma_tile<RB>(acc, a.Fsw, a.FKS, [&](int k) -> double {
+        if (!live) return 0.0;
+        if (k < nz) {
+            const double2 u = reinterpret_cast<const double2*>(a.us + size_t(k) * H2)[ci.mode];
+            const double2 v = reinterpret_cast<const double2*>(a.vs + size_t(k) * H2)[ci.mode];
+            return ci.part == 0 ? (-kxe * u.y - kye * v.y) : (kxe * u.x + kye * v.x);
+        }
+        if (k < 2 * nz) return a.ws[size_t(k - nz) * H2 + ci.c];
+        return 0.0;
+    });
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            double z = 0.0;
+            if (row < nz && live) z = -acc[rb][r] / (a.dt * (a.pl[row] + k2));
+            zs[wave][row][col] = z;
+        }
+    }
+    __syncthreads();
+    d4 phi[RB], dphi[RB];
+    zero_acc(phi);
+    zero_acc(dphi);
+    mfma_tile<RB>(phi, a.Bsw, a.BKS, [&](int k) -> double { return k < RB * 16 ? zs[wave][k][col] : 0.0; });
+    mfma_tile<RB>(dphi, a.DBsw, a.BKS, [&](int k) -> double { return k < RB * 16 ? zs[wave][k][col] : 0.0; });
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            const double ph = phi[rb][r];
+            const double other = __shfl_xor(ph, 1);       // the other part of this mode
+            if (!live || row >= nz) continue;
+            const size_t q = size_t(row) * H2 + ci.c;
+            if (!a.fused) {
+                a.phi[q] = ph;
+                a.dphi[q] = dphi[rb][r];
+                continue;
+            }
+            const double pre = ci.part == 0 ? ph : other, pim = ci.part == 0 ? other : ph;
+            const bool wall = row == 0 || row == nz - 1;
+            // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
+            const double du = ci.part == 0 ? -kxe * pim : kxe * pre;
+            const double dv = ci.part == 0 ? -kye * pim : kye * pre;
+            a.u[q] = wall ? 0.0 : a.us[q] - a.dt * du;
+            a.v[q] = wall ? 0.0 : a.vs[q] - a.dt * dv;
+            a.w[q] = wall ? 0.0 : a.ws[q] - a.dt * dphi[rb][r];
+            a.p[q] = a.p[q] + ph;
+        }
+    }
+}
+
+// mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
+// (DNS_pressure_BC_3D.f90:1702-1711): phi_0 = 0, phi_{1:} = M0 b_{1:}, b = w D1 w* / dt.
+__global__ __launch_bounds__(256) void k_poisson_mode0(Mode0Args a) {
+    extern __shared__ double sh[];
+    const int nz = a.nz;
+    double* ws = sh;            // [2][nz]
+    double* b = sh + 2 * nz;    // [2][nz]
+    double* x = sh + 4 * nz;    // [2][nz]
+    const int t = threadIdx.x;
+    for (int e = t; e < 2 * nz; e += blockDim.x) {
+        const int part = e / nz, k = e - part * nz;
+        ws[e] = a.ws[size_t(k) * a.H2 + part];
+    }
+    __syncthreads();
+    for (int e = t; e < 2 * nz; e += blockDim.x) {
+        const int part = e / nz, k = e - part * nz;
+        double s = 0.0;
+        for (int q = 0; q < nz; q++) s += a.d1[size_t(k) * nz + q] * ws[part * nz + q];
+        b[e] = a.wz[k] * s / a.dt;
+    }
+    __syncthreads();
+    for (int e = t; e < 2 * nz; e += blockDim.x) {
+        const int part = e / nz, k = e - part * nz;
+        double s = 0.0;
+        if (k > 0)
+            for (int q = 1; q < nz; q++) s += a.m0[size_t(k - 1) * (nz - 1) + (q - 1)] * b[part * nz + q];
+        x[e] = s;
+    }
+    __syncthreads();
+    for (int e = t; e < 2 * nz; e += blockDim.x) {
+        const int part = e / nz, k = e - part * nz;
+        double dx = 0.0;
+        for (int q = 0; q < nz; q++) dx += a.d1[size_t(k) * nz + q] * x[part * nz + q];
+        const size_t o = size_t(k) * a.H2 + part;
+        if (!a.fused) {
+            a.phi[o] = x[e];
+            a.dphi[o] = dx;
+        } else {
+            const bool wall = k == 0 || k == nz - 1;
+            a.u[o] = wall ? 0.0 : a.us[o];
+            a.v[o] = wall ? 0.0 : a.vs[o];
+            a.w[o] = wall ? 0.0 : ws[e] - a.dt * dx;
+            a.p[o] = a.p[o] + x[e];
+        }
+    }
+}
+
+// projection of the stage API (pressure_correction_3d :1903-1929), from stored phi, D1 phi
+__global__ __launch_bounds__(256) void k_correction(CorrArgs a) {
+    const size_t total = size_t(a.nz) * (a.H2 / 2);
+    const int nxh = a.nxh;
+    for (size_t e = blockIdx.x * size_t(blockDim.x) + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+        const size_t mode_all = e;
+        const int mode = int(mode_all % (a.H2 / 2));
+        const int j = mode / nxh, i = mode - j * nxh;
+        const double kxe = kx_eff(a.xw, i, a.nx), kye = ky_eff(a.yw, i, j, a.nx, a.ny);
+        const double2 ph = reinterpret_cast<const double2*>(a.phi)[e];
+        const double2 dp = reinterpret_cast<const double2*>(a.dphi)[e];
+        const double2 us = reinterpret_cast<const double2*>(a.us)[e];
+        const double2 vs = reinterpret_cast<const double2*>(a.vs)[e];
+        const double2 ws = reinterpret_cast<const double2*>(a.ws)[e];
+        reinterpret_cast<double2*>(a.u)[e] = make_double2(us.x - a.dt * (-kxe * ph.y), us.y - a.dt * (kxe * ph.x));
+        reinterpret_cast<double2*>(a.v)[e] = make_double2(vs.x - a.dt * (-kye * ph.y), vs.y - a.dt * (kye * ph.x));
+        reinterpret_cast<double2*>(a.w)[e] = make_double2(ws.x - a.dt * dp.x, ws.y - a.dt * dp.y);
+        double2 p = reinterpret_cast<double2*>(a.p)[e];
+        reinterpret_cast<double2*>(a.p)[e] = make_double2(p.x + ph.x, p.y + ph.y);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, double* u, double* v, double* w) {
+    const size_t top = size_t(nz - 1) * H2;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < H2; c += gridDim.x * blockDim.x) {
+        u[c] = 0.0; v[c] = 0.0; w[c] = 0.0;
+        u[top + c] = 0.0; v[top + c] = 0.0; w[top + c] = 0.0;
+    }
+}
+
+// apply_flow_control_forcing / update_pressure_gradient_3d / calculate_bulk_velocity_3d
+// (DNS_pressure_BC_3D.f90:1107-1148, 2434-2549), called with istart as in :1118.
+// U_b = sum_k w_k Re u_hat(0,0,k) / (nx ny sum w): the (0,0) coefficient of the
+// unnormalised forward transform is the plane sum.
+__global__ void k_flow_control(FlowArgs a) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    FlowState s = *a.st;
+    if (a.method == 1) {
+        s.G = a.target_G;
+    } else if (a.istart % a.freq == 0) {
+        double vol = 0.0;
+        for (int k = 0; k < a.nz; k++) vol += a.uh[size_t(k) * a.H2] * a.wz[k];
+        s.Ub = vol / (a.nxny * a.wsum);
+        s.err = a.target_Ub - s.Ub;
+        const double kp = a.gain, ki = 0.05 * a.gain;
+        const double gu = 3.0 / a.re + kp * s.err + ki * s.err_int;
+        const double gc = fmax(0.0, fmin(10.0, gu));
+        if (fabs(gc - gu) < 1.0e-12) {
+            if (a.istart <= a.freq) s.err_int = s.err_int + s.err * a.dt;
+            else s.err_int = s.err_int + 0.5 * a.dt * (s.err_prev + s.err);
+        }
+        s.err_prev = s.err;
+        s.G = gc;
+        s.gsum += s.G;
+        s.gcount += 1;
+        s.gavg = s.gsum / double(s.gcount);
+    }
+    *a.st = s;
+}
+
+// partial reductions: [max|u|, max|v|, max|w|, sum 0.5 |u|^2 w_k, max|d|]
+__global__ __launch_bounds__(256) void k_reduce_partial(ReduceArgs a) {
+    __shared__ double red[5][256];
+    double mu = 0, mv = 0, mw = 0, e = 0, md = 0;
+    const size_t N = size_t(a.npl) * a.nz;
+    for (size_t n = blockIdx.x * size_t(blockDim.x) + threadIdx.x; n < N; n += size_t(gridDim.x) * blockDim.x) {
+        const int k = int(n / size_t(a.npl));
+        double uu = 0, vv = 0, ww = 0;
+        if (a.u) { uu = a.u[n]; mu = fmax(mu, fabs(uu)); }
+        if (a.v) { vv = a.v[n]; mv = fmax(mv, fabs(vv)); }
+        if (a.w) { ww = a.w[n]; mw = fmax(mw, fabs(ww)); }
+        e += 0.5 * (uu * uu + vv * vv + ww * ww) * a.wz[k];
+        if (a.d) md = fmax(md, fabs(a.d[n]));
+    }
+    const int t = threadIdx.x;
+    red[0][t] = mu; red[1][t] = mv; red[2][t] = mw; red[3][t] = e; red[4][t] = md;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            red[0][t] = fmax(red[0][t], red[0][t + s]);
+            red[1][t] = fmax(red[1][t], red[1][t + s]);
+            red[2][t] = fmax(red[2][t], red[2][t + s]);
+            red[3][t] = red[3][t] + red[3][t + s];
+            red[4][t] = fmax(red[4][t], red[4][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t < 5) a.partial[size_t(blockIdx.x) * 5 + t] = red[t][0];
+}
+
+__global__ void k_reduce_final(ReduceArgs a) {
+    if (threadIdx.x != 0) return;
+    double r[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < a.nblocks; b++) {
+        const double* p = a.partial + size_t(b) * 5;
+        r[0] = fmax(r[0], p[0]); r[1] = fmax(r[1], p[1]); r[2] = fmax(r[2], p[2]);
+        r[3] += p[3]; r[4] = fmax(r[4], p[4]);
+    }
+    for (int i = 0; i < 5; i++) a.result[i] = r[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------
+static inline dim3 col_grid(int H2, int y) { return dim3((H2 + 63) / 64, y); }
+
+template <int RB>
+static hipError_t zderiv_rb(int mode, const ZArgs& a, int nf, hipStream_t s) {
+    if (mode == Z_VORT) hipLaunchKernelGGL((k_zderiv<RB, Z_VORT>), col_grid(a.H2, 1), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_zderiv<RB, Z_PLAIN>), col_grid(a.H2, nf), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <int RB>
+static hipError_t helm_rb(const HelmArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_helmholtz<RB>), col_grid(a.H2, 3), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <int RB>
+static hipError_t pois_rb(const PoisArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((k_poisson<RB>), col_grid(a.H2, 1), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s) {
+    switch (rb) {
+        case 1: return zderiv_rb<1>(mode, a, nf, s);
+        case 2: return zderiv_rb<2>(mode, a, nf, s);
+        case 3: return zderiv_rb<3>(mode, a, nf, s);
+        case 4: return zderiv_rb<4>(mode, a, nf, s);
+        case 5: return zderiv_rb<5>(mode, a, nf, s);
+        case 6: return zderiv_rb<6>(mode, a, nf, s);
+        case 7: return zderiv_rb<7>(mode, a, nf, s);
+        case 8: return zderiv_rb<8>(mode, a, nf, s);
+        case 9: return zderiv_rb<9>(mode, a, nf, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s) {
+    switch (rbi) {
+        case 1: return helm_rb<1>(a, s);
+        case 2: return helm_rb<2>(a, s);
+        case 3: return helm_rb<3>(a, s);
+        case 4: return helm_rb<4>(a, s);
+        case 5: return helm_rb<5>(a, s);
+        case 6: return helm_rb<6>(a, s);
+        case 7: return helm_rb<7>(a, s);
+        case 8: return helm_rb<8>(a, s);
+        case 9: return helm_rb<9>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s) {
+    switch (rb) {
+        case 1: return pois_rb<1>(a, s);
+        case 2: return pois_rb<2>(a, s);
+        case 3: return pois_rb<3>(a, s);
+        case 4: return pois_rb<4>(a, s);
+        case 5: return pois_rb<5>(a, s);
+        case 6: return pois_rb<6>(a, s);
+        case 7: return pois_rb<7>(a, s);
+        case 8: return pois_rb<8>(a, s);
+        case 9: return pois_rb<9>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+hipError_t launch_poisson_mode0(const Mode0Args& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_poisson_mode0, dim3(1), dim3(256), size_t(6) * a.nz * sizeof(double), s, a);
+    return hipGetLastError();
+}
+hipError_t launch_correction(const CorrArgs& a, hipStream_t s) {
+    const size_t total = size_t(a.nz) * (a.H2 / 2);
+    int blocks = int(std::min<size_t>((total + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_correction, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_zero_walls(int nz, int H2, double* u, double* v, double* w, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_walls, dim3((H2 + 255) / 256), dim3(256), 0, s, nz, H2, u, v, w);
+    return hipGetLastError();
+}
+hipError_t launch_flow_control(const FlowArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_flow_control, dim3(1), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partial, dim3(a.nblocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace dns

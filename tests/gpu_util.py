@@ -1,0 +1,28 @@
+"""Helpers shared by the GPU parity tests (oracle = CPU FP64 restatement)."""
+import numpy as np
+
+from oracle import oracle as orc
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+# config 1 (32x32x33, Lx=2pi, Ly=pi, Re=180, dt=0.002, G=2/Re, analytical mx=my=1 amp 0.01, dealias)
+CONFIG1 = dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+               use_dealias_23_rule=1, flow_control_method=1, target_pressure_gradient=2.0 / 180.0,
+               target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, controller_update_freq=7, istart=0,
+               u_centerline=1.0, enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1,
+               analytical_mode_y=1, perturbation_amplitude=0.01, seed=0, noise_kmax=8)
+
+
+def pair(**kw):
+    """An oracle and a GPU simulation on identical initial fields (the oracle's)."""
+    import f90_dns_amd as dns
+    o = orc.Oracle(**kw)
+    o.init_fields()
+    g = dns.Simulation(**kw)
+    u, v, w, p = o.get_fields()
+    g.set_fields(u, v, w, p)
+    return o, g

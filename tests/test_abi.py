@@ -1,0 +1,56 @@
+"""CPU-side checks of the C ABI library: it builds, loads and exports every declared symbol."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names.update(re.findall(r"\b(dns_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import f90_dns_amd
+    path = f90_dns_amd.library_path()
+    if not os.path.exists(path):
+        pytest.skip("libf90dns.so not built")
+    L = ctypes.CDLL(path)
+    missing = [n for n in sorted(declared_symbols()) if not hasattr(L, n)]
+    assert not missing, missing
+    assert len(declared_symbols()) > 25
+
+
+def test_version_string_without_gpu():
+    import f90_dns_amd
+    if not os.path.exists(f90_dns_amd.library_path()):
+        pytest.skip("libf90dns.so not built")
+    assert b"gfx950" in f90_dns_amd.lib().dns_version()
+
+
+def test_params_struct_matches_header():
+    """The ctypes mirror of dns_params lists the header's fields in order."""
+    import f90_dns_amd
+    txt = open(os.path.join(ROOT, "include", "f90dns.h")).read()
+    body = txt[txt.index("typedef struct dns_params"):txt.index("} dns_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl or decl.startswith("typedef"):
+            decl = decl.split("{", 1)[-1].strip()
+        if not decl:
+            continue
+        parts = decl.replace(",", " , ").split()
+        names = [t for t in parts[1:] if t not in (",", "long", "unsigned", "int", "double")]
+        fields += names
+    mine = [f for f, _ in f90_dns_amd.DnsParams._fields_]
+    assert fields == mine

@@ -1,0 +1,188 @@
+"""Parity of the HIP path (through the C ABI) with the CPU FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import CONFIG1, pair, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dns():
+    import f90_dns_amd
+    f90_dns_amd.lib()
+    return f90_dns_amd
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (96, 96), (128, 128), (128, 32), (96, 64), (48, 36)])
+def test_fft_forward_matches_numpy(dns, nx, ny):
+    _say(f"fft forward {nx}x{ny}")
+    rng = np.random.default_rng(nx + ny)
+    f = rng.standard_normal((5, ny, nx))
+    X = dns.FFTPlan(nx, ny).forward(f)
+    ref = np.fft.rfft2(f)
+    assert np.max(np.abs(X - ref)) < 1e-12 * np.sqrt(nx * ny) * np.abs(ref).max() / np.sqrt(nx * ny) + 1e-11
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (96, 96), (128, 128), (128, 32), (48, 36)])
+def test_fft_backward_non_hermitian_matches_oracle(dns, nx, ny):
+    _say(f"fft backward {nx}x{ny}")
+    rng = np.random.default_rng(7 + nx)
+    F = rng.standard_normal((3, ny, nx // 2 + 1)) + 1j * rng.standard_normal((3, ny, nx // 2 + 1))
+    got = dns.FFTPlan(nx, ny).backward(F)
+    want = orc.fft_c2r(F, nx)
+    assert np.max(np.abs(got - want)) < 1e-14 * max(1.0, np.abs(F).max())
+
+
+@pytest.mark.parametrize("nz", [17, 33, 65, 129])
+def test_zderiv_mfma_matches_numpy(dns, nz):
+    _say(f"zderiv nz={nz}")
+    g = dns.Simulation(nx=32, ny=16, nz=nz)
+    rng = np.random.default_rng(nz)
+    spec = rng.standard_normal(g.spectral_shape) + 1j * rng.standard_normal(g.spectral_shape)
+    D = orc.diff_matrix(nz)
+    want = np.einsum("km,myx->kyx", D, spec)
+    got = g.zderiv(spec)
+    assert np.max(np.abs(got - want)) < 1e-13 * np.abs(D).max() * np.abs(spec).max() * nz
+
+
+def test_operator_bases_reconstruct(dns):
+    for nz in (17, 33, 65, 129):
+        g = dns.Simulation(nx=16, ny=16, nz=nz, dt=0.002, re=180.0)
+        eh, ep = g.operator_error()
+        assert eh < 1e-15 and ep < 1e-15, (nz, eh, ep)
+
+
+def test_stage_parity_config1(dns):
+    """Each stage of one step against the oracle, on identical fields."""
+    _say("stage parity config 1")
+    o, g = pair(**CONFIG1)
+    nx, ny = CONFIG1["nx"], CONFIG1["ny"]
+    # warm both up 3 steps so p, wall gradients and all modes are nonzero
+    o.step(3)
+    for _ in range(3):
+        g.step(1)
+    g.synchronize()
+    u, v, w, p = o.get_fields()
+    gu, gv, gw, gp = g.get_fields()
+    for a, b in ((gu, u), (gv, v), (gw, w), (gp, p)):
+        assert rel_l2(a, b) < 1e-12
+    o.time_step_begin = None
+    g.begin_step()
+    o.compute_source_terms(); g.compute_source_terms()
+    G = o.diagnostics()["pressure_gradient"]
+    for c, s in enumerate(o.source_terms()):
+        sh = orc.fft_r2c(s)
+        if c == 0:
+            sh[:, 0, 0] -= G * nx * ny
+        got = g.spectral(7 + c)
+        assert rel_l2(got, sh) < 1e-11, c
+    o.viscous_step(); g.viscous_step()
+    for c, s in enumerate(o.star_fields()):
+        assert rel_l2(g.spectral(4 + c), orc.fft_r2c(s)) < 1e-11, c
+    o.solve_pressure(); g.solve_pressure()
+    assert rel_l2(g.spectral(12), orc.fft_r2c(o.phi())) < 1e-10
+    o.pressure_correction(); g.pressure_correction()
+    o.apply_boundary_conditions(); g.apply_boundary_conditions()
+    for a, b in zip(g.get_fields(), o.get_fields()):
+        assert rel_l2(a, b) < 1e-11
+
+
+def test_fused_step_equals_stages(dns):
+    _say("fused vs stage API")
+    _, g1 = pair(**CONFIG1)
+    _, g2 = pair(**CONFIG1)
+    for _ in range(4):
+        g1.step(1)
+        g2.begin_step(); g2.compute_source_terms(); g2.viscous_step(); g2.solve_pressure()
+        g2.pressure_correction(); g2.apply_boundary_conditions()
+    for a, b in zip(g1.get_fields(), g2.get_fields()):
+        assert rel_l2(a, b) < 1e-14
+
+
+def test_graph_step_equals_eager(dns):
+    _say("graph vs eager")
+    _, g1 = pair(**CONFIG1)
+    _, g2 = pair(**CONFIG1)
+    g1.step(5)
+    g2.step(2, graph=True)
+    g2.synchronize()
+    g2.step(3, graph=True)
+    for a, b in zip(g1.get_fields(), g2.get_fields()):
+        np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("dealias", [1, 0])
+def test_config1_100_steps(dns, dealias):
+    """u, v, w, p within relative L2 1e-10 of the FP64 oracle after 100 steps,
+    with the same energy history and max divergence order."""
+    _say(f"config1 100 steps dealias={dealias}")
+    kw = dict(CONFIG1, use_dealias_23_rule=dealias)
+    o, g = pair(**kw)
+    for block in range(10):
+        o.step(10)
+        g.step(10)
+        do, dg = o.diagnostics(), g.diagnostics()
+        assert abs(do["energy"] - dg["energy"]) < 1e-12 * do["energy"], block
+        assert abs(do["wmax"] - dg["wmax"]) < 1e-10 * max(do["wmax"], 1e-30) + 1e-18
+    fo = o.get_fields(); fg = g.get_fields()
+    errs = [rel_l2(a, b) for a, b in zip(fg, fo)]
+    _say(f"  rel L2 u v w p = {errs}")
+    assert max(errs) < 1e-10, errs
+    do_, dg_ = o.max_divergence(), g.max_divergence()
+    assert dg_ < 10 * do_ + 1e-13
+
+
+def test_pi_controller_trace(dns):
+    """Method 2 (constant bulk velocity), controller every step via istart = 0."""
+    _say("PI controller")
+    kw = dict(CONFIG1, flow_control_method=2, perturbation_kind=1, seed=11, noise_kmax=4,
+              perturbation_amplitude=0.05)
+    o, g = pair(**kw)
+    for _ in range(20):
+        o.step(1); g.step(1)
+        do, dg = o.diagnostics(), g.diagnostics()
+        for key in ("pressure_gradient", "bulk_velocity", "avg_pressure_gradient", "bulk_error_integral"):
+            assert abs(do[key] - dg[key]) < 1e-12 * max(1.0, abs(do[key])), key
+    for a, b in zip(g.get_fields(), o.get_fields()):
+        assert rel_l2(a, b) < 1e-10
+
+
+def test_trace_config_matches_reference_stdout(dns):
+    """The reference's own 15-step trace (standard_output.txt:88-104) from the GPU path."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "standard_output_trace.json")))
+    g = dns.Simulation(nx=32, ny=32, nz=17, xlen=2 * np.pi, ylen=np.pi, re=500.0, dt=0.002, u_centerline=1.5,
+                       flow_control_method=1, target_pressure_gradient=0.006, use_dealias_23_rule=1,
+                       enable_perturbations=1, perturbation_kind=0, perturbation_amplitude=0.01)
+    g.init_fields()
+    for row in fx["rows"]:
+        g.step(1)
+        d = g.diagnostics()
+        got = ["%.4E" % d[f] for f in ("umax", "vmax", "wmax", "energy", "pressure_gradient")]
+        want = ["%.4E" % float(row[k]) for k in ("umax", "vmax", "wmax", "E", "dPdx")]
+        assert got == want, row["step"]
+
+
+def test_seeded_noise_production_shape_short(dns):
+    """128x128x129, method 2, seeded noise: 3 steps against the oracle."""
+    _say("128^2x129 3 steps")
+    kw = dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+              use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,
+              target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, u_centerline=1.0, enable_perturbations=1,
+              perturbation_kind=1, seed=2024, noise_kmax=8, perturbation_amplitude=0.05)
+    o, g = pair(**kw)
+    o.step(3); g.step(3)
+    fo, fg = o.get_fields(), g.get_fields()
+    errs = [rel_l2(a, b) for a, b in zip(fg[:3], fo[:3])]
+    _say(f"  rel L2 u v w = {errs}")
+    assert max(errs) < 1e-10
