@@ -2,7 +2,9 @@
 
 Metric: grid-point steps per second (nx*ny*nz x full fractional steps / wall second) of the
 production shape 128x128x129 (Lx = 4 pi, Ly = 2 pi, Re = 180, dt = 0.002, PI control on the
-bulk velocity, 2/3 dealiasing, seeded stream-function noise of amplitude 0.05), FP64, state
+bulk velocity, 2/3 dealiasing, seeded stream-function noise of amplitude 0.05, exact LGL D1 --
+the reference's thresholded D1 is diagonal at nz = 129 and the run goes non-finite within ~90
+steps; both D1 choices cost the same on this path), FP64, state
 resident in HBM, steps replayed from a captured HIP graph.  Prints one JSON line.
 """
 import argparse
@@ -25,7 +27,7 @@ WORKLOADS = {
         use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,
         target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, controller_update_freq=7, istart=0,
         u_centerline=1.0, enable_perturbations=1, perturbation_kind=1, seed=2024, noise_kmax=8,
-        perturbation_amplitude=0.05),
+        perturbation_amplitude=0.05, lgl_derivative=1),
     "channel_32x32x33_fp64": dict(
         nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
         use_dealias_23_rule=1, flow_control_method=1, target_pressure_gradient=2.0 / 180.0,

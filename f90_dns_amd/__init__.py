@@ -37,7 +37,7 @@ class DnsParams(ctypes.Structure):
         ("enable_perturbations", ctypes.c_int), ("perturbation_kind", ctypes.c_int),
         ("analytical_mode_x", ctypes.c_int), ("analytical_mode_y", ctypes.c_int),
         ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
-        ("precision", ctypes.c_int),
+        ("precision", ctypes.c_int), ("lgl_derivative", ctypes.c_int),
     ]
 
 
@@ -57,6 +57,7 @@ DEFAULTS = dict(
     target_pressure_gradient=0.03, target_bulk_velocity=1.0, controller_gain=0.15, controller_update_freq=7,
     istart=0, u_centerline=1.5, enable_perturbations=0, perturbation_kind=0, analytical_mode_x=1,
     analytical_mode_y=1, perturbation_amplitude=0.01, seed=0, noise_kmax=8, precision=DNS_PREC_FP64,
+    lgl_derivative=0,
 )
 
 _lib = None
@@ -103,6 +104,8 @@ def lib():
             getattr(L, n).argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.dns_debug_spectral.argtypes = [_vp, ctypes.c_int, _vp]
         L.dns_debug_zderiv.argtypes = [_vp, _vp, _vp]
+        L.dns_debug_zderiv_variant.argtypes = [_vp, ctypes.c_int, _vp, _vp]
+        L.dns_debug_mfma_probe.argtypes = [_dp]
         L.dns_debug_operator_error.argtypes = [_vp, _dp]
         L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_profile_kernel_name.argtypes = [ctypes.c_int]

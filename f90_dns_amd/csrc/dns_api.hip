@@ -169,7 +169,7 @@ int dns_create(const dns_params* p, dns_ctx** out) {
         return bail(DNS_EDEVICE);
     }
     std::string err;
-    if (!build_operators(c->nz, p->dt, p->re, c->theta, p->use_crank_nicolson != 0, c->op, err)) {
+    if (!build_operators(c->nz, p->dt, p->re, c->theta, p->use_crank_nicolson != 0, p->lgl_derivative != 0, c->op, err)) {
         g_create_error = err;
         return bail(DNS_ESOLVER);
     }
@@ -604,6 +604,20 @@ int dns_debug_spectral(dns_ctx* c, int which, double* out) {
     return 0;
 }
 
+int dns_debug_zderiv_variant(dns_ctx* c, int variant, const double* spec_in, double* spec_out) {
+    if (!c || !spec_in || !spec_out || variant < 0 || variant > 2) return DNS_EINVAL;
+    if (variant == 0) return dns_debug_zderiv(c, spec_in, spec_out);
+    HIPCHECK(c, hipMemcpyAsync(c->tmp, spec_in, c->NS * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    ZArgs z{};
+    z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
+    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.in[0] = c->tmp; z.out[0] = c->phi; z.xw = c->xw; z.yw = c->yw;
+    HIPCHECK(c, launch_zderiv_variant(c->rb, variant, z, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(spec_out, c->phi, c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int dns_debug_zderiv(dns_ctx* c, const double* spec_in, double* spec_out) {
     if (!c || !spec_in || !spec_out) return DNS_EINVAL;
     HIPCHECK(c, hipMemcpyAsync(c->tmp, spec_in, c->NS * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -674,6 +688,19 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
     }
     for (int i = 0; i < NK; i++) out[i] = acc[i] / nsteps;
     return 0;
+}
+
+int dns_debug_mfma_probe(double* out) {
+    if (!out) return DNS_EINVAL;
+    std::string why;
+    if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
+    double* d = nullptr;
+    const size_t n = size_t(4096) * 256;
+    if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
+    hipError_t e = launch_mfma_probe(d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
 int dns_debug_operator_error(dns_ctx* c, double* out2) {

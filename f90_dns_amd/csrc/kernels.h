@@ -49,6 +49,8 @@ struct ZArgs {
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
 hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s);
+// out[0] = D1 in[0] with a variant of the contraction loop (1 branch-free loads, 2 delays)
+hipError_t launch_zderiv_variant(int rb, int var, const ZArgs& a, hipStream_t s);
 
 struct HelmArgs {
     int nz, m, H2, nxh, ny, nx;
@@ -127,5 +129,8 @@ struct ReduceArgs {
     int nblocks;
 };
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+
+// 4096 one-hot products of v_mfma_f64_16x16x4f64 (out: 4096 * 256 doubles)
+hipError_t launch_mfma_probe(double* out, hipStream_t s);
 
 }  // namespace dns

@@ -79,6 +79,30 @@ void differentiation_matrix(int n, const std::vector<double>& x, std::vector<dou
     }
 }
 
+void differentiation_matrix_exact(int n, const std::vector<double>& x, std::vector<double>& d) {
+    d.assign(size_t(n) * n, 0.0);
+    const int N = n - 1;
+    std::vector<long double> p(n);
+    for (int i = 0; i < n; i++) {
+        long double p0 = 1.0L, p1 = (long double)x[i];
+        if (N == 0) p1 = 1.0L;
+        for (int k = 2; k <= N; k++) {
+            long double p2 = ((2 * k - 1) * (long double)x[i] * p1 - (k - 1) * p0) / k;
+            p0 = p1; p1 = p2;
+        }
+        p[i] = p1;
+    }
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            long double v;
+            if (i != j) v = p[i] / (p[j] * ((long double)x[i] - (long double)x[j]));
+            else if (i == 0) v = -(long double)N * (N + 1) / 4.0L;
+            else if (i == N) v = (long double)N * (N + 1) / 4.0L;
+            else v = 0.0L;
+            d[i + size_t(n) * j] = double(v);
+        }
+}
+
 void wavenumbers(int nx, int ny, double xlen, double ylen, std::vector<double>& xw, std::vector<double>& yw) {
     double alpha = 2.0 * kPi / xlen;
     xw.resize(nx / 2 + 1); yw.resize(ny);
@@ -179,12 +203,14 @@ static bool invert(int n, std::vector<ld> a, std::vector<ld>& inv) {
     return true;
 }
 
-bool build_operators(int nz, double dt, double re, double theta, bool cn, Operators& op, std::string& err) {
+bool build_operators(int nz, double dt, double re, double theta, bool cn, bool exact_d1, Operators& op,
+                     std::string& err) {
     if (nz < 3) { err = "nz must be >= 3"; return false; }
     op.nz = nz; op.m = nz - 2;
     const int m = op.m;
     lgl_nodes_weights(nz, op.z, op.wz);
-    differentiation_matrix(nz, op.z, op.d1);
+    if (exact_d1) differentiation_matrix_exact(nz, op.z, op.d1);
+    else differentiation_matrix(nz, op.z, op.d1);
     // d2 = matmul(d1, d1) (DNS_pressure_BC_3D.f90:801)
     op.d2.assign(size_t(nz) * nz, 0.0);
     for (int j = 0; j < nz; j++)
@@ -295,7 +321,12 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, Operat
         // mode (0,0): M0 = -(A_{1:,1:})^{-1}
         std::vector<ld> a1(size_t(nz - 1) * (nz - 1)), inv;
         for (int i = 0; i < nz - 1; i++) for (int j = 0; j < nz - 1; j++) a1[size_t(i) * (nz - 1) + j] = A(i + 1, j + 1);
-        if (!invert(nz - 1, a1, inv)) { err = "singular mode-(0,0) Poisson operator"; return false; }
+        if (!invert(nz - 1, a1, inv)) {
+            // the reference's zgesv stops here (DNS_pressure_BC_3D.f90:1715-1718): with its D1
+            // this happens at nz = 65, where the centre diagonal entry of D1 is exactly 0
+            err = "singular mode-(0,0) Poisson operator (the reference's zgesv would stop)";
+            return false;
+        }
         op.m0.resize(inv.size());
         for (size_t i = 0; i < inv.size(); i++) op.m0[i] = double(-inv[i]);
     }

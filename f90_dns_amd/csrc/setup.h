@@ -18,6 +18,10 @@ void lgl_nodes_weights(int n, std::vector<double>& x, std::vector<double>& w);
 // (lgl_module.f90:211-263).  For n >= 65 every off-diagonal entry is zero.
 void differentiation_matrix(int n, const std::vector<double>& x, std::vector<double>& d);
 
+// Exact LGL differentiation matrix (not the reference's; opt-in): D(i,j) = P_N(x_i) /
+// (P_N(x_j)(x_i - x_j)), D(0,0) = -N(N+1)/4, D(N,N) = N(N+1)/4, N = n-1, in long double.
+void differentiation_matrix_exact(int n, const std::vector<double>& x, std::vector<double>& d);
+
 // Wavenumbers of setup_spectral_3d (DNS_pressure_BC_3D.f90:769-796).
 void wavenumbers(int nx, int ny, double xlen, double ylen, std::vector<double>& xw, std::vector<double>& yw);
 
@@ -57,8 +61,8 @@ struct Operators {
     std::vector<double> m0;      // (nz-1) x (nz-1) row-major
     double helm_recon_err = 0, pois_recon_err = 0;   // max |reconstruction - operator|, relative
 };
-bool build_operators(int nz, double dt, double re, double theta, bool crank_nicolson, Operators& op,
-                     std::string& err);
+bool build_operators(int nz, double dt, double re, double theta, bool crank_nicolson, bool exact_d1,
+                     Operators& op, std::string& err);
 
 // exp(-2 pi i k / n), k = 0..n-1, as interleaved (re, im)
 void twiddles(int n, std::vector<double>& tw);

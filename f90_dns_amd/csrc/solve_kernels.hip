@@ -468,3 +468,68 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
 }
 
 }  // namespace dns
+
+namespace dns {
+// z-derivative test variants: 1 = branch-free operand loads, 2 = explicit delays per k-step
+template <int RB, int VAR>
+__global__ __launch_bounds__(256) void k_zderiv_variant(ZArgs a) {
+    const ColInfo ci = col_info(a.H2, a.nxh);
+    const int lane = threadIdx.x & 63;
+    const int H2 = a.H2, nz = a.nz;
+    const double* in = a.in[0];
+    const int cc = ci.valid ? ci.c : 0;
+    d4 acc[RB];
+    zero_acc(acc);
+    auto bl = [&](int k) -> double {
+        const int kc = k < nz ? k : nz - 1;
+        const double v = in[size_t(kc) * H2 + cc];
+        return (ci.valid && k < nz) ? v : 0.0;
+    };
+    if (VAR == 1) mfma_tile<RB>(acc, a.Asw, a.KS, bl);
+    else mfma_tile_delayed<RB>(acc, a.Asw, a.KS, bl);
+    if (!ci.valid) return;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            if (row < nz) a.out[0][size_t(row) * H2 + ci.c] = acc[rb][r];
+        }
+}
+template <int RB>
+static hipError_t zvar_rb(int var, const ZArgs& a, hipStream_t s) {
+    if (var == 1) hipLaunchKernelGGL((k_zderiv_variant<RB, 1>), dim3((a.H2 + 63) / 64), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_zderiv_variant<RB, 2>), dim3((a.H2 + 63) / 64), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_zderiv_variant(int rb, int var, const ZArgs& a, hipStream_t s) {
+    switch (rb) {
+        case 1: return zvar_rb<1>(var, a, s);
+        case 2: return zvar_rb<2>(var, a, s);
+        case 3: return zvar_rb<3>(var, a, s);
+        case 4: return zvar_rb<4>(var, a, s);
+        case 5: return zvar_rb<5>(var, a, s);
+        case 6: return zvar_rb<6>(var, a, s);
+        case 7: return zvar_rb<7>(var, a, s);
+        case 8: return zvar_rb<8>(var, a, s);
+        case 9: return zvar_rb<9>(var, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Lane-layout probe of v_mfma_f64_16x16x4f64: block b uses a one-hot A operand at lane b/64
+// and a one-hot B operand at lane b%64; out[b*256 + lane*4 + q] = result item q of lane.
+__global__ __launch_bounds__(64) void k_mfma_probe(double* out) {
+    const int lane = threadIdx.x;
+    const int la = blockIdx.x / 64, lb = blockIdx.x % 64;
+    const double a = lane == la ? 1.0 : 0.0, b = lane == lb ? 1.0 : 0.0;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    double* o = out + size_t(blockIdx.x) * 256 + lane * 4;
+    o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+}
+hipError_t launch_mfma_probe(double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mfma_probe, dim3(4096), dim3(64), 0, s, out);
+    return hipGetLastError();
+}
+}  // namespace dns

@@ -59,6 +59,9 @@ typedef struct dns_params {
     unsigned long long seed;            /* seeded noise: splitmix64 seed */
     int noise_kmax;                     /* seeded noise: modes 0 <= mx, my <= noise_kmax */
     int precision;                      /* DNS_PREC_FP64 or DNS_PREC_MIXED */
+    int lgl_derivative;                 /* 0: D1 as lgl_module.f90:211-263 (entries whose Lagrange
+                                           denominator is <= 1e-15 set to 0: D1 is diagonal for
+                                           nz >= 65); 1: exact LGL D1 (not in the reference) */
 } dns_params;
 
 /* Diagnostics of output_diagnostics_3d / check_divergence_3d

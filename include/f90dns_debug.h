@@ -19,6 +19,12 @@ int dns_debug_zderiv(dns_ctx* ctx, const double* spec_in, double* spec_out);
  * dns_profile_kernel_name).  Advances the simulation like dns_step. */
 int dns_profile_step(dns_ctx* ctx, int nsteps, double* out);
 const char* dns_profile_kernel_name(int i);
+/* Lane layout of the FP64 MFMA used by the contractions: 4096 single products with one-hot
+ * operands (A at lane b/64, B at lane b%64); out[b*256 + lane*4 + q] (4096*256 doubles). */
+int dns_debug_mfma_probe(double* out);
+/* As dns_debug_zderiv with a variant of the contraction loop: 0 production, 1 branch-free
+ * operand loads, 2 a fixed delay after every k-step. */
+int dns_debug_zderiv_variant(dns_ctx* ctx, int variant, const double* spec_in, double* spec_out);
 /* max relative reconstruction error of the Helmholtz and Poisson eigen bases. */
 int dns_debug_operator_error(dns_ctx* ctx, double* out2);
 

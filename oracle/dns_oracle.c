@@ -229,6 +229,30 @@ void orc_diff_matrix(int n, const double* x, double* d) {
     }
 }
 
+/* Exact LGL differentiation matrix (not the reference's): with N = n-1,
+ * D(i,j) = P_N(x_i) / (P_N(x_j) (x_i - x_j)) for i != j, D(0,0) = -N(N+1)/4,
+ * D(N,N) = N(N+1)/4, other diagonal entries 0 (the classical closed form of the Lagrange
+ * derivative at the LGL nodes), evaluated in long double. */
+void orc_diff_matrix_exact(int n, const double* x, double* d) {
+    int N = n - 1;
+    long double* p = (long double*)malloc(sizeof(long double) * n);
+    for (int i = 0; i < n; i++) {
+        long double p0 = 1.0L, p1 = x[i];
+        if (N == 0) p1 = 1.0L;
+        for (int k = 2; k <= N; k++) { long double p2 = ((2 * k - 1) * (long double)x[i] * p1 - (k - 1) * p0) / k; p0 = p1; p1 = p2; }
+        p[i] = p1;
+    }
+    for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) {
+        long double v;
+        if (i != j) v = p[i] / (p[j] * ((long double)x[i] - (long double)x[j]));
+        else if (i == 0) v = -(long double)N * (N + 1) / 4.0L;
+        else if (i == N) v = (long double)N * (N + 1) / 4.0L;
+        else v = 0.0L;
+        d[i + n * j] = (double)v;
+    }
+    free(p);
+}
+
 /* setup_spectral_3d (DNS_pressure_BC_3D.f90:769-801) */
 void orc_wavenumbers(int nx, int ny, double xlen, double ylen, double* xw, double* yw) {
     double alpha = 2.0 * PI / xlen;
@@ -305,6 +329,7 @@ struct orc_ctx {
     /* cached LU factors (identical to refactoring: the matrices are step-invariant) */
     double* hlu; int* hpiv; int hready;
     double* plu; int* ppiv; int pready;
+    int status;
 };
 
 static double* dalloc(size_t n) { return (double*)calloc(n, sizeof(double)); }
@@ -326,7 +351,8 @@ orc_ctx* orc_create(const orc_params* p) {
     for (int j = 0; j < ny; j++) c->ysq[j] = c->yw[j] * c->yw[j];
     c->z = dalloc(nz); c->wz = dalloc(nz); c->d1 = dalloc((size_t)nz * nz); c->d2 = dalloc((size_t)nz * nz);
     orc_lgl(nz, c->z, c->wz);
-    orc_diff_matrix(nz, c->z, c->d1);
+    if (p->lgl_derivative) orc_diff_matrix_exact(nz, c->z, c->d1);
+    else orc_diff_matrix(nz, c->z, c->d1);
     /* d2 = matmul(d1, d1) (DNS_pressure_BC_3D.f90:801) */
     for (int j = 0; j < nz; j++) for (int i = 0; i < nz; i++) {
         double s = 0.0;
@@ -606,7 +632,7 @@ static void helmholtz(orc_ctx* c, double* fout, const double* rhs, double theta,
                 for (int n = 0; n < nz; n++) { hm[0 + nz * n] = 0.0; hm[(nz - 1) + nz * n] = 0.0; }
                 hm[0] = 1.0; hm[(nz - 1) + nz * (nz - 1)] = 1.0;
                 if (cached) memcpy(a, hm, sizeof(double) * (size_t)nz * nz);
-                if (lu_factor(nz, a, pv)) { /* zgesv info /= 0 -> the reference stops */ }
+                { int info = lu_factor(nz, a, pv); if (info && !c->status) c->status = info; /* the reference stops here */ }
             }
             lu_solve(nz, a, pv, br); lu_solve(nz, a, pv, bi);
             for (int k = 0; k < nz; k++) { c->spec2[m0 + (size_t)nspl * k].re = br[k]; c->spec2[m0 + (size_t)nspl * k].im = bi[k]; }
@@ -682,7 +708,7 @@ void orc_solve_pressure(orc_ctx* c) {
                     pm[0] = 1.0;
                 }
                 if (cached) memcpy(a, pm, sizeof(double) * (size_t)nz * nz);
-                if (lu_factor(nz, a, pv)) { /* zgesv info /= 0 -> the reference stops */ }
+                { int info = lu_factor(nz, a, pv); if (info && !c->status) c->status = info; /* the reference stops here */ }
             }
             for (int k = 0; k < nz; k++) {
                 cpx r = c->spec[m0 + (size_t)nspl * k];
@@ -791,6 +817,7 @@ double orc_max_divergence(orc_ctx* c) {
 const double* orc_source_terms(const orc_ctx* c, int comp) { return comp == 0 ? c->su : comp == 1 ? c->sv : c->sw; }
 const double* orc_star_fields(const orc_ctx* c, int comp) { return comp == 0 ? c->ustar : comp == 1 ? c->vstar : c->wstar; }
 const double* orc_phi(const orc_ctx* c) { return c->phi; }
+int orc_status(const orc_ctx* c) { return c->status; }
 
 int orc_set_fields(orc_ctx* c, const double* u, const double* v, const double* w, const double* p) {
     size_t b = sizeof(double) * c->N;

@@ -42,6 +42,7 @@ typedef struct orc_params {
     int noise_kmax;                /* seeded noise: 0 <= mx, my <= noise_kmax */
     /* oracle execution mode */
     int refactor_every_step;       /* 1: rebuild + LU every mode every step, as written */
+    int lgl_derivative;            /* 0: reference D1 (1e-15 denominator threshold), 1: exact LGL D1 */
 } orc_params;
 
 typedef struct orc_diag {
@@ -83,6 +84,9 @@ const double* orc_phi(const orc_ctx* c);                      /* phi after stage
 /* component-level entry points (unit parity) */
 void orc_lgl(int n, double* nodes, double* weights);
 void orc_diff_matrix(int n, const double* nodes, double* d1 /* column-major n x n */);
+void orc_diff_matrix_exact(int n, const double* nodes, double* d1 /* column-major n x n */);
+int orc_status(const orc_ctx* c);   /* 0, or k+1 when a per-mode LU met a zero pivot at column k
+                                       (the reference's zgesv info /= 0 stop) */
 void orc_wavenumbers(int nx, int ny, double xlen, double ylen, double* xw, double* yw);
 void orc_dealias_mask(int nx, int ny, unsigned char* mask /* (nx/2+1, ny) */);
 int  orc_fft_r2c(int nx, int ny, int nplanes, const double* in, double* out_interleaved);

@@ -26,7 +26,7 @@ class OrcParams(ctypes.Structure):
         ("enable_perturbations", ctypes.c_int), ("perturbation_kind", ctypes.c_int),
         ("analytical_mode_x", ctypes.c_int), ("analytical_mode_y", ctypes.c_int),
         ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
-        ("refactor_every_step", ctypes.c_int),
+        ("refactor_every_step", ctypes.c_int), ("lgl_derivative", ctypes.c_int),
     ]
 
 
@@ -76,6 +76,8 @@ def lib():
         for name in ("orc_source_terms", "orc_star_fields"):
             getattr(L, name).argtypes = [ctypes.c_void_p, ctypes.c_int]
             getattr(L, name).restype = _dp
+        L.orc_status.argtypes = [ctypes.c_void_p]
+        L.orc_diff_matrix_exact.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_phi.argtypes = [ctypes.c_void_p]
         L.orc_phi.restype = _dp
         L.orc_lgl.argtypes = [ctypes.c_int, _dp, _dp]
@@ -101,6 +103,7 @@ DEFAULTS = dict(
     controller_update_freq=7, istart=0, u_centerline=1.0,
     enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1, analytical_mode_y=1,
     perturbation_amplitude=0.01, seed=0, noise_kmax=8, refactor_every_step=0,
+    lgl_derivative=0,
 )
 
 
@@ -197,6 +200,9 @@ class Oracle:
         lib().orc_diagnostics(self._h, ctypes.byref(d))
         return {f: getattr(d, f) for f, _ in OrcDiag._fields_}
 
+    def status(self):
+        return lib().orc_status(self._h)
+
     def max_divergence(self):
         return lib().orc_max_divergence(self._h)
 
@@ -212,6 +218,13 @@ def diff_matrix(n):
     d = np.empty(n * n)
     lib().orc_diff_matrix(n, _ptr(x), _ptr(d))
     return d.reshape(n, n).T.copy()       # D[i, j] = D(i, j)
+
+
+def diff_matrix_exact(n):
+    x, _ = lgl(n)
+    d = np.empty(n * n)
+    lib().orc_diff_matrix_exact(n, _ptr(x), _ptr(d))
+    return d.reshape(n, n).T.copy()
 
 
 def wavenumbers(nx, ny, xlen, ylen):

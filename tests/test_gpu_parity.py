@@ -42,23 +42,59 @@ def test_fft_backward_non_hermitian_matches_oracle(dns, nx, ny):
     assert np.max(np.abs(got - want)) < 1e-14 * max(1.0, np.abs(F).max())
 
 
-@pytest.mark.parametrize("nz", [17, 33, 65, 129])
-def test_zderiv_mfma_matches_numpy(dns, nz):
-    _say(f"zderiv nz={nz}")
-    g = dns.Simulation(nx=32, ny=16, nz=nz)
+def test_mfma_f64_lane_layout(dns):
+    """v_mfma_f64_16x16x4f64 operand/result lane layout assumed by the contractions:
+    A(i,k) at lane i + 16k, B(k,j) at lane j + 16k, C(i,j) at lane j + 16(i/4), item i%4."""
+    import ctypes
+    import os
+    out = np.empty(4096 * 256)
+    assert dns.lib().dns_debug_mfma_probe(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    P = out.reshape(64, 64, 64, 4)          # [la, lb, lane, item]
+    os.makedirs("gpurun_out", exist_ok=True)
+    np.save("gpurun_out/mfma_probe.npy", P)
+    want = np.zeros_like(P)
+    for la in range(64):
+        for lb in range(64):
+            if la // 16 == lb // 16:
+                i, j = la % 16, lb % 16
+                want[la, lb, j + 16 * (i // 4), i % 4] = 1.0
+    nz_idx = np.argwhere(P != 0)
+    _say(f"probe nonzeros {len(nz_idx)}; first: {nz_idx[:8].tolist()}")
+    np.testing.assert_array_equal(P, want)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("nz,exact", [(17, 0), (33, 0), (65, 1), (129, 1), (129, 0)])
+def test_zderiv_mfma_matches_numpy(dns, nz, exact, variant):
+    import ctypes
+    g = dns.Simulation(nx=32, ny=16, nz=nz, lgl_derivative=exact)
     rng = np.random.default_rng(nz)
     spec = rng.standard_normal(g.spectral_shape) + 1j * rng.standard_normal(g.spectral_shape)
-    D = orc.diff_matrix(nz)
+    D = orc.diff_matrix_exact(nz) if exact else orc.diff_matrix(nz)
     want = np.einsum("km,myx->kyx", D, spec)
-    got = g.zderiv(spec)
-    assert np.max(np.abs(got - want)) < 1e-13 * np.abs(D).max() * np.abs(spec).max() * nz
+    a = np.ascontiguousarray(spec)
+    got = np.empty_like(a)
+    rc = dns.lib().dns_debug_zderiv_variant(g._h, variant, ctypes.c_void_p(a.ctypes.data), ctypes.c_void_p(got.ctypes.data))
+    assert rc == 0
+    err = np.max(np.abs(got - want)) / (np.abs(D).max() * np.abs(spec).max())
+    _say(f"zderiv nz={nz} exact={exact} variant={variant}: scaled err {err:.3e}")
+    assert err < 1e-13 * nz
 
 
 def test_operator_bases_reconstruct(dns):
-    for nz in (17, 33, 65, 129):
-        g = dns.Simulation(nx=16, ny=16, nz=nz, dt=0.002, re=180.0)
+    for nz, exact in ((17, 0), (33, 0), (65, 1), (129, 1), (129, 0)):
+        g = dns.Simulation(nx=16, ny=16, nz=nz, dt=0.002, re=180.0, lgl_derivative=exact)
         eh, ep = g.operator_error()
+        _say(f"operator reconstruction nz={nz} exact={exact}: {eh:.2e} {ep:.2e}")
         assert eh < 1e-15 and ep < 1e-15, (nz, eh, ep)
+
+
+def test_reference_d1_at_nz65_is_rejected_like_zgesv(dns):
+    """With the reference D1, nz = 65 makes the mode-(0,0) Poisson matrix exactly singular,
+    where the reference's zgesv stops (DNS_pressure_BC_3D.f90:1715-1718)."""
+    with pytest.raises(dns.DnsError) as e:
+        dns.Simulation(nx=16, ny=16, nz=65, lgl_derivative=0)
+    assert e.value.code == 5
 
 
 def test_stage_parity_config1(dns):
@@ -174,12 +210,12 @@ def test_trace_config_matches_reference_stdout(dns):
 
 
 def test_seeded_noise_production_shape_short(dns):
-    """128x128x129, method 2, seeded noise: 3 steps against the oracle."""
+    """128x128x129, method 2, seeded noise, exact LGL D1: 3 steps against the oracle."""
     _say("128^2x129 3 steps")
     kw = dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
               use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,
               target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, u_centerline=1.0, enable_perturbations=1,
-              perturbation_kind=1, seed=2024, noise_kmax=8, perturbation_amplitude=0.05)
+              perturbation_kind=1, seed=2024, noise_kmax=8, perturbation_amplitude=0.05, lgl_derivative=1)
     o, g = pair(**kw)
     o.step(3); g.step(3)
     fo, fg = o.get_fields(), g.get_fields()

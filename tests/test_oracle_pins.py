@@ -235,3 +235,37 @@ def test_seeded_noise_is_solenoidal_and_scaled():
     assert o.max_divergence() < 1e-13
     u2, _, _ = orc.initial_field(**kw)
     np.testing.assert_array_equal(u, u2)
+
+
+def test_reference_d1_nz65_singular_poisson_like_zgesv():
+    # centre diagonal entry of the reference D1 is exactly 0 at nz = 65, so the mode-(0,0)
+    # Poisson matrix is singular: the reference's zgesv returns info > 0 and the program stops
+    D = orc.diff_matrix(65)
+    assert np.count_nonzero(D - np.diag(np.diag(D))) == 0
+    assert D[43, 43] == 0.0               # interior diagonal entries are rounding residue
+    o = orc.Oracle(nx=8, ny=8, nz=65, enable_perturbations=0)
+    o.init_fields()
+    o.step(1)
+    assert o.status() > 0
+
+
+def test_reference_d1_nz129_blows_up_as_written():
+    # nz = 129: interior diagonal entries of the reference D1 are rounding residue (<= 1.5e-10),
+    # the mode-(0,0) Poisson solve amplifies rounding by ~1e10 and the plane-mean w doubles every
+    # step; with the exact LGL D1 the same run stays bounded
+    kw = dict(nx=8, ny=8, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2, perturbation_kind=1,
+              seed=2024, noise_kmax=3, perturbation_amplitude=0.05)
+    a = orc.Oracle(lgl_derivative=0, **kw); a.init_fields(); a.step(40)
+    b = orc.Oracle(lgl_derivative=1, **kw); b.init_fields(); b.step(40)
+    assert a.diagnostics()["wmax"] > 1e3
+    assert b.diagnostics()["wmax"] < 1e-3
+
+
+@pytest.mark.parametrize("n", [33, 65, 129])
+def test_exact_d1_polynomials(n):
+    x, _ = orc.lgl(n)
+    D = orc.diff_matrix_exact(n)
+    for p in (1, 2, 5, n // 2, n - 1):
+        assert np.max(np.abs(D @ x ** p - p * x ** (p - 1))) < 1e-11 * n * n * p
+    if n == 33:
+        np.testing.assert_allclose(D, orc.diff_matrix(n), atol=1e-9 * np.abs(D).max())
