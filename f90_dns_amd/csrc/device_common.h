@@ -113,7 +113,8 @@ __device__ __forceinline__ void fft_lds(double2* lds, const Factors& fac, int nb
 //   acc[rb] += A[rb*16 .. rb*16+15][0 .. 4*KS) * B[0 .. 4*KS)[c0 .. c0+15]
 // A is pre-swizzled on the host to the A-operand lane order of v_mfma_f64_16x16x4f64
 // (entry [(rb*KS + s)*64 + lane] = A(rb*16 + lane%16, 4s + lane/16)); bload(k) returns
-// B(k, c0 + lane%16).  Result: acc[rb][r] = C(rb*16 + 4*(lane/16) + r, c0 + lane%16).
+// B(k, c0 + lane%16).  Result: acc[rb][r] = C(rb*16 + lane/16 + 4*r, c0 + lane%16) (the result
+// layout of the instruction, checked by the lane probe in the GPU tests).
 // ---------------------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
 

@@ -703,6 +703,46 @@ int dns_debug_mfma_probe(double* out) {
     return e == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
+static void unswizzle(const SwizzledA& a, std::vector<double>& m) {
+    m.assign(size_t(a.rows) * a.cols, 0.0);
+    for (int rb = 0; rb < a.rb; rb++)
+        for (int s = 0; s < a.ks; s++)
+            for (int lane = 0; lane < 64; lane++) {
+                int r = rb * 16 + (lane & 15), cc = 4 * s + (lane >> 4);
+                if (r < a.rows && cc < a.cols) m[size_t(r) * a.cols + cc] = a.data[(size_t(rb) * a.ks + s) * 64 + lane];
+            }
+}
+
+int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols) {
+    Operators op;
+    std::string err;
+    if (!build_operators(nz, dt, re, cn ? 0.5 : 1.0, cn != 0, exact != 0, op, err)) { g_create_error = err; return DNS_ESOLVER; }
+    std::vector<double> m;
+    int r = 0, cc = 0;
+    auto vec = [&](const std::vector<double>& v) { m = v; r = int(v.size()); cc = 1; };
+    auto mat = [&](const SwizzledA& a) { unswizzle(a, m); r = a.rows; cc = a.cols; };
+    switch (which) {
+        case 0: mat(op.hF); break;
+        case 1: mat(op.hB); break;
+        case 2: vec(op.hc1); break;
+        case 3: vec(op.hc2); break;
+        case 4: vec(op.hl); break;
+        case 5: mat(op.pF); break;
+        case 6: mat(op.pB); break;
+        case 7: mat(op.pDB); break;
+        case 8: vec(op.pl); break;
+        case 9: m = op.m0; r = cc = nz - 1; break;
+        case 10: mat(op.D1); break;
+        case 11: vec(op.wz); break;
+        case 12: m.resize(size_t(nz) * nz); for (int i = 0; i < nz; i++) for (int j = 0; j < nz; j++) m[size_t(i) * nz + j] = op.d2[i + size_t(nz) * j]; r = cc = nz; break;
+        default: return DNS_EINVAL;
+    }
+    if (rows) *rows = r;
+    if (cols) *cols = cc;
+    if (out) std::memcpy(out, m.data(), m.size() * sizeof(double));
+    return 0;
+}
+
 int dns_debug_operator_error(dns_ctx* c, double* out2) {
     if (!c || !out2) return DNS_EINVAL;
     out2[0] = c->op.helm_recon_err;

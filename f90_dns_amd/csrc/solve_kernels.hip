@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
         for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = rb * 16 + 4 * (lane >> 4) + r;
+                const int row = rb * 16 + (lane >> 4) + 4 * r;
                 if (row >= nz) continue;
                 const double y = acc[rb][r];
                 if (MODE == Z_PLAIN) {
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     for (int rb = 0; rb < RBI; rb++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            const int row = rb * 16 + (lane >> 4) + 4 * r;
             double z = 0.0;
             if (row < m && ci.valid) z = (acc[rb][r] + a.c1[row] * bc1 + a.c2[row] * bc2) / (a.hl[row] + a.f * k2);
             zs[wave][row][col] = z;
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     for (int rb = 0; rb < RBI; rb++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            const int row = rb * 16 + (lane >> 4) + 4 * r;
             if (row < m) out[size_t(row + 1) * H2 + ci.c] = acc[rb][r];
         }
     }
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            const int row = rb * 16 + (lane >> 4) + 4 * r;
             double z = 0.0;
             if (row < nz && live) z = -acc[rb][r] / (a.dt * (a.pl[row] + k2));
             zs[wave][row][col] = z;
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            const int row = rb * 16 + (lane >> 4) + 4 * r;
             const double ph = phi[rb][r];
             const double other = __shfl_xor(ph, 1);       // the other part of this mode
             if (!live || row >= nz) continue;
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(256) void k_zderiv_variant(ZArgs a) {
     for (int rb = 0; rb < RB; rb++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + 4 * (lane >> 4) + r;
+            const int row = rb * 16 + (lane >> 4) + 4 * r;
             if (row < nz) a.out[0][size_t(row) * H2 + ci.c] = acc[rb][r];
         }
 }
