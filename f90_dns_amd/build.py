@@ -57,5 +57,40 @@ def build(force=True):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+BIN = os.path.join(HERE, "bin")
+F_MODULES = ["dns_c_api.f90", "fftw3_dns_module.f90"]
+F_PROGRAMS = {"dns_3d_gpu": "dns_3d_gpu.f90", "fft_module_check": "fft_module_check.f90"}
+
+
+def amdflang():
+    for c in (os.environ.get("F90DNS_FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran():
+    """Build the Fortran driver and the FFT-module check against libf90dns.so (ISO_C_BINDING).
+    Returns the list of executables, or [] when no Fortran compiler is installed."""
+    fc = amdflang()
+    if fc is None:
+        return []
+    mod = os.path.join(BUILD, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for name, main in F_PROGRAMS.items():
+        exe = os.path.join(BIN, name)
+        srcs = [os.path.join(FORTRAN, f) for f in F_MODULES + [main]]
+        cmd = [fc, "-O2", "-J", mod] + srcs + ["-L", HERE, "-lf90dns", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
-    print(build(force="--force" in sys.argv or True))
+    print(build(force=True))
+    print(build_fortran())

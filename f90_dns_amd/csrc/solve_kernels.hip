@@ -232,47 +232,68 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
 
 // mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
 // (DNS_pressure_BC_3D.f90:1702-1711): phi_0 = 0, phi_{1:} = M0 b_{1:}, b = w D1 w* / dt.
-__global__ __launch_bounds__(256) void k_poisson_mode0(Mode0Args a) {
+// Each matrix-vector row is one wavefront: coalesced row loads, a fixed-order wave reduction.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ __launch_bounds__(512) void k_poisson_mode0(Mode0Args a) {
     extern __shared__ double sh[];
     const int nz = a.nz;
     double* ws = sh;            // [2][nz]
     double* b = sh + 2 * nz;    // [2][nz]
     double* x = sh + 4 * nz;    // [2][nz]
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
     for (int e = t; e < 2 * nz; e += blockDim.x) {
         const int part = e / nz, k = e - part * nz;
         ws[e] = a.ws[size_t(k) * a.H2 + part];
     }
     __syncthreads();
-    for (int e = t; e < 2 * nz; e += blockDim.x) {
-        const int part = e / nz, k = e - part * nz;
-        double s = 0.0;
-        for (int q = 0; q < nz; q++) s += a.d1[size_t(k) * nz + q] * ws[part * nz + q];
-        b[e] = a.wz[k] * s / a.dt;
+    for (int k = wave; k < nz; k += nw) {           // b = w * (D1 w*) / dt
+        double s0 = 0.0, s1 = 0.0;
+        for (int q = lane; q < nz; q += 64) {
+            const double d = a.d1[size_t(k) * nz + q];
+            s0 += d * ws[q]; s1 += d * ws[nz + q];
+        }
+        s0 = wave_sum(s0); s1 = wave_sum(s1);
+        if (lane == 0) { b[k] = a.wz[k] * s0 / a.dt; b[nz + k] = a.wz[k] * s1 / a.dt; }
     }
     __syncthreads();
-    for (int e = t; e < 2 * nz; e += blockDim.x) {
-        const int part = e / nz, k = e - part * nz;
-        double s = 0.0;
+    for (int k = wave; k < nz; k += nw) {           // x_{1:} = M0 b_{1:}, x_0 = 0
+        double s0 = 0.0, s1 = 0.0;
         if (k > 0)
-            for (int q = 1; q < nz; q++) s += a.m0[size_t(k - 1) * (nz - 1) + (q - 1)] * b[part * nz + q];
-        x[e] = s;
+            for (int q = 1 + lane; q < nz; q += 64) {
+                const double mq = a.m0[size_t(k - 1) * (nz - 1) + (q - 1)];
+                s0 += mq * b[q]; s1 += mq * b[nz + q];
+            }
+        s0 = wave_sum(s0); s1 = wave_sum(s1);
+        if (lane == 0) { x[k] = s0; x[nz + k] = s1; }
     }
     __syncthreads();
-    for (int e = t; e < 2 * nz; e += blockDim.x) {
-        const int part = e / nz, k = e - part * nz;
-        double dx = 0.0;
-        for (int q = 0; q < nz; q++) dx += a.d1[size_t(k) * nz + q] * x[part * nz + q];
-        const size_t o = size_t(k) * a.H2 + part;
-        if (!a.fused) {
-            a.phi[o] = x[e];
-            a.dphi[o] = dx;
-        } else {
-            const bool wall = k == 0 || k == nz - 1;
-            a.u[o] = wall ? 0.0 : a.us[o];
-            a.v[o] = wall ? 0.0 : a.vs[o];
-            a.w[o] = wall ? 0.0 : ws[e] - a.dt * dx;
-            a.p[o] = a.p[o] + x[e];
+    for (int k = wave; k < nz; k += nw) {           // D1 x and the outputs of row k
+        double s0 = 0.0, s1 = 0.0;
+        for (int q = lane; q < nz; q += 64) {
+            const double d = a.d1[size_t(k) * nz + q];
+            s0 += d * x[q]; s1 += d * x[nz + q];
+        }
+        s0 = wave_sum(s0); s1 = wave_sum(s1);
+        if (lane < 2) {
+            const int part = lane;
+            const double dx = part ? s1 : s0;
+            const double xv = x[part * nz + k];
+            const size_t o = size_t(k) * a.H2 + part;
+            if (!a.fused) {
+                a.phi[o] = xv;
+                a.dphi[o] = dx;
+            } else {
+                const bool wall = k == 0 || k == nz - 1;
+                a.u[o] = wall ? 0.0 : a.us[o];
+                a.v[o] = wall ? 0.0 : a.vs[o];
+                a.w[o] = wall ? 0.0 : ws[part * nz + k] - a.dt * dx;
+                a.p[o] = a.p[o] + xv;
+            }
         }
     }
 }
@@ -311,15 +332,19 @@ __global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, double* u, d
 // (DNS_pressure_BC_3D.f90:1107-1148, 2434-2549), called with istart as in :1118.
 // U_b = sum_k w_k Re u_hat(0,0,k) / (nx ny sum w): the (0,0) coefficient of the
 // unnormalised forward transform is the plane sum.
-__global__ void k_flow_control(FlowArgs a) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) {
+    const int lane = threadIdx.x;
+    double part = 0.0;
+    if (a.method != 1)
+        for (int k = lane; k < a.nz; k += 64) part += a.uh[size_t(k) * a.H2] * a.wz[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if (lane != 0) return;
     FlowState s = *a.st;
     if (a.method == 1) {
         s.G = a.target_G;
     } else if (a.istart % a.freq == 0) {
-        double vol = 0.0;
-        for (int k = 0; k < a.nz; k++) vol += a.uh[size_t(k) * a.H2] * a.wz[k];
-        s.Ub = vol / (a.nxny * a.wsum);
+        s.Ub = part / (a.nxny * a.wsum);
         s.err = a.target_Ub - s.Ub;
         const double kp = a.gain, ki = 0.05 * a.gain;
         const double gu = 3.0 / a.re + kp * s.err + ki * s.err_int;
@@ -444,7 +469,7 @@ hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s) {
     }
 }
 hipError_t launch_poisson_mode0(const Mode0Args& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_poisson_mode0, dim3(1), dim3(256), size_t(6) * a.nz * sizeof(double), s, a);
+    hipLaunchKernelGGL(k_poisson_mode0, dim3(1), dim3(512), size_t(6) * a.nz * sizeof(double), s, a);
     return hipGetLastError();
 }
 hipError_t launch_correction(const CorrArgs& a, hipStream_t s) {

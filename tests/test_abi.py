@@ -54,3 +54,19 @@ def test_params_struct_matches_header():
         fields += names
     mine = [f for f, _ in f90_dns_amd.DnsParams._fields_]
     assert fields == mine
+
+
+def test_fortran_driver_builds_and_fails_cleanly_without_gpu(tmp_path):
+    """The Fortran driver (ISO_C_BINDING over the C ABI) links and reports a missing device
+    through dns_last_error instead of crashing."""
+    import subprocess
+    from f90_dns_amd import build as b
+    if b.amdflang() is None:
+        pytest.skip("no Fortran compiler")
+    exes = b.build_fortran()
+    assert any(e.endswith("dns_3d_gpu") for e in exes)
+    r = subprocess.run([os.path.join(b.BIN, "dns_3d_gpu"), os.path.join(ROOT, "tests", "golden", "trace_input.dat")],
+                       capture_output=True, text=True, cwd=tmp_path, timeout=60)
+    if r.returncode == 0:
+        pytest.skip("a GPU is present: covered by the GPU driver test")
+    assert "dns_create" in r.stdout and "ERROR" in r.stdout
