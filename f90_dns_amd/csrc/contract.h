@@ -1,0 +1,106 @@
+// contract.h -- block-level FP64 MFMA contraction of a 32-column tile of spectral columns.
+//
+// A block of 4 wavefronts owns 32 real columns (16 complex modes).  Output row blocks are split
+// across the waves (wave w takes row blocks w, w+4, w+8, ...), so each row of the operator is
+// read from HBM/L2 once per block; the B operand (the spectral data, prologue applied) is
+// produced once per element by one thread and staged through double-buffered LDS chunks of 16
+// rows, the next chunk prefetched into registers while the current one is consumed.
+#pragma once
+#include "device_common.h"
+
+namespace dns {
+
+constexpr int CT = 2;              // 16-column MFMA tiles per block
+constexpr int BCOLS = 16 * CT;     // real columns per block
+constexpr int BMODES = BCOLS / 2;  // complex modes per block
+constexpr int KC = 16;             // B rows per LDS chunk (4 MFMA k-steps)
+constexpr int LDB = BCOLS + 2;     // padded LDS row stride, doubles
+
+// acc[i][ct] += A[rb = wave + 4i] x B, B produced by bprod(k, mode_local) -> double2 (Re, Im)
+// of row k for the block's mode mode_local (zero beyond the operand's rows).  lds: 2*KC*LDB.
+template <int RBW, class BP>
+__device__ __forceinline__ void gemm_staged(d4 (&acc)[RBW][CT], const double* __restrict__ Asw, int RB, int KS,
+                                            BP&& bprod, double* lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kk = lane >> 4, cl = lane & 15;
+    const int t = threadIdx.x;
+    const int rt = t >> 4, mt = t & 15;            // staging: chunk row rt, mode mt
+    const int nchunks = (KS + 3) / 4;
+    double2 v = bprod(rt, mt);
+    lds[rt * LDB + 2 * mt] = v.x;
+    lds[rt * LDB + 2 * mt + 1] = v.y;
+    __syncthreads();
+    for (int ch = 0; ch < nchunks; ch++) {
+        const double* cur = lds + (ch & 1) * KC * LDB;
+        double2 nxt = make_double2(0.0, 0.0);
+        if (ch + 1 < nchunks) nxt = bprod((ch + 1) * KC + rt, mt);
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++) {
+            const int s = ch * 4 + sl;
+            if (s < KS) {
+                double b[CT];
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++) b[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
+#pragma unroll
+                for (int i = 0; i < RBW; i++) {
+                    const int rb = wave + 4 * i;
+                    if (rb < RB) {
+                        const double a = Asw[(size_t(rb) * KS + s) * 64 + lane];
+#pragma unroll
+                        for (int ct = 0; ct < CT; ct++)
+                            acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct], acc[i][ct], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (ch + 1 < nchunks) {
+            double* nb = lds + ((ch + 1) & 1) * KC * LDB;
+            nb[rt * LDB + 2 * mt] = nxt.x;
+            nb[rt * LDB + 2 * mt + 1] = nxt.y;
+        }
+        __syncthreads();
+    }
+}
+
+// acc[i][ct] += A[rb = wave + 4i] x Z with Z resident in LDS as zs[row * LDB + col] (rows >= 4*KS
+// never read).
+template <int RBW>
+__device__ __forceinline__ void gemm_lds(d4 (&acc)[RBW][CT], const double* __restrict__ Asw, int RB, int KS,
+                                         const double* zs) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kk = lane >> 4, cl = lane & 15;
+#pragma unroll 4
+    for (int s = 0; s < KS; s++) {
+        double b[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) b[ct] = zs[(4 * s + kk) * LDB + ct * 16 + cl];
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int rb = wave + 4 * i;
+            if (rb < RB) {
+                const double a = Asw[(size_t(rb) * KS + s) * 64 + lane];
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++)
+                    acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct], acc[i][ct], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <int RBW>
+__device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
+#pragma unroll
+    for (int i = 0; i < RBW; i++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) acc[i][ct] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// Output coordinates of acc[i][ct][r]: row = (wave + 4i)*16 + lane/16 + 4r, column =
+// blockIdx.x*BCOLS + ct*16 + lane%16.
+__device__ __forceinline__ int out_row(int i, int r) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    return (wave + 4 * i) * 16 + (lane >> 4) + 4 * r;
+}
+__device__ __forceinline__ int out_col(int ct) { return blockIdx.x * BCOLS + ct * 16 + (threadIdx.x & 15); }
+
+}  // namespace dns

@@ -133,25 +133,6 @@ __device__ __forceinline__ void mfma_tile(d4 (&acc)[RB], const double* __restric
     }
 }
 
-// Same contraction with a fixed delay (4 x s_nop 15) after every k-step, the accumulators
-// tied to the delay so that no read of them can be scheduled before it.
-template <int RB, class BL>
-__device__ __forceinline__ void mfma_tile_delayed(d4 (&acc)[RB], const double* __restrict__ Asw, int KS, BL&& bload) {
-    const int lane = threadIdx.x & 63;
-    const int kk = lane >> 4;
-    const double* ap = Asw + lane;
-    for (int s = 0; s < KS; s++) {
-        const double b = bload(4 * s + kk);
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const double a = ap[(size_t(rb) * KS + s) * 64];
-            acc[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rb], 0, 0, 0);
-        }
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++) asm volatile("s_nop 15\n\ts_nop 15\n\ts_nop 15\n\ts_nop 15" : "+v"(acc[rb]));
-    }
-}
-
 template <int RB>
 __device__ __forceinline__ void zero_acc(d4 (&acc)[RB]) {
 #pragma unroll

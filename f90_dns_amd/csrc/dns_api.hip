@@ -39,7 +39,7 @@ struct dns_ctx {
     FFTGeom g;
     Operators op;
     // constants
-    double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *d1rm = nullptr, *m0 = nullptr;
+    double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *e0 = nullptr, *f0 = nullptr;
     unsigned char* mask = nullptr;
     double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr;
     double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
@@ -188,12 +188,9 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     c->g.twy = reinterpret_cast<const double2*>(c->twy);
     wavenumbers(c->nx, c->ny, p->xlen, p->ylen, xw, yw);
     dealias_mask(c->nx, c->ny, mask);
-    std::vector<double> d1rm(size_t(c->nz) * c->nz);
-    for (int i = 0; i < c->nz; i++)
-        for (int j = 0; j < c->nz; j++) d1rm[size_t(i) * c->nz + j] = c->op.d1[i + size_t(c->nz) * j];
     const Operators& o = c->op;
     if ((r = upload(c, &c->xw, xw)) || (r = upload(c, &c->yw, yw)) || (r = upload(c, &c->mask, mask)) ||
-        (r = upload(c, &c->wz, o.wz)) || (r = upload(c, &c->d1rm, d1rm)) || (r = upload(c, &c->m0, o.m0)) ||
+        (r = upload(c, &c->wz, o.wz)) || (r = upload(c, &c->e0, o.e0)) || (r = upload(c, &c->f0, o.f0)) ||
         (r = upload(c, &c->D1sw, o.D1.data)) || (r = upload(c, &c->hF, o.hF.data)) || (r = upload(c, &c->hB, o.hB.data)) ||
         (r = upload(c, &c->hc1, o.hc1)) || (r = upload(c, &c->hc2, o.hc2)) || (r = upload(c, &c->hl, o.hl)) ||
         (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) || (r = upload(c, &c->pDB, o.pDB.data)) ||
@@ -403,7 +400,7 @@ static PoisArgs poisson_args(dns_ctx* c, int fused) {
 
 static Mode0Args mode0_args(dns_ctx* c, int fused) {
     Mode0Args m{};
-    m.nz = c->nz; m.H2 = c->H2; m.d1 = c->d1rm; m.m0 = c->m0; m.wz = c->wz; m.dt = c->p.dt; m.fused = fused;
+    m.nz = c->nz; m.H2 = c->H2; m.e0 = c->e0; m.f0 = c->f0; m.dt = c->p.dt; m.fused = fused;
     m.us = c->us; m.vs = c->vs; m.ws = c->ws;
     m.u = c->uh; m.v = c->vh; m.w = c->wh; m.p = c->ph;
     m.phi = c->phi; m.dphi = c->dphi;
@@ -600,20 +597,6 @@ int dns_debug_spectral(dns_ctx* c, int which, double* out) {
                            c->ox, c->oy, c->phi, c->dphi};
     if (which < 0 || which >= int(sizeof(src) / sizeof(src[0]))) return fail(c, DNS_EINVAL, "bad buffer index");
     HIPCHECK(c, hipMemcpyAsync(out, src[which], c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    return 0;
-}
-
-int dns_debug_zderiv_variant(dns_ctx* c, int variant, const double* spec_in, double* spec_out) {
-    if (!c || !spec_in || !spec_out || variant < 0 || variant > 2) return DNS_EINVAL;
-    if (variant == 0) return dns_debug_zderiv(c, spec_in, spec_out);
-    HIPCHECK(c, hipMemcpyAsync(c->tmp, spec_in, c->NS * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    ZArgs z{};
-    z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
-    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
-    z.in[0] = c->tmp; z.out[0] = c->phi; z.xw = c->xw; z.yw = c->yw;
-    HIPCHECK(c, launch_zderiv_variant(c->rb, variant, z, c->stream));
-    HIPCHECK(c, hipMemcpyAsync(spec_out, c->phi, c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
 }

@@ -49,8 +49,6 @@ struct ZArgs {
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
 hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s);
-// out[0] = D1 in[0] with a variant of the contraction loop (1 branch-free loads, 2 delays)
-hipError_t launch_zderiv_variant(int rb, int var, const ZArgs& a, hipStream_t s);
 
 struct HelmArgs {
     int nz, m, H2, nxh, ny, nx;
@@ -82,9 +80,8 @@ hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s);
 
 struct Mode0Args {
     int nz, H2;
-    const double* d1;            // row-major nz x nz
-    const double* m0;            // row-major (nz-1)^2
-    const double* wz;
+    const double* e0;            // row-major nz x nz: phi(0,0,:) = E0 w*(0,0,:)
+    const double* f0;            // row-major nz x nz: D1 phi(0,0,:) = F0 w*(0,0,:)
     double dt;
     int fused;
     const double* us; const double* vs; const double* ws;

@@ -329,6 +329,23 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
         }
         op.m0.resize(inv.size());
         for (size_t i = 0; i < inv.size(); i++) op.m0[i] = double(-inv[i]);
+        std::vector<ld> e0(size_t(nz) * nz, 0.0L);
+        for (int k = 1; k < nz; k++)
+            for (int q = 0; q < nz; q++) {
+                ld acc = 0;
+                for (int j = 1; j < nz; j++)
+                    acc += -inv[size_t(k - 1) * (nz - 1) + (j - 1)] * (ld)op.wz[j] * D1(j, q);
+                e0[size_t(k) * nz + q] = acc / (ld)dt;
+            }
+        op.e0.resize(e0.size());
+        op.f0.assign(size_t(nz) * nz, 0.0);
+        for (size_t i = 0; i < e0.size(); i++) op.e0[i] = double(e0[i]);
+        for (int k = 0; k < nz; k++)
+            for (int q = 0; q < nz; q++) {
+                ld acc = 0;
+                for (int j = 0; j < nz; j++) acc += D1(k, j) * e0[size_t(j) * nz + q];
+                op.f0[size_t(k) * nz + q] = double(acc);
+            }
     }
     return true;
 }

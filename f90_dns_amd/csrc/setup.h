@@ -45,7 +45,8 @@ SwizzledA swizzle_a(const std::vector<double>& rowmajor, int rows, int cols);
 //  Poisson (all nz unknowns), with Qf Lf Qf^T = W^{-1/2} A W^{-1/2}:
 //    y = Tf div + (Tf D1) w,  z = -y / (dt (lf + k^2)),  phi = Bf z,  D1 phi = (D1 Bf) z,
 //    Tf = Qf^T W^{1/2}, Bf = W^{-1/2} Qf.  Mode (0,0): phi_0 = 0, phi_{1:} = M0 b_{1:} with
-//    M0 = -(A_{1:,1:})^{-1} and b = w * (D1 w*) / dt.
+//    M0 = -(A_{1:,1:})^{-1} and b = w * (D1 w*) / dt, i.e. phi = E0 w*, D1 phi = F0 w* with
+//    E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0.
 struct Operators {
     int nz = 0, m = 0;
     double f = 0.0;              // theta dt / Re
@@ -59,6 +60,7 @@ struct Operators {
     SwizzledA pDB;               // nz x nz: D1 Bf
     std::vector<double> pl;      // length nz
     std::vector<double> m0;      // (nz-1) x (nz-1) row-major
+    std::vector<double> e0, f0;  // nz x nz row-major: mode-(0,0) phi = E0 w*, D1 phi = F0 w*
     double helm_recon_err = 0, pois_recon_err = 0;   // max |reconstruction - operator|, relative
 };
 bool build_operators(int nz, double dt, double re, double theta, bool crank_nicolson, bool exact_d1,

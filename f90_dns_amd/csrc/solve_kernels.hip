@@ -7,9 +7,12 @@
 // nz rows x H2 = 2 (nx/2+1) ny columns, column c = 2*mode + part (part 0 = Re, 1 = Im).
 #include <algorithm>
 
+#include "contract.h"
 #include "kernels.h"
 
 namespace dns {
+
+static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to 256 threads");
 
 struct ColInfo {
     int c, mode, part, i, j;
@@ -38,40 +41,49 @@ __device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx,
 // -------------------------------------------------------------------------------------
 // z derivative: out = D1 in  (compute_derivatives_3d dz, DNS_pressure_BC_3D.f90:2348-2358),
 // or the vorticity components that need it: omega_x = i ky w - D1 v, omega_y = D1 u - i kx w
-// (DNS_pressure_BC_3D.f90:1077-1079).
+// (DNS_pressure_BC_3D.f90:1077-1079).  One block = 16 modes, all nz rows.
 // -------------------------------------------------------------------------------------
-template <int RB, int MODE>
+template <int RBW, int MODE>
 __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
-    const ColInfo ci = col_info(a.H2, a.nxh);
-    const int lane = threadIdx.x & 63;
-    const int H2 = a.H2, nz = a.nz;
+    __shared__ double lds[2 * KC * LDB];
+    const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
+    const int mode0 = blockIdx.x * BMODES;
     const int nfield = MODE == Z_VORT ? 2 : 1;
     for (int q = 0; q < nfield; q++) {
         const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : blockIdx.y;   // vort: v first, then u
         const double* in = a.in[f];
-        d4 acc[RB];
-        zero_acc(acc);
-        mfma_tile<RB>(acc, a.Asw, a.KS, [&](int k) -> double {
-            return (ci.valid && k < nz) ? in[size_t(k) * H2 + ci.c] : 0.0;
-        });
-        if (!ci.valid) continue;
+        d4 acc[RBW][CT];
+        zero_acc2(acc);
+        gemm_staged<RBW>(acc, a.Asw, RB, a.KS, [&](int k, int ml) -> double2 {
+            const int mode = mode0 + ml;
+            if (k < nz && mode < H) return reinterpret_cast<const double2*>(in + size_t(k) * H2)[mode];
+            return make_double2(0.0, 0.0);
+        }, lds);
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
+        for (int ct = 0; ct < CT; ct++) {
+            const int col = out_col(ct);
+            if (col >= H2) continue;
+            const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = rb * 16 + (lane >> 4) + 4 * r;
-                if (row >= nz) continue;
-                const double y = acc[rb][r];
-                if (MODE == Z_PLAIN) {
-                    a.out[f][size_t(row) * H2 + ci.c] = y;
-                } else {
-                    const double2 w = reinterpret_cast<const double2*>(a.in[2] + size_t(row) * H2)[ci.mode];
-                    if (f == 1) {        // omega_x = i ky w - D1 v
-                        const double ky = a.yw[ci.j];
-                        a.out[0][size_t(row) * H2 + ci.c] = ci.part == 0 ? (-ky * w.y - y) : (ky * w.x - y);
-                    } else {             // omega_y = D1 u - i kx w
-                        const double kx = a.xw[ci.i];
-                        a.out[1][size_t(row) * H2 + ci.c] = ci.part == 0 ? (y + kx * w.y) : (y - kx * w.x);
+            for (int ii = 0; ii < RBW; ii++) {
+#pragma unroll 2
+                for (int r = 0; r < 4; r++) {
+                    const int row = out_row(ii, r);
+                    if (row >= nz) continue;
+                    const double y = acc[ii][ct][r];
+                    if (MODE == Z_PLAIN) {
+                        a.out[f][size_t(row) * H2 + col] = y;
+                    } else {
+                        const double wo = a.in[2][size_t(row) * H2 + (col ^ 1)];   // the other part of w
+                        const double wr = part == 0 ? a.in[2][size_t(row) * H2 + col] : wo;
+                        const double wi = part == 0 ? wo : a.in[2][size_t(row) * H2 + col];
+                        if (f == 1) {        // omega_x = i ky w - D1 v
+                            const double ky = a.yw[j];
+                            a.out[0][size_t(row) * H2 + col] = part == 0 ? (-ky * wi - y) : (ky * wr - y);
+                        } else {             // omega_y = D1 u - i kx w
+                            const double kx = a.xw[i];
+                            a.out[1][size_t(row) * H2 + col] = part == 0 ? (y + kx * wi) : (y - kx * wr);
+                        }
                     }
                 }
             }
@@ -88,67 +100,88 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
 // alpha = 1 - f k^2 (CN) or 1, S' = S + G nx ny at mode (0,0) of the u component, and
 // bc = -dt (i k_eff p_hat) at the walls (the Kim & Moin values -dt grad p, :1325-1328).
 // -------------------------------------------------------------------------------------
-template <int RBI>
+template <int RBW>
 __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
-    __shared__ double zs[4][RBI * 16][16];
-    const ColInfo ci = col_info(a.H2, a.nxh);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
+    __shared__ double smem[(4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB];
     const int comp = blockIdx.y;
-    const int H2 = a.H2, nz = a.nz, m = a.m;
+    const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2, RBI = (m + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mode0 = blockIdx.x * BMODES;
     const double* uh = a.uh[comp];
     const double* sh = a.sh[comp];
-    double k2 = 0.0, bc1 = 0.0, bc2 = 0.0, force = 0.0;
-    if (ci.valid) {
-        const double kx = a.xw[ci.i], ky = a.yw[ci.j];
-        k2 = kx * kx + ky * ky;
-        if (comp < 2) {
-            const double ke = comp == 0 ? kx_eff(a.xw, ci.i, a.nx) : ky_eff(a.yw, ci.i, ci.j, a.nx, a.ny);
-            const double2 p0 = reinterpret_cast<const double2*>(a.ph)[ci.mode];
-            const double2 p1 = reinterpret_cast<const double2*>(a.ph + size_t(nz - 1) * H2)[ci.mode];
-            // -dt * (i ke p)
-            bc1 = ci.part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x);
-            bc2 = ci.part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x);
+    const double force = comp == 0 ? (*a.G) * a.nxny : 0.0;
+    // per-lane column constants for the two 16-column tiles
+    double k2c[CT], bc1[CT], bc2[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = out_col(ct);
+        k2c[ct] = 0.0; bc1[ct] = 0.0; bc2[ct] = 0.0;
+        if (col < H2) {
+            const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
+            const double kx = a.xw[i], ky = a.yw[j];
+            k2c[ct] = kx * kx + ky * ky;
+            if (comp < 2) {
+                const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
+                const double2 p0 = reinterpret_cast<const double2*>(a.ph)[mode];
+                const double2 p1 = reinterpret_cast<const double2*>(a.ph + size_t(nz - 1) * H2)[mode];
+                bc1[ct] = part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x);    // -dt * (i ke p)
+                bc2[ct] = part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x);
+            }
         }
-        if (comp == 0 && ci.c == 0) force = (*a.G) * a.nxny;
     }
-    const double alpha = a.cn ? 1.0 - a.f * k2 : 1.0;
-    d4 acc[RBI];
-    zero_acc(acc);
-    mfma_tile<RBI>(acc, a.Fsw, a.FKS, [&](int k) -> double {
-        if (!ci.valid) return 0.0;
+    d4 acc[RBW][CT];
+    zero_acc2(acc);
+    gemm_staged<RBW>(acc, a.Fsw, RBI, a.FKS, [&](int k, int ml) -> double2 {
+        const int mode = mode0 + ml;
+        if (mode >= H) return make_double2(0.0, 0.0);
         if (k < m) {
-            const size_t q = size_t(k + 1) * H2 + ci.c;
-            return alpha * uh[q] + a.dt * (sh[q] + force);
+            const int j = mode / a.nxh, i = mode - j * a.nxh;
+            const double kx = a.xw[i], ky = a.yw[j];
+            const double alpha = a.cn ? 1.0 - a.f * (kx * kx + ky * ky) : 1.0;
+            const double2 u = reinterpret_cast<const double2*>(uh + size_t(k + 1) * H2)[mode];
+            const double2 sv = reinterpret_cast<const double2*>(sh + size_t(k + 1) * H2)[mode];
+            const double fr = mode == 0 ? force : 0.0;
+            return make_double2(alpha * u.x + a.dt * (sv.x + fr), alpha * u.y + a.dt * sv.y);
         }
-        if (k < m + nz) return uh[size_t(k - m) * H2 + ci.c];
-        return 0.0;
-    });
+        if (k < m + nz) return reinterpret_cast<const double2*>(uh + size_t(k - m) * H2)[mode];
+        return make_double2(0.0, 0.0);
+    }, smem);
+    double* zs = smem;
 #pragma unroll
-    for (int rb = 0; rb < RBI; rb++) {
+    for (int ct = 0; ct < CT; ct++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + (lane >> 4) + 4 * r;
-            double z = 0.0;
-            if (row < m && ci.valid) z = (acc[rb][r] + a.c1[row] * bc1 + a.c2[row] * bc2) / (a.hl[row] + a.f * k2);
-            zs[wave][row][col] = z;
+        for (int ii = 0; ii < RBW; ii++) {
+            if (wave + 4 * ii >= RBI) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = out_row(ii, r);
+                double z = 0.0;
+                if (row < m && out_col(ct) < H2)
+                    z = (acc[ii][ct][r] + a.c1[row] * bc1[ct] + a.c2[row] * bc2[ct]) / (a.hl[row] + a.f * k2c[ct]);
+                zs[row * LDB + ct * 16 + (lane & 15)] = z;
+            }
         }
     }
     __syncthreads();
-    zero_acc(acc);
-    mfma_tile<RBI>(acc, a.Bsw, a.BKS, [&](int k) -> double { return k < RBI * 16 ? zs[wave][k][col] : 0.0; });
-    if (!ci.valid) return;
+    zero_acc2(acc);
+    gemm_lds<RBW>(acc, a.Bsw, RBI, a.BKS, zs);
     double* out = a.out[comp];
 #pragma unroll
-    for (int rb = 0; rb < RBI; rb++) {
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = out_col(ct);
+        if (col >= H2) continue;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + (lane >> 4) + 4 * r;
-            if (row < m) out[size_t(row + 1) * H2 + ci.c] = acc[rb][r];
+        for (int ii = 0; ii < RBW; ii++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = out_row(ii, r);
+                if (wave + 4 * ii < RBI && row < m) out[size_t(row + 1) * H2 + col] = acc[ii][ct][r];
+            }
         }
-    }
-    if ((lane >> 4) == 0) {
-        out[ci.c] = bc1;
-        out[size_t(nz - 1) * H2 + ci.c] = bc2;
+        if (wave == 0 && (lane >> 4) == 0) {
+            out[col] = bc1[ct];
+            out[size_t(nz - 1) * H2 + col] = bc2[ct];
+        }
     }
 }
 
@@ -161,139 +194,127 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
 //   u = u* - dt i kx_eff phi, v = v* - dt i ky_eff phi, w = w* - dt D1 phi, p += phi.
 // Mode (0,0) is left to k_poisson_mode0.
 // -------------------------------------------------------------------------------------
-template <int RB>
+template <int RBW>
 __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
-    __shared__ double zs[4][RB * 16][16];
-    const ColInfo ci = col_info(a.H2, a.nxh);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15;
-    const int H2 = a.H2, nz = a.nz;
-    double k2 = 0.0, kxe = 0.0, kye = 0.0;
-    if (ci.valid) {
-        const double kx = a.xw[ci.i], ky = a.yw[ci.j];
-        k2 = kx * kx + ky * ky;
-        kxe = kx_eff(a.xw, ci.i, a.nx);
-        kye = ky_eff(a.yw, ci.i, ci.j, a.nx, a.ny);
-    }
-    const bool live = ci.valid && ci.mode != 0;
-    d4 acc[RB];
-    zero_acc(acc);
-    mfma_tile<RB>(acc, a.Fsw, a.FKS, [&](int k) -> double {
-        if (!live) return 0.0;
-        if (k < nz) {
-            const double2 u = reinterpret_cast<const double2*>(a.us + size_t(k) * H2)[ci.mode];
-            const double2 v = reinterpret_cast<const double2*>(a.vs + size_t(k) * H2)[ci.mode];
-            return ci.part == 0 ? (-kxe * u.y - kye * v.y) : (kxe * u.x + kye * v.x);
+    __shared__ double smem[(4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB];
+    const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int mode0 = blockIdx.x * BMODES;
+    double k2c[CT], kxe[CT], kye[CT];
+    bool live[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = out_col(ct);
+        const int mode = col >> 1, j = mode / a.nxh, i = mode - j * a.nxh;
+        live[ct] = col < H2 && mode != 0;
+        k2c[ct] = 1.0; kxe[ct] = 0.0; kye[ct] = 0.0;
+        if (col < H2) {
+            const double kx = a.xw[i], ky = a.yw[j];
+            k2c[ct] = kx * kx + ky * ky;
+            kxe[ct] = kx_eff(a.xw, i, a.nx);
+            kye[ct] = ky_eff(a.yw, i, j, a.nx, a.ny);
         }
-        if (k < 2 * nz) return a.ws[size_t(k - nz) * H2 + ci.c];
-        return 0.0;
-    });
+    }
+    d4 acc[RBW][CT];
+    zero_acc2(acc);
+    gemm_staged<RBW>(acc, a.Fsw, RB, a.FKS, [&](int k, int ml) -> double2 {
+        const int mode = mode0 + ml;
+        if (mode >= H || mode == 0) return make_double2(0.0, 0.0);
+        if (k < nz) {
+            const int j = mode / a.nxh, i = mode - j * a.nxh;
+            const double ke = kx_eff(a.xw, i, a.nx), kf = ky_eff(a.yw, i, j, a.nx, a.ny);
+            const double2 u = reinterpret_cast<const double2*>(a.us + size_t(k) * H2)[mode];
+            const double2 v = reinterpret_cast<const double2*>(a.vs + size_t(k) * H2)[mode];
+            return make_double2(-ke * u.y - kf * v.y, ke * u.x + kf * v.x);
+        }
+        if (k < 2 * nz) return reinterpret_cast<const double2*>(a.ws + size_t(k - nz) * H2)[mode];
+        return make_double2(0.0, 0.0);
+    }, smem);
+    double* zs = smem;
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
+    for (int ct = 0; ct < CT; ct++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + (lane >> 4) + 4 * r;
-            double z = 0.0;
-            if (row < nz && live) z = -acc[rb][r] / (a.dt * (a.pl[row] + k2));
-            zs[wave][row][col] = z;
+        for (int ii = 0; ii < RBW; ii++) {
+            if (wave + 4 * ii >= RB) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = out_row(ii, r);
+                double z = 0.0;
+                if (row < nz && live[ct]) z = -acc[ii][ct][r] / (a.dt * (a.pl[row] + k2c[ct]));
+                zs[row * LDB + ct * 16 + (lane & 15)] = z;
+            }
         }
     }
     __syncthreads();
-    d4 phi[RB], dphi[RB];
-    zero_acc(phi);
-    zero_acc(dphi);
-    mfma_tile<RB>(phi, a.Bsw, a.BKS, [&](int k) -> double { return k < RB * 16 ? zs[wave][k][col] : 0.0; });
-    mfma_tile<RB>(dphi, a.DBsw, a.BKS, [&](int k) -> double { return k < RB * 16 ? zs[wave][k][col] : 0.0; });
+    d4 phi[RBW][CT], dphi[RBW][CT];
+    zero_acc2(phi);
+    zero_acc2(dphi);
+    gemm_lds<RBW>(phi, a.Bsw, RB, a.BKS, zs);
+    gemm_lds<RBW>(dphi, a.DBsw, RB, a.BKS, zs);
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = out_col(ct);
+        const int part = col & 1;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + (lane >> 4) + 4 * r;
-            const double ph = phi[rb][r];
-            const double other = __shfl_xor(ph, 1);       // the other part of this mode
-            if (!live || row >= nz) continue;
-            const size_t q = size_t(row) * H2 + ci.c;
-            if (!a.fused) {
-                a.phi[q] = ph;
-                a.dphi[q] = dphi[rb][r];
-                continue;
+        for (int ii = 0; ii < RBW; ii++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = out_row(ii, r);
+                const double ph = phi[ii][ct][r];
+                const double other = __shfl_xor(ph, 1);       // the other part of this mode
+                if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
+                const size_t q = size_t(row) * H2 + col;
+                if (!a.fused) {
+                    a.phi[q] = ph;
+                    a.dphi[q] = dphi[ii][ct][r];
+                    continue;
+                }
+                const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
+                const bool wall = row == 0 || row == nz - 1;
+                // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
+                const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
+                const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
+                a.u[q] = wall ? 0.0 : a.us[q] - a.dt * du;
+                a.v[q] = wall ? 0.0 : a.vs[q] - a.dt * dv;
+                a.w[q] = wall ? 0.0 : a.ws[q] - a.dt * dphi[ii][ct][r];
+                a.p[q] = a.p[q] + ph;
             }
-            const double pre = ci.part == 0 ? ph : other, pim = ci.part == 0 ? other : ph;
-            const bool wall = row == 0 || row == nz - 1;
-            // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
-            const double du = ci.part == 0 ? -kxe * pim : kxe * pre;
-            const double dv = ci.part == 0 ? -kye * pim : kye * pre;
-            a.u[q] = wall ? 0.0 : a.us[q] - a.dt * du;
-            a.v[q] = wall ? 0.0 : a.vs[q] - a.dt * dv;
-            a.w[q] = wall ? 0.0 : a.ws[q] - a.dt * dphi[rb][r];
-            a.p[q] = a.p[q] + ph;
         }
     }
 }
 
 // mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
-// (DNS_pressure_BC_3D.f90:1702-1711): phi_0 = 0, phi_{1:} = M0 b_{1:}, b = w D1 w* / dt.
-// Each matrix-vector row is one wavefront: coalesced row loads, a fixed-order wave reduction.
+// (DNS_pressure_BC_3D.f90:1702-1711).  Its solution is linear in w*(0,0,:):
+// phi = E0 w*, D1 phi = F0 w*, with E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0
+// precomputed on the host.  One block per row k, one wavefront.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     return v;
 }
 
-__global__ __launch_bounds__(512) void k_poisson_mode0(Mode0Args a) {
-    extern __shared__ double sh[];
-    const int nz = a.nz;
-    double* ws = sh;            // [2][nz]
-    double* b = sh + 2 * nz;    // [2][nz]
-    double* x = sh + 4 * nz;    // [2][nz]
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
-    for (int e = t; e < 2 * nz; e += blockDim.x) {
-        const int part = e / nz, k = e - part * nz;
-        ws[e] = a.ws[size_t(k) * a.H2 + part];
+__global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) {
+    const int nz = a.nz, k = blockIdx.x, lane = threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, d0 = 0.0, d1 = 0.0;
+    for (int q = lane; q < nz; q += 64) {
+        const double w0 = a.ws[size_t(q) * a.H2], w1 = a.ws[size_t(q) * a.H2 + 1];
+        const double e = a.e0[size_t(k) * nz + q], f = a.f0[size_t(k) * nz + q];
+        s0 += e * w0; s1 += e * w1; d0 += f * w0; d1 += f * w1;
     }
-    __syncthreads();
-    for (int k = wave; k < nz; k += nw) {           // b = w * (D1 w*) / dt
-        double s0 = 0.0, s1 = 0.0;
-        for (int q = lane; q < nz; q += 64) {
-            const double d = a.d1[size_t(k) * nz + q];
-            s0 += d * ws[q]; s1 += d * ws[nz + q];
-        }
-        s0 = wave_sum(s0); s1 = wave_sum(s1);
-        if (lane == 0) { b[k] = a.wz[k] * s0 / a.dt; b[nz + k] = a.wz[k] * s1 / a.dt; }
-    }
-    __syncthreads();
-    for (int k = wave; k < nz; k += nw) {           // x_{1:} = M0 b_{1:}, x_0 = 0
-        double s0 = 0.0, s1 = 0.0;
-        if (k > 0)
-            for (int q = 1 + lane; q < nz; q += 64) {
-                const double mq = a.m0[size_t(k - 1) * (nz - 1) + (q - 1)];
-                s0 += mq * b[q]; s1 += mq * b[nz + q];
-            }
-        s0 = wave_sum(s0); s1 = wave_sum(s1);
-        if (lane == 0) { x[k] = s0; x[nz + k] = s1; }
-    }
-    __syncthreads();
-    for (int k = wave; k < nz; k += nw) {           // D1 x and the outputs of row k
-        double s0 = 0.0, s1 = 0.0;
-        for (int q = lane; q < nz; q += 64) {
-            const double d = a.d1[size_t(k) * nz + q];
-            s0 += d * x[q]; s1 += d * x[nz + q];
-        }
-        s0 = wave_sum(s0); s1 = wave_sum(s1);
-        if (lane < 2) {
-            const int part = lane;
-            const double dx = part ? s1 : s0;
-            const double xv = x[part * nz + k];
-            const size_t o = size_t(k) * a.H2 + part;
-            if (!a.fused) {
-                a.phi[o] = xv;
-                a.dphi[o] = dx;
-            } else {
-                const bool wall = k == 0 || k == nz - 1;
-                a.u[o] = wall ? 0.0 : a.us[o];
-                a.v[o] = wall ? 0.0 : a.vs[o];
-                a.w[o] = wall ? 0.0 : ws[part * nz + k] - a.dt * dx;
-                a.p[o] = a.p[o] + xv;
-            }
+    s0 = wave_sum(s0); s1 = wave_sum(s1); d0 = wave_sum(d0); d1 = wave_sum(d1);
+    if (lane < 2) {
+        const int part = lane;
+        const double xv = part ? s1 : s0, dx = part ? d1 : d0;
+        const size_t o = size_t(k) * a.H2 + part;
+        if (!a.fused) {
+            a.phi[o] = xv;
+            a.dphi[o] = dx;
+        } else {
+            const bool wall = k == 0 || k == nz - 1;
+            a.u[o] = wall ? 0.0 : a.us[o];
+            a.v[o] = wall ? 0.0 : a.vs[o];
+            a.w[o] = wall ? 0.0 : a.ws[o] - a.dt * dx;
+            a.p[o] = a.p[o] + xv;
         }
     }
 }
@@ -406,70 +427,47 @@ __global__ void k_reduce_final(ReduceArgs a) {
 // ---------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------
-static inline dim3 col_grid(int H2, int y) { return dim3((H2 + 63) / 64, y); }
-
-template <int RB>
-static hipError_t zderiv_rb(int mode, const ZArgs& a, int nf, hipStream_t s) {
-    if (mode == Z_VORT) hipLaunchKernelGGL((k_zderiv<RB, Z_VORT>), col_grid(a.H2, 1), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_zderiv<RB, Z_PLAIN>), col_grid(a.H2, nf), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-template <int RB>
-static hipError_t helm_rb(const HelmArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_helmholtz<RB>), col_grid(a.H2, 3), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-template <int RB>
-static hipError_t pois_rb(const PoisArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL((k_poisson<RB>), col_grid(a.H2, 1), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
+static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
+static inline int rbw_of(int rows) { return ((rows + 15) / 16 + 3) / 4; }
 
 hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s) {
-    switch (rb) {
-        case 1: return zderiv_rb<1>(mode, a, nf, s);
-        case 2: return zderiv_rb<2>(mode, a, nf, s);
-        case 3: return zderiv_rb<3>(mode, a, nf, s);
-        case 4: return zderiv_rb<4>(mode, a, nf, s);
-        case 5: return zderiv_rb<5>(mode, a, nf, s);
-        case 6: return zderiv_rb<6>(mode, a, nf, s);
-        case 7: return zderiv_rb<7>(mode, a, nf, s);
-        case 8: return zderiv_rb<8>(mode, a, nf, s);
-        case 9: return zderiv_rb<9>(mode, a, nf, s);
+    const int rbw = (rb + 3) / 4;
+    const dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
+    switch (rbw * 2 + (mode == Z_VORT)) {
+        case 2: hipLaunchKernelGGL((k_zderiv<1, Z_PLAIN>), g, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_zderiv<1, Z_VORT>), g, dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_zderiv<2, Z_PLAIN>), g, dim3(256), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((k_zderiv<2, Z_VORT>), g, dim3(256), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((k_zderiv<3, Z_PLAIN>), g, dim3(256), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((k_zderiv<3, Z_VORT>), g, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
 }
 
 hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s) {
-    switch (rbi) {
-        case 1: return helm_rb<1>(a, s);
-        case 2: return helm_rb<2>(a, s);
-        case 3: return helm_rb<3>(a, s);
-        case 4: return helm_rb<4>(a, s);
-        case 5: return helm_rb<5>(a, s);
-        case 6: return helm_rb<6>(a, s);
-        case 7: return helm_rb<7>(a, s);
-        case 8: return helm_rb<8>(a, s);
-        case 9: return helm_rb<9>(a, s);
+    const dim3 g = col_grid(a.H2, 3);
+    switch ((rbi + 3) / 4) {
+        case 1: hipLaunchKernelGGL((k_helmholtz<1>), g, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_helmholtz<2>), g, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_helmholtz<3>), g, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
 }
+
 hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s) {
-    switch (rb) {
-        case 1: return pois_rb<1>(a, s);
-        case 2: return pois_rb<2>(a, s);
-        case 3: return pois_rb<3>(a, s);
-        case 4: return pois_rb<4>(a, s);
-        case 5: return pois_rb<5>(a, s);
-        case 6: return pois_rb<6>(a, s);
-        case 7: return pois_rb<7>(a, s);
-        case 8: return pois_rb<8>(a, s);
-        case 9: return pois_rb<9>(a, s);
+    const dim3 g = col_grid(a.H2, 1);
+    switch ((rb + 3) / 4) {
+        case 1: hipLaunchKernelGGL((k_poisson<1>), g, dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_poisson<2>), g, dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_poisson<3>), g, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
 }
 hipError_t launch_poisson_mode0(const Mode0Args& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_poisson_mode0, dim3(1), dim3(512), size_t(6) * a.nz * sizeof(double), s, a);
+    hipLaunchKernelGGL(k_poisson_mode0, dim3(a.nz), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_correction(const CorrArgs& a, hipStream_t s) {
@@ -495,53 +493,6 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {
 }  // namespace dns
 
 namespace dns {
-// z-derivative test variants: 1 = branch-free operand loads, 2 = explicit delays per k-step
-template <int RB, int VAR>
-__global__ __launch_bounds__(256) void k_zderiv_variant(ZArgs a) {
-    const ColInfo ci = col_info(a.H2, a.nxh);
-    const int lane = threadIdx.x & 63;
-    const int H2 = a.H2, nz = a.nz;
-    const double* in = a.in[0];
-    const int cc = ci.valid ? ci.c : 0;
-    d4 acc[RB];
-    zero_acc(acc);
-    auto bl = [&](int k) -> double {
-        const int kc = k < nz ? k : nz - 1;
-        const double v = in[size_t(kc) * H2 + cc];
-        return (ci.valid && k < nz) ? v : 0.0;
-    };
-    if (VAR == 1) mfma_tile<RB>(acc, a.Asw, a.KS, bl);
-    else mfma_tile_delayed<RB>(acc, a.Asw, a.KS, bl);
-    if (!ci.valid) return;
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int row = rb * 16 + (lane >> 4) + 4 * r;
-            if (row < nz) a.out[0][size_t(row) * H2 + ci.c] = acc[rb][r];
-        }
-}
-template <int RB>
-static hipError_t zvar_rb(int var, const ZArgs& a, hipStream_t s) {
-    if (var == 1) hipLaunchKernelGGL((k_zderiv_variant<RB, 1>), dim3((a.H2 + 63) / 64), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_zderiv_variant<RB, 2>), dim3((a.H2 + 63) / 64), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-hipError_t launch_zderiv_variant(int rb, int var, const ZArgs& a, hipStream_t s) {
-    switch (rb) {
-        case 1: return zvar_rb<1>(var, a, s);
-        case 2: return zvar_rb<2>(var, a, s);
-        case 3: return zvar_rb<3>(var, a, s);
-        case 4: return zvar_rb<4>(var, a, s);
-        case 5: return zvar_rb<5>(var, a, s);
-        case 6: return zvar_rb<6>(var, a, s);
-        case 7: return zvar_rb<7>(var, a, s);
-        case 8: return zvar_rb<8>(var, a, s);
-        case 9: return zvar_rb<9>(var, a, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
 // Lane-layout probe of v_mfma_f64_16x16x4f64: block b uses a one-hot A operand at lane b/64
 // and a one-hot B operand at lane b%64; out[b*256 + lane*4 + q] = result item q of lane.
 __global__ __launch_bounds__(64) void k_mfma_probe(double* out) {

@@ -22,13 +22,6 @@ const char* dns_profile_kernel_name(int i);
 /* Lane layout of the FP64 MFMA used by the contractions: 4096 single products with one-hot
  * operands (A at lane b/64, B at lane b%64); out[b*256 + lane*4 + q] (4096*256 doubles). */
 int dns_debug_mfma_probe(double* out);
-/* As dns_debug_zderiv with a variant of the contraction loop: 0 production, 1 branch-free
- * operand loads, 2 a fixed delay after every k-step. */
-int dns_debug_zderiv_variant(dns_ctx* ctx, int variant, const double* spec_in, double* spec_out);
-/* Host-only (no GPU needed): one operator matrix of the per-mode solves, row-major.
- * which: 0 Helmholtz [T | f T D2_I], 1 B, 2 c1, 3 c2, 4 hl, 5 Poisson [Tf | Tf D1], 6 Bf,
- * 7 D1 Bf, 8 lf, 9 M0, 10 D1, 11 LGL weights, 12 D2.  With out == NULL only the shape. */
-int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols);
 /* max relative reconstruction error of the Helmholtz and Poisson eigen bases. */
 int dns_debug_operator_error(dns_ctx* ctx, double* out2);
 
