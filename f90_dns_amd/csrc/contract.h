@@ -18,6 +18,8 @@ constexpr int LDB = BCOLS + 2;     // padded LDS row stride, doubles
 
 // acc[i][ct] += A[rb = wave + 4i] x B, B produced by bprod(k, mode_local) -> double2 (Re, Im)
 // of row k for the block's mode mode_local (zero beyond the operand's rows).  lds: 2*KC*LDB.
+// Both operands are fetched one chunk ahead: B into registers then LDS, A (this wave's rows of
+// the next four k-steps) into registers.
 template <int RBW, class BP>
 __device__ __forceinline__ void gemm_staged(d4 (&acc)[RBW][CT], const double* __restrict__ Asw, int RB, int KS,
                                             BP&& bprod, double* lds) {
@@ -26,6 +28,19 @@ __device__ __forceinline__ void gemm_staged(d4 (&acc)[RBW][CT], const double* __
     const int t = threadIdx.x;
     const int rt = t >> 4, mt = t & 15;            // staging: chunk row rt, mode mt
     const int nchunks = (KS + 3) / 4;
+    double areg[RBW][4], anext[RBW][4];
+    auto load_a = [&](double (&dst)[RBW][4], int ch) {
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int rb = wave + 4 * i;
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++) {
+                const int s = ch * 4 + sl;
+                dst[i][sl] = (rb < RB && s < KS) ? Asw[(size_t(rb) * KS + s) * 64 + lane] : 0.0;
+            }
+        }
+    };
+    load_a(areg, 0);
     double2 v = bprod(rt, mt);
     lds[rt * LDB + 2 * mt] = v.x;
     lds[rt * LDB + 2 * mt + 1] = v.y;
@@ -33,30 +48,33 @@ __device__ __forceinline__ void gemm_staged(d4 (&acc)[RBW][CT], const double* __
     for (int ch = 0; ch < nchunks; ch++) {
         const double* cur = lds + (ch & 1) * KC * LDB;
         double2 nxt = make_double2(0.0, 0.0);
-        if (ch + 1 < nchunks) nxt = bprod((ch + 1) * KC + rt, mt);
+        const bool more = ch + 1 < nchunks;
+        if (more) {
+            nxt = bprod((ch + 1) * KC + rt, mt);
+            load_a(anext, ch + 1);
+        }
 #pragma unroll
         for (int sl = 0; sl < 4; sl++) {
-            const int s = ch * 4 + sl;
-            if (s < KS) {
-                double b[CT];
+            double b[CT];
 #pragma unroll
-                for (int ct = 0; ct < CT; ct++) b[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
+            for (int ct = 0; ct < CT; ct++) b[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
 #pragma unroll
-                for (int i = 0; i < RBW; i++) {
-                    const int rb = wave + 4 * i;
-                    if (rb < RB) {
-                        const double a = Asw[(size_t(rb) * KS + s) * 64 + lane];
+            for (int i = 0; i < RBW; i++) {
+                if (wave + 4 * i < RB) {
 #pragma unroll
-                        for (int ct = 0; ct < CT; ct++)
-                            acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct], acc[i][ct], 0, 0, 0);
-                    }
+                    for (int ct = 0; ct < CT; ct++)
+                        acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[i][sl], b[ct], acc[i][ct], 0, 0, 0);
                 }
             }
         }
-        if (ch + 1 < nchunks) {
+        if (more) {
             double* nb = lds + ((ch + 1) & 1) * KC * LDB;
             nb[rt * LDB + 2 * mt] = nxt.x;
             nb[rt * LDB + 2 * mt + 1] = nxt.y;
+#pragma unroll
+            for (int i = 0; i < RBW; i++)
+#pragma unroll
+                for (int sl = 0; sl < 4; sl++) areg[i][sl] = anext[i][sl];
         }
         __syncthreads();
     }
@@ -69,21 +87,27 @@ __device__ __forceinline__ void gemm_lds(d4 (&acc)[RBW][CT], const double* __res
                                          const double* zs) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kk = lane >> 4, cl = lane & 15;
-#pragma unroll 4
+    double areg[RBW], anext[RBW];
+#pragma unroll
+    for (int i = 0; i < RBW; i++) areg[i] = (wave + 4 * i < RB) ? Asw[(size_t(wave + 4 * i) * KS) * 64 + lane] : 0.0;
     for (int s = 0; s < KS; s++) {
+        const bool more = s + 1 < KS;
+#pragma unroll
+        for (int i = 0; i < RBW; i++)
+            anext[i] = (more && wave + 4 * i < RB) ? Asw[(size_t(wave + 4 * i) * KS + s + 1) * 64 + lane] : 0.0;
         double b[CT];
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) b[ct] = zs[(4 * s + kk) * LDB + ct * 16 + cl];
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            const int rb = wave + 4 * i;
-            if (rb < RB) {
-                const double a = Asw[(size_t(rb) * KS + s) * 64 + lane];
+            if (wave + 4 * i < RB) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++)
-                    acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[ct], acc[i][ct], 0, 0, 0);
+                    acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[i], b[ct], acc[i][ct], 0, 0, 0);
             }
         }
+#pragma unroll
+        for (int i = 0; i < RBW; i++) areg[i] = anext[i];
     }
 }
 

@@ -246,11 +246,9 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
         }
     }
     __syncthreads();
-    d4 phi[RBW][CT], dphi[RBW][CT];
-    zero_acc2(phi);
-    zero_acc2(dphi);
-    gemm_lds<RBW>(phi, a.Bsw, RB, a.BKS, zs);
-    gemm_lds<RBW>(dphi, a.DBsw, RB, a.BKS, zs);
+    // phi = Bf z: pressure increment and the horizontal projection
+    zero_acc2(acc);
+    gemm_lds<RBW>(acc, a.Bsw, RB, a.BKS, zs);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct);
@@ -260,13 +258,12 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                const double ph = phi[ii][ct][r];
+                const double ph = acc[ii][ct][r];
                 const double other = __shfl_xor(ph, 1);       // the other part of this mode
                 if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
                 const size_t q = size_t(row) * H2 + col;
                 if (!a.fused) {
                     a.phi[q] = ph;
-                    a.dphi[q] = dphi[ii][ct][r];
                     continue;
                 }
                 const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
@@ -276,8 +273,29 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
                 const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
                 a.u[q] = wall ? 0.0 : a.us[q] - a.dt * du;
                 a.v[q] = wall ? 0.0 : a.vs[q] - a.dt * dv;
-                a.w[q] = wall ? 0.0 : a.ws[q] - a.dt * dphi[ii][ct][r];
                 a.p[q] = a.p[q] + ph;
+            }
+        }
+    }
+    // D1 phi = (D1 Bf) z: the wall-normal projection
+    zero_acc2(acc);
+    gemm_lds<RBW>(acc, a.DBsw, RB, a.BKS, zs);
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = out_col(ct);
+#pragma unroll
+        for (int ii = 0; ii < RBW; ii++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = out_row(ii, r);
+                if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
+                const size_t q = size_t(row) * H2 + col;
+                if (!a.fused) {
+                    a.dphi[q] = acc[ii][ct][r];
+                    continue;
+                }
+                const bool wall = row == 0 || row == nz - 1;
+                a.w[q] = wall ? 0.0 : a.ws[q] - a.dt * acc[ii][ct][r];
             }
         }
     }
