@@ -105,6 +105,7 @@ def lib():
         L.dns_debug_spectral.argtypes = [_vp, ctypes.c_int, _vp]
         L.dns_debug_zderiv.argtypes = [_vp, _vp, _vp]
         L.dns_debug_mfma_probe.argtypes = [_dp]
+        L.dns_debug_mfma_probe_f32.argtypes = [_dp]
         L.dns_debug_operator_error.argtypes = [_vp, _dp]
         L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_profile_kernel_name.argtypes = [ctypes.c_int]

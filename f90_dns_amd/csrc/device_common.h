@@ -11,51 +11,64 @@ struct Factors {
     int f[14];
 };
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// complex arithmetic on double2 / float2
+template <class V> struct Scalar;
+template <> struct Scalar<double2> { using type = double; };
+template <> struct Scalar<float2> { using type = float; };
+
+__device__ __forceinline__ double2 mk2(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ float2 mk2(float a, float b) { return make_float2(a, b); }
+template <class V> __device__ __forceinline__ V cadd(V a, V b) { return mk2(a.x + b.x, a.y + b.y); }
+template <class V> __device__ __forceinline__ V csub(V a, V b) { return mk2(a.x - b.x, a.y - b.y); }
+template <class V> __device__ __forceinline__ V cmul(V a, V b) {
+    return mk2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // sign * i * z
-__device__ __forceinline__ double2 cmuli(double2 z, int sign) {
-    return sign > 0 ? make_double2(-z.y, z.x) : make_double2(z.y, -z.x);
+template <class V> __device__ __forceinline__ V cmuli(V z, int sign) {
+    return sign > 0 ? mk2(-z.y, z.x) : mk2(z.y, -z.x);
 }
 
 // In-place R-point DFT out_q = sum_r v_r exp(sign 2 pi i q r / R), unnormalised.
-template <int R> __device__ __forceinline__ void dft_small(double2 (&v)[R], int sign);
-
-template <> __device__ __forceinline__ void dft_small<2>(double2 (&v)[2], int) {
-    double2 a = v[0], b = v[1];
-    v[0] = cadd(a, b); v[1] = csub(a, b);
-}
-template <> __device__ __forceinline__ void dft_small<4>(double2 (&v)[4], int sign) {
-    double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    double2 s13 = cadd(v[1], v[3]), d13 = cmuli(csub(v[1], v[3]), sign);
-    v[0] = cadd(s02, s13); v[2] = csub(s02, s13);
-    v[1] = cadd(d02, d13); v[3] = csub(d02, d13);
-}
-template <> __device__ __forceinline__ void dft_small<3>(double2 (&v)[3], int sign) {
-    const double c = -0.5, s = 0.86602540378443864676372317075293618;   // cos, sin(2 pi/3)
-    double2 a = v[0], sum = cadd(v[1], v[2]), dif = csub(v[1], v[2]);
-    double2 t = make_double2(a.x + c * sum.x, a.y + c * sum.y);
-    double2 r = cmuli(make_double2(s * dif.x, s * dif.y), sign);
-    v[0] = cadd(a, sum);
-    v[1] = cadd(t, r);
-    v[2] = csub(t, r);
-}
+template <int R, class V> struct DFT;
+template <class V> struct DFT<2, V> {
+    static __device__ __forceinline__ void run(V (&v)[2], int) {
+        V a = v[0], b = v[1];
+        v[0] = cadd(a, b); v[1] = csub(a, b);
+    }
+};
+template <class V> struct DFT<4, V> {
+    static __device__ __forceinline__ void run(V (&v)[4], int sign) {
+        V s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        V s13 = cadd(v[1], v[3]), d13 = cmuli(csub(v[1], v[3]), sign);
+        v[0] = cadd(s02, s13); v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13); v[3] = csub(d02, d13);
+    }
+};
+template <class V> struct DFT<3, V> {
+    static __device__ __forceinline__ void run(V (&v)[3], int sign) {
+        using S = typename Scalar<V>::type;
+        const S c = S(-0.5), s = S(0.86602540378443864676372317075293618);   // cos, sin(2 pi/3)
+        V a = v[0], sum = cadd(v[1], v[2]), dif = csub(v[1], v[2]);
+        V t = mk2(a.x + c * sum.x, a.y + c * sum.y);
+        V r = cmuli(mk2(s * dif.x, s * dif.y), sign);
+        v[0] = cadd(a, sum);
+        v[1] = cadd(t, r);
+        v[2] = csub(t, r);
+    }
+};
 
 // One Stockham stage over nb independent length-N transforms held in LDS.  Element e of
 // transform b lives at lds[b*bs + e*es].  Every thread first reads all inputs of its
 // butterflies into registers, then (after a barrier) writes the outputs, so the stage
 // runs in place.  tw is exp(-2 pi i k / L) for k < L with L a multiple of N.
 // BFAST: consecutive threads take consecutive transforms (b), else consecutive j.
-template <int R, int NT, int MAXE, bool BFAST>
-__device__ __forceinline__ void fft_stage(double2* lds, int N, int Ns, int nb, int es, int bs,
-                                          const double2* __restrict__ tw, int L, int sign) {
+template <int R, int NT, int MAXE, bool BFAST, class V>
+__device__ __forceinline__ void fft_stage(V* lds, int N, int Ns, int nb, int es, int bs,
+                                          const V* __restrict__ tw, int L, int sign) {
     constexpr int MAXB = (MAXE / R + NT - 1) / NT;
     const int m = N / R;
     const int total = nb * m;
-    double2 v[MAXB][R];
+    V v[MAXB][R];
     int bidx[MAXB], jidx[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
@@ -81,12 +94,12 @@ __device__ __forceinline__ void fft_stage(double2* lds, int N, int Ns, int nb, i
             if (k) {
 #pragma unroll
                 for (int r = 1; r < R; r++) {
-                    double2 w = tw[k * r * twstep];
+                    V w = tw[k * r * twstep];
                     if (sign > 0) w.y = -w.y;
                     v[q][r] = cmul(v[q][r], w);
                 }
             }
-            dft_small<R>(v[q], sign);
+            DFT<R, V>::run(v[q], sign);
             const int base = kq * Ns * R + k;
 #pragma unroll
             for (int r = 0; r < R; r++) lds[b * bs + (base + r * Ns) * es] = v[q][r];
@@ -95,15 +108,15 @@ __device__ __forceinline__ void fft_stage(double2* lds, int N, int Ns, int nb, i
     __syncthreads();
 }
 
-template <int NT, int MAXE, bool BFAST>
-__device__ __forceinline__ void fft_lds(double2* lds, const Factors& fac, int nb, int es, int bs,
-                                        const double2* __restrict__ tw, int L, int sign) {
+template <int NT, int MAXE, bool BFAST, class V>
+__device__ __forceinline__ void fft_lds(V* lds, const Factors& fac, int nb, int es, int bs,
+                                        const V* __restrict__ tw, int L, int sign) {
     int Ns = 1;
     for (int s = 0; s < fac.nf; s++) {
         const int R = fac.f[s];
-        if (R == 4) fft_stage<4, NT, MAXE, BFAST>(lds, fac.n, Ns, nb, es, bs, tw, L, sign);
-        else if (R == 2) fft_stage<2, NT, MAXE, BFAST>(lds, fac.n, Ns, nb, es, bs, tw, L, sign);
-        else fft_stage<3, NT, MAXE, BFAST>(lds, fac.n, Ns, nb, es, bs, tw, L, sign);
+        if (R == 4) fft_stage<4, NT, MAXE, BFAST, V>(lds, fac.n, Ns, nb, es, bs, tw, L, sign);
+        else if (R == 2) fft_stage<2, NT, MAXE, BFAST, V>(lds, fac.n, Ns, nb, es, bs, tw, L, sign);
+        else fft_stage<3, NT, MAXE, BFAST, V>(lds, fac.n, Ns, nb, es, bs, tw, L, sign);
         Ns *= R;
     }
 }

@@ -35,10 +35,12 @@ struct dns_ctx {
     size_t N, NS;
     double theta, ybar;
     int rb, rbi;
+    bool mixed = false;          // FP32 transforms, product and omega/S storage; FP64 solves
     hipStream_t stream = nullptr;
     FFTGeom g;
     Operators op;
     // constants
+    float *twx32 = nullptr, *twy32 = nullptr;
     double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *e0 = nullptr, *f0 = nullptr;
     unsigned char* mask = nullptr;
     double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr;
@@ -137,7 +139,8 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     if (p->nz < 5 || p->nz > 144) return fail(nullptr, DNS_EINVAL, "nz must be in [5, 144]");
     if (!(p->dt > 0) || !(p->re > 0) || !(p->xlen > 0) || !(p->ylen > 0)) return fail(nullptr, DNS_EINVAL, "dt, re, xlen, ylen must be positive");
     if (p->use_weak_form_divergence) return fail(nullptr, DNS_EUNSUPPORTED, "weak-form pressure RHS is not implemented");
-    if (p->precision != DNS_PREC_FP64) return fail(nullptr, DNS_EUNSUPPORTED, "only DNS_PREC_FP64 is implemented");
+    if (p->precision != DNS_PREC_FP64 && p->precision != DNS_PREC_MIXED)
+        return fail(nullptr, DNS_EINVAL, "precision must be DNS_PREC_FP64 or DNS_PREC_MIXED");
     if (p->flow_control_method != 1 && p->flow_control_method != 2)
         return fail(nullptr, DNS_EINVAL, "flow_control_method must be 1 or 2");
     if (p->flow_control_method == 2 && (p->target_bulk_velocity <= 0 || p->controller_gain <= 0 || p->controller_update_freq <= 0))
@@ -154,10 +157,12 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     c->ybar = 2.0;
     c->rb = (c->nz + 15) / 16;
     c->rbi = (c->nz - 2 + 15) / 16;
+    c->mixed = p->precision == DNS_PREC_MIXED;
     auto bail = [&](int code) { int r = code; destroy_impl(c); return r; };
 
-    if (!fft_geom(c->nx, c->ny, c->g)) {
-        g_create_error = "plane too large for the LDS-resident FP64 transform (nx/2+1)*ny must be <= 8320";
+    if (!fft_geom(c->nx, c->ny, c->g) || (!c->mixed && c->g.cls64 < 0)) {
+        g_create_error = "plane too large for the LDS-resident transform: (nx/2+1)*ny must be <= 8320 in FP64, "
+                         "<= 18624 in mixed precision";
         return bail(DNS_EUNSUPPORTED);
     }
     if (!make_factors(c->ny, c->g.fy) || !make_factors(c->nx / 2, c->g.fx)) {
@@ -181,11 +186,15 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     std::vector<double> tw, xw, yw;
     std::vector<unsigned char> mask;
     twiddles(c->nx, tw);
-    if ((r = upload(c, &c->twx, tw))) { g_create_error = c->err; return bail(r); }
+    std::vector<float> tw32(tw.begin(), tw.end());
+    if ((r = upload(c, &c->twx, tw)) || (r = upload(c, &c->twx32, tw32))) { g_create_error = c->err; return bail(r); }
     twiddles(c->ny, tw);
-    if ((r = upload(c, &c->twy, tw))) { g_create_error = c->err; return bail(r); }
+    tw32.assign(tw.begin(), tw.end());
+    if ((r = upload(c, &c->twy, tw)) || (r = upload(c, &c->twy32, tw32))) { g_create_error = c->err; return bail(r); }
     c->g.twx = reinterpret_cast<const double2*>(c->twx);
     c->g.twy = reinterpret_cast<const double2*>(c->twy);
+    c->g.twx32 = reinterpret_cast<const float2*>(c->twx32);
+    c->g.twy32 = reinterpret_cast<const float2*>(c->twy32);
     wavenumbers(c->nx, c->ny, p->xlen, p->ylen, xw, yw);
     dealias_mask(c->nx, c->ny, mask);
     const Operators& o = c->op;
@@ -237,14 +246,14 @@ static int r2c_fields(dns_ctx* c, const double* const* src, double* const* dst, 
     FFTArgs a = fft_args(c);
     for (int f = 0; f < nf; f++) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
     a.sstride = c->npl; a.dstride = c->H2;
-    HIPCHECK(c, launch_r2c(c->g, R2C_PLAIN, a, c->nz, nf, c->stream));
+    HIPCHECK(c, launch_r2c(c->g, R2C_PLAIN, a, c->nz, nf, false, c->stream));
     return 0;
 }
 static int c2r_fields(dns_ctx* c, const double* const* src, double* const* dst, int nf) {
     FFTArgs a = fft_args(c);
     for (int f = 0; f < nf; f++) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
     a.sstride = c->H2; a.dstride = c->npl;
-    HIPCHECK(c, launch_c2r(c->g, C2R_PLAIN, a, c->nz, nf, c->stream));
+    HIPCHECK(c, launch_c2r(c->g, C2R_PLAIN, a, c->nz, nf, false, c->stream));
     return 0;
 }
 
@@ -344,26 +353,45 @@ static FlowArgs flow_args(dns_ctx* c) {
     return fa;
 }
 
-int dns_compute_source_terms(dns_ctx* c) {
-    if (!c) return DNS_EINVAL;
+static int nl_zderiv(dns_ctx* c) {
     ZArgs z{};
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
     z.Asw = c->D1sw; z.KS = c->op.D1.ks;
     z.in[0] = c->uh; z.in[1] = c->vh; z.in[2] = c->wh;
     z.out[0] = c->ox; z.out[1] = c->oy;
     z.xw = c->xw; z.yw = c->yw;
+    z.out_f32 = c->mixed ? 1 : 0;
     HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->stream));
+    return 0;
+}
+
+static int nl_c2r(dns_ctx* c) {
     FFTArgs a = fft_args(c);
     a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->wh; a.src[3] = c->ox; a.src[4] = c->oy;
     for (int f = 0; f < 6; f++) a.dst[f] = c->phys[f];
     a.sstride = c->H2; a.dstride = c->npl;
-    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz, 6, c->stream));
+    a.src_f32 = c->mixed ? 0x18u : 0u;
+    a.dst_f32 = c->mixed ? 1 : 0;
+    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz, 6, c->mixed, c->stream));
+    return 0;
+}
+
+static int nl_r2c(dns_ctx* c) {
     FFTArgs b = fft_args(c);
     for (int f = 0; f < 6; f++) b.src[f] = c->phys[f];
     for (int f = 0; f < 3; f++) b.dst[f] = c->sh[f];
     b.mask = c->p.use_dealias_23_rule ? c->mask : nullptr;
     b.sstride = c->npl; b.dstride = c->H2;
-    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz, 3, c->stream));
+    b.src_f32 = c->mixed ? 0x3Fu : 0u;
+    b.dst_f32 = c->mixed ? 1 : 0;
+    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz, 3, c->mixed, c->stream));
+    return 0;
+}
+
+int dns_compute_source_terms(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    int r;
+    if ((r = nl_zderiv(c)) || (r = nl_c2r(c)) || (r = nl_r2c(c))) return r;
     FlowArgs fa = flow_args(c);
     HIPCHECK(c, launch_flow_control(fa, c->stream));
     return 0;
@@ -383,6 +411,7 @@ int dns_viscous_step(dns_ctx* c) {
     h.out[0] = c->us; h.out[1] = c->vs; h.out[2] = c->ws;
     h.ph = c->ph;
     h.G = &c->flow->G;
+    h.s_f32 = c->mixed ? 1 : 0;
     HIPCHECK(c, launch_helmholtz(c->rbi, h, c->stream));
     return 0;
 }
@@ -582,7 +611,7 @@ int dns_check_divergence(dns_ctx* c, double* max_div) {
     FFTArgs a = fft_args(c);
     a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->tmp; a.dst[0] = c->phys[0];
     a.sstride = c->H2; a.dstride = c->npl;
-    HIPCHECK(c, launch_c2r(c->g, C2R_DIV, a, c->nz, 1, c->stream));
+    HIPCHECK(c, launch_c2r(c->g, C2R_DIV, a, c->nz, 1, false, c->stream));
     double res[5];
     int r = reduce(c, nullptr, nullptr, nullptr, c->phys[0], res);
     if (r) return r;
@@ -631,25 +660,11 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
     for (int s = 0; s < nsteps; s++) {
         dns_begin_step(c);
         HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
-        ZArgs z{};
-        z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
-        z.Asw = c->D1sw; z.KS = c->op.D1.ks;
-        z.in[0] = c->uh; z.in[1] = c->vh; z.in[2] = c->wh; z.out[0] = c->ox; z.out[1] = c->oy;
-        z.xw = c->xw; z.yw = c->yw;
-        HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->stream));
+        if ((r = nl_zderiv(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-        FFTArgs a = fft_args(c);
-        a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->wh; a.src[3] = c->ox; a.src[4] = c->oy;
-        for (int f = 0; f < 6; f++) a.dst[f] = c->phys[f];
-        a.sstride = c->H2; a.dstride = c->npl;
-        HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz, 6, c->stream));
+        if ((r = nl_c2r(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-        FFTArgs b = fft_args(c);
-        for (int f = 0; f < 6; f++) b.src[f] = c->phys[f];
-        for (int f = 0; f < 3; f++) b.dst[f] = c->sh[f];
-        b.mask = c->p.use_dealias_23_rule ? c->mask : nullptr;
-        b.sstride = c->npl; b.dstride = c->H2;
-        HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz, 3, c->stream));
+        if ((r = nl_r2c(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
         FlowArgs fa = flow_args(c);
         HIPCHECK(c, launch_flow_control(fa, c->stream));
@@ -671,6 +686,19 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
     }
     for (int i = 0; i < NK; i++) out[i] = acc[i] / nsteps;
     return 0;
+}
+
+int dns_debug_mfma_probe_f32(double* out) {
+    if (!out) return DNS_EINVAL;
+    std::string why;
+    if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
+    double* d = nullptr;
+    const size_t n = size_t(4096) * 256;
+    if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
+    hipError_t e = launch_mfma_probe_f32(d, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
 int dns_debug_mfma_probe(double* out) {
@@ -738,7 +766,9 @@ int dns_debug_operator_error(dns_ctx* c, double* out2) {
 
 struct dns_fft_plan {
     FFTGeom g;
+    bool f32 = false;
     double *twx = nullptr, *twy = nullptr;
+    float *twx32 = nullptr, *twy32 = nullptr;
     double *din = nullptr, *dout = nullptr;
     size_t cap_in = 0, cap_out = 0;
     hipStream_t stream = nullptr;
@@ -758,12 +788,14 @@ extern "C" {
 int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
     if (!out) return DNS_EINVAL;
     *out = nullptr;
-    if (precision != DNS_PREC_FP64) { g_create_error = "only FP64 plans are implemented"; return DNS_EUNSUPPORTED; }
+    if (precision != DNS_PREC_FP64 && precision != DNS_PREC_MIXED) { g_create_error = "bad precision"; return DNS_EINVAL; }
     if (nx < 4 || ny < 4 || (nx & 1) || (ny & 1)) { g_create_error = "nx, ny must be even and >= 4"; return DNS_EINVAL; }
     std::string why;
     if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
     dns_fft_plan* p = new dns_fft_plan();
-    if (!fft_geom(nx, ny, p->g) || !make_factors(ny, p->g.fy) || !make_factors(nx / 2, p->g.fx)) {
+    p->f32 = precision == DNS_PREC_MIXED;
+    if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0) || !make_factors(ny, p->g.fy) ||
+        !make_factors(nx / 2, p->g.fx)) {
         delete p;
         g_create_error = "unsupported plane size";
         return DNS_EUNSUPPORTED;
@@ -777,6 +809,17 @@ int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
     (void)hipMemcpy(p->twy, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
     p->g.twx = reinterpret_cast<const double2*>(p->twx);
     p->g.twy = reinterpret_cast<const double2*>(p->twy);
+    {
+        std::vector<float> t32;
+        twiddles(nx, tw); t32.assign(tw.begin(), tw.end());
+        if (hipMalloc(&p->twx32, t32.size() * sizeof(float)) != hipSuccess) return DNS_ENOMEM;
+        (void)hipMemcpy(p->twx32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
+        twiddles(ny, tw); t32.assign(tw.begin(), tw.end());
+        if (hipMalloc(&p->twy32, t32.size() * sizeof(float)) != hipSuccess) return DNS_ENOMEM;
+        (void)hipMemcpy(p->twy32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
+        p->g.twx32 = reinterpret_cast<const float2*>(p->twx32);
+        p->g.twy32 = reinterpret_cast<const float2*>(p->twy32);
+    }
     (void)fft_prepare(p->g);
     (void)hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     *out = p;
@@ -786,7 +829,7 @@ int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
 int dns_fft_plan_destroy(dns_fft_plan* p) {
     if (!p) return 0;
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    (void)hipFree(p->twx); (void)hipFree(p->twy);
+    (void)hipFree(p->twx); (void)hipFree(p->twy); (void)hipFree(p->twx32); (void)hipFree(p->twy32);
     if (p->din) (void)hipFree(p->din);
     if (p->dout) (void)hipFree(p->dout);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -801,7 +844,7 @@ int dns_fft_forward_device(dns_fft_plan* p, const double* in, double* out, int n
     a.src[0] = in; a.dst[0] = out;
     a.sstride = (long long)p->g.nx * p->g.ny; a.dstride = 2LL * p->g.nxh * p->g.ny;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
-    return launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, s) == hipSuccess ? 0 : DNS_EDEVICE;
+    return launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
 int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
@@ -811,7 +854,7 @@ int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int 
     a.src[0] = in; a.dst[0] = out;
     a.sstride = 2LL * p->g.nxh * p->g.ny; a.dstride = (long long)p->g.nx * p->g.ny;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
-    return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, s) == hipSuccess ? 0 : DNS_EDEVICE;
+    return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
 int dns_fft_forward(dns_fft_plan* p, const double* in, double* out, int nplanes) {

@@ -13,39 +13,45 @@ enum R2COp { R2C_PLAIN = 0, R2C_CROSS = 1 };
 struct FFTGeom {
     int nx, ny, nxh, M;          // M = nx/2
     Factors fy, fx;              // factors of ny and of M
-    const double2* twx;          // exp(-2 pi i k/nx), k < nx
+    const double2* twx;          // exp(-2 pi i k/nx), k < nx (FP64)
     const double2* twy;          // exp(-2 pi i k/ny), k < ny
-    int cls;                     // plane-size class (0, 1, 2), -1 = unsupported
-    size_t lds_bytes;
+    const float2* twx32;         // the same tables rounded to FP32
+    const float2* twy32;
+    int cls64, cls32;            // plane-size class for FP64 / FP32 compute (-1 = unsupported)
 };
 
+// Operands of the plane kernels.  src/dst hold double or float data: bit f of src_f32 marks
+// src[f] as float; dst_f32 marks all destinations as float.  Strides are in elements.
 struct FFTArgs {
     int nx, ny, nxh, M;
     Factors fy, fx;
-    const double2* twx;
-    const double2* twy;
+    const void* twx;
+    const void* twy;
     const double* xw;
     const double* yw;
     const unsigned char* mask;   // r2c: dealias mask (null = none)
-    const double* src[6];
-    double* dst[6];
-    long long sstride, dstride;  // doubles between consecutive planes of one field
+    const void* src[6];
+    void* dst[6];
+    long long sstride, dstride;
+    unsigned src_f32;
+    int dst_f32;
 };
 
 bool fft_geom(int nx, int ny, FFTGeom& g);
-// raise the dynamic-LDS limit of the plane kernels of g's size class (once per process)
+// raise the dynamic-LDS limit of the plane kernels of g's size classes (once per process)
 hipError_t fft_prepare(const FFTGeom& g);
-// c2r: grid (nplanes, nfields); r2c likewise.  Return hipSuccess or the launch error.
-hipError_t launch_c2r(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, hipStream_t s);
-hipError_t launch_r2c(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, hipStream_t s);
+// grid (nplanes, nfields); f32: compute in FP32 (else FP64)
+hipError_t launch_c2r(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, bool f32, hipStream_t s);
+hipError_t launch_r2c(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, bool f32, hipStream_t s);
 
 // ---- wall-normal contractions and per-mode solves ---------------------------------------
 struct ZArgs {
     int nz, H2, nxh, ny, nx;     // H2 = 2 * (nx/2+1) * ny real columns per plane
     const double* Asw; int KS;   // swizzled operator (nz rows)
     const double* in[3];
-    double* out[3];
+    double* out[3];              // float when out_f32
     const double* xw; const double* yw;
+    int out_f32;
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
 hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s);
@@ -58,8 +64,9 @@ struct HelmArgs {
     double f, dt, nxny;
     int cn;
     const double* xw; const double* yw;
-    const double* uh[3]; const double* sh[3]; double* out[3];
+    const double* uh[3]; const double* sh[3]; double* out[3];   // sh float when s_f32
     const double* ph;
+    int s_f32;
     const double* G;             // device scalar: current pressure gradient
 };
 hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s);
@@ -129,5 +136,6 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 
 // 4096 one-hot products of v_mfma_f64_16x16x4f64 (out: 4096 * 256 doubles)
 hipError_t launch_mfma_probe(double* out, hipStream_t s);
+hipError_t launch_mfma_probe_f32(double* out, hipStream_t s);
 
 }  // namespace dns

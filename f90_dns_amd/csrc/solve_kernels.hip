@@ -74,16 +74,20 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
                     if (MODE == Z_PLAIN) {
                         a.out[f][size_t(row) * H2 + col] = y;
                     } else {
+                        double res;
                         const double wo = a.in[2][size_t(row) * H2 + (col ^ 1)];   // the other part of w
                         const double wr = part == 0 ? a.in[2][size_t(row) * H2 + col] : wo;
                         const double wi = part == 0 ? wo : a.in[2][size_t(row) * H2 + col];
                         if (f == 1) {        // omega_x = i ky w - D1 v
                             const double ky = a.yw[j];
-                            a.out[0][size_t(row) * H2 + col] = part == 0 ? (-ky * wi - y) : (ky * wr - y);
+                            res = part == 0 ? (-ky * wi - y) : (ky * wr - y);
                         } else {             // omega_y = D1 u - i kx w
                             const double kx = a.xw[i];
-                            a.out[1][size_t(row) * H2 + col] = part == 0 ? (y + kx * wi) : (y - kx * wr);
+                            res = part == 0 ? (y + kx * wi) : (y - kx * wr);
                         }
+                        const int o = f == 1 ? 0 : 1;
+                        if (a.out_f32) reinterpret_cast<float*>(a.out[o])[size_t(row) * H2 + col] = float(res);
+                        else a.out[o][size_t(row) * H2 + col] = res;
                     }
                 }
             }
@@ -139,7 +143,13 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
             const double kx = a.xw[i], ky = a.yw[j];
             const double alpha = a.cn ? 1.0 - a.f * (kx * kx + ky * ky) : 1.0;
             const double2 u = reinterpret_cast<const double2*>(uh + size_t(k + 1) * H2)[mode];
-            const double2 sv = reinterpret_cast<const double2*>(sh + size_t(k + 1) * H2)[mode];
+            double2 sv;
+            if (a.s_f32) {
+                const float2 sf = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(sh) + size_t(k + 1) * H2)[mode];
+                sv = make_double2(double(sf.x), double(sf.y));
+            } else {
+                sv = reinterpret_cast<const double2*>(sh + size_t(k + 1) * H2)[mode];
+            }
             const double fr = mode == 0 ? force : 0.0;
             return make_double2(alpha * u.x + a.dt * (sv.x + fr), alpha * u.y + a.dt * sv.y);
         }
@@ -524,6 +534,22 @@ __global__ __launch_bounds__(64) void k_mfma_probe(double* out) {
 }
 hipError_t launch_mfma_probe(double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_mfma_probe, dim3(4096), dim3(64), 0, s, out);
+    return hipGetLastError();
+}
+
+// Same probe for v_mfma_f32_16x16x4f32 (result items converted to double).
+typedef float f4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(64) void k_mfma_probe_f32(double* out) {
+    const int lane = threadIdx.x;
+    const int la = blockIdx.x / 64, lb = blockIdx.x % 64;
+    const float a = lane == la ? 1.0f : 0.0f, b = lane == lb ? 1.0f : 0.0f;
+    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    double* o = out + size_t(blockIdx.x) * 256 + lane * 4;
+    o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2]; o[3] = acc[3];
+}
+hipError_t launch_mfma_probe_f32(double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_mfma_probe_f32, dim3(4096), dim3(64), 0, s, out);
     return hipGetLastError();
 }
 }  // namespace dns

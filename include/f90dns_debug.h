@@ -22,6 +22,8 @@ const char* dns_profile_kernel_name(int i);
 /* Lane layout of the FP64 MFMA used by the contractions: 4096 single products with one-hot
  * operands (A at lane b/64, B at lane b%64); out[b*256 + lane*4 + q] (4096*256 doubles). */
 int dns_debug_mfma_probe(double* out);
+/* The same probe for v_mfma_f32_16x16x4f32 (the mixed-precision contraction). */
+int dns_debug_mfma_probe_f32(double* out);
 /* max relative reconstruction error of the Helmholtz and Poisson eigen bases. */
 int dns_debug_operator_error(dns_ctx* ctx, double* out2);
 

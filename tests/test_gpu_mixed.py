@@ -1,0 +1,74 @@
+"""Mixed precision: FP32 plane transforms, rotational product and omega/S storage, FP64 solves,
+walls, divergence and bulk integral.  Error against the FP64 oracle is stated per field."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests.gpu_util import CONFIG1, field_errors, max_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dns():
+    import f90_dns_amd
+    f90_dns_amd.lib()
+    return f90_dns_amd
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 32), (128, 128), (192, 192), (96, 64)])
+def test_fp32_plane_transforms(dns, nx, ny):
+    rng = np.random.default_rng(nx * 7 + ny)
+    f = rng.standard_normal((3, ny, nx))
+    plan = dns.FFTPlan(nx, ny, precision=dns.DNS_PREC_MIXED)
+    X = plan.forward(f)
+    ref = np.fft.rfft2(f)
+    err = np.max(np.abs(X - ref)) / np.abs(ref).max()
+    back = plan.backward(ref)
+    err_b = np.max(np.abs(back - f)) / np.abs(f).max()
+    _say(f"fp32 fft {nx}x{ny}: fwd {err:.2e} bwd {err_b:.2e}")
+    assert err < 2e-6 and err_b < 2e-6
+
+
+def test_mixed_config1_500_steps_within_1e6_of_fp64_oracle(dns):
+    """North-star bar for the mixed path: each field within 1e-6 (relative L2; w on the velocity
+    scale) of the FP64 oracle after 500 steps."""
+    o = orc.Oracle(**CONFIG1)
+    o.init_fields()
+    g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **CONFIG1)
+    g.set_fields(*o.get_fields())
+    g64 = dns.Simulation(**CONFIG1)
+    g64.set_fields(*o.get_fields())
+    o.step(500)
+    g.step(500)
+    g64.step(500)
+    ref = o.get_fields()
+    e_mixed = field_errors(g.get_fields(), ref)
+    e_64 = field_errors(g64.get_fields(), ref)
+    _say(f"mixed vs FP64 oracle after 500 steps: {e_mixed}")
+    _say(f"FP64 GPU vs FP64 oracle after 500 steps: {e_64}")
+    assert max_err(e_mixed) < 1e-6, e_mixed
+    assert max_err(e_64) < 1e-10, e_64
+
+
+def test_mixed_production_shape_short(dns):
+    kw = dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+              use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,
+              target_bulk_velocity=2.0 / 3.0, u_centerline=1.0, enable_perturbations=1, perturbation_kind=1,
+              seed=2024, noise_kmax=8, perturbation_amplitude=0.05, lgl_derivative=1)
+    o = orc.Oracle(**kw)
+    o.init_fields()
+    g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **kw)
+    g.set_fields(*o.get_fields())
+    o.step(3)
+    g.step(3)
+    e = field_errors(g.get_fields(), o.get_fields())
+    _say(f"mixed 128^2x129 after 3 steps: {e}")
+    assert max_err(e) < 1e-6, e

@@ -63,6 +63,18 @@ def test_mfma_f64_lane_layout(dns):
     np.testing.assert_array_equal(P, want)
 
 
+def test_mfma_f32_lane_layout_probe(dns):
+    """Records the v_mfma_f32_16x16x4f32 lane layout (used by the mixed-precision contraction)."""
+    import ctypes
+    import os
+    out = np.empty(4096 * 256)
+    assert dns.lib().dns_debug_mfma_probe_f32(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    P = out.reshape(64, 64, 64, 4)
+    os.makedirs("gpurun_out", exist_ok=True)
+    np.save("gpurun_out/mfma_probe_f32.npy", P)
+    assert np.count_nonzero(P) == 1024 and set(np.unique(P)) == {0.0, 1.0}
+
+
 @pytest.mark.parametrize("nz,exact", [(17, 0), (33, 0), (65, 1), (129, 1), (129, 0), (9, 0), (48, 1)])
 def test_zderiv_mfma_matches_numpy(dns, nz, exact):
     g = dns.Simulation(nx=32, ny=16, nz=nz, lgl_derivative=exact)
