@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant-steps", type=int, default=200,
+                    help="steps for the mixed-precision line reported beside the FP64 value (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the ensemble/serving "
@@ -132,6 +134,26 @@ def main():
         roof["kernels_ms"] = prof
     print("[bench] kernel ms: " + json.dumps(prof), file=sys.stderr, flush=True)
 
+    variants = {}
+    if args.variant_steps > 0:
+        sim.close()
+        m = dns.Simulation(precision=dns.DNS_PREC_MIXED, **kw)
+        m.init_fields()
+        m.step(args.warmup, graph=graph)
+        m.synchronize()
+        t0 = time.perf_counter()
+        m.step(args.variant_steps, graph=graph)
+        m.synchronize()
+        mw = time.perf_counter() - t0
+        ml = m.step_timing()
+        variants["mixed_precision"] = {
+            "value": npts * args.variant_steps / mw, "unit": "grid-point-steps/s", "steps": args.variant_steps,
+            "ms_per_step": mw / args.variant_steps * 1e3, "speedup_vs_fp64": (npts * args.variant_steps / mw) / value,
+            "step_latency_ms": {"p50": ml["p50_ms"], "p99": ml["p99_ms"]},
+            "kernels_ms": m.profile_step(args.profile_steps) if args.profile_steps > 0 else {},
+            "precision": "FP32 transforms, product, D1 output and S storage; FP64 solves"}
+        m.close()
+
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:
         print(f"[bench] cpu baseline: {args.cpu_steps} oracle step(s)", file=sys.stderr, flush=True)
@@ -155,6 +177,7 @@ def main():
         "step_latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "min": lat["min_ms"], "mean": lat["mean_ms"]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "variants": variants,
     }
     print(json.dumps(out))
 

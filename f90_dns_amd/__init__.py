@@ -110,6 +110,18 @@ def lib():
         L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_profile_kernel_name.argtypes = [ctypes.c_int]
         L.dns_profile_kernel_name.restype = ctypes.c_char_p
+        L.dns_create_batch.argtypes = [ctypes.POINTER(DnsParams), ctypes.c_int, ctypes.POINTER(_vp)]
+        L.dns_members.argtypes = [_vp]
+        L.dns_device_bytes.argtypes = [_vp]
+        L.dns_device_bytes.restype = ctypes.c_size_t
+        for n in ("dns_set_member_fields", "dns_get_member_fields"):
+            getattr(L, n).argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
+        L.dns_set_member_controller.argtypes = [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.dns_member_diagnostics.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(DnsDiag)]
+        L.dns_member_divergence.argtypes = [_vp, ctypes.c_int, _dp]
+        L.dns_enqueue_step.argtypes = [_vp, ctypes.c_int]
+        L.dns_query.argtypes = [_vp]
         _lib = L
     return _lib
 
@@ -134,18 +146,34 @@ def _arr(a, shape):
 
 
 class Simulation:
-    """One resident simulation (a dns_ctx).  Physical fields are numpy (nz, ny, nx) arrays
-    in the reference's Fortran order f(i,j,k)."""
+    """One resident context (a dns_ctx).  Physical fields are numpy (nz, ny, nx) arrays in the
+    reference's Fortran order f(i,j,k).
 
-    def __init__(self, **kw):
+    members: optional list of per-member parameter overrides (dicts) on top of **kw; the
+    context then advances len(members) flows of one shape together (dns_create_batch).  The
+    field, controller and diagnostics methods take the member index (default 0)."""
+
+    def __init__(self, members=None, **kw):
         self.params = make_params(**kw)
+        if members is None:
+            plist = [self.params]
+        else:
+            plist = [make_params(**{**kw, **m}) for m in members]
+            self.params = plist[0]
+        self.member_params = plist
+        self.members = len(plist)
         self.nx, self.ny, self.nz = self.params.nx, self.params.ny, self.params.nz
         h = _vp()
         L = lib()
-        rc = L.dns_create(ctypes.byref(self.params), ctypes.byref(h))
+        arr = (DnsParams * len(plist))(*plist)
+        rc = L.dns_create_batch(arr, len(plist), ctypes.byref(h))
         if rc != 0:
             raise DnsError(rc, L.dns_last_error(None).decode())
         self._h = h
+
+    @property
+    def device_bytes(self):
+        return lib().dns_device_bytes(self._h)
 
     def _check(self, rc):
         if rc != 0:
@@ -173,17 +201,18 @@ class Simulation:
     def init_fields(self):
         self._check(lib().dns_init_fields(self._h))
 
-    def set_fields(self, u=None, v=None, w=None, p=None):
+    def set_fields(self, u=None, v=None, w=None, p=None, member=0):
         arrs = [None if a is None else _arr(a, self.shape) for a in (u, v, w, p)]
-        self._check(lib().dns_set_fields(self._h, *[None if a is None else a.ctypes.data for a in arrs]))
+        self._check(lib().dns_set_member_fields(self._h, member,
+                                                *[None if a is None else a.ctypes.data for a in arrs]))
 
-    def get_fields(self):
+    def get_fields(self, member=0):
         out = [np.empty(self.shape) for _ in range(4)]
-        self._check(lib().dns_get_fields(self._h, *[a.ctypes.data for a in out]))
+        self._check(lib().dns_get_member_fields(self._h, member, *[a.ctypes.data for a in out]))
         return out
 
-    def set_controller(self, G, integral=0.0, prev=0.0, gsum=0.0, gcount=0):
-        self._check(lib().dns_set_controller(self._h, G, integral, prev, gsum, gcount))
+    def set_controller(self, G, integral=0.0, prev=0.0, gsum=0.0, gcount=0, member=0):
+        self._check(lib().dns_set_member_controller(self._h, member, G, integral, prev, gsum, gcount))
 
     # the stage subroutines, DNS_pressure_BC_3D.f90:314-326
     def begin_step(self):
@@ -216,14 +245,24 @@ class Simulation:
         self._check(lib().dns_step_timing(self._h, out))
         return dict(min_ms=out[0], p50_ms=out[1], p99_ms=out[2], mean_ms=out[3])
 
-    def diagnostics(self):
+    def enqueue_step(self, graph=True):
+        """One step enqueued on the context stream; returns without waiting."""
+        self._check(lib().dns_enqueue_step(self._h, 1 if graph else 0))
+
+    def busy(self):
+        rc = lib().dns_query(self._h)
+        if rc not in (0, 1):
+            self._check(rc)
+        return rc == 1
+
+    def diagnostics(self, member=0):
         d = DnsDiag()
-        self._check(lib().dns_diagnostics(self._h, ctypes.byref(d)))
+        self._check(lib().dns_member_diagnostics(self._h, member, ctypes.byref(d)))
         return {f: getattr(d, f) for f, _ in DnsDiag._fields_}
 
-    def max_divergence(self):
+    def max_divergence(self, member=0):
         v = ctypes.c_double()
-        self._check(lib().dns_check_divergence(self._h, ctypes.byref(v)))
+        self._check(lib().dns_member_divergence(self._h, member, ctypes.byref(v)))
         return v.value
 
     def spectral(self, which):

@@ -7,6 +7,11 @@
 // Hermitian spectrum, and every operator of the step maps Hermitian spectra to Hermitian
 // spectra except the i*k derivatives, whose c2r/r2c round trips the kernels apply as
 // effective wavenumbers (x Nyquist column -> 0; y Nyquist row of the kx = 0, nx/2 columns -> 0).
+//
+// A context holds B >= 1 members of one shape (nx, ny, nz, dt, Lx, Ly, scheme flags); each
+// member has its own Re, flow-control targets and initial field.  Members are stored one after
+// another in every field array; the plane kernels see them as B*nz planes and the z-kernels
+// take the member from blockIdx.z, so one launch per stage advances every member.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,44 +29,49 @@
 using namespace dns;
 
 namespace {
-
 thread_local std::string g_create_error;
+}
 
-}  // namespace
+void dns_set_create_error(const std::string& msg) { g_create_error = msg; }
 
 struct dns_ctx {
-    dns_params p;
+    dns_params p;                 // member 0 (the shape fields are equal for every member)
+    std::vector<dns_params> mp;   // every member
+    int B = 1;
     int nx, ny, nz, nxh, H2, npl;
-    size_t N, NS;
+    size_t N, NS;                 // per member: physical doubles, spectral doubles
     double theta, ybar;
     int rb, rbi;
-    bool mixed = false;          // FP32 transforms, product and omega/S storage; FP64 solves
+    bool mixed = false;           // FP32 transforms, product and omega/S storage; FP64 solves
     hipStream_t stream = nullptr;
     FFTGeom g;
-    Operators op;
+    Operators op;                 // member 0; the Re-independent parts are shared
+    size_t hf_stride = 0;         // doubles per member of the stacked Helmholtz forward operator
     // constants
     float *twx32 = nullptr, *twy32 = nullptr;
     double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *e0 = nullptr, *f0 = nullptr;
     unsigned char* mask = nullptr;
-    double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr;
+    double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr, *fvec = nullptr;
     double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
-    // fields
+    // fields, B members each
     double *uh = nullptr, *vh = nullptr, *wh = nullptr, *ph = nullptr;
     double *us = nullptr, *vs = nullptr, *ws = nullptr;
     double *ox = nullptr, *oy = nullptr, *sh[3] = {nullptr, nullptr, nullptr};
     double *phys[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double *phi = nullptr, *dphi = nullptr, *tmp = nullptr;
     FlowState* flow = nullptr;
+    FlowMember* fmem = nullptr;
     double *red_partial = nullptr, *red_result = nullptr;
     int red_blocks = 512;
     std::vector<void*> allocs;
+    size_t bytes = 0;
     double time = 0.0;
     int step = 0;
     std::string err;
     // timing
     std::vector<hipEvent_t> ev;
     std::vector<double> step_ms;
-    // graph
+    // graph of one step
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
 };
@@ -80,9 +90,11 @@ static int fail(dns_ctx* c, int code, const std::string& msg) {
 template <class T>
 static int dalloc(dns_ctx* c, T** ptr, size_t n) {
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    const size_t nb = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, nb);
     if (e != hipSuccess) return fail(c, DNS_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     c->allocs.push_back(q);
+    c->bytes += nb;
     *ptr = static_cast<T*>(q);
     return 0;
 }
@@ -94,7 +106,7 @@ static int upload(dns_ctx* c, T** ptr, const std::vector<T>& h) {
     return 0;
 }
 
-static bool check_gfx950(std::string& why) {
+bool dns_check_gfx950(std::string& why) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) { why = "no HIP device"; return false; }
     hipDeviceProp_t prop;
@@ -106,7 +118,7 @@ static bool check_gfx950(std::string& why) {
     return true;
 }
 
-static bool make_factors(int n, Factors& f) {
+bool dns_make_factors(int n, Factors& f) {
     std::vector<int> v;
     if (!radix_factors(n, v) || v.size() > 14) return false;
     f.n = n; f.nf = int(v.size());
@@ -126,30 +138,55 @@ static int destroy_impl(dns_ctx* c) {
     return 0;
 }
 
+static std::string check_params(const dns_params* p) {
+    if (p->nx < 4 || p->ny < 4 || (p->nx & 1) || (p->ny & 1)) return "nx and ny must be even and >= 4";
+    if (p->nz < 5 || p->nz > 144) return "nz must be in [5, 144]";
+    if (!(p->dt > 0) || !(p->re > 0) || !(p->xlen > 0) || !(p->ylen > 0)) return "dt, re, xlen, ylen must be positive";
+    if (p->precision != DNS_PREC_FP64 && p->precision != DNS_PREC_MIXED)
+        return "precision must be DNS_PREC_FP64 or DNS_PREC_MIXED";
+    if (p->flow_control_method != 1 && p->flow_control_method != 2) return "flow_control_method must be 1 or 2";
+    if (p->flow_control_method == 2 &&
+        (p->target_bulk_velocity <= 0 || p->controller_gain <= 0 || p->controller_update_freq <= 0))
+        return "method 2 needs positive target_bulk_velocity, controller_gain, controller_update_freq";
+    return "";
+}
+
+// members of one context share the grid, the step, the domain and the scheme flags
+static bool same_shape(const dns_params& a, const dns_params& b) {
+    return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dt == b.dt && a.xlen == b.xlen && a.ylen == b.ylen &&
+           a.use_crank_nicolson == b.use_crank_nicolson && a.use_dealias_23_rule == b.use_dealias_23_rule &&
+           a.use_weak_form_divergence == b.use_weak_form_divergence && a.precision == b.precision &&
+           a.lgl_derivative == b.lgl_derivative;
+}
+
 extern "C" {
 
-const char* dns_version(void) { return "f90_dns_amd 0.1 (gfx950, FP64 spectral-resident step)"; }
+const char* dns_version(void) {
+    return "f90_dns_amd 0.2 (gfx950, spectral-resident step, FP64 and mixed precision, batched members)";
+}
 
 const char* dns_last_error(const dns_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
 
-int dns_create(const dns_params* p, dns_ctx** out) {
-    if (!p || !out) return fail(nullptr, DNS_EINVAL, "null argument");
+int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
+    if (!params || !out || nmembers < 1) return fail(nullptr, DNS_EINVAL, "null argument or no members");
     *out = nullptr;
-    if (p->nx < 4 || p->ny < 4 || (p->nx & 1) || (p->ny & 1)) return fail(nullptr, DNS_EINVAL, "nx and ny must be even and >= 4");
-    if (p->nz < 5 || p->nz > 144) return fail(nullptr, DNS_EINVAL, "nz must be in [5, 144]");
-    if (!(p->dt > 0) || !(p->re > 0) || !(p->xlen > 0) || !(p->ylen > 0)) return fail(nullptr, DNS_EINVAL, "dt, re, xlen, ylen must be positive");
-    if (p->use_weak_form_divergence) return fail(nullptr, DNS_EUNSUPPORTED, "weak-form pressure RHS is not implemented");
-    if (p->precision != DNS_PREC_FP64 && p->precision != DNS_PREC_MIXED)
-        return fail(nullptr, DNS_EINVAL, "precision must be DNS_PREC_FP64 or DNS_PREC_MIXED");
-    if (p->flow_control_method != 1 && p->flow_control_method != 2)
-        return fail(nullptr, DNS_EINVAL, "flow_control_method must be 1 or 2");
-    if (p->flow_control_method == 2 && (p->target_bulk_velocity <= 0 || p->controller_gain <= 0 || p->controller_update_freq <= 0))
-        return fail(nullptr, DNS_EINVAL, "method 2 needs positive target_bulk_velocity, controller_gain, controller_update_freq");
+    for (int b = 0; b < nmembers; b++) {
+        const std::string e = check_params(&params[b]);
+        if (!e.empty()) return fail(nullptr, DNS_EINVAL, nmembers > 1 ? "member " + std::to_string(b) + ": " + e : e);
+        if (params[b].use_weak_form_divergence)
+            return fail(nullptr, DNS_EUNSUPPORTED, "weak-form pressure RHS is not implemented");
+        if (!same_shape(params[0], params[b]))
+            return fail(nullptr, DNS_EINVAL,
+                        "member " + std::to_string(b) + " differs from member 0 in grid, dt, domain or scheme flags");
+    }
     std::string why;
-    if (!check_gfx950(why)) return fail(nullptr, DNS_EDEVICE, why);
+    if (!dns_check_gfx950(why)) return fail(nullptr, DNS_EDEVICE, why);
+    const dns_params* p = &params[0];
 
     dns_ctx* c = new dns_ctx();
     c->p = *p;
+    c->mp.assign(params, params + nmembers);
+    c->B = nmembers;
     c->nx = p->nx; c->ny = p->ny; c->nz = p->nz; c->nxh = p->nx / 2 + 1;
     c->H2 = 2 * c->nxh * c->ny; c->npl = c->nx * c->ny;
     c->N = size_t(c->npl) * c->nz; c->NS = size_t(c->H2) * c->nz;
@@ -158,14 +195,19 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     c->rb = (c->nz + 15) / 16;
     c->rbi = (c->nz - 2 + 15) / 16;
     c->mixed = p->precision == DNS_PREC_MIXED;
-    auto bail = [&](int code) { int r = code; destroy_impl(c); return r; };
+    g_create_error.clear();
+    auto bail = [&](int code) {
+        if (g_create_error.empty()) g_create_error = c->err;
+        destroy_impl(c);
+        return code;
+    };
 
     if (!fft_geom(c->nx, c->ny, c->g) || (!c->mixed && c->g.cls64 < 0)) {
         g_create_error = "plane too large for the LDS-resident transform: (nx/2+1)*ny must be <= 8320 in FP64, "
                          "<= 18624 in mixed precision";
         return bail(DNS_EUNSUPPORTED);
     }
-    if (!make_factors(c->ny, c->g.fy) || !make_factors(c->nx / 2, c->g.fx)) {
+    if (!dns_make_factors(c->ny, c->g.fy) || !dns_make_factors(c->nx / 2, c->g.fx)) {
         g_create_error = "nx/2 and ny must be of the form 2^a 3^b";
         return bail(DNS_EINVAL);
     }
@@ -173,10 +215,22 @@ int dns_create(const dns_params* p, dns_ctx** out) {
         g_create_error = "hipFuncSetAttribute failed for the plane kernels";
         return bail(DNS_EDEVICE);
     }
+    // operators: the Helmholtz parts depend on Re and are stacked per member
     std::string err;
-    if (!build_operators(c->nz, p->dt, p->re, c->theta, p->use_crank_nicolson != 0, p->lgl_derivative != 0, c->op, err)) {
-        g_create_error = err;
-        return bail(DNS_ESOLVER);
+    std::vector<double> hF, c1, c2, hl, fv;
+    for (int b = 0; b < c->B; b++) {
+        Operators ob;
+        if (!build_operators(c->nz, p->dt, params[b].re, c->theta, p->use_crank_nicolson != 0,
+                             p->lgl_derivative != 0, ob, err)) {
+            g_create_error = err;
+            return bail(DNS_ESOLVER);
+        }
+        if (b == 0) { c->op = ob; c->hf_stride = ob.hF.data.size(); }
+        hF.insert(hF.end(), ob.hF.data.begin(), ob.hF.data.end());
+        c1.insert(c1.end(), ob.hc1.begin(), ob.hc1.end());
+        c2.insert(c2.end(), ob.hc2.begin(), ob.hc2.end());
+        hl.insert(hl.end(), ob.hl.begin(), ob.hl.end());
+        fv.push_back(ob.f);
     }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
@@ -187,10 +241,10 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     std::vector<unsigned char> mask;
     twiddles(c->nx, tw);
     std::vector<float> tw32(tw.begin(), tw.end());
-    if ((r = upload(c, &c->twx, tw)) || (r = upload(c, &c->twx32, tw32))) { g_create_error = c->err; return bail(r); }
+    if ((r = upload(c, &c->twx, tw)) || (r = upload(c, &c->twx32, tw32))) return bail(r);
     twiddles(c->ny, tw);
     tw32.assign(tw.begin(), tw.end());
-    if ((r = upload(c, &c->twy, tw)) || (r = upload(c, &c->twy32, tw32))) { g_create_error = c->err; return bail(r); }
+    if ((r = upload(c, &c->twy, tw)) || (r = upload(c, &c->twy32, tw32))) return bail(r);
     c->g.twx = reinterpret_cast<const double2*>(c->twx);
     c->g.twy = reinterpret_cast<const double2*>(c->twy);
     c->g.twx32 = reinterpret_cast<const float2*>(c->twx32);
@@ -200,29 +254,34 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     const Operators& o = c->op;
     if ((r = upload(c, &c->xw, xw)) || (r = upload(c, &c->yw, yw)) || (r = upload(c, &c->mask, mask)) ||
         (r = upload(c, &c->wz, o.wz)) || (r = upload(c, &c->e0, o.e0)) || (r = upload(c, &c->f0, o.f0)) ||
-        (r = upload(c, &c->D1sw, o.D1.data)) || (r = upload(c, &c->hF, o.hF.data)) || (r = upload(c, &c->hB, o.hB.data)) ||
-        (r = upload(c, &c->hc1, o.hc1)) || (r = upload(c, &c->hc2, o.hc2)) || (r = upload(c, &c->hl, o.hl)) ||
-        (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) || (r = upload(c, &c->pDB, o.pDB.data)) ||
-        (r = upload(c, &c->pl, o.pl))) {
-        g_create_error = c->err;
+        (r = upload(c, &c->D1sw, o.D1.data)) || (r = upload(c, &c->hF, hF)) || (r = upload(c, &c->hB, o.hB.data)) ||
+        (r = upload(c, &c->hc1, c1)) || (r = upload(c, &c->hc2, c2)) || (r = upload(c, &c->hl, hl)) ||
+        (r = upload(c, &c->fvec, fv)) || (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) ||
+        (r = upload(c, &c->pDB, o.pDB.data)) || (r = upload(c, &c->pl, o.pl)))
         return bail(r);
-    }
+    const size_t NSB = c->NS * c->B, NB = c->N * c->B;
     double** specs[] = {&c->uh, &c->vh, &c->wh, &c->ph, &c->us, &c->vs, &c->ws, &c->ox, &c->oy,
                         &c->sh[0], &c->sh[1], &c->sh[2], &c->phi, &c->dphi, &c->tmp};
     for (double** q : specs)
-        if ((r = dalloc(c, q, c->NS))) { g_create_error = c->err; return bail(r); }
+        if ((r = dalloc(c, q, NSB))) return bail(r);
     for (int f = 0; f < 6; f++)
-        if ((r = dalloc(c, &c->phys[f], c->N))) { g_create_error = c->err; return bail(r); }
-    if ((r = dalloc(c, &c->flow, 1)) || (r = dalloc(c, &c->red_partial, size_t(c->red_blocks) * 5)) ||
-        (r = dalloc(c, &c->red_result, 5))) {
-        g_create_error = c->err;
+        if ((r = dalloc(c, &c->phys[f], NB))) return bail(r);
+    if ((r = dalloc(c, &c->flow, c->B)) || (r = dalloc(c, &c->fmem, c->B)) ||
+        (r = dalloc(c, &c->red_partial, size_t(c->red_blocks) * 5)) || (r = dalloc(c, &c->red_result, 5)))
         return bail(r);
+    for (double** q : specs) (void)hipMemset(*q, 0, NSB * sizeof(double));
+    std::vector<FlowState> fs(c->B);
+    std::vector<FlowMember> fm(c->B);
+    for (int b = 0; b < c->B; b++) {
+        const dns_params& q = params[b];
+        fs[b] = FlowState{};
+        fs[b].G = q.target_pressure_gradient;
+        fs[b].gavg = q.target_pressure_gradient;
+        fm[b] = FlowMember{q.target_pressure_gradient, q.target_bulk_velocity, q.controller_gain, q.re,
+                           q.flow_control_method, q.controller_update_freq, q.istart, 0};
     }
-    for (double** q : specs) (void)hipMemset(*q, 0, c->NS * sizeof(double));
-    FlowState fs{};
-    fs.G = p->target_pressure_gradient;
-    fs.gavg = p->target_pressure_gradient;
-    if (hipMemcpy(c->flow, &fs, sizeof(fs), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(c->flow, fs.data(), sizeof(FlowState) * c->B, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->fmem, fm.data(), sizeof(FlowMember) * c->B, hipMemcpyHostToDevice) != hipSuccess) {
         g_create_error = "upload of the flow-control state failed";
         return bail(DNS_EDEVICE);
     }
@@ -230,9 +289,15 @@ int dns_create(const dns_params* p, dns_ctx** out) {
     return 0;
 }
 
+int dns_create(const dns_params* p, dns_ctx** out) { return dns_create_batch(p, 1, out); }
+
 int dns_destroy(dns_ctx* c) { return destroy_impl(c); }
 
 void* dns_stream(dns_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int dns_members(const dns_ctx* c) { return c ? c->B : 0; }
+
+size_t dns_device_bytes(const dns_ctx* c) { return c ? c->bytes : 0; }
 
 static FFTArgs fft_args(dns_ctx* c) {
     FFTArgs a;
@@ -241,20 +306,58 @@ static FFTArgs fft_args(dns_ctx* c) {
     return a;
 }
 
-// forward transform of physical fields (device pointers, nf <= 4) into spectral fields
-static int r2c_fields(dns_ctx* c, const double* const* src, double* const* dst, int nf) {
+// transforms of nplanes consecutive planes (device pointers, FP64 storage)
+static int r2c_planes(dns_ctx* c, const double* src, double* dst, int nplanes) {
     FFTArgs a = fft_args(c);
-    for (int f = 0; f < nf; f++) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
+    a.src[0] = src; a.dst[0] = dst;
     a.sstride = c->npl; a.dstride = c->H2;
-    HIPCHECK(c, launch_r2c(c->g, R2C_PLAIN, a, c->nz, nf, false, c->stream));
+    HIPCHECK(c, launch_r2c(c->g, R2C_PLAIN, a, nplanes, 1, false, c->stream));
     return 0;
 }
-static int c2r_fields(dns_ctx* c, const double* const* src, double* const* dst, int nf) {
+static int c2r_planes(dns_ctx* c, const double* src, double* dst, int nplanes) {
     FFTArgs a = fft_args(c);
-    for (int f = 0; f < nf; f++) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
+    a.src[0] = src; a.dst[0] = dst;
     a.sstride = c->H2; a.dstride = c->npl;
-    HIPCHECK(c, launch_c2r(c->g, C2R_PLAIN, a, c->nz, nf, false, c->stream));
+    HIPCHECK(c, launch_c2r(c->g, C2R_PLAIN, a, nplanes, 1, false, c->stream));
     return 0;
+}
+
+static bool bad_member(const dns_ctx* c, int b) { return !c || b < 0 || b >= c->B; }
+
+int dns_set_member_fields(dns_ctx* c, int b, const double* u, const double* v, const double* w, const double* p) {
+    if (bad_member(c, b)) return DNS_EINVAL;
+    const double* in[4] = {u, v, w, p};
+    double* outs[4] = {c->uh, c->vh, c->wh, c->ph};
+    for (int f = 0; f < 4; f++) {
+        if (!in[f]) continue;
+        HIPCHECK(c, hipMemcpyAsync(c->phys[0], in[f], c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        int r = r2c_planes(c, c->phys[0], outs[f] + size_t(b) * c->NS, c->nz);
+        if (r) return r;
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_get_member_fields(dns_ctx* c, int b, double* u, double* v, double* w, double* p) {
+    if (bad_member(c, b)) return DNS_EINVAL;
+    double* outs[4] = {u, v, w, p};
+    const double* in[4] = {c->uh, c->vh, c->wh, c->ph};
+    for (int f = 0; f < 4; f++) {
+        if (!outs[f]) continue;
+        int r = c2r_planes(c, in[f] + size_t(b) * c->NS, c->phys[f], c->nz);
+        if (r) return r;
+        HIPCHECK(c, hipMemcpyAsync(outs[f], c->phys[f], c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dns_set_fields(dns_ctx* c, const double* u, const double* v, const double* w, const double* p) {
+    return dns_set_member_fields(c, 0, u, v, w, p);
+}
+
+int dns_get_fields(dns_ctx* c, double* u, double* v, double* w, double* p) {
+    return dns_get_member_fields(c, 0, u, v, w, p);
 }
 
 int dns_set_fields_device(dns_ctx* c, const double* u, const double* v, const double* w, const double* p) {
@@ -263,22 +366,7 @@ int dns_set_fields_device(dns_ctx* c, const double* u, const double* v, const do
     double* outs[4] = {c->uh, c->vh, c->wh, c->ph};
     for (int f = 0; f < 4; f++) {
         if (!in[f]) continue;
-        int r = r2c_fields(c, &in[f], &outs[f], 1);
-        if (r) return r;
-    }
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    return 0;
-}
-
-int dns_set_fields(dns_ctx* c, const double* u, const double* v, const double* w, const double* p) {
-    if (!c) return DNS_EINVAL;
-    const double* in[4] = {u, v, w, p};
-    double* outs[4] = {c->uh, c->vh, c->wh, c->ph};
-    for (int f = 0; f < 4; f++) {
-        if (!in[f]) continue;
-        HIPCHECK(c, hipMemcpyAsync(c->phys[0], in[f], c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        const double* s = c->phys[0];
-        int r = r2c_fields(c, &s, &outs[f], 1);
+        int r = r2c_planes(c, in[f], outs[f], c->nz);
         if (r) return r;
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -291,46 +379,42 @@ int dns_get_fields_device(dns_ctx* c, double* u, double* v, double* w, double* p
     const double* in[4] = {c->uh, c->vh, c->wh, c->ph};
     for (int f = 0; f < 4; f++) {
         if (!outs[f]) continue;
-        int r = c2r_fields(c, &in[f], &outs[f], 1);
+        int r = c2r_planes(c, in[f], outs[f], c->nz);
         if (r) return r;
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
-int dns_get_fields(dns_ctx* c, double* u, double* v, double* w, double* p) {
+int dns_init_fields(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    double* outs[4] = {u, v, w, p};
-    const double* in[4] = {c->uh, c->vh, c->wh, c->ph};
-    int r = c2r_fields(c, in, c->phys, 4);
-    if (r) return r;
-    for (int f = 0; f < 4; f++)
-        if (outs[f]) HIPCHECK(c, hipMemcpyAsync(outs[f], c->phys[f], c->N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    std::vector<double> zero(c->N, 0.0);
+    for (int b = 0; b < c->B; b++) {
+        const dns_params& q = c->mp[b];
+        InitSpec s{c->nx, c->ny, c->nz, q.xlen, q.ylen, q.u_centerline, q.enable_perturbations,
+                   q.perturbation_kind, q.analytical_mode_x, q.analytical_mode_y, q.perturbation_amplitude,
+                   (uint64_t)q.seed, q.noise_kmax};
+        std::vector<double> u, v, w;
+        initial_field(s, u, v, w);
+        int r = dns_set_member_fields(c, b, u.data(), v.data(), w.data(), zero.data());
+        if (r) return r;
+    }
+    c->time = 0.0; c->step = 0;
     return 0;
 }
 
-int dns_init_fields(dns_ctx* c) {
-    if (!c) return DNS_EINVAL;
-    InitSpec s{c->nx, c->ny, c->nz, c->p.xlen, c->p.ylen, c->p.u_centerline, c->p.enable_perturbations,
-               c->p.perturbation_kind, c->p.analytical_mode_x, c->p.analytical_mode_y, c->p.perturbation_amplitude,
-               (uint64_t)c->p.seed, c->p.noise_kmax};
-    std::vector<double> u, v, w;
-    initial_field(s, u, v, w);
-    std::vector<double> zero(c->N, 0.0);
-    int r = dns_set_fields(c, u.data(), v.data(), w.data(), zero.data());
-    c->time = 0.0; c->step = 0;
-    return r;
-}
-
-int dns_set_controller(dns_ctx* c, double G, double integral, double prev, double gsum, int gcount) {
-    if (!c) return DNS_EINVAL;
+int dns_set_member_controller(dns_ctx* c, int b, double G, double integral, double prev, double gsum, int gcount) {
+    if (bad_member(c, b)) return DNS_EINVAL;
     FlowState fs{};
     fs.G = G; fs.err_int = integral; fs.err_prev = prev; fs.gsum = gsum; fs.gcount = gcount;
     fs.gavg = gcount > 0 ? gsum / double(gcount) : G;
-    HIPCHECK(c, hipMemcpyAsync(c->flow, &fs, sizeof(fs), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->flow + b, &fs, sizeof(fs), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+int dns_set_controller(dns_ctx* c, double G, double integral, double prev, double gsum, int gcount) {
+    return dns_set_member_controller(c, 0, G, integral, prev, gsum, gcount);
 }
 
 // ---- stages -----------------------------------------------------------------------------
@@ -343,25 +427,25 @@ int dns_begin_step(dns_ctx* c) {
 
 static FlowArgs flow_args(dns_ctx* c) {
     FlowArgs fa{};
-    fa.nz = c->nz; fa.method = c->p.flow_control_method; fa.freq = c->p.controller_update_freq; fa.istart = c->p.istart;
-    fa.target_G = c->p.target_pressure_gradient; fa.target_Ub = c->p.target_bulk_velocity; fa.gain = c->p.controller_gain;
-    fa.re = c->p.re; fa.dt = c->p.dt; fa.nxny = double(c->nx * c->ny);
+    fa.nz = c->nz; fa.dt = c->p.dt; fa.nxny = double(c->nx * c->ny);
     double ws = 0.0;
     for (int k = 0; k < c->nz; k++) ws += c->op.wz[k];
     fa.wsum = ws;
-    fa.uh = c->uh; fa.H2 = c->H2; fa.wz = c->wz; fa.st = c->flow;
+    fa.uh = c->uh; fa.H2 = c->H2; fa.mstride = (long long)c->NS; fa.wz = c->wz;
+    fa.mem = c->fmem; fa.st = c->flow;
     return fa;
 }
 
 static int nl_zderiv(dns_ctx* c) {
     ZArgs z{};
+    z.mstride = (long long)c->NS;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
     z.Asw = c->D1sw; z.KS = c->op.D1.ks;
     z.in[0] = c->uh; z.in[1] = c->vh; z.in[2] = c->wh;
     z.out[0] = c->ox; z.out[1] = c->oy;
     z.xw = c->xw; z.yw = c->yw;
     z.out_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->stream));
+    HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->B, c->stream));
     return 0;
 }
 
@@ -372,7 +456,7 @@ static int nl_c2r(dns_ctx* c) {
     a.sstride = c->H2; a.dstride = c->npl;
     a.src_f32 = c->mixed ? 0x18u : 0u;
     a.dst_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz, 6, c->mixed, c->stream));
+    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz * c->B, 6, c->mixed, c->stream));
     return 0;
 }
 
@@ -384,40 +468,46 @@ static int nl_r2c(dns_ctx* c) {
     b.sstride = c->npl; b.dstride = c->H2;
     b.src_f32 = c->mixed ? 0x3Fu : 0u;
     b.dst_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz, 3, c->mixed, c->stream));
+    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz * c->B, 3, c->mixed, c->stream));
+    return 0;
+}
+
+static int flow_control(dns_ctx* c) {
+    FlowArgs fa = flow_args(c);
+    HIPCHECK(c, launch_flow_control(fa, c->B, c->stream));
     return 0;
 }
 
 int dns_compute_source_terms(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
     int r;
-    if ((r = nl_zderiv(c)) || (r = nl_c2r(c)) || (r = nl_r2c(c))) return r;
-    FlowArgs fa = flow_args(c);
-    HIPCHECK(c, launch_flow_control(fa, c->stream));
+    if ((r = nl_zderiv(c)) || (r = nl_c2r(c)) || (r = nl_r2c(c)) || (r = flow_control(c))) return r;
     return 0;
 }
 
 int dns_viscous_step(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
     HelmArgs h{};
+    h.mstride = (long long)c->NS; h.fstride = (long long)c->hf_stride;
     h.nz = c->nz; h.m = c->nz - 2; h.H2 = c->H2; h.nxh = c->nxh; h.ny = c->ny; h.nx = c->nx;
     h.Fsw = c->hF; h.FKS = c->op.hF.ks; h.Bsw = c->hB; h.BKS = c->op.hB.ks;
-    h.c1 = c->hc1; h.c2 = c->hc2; h.hl = c->hl;
-    h.f = c->op.f; h.dt = c->p.dt; h.nxny = double(c->nx * c->ny);
+    h.c1 = c->hc1; h.c2 = c->hc2; h.hl = c->hl; h.fvec = c->fvec;
+    h.dt = c->p.dt; h.nxny = double(c->nx * c->ny);
     h.cn = c->p.use_crank_nicolson ? 1 : 0;
     h.xw = c->xw; h.yw = c->yw;
     h.uh[0] = c->uh; h.uh[1] = c->vh; h.uh[2] = c->wh;
     for (int f = 0; f < 3; f++) h.sh[f] = c->sh[f];
     h.out[0] = c->us; h.out[1] = c->vs; h.out[2] = c->ws;
     h.ph = c->ph;
-    h.G = &c->flow->G;
     h.s_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_helmholtz(c->rbi, h, c->stream));
+    h.flow = c->flow;
+    HIPCHECK(c, launch_helmholtz(c->rbi, h, c->B, c->stream));
     return 0;
 }
 
 static PoisArgs poisson_args(dns_ctx* c, int fused) {
     PoisArgs a{};
+    a.mstride = (long long)c->NS;
     a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
     a.Fsw = c->pF; a.FKS = c->op.pF.ks; a.Bsw = c->pB; a.DBsw = c->pDB; a.BKS = c->op.pB.ks;
     a.pl = c->pl; a.dt = c->p.dt; a.fused = fused; a.xw = c->xw; a.yw = c->yw;
@@ -429,6 +519,7 @@ static PoisArgs poisson_args(dns_ctx* c, int fused) {
 
 static Mode0Args mode0_args(dns_ctx* c, int fused) {
     Mode0Args m{};
+    m.mstride = (long long)c->NS;
     m.nz = c->nz; m.H2 = c->H2; m.e0 = c->e0; m.f0 = c->f0; m.dt = c->p.dt; m.fused = fused;
     m.us = c->us; m.vs = c->vs; m.ws = c->ws;
     m.u = c->uh; m.v = c->vh; m.w = c->wh; m.p = c->ph;
@@ -438,9 +529,9 @@ static Mode0Args mode0_args(dns_ctx* c, int fused) {
 
 static int pressure(dns_ctx* c, int fused) {
     PoisArgs a = poisson_args(c, fused);
-    HIPCHECK(c, launch_poisson(c->rb, a, c->stream));
+    HIPCHECK(c, launch_poisson(c->rb, a, c->B, c->stream));
     Mode0Args m = mode0_args(c, fused);
-    HIPCHECK(c, launch_poisson_mode0(m, c->stream));
+    HIPCHECK(c, launch_poisson_mode0(m, c->B, c->stream));
     return 0;
 }
 
@@ -452,7 +543,8 @@ int dns_solve_pressure(dns_ctx* c) {
 int dns_pressure_correction(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
     CorrArgs a{};
-    a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
+    a.nz = c->nz * c->B;   // members are consecutive planes and the update is per element
+    a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
     a.xw = c->xw; a.yw = c->yw;
     a.us = c->us; a.vs = c->vs; a.ws = c->ws; a.phi = c->phi; a.dphi = c->dphi;
     a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph; a.dt = c->p.dt;
@@ -462,7 +554,7 @@ int dns_pressure_correction(dns_ctx* c) {
 
 int dns_apply_boundary_conditions(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    HIPCHECK(c, launch_zero_walls(c->nz, c->H2, c->uh, c->vh, c->wh, c->stream));
+    HIPCHECK(c, launch_zero_walls(c->nz, c->H2, (long long)c->NS, c->B, c->uh, c->vh, c->wh, c->stream));
     return 0;
 }
 
@@ -508,20 +600,27 @@ int dns_step(dns_ctx* c, int nsteps) {
     return 0;
 }
 
+// capture one fused step without executing it
+static int ensure_graph(dns_ctx* c) {
+    if (c->gexec) return 0;
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int r = one_step_fused(c);
+    hipGraph_t gph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &gph);
+    if (r) {
+        if (gph) (void)hipGraphDestroy(gph);
+        return r;
+    }
+    if (e != hipSuccess) return fail(c, DNS_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    c->graph = gph;
+    HIPCHECK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
 int dns_step_graph(dns_ctx* c, int nsteps) {
     if (!c || nsteps < 0) return DNS_EINVAL;
     int r;
-    if (!c->gexec) {
-        // one eager step first would change the state; capture without executing instead
-        HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        r = one_step_fused(c);
-        hipGraph_t gph = nullptr;
-        hipError_t e = hipStreamEndCapture(c->stream, &gph);
-        if (r) return r;
-        if (e != hipSuccess) return fail(c, DNS_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        c->graph = gph;
-        HIPCHECK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-    }
+    if ((r = ensure_graph(c))) return r;
     if ((r = ensure_events(c, nsteps + 1))) return r;
     HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
     for (int s = 0; s < nsteps; s++) {
@@ -534,25 +633,48 @@ int dns_step_graph(dns_ctx* c, int nsteps) {
     return 0;
 }
 
+int dns_enqueue_step(dns_ctx* c, int use_graph) {
+    if (!c) return DNS_EINVAL;
+    int r;
+    if (use_graph && (r = ensure_graph(c))) return r;
+    dns_begin_step(c);
+    if (use_graph) {
+        HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+        return 0;
+    }
+    return one_step_fused(c);
+}
+
 int dns_synchronize(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
+int dns_query(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return fail(c, DNS_EDEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
 int dns_step_timing(dns_ctx* c, double* out4) {
     if (!c || !out4) return DNS_EINVAL;
     if (!c->step_ms.empty() && c->step_ms[0] < 0) {
-        int n = int(-c->step_ms[0]);
-        int r = collect_timing(c, n);
-        if (r) return r;
+        const int n = int(-c->step_ms[0]);
+        if (n == 0) c->step_ms.clear();
+        else {
+            int r = collect_timing(c, n);
+            if (r) return r;
+        }
     }
     if (c->step_ms.empty()) { out4[0] = out4[1] = out4[2] = out4[3] = 0.0; return 0; }
     std::vector<double> t = c->step_ms;
     std::sort(t.begin(), t.end());
     auto pct = [&](double q) {
-        double pos = q * (t.size() - 1);
-        size_t lo = size_t(std::floor(pos)), hi = std::min(t.size() - 1, lo + 1);
+        const double pos = q * (t.size() - 1);
+        const size_t lo = size_t(std::floor(pos)), hi = std::min(t.size() - 1, lo + 1);
         return t[lo] + (pos - lo) * (t[hi] - t[lo]);
     };
     double sum = 0.0;
@@ -572,18 +694,22 @@ static int reduce(dns_ctx* c, const double* u, const double* v, const double* w,
     return 0;
 }
 
-int dns_diagnostics(dns_ctx* c, dns_diag* d) {
-    if (!c || !d) return DNS_EINVAL;
-    const double* in[3] = {c->uh, c->vh, c->wh};
-    int r = c2r_fields(c, in, c->phys, 3);
-    if (r) return r;
+int dns_member_diagnostics(dns_ctx* c, int b, dns_diag* d) {
+    if (bad_member(c, b) || !d) return DNS_EINVAL;
+    const size_t mo = size_t(b) * c->NS;
+    const double* in[3] = {c->uh + mo, c->vh + mo, c->wh + mo};
+    for (int f = 0; f < 3; f++) {
+        int r = c2r_planes(c, in[f], c->phys[f], c->nz);
+        if (r) return r;
+    }
     double res[5];
-    if ((r = reduce(c, c->phys[0], c->phys[1], c->phys[2], nullptr, res))) return r;
+    int r = reduce(c, c->phys[0], c->phys[1], c->phys[2], nullptr, res);
+    if (r) return r;
     std::vector<double> u00(c->nz);
-    HIPCHECK(c, hipMemcpy2DAsync(u00.data(), sizeof(double), c->uh, size_t(c->H2) * sizeof(double), sizeof(double),
-                                 c->nz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpy2DAsync(u00.data(), sizeof(double), c->uh + mo, size_t(c->H2) * sizeof(double),
+                                 sizeof(double), c->nz, hipMemcpyDeviceToHost, c->stream));
     FlowState fs;
-    HIPCHECK(c, hipMemcpyAsync(&fs, c->flow, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(&fs, c->flow + b, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     const double xlen = c->p.xlen, ylen = c->p.ylen, ybar = c->ybar;
     const double fac = (xlen / double(c->nx)) * (ylen / double(c->ny)) * (ybar / 2.0);
@@ -601,15 +727,19 @@ int dns_diagnostics(dns_ctx* c, dns_diag* d) {
     return 0;
 }
 
-int dns_check_divergence(dns_ctx* c, double* max_div) {
-    if (!c || !max_div) return DNS_EINVAL;
+int dns_diagnostics(dns_ctx* c, dns_diag* d) { return dns_member_diagnostics(c, 0, d); }
+
+int dns_member_divergence(dns_ctx* c, int b, double* max_div) {
+    if (bad_member(c, b) || !max_div) return DNS_EINVAL;
+    const size_t mo = size_t(b) * c->NS;
     ZArgs z{};
+    z.mstride = 0;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
     z.Asw = c->D1sw; z.KS = c->op.D1.ks;
-    z.in[0] = c->wh; z.out[0] = c->tmp; z.xw = c->xw; z.yw = c->yw;
-    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, c->stream));
+    z.in[0] = c->wh + mo; z.out[0] = c->tmp; z.xw = c->xw; z.yw = c->yw;
+    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, 1, c->stream));
     FFTArgs a = fft_args(c);
-    a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->tmp; a.dst[0] = c->phys[0];
+    a.src[0] = c->uh + mo; a.src[1] = c->vh + mo; a.src[2] = c->tmp; a.dst[0] = c->phys[0];
     a.sstride = c->H2; a.dstride = c->npl;
     HIPCHECK(c, launch_c2r(c->g, C2R_DIV, a, c->nz, 1, false, c->stream));
     double res[5];
@@ -618,6 +748,8 @@ int dns_check_divergence(dns_ctx* c, double* max_div) {
     *max_div = res[4];
     return 0;
 }
+
+int dns_check_divergence(dns_ctx* c, double* max_div) { return dns_member_divergence(c, 0, max_div); }
 
 // ---- debug hooks (include/f90dns_debug.h) -------------------------------------------------
 int dns_debug_spectral(dns_ctx* c, int which, double* out) {
@@ -634,10 +766,11 @@ int dns_debug_zderiv(dns_ctx* c, const double* spec_in, double* spec_out) {
     if (!c || !spec_in || !spec_out) return DNS_EINVAL;
     HIPCHECK(c, hipMemcpyAsync(c->tmp, spec_in, c->NS * sizeof(double), hipMemcpyHostToDevice, c->stream));
     ZArgs z{};
+    z.mstride = 0;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
     z.Asw = c->D1sw; z.KS = c->op.D1.ks;
     z.in[0] = c->tmp; z.out[0] = c->phi; z.xw = c->xw; z.yw = c->yw;
-    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, c->stream));
+    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, 1, c->stream));
     HIPCHECK(c, hipMemcpyAsync(spec_out, c->phi, c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -666,16 +799,15 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
         HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
         if ((r = nl_r2c(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-        FlowArgs fa = flow_args(c);
-        HIPCHECK(c, launch_flow_control(fa, c->stream));
+        if ((r = flow_control(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
         if ((r = dns_viscous_step(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
         PoisArgs pa = poisson_args(c, 1);
-        HIPCHECK(c, launch_poisson(c->rb, pa, c->stream));
+        HIPCHECK(c, launch_poisson(c->rb, pa, c->B, c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
         Mode0Args ma = mode0_args(c, 1);
-        HIPCHECK(c, launch_poisson_mode0(ma, c->stream));
+        HIPCHECK(c, launch_poisson_mode0(ma, c->B, c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));
         HIPCHECK(c, hipEventSynchronize(c->ev[7]));
         for (int i = 0; i < NK; i++) {
@@ -688,38 +820,28 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
     return 0;
 }
 
-int dns_debug_mfma_probe_f32(double* out) {
+static int run_probe(double* out, hipError_t (*launch)(double*, hipStream_t)) {
     if (!out) return DNS_EINVAL;
     std::string why;
-    if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
+    if (!dns_check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
     double* d = nullptr;
     const size_t n = size_t(4096) * 256;
     if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
-    hipError_t e = launch_mfma_probe_f32(d, nullptr);
+    hipError_t e = launch(d, nullptr);
     if (e == hipSuccess) e = hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     return e == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
-int dns_debug_mfma_probe(double* out) {
-    if (!out) return DNS_EINVAL;
-    std::string why;
-    if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
-    double* d = nullptr;
-    const size_t n = size_t(4096) * 256;
-    if (hipMalloc(&d, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
-    hipError_t e = launch_mfma_probe(d, nullptr);
-    if (e == hipSuccess) e = hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(d);
-    return e == hipSuccess ? 0 : DNS_EDEVICE;
-}
+int dns_debug_mfma_probe(double* out) { return run_probe(out, launch_mfma_probe); }
+int dns_debug_mfma_probe_f32(double* out) { return run_probe(out, launch_mfma_probe_f32); }
 
 static void unswizzle(const SwizzledA& a, std::vector<double>& m) {
     m.assign(size_t(a.rows) * a.cols, 0.0);
     for (int rb = 0; rb < a.rb; rb++)
         for (int s = 0; s < a.ks; s++)
             for (int lane = 0; lane < 64; lane++) {
-                int r = rb * 16 + (lane & 15), cc = 4 * s + (lane >> 4);
+                const int r = rb * 16 + (lane & 15), cc = 4 * s + (lane >> 4);
                 if (r < a.rows && cc < a.cols) m[size_t(r) * a.cols + cc] = a.data[(size_t(rb) * a.ks + s) * 64 + lane];
             }
 }
@@ -727,7 +849,10 @@ static void unswizzle(const SwizzledA& a, std::vector<double>& m) {
 int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols) {
     Operators op;
     std::string err;
-    if (!build_operators(nz, dt, re, cn ? 0.5 : 1.0, cn != 0, exact != 0, op, err)) { g_create_error = err; return DNS_ESOLVER; }
+    if (!build_operators(nz, dt, re, cn ? 0.5 : 1.0, cn != 0, exact != 0, op, err)) {
+        g_create_error = err;
+        return DNS_ESOLVER;
+    }
     std::vector<double> m;
     int r = 0, cc = 0;
     auto vec = [&](const std::vector<double>& v) { m = v; r = int(v.size()); cc = 1; };
@@ -745,7 +870,12 @@ int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int whic
         case 9: m = op.m0; r = cc = nz - 1; break;
         case 10: mat(op.D1); break;
         case 11: vec(op.wz); break;
-        case 12: m.resize(size_t(nz) * nz); for (int i = 0; i < nz; i++) for (int j = 0; j < nz; j++) m[size_t(i) * nz + j] = op.d2[i + size_t(nz) * j]; r = cc = nz; break;
+        case 12:
+            m.resize(size_t(nz) * nz);
+            for (int i = 0; i < nz; i++)
+                for (int j = 0; j < nz; j++) m[size_t(i) * nz + j] = op.d2[i + size_t(nz) * j];
+            r = cc = nz;
+            break;
         default: return DNS_EINVAL;
     }
     if (rows) *rows = r;
@@ -759,124 +889,6 @@ int dns_debug_operator_error(dns_ctx* c, double* out2) {
     out2[0] = c->op.helm_recon_err;
     out2[1] = c->op.pois_recon_err;
     return 0;
-}
-
-// ---- FFT module -----------------------------------------------------------------------------
-}  // extern "C"
-
-struct dns_fft_plan {
-    FFTGeom g;
-    bool f32 = false;
-    double *twx = nullptr, *twy = nullptr;
-    float *twx32 = nullptr, *twy32 = nullptr;
-    double *din = nullptr, *dout = nullptr;
-    size_t cap_in = 0, cap_out = 0;
-    hipStream_t stream = nullptr;
-};
-
-static int plan_buf(double** p, size_t& cap, size_t n) {
-    if (cap >= n) return 0;
-    if (*p) (void)hipFree(*p);
-    *p = nullptr; cap = 0;
-    if (hipMalloc(p, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
-    cap = n;
-    return 0;
-}
-
-extern "C" {
-
-int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
-    if (!out) return DNS_EINVAL;
-    *out = nullptr;
-    if (precision != DNS_PREC_FP64 && precision != DNS_PREC_MIXED) { g_create_error = "bad precision"; return DNS_EINVAL; }
-    if (nx < 4 || ny < 4 || (nx & 1) || (ny & 1)) { g_create_error = "nx, ny must be even and >= 4"; return DNS_EINVAL; }
-    std::string why;
-    if (!check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
-    dns_fft_plan* p = new dns_fft_plan();
-    p->f32 = precision == DNS_PREC_MIXED;
-    if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0) || !make_factors(ny, p->g.fy) ||
-        !make_factors(nx / 2, p->g.fx)) {
-        delete p;
-        g_create_error = "unsupported plane size";
-        return DNS_EUNSUPPORTED;
-    }
-    std::vector<double> tw;
-    twiddles(nx, tw);
-    if (hipMalloc(&p->twx, tw.size() * sizeof(double)) != hipSuccess) { delete p; return DNS_ENOMEM; }
-    (void)hipMemcpy(p->twx, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
-    twiddles(ny, tw);
-    if (hipMalloc(&p->twy, tw.size() * sizeof(double)) != hipSuccess) { (void)hipFree(p->twx); delete p; return DNS_ENOMEM; }
-    (void)hipMemcpy(p->twy, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
-    p->g.twx = reinterpret_cast<const double2*>(p->twx);
-    p->g.twy = reinterpret_cast<const double2*>(p->twy);
-    {
-        std::vector<float> t32;
-        twiddles(nx, tw); t32.assign(tw.begin(), tw.end());
-        if (hipMalloc(&p->twx32, t32.size() * sizeof(float)) != hipSuccess) return DNS_ENOMEM;
-        (void)hipMemcpy(p->twx32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
-        twiddles(ny, tw); t32.assign(tw.begin(), tw.end());
-        if (hipMalloc(&p->twy32, t32.size() * sizeof(float)) != hipSuccess) return DNS_ENOMEM;
-        (void)hipMemcpy(p->twy32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
-        p->g.twx32 = reinterpret_cast<const float2*>(p->twx32);
-        p->g.twy32 = reinterpret_cast<const float2*>(p->twy32);
-    }
-    (void)fft_prepare(p->g);
-    (void)hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
-    *out = p;
-    return 0;
-}
-
-int dns_fft_plan_destroy(dns_fft_plan* p) {
-    if (!p) return 0;
-    if (p->stream) (void)hipStreamSynchronize(p->stream);
-    (void)hipFree(p->twx); (void)hipFree(p->twy); (void)hipFree(p->twx32); (void)hipFree(p->twy32);
-    if (p->din) (void)hipFree(p->din);
-    if (p->dout) (void)hipFree(p->dout);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
-    delete p;
-    return 0;
-}
-
-int dns_fft_forward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
-    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
-    FFTArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.src[0] = in; a.dst[0] = out;
-    a.sstride = (long long)p->g.nx * p->g.ny; a.dstride = 2LL * p->g.nxh * p->g.ny;
-    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
-    return launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
-}
-
-int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
-    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
-    FFTArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.src[0] = in; a.dst[0] = out;
-    a.sstride = 2LL * p->g.nxh * p->g.ny; a.dstride = (long long)p->g.nx * p->g.ny;
-    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
-    return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
-}
-
-int dns_fft_forward(dns_fft_plan* p, const double* in, double* out, int nplanes) {
-    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
-    size_t nr = size_t(p->g.nx) * p->g.ny * nplanes, nc = size_t(2) * p->g.nxh * p->g.ny * nplanes;
-    if (plan_buf(&p->din, p->cap_in, nr) || plan_buf(&p->dout, p->cap_out, nc)) return DNS_ENOMEM;
-    if (hipMemcpyAsync(p->din, in, nr * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) return DNS_EDEVICE;
-    int r = dns_fft_forward_device(p, p->din, p->dout, nplanes, p->stream);
-    if (r) return r;
-    if (hipMemcpyAsync(out, p->dout, nc * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) return DNS_EDEVICE;
-    return hipStreamSynchronize(p->stream) == hipSuccess ? 0 : DNS_EDEVICE;
-}
-
-int dns_fft_backward(dns_fft_plan* p, const double* in, double* out, int nplanes) {
-    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
-    size_t nr = size_t(p->g.nx) * p->g.ny * nplanes, nc = size_t(2) * p->g.nxh * p->g.ny * nplanes;
-    if (plan_buf(&p->din, p->cap_in, nc) || plan_buf(&p->dout, p->cap_out, nr)) return DNS_ENOMEM;
-    if (hipMemcpyAsync(p->din, in, nc * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) return DNS_EDEVICE;
-    int r = dns_fft_backward_device(p, p->din, p->dout, nplanes, p->stream);
-    if (r) return r;
-    if (hipMemcpyAsync(out, p->dout, nr * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) return DNS_EDEVICE;
-    return hipStreamSynchronize(p->stream) == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
 }  // extern "C"

@@ -1,0 +1,136 @@
+// fft_plan.hip -- the FFT module of the C ABI (dns_fft_*): plane transforms on host or
+// device arrays, the drop-in for fftw3_dns_module.f90:19-152.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/f90dns.h"
+#include "kernels.h"
+#include "setup.h"
+
+using namespace dns;
+
+// the context-creation error slot of dns_api.hip (dns_last_error(NULL))
+void dns_set_create_error(const std::string& msg);
+bool dns_check_gfx950(std::string& why);
+bool dns_make_factors(int n, Factors& f);
+
+
+struct dns_fft_plan {
+    FFTGeom g;
+    bool f32 = false;
+    double *twx = nullptr, *twy = nullptr;
+    float *twx32 = nullptr, *twy32 = nullptr;
+    double *din = nullptr, *dout = nullptr;
+    size_t cap_in = 0, cap_out = 0;
+    hipStream_t stream = nullptr;
+};
+
+static int plan_buf(double** p, size_t& cap, size_t n) {
+    if (cap >= n) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr; cap = 0;
+    if (hipMalloc(p, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
+    cap = n;
+    return 0;
+}
+
+extern "C" {
+
+int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
+    if (!out) return DNS_EINVAL;
+    *out = nullptr;
+    if (precision != DNS_PREC_FP64 && precision != DNS_PREC_MIXED) { dns_set_create_error("bad precision"); return DNS_EINVAL; }
+    if (nx < 4 || ny < 4 || (nx & 1) || (ny & 1)) { dns_set_create_error("nx, ny must be even and >= 4"); return DNS_EINVAL; }
+    std::string why;
+    if (!dns_check_gfx950(why)) { dns_set_create_error(why); return DNS_EDEVICE; }
+    dns_fft_plan* p = new dns_fft_plan();
+    p->f32 = precision == DNS_PREC_MIXED;
+    if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0) || !dns_make_factors(ny, p->g.fy) ||
+        !dns_make_factors(nx / 2, p->g.fx)) {
+        delete p;
+        dns_set_create_error("unsupported plane size");
+        return DNS_EUNSUPPORTED;
+    }
+    std::vector<double> tw;
+    twiddles(nx, tw);
+    if (hipMalloc(&p->twx, tw.size() * sizeof(double)) != hipSuccess) { delete p; return DNS_ENOMEM; }
+    (void)hipMemcpy(p->twx, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    twiddles(ny, tw);
+    if (hipMalloc(&p->twy, tw.size() * sizeof(double)) != hipSuccess) { (void)hipFree(p->twx); delete p; return DNS_ENOMEM; }
+    (void)hipMemcpy(p->twy, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice);
+    p->g.twx = reinterpret_cast<const double2*>(p->twx);
+    p->g.twy = reinterpret_cast<const double2*>(p->twy);
+    {
+        std::vector<float> t32;
+        twiddles(nx, tw); t32.assign(tw.begin(), tw.end());
+        if (hipMalloc(&p->twx32, t32.size() * sizeof(float)) != hipSuccess) return DNS_ENOMEM;
+        (void)hipMemcpy(p->twx32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
+        twiddles(ny, tw); t32.assign(tw.begin(), tw.end());
+        if (hipMalloc(&p->twy32, t32.size() * sizeof(float)) != hipSuccess) return DNS_ENOMEM;
+        (void)hipMemcpy(p->twy32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice);
+        p->g.twx32 = reinterpret_cast<const float2*>(p->twx32);
+        p->g.twy32 = reinterpret_cast<const float2*>(p->twy32);
+    }
+    (void)fft_prepare(p->g);
+    (void)hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    *out = p;
+    return 0;
+}
+
+int dns_fft_plan_destroy(dns_fft_plan* p) {
+    if (!p) return 0;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(p->twx); (void)hipFree(p->twy); (void)hipFree(p->twx32); (void)hipFree(p->twy32);
+    if (p->din) (void)hipFree(p->din);
+    if (p->dout) (void)hipFree(p->dout);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+int dns_fft_forward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    FFTArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.src[0] = in; a.dst[0] = out;
+    a.sstride = (long long)p->g.nx * p->g.ny; a.dstride = 2LL * p->g.nxh * p->g.ny;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    return launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int nplanes, void* stream) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    FFTArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.src[0] = in; a.dst[0] = out;
+    a.sstride = 2LL * p->g.nxh * p->g.ny; a.dstride = (long long)p->g.nx * p->g.ny;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_forward(dns_fft_plan* p, const double* in, double* out, int nplanes) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    size_t nr = size_t(p->g.nx) * p->g.ny * nplanes, nc = size_t(2) * p->g.nxh * p->g.ny * nplanes;
+    if (plan_buf(&p->din, p->cap_in, nr) || plan_buf(&p->dout, p->cap_out, nc)) return DNS_ENOMEM;
+    if (hipMemcpyAsync(p->din, in, nr * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) return DNS_EDEVICE;
+    int r = dns_fft_forward_device(p, p->din, p->dout, nplanes, p->stream);
+    if (r) return r;
+    if (hipMemcpyAsync(out, p->dout, nc * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) return DNS_EDEVICE;
+    return hipStreamSynchronize(p->stream) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_backward(dns_fft_plan* p, const double* in, double* out, int nplanes) {
+    if (!p || !in || !out || nplanes < 0) return DNS_EINVAL;
+    size_t nr = size_t(p->g.nx) * p->g.ny * nplanes, nc = size_t(2) * p->g.nxh * p->g.ny * nplanes;
+    if (plan_buf(&p->din, p->cap_in, nc) || plan_buf(&p->dout, p->cap_out, nr)) return DNS_ENOMEM;
+    if (hipMemcpyAsync(p->din, in, nc * sizeof(double), hipMemcpyHostToDevice, p->stream) != hipSuccess) return DNS_EDEVICE;
+    int r = dns_fft_backward_device(p, p->din, p->dout, nplanes, p->stream);
+    if (r) return r;
+    if (hipMemcpyAsync(out, p->dout, nr * sizeof(double), hipMemcpyDeviceToHost, p->stream) != hipSuccess) return DNS_EDEVICE;
+    return hipStreamSynchronize(p->stream) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+}  // extern "C"

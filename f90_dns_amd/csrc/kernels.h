@@ -45,7 +45,10 @@ hipError_t launch_c2r(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfie
 hipError_t launch_r2c(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, bool f32, hipStream_t s);
 
 // ---- wall-normal contractions and per-mode solves ---------------------------------------
+// Batched contexts hold B members of one shape; member b's spectral field starts at
+// b * mstride elements and the z-kernels take the member from blockIdx.z.
 struct ZArgs {
+    long long mstride;
     int nz, H2, nxh, ny, nx;     // H2 = 2 * (nx/2+1) * ny real columns per plane
     const double* Asw; int KS;   // swizzled operator (nz rows)
     const double* in[3];
@@ -54,24 +57,28 @@ struct ZArgs {
     int out_f32;
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
-hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s);
+hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
 
+struct FlowState;
 struct HelmArgs {
+    long long mstride, fstride;  // member strides of the fields and of Fsw (per-member Re)
     int nz, m, H2, nxh, ny, nx;
     const double* Fsw; int FKS;  // m x (m + nz) [T | f T D2]  (KS covers m cols when not CN)
     const double* Bsw; int BKS;  // m x m
-    const double* c1; const double* c2; const double* hl;
-    double f, dt, nxny;
+    const double* c1; const double* c2; const double* hl;   // m per member
+    const double* fvec;          // theta dt / Re per member
+    double dt, nxny;
     int cn;
     const double* xw; const double* yw;
     const double* uh[3]; const double* sh[3]; double* out[3];   // sh float when s_f32
     const double* ph;
     int s_f32;
-    const double* G;             // device scalar: current pressure gradient
+    const FlowState* flow;       // per member: current pressure gradient G
 };
-hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s);
+hipError_t launch_helmholtz(int rbi, const HelmArgs& a, int nmembers, hipStream_t s);
 
 struct PoisArgs {
+    long long mstride;
     int nz, H2, nxh, ny, nx;
     const double* Fsw; int FKS;  // nz x 2nz
     const double* Bsw; const double* DBsw; int BKS;
@@ -83,9 +90,10 @@ struct PoisArgs {
     double* u; double* v; double* w; double* p;             // fused outputs
     double* phi; double* dphi;                              // stage outputs
 };
-hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s);
+hipError_t launch_poisson(int rb, const PoisArgs& a, int nmembers, hipStream_t s);
 
 struct Mode0Args {
+    long long mstride;
     int nz, H2;
     const double* e0;            // row-major nz x nz: phi(0,0,:) = E0 w*(0,0,:)
     const double* f0;            // row-major nz x nz: D1 phi(0,0,:) = F0 w*(0,0,:)
@@ -95,7 +103,7 @@ struct Mode0Args {
     double* u; double* v; double* w; double* p;
     double* phi; double* dphi;
 };
-hipError_t launch_poisson_mode0(const Mode0Args& a, hipStream_t s);
+hipError_t launch_poisson_mode0(const Mode0Args& a, int nmembers, hipStream_t s);
 
 struct CorrArgs {
     int nz, H2, nxh, ny, nx;
@@ -106,22 +114,28 @@ struct CorrArgs {
     double dt;
 };
 hipError_t launch_correction(const CorrArgs& a, hipStream_t s);
-hipError_t launch_zero_walls(int nz, int H2, double* u, double* v, double* w, hipStream_t s);
+hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nmembers, double* u, double* v, double* w,
+                             hipStream_t s);
 
-// PI / constant forcing state, device resident (update_pressure_gradient_3d)
+// PI / constant forcing state, device resident (update_pressure_gradient_3d), one per member
 struct FlowState {
     double G, Ub, err, err_int, err_prev, gsum, gavg;
     int gcount, pad;
 };
+struct FlowMember {
+    double target_G, target_Ub, gain, re;
+    int method, freq, istart, pad;
+};
 struct FlowArgs {
-    int nz, method, freq, istart;
-    double target_G, target_Ub, gain, re, dt, nxny, wsum;
+    int nz;
+    double dt, nxny, wsum;
     const double* uh;            // u-hat: reads Re u(0,0,k) at stride H2
-    long long H2;
+    long long H2, mstride;
     const double* wz;
+    const FlowMember* mem;
     FlowState* st;
 };
-hipError_t launch_flow_control(const FlowArgs& a, hipStream_t s);
+hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
 
 // reductions over physical fields: max|u|,|v|,|w|, sum 0.5|u|^2 w_k, max|d| (any may be null)
 struct ReduceArgs {

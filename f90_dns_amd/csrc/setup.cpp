@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace dns {
 
@@ -181,6 +184,28 @@ static void jacobi_eig(int n, std::vector<ld>& A, std::vector<ld>& V, std::vecto
     for (int i = 0; i < n; i++) lam[i] = A[size_t(i) * n + i];
 }
 
+// The eigen bases and the mode-(0,0) inverse depend only on (nz, D1 choice): computed once per
+// process and shared by every context (Re and dt enter through scalings and cheap products).
+struct EigEntry {
+    std::vector<ld> Q, lam;
+};
+static std::mutex g_eig_mu;
+static std::map<std::tuple<int, int, int>, EigEntry> g_eig;
+
+static void cached_eig(int n, int nz, int exact, int which, const std::vector<ld>& A, std::vector<ld>& Q,
+                       std::vector<ld>& lam) {
+    const auto key = std::make_tuple(nz, exact, which);
+    {
+        std::lock_guard<std::mutex> lk(g_eig_mu);
+        auto it = g_eig.find(key);
+        if (it != g_eig.end()) { Q = it->second.Q; lam = it->second.lam; return; }
+    }
+    std::vector<ld> a = A;
+    jacobi_eig(n, a, Q, lam);
+    std::lock_guard<std::mutex> lk(g_eig_mu);
+    g_eig[key] = EigEntry{Q, lam};
+}
+
 // inverse of a dense matrix (row-major) by Gauss-Jordan with partial pivoting, long double
 static bool invert(int n, std::vector<ld> a, std::vector<ld>& inv) {
     inv.assign(size_t(n) * n, 0.0L);
@@ -248,7 +273,7 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
         for (int i = 0; i < m; i++)
             for (int j = 0; j < m; j++) At[size_t(i) * m + j] = isw[i + 1] * A(i + 1, j + 1) * isw[j + 1];
         std::vector<ld> A0 = At;
-        jacobi_eig(m, At, Q, lam);
+        cached_eig(m, nz, exact_d1 ? 1 : 0, 0, At, Q, lam);
         // reconstruction check
         ld e = 0, s = 0;
         for (int i = 0; i < m; i++) for (int j = 0; j < m; j++) {
@@ -289,7 +314,7 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
         for (int i = 0; i < nz; i++)
             for (int j = 0; j < nz; j++) Af[size_t(i) * nz + j] = isw[i] * A(i, j) * isw[j];
         std::vector<ld> A0 = Af;
-        jacobi_eig(nz, Af, Q, lam);
+        cached_eig(nz, nz, exact_d1 ? 1 : 0, 1, Af, Q, lam);
         ld e = 0, s = 0;
         for (int i = 0; i < nz; i++) for (int j = 0; j < nz; j++) {
             ld r = 0; for (int k = 0; k < nz; k++) r += Q[size_t(i) * nz + k] * lam[k] * Q[size_t(j) * nz + k];
@@ -321,7 +346,23 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
         // mode (0,0): M0 = -(A_{1:,1:})^{-1}
         std::vector<ld> a1(size_t(nz - 1) * (nz - 1)), inv;
         for (int i = 0; i < nz - 1; i++) for (int j = 0; j < nz - 1; j++) a1[size_t(i) * (nz - 1) + j] = A(i + 1, j + 1);
-        if (!invert(nz - 1, a1, inv)) {
+        bool ok;
+        {
+            // cache the inverse as an "eigen entry" (Q = inverse, lam empty; lam = {0} marks singular)
+            const auto key = std::make_tuple(nz, exact_d1 ? 1 : 0, 2);
+            std::unique_lock<std::mutex> lk(g_eig_mu);
+            auto it = g_eig.find(key);
+            if (it != g_eig.end()) {
+                inv = it->second.Q;
+                ok = it->second.lam.empty();
+            } else {
+                lk.unlock();
+                ok = invert(nz - 1, a1, inv);
+                lk.lock();
+                g_eig[key] = EigEntry{inv, ok ? std::vector<ld>() : std::vector<ld>{0.0L}};
+            }
+        }
+        if (!ok) {
             // the reference's zgesv stops here (DNS_pressure_BC_3D.f90:1715-1718): with its D1
             // this happens at nz = 65, where the centre diagonal entry of D1 is exactly 0
             err = "singular mode-(0,0) Poisson operator (the reference's zgesv would stop)";

@@ -49,9 +49,11 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
     const int mode0 = blockIdx.x * BMODES;
     const int nfield = MODE == Z_VORT ? 2 : 1;
+    const size_t mo = size_t(blockIdx.z) * a.mstride;
+    const double* w_in = a.in[2] + mo;
     for (int q = 0; q < nfield; q++) {
         const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : blockIdx.y;   // vort: v first, then u
-        const double* in = a.in[f];
+        const double* in = a.in[f] + mo;
         d4 acc[RBW][CT];
         zero_acc2(acc);
         gemm_staged<RBW>(acc, a.Asw, RB, a.KS, [&](int k, int ml) -> double2 {
@@ -72,12 +74,12 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
                     if (row >= nz) continue;
                     const double y = acc[ii][ct][r];
                     if (MODE == Z_PLAIN) {
-                        a.out[f][size_t(row) * H2 + col] = y;
+                        a.out[f][mo + size_t(row) * H2 + col] = y;
                     } else {
                         double res;
-                        const double wo = a.in[2][size_t(row) * H2 + (col ^ 1)];   // the other part of w
-                        const double wr = part == 0 ? a.in[2][size_t(row) * H2 + col] : wo;
-                        const double wi = part == 0 ? wo : a.in[2][size_t(row) * H2 + col];
+                        const double wo = w_in[size_t(row) * H2 + (col ^ 1)];   // the other part of w
+                        const double wr = part == 0 ? w_in[size_t(row) * H2 + col] : wo;
+                        const double wi = part == 0 ? wo : w_in[size_t(row) * H2 + col];
                         if (f == 1) {        // omega_x = i ky w - D1 v
                             const double ky = a.yw[j];
                             res = part == 0 ? (-ky * wi - y) : (ky * wr - y);
@@ -86,8 +88,8 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
                             res = part == 0 ? (y + kx * wi) : (y - kx * wr);
                         }
                         const int o = f == 1 ? 0 : 1;
-                        if (a.out_f32) reinterpret_cast<float*>(a.out[o])[size_t(row) * H2 + col] = float(res);
-                        else a.out[o][size_t(row) * H2 + col] = res;
+                        if (a.out_f32) reinterpret_cast<float*>(a.out[o])[mo + size_t(row) * H2 + col] = float(res);
+                        else a.out[o][mo + size_t(row) * H2 + col] = res;
                     }
                 }
             }
@@ -107,13 +109,21 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
 template <int RBW>
 __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     __shared__ double smem[(4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB];
-    const int comp = blockIdx.y;
+    const int comp = blockIdx.y, mb = blockIdx.z;
     const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2, RBI = (m + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int mode0 = blockIdx.x * BMODES;
-    const double* uh = a.uh[comp];
-    const double* sh = a.sh[comp];
-    const double force = comp == 0 ? (*a.G) * a.nxny : 0.0;
+    const size_t mo = size_t(mb) * a.mstride;
+    const double* uh = a.uh[comp] + mo;
+    const double* sh = a.s_f32 ? reinterpret_cast<const double*>(reinterpret_cast<const float*>(a.sh[comp]) + mo)
+                               : a.sh[comp] + mo;
+    const double* ph = a.ph + mo;
+    const double* Fsw = a.Fsw + size_t(mb) * a.fstride;
+    const double* c1v = a.c1 + size_t(mb) * m;
+    const double* c2v = a.c2 + size_t(mb) * m;
+    const double* hlv = a.hl + size_t(mb) * m;
+    const double fco = a.fvec[mb];
+    const double force = comp == 0 ? a.flow[mb].G * a.nxny : 0.0;
     // per-lane column constants for the two 16-column tiles
     double k2c[CT], bc1[CT], bc2[CT];
 #pragma unroll
@@ -126,8 +136,8 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
             k2c[ct] = kx * kx + ky * ky;
             if (comp < 2) {
                 const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
-                const double2 p0 = reinterpret_cast<const double2*>(a.ph)[mode];
-                const double2 p1 = reinterpret_cast<const double2*>(a.ph + size_t(nz - 1) * H2)[mode];
+                const double2 p0 = reinterpret_cast<const double2*>(ph)[mode];
+                const double2 p1 = reinterpret_cast<const double2*>(ph + size_t(nz - 1) * H2)[mode];
                 bc1[ct] = part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x);    // -dt * (i ke p)
                 bc2[ct] = part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x);
             }
@@ -135,13 +145,13 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     }
     d4 acc[RBW][CT];
     zero_acc2(acc);
-    gemm_staged<RBW>(acc, a.Fsw, RBI, a.FKS, [&](int k, int ml) -> double2 {
+    gemm_staged<RBW>(acc, Fsw, RBI, a.FKS, [&](int k, int ml) -> double2 {
         const int mode = mode0 + ml;
         if (mode >= H) return make_double2(0.0, 0.0);
         if (k < m) {
             const int j = mode / a.nxh, i = mode - j * a.nxh;
             const double kx = a.xw[i], ky = a.yw[j];
-            const double alpha = a.cn ? 1.0 - a.f * (kx * kx + ky * ky) : 1.0;
+            const double alpha = a.cn ? 1.0 - fco * (kx * kx + ky * ky) : 1.0;
             const double2 u = reinterpret_cast<const double2*>(uh + size_t(k + 1) * H2)[mode];
             double2 sv;
             if (a.s_f32) {
@@ -167,7 +177,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
                 const int row = out_row(ii, r);
                 double z = 0.0;
                 if (row < m && out_col(ct) < H2)
-                    z = (acc[ii][ct][r] + a.c1[row] * bc1[ct] + a.c2[row] * bc2[ct]) / (a.hl[row] + a.f * k2c[ct]);
+                    z = (acc[ii][ct][r] + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
                 zs[row * LDB + ct * 16 + (lane & 15)] = z;
             }
         }
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     __syncthreads();
     zero_acc2(acc);
     gemm_lds<RBW>(acc, a.Bsw, RBI, a.BKS, zs);
-    double* out = a.out[comp];
+    double* out = a.out[comp] + mo;
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct);
@@ -210,6 +220,10 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int mode0 = blockIdx.x * BMODES;
+    const size_t mo = size_t(blockIdx.z) * a.mstride;
+    const double* us = a.us + mo; const double* vs = a.vs + mo; const double* ws = a.ws + mo;
+    double* uo = a.u + mo; double* vo = a.v + mo; double* wo = a.w + mo; double* po = a.p + mo;
+    double* phio = a.phi + mo; double* dphio = a.dphi + mo;
     double k2c[CT], kxe[CT], kye[CT];
     bool live[CT];
 #pragma unroll
@@ -233,11 +247,11 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
         if (k < nz) {
             const int j = mode / a.nxh, i = mode - j * a.nxh;
             const double ke = kx_eff(a.xw, i, a.nx), kf = ky_eff(a.yw, i, j, a.nx, a.ny);
-            const double2 u = reinterpret_cast<const double2*>(a.us + size_t(k) * H2)[mode];
-            const double2 v = reinterpret_cast<const double2*>(a.vs + size_t(k) * H2)[mode];
+            const double2 u = reinterpret_cast<const double2*>(us + size_t(k) * H2)[mode];
+            const double2 v = reinterpret_cast<const double2*>(vs + size_t(k) * H2)[mode];
             return make_double2(-ke * u.y - kf * v.y, ke * u.x + kf * v.x);
         }
-        if (k < 2 * nz) return reinterpret_cast<const double2*>(a.ws + size_t(k - nz) * H2)[mode];
+        if (k < 2 * nz) return reinterpret_cast<const double2*>(ws + size_t(k - nz) * H2)[mode];
         return make_double2(0.0, 0.0);
     }, smem);
     double* zs = smem;
@@ -273,7 +287,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
                 if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
                 const size_t q = size_t(row) * H2 + col;
                 if (!a.fused) {
-                    a.phi[q] = ph;
+                    phio[q] = ph;
                     continue;
                 }
                 const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
@@ -281,9 +295,9 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
                 // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
                 const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
                 const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
-                a.u[q] = wall ? 0.0 : a.us[q] - a.dt * du;
-                a.v[q] = wall ? 0.0 : a.vs[q] - a.dt * dv;
-                a.p[q] = a.p[q] + ph;
+                uo[q] = wall ? 0.0 : us[q] - a.dt * du;
+                vo[q] = wall ? 0.0 : vs[q] - a.dt * dv;
+                po[q] = po[q] + ph;
             }
         }
     }
@@ -301,11 +315,11 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
                 if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
                 const size_t q = size_t(row) * H2 + col;
                 if (!a.fused) {
-                    a.dphi[q] = acc[ii][ct][r];
+                    dphio[q] = acc[ii][ct][r];
                     continue;
                 }
                 const bool wall = row == 0 || row == nz - 1;
-                a.w[q] = wall ? 0.0 : a.ws[q] - a.dt * acc[ii][ct][r];
+                wo[q] = wall ? 0.0 : ws[q] - a.dt * acc[ii][ct][r];
             }
         }
     }
@@ -323,9 +337,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) {
     const int nz = a.nz, k = blockIdx.x, lane = threadIdx.x;
+    const size_t mo = size_t(blockIdx.y) * a.mstride;
+    const double* ws = a.ws + mo;
     double s0 = 0.0, s1 = 0.0, d0 = 0.0, d1 = 0.0;
     for (int q = lane; q < nz; q += 64) {
-        const double w0 = a.ws[size_t(q) * a.H2], w1 = a.ws[size_t(q) * a.H2 + 1];
+        const double w0 = ws[size_t(q) * a.H2], w1 = ws[size_t(q) * a.H2 + 1];
         const double e = a.e0[size_t(k) * nz + q], f = a.f0[size_t(k) * nz + q];
         s0 += e * w0; s1 += e * w1; d0 += f * w0; d1 += f * w1;
     }
@@ -333,7 +349,7 @@ __global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) {
     if (lane < 2) {
         const int part = lane;
         const double xv = part ? s1 : s0, dx = part ? d1 : d0;
-        const size_t o = size_t(k) * a.H2 + part;
+        const size_t o = mo + size_t(k) * a.H2 + part;
         if (!a.fused) {
             a.phi[o] = xv;
             a.dphi[o] = dx;
@@ -369,7 +385,9 @@ __global__ __launch_bounds__(256) void k_correction(CorrArgs a) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, double* u, double* v, double* w) {
+__global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, long long mstride, double* u, double* v, double* w) {
+    const size_t mo = size_t(blockIdx.y) * mstride;
+    u += mo; v += mo; w += mo;
     const size_t top = size_t(nz - 1) * H2;
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < H2; c += gridDim.x * blockDim.x) {
         u[c] = 0.0; v[c] = 0.0; w[c] = 0.0;
@@ -382,24 +400,26 @@ __global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, double* u, d
 // U_b = sum_k w_k Re u_hat(0,0,k) / (nx ny sum w): the (0,0) coefficient of the
 // unnormalised forward transform is the plane sum.
 __global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, mb = blockIdx.x;
+    const FlowMember P = a.mem[mb];
+    const double* uh = a.uh + size_t(mb) * a.mstride;
     double part = 0.0;
-    if (a.method != 1)
-        for (int k = lane; k < a.nz; k += 64) part += a.uh[size_t(k) * a.H2] * a.wz[k];
+    if (P.method != 1)
+        for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * a.wz[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if (lane != 0) return;
-    FlowState s = *a.st;
-    if (a.method == 1) {
-        s.G = a.target_G;
-    } else if (a.istart % a.freq == 0) {
+    FlowState s = a.st[mb];
+    if (P.method == 1) {
+        s.G = P.target_G;
+    } else if (P.istart % P.freq == 0) {
         s.Ub = part / (a.nxny * a.wsum);
-        s.err = a.target_Ub - s.Ub;
-        const double kp = a.gain, ki = 0.05 * a.gain;
-        const double gu = 3.0 / a.re + kp * s.err + ki * s.err_int;
+        s.err = P.target_Ub - s.Ub;
+        const double kp = P.gain, ki = 0.05 * P.gain;
+        const double gu = 3.0 / P.re + kp * s.err + ki * s.err_int;
         const double gc = fmax(0.0, fmin(10.0, gu));
         if (fabs(gc - gu) < 1.0e-12) {
-            if (a.istart <= a.freq) s.err_int = s.err_int + s.err * a.dt;
+            if (P.istart <= P.freq) s.err_int = s.err_int + s.err * a.dt;
             else s.err_int = s.err_int + 0.5 * a.dt * (s.err_prev + s.err);
         }
         s.err_prev = s.err;
@@ -408,7 +428,7 @@ __global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) {
         s.gcount += 1;
         s.gavg = s.gsum / double(s.gcount);
     }
-    *a.st = s;
+    a.st[mb] = s;
 }
 
 // partial reductions: [max|u|, max|v|, max|w|, sum 0.5 |u|^2 w_k, max|d|]
@@ -458,9 +478,10 @@ __global__ void k_reduce_final(ReduceArgs a) {
 static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
 static inline int rbw_of(int rows) { return ((rows + 15) / 16 + 3) / 4; }
 
-hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s) {
+hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
     const int rbw = (rb + 3) / 4;
-    const dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
+    dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
+    g.z = nm;
     switch (rbw * 2 + (mode == Z_VORT)) {
         case 2: hipLaunchKernelGGL((k_zderiv<1, Z_PLAIN>), g, dim3(256), 0, s, a); break;
         case 3: hipLaunchKernelGGL((k_zderiv<1, Z_VORT>), g, dim3(256), 0, s, a); break;
@@ -473,8 +494,9 @@ hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s) {
-    const dim3 g = col_grid(a.H2, 3);
+hipError_t launch_helmholtz(int rbi, const HelmArgs& a, int nm, hipStream_t s) {
+    dim3 g = col_grid(a.H2, 3);
+    g.z = nm;
     switch ((rbi + 3) / 4) {
         case 1: hipLaunchKernelGGL((k_helmholtz<1>), g, dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL((k_helmholtz<2>), g, dim3(256), 0, s, a); break;
@@ -484,8 +506,9 @@ hipError_t launch_helmholtz(int rbi, const HelmArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s) {
-    const dim3 g = col_grid(a.H2, 1);
+hipError_t launch_poisson(int rb, const PoisArgs& a, int nm, hipStream_t s) {
+    dim3 g = col_grid(a.H2, 1);
+    g.z = nm;
     switch ((rb + 3) / 4) {
         case 1: hipLaunchKernelGGL((k_poisson<1>), g, dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL((k_poisson<2>), g, dim3(256), 0, s, a); break;
@@ -494,8 +517,8 @@ hipError_t launch_poisson(int rb, const PoisArgs& a, hipStream_t s) {
     }
     return hipGetLastError();
 }
-hipError_t launch_poisson_mode0(const Mode0Args& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_poisson_mode0, dim3(a.nz), dim3(64), 0, s, a);
+hipError_t launch_poisson_mode0(const Mode0Args& a, int nm, hipStream_t s) {
+    hipLaunchKernelGGL(k_poisson_mode0, dim3(a.nz, nm), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_correction(const CorrArgs& a, hipStream_t s) {
@@ -504,12 +527,13 @@ hipError_t launch_correction(const CorrArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_correction, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_zero_walls(int nz, int H2, double* u, double* v, double* w, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_walls, dim3((H2 + 255) / 256), dim3(256), 0, s, nz, H2, u, v, w);
+hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nm, double* u, double* v, double* w,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_walls, dim3((H2 + 255) / 256, nm), dim3(256), 0, s, nz, H2, mstride, u, v, w);
     return hipGetLastError();
 }
-hipError_t launch_flow_control(const FlowArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_flow_control, dim3(1), dim3(64), 0, s, a);
+hipError_t launch_flow_control(const FlowArgs& a, int nm, hipStream_t s) {
+    hipLaunchKernelGGL(k_flow_control, dim3(nm), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {

@@ -21,6 +21,8 @@
 #ifndef F90DNS_H
 #define F90DNS_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -122,6 +124,29 @@ int dns_check_divergence(dns_ctx* ctx, double* max_div);
 int dns_step_timing(dns_ctx* ctx, double* out4);
 /* Name of the HIP stream handle (hipStream_t) the context runs on. */
 void* dns_stream(dns_ctx* ctx);
+/* One step (begin + five stages, pressure fused) enqueued on the context stream, eager or
+ * replayed from the captured graph; returns without waiting. */
+int dns_enqueue_step(dns_ctx* ctx, int use_graph);
+/* 0: all work of the context has finished; 1: work still running. */
+int dns_query(dns_ctx* ctx);
+
+/* ---- Batched members (no counterpart in the reference, which runs one flow) ---------
+ * One context holds nmembers independent flows of the same grid, dt, domain, scheme flags
+ * and precision; Re, the flow-control targets and the initial perturbation may differ per
+ * member.  Every stage launch advances all members, so a step of B members costs far less
+ * than B steps.  A member's results do not depend on the other members: each member is
+ * computed by the same arithmetic as a context of one.  The unsuffixed field and
+ * diagnostics functions act on member 0. */
+int dns_create_batch(const dns_params* members, int nmembers, dns_ctx** out);
+int dns_members(const dns_ctx* ctx);
+size_t dns_device_bytes(const dns_ctx* ctx);     /* HBM held by the context */
+int dns_set_member_fields(dns_ctx* ctx, int member, const double* u, const double* v, const double* w,
+                          const double* p);
+int dns_get_member_fields(dns_ctx* ctx, int member, double* u, double* v, double* w, double* p);
+int dns_set_member_controller(dns_ctx* ctx, int member, double G, double integral, double prev, double gsum,
+                              int gcount);
+int dns_member_diagnostics(dns_ctx* ctx, int member, dns_diag* d);
+int dns_member_divergence(dns_ctx* ctx, int member, double* max_div);
 
 /* ---- FFT module (fftw3_dns_module.f90:19-152) --------------------------------
  * Plane transforms real(nx,ny) <-> complex(nx/2+1,ny): forward unnormalised, e^{-i};

@@ -70,3 +70,30 @@ def test_fortran_driver_builds_and_fails_cleanly_without_gpu(tmp_path):
     if r.returncode == 0:
         pytest.skip("a GPU is present: covered by the GPU driver test")
     assert "dns_create" in r.stdout and "ERROR" in r.stdout
+
+
+def test_batch_rejects_members_of_different_shape_before_touching_a_device():
+    """dns_create_batch validates every member (and that they share grid, dt, domain, flags)
+    before any HIP call, so these errors come back on a machine without a GPU too."""
+    import ctypes
+    import f90_dns_amd as dns
+    if not os.path.exists(dns.library_path()):
+        pytest.skip("libf90dns.so not built")
+    L = dns.lib()
+    a = dns.make_params(nx=32, ny=32, nz=33)
+    b = dns.make_params(nx=32, ny=32, nz=65)
+    arr = (dns.DnsParams * 2)(a, b)
+    h = ctypes.c_void_p()
+    assert L.dns_create_batch(arr, 2, ctypes.byref(h)) == 1
+    assert b"member 1 differs" in L.dns_last_error(None)
+    c = dns.make_params(nx=32, ny=32, nz=33, re=-1.0)
+    arr = (dns.DnsParams * 2)(a, c)
+    assert L.dns_create_batch(arr, 2, ctypes.byref(h)) == 1
+    assert b"member 1" in L.dns_last_error(None)
+    assert L.dns_create_batch(arr, 0, ctypes.byref(h)) == 1
+    d = dns.make_params(nx=32, ny=32, nz=33, re=300.0, seed=5)
+    arr = (dns.DnsParams * 2)(a, d)
+    rc = L.dns_create_batch(arr, 2, ctypes.byref(h))
+    assert rc in (0, 3)
+    if rc == 0:
+        L.dns_destroy(h)
