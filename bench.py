@@ -76,6 +76,71 @@ def cpu_baseline(kw, steps):
                       f"{dt:.2f} s wall"}
 
 
+def ensemble_variant(dns, steps, warmup, precision, autotune=True):
+    """The 64-member sweep (nx, ny in {32, 64, 96}, nz in {33, 65}, Re ~ U[100, 400], amplitude
+    ~ logU[0.005, 0.1]) advanced as one ragged ensemble, one graph replay per step."""
+    from f90_dns_amd.workloads import ensemble_members
+    mem = ensemble_members(64, precision=precision)
+    ens = dns.Ensemble(mem)
+    ens.init_fields()
+    tune = ens.autotune(steps_per_trial=5) if autotune else None
+    ens.step(warmup)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.synchronize()
+    wall = time.perf_counter() - t0
+    lat = ens.step_timing()
+    pts = sum(ens.points(i) for i in range(len(mem)))
+    finite = all(np.isfinite(ens.diagnostics(i)["energy"]) for i in range(0, len(mem), 7))
+    out = {"value": pts * steps / wall, "unit": "grid-point-steps/s", "members": len(mem), "groups": ens.groups,
+           "grid_points": pts, "steps": steps, "ms_per_step": wall / steps * 1e3,
+           "step_latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"]},
+           "hbm_gib": ens.device_bytes / 2 ** 30, "finite": bool(finite),
+           "precision": "mixed" if precision else "fp64",
+           "tuning": tune["chosen"] if tune else None}
+    ens.close()
+    return out
+
+
+def serving_variant(dns, njobs, rate, budget_gib, slots):
+    """Served sweep: seeded Poisson arrivals of mixed-precision jobs (grids up to 192x192x129,
+    nz in {65, 129}, 50-1000 steps), admitted by HBM budget, continuous batching per step."""
+    from f90_dns_amd.workloads import served_jobs
+    jobs = served_jobs(njobs, rate_per_s=rate)
+    # set-up outside the timed region: the wall-normal eigenbases of each nz (a one-time
+    # cost per process, like planning a transform)
+    for nz in sorted({p["nz"] for _, p, _ in jobs}):
+        dns.estimate_bytes(1, **[p for _, p, _ in jobs if p["nz"] == nz][0])
+    srv = dns.Server(int(budget_gib * 2 ** 30), slots_per_group=slots)
+    t0 = time.perf_counter()
+    i = 0
+    while True:
+        now = time.perf_counter() - t0
+        while i < len(jobs) and jobs[i][0] <= now:
+            srv.submit(jobs[i][2], **jobs[i][1])
+            i += 1
+        running = srv.tick()
+        if i == len(jobs) and srv.pending() == 0:
+            break
+        if running == 0 and srv.pending() == 0 and i < len(jobs):
+            time.sleep(max(0.0, min(0.01, jobs[i][0] - (time.perf_counter() - t0))))
+    srv.drain()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    turn = sorted(srv.times(j)["turnaround_s"] for j in range(1, len(jobs) + 1))
+    out = {"value": st["grid_point_steps"] / wall, "unit": "grid-point-steps/s", "jobs": len(jobs),
+           "arrival_rate_per_s": rate, "done": st["done"], "failed": st["failed"], "rejected": st["rejected"],
+           "job_steps": st["job_steps"], "ticks": st["ticks"], "wall_s": wall,
+           "device_busy_frac": st["device_ms"] * 1e-3 / wall,
+           "job_step_latency_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
+           "turnaround_s": {"p50": float(np.percentile(turn, 50)), "p99": float(np.percentile(turn, 99))},
+           "hbm_budget_gib": budget_gib, "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_running": st["max_running"],
+           "precision": "mixed"}
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +153,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=200,
                     help="steps for the mixed-precision line reported beside the FP64 value (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=100, help="steps of the 64-member ensemble lines (0 = skip)")
+    ap.add_argument("--serve-jobs", type=int, default=32, help="jobs of the served sweep (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrival rate of the served sweep (1/s)")
+    ap.add_argument("--serve-budget-gib", type=float, default=24.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the ensemble/serving "
@@ -153,6 +222,18 @@ def main():
             "kernels_ms": m.profile_step(args.profile_steps) if args.profile_steps > 0 else {},
             "precision": "FP32 transforms, product, D1 output and S storage; FP64 solves"}
         m.close()
+
+    if args.ensemble_steps > 0:
+        for prec, name in ((1, "ensemble64_mixed"), (0, "ensemble64_fp64")):
+            print(f"[bench] {name}: {args.ensemble_steps} steps", file=sys.stderr, flush=True)
+            v = ensemble_variant(dns, args.ensemble_steps, args.warmup, prec)
+            v["vs_plain_fp64_128"] = v["value"] / value
+            variants[name] = v
+    if args.serve_jobs > 0:
+        print(f"[bench] served sweep: {args.serve_jobs} jobs", file=sys.stderr, flush=True)
+        v = serving_variant(dns, args.serve_jobs, args.serve_rate, args.serve_budget_gib, 4)
+        v["vs_plain_fp64_128"] = v["value"] / value
+        variants["served_sweep_mixed"] = v
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:

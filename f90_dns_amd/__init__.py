@@ -50,6 +50,19 @@ class DnsDiag(ctypes.Structure):
     ]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("ticks", ctypes.c_longlong), ("job_steps", ctypes.c_longlong), ("grid_point_steps", ctypes.c_double),
+        ("device_ms", ctypes.c_double), ("step_p50_ms", ctypes.c_double), ("step_p99_ms", ctypes.c_double),
+        ("bytes", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t), ("max_running", ctypes.c_int),
+        ("queued", ctypes.c_int), ("running", ctypes.c_int), ("done", ctypes.c_int), ("failed", ctypes.c_int),
+        ("rejected", ctypes.c_int),
+    ]
+
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "rejected"}
+
+
 # reference defaults (read_input_3d, DNS_pressure_BC_3D.f90:571-603) where the 3D step reads them
 DEFAULTS = dict(
     nx=128, ny=32, nz=33, xlen=2 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.01,
@@ -120,6 +133,44 @@ def lib():
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.dns_member_diagnostics.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(DnsDiag)]
         L.dns_member_divergence.argtypes = [_vp, ctypes.c_int, _dp]
+        L.dns_set_member_params.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(DnsParams)]
+        L.dns_init_member_fields.argtypes = [_vp, ctypes.c_int]
+        L.dns_ensemble_create.argtypes = [ctypes.POINTER(DnsParams), ctypes.c_int, ctypes.POINTER(_vp)]
+        L.dns_ensemble_destroy.argtypes = [_vp]
+        L.dns_ensemble_last_error.argtypes = [_vp]
+        L.dns_ensemble_last_error.restype = ctypes.c_char_p
+        for n in ("dns_ensemble_members", "dns_ensemble_groups", "dns_ensemble_init_fields",
+                  "dns_ensemble_synchronize"):
+            getattr(L, n).argtypes = [_vp]
+        L.dns_ensemble_device_bytes.argtypes = [_vp]
+        L.dns_ensemble_device_bytes.restype = ctypes.c_size_t
+        L.dns_ensemble_member_shape.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp]
+        for n in ("dns_ensemble_set_member_fields", "dns_ensemble_get_member_fields"):
+            getattr(L, n).argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
+        L.dns_ensemble_member_diagnostics.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(DnsDiag)]
+        L.dns_ensemble_step.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.dns_ensemble_step_timing.argtypes = [_vp, _dp]
+        L.dns_ensemble_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
+        L.dns_ensemble_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.dns_estimate_bytes.argtypes = [ctypes.POINTER(DnsParams), ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
+        L.dns_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
+        L.dns_server_destroy.argtypes = [_vp]
+        L.dns_server_last_error.argtypes = [_vp]
+        L.dns_server_last_error.restype = ctypes.c_char_p
+        L.dns_server_submit.argtypes = [_vp, ctypes.POINTER(DnsParams), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.dns_server_tick.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+        L.dns_server_drain.argtypes = [_vp]
+        L.dns_server_job_status.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int)]
+        L.dns_server_job_message.argtypes = [_vp, ctypes.c_int]
+        L.dns_server_job_message.restype = ctypes.c_char_p
+        L.dns_server_job_result.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.POINTER(DnsDiag)]
+        L.dns_server_job_times.argtypes = [_vp, ctypes.c_int, _dp]
+        L.dns_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
+        L.dns_server_set_health_interval.argtypes = [_vp, ctypes.c_int]
+        L.dns_server_pending.argtypes = [_vp]
+        L.dns_debug_initial_field.argtypes = [ctypes.POINTER(DnsParams), _vp, _vp, _vp]
         L.dns_enqueue_step.argtypes = [_vp, ctypes.c_int]
         L.dns_query.argtypes = [_vp]
         _lib = L
@@ -287,6 +338,194 @@ class Simulation:
         out = (ctypes.c_double * 2)()
         self._check(lib().dns_debug_operator_error(self._h, out))
         return out[0], out[1]
+
+
+def host_initial_field(**kw):
+    """initialize_fields_3d evaluated point by point on the host (include/f90dns_debug.h)."""
+    p = make_params(**kw)
+    out = [np.empty((p.nz, p.ny, p.nx)) for _ in range(3)]
+    rc = lib().dns_debug_initial_field(ctypes.byref(p), *[a.ctypes.data for a in out])
+    if rc != 0:
+        raise DnsError(rc, lib().dns_last_error(None).decode())
+    return out
+
+
+def estimate_bytes(nmembers=1, **kw):
+    """HBM bytes a context of nmembers members of this shape holds (before creating it)."""
+    p = make_params(**kw)
+    out = ctypes.c_size_t()
+    rc = lib().dns_estimate_bytes(ctypes.byref(p), nmembers, ctypes.byref(out))
+    if rc != 0:
+        raise DnsError(rc, lib().dns_ensemble_last_error(None).decode())
+    return out.value
+
+
+class Ensemble:
+    """Members of different grids (include/f90dns_ensemble.h): one launch per stage and kernel
+    class for all of them, the step replayed from one captured graph.  members: list of
+    per-member parameter dicts on top of **common."""
+
+    def __init__(self, members, **common):
+        self.member_params = [make_params(**{**common, **m}) for m in members]
+        arr = (DnsParams * len(members))(*self.member_params)
+        h = _vp()
+        L = lib()
+        rc = L.dns_ensemble_create(arr, len(members), ctypes.byref(h))
+        if rc != 0:
+            raise DnsError(rc, L.dns_ensemble_last_error(None).decode())
+        self._h = h
+        self.members = len(members)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DnsError(rc, lib().dns_ensemble_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dns_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def groups(self):
+        return lib().dns_ensemble_groups(self._h)
+
+    @property
+    def device_bytes(self):
+        return lib().dns_ensemble_device_bytes(self._h)
+
+    def shape(self, m):
+        nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().dns_ensemble_member_shape(self._h, m, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        return (nz.value, ny.value, nx.value)
+
+    def points(self, m):
+        nz, ny, nx = self.shape(m)
+        return nx * ny * nz
+
+    def init_fields(self):
+        self._check(lib().dns_ensemble_init_fields(self._h))
+
+    def set_fields(self, m, u=None, v=None, w=None, p=None):
+        sh = self.shape(m)
+        arrs = [None if a is None else _arr(a, sh) for a in (u, v, w, p)]
+        self._check(lib().dns_ensemble_set_member_fields(self._h, m,
+                                                         *[None if a is None else a.ctypes.data for a in arrs]))
+
+    def get_fields(self, m):
+        out = [np.empty(self.shape(m)) for _ in range(4)]
+        self._check(lib().dns_ensemble_get_member_fields(self._h, m, *[a.ctypes.data for a in out]))
+        return out
+
+    def diagnostics(self, m):
+        d = DnsDiag()
+        self._check(lib().dns_ensemble_member_diagnostics(self._h, m, ctypes.byref(d)))
+        return {f: getattr(d, f) for f, _ in DnsDiag._fields_}
+
+    def step(self, n=1, graph=True):
+        self._check(lib().dns_ensemble_step(self._h, n, 1 if graph else 0))
+
+    def synchronize(self):
+        self._check(lib().dns_ensemble_synchronize(self._h))
+
+    def step_timing(self):
+        out = (ctypes.c_double * 4)()
+        self._check(lib().dns_ensemble_step_timing(self._h, out))
+        return dict(min_ms=out[0], p50_ms=out[1], p99_ms=out[2], mean_ms=out[3])
+
+    def set_tuning(self, lanes=1, interleave=0, promote_fp64=(0, 0, 0), promote_fp32=(0, 0, 0)):
+        a = (ctypes.c_int * 3)(*promote_fp64)
+        b = (ctypes.c_int * 3)(*promote_fp32)
+        self._check(lib().dns_ensemble_set_tuning(self._h, lanes, interleave, a, b))
+
+    def autotune(self, steps_per_trial=5):
+        import json
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(lib().dns_ensemble_autotune(self._h, steps_per_trial, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+
+class Server:
+    """Serving loop (include/f90dns_server.h): jobs admitted by HBM budget into slots of a
+    ragged ensemble, every running job advanced by each tick."""
+
+    def __init__(self, hbm_budget_bytes, slots_per_group=4, graph=True):
+        h = _vp()
+        rc = lib().dns_server_create(int(hbm_budget_bytes), slots_per_group, 1 if graph else 0, ctypes.byref(h))
+        if rc != 0:
+            raise DnsError(rc, lib().dns_ensemble_last_error(None).decode())
+        self._h = h
+        self._shape = {}
+
+    def _check(self, rc):
+        if rc != 0:
+            raise DnsError(rc, lib().dns_server_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().dns_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, nsteps, keep_fields=False, **kw):
+        p = make_params(**kw)
+        jid = ctypes.c_int()
+        self._check(lib().dns_server_submit(self._h, ctypes.byref(p), nsteps, 1 if keep_fields else 0,
+                                            ctypes.byref(jid)))
+        self._shape[jid.value] = (p.nz, p.ny, p.nx)
+        return jid.value
+
+    def tick(self):
+        n = ctypes.c_int()
+        self._check(lib().dns_server_tick(self._h, ctypes.byref(n)))
+        return n.value
+
+    def drain(self):
+        self._check(lib().dns_server_drain(self._h))
+
+    def status(self, jid):
+        st, done = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().dns_server_job_status(self._h, jid, ctypes.byref(st), ctypes.byref(done)))
+        return JOB_STATES[st.value], done.value
+
+    def message(self, jid):
+        return lib().dns_server_job_message(self._h, jid).decode()
+
+    def result(self, jid, fields=True):
+        d = DnsDiag()
+        if fields:
+            out = [np.empty(self._shape[jid]) for _ in range(4)]
+            self._check(lib().dns_server_job_result(self._h, jid, *[a.ctypes.data for a in out], ctypes.byref(d)))
+        else:
+            out = None
+            self._check(lib().dns_server_job_result(self._h, jid, None, None, None, None, ctypes.byref(d)))
+        return out, {f: getattr(d, f) for f, _ in DnsDiag._fields_}
+
+    def times(self, jid):
+        out = (ctypes.c_double * 5)()
+        self._check(lib().dns_server_job_times(self._h, jid, out))
+        return dict(queue_s=out[0], turnaround_s=out[1], step_p50_ms=out[2], step_p99_ms=out[3], steps=int(out[4]))
+
+    def stats(self):
+        st = ServerStats()
+        self._check(lib().dns_server_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def pending(self):
+        return lib().dns_server_pending(self._h)
+
+    def set_health_interval(self, ticks):
+        self._check(lib().dns_server_set_health_interval(self._h, ticks))
 
 
 class FFTPlan:

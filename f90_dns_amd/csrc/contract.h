@@ -119,12 +119,12 @@ __device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
         for (int ct = 0; ct < CT; ct++) acc[i][ct] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
-// Output coordinates of acc[i][ct][r]: row = (wave + 4i)*16 + lane/16 + 4r, column =
-// blockIdx.x*BCOLS + ct*16 + lane%16.
+// Output coordinates of acc[i][ct][r] in the block's column tile cb: row = (wave + 4i)*16 +
+// lane/16 + 4r, column = cb*BCOLS + ct*16 + lane%16.
 __device__ __forceinline__ int out_row(int i, int r) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     return (wave + 4 * i) * 16 + (lane >> 4) + 4 * r;
 }
-__device__ __forceinline__ int out_col(int ct) { return blockIdx.x * BCOLS + ct * 16 + (threadIdx.x & 15); }
+__device__ __forceinline__ int out_col(int ct, int cb) { return cb * BCOLS + ct * 16 + (threadIdx.x & 15); }
 
 }  // namespace dns

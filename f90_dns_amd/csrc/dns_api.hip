@@ -23,6 +23,7 @@
 
 #include "../../include/f90dns.h"
 #include "../../include/f90dns_debug.h"
+#include "context.h"
 #include "kernels.h"
 #include "setup.h"
 
@@ -33,48 +34,6 @@ thread_local std::string g_create_error;
 }
 
 void dns_set_create_error(const std::string& msg) { g_create_error = msg; }
-
-struct dns_ctx {
-    dns_params p;                 // member 0 (the shape fields are equal for every member)
-    std::vector<dns_params> mp;   // every member
-    int B = 1;
-    int nx, ny, nz, nxh, H2, npl;
-    size_t N, NS;                 // per member: physical doubles, spectral doubles
-    double theta, ybar;
-    int rb, rbi;
-    bool mixed = false;           // FP32 transforms, product and omega/S storage; FP64 solves
-    hipStream_t stream = nullptr;
-    FFTGeom g;
-    Operators op;                 // member 0; the Re-independent parts are shared
-    size_t hf_stride = 0;         // doubles per member of the stacked Helmholtz forward operator
-    // constants
-    float *twx32 = nullptr, *twy32 = nullptr;
-    double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *e0 = nullptr, *f0 = nullptr;
-    unsigned char* mask = nullptr;
-    double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr, *fvec = nullptr;
-    double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
-    // fields, B members each
-    double *uh = nullptr, *vh = nullptr, *wh = nullptr, *ph = nullptr;
-    double *us = nullptr, *vs = nullptr, *ws = nullptr;
-    double *ox = nullptr, *oy = nullptr, *sh[3] = {nullptr, nullptr, nullptr};
-    double *phys[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    double *phi = nullptr, *dphi = nullptr, *tmp = nullptr;
-    FlowState* flow = nullptr;
-    FlowMember* fmem = nullptr;
-    double *red_partial = nullptr, *red_result = nullptr;
-    int red_blocks = 512;
-    std::vector<void*> allocs;
-    size_t bytes = 0;
-    double time = 0.0;
-    int step = 0;
-    std::string err;
-    // timing
-    std::vector<hipEvent_t> ev;
-    std::vector<double> step_ms;
-    // graph of one step
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-};
 
 static int fail(dns_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg; else g_create_error = msg;
@@ -266,6 +225,7 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
         if ((r = dalloc(c, q, NSB))) return bail(r);
     for (int f = 0; f < 6; f++)
         if ((r = dalloc(c, &c->phys[f], NB))) return bail(r);
+    if (c->g.cls64 < 0 && (r = dalloc(c, &c->io, c->NS))) return bail(r);
     if ((r = dalloc(c, &c->flow, c->B)) || (r = dalloc(c, &c->fmem, c->B)) ||
         (r = dalloc(c, &c->red_partial, size_t(c->red_blocks) * 5)) || (r = dalloc(c, &c->red_result, 5)))
         return bail(r);
@@ -306,8 +266,13 @@ static FFTArgs fft_args(dns_ctx* c) {
     return a;
 }
 
-// transforms of nplanes consecutive planes (device pointers, FP64 storage)
+// transforms of nplanes <= nz consecutive planes (device pointers, FP64 storage, FP64
+// arithmetic; two passes through c->io where the plane does not fit the LDS in FP64)
 static int r2c_planes(dns_ctx* c, const double* src, double* dst, int nplanes) {
+    if (c->g.cls64 < 0) {
+        HIPCHECK(c, launch_two_pass(c->g, true, src, dst, c->io, nplanes, c->npl, c->H2, c->stream));
+        return 0;
+    }
     FFTArgs a = fft_args(c);
     a.src[0] = src; a.dst[0] = dst;
     a.sstride = c->npl; a.dstride = c->H2;
@@ -315,6 +280,10 @@ static int r2c_planes(dns_ctx* c, const double* src, double* dst, int nplanes) {
     return 0;
 }
 static int c2r_planes(dns_ctx* c, const double* src, double* dst, int nplanes) {
+    if (c->g.cls64 < 0) {
+        HIPCHECK(c, launch_two_pass(c->g, false, src, dst, c->io, nplanes, c->H2, c->npl, c->stream));
+        return 0;
+    }
     FFTArgs a = fft_args(c);
     a.src[0] = src; a.dst[0] = dst;
     a.sstride = c->H2; a.dstride = c->npl;
@@ -386,17 +355,12 @@ int dns_get_fields_device(dns_ctx* c, double* u, double* v, double* w, double* p
     return 0;
 }
 
+int dns_init_member_fields(dns_ctx* c, int b);
+
 int dns_init_fields(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    std::vector<double> zero(c->N, 0.0);
     for (int b = 0; b < c->B; b++) {
-        const dns_params& q = c->mp[b];
-        InitSpec s{c->nx, c->ny, c->nz, q.xlen, q.ylen, q.u_centerline, q.enable_perturbations,
-                   q.perturbation_kind, q.analytical_mode_x, q.analytical_mode_y, q.perturbation_amplitude,
-                   (uint64_t)q.seed, q.noise_kmax};
-        std::vector<double> u, v, w;
-        initial_field(s, u, v, w);
-        int r = dns_set_member_fields(c, b, u.data(), v.data(), w.data(), zero.data());
+        int r = dns_init_member_fields(c, b);
         if (r) return r;
     }
     c->time = 0.0; c->step = 0;
@@ -413,6 +377,70 @@ int dns_set_member_controller(dns_ctx* c, int b, double G, double integral, doub
     return 0;
 }
 
+int dns_set_member_params(dns_ctx* c, int b, const dns_params* q) {
+    if (bad_member(c, b) || !q) return DNS_EINVAL;
+    const std::string e = check_params(q);
+    if (!e.empty()) return fail(c, DNS_EINVAL, e);
+    if (!same_shape(c->p, *q)) return fail(c, DNS_EINVAL, "member parameters differ in grid, dt, domain or scheme flags");
+    Operators ob;
+    std::string err;
+    if (!build_operators(c->nz, c->p.dt, q->re, c->theta, c->p.use_crank_nicolson != 0, c->p.lgl_derivative != 0, ob,
+                         err))
+        return fail(c, DNS_ESOLVER, err);
+    const int m = c->nz - 2;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipMemcpy(c->hF + size_t(b) * c->hf_stride, ob.hF.data.data(), c->hf_stride * sizeof(double),
+                          hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->hc1 + size_t(b) * m, ob.hc1.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->hc2 + size_t(b) * m, ob.hc2.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->hl + size_t(b) * m, ob.hl.data(), m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->fvec + b, &ob.f, sizeof(double), hipMemcpyHostToDevice));
+    FlowState fs{};
+    fs.G = q->target_pressure_gradient;
+    fs.gavg = q->target_pressure_gradient;
+    FlowMember fm{q->target_pressure_gradient, q->target_bulk_velocity, q->controller_gain, q->re,
+                  q->flow_control_method, q->controller_update_freq, q->istart, 0};
+    HIPCHECK(c, hipMemcpy(c->flow + b, &fs, sizeof(fs), hipMemcpyHostToDevice));
+    HIPCHECK(c, hipMemcpy(c->fmem + b, &fm, sizeof(fm), hipMemcpyHostToDevice));
+    c->mp[b] = *q;
+    if (b == 0) c->p = *q;
+    return 0;
+}
+
+// initialize_fields_3d for one member: the separable form is built on the host (planar
+// field, wall-normal profile, amplitude scale), the 3D fields on the device
+int dns_init_member_fields(dns_ctx* c, int b) {
+    if (bad_member(c, b)) return DNS_EINVAL;
+    const dns_params& q = c->mp[b];
+    InitSpec s{c->nx, c->ny, c->nz, q.xlen, q.ylen, q.u_centerline, q.enable_perturbations,
+               q.perturbation_kind, q.analytical_mode_x, q.analytical_mode_y, q.perturbation_amplitude,
+               (uint64_t)q.seed, q.noise_kmax};
+    PlanarInit pl;
+    initial_planar(s, pl);
+    // staging in the physical work buffer 3 (B*N >= 2*npl + 2*nz doubles)
+    double* st = c->phys[3];
+    const size_t npl = size_t(c->npl);
+    HIPCHECK(c, hipMemcpyAsync(st, pl.u2.data(), npl * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(st + npl, pl.v2.data(), npl * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(st + 2 * npl, pl.U.data(), c->nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(st + 2 * npl + c->nz, pl.wd.data(), c->nz * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+    InitArgs a{};
+    a.npl = c->npl; a.nz = c->nz;
+    a.u2 = st; a.v2 = st + npl; a.U = st + 2 * npl; a.wd = st + 2 * npl + c->nz;
+    a.sf = pl.sf; a.scaled = pl.scaled; a.perturbed = pl.perturbed;
+    a.u = c->phys[0]; a.v = c->phys[1]; a.w = c->phys[2];
+    HIPCHECK(c, launch_init_fields(a, c->stream));
+    const size_t mo = size_t(b) * c->NS;
+    int r;
+    if ((r = r2c_planes(c, c->phys[0], c->uh + mo, c->nz)) || (r = r2c_planes(c, c->phys[1], c->vh + mo, c->nz)) ||
+        (r = r2c_planes(c, c->phys[2], c->wh + mo, c->nz)))
+        return r;
+    HIPCHECK(c, hipMemsetAsync(c->ph + mo, 0, c->NS * sizeof(double), c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int dns_set_controller(dns_ctx* c, double G, double integral, double prev, double gsum, int gcount) {
     return dns_set_member_controller(c, 0, G, integral, prev, gsum, gcount);
 }
@@ -425,7 +453,7 @@ int dns_begin_step(dns_ctx* c) {
     return 0;
 }
 
-static FlowArgs flow_args(dns_ctx* c) {
+extern "C++" FlowArgs dns::ctx_flow_args(dns_ctx* c) {
     FlowArgs fa{};
     fa.nz = c->nz; fa.dt = c->p.dt; fa.nxny = double(c->nx * c->ny);
     double ws = 0.0;
@@ -436,7 +464,7 @@ static FlowArgs flow_args(dns_ctx* c) {
     return fa;
 }
 
-static int nl_zderiv(dns_ctx* c) {
+extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
     ZArgs z{};
     z.mstride = (long long)c->NS;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
@@ -445,22 +473,30 @@ static int nl_zderiv(dns_ctx* c) {
     z.out[0] = c->ox; z.out[1] = c->oy;
     z.xw = c->xw; z.yw = c->yw;
     z.out_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, z, 1, c->B, c->stream));
+    return z;
+}
+
+static int nl_zderiv(dns_ctx* c) {
+    HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, ctx_vort_args(c), 1, c->B, c->stream));
     return 0;
 }
 
-static int nl_c2r(dns_ctx* c) {
+extern "C++" FFTArgs dns::ctx_c2r_nl_args(dns_ctx* c) {
     FFTArgs a = fft_args(c);
     a.src[0] = c->uh; a.src[1] = c->vh; a.src[2] = c->wh; a.src[3] = c->ox; a.src[4] = c->oy;
     for (int f = 0; f < 6; f++) a.dst[f] = c->phys[f];
     a.sstride = c->H2; a.dstride = c->npl;
     a.src_f32 = c->mixed ? 0x18u : 0u;
     a.dst_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, c->nz * c->B, 6, c->mixed, c->stream));
+    return a;
+}
+
+static int nl_c2r(dns_ctx* c) {
+    HIPCHECK(c, launch_c2r(c->g, C2R_NL, ctx_c2r_nl_args(c), c->nz * c->B, 6, c->mixed, c->stream));
     return 0;
 }
 
-static int nl_r2c(dns_ctx* c) {
+extern "C++" FFTArgs dns::ctx_r2c_cross_args(dns_ctx* c) {
     FFTArgs b = fft_args(c);
     for (int f = 0; f < 6; f++) b.src[f] = c->phys[f];
     for (int f = 0; f < 3; f++) b.dst[f] = c->sh[f];
@@ -468,13 +504,16 @@ static int nl_r2c(dns_ctx* c) {
     b.sstride = c->npl; b.dstride = c->H2;
     b.src_f32 = c->mixed ? 0x3Fu : 0u;
     b.dst_f32 = c->mixed ? 1 : 0;
-    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, b, c->nz * c->B, 3, c->mixed, c->stream));
+    return b;
+}
+
+static int nl_r2c(dns_ctx* c) {
+    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, ctx_r2c_cross_args(c), c->nz * c->B, 3, c->mixed, c->stream));
     return 0;
 }
 
 static int flow_control(dns_ctx* c) {
-    FlowArgs fa = flow_args(c);
-    HIPCHECK(c, launch_flow_control(fa, c->B, c->stream));
+    HIPCHECK(c, launch_flow_control(ctx_flow_args(c), c->B, c->stream));
     return 0;
 }
 
@@ -485,8 +524,7 @@ int dns_compute_source_terms(dns_ctx* c) {
     return 0;
 }
 
-int dns_viscous_step(dns_ctx* c) {
-    if (!c) return DNS_EINVAL;
+extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
     HelmArgs h{};
     h.mstride = (long long)c->NS; h.fstride = (long long)c->hf_stride;
     h.nz = c->nz; h.m = c->nz - 2; h.H2 = c->H2; h.nxh = c->nxh; h.ny = c->ny; h.nx = c->nx;
@@ -501,11 +539,16 @@ int dns_viscous_step(dns_ctx* c) {
     h.ph = c->ph;
     h.s_f32 = c->mixed ? 1 : 0;
     h.flow = c->flow;
-    HIPCHECK(c, launch_helmholtz(c->rbi, h, c->B, c->stream));
+    return h;
+}
+
+int dns_viscous_step(dns_ctx* c) {
+    if (!c) return DNS_EINVAL;
+    HIPCHECK(c, launch_helmholtz(c->rbi, ctx_helm_args(c), c->B, c->stream));
     return 0;
 }
 
-static PoisArgs poisson_args(dns_ctx* c, int fused) {
+extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
     PoisArgs a{};
     a.mstride = (long long)c->NS;
     a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
@@ -517,7 +560,7 @@ static PoisArgs poisson_args(dns_ctx* c, int fused) {
     return a;
 }
 
-static Mode0Args mode0_args(dns_ctx* c, int fused) {
+extern "C++" Mode0Args dns::ctx_mode0_args(dns_ctx* c, int fused) {
     Mode0Args m{};
     m.mstride = (long long)c->NS;
     m.nz = c->nz; m.H2 = c->H2; m.e0 = c->e0; m.f0 = c->f0; m.dt = c->p.dt; m.fused = fused;
@@ -528,9 +571,9 @@ static Mode0Args mode0_args(dns_ctx* c, int fused) {
 }
 
 static int pressure(dns_ctx* c, int fused) {
-    PoisArgs a = poisson_args(c, fused);
+    PoisArgs a = ctx_pois_args(c, fused);
     HIPCHECK(c, launch_poisson(c->rb, a, c->B, c->stream));
-    Mode0Args m = mode0_args(c, fused);
+    Mode0Args m = ctx_mode0_args(c, fused);
     HIPCHECK(c, launch_poisson_mode0(m, c->B, c->stream));
     return 0;
 }
@@ -738,10 +781,17 @@ int dns_member_divergence(dns_ctx* c, int b, double* max_div) {
     z.Asw = c->D1sw; z.KS = c->op.D1.ks;
     z.in[0] = c->wh + mo; z.out[0] = c->tmp; z.xw = c->xw; z.yw = c->yw;
     HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, 1, c->stream));
-    FFTArgs a = fft_args(c);
-    a.src[0] = c->uh + mo; a.src[1] = c->vh + mo; a.src[2] = c->tmp; a.dst[0] = c->phys[0];
-    a.sstride = c->H2; a.dstride = c->npl;
-    HIPCHECK(c, launch_c2r(c->g, C2R_DIV, a, c->nz, 1, false, c->stream));
+    if (c->g.cls64 < 0) {
+        // combine into the omega_x buffer (free outside dns_compute_source_terms), then two passes
+        HIPCHECK(c, launch_div_spec(c->g, c->xw, c->yw, c->uh + mo, c->vh + mo, c->tmp, c->ox, c->nz, c->stream));
+        int r = c2r_planes(c, c->ox, c->phys[0], c->nz);
+        if (r) return r;
+    } else {
+        FFTArgs a = fft_args(c);
+        a.src[0] = c->uh + mo; a.src[1] = c->vh + mo; a.src[2] = c->tmp; a.dst[0] = c->phys[0];
+        a.sstride = c->H2; a.dstride = c->npl;
+        HIPCHECK(c, launch_c2r(c->g, C2R_DIV, a, c->nz, 1, false, c->stream));
+    }
     double res[5];
     int r = reduce(c, nullptr, nullptr, nullptr, c->phys[0], res);
     if (r) return r;
@@ -803,10 +853,10 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
         HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
         if ((r = dns_viscous_step(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
-        PoisArgs pa = poisson_args(c, 1);
+        PoisArgs pa = ctx_pois_args(c, 1);
         HIPCHECK(c, launch_poisson(c->rb, pa, c->B, c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
-        Mode0Args ma = mode0_args(c, 1);
+        Mode0Args ma = ctx_mode0_args(c, 1);
         HIPCHECK(c, launch_poisson_mode0(ma, c->B, c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));
         HIPCHECK(c, hipEventSynchronize(c->ev[7]));
@@ -835,6 +885,21 @@ static int run_probe(double* out, hipError_t (*launch)(double*, hipStream_t)) {
 
 int dns_debug_mfma_probe(double* out) { return run_probe(out, launch_mfma_probe); }
 int dns_debug_mfma_probe_f32(double* out) { return run_probe(out, launch_mfma_probe_f32); }
+
+int dns_debug_initial_field(const dns_params* q, double* u, double* v, double* w) {
+    if (!q || !u || !v || !w) return DNS_EINVAL;
+    const std::string e = check_params(q);
+    if (!e.empty()) { g_create_error = e; return DNS_EINVAL; }
+    InitSpec s{q->nx, q->ny, q->nz, q->xlen, q->ylen, q->u_centerline, q->enable_perturbations,
+               q->perturbation_kind, q->analytical_mode_x, q->analytical_mode_y, q->perturbation_amplitude,
+               (uint64_t)q->seed, q->noise_kmax};
+    std::vector<double> a, b, c;
+    initial_field(s, a, b, c);
+    std::memcpy(u, a.data(), a.size() * sizeof(double));
+    std::memcpy(v, b.data(), b.size() * sizeof(double));
+    std::memcpy(w, c.data(), c.size() * sizeof(double));
+    return 0;
+}
 
 static void unswizzle(const SwizzledA& a, std::vector<double>& m) {
     m.assign(size_t(a.rows) * a.cols, 0.0);

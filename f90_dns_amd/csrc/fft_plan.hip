@@ -24,7 +24,8 @@ struct dns_fft_plan {
     double *twx = nullptr, *twy = nullptr;
     float *twx32 = nullptr, *twy32 = nullptr;
     double *din = nullptr, *dout = nullptr;
-    size_t cap_in = 0, cap_out = 0;
+    double* scr = nullptr;       // scratch of the two-pass FP64 transforms
+    size_t cap_in = 0, cap_out = 0, cap_scr = 0;
     hipStream_t stream = nullptr;
 };
 
@@ -48,7 +49,8 @@ int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
     if (!dns_check_gfx950(why)) { dns_set_create_error(why); return DNS_EDEVICE; }
     dns_fft_plan* p = new dns_fft_plan();
     p->f32 = precision == DNS_PREC_MIXED;
-    if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0) || !dns_make_factors(ny, p->g.fy) ||
+    if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0 && (nx / 2 > 256 || ny > 256)) ||
+        !dns_make_factors(ny, p->g.fy) ||
         !dns_make_factors(nx / 2, p->g.fx)) {
         delete p;
         dns_set_create_error("unsupported plane size");
@@ -86,6 +88,7 @@ int dns_fft_plan_destroy(dns_fft_plan* p) {
     (void)hipFree(p->twx); (void)hipFree(p->twy); (void)hipFree(p->twx32); (void)hipFree(p->twy32);
     if (p->din) (void)hipFree(p->din);
     if (p->dout) (void)hipFree(p->dout);
+    if (p->scr) (void)hipFree(p->scr);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return 0;
@@ -98,6 +101,11 @@ int dns_fft_forward_device(dns_fft_plan* p, const double* in, double* out, int n
     a.src[0] = in; a.dst[0] = out;
     a.sstride = (long long)p->g.nx * p->g.ny; a.dstride = 2LL * p->g.nxh * p->g.ny;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    if (!p->f32 && p->g.cls64 < 0) {
+        if (plan_buf(&p->scr, p->cap_scr, size_t(a.dstride) * nplanes)) return DNS_ENOMEM;
+        return launch_two_pass(p->g, true, in, out, p->scr, nplanes, a.sstride, a.dstride, s) == hipSuccess ? 0
+                                                                                                       : DNS_EDEVICE;
+    }
     return launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
@@ -108,6 +116,11 @@ int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int 
     a.src[0] = in; a.dst[0] = out;
     a.sstride = 2LL * p->g.nxh * p->g.ny; a.dstride = (long long)p->g.nx * p->g.ny;
     hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    if (!p->f32 && p->g.cls64 < 0) {
+        if (plan_buf(&p->scr, p->cap_scr, size_t(a.sstride) * nplanes)) return DNS_ENOMEM;
+        return launch_two_pass(p->g, false, in, out, p->scr, nplanes, a.sstride, a.dstride, s) == hipSuccess ? 0
+                                                                                                        : DNS_EDEVICE;
+    }
     return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
 }
 

@@ -6,6 +6,14 @@
 
 namespace dns {
 
+// Ragged launches: one launch serves tiles of several contexts ("groups") of different shape.
+// Block b of a ragged launch works on tile map[b] of group map[b].g, whose argument block is
+// descs[map[b].g]; x = plane (plane kernels), column block (z-kernels) or row (mode (0,0)),
+// z = member within the group.
+struct Tile {
+    int g, x, z;
+};
+
 // ---- plane FFT kernels ------------------------------------------------------------------
 enum C2ROp { C2R_PLAIN = 0, C2R_NL = 1, C2R_DIV = 2 };
 enum R2COp { R2C_PLAIN = 0, R2C_CROSS = 1 };
@@ -38,11 +46,42 @@ struct FFTArgs {
 };
 
 bool fft_geom(int nx, int ny, FFTGeom& g);
+// fill the geometry fields of a (shape, factors, twiddles of the compute precision)
+void fft_fill(const FFTGeom& g, FFTArgs& a, bool f32);
+// max complex elements of the half plane for size class c (-1: no such class for f32)
+int fft_class_capacity(int c, bool f32);
 // raise the dynamic-LDS limit of the plane kernels of g's size classes (once per process)
 hipError_t fft_prepare(const FFTGeom& g);
 // grid (nplanes, nfields); f32: compute in FP32 (else FP64)
 hipError_t launch_c2r(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, bool f32, hipStream_t s);
 hipError_t launch_r2c(const FFTGeom& g, int op, FFTArgs a, int nplanes, int nfields, bool f32, hipStream_t s);
+// ragged: every descriptor already filled (fft_fill) for size class cls; lds = dynamic LDS bytes
+hipError_t launch_c2r_rag(int cls, int op, bool f32, const FFTArgs* descs, const Tile* map, int ntiles, int nfields,
+                          size_t lds, hipStream_t s);
+hipError_t launch_r2c_rag(int cls, int op, bool f32, const FFTArgs* descs, const Tile* map, int ntiles, int nfields,
+                          size_t lds, hipStream_t s);
+// raise the dynamic-LDS limit of every plane-kernel class (once per process)
+hipError_t fft_prepare_all();
+
+// Two-pass FP64 transforms of nplanes planes (any plane size with nx/2, ny <= 256): rows and
+// columns staged in LDS separately, through scratch (nplanes * 2 (nx/2+1) ny doubles).
+// forward: real (stride sstride) -> half-complex (stride dstride), unnormalised; inverse:
+// half-complex -> real divided by nx*ny.  Strides in doubles.
+struct TwoPassArgs {
+    int nx, ny, nxh, M;
+    Factors fx, fy;
+    const double2* twx;
+    const double2* twy;
+    const double* src;
+    double* dst;
+    long long sstride, dstride;
+    double scale;
+    const unsigned char* mask;
+};
+hipError_t launch_two_pass(const FFTGeom& g, bool forward, const double* src, double* dst, double* scratch,
+                           int nplanes, long long sstride, long long dstride, hipStream_t s);
+hipError_t launch_div_spec(const FFTGeom& g, const double* xw, const double* yw, const double* u, const double* v,
+                           const double* dw, double* out, int nplanes, hipStream_t s);
 
 // ---- wall-normal contractions and per-mode solves ---------------------------------------
 // Batched contexts hold B members of one shape; member b's spectral field starts at
@@ -136,6 +175,23 @@ struct FlowArgs {
     FlowState* st;
 };
 hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
+
+// ragged variants of the step kernels (rbw = row blocks per wave, (rows/16 + 3)/4)
+hipError_t launch_zderiv_rag(int rbw, int mode, const ZArgs* d, const Tile* map, int ntiles, int nf, hipStream_t s);
+hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int ntiles, hipStream_t s);
+hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int ntiles, hipStream_t s);
+hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int ntiles, hipStream_t s);
+hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int ntiles, hipStream_t s);
+
+// physical initial fields of one member from the separable form (setup.h PlanarInit):
+// u = U[k] + scale(u2 wd[k]), v = scale(v2 wd[k]), w = 0, walls zero
+struct InitArgs {
+    long long npl; int nz;
+    const double* u2; const double* v2; const double* U; const double* wd;
+    double sf; int scaled, perturbed;
+    double* u; double* v; double* w;
+};
+hipError_t launch_init_fields(const InitArgs& a, hipStream_t s);
 
 // reductions over physical fields: max|u|,|v|,|w|, sum 0.5|u|^2 w_k, max|d| (any may be null)
 struct ReduceArgs {

@@ -6,6 +6,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <cstdint>
+#include <memory>
 #include <tuple>
 
 namespace dns {
@@ -228,8 +230,17 @@ static bool invert(int n, std::vector<ld> a, std::vector<ld>& inv) {
     return true;
 }
 
-bool build_operators(int nz, double dt, double re, double theta, bool cn, bool exact_d1, Operators& op,
-                     std::string& err) {
+// Everything but the Re-dependent scaling of the Helmholtz operator; cached per (nz, dt, D1
+// variant).  The Helmholtz parts are kept unscaled: T, T D2_I, the lifting sums and the
+// eigenvalues, in long double as they enter the scaled entries.
+struct OpBase {
+    Operators op;                       // every Re-independent field filled; hF, hc1, hc2, hl, f empty
+    std::vector<double> FT;             // m x m: T
+    std::vector<ld> TD2, c1, c2, lam;   // m x nz, m, m, m
+};
+
+static bool build_base(int nz, double dt, bool exact_d1, OpBase& base, std::string& err) {
+    Operators& op = base.op;
     if (nz < 3) { err = "nz must be >= 3"; return false; }
     op.nz = nz; op.m = nz - 2;
     const int m = op.m;
@@ -265,9 +276,7 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
         op.D1 = swizzle_a(rm, nz, nz);
     }
 
-    // ---- Helmholtz ----
-    const ld f = (ld)theta * (ld)dt / (ld)re;
-    op.f = double(f);
+    // ---- Helmholtz (unscaled) ----
     {
         std::vector<ld> At(size_t(m) * m), Q, lam;
         for (int i = 0; i < m; i++)
@@ -281,30 +290,28 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
             e = std::max(e, std::fabs(r - A0[size_t(i) * m + j])); s = std::max(s, std::fabs(A0[size_t(i) * m + j]));
         }
         op.helm_recon_err = double(s > 0 ? e / s : e);
-        const int K = m + nz;
-        std::vector<double> F(size_t(m) * K, 0.0), B(size_t(m) * m, 0.0);
-        op.hc1.assign(m, 0.0); op.hc2.assign(m, 0.0); op.hl.assign(m, 0.0);
+        base.FT.assign(size_t(m) * m, 0.0);
+        base.TD2.assign(size_t(m) * nz, 0.0L);
+        base.c1.assign(m, 0.0L); base.c2.assign(m, 0.0L);
+        base.lam = lam;
+        std::vector<double> B(size_t(m) * m, 0.0);
         for (int j = 0; j < m; j++) {
             // T(j, i) = Q(i, j) sqrt(w_i)
-            for (int i = 0; i < m; i++) F[size_t(j) * K + i] = double(Q[size_t(i) * m + j] * sw[i + 1]);
-            if (cn) {
-                for (int c = 0; c < nz; c++) {          // f * (T D2_I)(j, c)
-                    ld acc = 0;
-                    for (int i = 0; i < m; i++) acc += Q[size_t(i) * m + j] * sw[i + 1] * D2(i + 1, c);
-                    F[size_t(j) * K + m + c] = double(f * acc);
-                }
+            for (int i = 0; i < m; i++) base.FT[size_t(j) * m + i] = double(Q[size_t(i) * m + j] * sw[i + 1]);
+            for (int c = 0; c < nz; c++) {              // (T D2_I)(j, c)
+                ld acc = 0;
+                for (int i = 0; i < m; i++) acc += Q[size_t(i) * m + j] * sw[i + 1] * D2(i + 1, c);
+                base.TD2[size_t(j) * nz + c] = acc;
             }
             ld c1 = 0, c2 = 0;
             for (int i = 0; i < m; i++) {
                 c1 += Q[size_t(i) * m + j] * isw[i + 1] * A(i + 1, 0);
                 c2 += Q[size_t(i) * m + j] * isw[i + 1] * A(i + 1, nz - 1);
             }
-            op.hc1[j] = double(-2.0L * f * c1);
-            op.hc2[j] = double(-2.0L * f * c2);
-            op.hl[j] = double(f * lam[j] + 1.0L);
+            base.c1[j] = c1;
+            base.c2[j] = c2;
             for (int i = 0; i < m; i++) B[size_t(i) * m + j] = double(isw[i + 1] * Q[size_t(i) * m + j]);
         }
-        op.hF = swizzle_a(F, m, K);
         op.hB = swizzle_a(B, m, m);
     }
 
@@ -388,6 +395,47 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
                 op.f0[size_t(k) * nz + q] = double(acc);
             }
     }
+    return true;
+}
+
+static std::mutex g_base_mu;
+static std::map<std::tuple<int, uint64_t, int>, std::shared_ptr<const OpBase>> g_base;
+
+bool build_operators(int nz, double dt, double re, double theta, bool cn, bool exact_d1, Operators& op,
+                     std::string& err) {
+    if (nz < 3) { err = "nz must be >= 3"; return false; }
+    uint64_t dtbits;
+    std::memcpy(&dtbits, &dt, sizeof(dt));
+    const auto key = std::make_tuple(nz, dtbits, exact_d1 ? 1 : 0);
+    std::shared_ptr<const OpBase> base;
+    {
+        std::lock_guard<std::mutex> lk(g_base_mu);
+        auto it = g_base.find(key);
+        if (it != g_base.end()) base = it->second;
+    }
+    if (!base) {
+        auto b = std::make_shared<OpBase>();
+        if (!build_base(nz, dt, exact_d1, *b, err)) return false;
+        std::lock_guard<std::mutex> lk(g_base_mu);
+        g_base[key] = b;
+        base = b;
+    }
+    op = base->op;
+    const int m = op.m, K = m + nz;
+    // Helmholtz scaling by f = theta dt / Re (solve_viscous_helmholtz, DNS_pressure_BC_3D.f90:1425-1521)
+    const ld f = (ld)theta * (ld)dt / (ld)re;
+    op.f = double(f);
+    std::vector<double> F(size_t(m) * K, 0.0);
+    op.hc1.assign(m, 0.0); op.hc2.assign(m, 0.0); op.hl.assign(m, 0.0);
+    for (int j = 0; j < m; j++) {
+        for (int i = 0; i < m; i++) F[size_t(j) * K + i] = base->FT[size_t(j) * m + i];
+        if (cn)
+            for (int c = 0; c < nz; c++) F[size_t(j) * K + m + c] = double(f * base->TD2[size_t(j) * nz + c]);
+        op.hc1[j] = double(-2.0L * f * base->c1[j]);
+        op.hc2[j] = double(-2.0L * f * base->c2[j]);
+        op.hl[j] = double(f * base->lam[j] + 1.0L);
+    }
+    op.hF = swizzle_a(F, m, K);
     return true;
 }
 
@@ -483,6 +531,71 @@ void initial_field(const InitSpec& p, std::vector<double>& u, std::vector<double
     for (size_t q = 0; q < npl; q++) {
         u[q] = v[q] = w[q] = 0.0;
         u[npl * (nz - 1) + q] = v[npl * (nz - 1) + q] = w[npl * (nz - 1) + q] = 0.0;
+    }
+}
+
+void initial_planar(const InitSpec& p, PlanarInit& o) {
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const size_t npl = size_t(nx) * ny;
+    std::vector<double> z, wz;
+    lgl_nodes_weights(nz, z, wz);
+    o.U.assign(nz, 0.0);
+    o.wd.assign(nz, 0.0);
+    for (int k = 0; k < nz; k++) {
+        o.U[k] = p.u_centerline * (1.0 - z[k] * z[k]);
+        o.wd[k] = 1.0 - z[k] * z[k];
+    }
+    o.u2.assign(npl, 0.0);
+    o.v2.assign(npl, 0.0);
+    o.perturbed = p.enable_perturbations ? 1 : 0;
+    o.scaled = 0;
+    o.sf = 1.0;
+    if (!p.enable_perturbations) return;
+    if (p.kind == 0) {
+        // -(ky sin cos wd) = (-(ky sin cos)) wd and kx cos sin wd = (kx cos sin) wd, exactly
+        const double kx = 2.0 * kPi * double(p.mode_x) / p.xlen;
+        const double ky = 2.0 * kPi * double(p.mode_y) / p.ylen;
+        for (int j = 0; j < ny; j++) {
+            const double y = double(j) * p.ylen / double(ny);
+            for (int i = 0; i < nx; i++) {
+                const double x = double(i) * p.xlen / double(nx);
+                o.u2[i + size_t(nx) * j] = -(ky * std::sin(kx * x) * std::cos(ky * y));
+                o.v2[i + size_t(nx) * j] = kx * std::cos(kx * x) * std::sin(ky * y);
+            }
+        }
+    } else {
+        uint64_t st = p.seed;
+        for (int mx = 0; mx <= p.kmax; mx++)
+            for (int my = 0; my <= p.kmax; my++) {
+                if (mx == 0 && my == 0) continue;
+                const double ph1 = 2.0 * kPi * u01(st), ph2 = 2.0 * kPi * u01(st);
+                const double a = 1.0 / double(mx * mx + my * my);
+                const double kx = 2.0 * kPi * double(mx) / p.xlen, ky = 2.0 * kPi * double(my) / p.ylen;
+                for (int j = 0; j < ny; j++) {
+                    const double y = double(j) * p.ylen / double(ny);
+                    const double sy = std::sin(ky * y + ph2), cy = std::cos(ky * y + ph2);
+                    for (int i = 0; i < nx; i++) {
+                        const double x = double(i) * p.xlen / double(nx);
+                        const double sx = std::sin(kx * x + ph1), cx = std::cos(kx * x + ph1);
+                        o.u2[i + size_t(nx) * j] += -(a * ky * sx * cy);
+                        o.v2[i + size_t(nx) * j] += a * kx * cx * sy;
+                    }
+                }
+            }
+    }
+    // max over all points of |(u2 wd, v2 wd)|: every rounded operation is monotone in wd >= 0,
+    // so the maximum sits on the plane of largest wd
+    double wmax = 0.0;
+    for (int k = 0; k < nz; k++) wmax = std::max(wmax, o.wd[k]);
+    double cm = 0.0;
+    for (size_t q = 0; q < npl; q++) {
+        const double a = o.u2[q] * wmax, b = o.v2[q] * wmax;
+        const double s = std::sqrt(a * a + b * b + 0.0 * 0.0);
+        if (s > cm) cm = s;
+    }
+    if (cm > 1.0e-12) {
+        o.scaled = 1;
+        o.sf = p.amplitude / cm;
     }
 }
 

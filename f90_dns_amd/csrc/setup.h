@@ -84,4 +84,15 @@ struct InitSpec {
 // perturbation_analytical_simple.f90:24-155); kind 1 = seeded stream-function noise.
 void initial_field(const InitSpec& s, std::vector<double>& u, std::vector<double>& v, std::vector<double>& w);
 
+// The same field in separable form, for assembly on the device: at interior plane k,
+//   u = U[k] + scale(u2[q] * wd[k]),  v = scale(v2[q] * wd[k]),  w = 0,
+// with scale(x) = x * sf when scaled (else x), every operation rounded as in initial_field;
+// walls are zero.  u2, v2 are planar (nx*ny), U and wd have nz entries.
+struct PlanarInit {
+    std::vector<double> u2, v2, U, wd;
+    double sf = 1.0;
+    int scaled = 0, perturbed = 0;
+};
+void initial_planar(const InitSpec& s, PlanarInit& out);
+
 }  // namespace dns

@@ -14,22 +14,6 @@ namespace dns {
 
 static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to 256 threads");
 
-struct ColInfo {
-    int c, mode, part, i, j;
-    bool valid;
-};
-__device__ __forceinline__ ColInfo col_info(int H2, int nxh) {
-    ColInfo ci;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    ci.c = blockIdx.x * 64 + wave * 16 + (lane & 15);
-    ci.valid = ci.c < H2;
-    ci.mode = ci.c >> 1;
-    ci.part = ci.c & 1;
-    ci.j = ci.mode / nxh;
-    ci.i = ci.mode - ci.j * nxh;
-    return ci;
-}
-
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
 // the x Nyquist column gives an anti-Hermitian column (zero after the round trip), and so
 // does the y Nyquist row in the kx = 0 and kx = nx/2 columns.
@@ -43,16 +27,16 @@ __device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx,
 // or the vorticity components that need it: omega_x = i ky w - D1 v, omega_y = D1 u - i kx w
 // (DNS_pressure_BC_3D.f90:1077-1079).  One block = 16 modes, all nz rows.
 // -------------------------------------------------------------------------------------
+// Tile (cb, by, mb): column block cb of member mb; by = field (Z_PLAIN).
 template <int RBW, int MODE>
-__global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
-    __shared__ double lds[2 * KC * LDB];
+__device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int mb, double* lds) {
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
-    const int mode0 = blockIdx.x * BMODES;
+    const int mode0 = cb * BMODES;
     const int nfield = MODE == Z_VORT ? 2 : 1;
-    const size_t mo = size_t(blockIdx.z) * a.mstride;
+    const size_t mo = size_t(mb) * a.mstride;
     const double* w_in = a.in[2] + mo;
     for (int q = 0; q < nfield; q++) {
-        const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : blockIdx.y;   // vort: v first, then u
+        const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : by;   // vort: v first, then u
         const double* in = a.in[f] + mo;
         d4 acc[RBW][CT];
         zero_acc2(acc);
@@ -63,7 +47,7 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
         }, lds);
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
-            const int col = out_col(ct);
+            const int col = out_col(ct, cb);
             if (col >= H2) continue;
             const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
 #pragma unroll
@@ -97,6 +81,19 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
     }
 }
 
+template <int RBW, int MODE>
+__global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
+    __shared__ double lds[2 * KC * LDB];
+    zderiv_tile<RBW, MODE>(a, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+}
+template <int RBW, int MODE>
+__global__ __launch_bounds__(256) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
+    __shared__ double lds[2 * KC * LDB];
+    const Tile t = map[blockIdx.x];
+    const ZArgs& a = d[t.g];
+    zderiv_tile<RBW, MODE>(a, t.x, blockIdx.y, t.z, lds);
+}
+
 // -------------------------------------------------------------------------------------
 // Helmholtz, one velocity component per grid.y (solve_viscous_helmholtz,
 // DNS_pressure_BC_3D.f90:1425-1521, with the explicit half of Crank-Nicolson from
@@ -107,12 +104,13 @@ __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
 // bc = -dt (i k_eff p_hat) at the walls (the Kim & Moin values -dt grad p, :1325-1328).
 // -------------------------------------------------------------------------------------
 template <int RBW>
-__global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
-    __shared__ double smem[(4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB];
-    const int comp = blockIdx.y, mb = blockIdx.z;
+constexpr int solve_lds() { return (4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB; }
+
+template <int RBW>
+__device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int comp, int mb, double* smem) {
     const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2, RBI = (m + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int mode0 = blockIdx.x * BMODES;
+    const int mode0 = cb * BMODES;
     const size_t mo = size_t(mb) * a.mstride;
     const double* uh = a.uh[comp] + mo;
     const double* sh = a.s_f32 ? reinterpret_cast<const double*>(reinterpret_cast<const float*>(a.sh[comp]) + mo)
@@ -128,7 +126,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     double k2c[CT], bc1[CT], bc2[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct);
+        const int col = out_col(ct, cb);
         k2c[ct] = 0.0; bc1[ct] = 0.0; bc2[ct] = 0.0;
         if (col < H2) {
             const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
@@ -176,7 +174,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
                 double z = 0.0;
-                if (row < m && out_col(ct) < H2)
+                if (row < m && out_col(ct, cb) < H2)
                     z = (acc[ii][ct][r] + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
                 zs[row * LDB + ct * 16 + (lane & 15)] = z;
             }
@@ -188,7 +186,7 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     double* out = a.out[comp] + mo;
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct);
+        const int col = out_col(ct, cb);
         if (col >= H2) continue;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
@@ -205,6 +203,19 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     }
 }
 
+template <int RBW>
+__global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
+    __shared__ double smem[solve_lds<RBW>()];
+    helmholtz_tile<RBW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+}
+template <int RBW>
+__global__ __launch_bounds__(256) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
+    __shared__ double smem[solve_lds<RBW>()];
+    const Tile t = map[blockIdx.x];
+    const HelmArgs& a = d[t.g];
+    helmholtz_tile<RBW>(a, t.x, blockIdx.y, t.z, smem);
+}
+
 // -------------------------------------------------------------------------------------
 // Pressure (solve_pressure_3d_standard :1654-1733 with compute_divergence_3d :1858-1879)
 // and, when fused, the projection + wall values (pressure_correction_3d :1903-1929,
@@ -215,12 +226,11 @@ __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
 // Mode (0,0) is left to k_poisson_mode0.
 // -------------------------------------------------------------------------------------
 template <int RBW>
-__global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
-    __shared__ double smem[(4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB];
+__device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int mode0 = blockIdx.x * BMODES;
-    const size_t mo = size_t(blockIdx.z) * a.mstride;
+    const int mode0 = cb * BMODES;
+    const size_t mo = size_t(mb) * a.mstride;
     const double* us = a.us + mo; const double* vs = a.vs + mo; const double* ws = a.ws + mo;
     double* uo = a.u + mo; double* vo = a.v + mo; double* wo = a.w + mo; double* po = a.p + mo;
     double* phio = a.phi + mo; double* dphio = a.dphi + mo;
@@ -228,7 +238,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     bool live[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct);
+        const int col = out_col(ct, cb);
         const int mode = col >> 1, j = mode / a.nxh, i = mode - j * a.nxh;
         live[ct] = col < H2 && mode != 0;
         k2c[ct] = 1.0; kxe[ct] = 0.0; kye[ct] = 0.0;
@@ -275,7 +285,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     gemm_lds<RBW>(acc, a.Bsw, RB, a.BKS, zs);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct);
+        const int col = out_col(ct, cb);
         const int part = col & 1;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
@@ -306,7 +316,7 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     gemm_lds<RBW>(acc, a.DBsw, RB, a.BKS, zs);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct);
+        const int col = out_col(ct, cb);
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
 #pragma unroll
@@ -325,6 +335,19 @@ __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     }
 }
 
+template <int RBW>
+__global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
+    __shared__ double smem[solve_lds<RBW>()];
+    poisson_tile<RBW>(a, blockIdx.x, blockIdx.z, smem);
+}
+template <int RBW>
+__global__ __launch_bounds__(256) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
+    __shared__ double smem[solve_lds<RBW>()];
+    const Tile t = map[blockIdx.x];
+    const PoisArgs& a = d[t.g];
+    poisson_tile<RBW>(a, t.x, t.z, smem);
+}
+
 // mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
 // (DNS_pressure_BC_3D.f90:1702-1711).  Its solution is linear in w*(0,0,:):
 // phi = E0 w*, D1 phi = F0 w*, with E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0
@@ -335,9 +358,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) {
-    const int nz = a.nz, k = blockIdx.x, lane = threadIdx.x;
-    const size_t mo = size_t(blockIdx.y) * a.mstride;
+__device__ __forceinline__ void mode0_tile(const Mode0Args& a, int k, int mb) {
+    const int nz = a.nz, lane = threadIdx.x;
+    const size_t mo = size_t(mb) * a.mstride;
     const double* ws = a.ws + mo;
     double s0 = 0.0, s1 = 0.0, d0 = 0.0, d1 = 0.0;
     for (int q = lane; q < nz; q += 64) {
@@ -361,6 +384,13 @@ __global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) {
             a.p[o] = a.p[o] + xv;
         }
     }
+}
+
+__global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) { mode0_tile(a, blockIdx.x, blockIdx.y); }
+__global__ __launch_bounds__(64) void k_poisson_mode0_rag(const Mode0Args* __restrict__ d, const Tile* __restrict__ map) {
+    const Tile t = map[blockIdx.x];
+    const Mode0Args& a = d[t.g];
+    mode0_tile(a, t.x, t.z);
 }
 
 // projection of the stage API (pressure_correction_3d :1903-1929), from stored phi, D1 phi
@@ -399,19 +429,22 @@ __global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, long long ms
 // (DNS_pressure_BC_3D.f90:1107-1148, 2434-2549), called with istart as in :1118.
 // U_b = sum_k w_k Re u_hat(0,0,k) / (nx ny sum w): the (0,0) coefficient of the
 // unnormalised forward transform is the plane sum.
-__global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) {
-    const int lane = threadIdx.x, mb = blockIdx.x;
+__device__ __forceinline__ void flow_tile(const FlowArgs& a, int mb) {
+    const int lane = threadIdx.x;
     const FlowMember P = a.mem[mb];
     const double* uh = a.uh + size_t(mb) * a.mstride;
+    // the bulk integral is also formed under constant forcing: a non-finite value there flags
+    // a member whose fields have gone non-finite (any such point reaches mode (0,0) of u
+    // through the product's forward transform within one step)
     double part = 0.0;
-    if (P.method != 1)
-        for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * a.wz[k];
+    for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * a.wz[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if (lane != 0) return;
     FlowState s = a.st[mb];
     if (P.method == 1) {
         s.G = P.target_G;
+        s.Ub = part / (a.nxny * a.wsum);
     } else if (P.istart % P.freq == 0) {
         s.Ub = part / (a.nxny * a.wsum);
         s.err = P.target_Ub - s.Ub;
@@ -429,6 +462,34 @@ __global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) {
         s.gavg = s.gsum / double(s.gcount);
     }
     a.st[mb] = s;
+}
+
+__global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) { flow_tile(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_flow_control_rag(const FlowArgs* __restrict__ d, const Tile* __restrict__ map) {
+    const Tile t = map[blockIdx.x];
+    const FlowArgs& a = d[t.g];
+    flow_tile(a, t.z);
+}
+
+// explicit roundings (no contraction into fma) reproduce the host generator bit for bit
+__global__ __launch_bounds__(256) void k_init_fields(InitArgs a) {
+    const size_t N = size_t(a.npl) * a.nz;
+    for (size_t n = blockIdx.x * size_t(blockDim.x) + threadIdx.x; n < N; n += size_t(gridDim.x) * blockDim.x) {
+        const int k = int(n / size_t(a.npl));
+        const size_t q = n - size_t(k) * a.npl;
+        double u = 0.0, v = 0.0;
+        if (k != 0 && k != a.nz - 1) {
+            double up = 0.0, vp = 0.0;
+            if (a.perturbed) {
+                up = __dmul_rn(a.u2[q], a.wd[k]);
+                vp = __dmul_rn(a.v2[q], a.wd[k]);
+                if (a.scaled) { up = __dmul_rn(up, a.sf); vp = __dmul_rn(vp, a.sf); }
+            }
+            u = __dadd_rn(a.U[k], up);
+            v = __dadd_rn(0.0, vp);
+        }
+        a.u[n] = u; a.v[n] = v; a.w[n] = 0.0;
+    }
 }
 
 // partial reductions: [max|u|, max|v|, max|w|, sum 0.5 |u|^2 w_k, max|d|]
@@ -534,6 +595,57 @@ hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nm, double* 
 }
 hipError_t launch_flow_control(const FlowArgs& a, int nm, hipStream_t s) {
     hipLaunchKernelGGL(k_flow_control, dim3(nm), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_zderiv_rag(int rbw, int mode, const ZArgs* d, const Tile* map, int nt, int nf, hipStream_t s) {
+    if (nt <= 0) return hipSuccess;
+    const dim3 g(nt, mode == Z_VORT ? 1 : nf);
+    switch (rbw * 2 + (mode == Z_VORT)) {
+        case 2: hipLaunchKernelGGL((k_zderiv_rag<1, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
+        case 3: hipLaunchKernelGGL((k_zderiv_rag<1, Z_VORT>), g, dim3(256), 0, s, d, map); break;
+        case 4: hipLaunchKernelGGL((k_zderiv_rag<2, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
+        case 5: hipLaunchKernelGGL((k_zderiv_rag<2, Z_VORT>), g, dim3(256), 0, s, d, map); break;
+        case 6: hipLaunchKernelGGL((k_zderiv_rag<3, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
+        case 7: hipLaunchKernelGGL((k_zderiv_rag<3, Z_VORT>), g, dim3(256), 0, s, d, map); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int nt, hipStream_t s) {
+    if (nt <= 0) return hipSuccess;
+    const dim3 g(nt, 3);
+    switch (rbw) {
+        case 1: hipLaunchKernelGGL((k_helmholtz_rag<1>), g, dim3(256), 0, s, d, map); break;
+        case 2: hipLaunchKernelGGL((k_helmholtz_rag<2>), g, dim3(256), 0, s, d, map); break;
+        case 3: hipLaunchKernelGGL((k_helmholtz_rag<3>), g, dim3(256), 0, s, d, map); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int nt, hipStream_t s) {
+    if (nt <= 0) return hipSuccess;
+    switch (rbw) {
+        case 1: hipLaunchKernelGGL((k_poisson_rag<1>), dim3(nt), dim3(256), 0, s, d, map); break;
+        case 2: hipLaunchKernelGGL((k_poisson_rag<2>), dim3(nt), dim3(256), 0, s, d, map); break;
+        case 3: hipLaunchKernelGGL((k_poisson_rag<3>), dim3(nt), dim3(256), 0, s, d, map); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int nt, hipStream_t s) {
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_poisson_mode0_rag, dim3(nt), dim3(64), 0, s, d, map);
+    return hipGetLastError();
+}
+hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int nt, hipStream_t s) {
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_flow_control_rag, dim3(nt), dim3(64), 0, s, d, map);
+    return hipGetLastError();
+}
+hipError_t launch_init_fields(const InitArgs& a, hipStream_t s) {
+    const size_t N = size_t(a.npl) * a.nz;
+    const int blocks = int(std::min<size_t>((N + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_init_fields, dim3(blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s) {

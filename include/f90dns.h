@@ -145,6 +145,11 @@ int dns_set_member_fields(dns_ctx* ctx, int member, const double* u, const doubl
 int dns_get_member_fields(dns_ctx* ctx, int member, double* u, double* v, double* w, double* p);
 int dns_set_member_controller(dns_ctx* ctx, int member, double G, double integral, double prev, double gsum,
                               int gcount);
+/* Reassign member `member` (same shape as the context): its Re, controller and perturbation
+ * parameters; resets its controller state.  dns_init_member_fields then gives it the initial
+ * field of those parameters (initialize_fields_3d for one member). */
+int dns_set_member_params(dns_ctx* ctx, int member, const dns_params* p);
+int dns_init_member_fields(dns_ctx* ctx, int member);
 int dns_member_diagnostics(dns_ctx* ctx, int member, dns_diag* d);
 int dns_member_divergence(dns_ctx* ctx, int member, double* max_div);
 

@@ -24,6 +24,13 @@ const char* dns_profile_kernel_name(int i);
 int dns_debug_mfma_probe(double* out);
 /* The same probe for v_mfma_f32_16x16x4f32 (the mixed-precision contraction). */
 int dns_debug_mfma_probe_f32(double* out);
+/* The initial fields of initialize_fields_3d for p, generated point by point on the host
+ * (host only, no device needed); dns_init_fields assembles the same fields on the device. */
+int dns_debug_initial_field(const dns_params* p, double* u, double* v, double* w);
+/* One operator of the per-mode solves for (nz, dt, Re, theta, D1 variant), host only, row-major
+ * (which: 0 Helmholtz forward, 1 back, 2 c1, 3 c2, 4 eigenvalues, 5 Poisson forward, 6 back,
+ * 7 D1 back, 8 eigenvalues, 9 mode-(0,0) matrix, 10 D1, 11 LGL weights, 12 D2). */
+int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols);
 /* max relative reconstruction error of the Helmholtz and Poisson eigen bases. */
 int dns_debug_operator_error(dns_ctx* ctx, double* out2);
 

@@ -97,3 +97,22 @@ def test_batch_rejects_members_of_different_shape_before_touching_a_device():
     assert rc in (0, 3)
     if rc == 0:
         L.dns_destroy(h)
+
+
+def test_host_initial_field_matches_oracle_generator():
+    """The library's host generator (the reference for the device assembly) against the
+    oracle's own restatement of initialize_fields_3d, analytical and seeded-noise kinds."""
+    import numpy as np
+    import f90_dns_amd as dns
+    from oracle import oracle as orc
+    if not os.path.exists(dns.library_path()):
+        pytest.skip("libf90dns.so not built")
+    for kw in [dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, enable_perturbations=1, perturbation_kind=0,
+                    u_centerline=1.0, perturbation_amplitude=0.01),
+               dict(nx=48, ny=36, nz=17, xlen=4 * np.pi, ylen=np.pi, enable_perturbations=1, perturbation_kind=1,
+                    seed=123, noise_kmax=5, perturbation_amplitude=0.05, u_centerline=1.5)]:
+        u, v, w = dns.host_initial_field(**kw)
+        o = orc.Oracle(**kw)
+        o.init_fields()
+        ou, ov, ow, _ = o.get_fields()
+        assert np.max(np.abs(u - ou)) < 1e-14 and np.max(np.abs(v - ov)) < 1e-14 and np.max(np.abs(w - ow)) == 0

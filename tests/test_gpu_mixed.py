@@ -72,3 +72,52 @@ def test_mixed_production_shape_short(dns):
     e = field_errors(g.get_fields(), o.get_fields())
     _say(f"mixed 128^2x129 after 3 steps: {e}")
     assert max_err(e) < 1e-6, e
+
+
+def test_mixed_192_plane_fields_in_and_out(dns):
+    """192 x 192 planes: the step runs its transforms in FP32; field input/output, diagnostics
+    and the divergence check use the two-pass FP64 transforms.  The set/get round trip is
+    exact to FP64 rounding, and a few steps stay finite and divergence-free."""
+    kw = dict(nx=192, ny=192, nz=65, xlen=4 * np.pi, ylen=2 * np.pi, re=300.0, dt=0.002, use_dealias_23_rule=1,
+              flow_control_method=1, target_pressure_gradient=2.0 / 300.0, u_centerline=1.0,
+              enable_perturbations=1, perturbation_kind=1, seed=9, noise_kmax=6, perturbation_amplitude=0.05,
+              lgl_derivative=1, precision=dns.DNS_PREC_MIXED)
+    s = dns.Simulation(**kw)
+    s.init_fields()
+    u, v, w, p = s.get_fields()
+    o = orc.Oracle(**{k: x for k, x in kw.items() if k != "precision"})
+    o.init_fields()
+    ou, ov, ow, _ = o.get_fields()
+    assert np.max(np.abs(u - ou)) < 1e-13 and np.max(np.abs(v - ov)) < 1e-13 and np.max(np.abs(w - ow)) < 1e-13
+    s.set_fields(u, v, w, p)
+    u2, v2, w2, _ = s.get_fields()
+    assert np.max(np.abs(u2 - u)) < 1e-13 and np.max(np.abs(v2 - v)) < 1e-13
+    d0 = s.diagnostics()
+    od = o.diagnostics()
+    assert abs(d0["energy"] - od["energy"]) < 1e-12 * od["energy"]
+    s.step(10, graph=True)
+    d = s.diagnostics()
+    assert np.isfinite(d["energy"]) and d["step"] == 10
+    assert abs(d["energy"] - d0["energy"]) < 1e-2 * d0["energy"]
+    # the collocation projection leaves O(1e-5) divergence next to the walls (the oracle's
+    # FP64 runs show the same order); the bound only guards against a broken projection
+    assert s.max_divergence() < 1e-3
+    _say(f"192x192x65 mixed: E {d0['energy']:.6e} -> {d['energy']:.6e}, div {s.max_divergence():.2e}")
+
+
+@pytest.mark.parametrize("kw", [
+    dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, enable_perturbations=1, perturbation_kind=0,
+         u_centerline=1.0, perturbation_amplitude=0.01),
+    dict(nx=96, ny=64, nz=65, xlen=4 * np.pi, ylen=2 * np.pi, enable_perturbations=1, perturbation_kind=1, seed=44,
+         noise_kmax=8, perturbation_amplitude=0.05, u_centerline=1.0, lgl_derivative=1),
+    dict(nx=48, ny=36, nz=18, enable_perturbations=0, u_centerline=1.5)])
+def test_device_initial_field_is_bitwise_the_host_generator(dns, kw):
+    """dns_init_fields assembles the separable initial field on the device; it must equal the
+    host's point-by-point generator bit for bit (both then take the same forward transform)."""
+    a = dns.Simulation(**kw)
+    a.init_fields()
+    b = dns.Simulation(**kw)
+    u, v, w = dns.host_initial_field(**kw)
+    b.set_fields(u, v, w, np.zeros_like(u))
+    for x, y in zip(a.get_fields(), b.get_fields()):
+        assert np.array_equal(x, y)
