@@ -170,6 +170,7 @@ def lib():
         L.dns_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         L.dns_server_set_health_interval.argtypes = [_vp, ctypes.c_int]
         L.dns_server_pending.argtypes = [_vp]
+        L.dns_debug_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
         L.dns_debug_initial_field.argtypes = [ctypes.POINTER(DnsParams), _vp, _vp, _vp]
         L.dns_enqueue_step.argtypes = [_vp, ctypes.c_int]
         L.dns_query.argtypes = [_vp]

@@ -1,114 +1,200 @@
-// contract.h -- block-level FP64 MFMA contraction of a 32-column tile of spectral columns.
+// contract.h -- block-level FP64 MFMA contractions of the folded wall-normal operators over a
+// 32-column tile of spectral columns.
 //
-// A block of 4 wavefronts owns 32 real columns (16 complex modes).  Output row blocks are split
-// across the waves (wave w takes row blocks w, w+4, w+8, ...), so each row of the operator is
-// read from HBM/L2 once per block; the B operand (the spectral data, prologue applied) is
-// produced once per element by one thread and staged through double-buffered LDS chunks of 16
-// rows, the next chunk prefetched into registers while the current one is consumed.
+// A block of 4 wavefronts owns 32 real columns (16 complex modes).  A folded operator has two
+// row groups (setup.h FoldedA): group 0 acts on panel 0, group 1 on panel 1, both K rows.  Wave
+// w takes the row-block pairs b = w, w + 4, ...: group-0 block b and group-1 block b, so the two
+// results of one output row pair (g0_i, g1_i) sit in the same lane and item and unfold in
+// registers.  The panels (the spectral data, folded and prologue applied) are produced once per
+// element by one thread and staged through double-buffered LDS chunks of KC rows; the next
+// chunk and the wave's A operands for it are fetched while the current one is consumed.
 #pragma once
 #include "device_common.h"
+#include "kernels.h"
 
 namespace dns {
 
 constexpr int CT = 2;              // 16-column MFMA tiles per block
 constexpr int BCOLS = 16 * CT;     // real columns per block
 constexpr int BMODES = BCOLS / 2;  // complex modes per block
-constexpr int KC = 16;             // B rows per LDS chunk (4 MFMA k-steps)
+constexpr int KC = 16;             // panel rows per LDS chunk (4 MFMA k-steps)
 constexpr int LDB = BCOLS + 2;     // padded LDS row stride, doubles
+constexpr int STAGE_LDS = 4 * KC * LDB;   // doubles of the staging buffers (2 buffers x 2 panels)
 
-// acc[i][ct] += A[rb = wave + 4i] x B, B produced by bprod(k, mode_local) -> double2 (Re, Im)
-// of row k for the block's mode mode_local (zero beyond the operand's rows).  lds: 2*KC*LDB.
-// Both operands are fetched one chunk ahead: B into registers then LDS, A (this wave's rows of
-// the next four k-steps) into registers.
-template <int RBW, class BP>
-__device__ __forceinline__ void gemm_staged(d4 (&acc)[RBW][CT], const double* __restrict__ Asw, int RB, int KS,
-                                            BP&& bprod, double* lds) {
+struct FoldPair {
+    double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
+};
+
+// acc0[i][ct] += A0[block b] x P0, acc1[i][ct] += A1[block b] x P1 for b = wave + 4i;
+// fold(kf) gives panel row kf for the staging thread's mode (thread t stages row t/16 of a
+// chunk for mode t%16), zero beyond K.  lds: STAGE_LDS doubles.
+// With op.extra, xacc accumulates the staging thread's share of the dense extra row times
+// panel 0 (reduce_extra sums the shares of a column).
+template <int RBW, class FOLD>
+__device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FOLD&& fold,
+                                          double* lds, double2& xacc) {
+    const double* xrow = fold_xrow(op);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kk = lane >> 4, cl = lane & 15;
-    const int t = threadIdx.x;
-    const int rt = t >> 4, mt = t & 15;            // staging: chunk row rt, mode mt
-    const int nchunks = (KS + 3) / 4;
-    double areg[RBW][4], anext[RBW][4];
-    auto load_a = [&](double (&dst)[RBW][4], int ch) {
+    const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
+    const int nch = (op.ks + 3) / 4;
+    const double* a0p[RBW];
+    const double* a1p[RBW];
+#pragma unroll
+    for (int i = 0; i < RBW; i++) {
+        const int b = wave + 4 * i;
+        a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
+        a1p[i] = op.A + (size_t(op.rb0 + b) * op.ksp) * 64 + lane;
+    }
+    double ar0[RBW][4], ar1[RBW][4], an0[RBW][4], an1[RBW][4];
+    auto load = [&](double (&d0)[RBW][4], double (&d1)[RBW][4], int ch) {
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            const int rb = wave + 4 * i;
+            const int b = wave + 4 * i;
 #pragma unroll
             for (int sl = 0; sl < 4; sl++) {
-                const int s = ch * 4 + sl;
-                dst[i][sl] = (rb < RB && s < KS) ? Asw[(size_t(rb) * KS + s) * 64 + lane] : 0.0;
+                d0[i][sl] = b < op.rb0 ? a0p[i][(ch * 4 + sl) * 64] : 0.0;
+                d1[i][sl] = b < op.rb1 ? a1p[i][(ch * 4 + sl) * 64] : 0.0;
             }
         }
     };
-    load_a(areg, 0);
-    double2 v = bprod(rt, mt);
-    lds[rt * LDB + 2 * mt] = v.x;
-    lds[rt * LDB + 2 * mt + 1] = v.y;
+    auto stage = [&](double* buf, const FoldPair& v, int kf) {
+        if (op.extra) {
+            const double c = xrow[kf];
+            xacc.x += c * v.p0.x;
+            xacc.y += c * v.p0.y;
+        }
+        buf[rt * LDB + 2 * mt] = v.p0.x;
+        buf[rt * LDB + 2 * mt + 1] = v.p0.y;
+        buf[KC * LDB + rt * LDB + 2 * mt] = v.p1.x;
+        buf[KC * LDB + rt * LDB + 2 * mt + 1] = v.p1.y;
+    };
+    load(ar0, ar1, 0);
+    stage(lds, fold(rt), rt);
     __syncthreads();
-    for (int ch = 0; ch < nchunks; ch++) {
-        const double* cur = lds + (ch & 1) * KC * LDB;
-        double2 nxt = make_double2(0.0, 0.0);
-        const bool more = ch + 1 < nchunks;
+    for (int ch = 0; ch < nch; ch++) {
+        const double* cur = lds + (ch & 1) * 2 * KC * LDB;
+        const bool more = ch + 1 < nch;
+        FoldPair nxt;
         if (more) {
-            nxt = bprod((ch + 1) * KC + rt, mt);
-            load_a(anext, ch + 1);
+            nxt = fold((ch + 1) * KC + rt);
+            load(an0, an1, ch + 1);
         }
 #pragma unroll
         for (int sl = 0; sl < 4; sl++) {
-            double b[CT];
+            if (ch * 4 + sl < op.ks) {
+                double b0[CT], b1[CT];
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++) b[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
+                for (int ct = 0; ct < CT; ct++) {
+                    b0[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
+                    b1[ct] = cur[KC * LDB + (sl * 4 + kk) * LDB + ct * 16 + cl];
+                }
 #pragma unroll
-            for (int i = 0; i < RBW; i++) {
-                if (wave + 4 * i < RB) {
+                for (int i = 0; i < RBW; i++) {
+                    const int b = wave + 4 * i;
+                    if (b < op.rb0) {
 #pragma unroll
-                    for (int ct = 0; ct < CT; ct++)
-                        acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[i][sl], b[ct], acc[i][ct], 0, 0, 0);
+                        for (int ct = 0; ct < CT; ct++)
+                            acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
+                    }
+                    if (b < op.rb1) {
+#pragma unroll
+                        for (int ct = 0; ct < CT; ct++)
+                            acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
+                    }
                 }
             }
         }
         if (more) {
-            double* nb = lds + ((ch + 1) & 1) * KC * LDB;
-            nb[rt * LDB + 2 * mt] = nxt.x;
-            nb[rt * LDB + 2 * mt + 1] = nxt.y;
+            stage(lds + ((ch + 1) & 1) * 2 * KC * LDB, nxt, (ch + 1) * KC + rt);
 #pragma unroll
             for (int i = 0; i < RBW; i++)
 #pragma unroll
-                for (int sl = 0; sl < 4; sl++) areg[i][sl] = anext[i][sl];
+                for (int sl = 0; sl < 4; sl++) {
+                    ar0[i][sl] = an0[i][sl];
+                    ar1[i][sl] = an1[i][sl];
+                }
         }
         __syncthreads();
     }
 }
 
-// acc[i][ct] += A[rb = wave + 4i] x Z with Z resident in LDS as zs[row * LDB + col] (rows >= 4*KS
-// never read).
+// The same contraction with both panels resident in LDS (z0, z1: >= 4*ks rows, row stride LDB).
 template <int RBW>
-__device__ __forceinline__ void gemm_lds(d4 (&acc)[RBW][CT], const double* __restrict__ Asw, int RB, int KS,
-                                         const double* zs) {
+__device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op,
+                                              const double* z0, const double* z1) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kk = lane >> 4, cl = lane & 15;
-    double areg[RBW], anext[RBW];
+    const double* a0p[RBW];
+    const double* a1p[RBW];
+    double ar0[RBW], ar1[RBW], an0[RBW], an1[RBW];
 #pragma unroll
-    for (int i = 0; i < RBW; i++) areg[i] = (wave + 4 * i < RB) ? Asw[(size_t(wave + 4 * i) * KS) * 64 + lane] : 0.0;
-    for (int s = 0; s < KS; s++) {
-        const bool more = s + 1 < KS;
-#pragma unroll
-        for (int i = 0; i < RBW; i++)
-            anext[i] = (more && wave + 4 * i < RB) ? Asw[(size_t(wave + 4 * i) * KS + s + 1) * 64 + lane] : 0.0;
-        double b[CT];
-#pragma unroll
-        for (int ct = 0; ct < CT; ct++) b[ct] = zs[(4 * s + kk) * LDB + ct * 16 + cl];
+    for (int i = 0; i < RBW; i++) {
+        const int b = wave + 4 * i;
+        a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
+        a1p[i] = op.A + (size_t(op.rb0 + b) * op.ksp) * 64 + lane;
+        ar0[i] = b < op.rb0 ? a0p[i][0] : 0.0;
+        ar1[i] = b < op.rb1 ? a1p[i][0] : 0.0;
+    }
+    for (int s = 0; s < op.ks; s++) {
+        // ksp >= ks + 1: the prefetch of k-step s + 1 stays inside the block
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            if (wave + 4 * i < RB) {
+            const int b = wave + 4 * i;
+            an0[i] = b < op.rb0 ? a0p[i][(s + 1) * 64] : 0.0;
+            an1[i] = b < op.rb1 ? a1p[i][(s + 1) * 64] : 0.0;
+        }
+        double b0[CT], b1[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            b0[ct] = z0[(4 * s + kk) * LDB + ct * 16 + cl];
+            b1[ct] = z1[(4 * s + kk) * LDB + ct * 16 + cl];
+        }
+#pragma unroll
+        for (int i = 0; i < RBW; i++) {
+            const int b = wave + 4 * i;
+            if (b < op.rb0) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++)
-                    acc[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[i], b[ct], acc[i][ct], 0, 0, 0);
+                    acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i], b0[ct], acc0[i][ct], 0, 0, 0);
+            }
+            if (b < op.rb1) {
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++)
+                    acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i], b1[ct], acc1[i][ct], 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int i = 0; i < RBW; i++) areg[i] = anext[i];
+        for (int i = 0; i < RBW; i++) {
+            ar0[i] = an0[i];
+            ar1[i] = an1[i];
+        }
     }
+}
+
+// Column t < BCOLS of the extra row from the staging threads' shares (mode t/2, part t%2);
+// every thread calls it (it synchronises); uses 2 * 256 doubles of lds.
+__device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds) {
+    const int t = threadIdx.x;
+    lds[2 * t] = xacc.x;
+    lds[2 * t + 1] = xacc.y;
+    __syncthreads();
+    double v = 0.0;
+    if (t < BCOLS) {
+        const int ml = t >> 1, part = t & 1;
+        for (int rt = 0; rt < 16; rt++) v += lds[2 * (rt * 16 + ml) + part];
+    }
+    __syncthreads();
+    return v;
+}
+
+// Column t < BCOLS of the extra row applied to an LDS-resident panel (rows 4 ks, stride LDB).
+__device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) {
+    const double* x = fold_xrow(op);
+    const int t = threadIdx.x;
+    double v = 0.0;
+    for (int j = 0; j < 4 * op.ks; j++) v += x[j] * z0[j * LDB + t];
+    return v;
 }
 
 template <int RBW>
@@ -119,12 +205,15 @@ __device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
         for (int ct = 0; ct < CT; ct++) acc[i][ct] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
-// Output coordinates of acc[i][ct][r] in the block's column tile cb: row = (wave + 4i)*16 +
-// lane/16 + 4r, column = cb*BCOLS + ct*16 + lane%16.
+// Coordinates of acc[i][ct][r]: row within the group = (wave + 4i)*16 + lane/16 + 4r (the
+// result layout of the instruction), column = cb*BCOLS + ct*16 + lane%16.
 __device__ __forceinline__ int out_row(int i, int r) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     return (wave + 4 * i) * 16 + (lane >> 4) + 4 * r;
 }
 __device__ __forceinline__ int out_col(int ct, int cb) { return cb * BCOLS + ct * 16 + (threadIdx.x & 15); }
+
+// panel rows of the LDS-resident z panels of a back contraction with k-steps ks
+__host__ __device__ inline int zpanel_rows(int ks) { return 4 * ks; }
 
 }  // namespace dns

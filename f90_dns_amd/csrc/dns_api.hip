@@ -213,7 +213,7 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     const Operators& o = c->op;
     if ((r = upload(c, &c->xw, xw)) || (r = upload(c, &c->yw, yw)) || (r = upload(c, &c->mask, mask)) ||
         (r = upload(c, &c->wz, o.wz)) || (r = upload(c, &c->e0, o.e0)) || (r = upload(c, &c->f0, o.f0)) ||
-        (r = upload(c, &c->D1sw, o.D1.data)) || (r = upload(c, &c->hF, hF)) || (r = upload(c, &c->hB, o.hB.data)) ||
+        (r = upload(c, &c->D1sw, o.zD1.data)) || (r = upload(c, &c->hF, hF)) || (r = upload(c, &c->hB, o.hB.data)) ||
         (r = upload(c, &c->hc1, c1)) || (r = upload(c, &c->hc2, c2)) || (r = upload(c, &c->hl, hl)) ||
         (r = upload(c, &c->fvec, fv)) || (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) ||
         (r = upload(c, &c->pDB, o.pDB.data)) || (r = upload(c, &c->pl, o.pl)))
@@ -258,6 +258,13 @@ void* dns_stream(dns_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int dns_members(const dns_ctx* c) { return c ? c->B : 0; }
 
 size_t dns_device_bytes(const dns_ctx* c) { return c ? c->bytes : 0; }
+
+static FoldOp fold_op(const FoldedA& f, const double* dev) {
+    FoldOp o;
+    o.A = dev; o.rb0 = f.rb0; o.rb1 = f.rb1; o.ks = f.ks; o.ksp = f.ksp; o.rows0 = f.rows0; o.rows1 = f.rows1;
+    o.extra = f.extra;
+    return o;
+}
 
 static FFTArgs fft_args(dns_ctx* c) {
     FFTArgs a;
@@ -468,7 +475,7 @@ extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
     ZArgs z{};
     z.mstride = (long long)c->NS;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
-    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.d1 = fold_op(c->op.zD1, c->D1sw);
     z.in[0] = c->uh; z.in[1] = c->vh; z.in[2] = c->wh;
     z.out[0] = c->ox; z.out[1] = c->oy;
     z.xw = c->xw; z.yw = c->yw;
@@ -477,7 +484,7 @@ extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
 }
 
 static int nl_zderiv(dns_ctx* c) {
-    HIPCHECK(c, launch_zderiv(c->rb, Z_VORT, ctx_vort_args(c), 1, c->B, c->stream));
+    HIPCHECK(c, launch_zderiv(Z_VORT, ctx_vort_args(c), 1, c->B, c->stream));
     return 0;
 }
 
@@ -528,7 +535,7 @@ extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
     HelmArgs h{};
     h.mstride = (long long)c->NS; h.fstride = (long long)c->hf_stride;
     h.nz = c->nz; h.m = c->nz - 2; h.H2 = c->H2; h.nxh = c->nxh; h.ny = c->ny; h.nx = c->nx;
-    h.Fsw = c->hF; h.FKS = c->op.hF.ks; h.Bsw = c->hB; h.BKS = c->op.hB.ks;
+    h.hF = fold_op(c->op.hF, c->hF); h.hB = fold_op(c->op.hB, c->hB);
     h.c1 = c->hc1; h.c2 = c->hc2; h.hl = c->hl; h.fvec = c->fvec;
     h.dt = c->p.dt; h.nxny = double(c->nx * c->ny);
     h.cn = c->p.use_crank_nicolson ? 1 : 0;
@@ -544,7 +551,7 @@ extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
 
 int dns_viscous_step(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    HIPCHECK(c, launch_helmholtz(c->rbi, ctx_helm_args(c), c->B, c->stream));
+    HIPCHECK(c, launch_helmholtz(ctx_helm_args(c), c->B, c->stream));
     return 0;
 }
 
@@ -552,7 +559,7 @@ extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
     PoisArgs a{};
     a.mstride = (long long)c->NS;
     a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
-    a.Fsw = c->pF; a.FKS = c->op.pF.ks; a.Bsw = c->pB; a.DBsw = c->pDB; a.BKS = c->op.pB.ks;
+    a.pF = fold_op(c->op.pF, c->pF); a.pB = fold_op(c->op.pB, c->pB); a.pDB = fold_op(c->op.pDB, c->pDB);
     a.pl = c->pl; a.dt = c->p.dt; a.fused = fused; a.xw = c->xw; a.yw = c->yw;
     a.us = c->us; a.vs = c->vs; a.ws = c->ws;
     a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph;
@@ -572,7 +579,7 @@ extern "C++" Mode0Args dns::ctx_mode0_args(dns_ctx* c, int fused) {
 
 static int pressure(dns_ctx* c, int fused) {
     PoisArgs a = ctx_pois_args(c, fused);
-    HIPCHECK(c, launch_poisson(c->rb, a, c->B, c->stream));
+    HIPCHECK(c, launch_poisson(a, c->B, c->stream));
     Mode0Args m = ctx_mode0_args(c, fused);
     HIPCHECK(c, launch_poisson_mode0(m, c->B, c->stream));
     return 0;
@@ -778,9 +785,9 @@ int dns_member_divergence(dns_ctx* c, int b, double* max_div) {
     ZArgs z{};
     z.mstride = 0;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
-    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.d1 = fold_op(c->op.zD1, c->D1sw);
     z.in[0] = c->wh + mo; z.out[0] = c->tmp; z.xw = c->xw; z.yw = c->yw;
-    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, 1, c->stream));
+    HIPCHECK(c, launch_zderiv(Z_PLAIN, z, 1, 1, c->stream));
     if (c->g.cls64 < 0) {
         // combine into the omega_x buffer (free outside dns_compute_source_terms), then two passes
         HIPCHECK(c, launch_div_spec(c->g, c->xw, c->yw, c->uh + mo, c->vh + mo, c->tmp, c->ox, c->nz, c->stream));
@@ -818,9 +825,9 @@ int dns_debug_zderiv(dns_ctx* c, const double* spec_in, double* spec_out) {
     ZArgs z{};
     z.mstride = 0;
     z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
-    z.Asw = c->D1sw; z.KS = c->op.D1.ks;
+    z.d1 = fold_op(c->op.zD1, c->D1sw);
     z.in[0] = c->tmp; z.out[0] = c->phi; z.xw = c->xw; z.yw = c->yw;
-    HIPCHECK(c, launch_zderiv(c->rb, Z_PLAIN, z, 1, 1, c->stream));
+    HIPCHECK(c, launch_zderiv(Z_PLAIN, z, 1, 1, c->stream));
     HIPCHECK(c, hipMemcpyAsync(spec_out, c->phi, c->NS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -854,7 +861,7 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
         if ((r = dns_viscous_step(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
         PoisArgs pa = ctx_pois_args(c, 1);
-        HIPCHECK(c, launch_poisson(c->rb, pa, c->B, c->stream));
+        HIPCHECK(c, launch_poisson(pa, c->B, c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
         Mode0Args ma = ctx_mode0_args(c, 1);
         HIPCHECK(c, launch_poisson_mode0(ma, c->B, c->stream));
@@ -883,6 +890,31 @@ static int run_probe(double* out, hipError_t (*launch)(double*, hipStream_t)) {
     return e == hipSuccess ? 0 : DNS_EDEVICE;
 }
 
+int dns_debug_copy(size_t n, int repeats, double* ms_out) {
+    if (n < 2 || (n & 1) || repeats < 1) return DNS_EINVAL;
+    std::string why;
+    if (!dns_check_gfx950(why)) { g_create_error = why; return DNS_EDEVICE; }
+    double *x = nullptr, *y = nullptr;
+    if (hipMalloc(&x, n * sizeof(double)) != hipSuccess) return DNS_ENOMEM;
+    if (hipMalloc(&y, n * sizeof(double)) != hipSuccess) { (void)hipFree(x); return DNS_ENOMEM; }
+    hipError_t e = hipMemset(x, 0, n * sizeof(double));
+    hipEvent_t a = nullptr, b = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, nullptr);
+    for (int r = 0; r < repeats && e == hipSuccess; r++) e = launch_copy(x, y, n, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(b, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    if (ms_out) *ms_out = double(ms) / repeats;
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipFree(x);
+    (void)hipFree(y);
+    return e == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
 int dns_debug_mfma_probe(double* out) { return run_probe(out, launch_mfma_probe); }
 int dns_debug_mfma_probe_f32(double* out) { return run_probe(out, launch_mfma_probe_f32); }
 
@@ -901,14 +933,20 @@ int dns_debug_initial_field(const dns_params* q, double* u, double* v, double* w
     return 0;
 }
 
-static void unswizzle(const SwizzledA& a, std::vector<double>& m) {
-    m.assign(size_t(a.rows) * a.cols, 0.0);
-    for (int rb = 0; rb < a.rb; rb++)
-        for (int s = 0; s < a.ks; s++)
-            for (int lane = 0; lane < 64; lane++) {
-                const int r = rb * 16 + (lane & 15), cc = 4 * s + (lane >> 4);
-                if (r < a.rows && cc < a.cols) m[size_t(r) * a.cols + cc] = a.data[(size_t(rb) * a.ks + s) * 64 + lane];
-            }
+// dense (rows0 + rows1) x K: group-0 rows, then group-1 rows
+static void unswizzle(const FoldedA& a, std::vector<double>& m) {
+    const int rows = a.rows0 + a.rows1;
+    m.assign(size_t(rows) * a.K, 0.0);
+    auto get = [&](int cb, int r, int c) {
+        const int s = c / 4, lane = (r % 16) + 16 * (c % 4);
+        return a.data[(size_t(cb) * a.ksp + s) * 64 + lane];
+    };
+    for (int r = 0; r < a.rows0; r++)
+        for (int c = 0; c < a.K; c++)
+            m[size_t(r) * a.K + c] = (a.extra && r == a.rows0 - 1) ? a.data[size_t(a.rb0 + a.rb1) * a.ksp * 64 + c]
+                                                                   : get(r / 16, r, c);
+    for (int r = 0; r < a.rows1; r++)
+        for (int c = 0; c < a.K; c++) m[size_t(a.rows0 + r) * a.K + c] = get(a.rb0 + r / 16, r, c);
 }
 
 int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols) {
@@ -921,7 +959,7 @@ int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int whic
     std::vector<double> m;
     int r = 0, cc = 0;
     auto vec = [&](const std::vector<double>& v) { m = v; r = int(v.size()); cc = 1; };
-    auto mat = [&](const SwizzledA& a) { unswizzle(a, m); r = a.rows; cc = a.cols; };
+    auto mat = [&](const FoldedA& a) { unswizzle(a, m); r = a.rows0 + a.rows1; cc = a.K; };
     switch (which) {
         case 0: mat(op.hF); break;
         case 1: mat(op.hB); break;
@@ -933,7 +971,7 @@ int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int whic
         case 7: mat(op.pDB); break;
         case 8: vec(op.pl); break;
         case 9: m = op.m0; r = cc = nz - 1; break;
-        case 10: mat(op.D1); break;
+        case 10: mat(op.zD1); break;
         case 11: vec(op.wz); break;
         case 12:
             m.resize(size_t(nz) * nz);

@@ -37,7 +37,7 @@ bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) 
     b += d(2 * p.nx) + std::max<size_t>(2 * p.nx, 1) * sizeof(float);   // x twiddles
     b += d(2 * p.ny) + std::max<size_t>(2 * p.ny, 1) * sizeof(float);   // y twiddles
     b += d(p.nx / 2 + 1) + d(p.ny) + std::max<size_t>(nxh * p.ny, 1);  // wavenumbers, mask
-    b += d(nz) + d(nz * nz) * 2 + d(o.D1.data.size());
+    b += d(nz) + d(nz * nz) * 2 + d(o.zD1.data.size());
     b += d(o.hF.data.size() * B) + d(o.hB.data.size()) + 3 * d(m * B) + d(B);
     b += d(o.pF.data.size()) + d(o.pB.data.size()) + d(o.pDB.data.size()) + d(nz);
     b += 15 * d(NS * B) + 6 * d(N * B);
@@ -146,8 +146,6 @@ static hipError_t grow(T** p, size_t& cap, size_t n) {
     return e;
 }
 
-static int rows_rbw(int rows) { return ((rows + 15) / 16 + 3) / 4; }
-
 int Ensemble::rebuild() {
     ECHECK(hipStreamSynchronize(s0));
     ECHECK(hipStreamSynchronize(s1));
@@ -212,10 +210,12 @@ int Ensemble::rebuild() {
                     int k = base + tune.promote[f32][base];
                     while (k > base && fft_class_capacity(k, c->mixed) < 0) k--;
                     key = k;
-                } else if (kind == K_ZVORT || kind == K_POIS) {
-                    key = rows_rbw(c->nz); f32 = 0;
+                } else if (kind == K_ZVORT) {
+                    key = fold_rbw(hz[g].d1); f32 = 0;
+                } else if (kind == K_POIS) {
+                    key = fold_rbw(hp[g].pF); f32 = 0;
                 } else if (kind == K_HELM) {
-                    key = rows_rbw(c->nz - 2); f32 = 0;
+                    key = fold_rbw(hh[g].hF); f32 = 0;
                 } else {
                     f32 = 0;
                 }
@@ -235,6 +235,18 @@ int Ensemble::rebuild() {
                         for (int z = 0; z < B; z++)
                             if (act[z])
                                 for (int k = 0; k < c->nz; k++) tiles.push_back(Tile{g, z * c->nz + k, z});
+                    } else if (kind == K_HELM || kind == K_POIS) {
+                        L.lds = std::max(L.lds, kind == K_HELM ? helm_lds_bytes(hh[g]) : pois_lds_bytes(hp[g]));
+                        const int ncb = (c->H2 + BCOLS - 1) / BCOLS;
+                        if (tune.interleave) {
+                            for (int x = 0; x < ncb; x++)
+                                for (int z = 0; z < B; z++)
+                                    if (act[z]) tiles.push_back(Tile{g, x, z});
+                        } else {
+                            for (int z = 0; z < B; z++)
+                                if (act[z])
+                                    for (int x = 0; x < ncb; x++) tiles.push_back(Tile{g, x, z});
+                        }
                     } else if (kind == K_FLOW) {
                         for (int z = 0; z < B; z++)
                             if (act[z]) tiles.push_back(Tile{g, 0, z});
@@ -293,8 +305,8 @@ int Ensemble::launch(const Launch& L, hipStream_t s) {
         case K_C2R: e = launch_c2r_rag(L.cls, C2R_NL, L.f32 != 0, d_c2r, map, L.count, L.nf, L.lds, s); break;
         case K_R2C: e = launch_r2c_rag(L.cls, R2C_CROSS, L.f32 != 0, d_r2c, map, L.count, L.nf, L.lds, s); break;
         case K_FLOW: e = launch_flow_control_rag(d_fl, map, L.count, s); break;
-        case K_HELM: e = launch_helmholtz_rag(L.cls, d_h, map, L.count, s); break;
-        case K_POIS: e = launch_poisson_rag(L.cls, d_p, map, L.count, s); break;
+        case K_HELM: e = launch_helmholtz_rag(L.cls, d_h, map, L.count, L.lds, s); break;
+        case K_POIS: e = launch_poisson_rag(L.cls, d_p, map, L.count, L.lds, s); break;
         case K_MODE0: e = launch_poisson_mode0_rag(d_m0, map, L.count, s); break;
         default: e = hipErrorInvalidValue;
     }

@@ -86,25 +86,39 @@ hipError_t launch_div_spec(const FFTGeom& g, const double* xw, const double* yw,
 // ---- wall-normal contractions and per-mode solves ---------------------------------------
 // Batched contexts hold B members of one shape; member b's spectral field starts at
 // b * mstride elements and the z-kernels take the member from blockIdx.z.
+
+// device view of a folded operator (setup.h FoldedA; ksp >= ks + 1 k-steps stored per block)
+struct FoldOp {
+    const double* A;
+    int rb0, rb1, ks, ksp, rows0, rows1;
+    int extra;                   // 1: group-0 row rows0-1 is the dense row after the blocks
+};
+// the dense extra row of a folded operator (valid when extra)
+__host__ __device__ inline const double* fold_xrow(const FoldOp& o) {
+    return o.A + size_t(o.rb0 + o.rb1) * o.ksp * 64;
+}
+
 struct ZArgs {
     long long mstride;
     int nz, H2, nxh, ny, nx;     // H2 = 2 * (nx/2+1) * ny real columns per plane
-    const double* Asw; int KS;   // swizzled operator (nz rows)
+    FoldOp d1;                   // folded D1: s (centre group) on X_O | r on X_E
     const double* in[3];
     double* out[3];              // float when out_f32
     const double* xw; const double* yw;
     int out_f32;
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
-hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
+// pairs of row blocks per wave of a folded operator
+inline int fold_rbw(const FoldOp& o) { return ((o.rb0 > o.rb1 ? o.rb0 : o.rb1) + 3) / 4; }
+hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
 
 struct FlowState;
 struct HelmArgs {
-    long long mstride, fstride;  // member strides of the fields and of Fsw (per-member Re)
+    long long mstride, fstride;  // member strides of the fields and of hF.A (per-member Re)
     int nz, m, H2, nxh, ny, nx;
-    const double* Fsw; int FKS;  // m x (m + nz) [T | f T D2]  (KS covers m cols when not CN)
-    const double* Bsw; int BKS;  // m x m
-    const double* c1; const double* c2; const double* hl;   // m per member
+    FoldOp hF;                   // [T | f T D2_I] folded: even | odd eigen rows (member 0's A)
+    FoldOp hB;                   // p | q
+    const double* c1; const double* c2; const double* hl;   // m per member, even rows first
     const double* fvec;          // theta dt / Re per member
     double dt, nxny;
     int cn;
@@ -114,14 +128,15 @@ struct HelmArgs {
     int s_f32;
     const FlowState* flow;       // per member: current pressure gradient G
 };
-hipError_t launch_helmholtz(int rbi, const HelmArgs& a, int nmembers, hipStream_t s);
+hipError_t launch_helmholtz(const HelmArgs& a, int nmembers, hipStream_t s);
+// dynamic LDS bytes of the Helmholtz / Poisson kernels (staging, then the two z panels)
+size_t helm_lds_bytes(const HelmArgs& a);
 
 struct PoisArgs {
     long long mstride;
     int nz, H2, nxh, ny, nx;
-    const double* Fsw; int FKS;  // nz x 2nz
-    const double* Bsw; const double* DBsw; int BKS;
-    const double* pl;
+    FoldOp pF, pB, pDB;          // folded [Tf | Tf D1], Bf, D1 Bf
+    const double* pl;            // even eigenvalues first
     double dt;
     int fused;                   // 1: apply correction + walls; 0: write phi, D1 phi only
     const double* xw; const double* yw;
@@ -129,7 +144,8 @@ struct PoisArgs {
     double* u; double* v; double* w; double* p;             // fused outputs
     double* phi; double* dphi;                              // stage outputs
 };
-hipError_t launch_poisson(int rb, const PoisArgs& a, int nmembers, hipStream_t s);
+hipError_t launch_poisson(const PoisArgs& a, int nmembers, hipStream_t s);
+size_t pois_lds_bytes(const PoisArgs& a);
 
 struct Mode0Args {
     long long mstride;
@@ -176,10 +192,11 @@ struct FlowArgs {
 };
 hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
 
-// ragged variants of the step kernels (rbw = row blocks per wave, (rows/16 + 3)/4)
+// ragged variants of the step kernels (rbw = fold_rbw of the groups' operators; lds = dynamic
+// LDS bytes, the largest over the launch's groups)
 hipError_t launch_zderiv_rag(int rbw, int mode, const ZArgs* d, const Tile* map, int ntiles, int nf, hipStream_t s);
-hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int ntiles, hipStream_t s);
-hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int ntiles, hipStream_t s);
+hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int ntiles, size_t lds, hipStream_t s);
+hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int ntiles, size_t lds, hipStream_t s);
 hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int ntiles, hipStream_t s);
 hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int ntiles, hipStream_t s);
 
@@ -192,6 +209,8 @@ struct InitArgs {
     double* u; double* v; double* w;
 };
 hipError_t launch_init_fields(const InitArgs& a, hipStream_t s);
+// y = x for n doubles (n even)
+hipError_t launch_copy(const double* x, double* y, size_t n, hipStream_t s);
 
 // reductions over physical fields: max|u|,|v|,|w|, sum 0.5|u|^2 w_k, max|d| (any may be null)
 struct ReduceArgs {

@@ -37,28 +37,56 @@ struct SwizzledA {
 };
 SwizzledA swizzle_a(const std::vector<double>& rowmajor, int rows, int cols);
 
+// Even/odd folding.  The LGL nodes are symmetric (z_{n-1-k} = -z_k), so D1 is
+// centro-antisymmetric, D2 and the stiffness matrix are centrosymmetric, and the eigenbases of
+// the per-mode solves are even or odd vectors.  A column vector x of length n = 2h + r
+// (r = n mod 2) folds into X_E(c) = x_c + x_{n-1-c} (c < h) with X_E(h) = x_h when r = 1, and
+// X_O(c) = x_c - x_{n-1-c} (c < h).  Every wall-normal operator then splits into two blocks of
+// half size; an output that comes back in physical rows unfolds from its two groups g0 (h + r
+// rows, the one holding the centre) and g1 (h rows) as out_i = g0_i + g1_i,
+// out_{n-1-i} = g0_i - g1_i (i < h), out_h = g0_h.
+//
+// FoldedA: the two row groups in the lane order of v_mfma_f64_16x16x4f64's A operand; group g
+// acts on its own K-row panel.  Block cb (group-0 blocks first, then rb1 group-1 blocks),
+// k-step s: entry [(cb*ksp + s)*64 + lane] = M_g(16*(cb - first_g) + lane%16, 4s + lane/16);
+// ks = ceil(K/4) k-steps are used; ksp (a multiple of 4, >= ks + 1) are stored, the tail zero.
+// When rows0 = 16 k + 1 (the centre row or the extra even eigenvector of an odd column), that
+// last group-0 row is not in the blocks (rb0 = k) but stored dense after them (extra = 1,
+// 4 ksp entries) and computed as a dot product, so the row-block pairs stay k per group.
+struct FoldedA {
+    int rows0 = 0, rows1 = 0, K = 0, rb0 = 0, rb1 = 0, ks = 0, ksp = 0, extra = 0;
+    std::vector<double> data;
+};
+FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector<double>& M1, int rows1, int K);
+
 // Operators of one grid and run (nz, dt, Re, theta):
 //  Helmholtz (interior unknowns I = 1..nz-2, m = nz-2), with Q L Q^T = W_I^{-1/2} A_II W_I^{-1/2}:
 //    y = T (alpha u_I + dt S_I) + f*(T D2_I) u + c1*bc1 + c2*bc2,  z = y / (hl + f k^2),
 //    x_I = B z, x_0 = bc1, x_{nz-1} = bc2,
 //    T = Q^T W_I^{1/2}, B = W_I^{-1/2} Q, c_{1,2} = -2 f Q^T W_I^{-1/2} A_{I,{0,nz-1}}, hl = f lambda + 1.
+//    Folded: eigenvectors ordered even (group 0), then odd (group 1); the forward operator acts on
+//    the panels [v_E ; u_E] (group 0) and [v_O ; u_O] (group 1) with v = alpha u_I + dt S_I folded
+//    over the interior and u over all nz rows; B z unfolds (p from z_E, q from z_O).
 //  Poisson (all nz unknowns), with Qf Lf Qf^T = W^{-1/2} A W^{-1/2}:
 //    y = Tf div + (Tf D1) w,  z = -y / (dt (lf + k^2)),  phi = Bf z,  D1 phi = (D1 Bf) z,
-//    Tf = Qf^T W^{1/2}, Bf = W^{-1/2} Qf.  Mode (0,0): phi_0 = 0, phi_{1:} = M0 b_{1:} with
+//    Tf = Qf^T W^{1/2}, Bf = W^{-1/2} Qf.  Folded: group 0 (even) on [div_E ; w_O], group 1 (odd)
+//    on [div_O ; w_E]; phi unfolds from (p = Bf_E z_E, q = Bf_O z_O), D1 phi from
+//    (s = (D1 Bf)_O z_O, r = (D1 Bf)_E z_E).  Mode (0,0): phi_0 = 0, phi_{1:} = M0 b_{1:} with
 //    M0 = -(A_{1:,1:})^{-1} and b = w * (D1 w*) / dt, i.e. phi = E0 w*, D1 phi = F0 w* with
 //    E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0.
+//  z derivative: D1 x unfolds from (s = D1 x_odd on X_O, r = D1 x_even on X_E).
 struct Operators {
     int nz = 0, m = 0;
     double f = 0.0;              // theta dt / Re
     std::vector<double> z, wz, d1, d2, stiff;     // d1, d2, stiff column-major nz x nz
-    SwizzledA D1;                // nz x nz
-    SwizzledA hF;                // m x (m + nz): [T | f T D2_I]
-    SwizzledA hB;                // m x m
-    std::vector<double> hc1, hc2, hl;   // length m
-    SwizzledA pF;                // nz x 2nz: [Tf | Tf D1]
-    SwizzledA pB;                // nz x nz: Bf
-    SwizzledA pDB;               // nz x nz: D1 Bf
-    std::vector<double> pl;      // length nz
+    FoldedA zD1;                 // s: (h+r) rows on X_O | r: h rows on X_E, K = h + r
+    FoldedA hF;                  // even | odd eigen rows on [v ; u] panels, K = mE + (h + r)
+    FoldedA hB;                  // p | q rows on z_E | z_O, K = mE
+    std::vector<double> hc1, hc2, hl;   // length m, even eigen rows first
+    FoldedA pF;                  // even | odd eigen rows on [div_E ; w_O] | [div_O ; w_E]
+    FoldedA pB;                  // p | q on z_E | z_O
+    FoldedA pDB;                 // s | r on z_O | z_E
+    std::vector<double> pl;      // length nz, even first
     std::vector<double> m0;      // (nz-1) x (nz-1) row-major
     std::vector<double> e0, f0;  // nz x nz row-major: mode-(0,0) phi = E0 w*, D1 phi = F0 w*
     double helm_recon_err = 0, pois_recon_err = 0;   // max |reconstruction - operator|, relative

@@ -13,6 +13,7 @@
 namespace dns {
 
 static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to 256 threads");
+static_assert(KC == 16, "a chunk is 16 panel rows");
 
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
 // the x Nyquist column gives an anti-Hermitian column (zero after the round trip), and so
@@ -25,56 +26,82 @@ __device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx,
 // -------------------------------------------------------------------------------------
 // z derivative: out = D1 in  (compute_derivatives_3d dz, DNS_pressure_BC_3D.f90:2348-2358),
 // or the vorticity components that need it: omega_x = i ky w - D1 v, omega_y = D1 u - i kx w
-// (DNS_pressure_BC_3D.f90:1077-1079).  One block = 16 modes, all nz rows.
+// (DNS_pressure_BC_3D.f90:1077-1079).  Folded: s = D1 x_odd on X_O (rows up to the centre),
+// r = D1 x_even on X_E, out_k = s_k + r_k, out_{nz-1-k} = s_k - r_k.  One block = 16 modes.
 // -------------------------------------------------------------------------------------
-// Tile (cb, by, mb): column block cb of member mb; by = field (Z_PLAIN).
 template <int RBW, int MODE>
 __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int mb, double* lds) {
-    const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
-    const int mode0 = cb * BMODES;
+    const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
+    const int wave = threadIdx.x >> 6;
+    const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
+    const bool sval = smode < H;
+    const size_t rs = size_t(H2) / 2;
     const int nfield = MODE == Z_VORT ? 2 : 1;
     const size_t mo = size_t(mb) * a.mstride;
     const double* w_in = a.in[2] + mo;
     for (int q = 0; q < nfield; q++) {
         const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : by;   // vort: v first, then u
-        const double* in = a.in[f] + mo;
-        d4 acc[RBW][CT];
-        zero_acc2(acc);
-        gemm_staged<RBW>(acc, a.Asw, RB, a.KS, [&](int k, int ml) -> double2 {
-            const int mode = mode0 + ml;
-            if (k < nz && mode < H) return reinterpret_cast<const double2*>(in + size_t(k) * H2)[mode];
-            return make_double2(0.0, 0.0);
-        }, lds);
+        const double2* x2 = reinterpret_cast<const double2*>(a.in[f] + mo) + (sval ? smode : 0);
+        auto fold = [&](int kf) -> FoldPair {
+            FoldPair r{make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+            if (!sval) return r;
+            if (kf < hF) {
+                const double2 xa = x2[kf * rs], xb = x2[(nz - 1 - kf) * rs];
+                r.p0 = make_double2(xa.x - xb.x, xa.y - xb.y);
+                r.p1 = make_double2(xa.x + xb.x, xa.y + xb.y);
+            } else if (kf < LE) {
+                r.p1 = x2[hF * rs];
+            }
+            return r;
+        };
+        auto emit = [&](int node, int col, double y) {
+            if (MODE == Z_PLAIN) {
+                a.out[f][mo + size_t(node) * H2 + col] = y;
+                return;
+            }
+            const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
+            const double wo = w_in[size_t(node) * H2 + (col ^ 1)];   // the other part of w
+            const double wc = w_in[size_t(node) * H2 + col];
+            const double wr = part == 0 ? wc : wo, wi = part == 0 ? wo : wc;
+            double res;
+            if (f == 1) {        // omega_x = i ky w - D1 v
+                const double ky = a.yw[j];
+                res = part == 0 ? (-ky * wi - y) : (ky * wr - y);
+            } else {             // omega_y = D1 u - i kx w
+                const double kx = a.xw[i];
+                res = part == 0 ? (y + kx * wi) : (y - kx * wr);
+            }
+            const int o = f == 1 ? 0 : 1;
+            if (a.out_f32) reinterpret_cast<float*>(a.out[o])[mo + size_t(node) * H2 + col] = float(res);
+            else a.out[o][mo + size_t(node) * H2 + col] = res;
+        };
+        d4 acc0[RBW][CT], acc1[RBW][CT];
+        zero_acc2(acc0);
+        zero_acc2(acc1);
+        double2 xacc = make_double2(0.0, 0.0);
+        fold_gemm<RBW>(acc0, acc1, a.d1, fold, lds, xacc);
+        if (a.d1.extra) {
+            // the centre row s_h (out_h = s_h)
+            const double xs = reduce_extra(xacc, lds);
+            const int col = cb * BCOLS + int(threadIdx.x);
+            if (threadIdx.x < BCOLS && col < H2) emit(hF, col, xs);
+        }
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
             const int col = out_col(ct, cb);
             if (col >= H2) continue;
-            const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
-#pragma unroll 2
+                const int b = wave + 4 * ii;
+                if (b >= a.d1.rb0) continue;
+#pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int row = out_row(ii, r);
-                    if (row >= nz) continue;
-                    const double y = acc[ii][ct][r];
-                    if (MODE == Z_PLAIN) {
-                        a.out[f][mo + size_t(row) * H2 + col] = y;
-                    } else {
-                        double res;
-                        const double wo = w_in[size_t(row) * H2 + (col ^ 1)];   // the other part of w
-                        const double wr = part == 0 ? w_in[size_t(row) * H2 + col] : wo;
-                        const double wi = part == 0 ? wo : w_in[size_t(row) * H2 + col];
-                        if (f == 1) {        // omega_x = i ky w - D1 v
-                            const double ky = a.yw[j];
-                            res = part == 0 ? (-ky * wi - y) : (ky * wr - y);
-                        } else {             // omega_y = D1 u - i kx w
-                            const double kx = a.xw[i];
-                            res = part == 0 ? (y + kx * wi) : (y - kx * wr);
-                        }
-                        const int o = f == 1 ? 0 : 1;
-                        if (a.out_f32) reinterpret_cast<float*>(a.out[o])[mo + size_t(row) * H2 + col] = float(res);
-                        else a.out[o][mo + size_t(row) * H2 + col] = res;
-                    }
+                    if (row >= LE) continue;
+                    const double sv = acc0[ii][ct][r];
+                    const double rv = (row < hF && b < a.d1.rb1) ? acc1[ii][ct][r] : 0.0;
+                    emit(row, col, sv + rv);
+                    if (row < hF) emit(nz - 1 - row, col, sv - rv);
                 }
             }
         }
@@ -83,12 +110,12 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
 
 template <int RBW, int MODE>
 __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
-    __shared__ double lds[2 * KC * LDB];
+    __shared__ double lds[STAGE_LDS];
     zderiv_tile<RBW, MODE>(a, blockIdx.x, blockIdx.y, blockIdx.z, lds);
 }
 template <int RBW, int MODE>
 __global__ __launch_bounds__(256) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
-    __shared__ double lds[2 * KC * LDB];
+    __shared__ double lds[STAGE_LDS];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
     zderiv_tile<RBW, MODE>(a, t.x, blockIdx.y, t.z, lds);
@@ -102,26 +129,69 @@ __global__ __launch_bounds__(256) void k_zderiv_rag(const ZArgs* __restrict__ d,
 //   z = y / (hl + f k^2), u*_I = B z, u*_0 = bc1, u*_{nz-1} = bc2,
 // alpha = 1 - f k^2 (CN) or 1, S' = S + G nx ny at mode (0,0) of the u component, and
 // bc = -dt (i k_eff p_hat) at the walls (the Kim & Moin values -dt grad p, :1325-1328).
+// Folded: even eigen rows on [v_E ; u_E], odd rows on [v_O ; u_O] (v = alpha u_I + dt S'_I over
+// the interior, u over the whole column); u*_I unfolds from p = B_E z_E and q = B_O z_O.
 // -------------------------------------------------------------------------------------
 template <int RBW>
-constexpr int solve_lds() { return (4 * RBW * 16 > 2 * KC ? 4 * RBW * 16 : 2 * KC) * LDB; }
-
-template <int RBW>
 __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int comp, int mb, double* smem) {
-    const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2, RBI = (m + 15) / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int mode0 = cb * BMODES;
-    const size_t mo = size_t(mb) * a.mstride;
+    const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2;
+    const int hI = m / 2, L1 = hI + (m & 1), hF = nz / 2, LE = hF + (nz & 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
+    const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
     const double* uh = a.uh[comp] + mo;
-    const double* sh = a.s_f32 ? reinterpret_cast<const double*>(reinterpret_cast<const float*>(a.sh[comp]) + mo)
-                               : a.sh[comp] + mo;
     const double* ph = a.ph + mo;
-    const double* Fsw = a.Fsw + size_t(mb) * a.fstride;
+    FoldOp F = a.hF;
+    F.A += size_t(mb) * a.fstride;
     const double* c1v = a.c1 + size_t(mb) * m;
     const double* c2v = a.c2 + size_t(mb) * m;
     const double* hlv = a.hl + size_t(mb) * m;
     const double fco = a.fvec[mb];
     const double force = comp == 0 ? a.flow[mb].G * a.nxny : 0.0;
+    // staging thread: its mode's constants, once
+    const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
+    const bool sval = smode < H;
+    double alpha = 1.0, fr = 0.0;
+    if (sval) {
+        const int j = smode / a.nxh, i = smode - j * a.nxh;
+        const double kx = a.xw[i], ky = a.yw[j];
+        alpha = a.cn ? 1.0 - fco * (kx * kx + ky * ky) : 1.0;
+        fr = smode == 0 ? force : 0.0;
+    }
+    const double2* u2 = reinterpret_cast<const double2*>(uh) + (sval ? smode : 0);
+    const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.sh[comp]) + mo) + (sval ? smode : 0);
+    const double2* sd2 = reinterpret_cast<const double2*>(a.sh[comp] + mo) + (sval ? smode : 0);
+    auto vval = [&](int node) -> double2 {
+        const double2 u = u2[node * rs];
+        double2 sv;
+        if (a.s_f32) {
+            const float2 sf = sf2[node * rs];
+            sv = make_double2(double(sf.x), double(sf.y));
+        } else {
+            sv = sd2[node * rs];
+        }
+        return make_double2(alpha * u.x + a.dt * (sv.x + fr), alpha * u.y + a.dt * sv.y);
+    };
+    auto fold = [&](int kf) -> FoldPair {
+        FoldPair r{make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+        if (!sval) return r;
+        if (kf < hI) {
+            const double2 va = vval(kf + 1), vb = vval(m - kf);
+            r.p0 = make_double2(va.x + vb.x, va.y + vb.y);
+            r.p1 = make_double2(va.x - vb.x, va.y - vb.y);
+        } else if (kf < L1) {
+            r.p0 = vval(hI + 1);
+        } else {
+            const int c = kf - L1;
+            if (c < hF) {
+                const double2 ua = u2[c * rs], ub = u2[(nz - 1 - c) * rs];
+                r.p0 = make_double2(ua.x + ub.x, ua.y + ub.y);
+                r.p1 = make_double2(ua.x - ub.x, ua.y - ub.y);
+            } else if (c < LE) {
+                r.p0 = u2[hF * rs];
+            }
+        }
+        return r;
+    };
     // per-lane column constants for the two 16-column tiles
     double k2c[CT], bc1[CT], bc2[CT];
 #pragma unroll
@@ -141,59 +211,64 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
             }
         }
     }
-    d4 acc[RBW][CT];
-    zero_acc2(acc);
-    gemm_staged<RBW>(acc, Fsw, RBI, a.FKS, [&](int k, int ml) -> double2 {
-        const int mode = mode0 + ml;
-        if (mode >= H) return make_double2(0.0, 0.0);
-        if (k < m) {
-            const int j = mode / a.nxh, i = mode - j * a.nxh;
-            const double kx = a.xw[i], ky = a.yw[j];
-            const double alpha = a.cn ? 1.0 - fco * (kx * kx + ky * ky) : 1.0;
-            const double2 u = reinterpret_cast<const double2*>(uh + size_t(k + 1) * H2)[mode];
-            double2 sv;
-            if (a.s_f32) {
-                const float2 sf = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(sh) + size_t(k + 1) * H2)[mode];
-                sv = make_double2(double(sf.x), double(sf.y));
-            } else {
-                sv = reinterpret_cast<const double2*>(sh + size_t(k + 1) * H2)[mode];
-            }
-            const double fr = mode == 0 ? force : 0.0;
-            return make_double2(alpha * u.x + a.dt * (sv.x + fr), alpha * u.y + a.dt * sv.y);
-        }
-        if (k < m + nz) return reinterpret_cast<const double2*>(uh + size_t(k - m) * H2)[mode];
-        return make_double2(0.0, 0.0);
-    }, smem);
-    double* zs = smem;
+    d4 acc0[RBW][CT], acc1[RBW][CT];
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    double2 xacc = make_double2(0.0, 0.0);
+    fold_gemm<RBW>(acc0, acc1, F, fold, smem, xacc);
+    const double xv = F.extra ? reduce_extra(xacc, smem) : 0.0;
+    const int ZR = zpanel_rows(a.hB.ks);
+    double* z0 = smem;
+    double* z1 = smem + ZR * LDB;
+    for (int e = threadIdx.x; e < 2 * ZR * LDB; e += 256) smem[e] = 0.0;
+    __syncthreads();
+    if (F.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2) {
+        const int row = F.rows0 - 1, ct = threadIdx.x >> 4;
+        z0[row * LDB + threadIdx.x] = (xv + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
+    }
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
+        const bool cv = out_col(ct, cb) < H2;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
-            if (wave + 4 * ii >= RBI) continue;
+            const int b = wave + 4 * ii;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                double z = 0.0;
-                if (row < m && out_col(ct, cb) < H2)
-                    z = (acc[ii][ct][r] + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
-                zs[row * LDB + ct * 16 + (lane & 15)] = z;
+                if (cv && b < F.rb0 && row < F.rows0)
+                    z0[row * LDB + ct * 16 + cl] =
+                        (acc0[ii][ct][r] + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
+                if (cv && b < F.rb1 && row < F.rows1) {
+                    const int g = F.rows0 + row;
+                    z1[row * LDB + ct * 16 + cl] =
+                        (acc1[ii][ct][r] + c1v[g] * bc1[ct] + c2v[g] * bc2[ct]) / (hlv[g] + fco * k2c[ct]);
+                }
             }
         }
     }
     __syncthreads();
-    zero_acc2(acc);
-    gemm_lds<RBW>(acc, a.Bsw, RBI, a.BKS, zs);
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    fold_gemm_lds<RBW>(acc0, acc1, a.hB, z0, z1);
     double* out = a.out[comp] + mo;
+    if (a.hB.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2)   // the interior centre row
+        out[size_t(a.hB.rows0) * H2 + cb * BCOLS + threadIdx.x] = extra_dot(a.hB, z0);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb);
         if (col >= H2) continue;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
+            const int b = wave + 4 * ii;
+            if (b >= a.hB.rb0) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                if (wave + 4 * ii < RBI && row < m) out[size_t(row + 1) * H2 + col] = acc[ii][ct][r];
+                if (row >= L1) continue;
+                const double p = acc0[ii][ct][r];
+                const double q = (row < hI && b < a.hB.rb1) ? acc1[ii][ct][r] : 0.0;
+                out[size_t(row + 1) * H2 + col] = p + q;
+                if (row < hI) out[size_t(m - row) * H2 + col] = p - q;
             }
         }
         if (wave == 0 && (lane >> 4) == 0) {
@@ -205,12 +280,12 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
 
 template <int RBW>
 __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
-    __shared__ double smem[solve_lds<RBW>()];
+    extern __shared__ double smem[];
     helmholtz_tile<RBW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW>
 __global__ __launch_bounds__(256) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
-    __shared__ double smem[solve_lds<RBW>()];
+    extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
     helmholtz_tile<RBW>(a, t.x, blockIdx.y, t.z, smem);
@@ -223,17 +298,55 @@ __global__ __launch_bounds__(256) void k_helmholtz_rag(const HelmArgs* __restric
 //   y = Tf div + (Tf D1) w*,  div = i kx_eff u* + i ky_eff v*,
 //   z = -y / (dt (lf + k^2)), phi = Bf z, D1 phi = (D1 Bf) z,
 //   u = u* - dt i kx_eff phi, v = v* - dt i ky_eff phi, w = w* - dt D1 phi, p += phi.
-// Mode (0,0) is left to k_poisson_mode0.
+// Folded: even rows on [div_E ; w_O], odd rows on [div_O ; w_E]; phi unfolds from
+// (Bf_E z_E, Bf_O z_O), D1 phi from ((D1 Bf)_O z_O, (D1 Bf)_E z_E).  Mode (0,0) is left to
+// k_poisson_mode0.
 // -------------------------------------------------------------------------------------
 template <int RBW>
 __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
-    const int H2 = a.H2, nz = a.nz, H = H2 / 2, RB = (nz + 15) / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int mode0 = cb * BMODES;
-    const size_t mo = size_t(mb) * a.mstride;
+    const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
+    const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
     const double* us = a.us + mo; const double* vs = a.vs + mo; const double* ws = a.ws + mo;
     double* uo = a.u + mo; double* vo = a.v + mo; double* wo = a.w + mo; double* po = a.p + mo;
     double* phio = a.phi + mo; double* dphio = a.dphi + mo;
+    // staging thread's mode
+    const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
+    const bool sval = smode < H && smode != 0;
+    double skx = 0.0, sky = 0.0;
+    if (sval) {
+        const int j = smode / a.nxh, i = smode - j * a.nxh;
+        skx = kx_eff(a.xw, i, a.nx);
+        sky = ky_eff(a.yw, i, j, a.nx, a.ny);
+    }
+    const double2* us2 = reinterpret_cast<const double2*>(us) + (sval ? smode : 0);
+    const double2* vs2 = reinterpret_cast<const double2*>(vs) + (sval ? smode : 0);
+    const double2* ws2 = reinterpret_cast<const double2*>(ws) + (sval ? smode : 0);
+    auto div = [&](int node) -> double2 {
+        const double2 u = us2[node * rs], v = vs2[node * rs];
+        return make_double2(-skx * u.y - sky * v.y, skx * u.x + sky * v.x);
+    };
+    auto fold = [&](int kf) -> FoldPair {
+        FoldPair r{make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+        if (!sval) return r;
+        if (kf < hF) {
+            const double2 da = div(kf), db = div(nz - 1 - kf);
+            r.p0 = make_double2(da.x + db.x, da.y + db.y);
+            r.p1 = make_double2(da.x - db.x, da.y - db.y);
+        } else if (kf < LE) {
+            r.p0 = div(hF);
+        } else {
+            const int c = kf - LE;
+            if (c < hF) {
+                const double2 wa = ws2[c * rs], wb = ws2[(nz - 1 - c) * rs];
+                r.p0 = make_double2(wa.x - wb.x, wa.y - wb.y);
+                r.p1 = make_double2(wa.x + wb.x, wa.y + wb.y);
+            } else if (c < LE) {
+                r.p1 = ws2[hF * rs];
+            }
+        }
+        return r;
+    };
     double k2c[CT], kxe[CT], kye[CT];
     bool live[CT];
 #pragma unroll
@@ -249,100 +362,153 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
             kye[ct] = ky_eff(a.yw, i, j, a.nx, a.ny);
         }
     }
-    d4 acc[RBW][CT];
-    zero_acc2(acc);
-    gemm_staged<RBW>(acc, a.Fsw, RB, a.FKS, [&](int k, int ml) -> double2 {
-        const int mode = mode0 + ml;
-        if (mode >= H || mode == 0) return make_double2(0.0, 0.0);
-        if (k < nz) {
-            const int j = mode / a.nxh, i = mode - j * a.nxh;
-            const double ke = kx_eff(a.xw, i, a.nx), kf = ky_eff(a.yw, i, j, a.nx, a.ny);
-            const double2 u = reinterpret_cast<const double2*>(us + size_t(k) * H2)[mode];
-            const double2 v = reinterpret_cast<const double2*>(vs + size_t(k) * H2)[mode];
-            return make_double2(-ke * u.y - kf * v.y, ke * u.x + kf * v.x);
-        }
-        if (k < 2 * nz) return reinterpret_cast<const double2*>(ws + size_t(k - nz) * H2)[mode];
-        return make_double2(0.0, 0.0);
-    }, smem);
-    double* zs = smem;
+    d4 acc0[RBW][CT], acc1[RBW][CT];
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    double2 xacc = make_double2(0.0, 0.0);
+    fold_gemm<RBW>(acc0, acc1, a.pF, fold, smem, xacc);
+    const double xv = a.pF.extra ? reduce_extra(xacc, smem) : 0.0;
+    const int ZR = zpanel_rows(a.pB.ks);
+    double* z0 = smem;
+    double* z1 = smem + ZR * LDB;
+    for (int e = threadIdx.x; e < 2 * ZR * LDB; e += 256) smem[e] = 0.0;
+    __syncthreads();
+    if (a.pF.extra && threadIdx.x < BCOLS) {
+        const int row = a.pF.rows0 - 1, ct = threadIdx.x >> 4;
+        if (live[ct]) z0[row * LDB + threadIdx.x] = -xv / (a.dt * (a.pl[row] + k2c[ct]));
+    }
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
-            if (wave + 4 * ii >= RB) continue;
+            const int b = wave + 4 * ii;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                double z = 0.0;
-                if (row < nz && live[ct]) z = -acc[ii][ct][r] / (a.dt * (a.pl[row] + k2c[ct]));
-                zs[row * LDB + ct * 16 + (lane & 15)] = z;
+                if (live[ct] && b < a.pF.rb0 && row < a.pF.rows0)
+                    z0[row * LDB + ct * 16 + cl] = -acc0[ii][ct][r] / (a.dt * (a.pl[row] + k2c[ct]));
+                if (live[ct] && b < a.pF.rb1 && row < a.pF.rows1)
+                    z1[row * LDB + ct * 16 + cl] = -acc1[ii][ct][r] / (a.dt * (a.pl[a.pF.rows0 + row] + k2c[ct]));
             }
         }
     }
     __syncthreads();
     // phi = Bf z: pressure increment and the horizontal projection
-    zero_acc2(acc);
-    gemm_lds<RBW>(acc, a.Bsw, RB, a.BKS, zs);
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    fold_gemm_lds<RBW>(acc0, acc1, a.pB, z0, z1);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb);
         const int part = col & 1;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
+            const int b = wave + 4 * ii;
+            if (b >= a.pB.rb0) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                const double ph = acc[ii][ct][r];
-                const double other = __shfl_xor(ph, 1);       // the other part of this mode
-                if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
-                const size_t q = size_t(row) * H2 + col;
-                if (!a.fused) {
-                    phio[q] = ph;
-                    continue;
+                const double p = acc0[ii][ct][r];
+                const double q = (row < hF && b < a.pB.rb1) ? acc1[ii][ct][r] : 0.0;
+                const double pha = p + q, phb = p - q;
+                const double oa = __shfl_xor(pha, 1), ob = __shfl_xor(phb, 1);   // the other part of the mode
+                if (!live[ct] || row >= LE) continue;
+#pragma unroll
+                for (int side = 0; side < 2; side++) {
+                    if (side == 1 && row >= hF) continue;
+                    const int node = side == 0 ? row : nz - 1 - row;
+                    const double ph = side == 0 ? pha : phb, other = side == 0 ? oa : ob;
+                    const size_t qn = size_t(node) * H2 + col;
+                    if (!a.fused) {
+                        phio[qn] = ph;
+                        continue;
+                    }
+                    const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
+                    const bool wall = node == 0 || node == nz - 1;
+                    // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
+                    const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
+                    const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
+                    uo[qn] = wall ? 0.0 : us[qn] - a.dt * du;
+                    vo[qn] = wall ? 0.0 : vs[qn] - a.dt * dv;
+                    po[qn] = po[qn] + ph;
                 }
-                const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
-                const bool wall = row == 0 || row == nz - 1;
-                // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
-                const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
-                const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
-                uo[q] = wall ? 0.0 : us[q] - a.dt * du;
-                vo[q] = wall ? 0.0 : vs[q] - a.dt * dv;
-                po[q] = po[q] + ph;
             }
         }
     }
-    // D1 phi = (D1 Bf) z: the wall-normal projection
-    zero_acc2(acc);
-    gemm_lds<RBW>(acc, a.DBsw, RB, a.BKS, zs);
+    if (a.pB.extra && wave == 0) {
+        // the centre row phi_h (odd part zero there); lanes t and t^1 hold the two parts of a mode
+        const int t = threadIdx.x, ct = (t >> 4) & 1;
+        const double ph = t < BCOLS ? extra_dot(a.pB, z0) : 0.0;
+        const double other = __shfl_xor(ph, 1);
+        const int col = cb * BCOLS + t, node = a.pB.rows0 - 1;
+        if (t < BCOLS && live[ct]) {
+            const size_t qn = size_t(node) * H2 + col;
+            if (!a.fused) {
+                phio[qn] = ph;
+            } else {
+                const int part = col & 1;
+                const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
+                const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
+                const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
+                uo[qn] = us[qn] - a.dt * du;
+                vo[qn] = vs[qn] - a.dt * dv;
+                po[qn] = po[qn] + ph;
+            }
+        }
+    }
+    // D1 phi = (D1 Bf) z: the wall-normal projection (s on z_O, r on z_E)
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    fold_gemm_lds<RBW>(acc0, acc1, a.pDB, z1, z0);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb);
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
+            const int b = wave + 4 * ii;
+            if (b >= a.pDB.rb0) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                if (!live[ct] || wave + 4 * ii >= RB || row >= nz) continue;
-                const size_t q = size_t(row) * H2 + col;
-                if (!a.fused) {
-                    dphio[q] = acc[ii][ct][r];
-                    continue;
+                if (!live[ct] || row >= LE) continue;
+                const double s = acc0[ii][ct][r];
+                const double rv = (row < hF && b < a.pDB.rb1) ? acc1[ii][ct][r] : 0.0;
+#pragma unroll
+                for (int side = 0; side < 2; side++) {
+                    if (side == 1 && row >= hF) continue;
+                    const int node = side == 0 ? row : nz - 1 - row;
+                    const double dph = side == 0 ? s + rv : s - rv;
+                    const size_t qn = size_t(node) * H2 + col;
+                    if (!a.fused) {
+                        dphio[qn] = dph;
+                        continue;
+                    }
+                    const bool wall = node == 0 || node == nz - 1;
+                    wo[qn] = wall ? 0.0 : ws[qn] - a.dt * dph;
                 }
-                const bool wall = row == 0 || row == nz - 1;
-                wo[q] = wall ? 0.0 : ws[q] - a.dt * acc[ii][ct][r];
             }
+        }
+    }
+    if (a.pDB.extra && threadIdx.x < BCOLS) {
+        // the centre row of D1 phi (s_h, on z_O)
+        const int t = threadIdx.x, ct = t >> 4, col = cb * BCOLS + t, node = a.pDB.rows0 - 1;
+        if (live[ct]) {
+            const double dph = extra_dot(a.pDB, z1);
+            const size_t qn = size_t(node) * H2 + col;
+            if (!a.fused) dphio[qn] = dph;
+            else wo[qn] = ws[qn] - a.dt * dph;
         }
     }
 }
 
 template <int RBW>
 __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
-    __shared__ double smem[solve_lds<RBW>()];
+    extern __shared__ double smem[];
     poisson_tile<RBW>(a, blockIdx.x, blockIdx.z, smem);
 }
 template <int RBW>
 __global__ __launch_bounds__(256) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
-    __shared__ double smem[solve_lds<RBW>()];
+    extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const PoisArgs& a = d[t.g];
     poisson_tile<RBW>(a, t.x, t.z, smem);
@@ -537,43 +703,46 @@ __global__ void k_reduce_final(ReduceArgs a) {
 // launchers
 // ---------------------------------------------------------------------------------------
 static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
-static inline int rbw_of(int rows) { return ((rows + 15) / 16 + 3) / 4; }
 
-hipError_t launch_zderiv(int rb, int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
-    const int rbw = (rb + 3) / 4;
+size_t helm_lds_bytes(const HelmArgs& a) {
+    return sizeof(double) * std::max<size_t>(STAGE_LDS, size_t(2) * zpanel_rows(a.hB.ks) * LDB);
+}
+size_t pois_lds_bytes(const PoisArgs& a) {
+    return sizeof(double) * std::max<size_t>(STAGE_LDS, size_t(2) * zpanel_rows(a.pB.ks) * LDB);
+}
+
+hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
     g.z = nm;
-    switch (rbw * 2 + (mode == Z_VORT)) {
+    switch (fold_rbw(a.d1) * 2 + (mode == Z_VORT)) {
         case 2: hipLaunchKernelGGL((k_zderiv<1, Z_PLAIN>), g, dim3(256), 0, s, a); break;
         case 3: hipLaunchKernelGGL((k_zderiv<1, Z_VORT>), g, dim3(256), 0, s, a); break;
         case 4: hipLaunchKernelGGL((k_zderiv<2, Z_PLAIN>), g, dim3(256), 0, s, a); break;
         case 5: hipLaunchKernelGGL((k_zderiv<2, Z_VORT>), g, dim3(256), 0, s, a); break;
-        case 6: hipLaunchKernelGGL((k_zderiv<3, Z_PLAIN>), g, dim3(256), 0, s, a); break;
-        case 7: hipLaunchKernelGGL((k_zderiv<3, Z_VORT>), g, dim3(256), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_helmholtz(int rbi, const HelmArgs& a, int nm, hipStream_t s) {
+hipError_t launch_helmholtz(const HelmArgs& a, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, 3);
     g.z = nm;
-    switch ((rbi + 3) / 4) {
-        case 1: hipLaunchKernelGGL((k_helmholtz<1>), g, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_helmholtz<2>), g, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_helmholtz<3>), g, dim3(256), 0, s, a); break;
+    const size_t lds = helm_lds_bytes(a);
+    switch (fold_rbw(a.hF)) {
+        case 1: hipLaunchKernelGGL((k_helmholtz<1>), g, dim3(256), lds, s, a); break;
+        case 2: hipLaunchKernelGGL((k_helmholtz<2>), g, dim3(256), lds, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_poisson(int rb, const PoisArgs& a, int nm, hipStream_t s) {
+hipError_t launch_poisson(const PoisArgs& a, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, 1);
     g.z = nm;
-    switch ((rb + 3) / 4) {
-        case 1: hipLaunchKernelGGL((k_poisson<1>), g, dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_poisson<2>), g, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_poisson<3>), g, dim3(256), 0, s, a); break;
+    const size_t lds = pois_lds_bytes(a);
+    switch (fold_rbw(a.pF)) {
+        case 1: hipLaunchKernelGGL((k_poisson<1>), g, dim3(256), lds, s, a); break;
+        case 2: hipLaunchKernelGGL((k_poisson<2>), g, dim3(256), lds, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -605,29 +774,25 @@ hipError_t launch_zderiv_rag(int rbw, int mode, const ZArgs* d, const Tile* map,
         case 3: hipLaunchKernelGGL((k_zderiv_rag<1, Z_VORT>), g, dim3(256), 0, s, d, map); break;
         case 4: hipLaunchKernelGGL((k_zderiv_rag<2, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
         case 5: hipLaunchKernelGGL((k_zderiv_rag<2, Z_VORT>), g, dim3(256), 0, s, d, map); break;
-        case 6: hipLaunchKernelGGL((k_zderiv_rag<3, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
-        case 7: hipLaunchKernelGGL((k_zderiv_rag<3, Z_VORT>), g, dim3(256), 0, s, d, map); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int nt, hipStream_t s) {
+hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     const dim3 g(nt, 3);
     switch (rbw) {
-        case 1: hipLaunchKernelGGL((k_helmholtz_rag<1>), g, dim3(256), 0, s, d, map); break;
-        case 2: hipLaunchKernelGGL((k_helmholtz_rag<2>), g, dim3(256), 0, s, d, map); break;
-        case 3: hipLaunchKernelGGL((k_helmholtz_rag<3>), g, dim3(256), 0, s, d, map); break;
+        case 1: hipLaunchKernelGGL((k_helmholtz_rag<1>), g, dim3(256), lds, s, d, map); break;
+        case 2: hipLaunchKernelGGL((k_helmholtz_rag<2>), g, dim3(256), lds, s, d, map); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int nt, hipStream_t s) {
+hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     switch (rbw) {
-        case 1: hipLaunchKernelGGL((k_poisson_rag<1>), dim3(nt), dim3(256), 0, s, d, map); break;
-        case 2: hipLaunchKernelGGL((k_poisson_rag<2>), dim3(nt), dim3(256), 0, s, d, map); break;
-        case 3: hipLaunchKernelGGL((k_poisson_rag<3>), dim3(nt), dim3(256), 0, s, d, map); break;
+        case 1: hipLaunchKernelGGL((k_poisson_rag<1>), dim3(nt), dim3(256), lds, s, d, map); break;
+        case 2: hipLaunchKernelGGL((k_poisson_rag<2>), dim3(nt), dim3(256), lds, s, d, map); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -642,6 +807,16 @@ hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int nt, h
     hipLaunchKernelGGL(k_flow_control_rag, dim3(nt), dim3(64), 0, s, d, map);
     return hipGetLastError();
 }
+// plain copy of n doubles (calibration of the HBM byte counters)
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ x, double2* __restrict__ y, size_t n2) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n2; i += size_t(gridDim.x) * blockDim.x) y[i] = x[i];
+}
+hipError_t launch_copy(const double* x, double* y, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, reinterpret_cast<const double2*>(x),
+                       reinterpret_cast<double2*>(y), n / 2);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_fields(const InitArgs& a, hipStream_t s) {
     const size_t N = size_t(a.npl) * a.nz;
     const int blocks = int(std::min<size_t>((N + 255) / 256, 8192));

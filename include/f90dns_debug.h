@@ -31,6 +31,10 @@ int dns_debug_initial_field(const dns_params* p, double* u, double* v, double* w
  * (which: 0 Helmholtz forward, 1 back, 2 c1, 3 c2, 4 eigenvalues, 5 Poisson forward, 6 back,
  * 7 D1 back, 8 eigenvalues, 9 mode-(0,0) matrix, 10 D1, 11 LGL weights, 12 D2). */
 int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols);
+/* repeats launches of a plain copy kernel over n doubles (n even) on fresh buffers; mean ms
+ * per launch in ms_out.  Calibrates the HBM byte counters of a profiler: each launch reads
+ * and writes 8n bytes. */
+int dns_debug_copy(size_t n, int repeats, double* ms_out);
 /* max relative reconstruction error of the Helmholtz and Poisson eigen bases. */
 int dns_debug_operator_error(dns_ctx* ctx, double* out2);
 
