@@ -439,9 +439,9 @@ class Ensemble:
         self._check(lib().dns_ensemble_step_timing(self._h, out))
         return dict(min_ms=out[0], p50_ms=out[1], p99_ms=out[2], mean_ms=out[3])
 
-    def set_tuning(self, lanes=1, interleave=0, promote_fp64=(0, 0, 0), promote_fp32=(0, 0, 0)):
-        a = (ctypes.c_int * 3)(*promote_fp64)
-        b = (ctypes.c_int * 3)(*promote_fp32)
+    def set_tuning(self, lanes=1, interleave=0, promote_fp64=(0, 0, 0, 0), promote_fp32=(0, 0, 0, 0)):
+        a = (ctypes.c_int * 4)(*(list(promote_fp64) + [0] * 4)[:4])
+        b = (ctypes.c_int * 4)(*(list(promote_fp32) + [0] * 4)[:4])
         self._check(lib().dns_ensemble_set_tuning(self._h, lanes, interleave, a, b))
 
     def autotune(self, steps_per_trial=5):

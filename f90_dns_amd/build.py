@@ -18,8 +18,10 @@ HEADERS = ["setup.h", "device_common.h", "kernels.h", "contract.h", "context.h",
 ARCH = os.environ.get("F90DNS_ARCH", "gfx950")
 # packed FP32 instructions are disabled: the device code relies on scalar/vector FP64 and
 # plain FP32 arithmetic only.
+# MFMA accumulators in VGPRs (the AGPR form made the compiler copy them around every MFMA)
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-amdgpu-mfma-vgpr-form",
+         "-Wall", "-Wno-unused-function"]
 
 
 def hipcc():

@@ -34,27 +34,27 @@ template <int RBW, class FOLD>
 __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FOLD&& fold,
                                           double* lds, double2& xacc) {
     const double* xrow = fold_xrow(op);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, cl = lane & 15;
     const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
     const int nch = (op.ks + 3) / 4;
+    const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
     const double* a0p[RBW];
     const double* a1p[RBW];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + 4 * i;
         a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
-        a1p[i] = op.A + (size_t(op.rb0 + b) * op.ksp) * 64 + lane;
+        a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
     }
     double ar0[RBW][4], ar1[RBW][4], an0[RBW][4], an1[RBW][4];
     auto load = [&](double (&d0)[RBW][4], double (&d1)[RBW][4], int ch) {
 #pragma unroll
         for (int i = 0; i < RBW; i++) {
-            const int b = wave + 4 * i;
 #pragma unroll
             for (int sl = 0; sl < 4; sl++) {
-                d0[i][sl] = b < op.rb0 ? a0p[i][(ch * 4 + sl) * 64] : 0.0;
-                d1[i][sl] = b < op.rb1 ? a1p[i][(ch * 4 + sl) * 64] : 0.0;
+                d0[i][sl] = a0p[i][(ch * 4 + sl) * 64];
+                d1[i][sl] = a1p[i][(ch * 4 + sl) * 64];
             }
         }
     };
@@ -80,30 +80,25 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
             nxt = fold((ch + 1) * KC + rt);
             load(an0, an1, ch + 1);
         }
+        // straight-line MFMA chain: absent blocks have zero A operands, the k-steps past ks zero A
+        // columns and zero panel rows; a wave without blocks skips it (scalar branch)
+        if (active) {
 #pragma unroll
         for (int sl = 0; sl < 4; sl++) {
-            if (ch * 4 + sl < op.ks) {
-                double b0[CT], b1[CT];
+            double b0[CT], b1[CT];
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                b0[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
+                b1[ct] = cur[KC * LDB + (sl * 4 + kk) * LDB + ct * 16 + cl];
+            }
+#pragma unroll
+            for (int i = 0; i < RBW; i++)
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++) {
-                    b0[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
-                    b1[ct] = cur[KC * LDB + (sl * 4 + kk) * LDB + ct * 16 + cl];
+                    acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
+                    acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
                 }
-#pragma unroll
-                for (int i = 0; i < RBW; i++) {
-                    const int b = wave + 4 * i;
-                    if (b < op.rb0) {
-#pragma unroll
-                        for (int ct = 0; ct < CT; ct++)
-                            acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
-                    }
-                    if (b < op.rb1) {
-#pragma unroll
-                        for (int ct = 0; ct < CT; ct++)
-                            acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
-                    }
-                }
-            }
+        }
         }
         if (more) {
             stage(lds + ((ch + 1) & 1) * 2 * KC * LDB, nxt, (ch + 1) * KC + rt);
@@ -119,56 +114,65 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     }
 }
 
-// The same contraction with both panels resident in LDS (z0, z1: >= 4*ks rows, row stride LDB).
+// The same contraction with both panels resident in LDS (z0, z1: >= 4*ks rows, row stride LDB);
+// the A operands are fetched one chunk of 4 k-steps ahead (ksp is a multiple of 4 >= ks + 1,
+// so the chunk after the last one stays inside the block).
 template <int RBW>
 __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op,
                                               const double* z0, const double* z1) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, cl = lane & 15;
+    const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
+    if (!active) return;
+    const int nch = (op.ks + 3) / 4;
     const double* a0p[RBW];
     const double* a1p[RBW];
-    double ar0[RBW], ar1[RBW], an0[RBW], an1[RBW];
+    double ar0[RBW][4], ar1[RBW][4], an0[RBW][4], an1[RBW][4];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + 4 * i;
         a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
-        a1p[i] = op.A + (size_t(op.rb0 + b) * op.ksp) * 64 + lane;
-        ar0[i] = b < op.rb0 ? a0p[i][0] : 0.0;
-        ar1[i] = b < op.rb1 ? a1p[i][0] : 0.0;
+        a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
+#pragma unroll
+        for (int sl = 0; sl < 4; sl++) {
+            ar0[i][sl] = a0p[i][sl * 64];
+            ar1[i][sl] = a1p[i][sl * 64];
+        }
     }
-    for (int s = 0; s < op.ks; s++) {
-        // ksp >= ks + 1: the prefetch of k-step s + 1 stays inside the block
+    for (int ch = 0; ch < nch; ch++) {
+        if (ch + 1 < nch) {
 #pragma unroll
-        for (int i = 0; i < RBW; i++) {
-            const int b = wave + 4 * i;
-            an0[i] = b < op.rb0 ? a0p[i][(s + 1) * 64] : 0.0;
-            an1[i] = b < op.rb1 ? a1p[i][(s + 1) * 64] : 0.0;
-        }
-        double b0[CT], b1[CT];
+            for (int i = 0; i < RBW; i++)
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-            b0[ct] = z0[(4 * s + kk) * LDB + ct * 16 + cl];
-            b1[ct] = z1[(4 * s + kk) * LDB + ct * 16 + cl];
+                for (int sl = 0; sl < 4; sl++) {
+                    an0[i][sl] = a0p[i][((ch + 1) * 4 + sl) * 64];
+                    an1[i][sl] = a1p[i][((ch + 1) * 4 + sl) * 64];
+                }
         }
 #pragma unroll
-        for (int i = 0; i < RBW; i++) {
-            const int b = wave + 4 * i;
-            if (b < op.rb0) {
+        for (int sl = 0; sl < 4; sl++) {
+            const int s = ch * 4 + sl;
+            double b0[CT], b1[CT];
 #pragma unroll
-                for (int ct = 0; ct < CT; ct++)
-                    acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i], b0[ct], acc0[i][ct], 0, 0, 0);
+            for (int ct = 0; ct < CT; ct++) {
+                b0[ct] = z0[(4 * s + kk) * LDB + ct * 16 + cl];
+                b1[ct] = z1[(4 * s + kk) * LDB + ct * 16 + cl];
             }
-            if (b < op.rb1) {
 #pragma unroll
-                for (int ct = 0; ct < CT; ct++)
-                    acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i], b1[ct], acc1[i][ct], 0, 0, 0);
+            for (int i = 0; i < RBW; i++)
+#pragma unroll
+                for (int ct = 0; ct < CT; ct++) {
+                    acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
+                    acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < RBW; i++)
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++) {
+                ar0[i][sl] = an0[i][sl];
+                ar1[i][sl] = an1[i][sl];
             }
-        }
-#pragma unroll
-        for (int i = 0; i < RBW; i++) {
-            ar0[i] = an0[i];
-            ar1[i] = an1[i];
-        }
     }
 }
 
@@ -197,6 +201,35 @@ __device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) 
     return v;
 }
 
+// The two row groups of a folded result into LDS panels G0 (rows0 rows) and G1 (rows1 rows),
+// row stride LDB; the extra group-0 row (computed elsewhere) is written by the caller.
+template <int RBW>
+__device__ __forceinline__ void store_groups(const d4 (&acc0)[RBW][CT], const d4 (&acc1)[RBW][CT], const FoldOp& op,
+                                             double* G0, double* G1) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
+#pragma unroll
+    for (int ii = 0; ii < RBW; ii++) {
+        const int b = wave + 4 * ii;
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = (wave + 4 * ii) * 16 + (lane >> 4) + 4 * r;
+                if (b < op.rb0 && row < op.rows0) G0[row * LDB + ct * 16 + cl] = acc0[ii][ct][r];
+                if (b < op.rb1 && row < op.rows1) G1[row * LDB + ct * 16 + cl] = acc1[ii][ct][r];
+            }
+    }
+}
+
+// Unfolded row `node` (of n) at panel column c from LDS groups G0, G1 (h = n/2 odd-group rows).
+__device__ __forceinline__ double unfold_lds(const double* G0, const double* G1, int n, int node, int c) {
+    const int h = n / 2;
+    if (node < h) return G0[node * LDB + c] + G1[node * LDB + c];
+    if (node == h && (n & 1)) return G0[h * LDB + c];
+    const int i = n - 1 - node;
+    return G0[i * LDB + c] - G1[i * LDB + c];
+}
+
 template <int RBW>
 __device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
 #pragma unroll
@@ -208,12 +241,13 @@ __device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
 // Coordinates of acc[i][ct][r]: row within the group = (wave + 4i)*16 + lane/16 + 4r (the
 // result layout of the instruction), column = cb*BCOLS + ct*16 + lane%16.
 __device__ __forceinline__ int out_row(int i, int r) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     return (wave + 4 * i) * 16 + (lane >> 4) + 4 * r;
 }
 __device__ __forceinline__ int out_col(int ct, int cb) { return cb * BCOLS + ct * 16 + (threadIdx.x & 15); }
 
-// panel rows of the LDS-resident z panels of a back contraction with k-steps ks
-__host__ __device__ inline int zpanel_rows(int ks) { return 4 * ks; }
+// panel rows of the LDS-resident z panels of a back contraction with k-steps ks: whole chunks
+// of 16 rows (the chunked contraction reads them all; rows past 4 ks are zero)
+__host__ __device__ inline int zpanel_rows(int ks) { return 16 * ((ks + 3) / 4); }
 
 }  // namespace dns

@@ -263,6 +263,7 @@ static FoldOp fold_op(const FoldedA& f, const double* dev) {
     FoldOp o;
     o.A = dev; o.rb0 = f.rb0; o.rb1 = f.rb1; o.ks = f.ks; o.ksp = f.ksp; o.rows0 = f.rows0; o.rows1 = f.rows1;
     o.extra = f.extra;
+    o.pb = f.pb;
     return o;
 }
 
@@ -943,10 +944,10 @@ static void unswizzle(const FoldedA& a, std::vector<double>& m) {
     };
     for (int r = 0; r < a.rows0; r++)
         for (int c = 0; c < a.K; c++)
-            m[size_t(r) * a.K + c] = (a.extra && r == a.rows0 - 1) ? a.data[size_t(a.rb0 + a.rb1) * a.ksp * 64 + c]
+            m[size_t(r) * a.K + c] = (a.extra && r == a.rows0 - 1) ? a.data[size_t(2 * a.pb) * a.ksp * 64 + c]
                                                                    : get(r / 16, r, c);
     for (int r = 0; r < a.rows1; r++)
-        for (int c = 0; c < a.K; c++) m[size_t(a.rows0 + r) * a.K + c] = get(a.rb0 + r / 16, r, c);
+        for (int c = 0; c < a.K; c++) m[size_t(a.rows0 + r) * a.K + c] = get(a.pb + r / 16, r, c);
 }
 
 int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols) {

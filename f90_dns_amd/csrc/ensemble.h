@@ -23,7 +23,7 @@ enum StageKind { K_ZVORT = 0, K_C2R, K_R2C, K_FLOW, K_HELM, K_POIS, K_MODE0, K_C
 
 // Tunable launch shapes (chosen by Ensemble::autotune or set by hand).
 struct EnsTuning {
-    int promote[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // [f32][size class]: extra class levels for plane kernels
+    int promote[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};   // [f32][size class]: extra class levels (more threads)
     int lanes = 1;                                       // streams the step is split over (1 or 2)
     int interleave = 0;                                  // 1: z-kernel tiles ordered column-block-major across members
 };

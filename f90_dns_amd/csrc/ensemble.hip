@@ -211,11 +211,11 @@ int Ensemble::rebuild() {
                     while (k > base && fft_class_capacity(k, c->mixed) < 0) k--;
                     key = k;
                 } else if (kind == K_ZVORT) {
-                    key = fold_rbw(hz[g].d1); f32 = 0;
+                    key = fold_var(hz[g].d1); f32 = 0;
                 } else if (kind == K_POIS) {
-                    key = fold_rbw(hp[g].pF); f32 = 0;
+                    key = fold_var(hp[g].pF); f32 = 0;
                 } else if (kind == K_HELM) {
-                    key = fold_rbw(hh[g].hF); f32 = 0;
+                    key = fold_var(hh[g].hF); f32 = hh[g].s_f32 ? 1 : 0;
                 } else {
                     f32 = 0;
                 }
@@ -235,8 +235,10 @@ int Ensemble::rebuild() {
                         for (int z = 0; z < B; z++)
                             if (act[z])
                                 for (int k = 0; k < c->nz; k++) tiles.push_back(Tile{g, z * c->nz + k, z});
-                    } else if (kind == K_HELM || kind == K_POIS) {
-                        L.lds = std::max(L.lds, kind == K_HELM ? helm_lds_bytes(hh[g]) : pois_lds_bytes(hp[g]));
+                    } else if (kind == K_HELM || kind == K_POIS || kind == K_ZVORT) {
+                        L.lds = std::max(L.lds, kind == K_HELM   ? helm_lds_bytes(hh[g])
+                                                : kind == K_POIS ? pois_lds_bytes(hp[g])
+                                                                 : zderiv_lds_bytes(hz[g]));
                         const int ncb = (c->H2 + BCOLS - 1) / BCOLS;
                         if (tune.interleave) {
                             for (int x = 0; x < ncb; x++)
@@ -301,11 +303,11 @@ int Ensemble::launch(const Launch& L, hipStream_t s) {
     const Tile* map = d_tiles + L.off;
     hipError_t e = hipSuccess;
     switch (L.kind) {
-        case K_ZVORT: e = launch_zderiv_rag(L.cls, Z_VORT, d_z, map, L.count, 1, s); break;
+        case K_ZVORT: e = launch_zderiv_rag(L.cls, Z_VORT, d_z, map, L.count, 1, L.lds, s); break;
         case K_C2R: e = launch_c2r_rag(L.cls, C2R_NL, L.f32 != 0, d_c2r, map, L.count, L.nf, L.lds, s); break;
         case K_R2C: e = launch_r2c_rag(L.cls, R2C_CROSS, L.f32 != 0, d_r2c, map, L.count, L.nf, L.lds, s); break;
         case K_FLOW: e = launch_flow_control_rag(d_fl, map, L.count, s); break;
-        case K_HELM: e = launch_helmholtz_rag(L.cls, d_h, map, L.count, L.lds, s); break;
+        case K_HELM: e = launch_helmholtz_rag(L.cls, L.f32 != 0, d_h, map, L.count, L.lds, s); break;
         case K_POIS: e = launch_poisson_rag(L.cls, d_p, map, L.count, L.lds, s); break;
         case K_MODE0: e = launch_poisson_mode0_rag(d_m0, map, L.count, s); break;
         default: e = hipErrorInvalidValue;
@@ -499,8 +501,9 @@ int Ensemble::autotune(int steps, std::string& report) {
         int rr = time_steps(steps, ms);
         if (rr) return rr;
         rep << (first ? "" : ", ") << "{\"lanes\": " << t.lanes << ", \"interleave\": " << t.interleave
-            << ", \"promote\": [" << t.promote[0][0] << "," << t.promote[0][1] << "," << t.promote[1][0] << ","
-            << t.promote[1][1] << "," << t.promote[1][2] << "], \"ms_per_step\": " << ms << "}";
+            << ", \"promote_fp64\": [" << t.promote[0][0] << "," << t.promote[0][1] << "," << t.promote[0][2] << ","
+            << t.promote[0][3] << "], \"promote_fp32\": [" << t.promote[1][0] << "," << t.promote[1][1] << ","
+            << t.promote[1][2] << "," << t.promote[1][3] << "], \"ms_per_step\": " << ms << "}";
         first = false;
         return 0;
     };
@@ -524,7 +527,7 @@ int Ensemble::autotune(int steps, std::string& report) {
         if (ms < best_ms) { best_ms = ms; best = t; }
     }
     for (int f32 = 0; f32 < 2; f32++) {
-        for (int c = 0; c < 3; c++) {
+        for (int c = 0; c < 4; c++) {
             bool used = false;
             for (auto& g : groups)
                 if (g.c && (g.c->mixed ? 1 : 0) == f32 && (f32 ? g.c->g.cls32 : g.c->g.cls64) == c) used = true;
@@ -543,8 +546,8 @@ int Ensemble::autotune(int steps, std::string& report) {
     dirty = true;
     rep << "], \"chosen\": {\"lanes\": " << best.lanes << ", \"interleave\": " << best.interleave
         << ", \"promote_fp64\": [" << best.promote[0][0] << "," << best.promote[0][1] << "," << best.promote[0][2]
-        << "], \"promote_fp32\": [" << best.promote[1][0] << "," << best.promote[1][1] << "," << best.promote[1][2]
-        << "]}, \"ms_per_step\": " << best_ms << "}";
+        << "," << best.promote[0][3] << "], \"promote_fp32\": [" << best.promote[1][0] << "," << best.promote[1][1]
+        << "," << best.promote[1][2] << "," << best.promote[1][3] << "]}, \"ms_per_step\": " << best_ms << "}";
     report = rep.str();
     if ((r = restore_state(saved))) return r;
     for (size_t g = 0; g < groups.size(); g++)
@@ -703,11 +706,10 @@ int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const in
     if (r) return r;
     e->e.tune.lanes = lanes;
     e->e.tune.interleave = interleave ? 1 : 0;
-    for (int c = 0; c < 4; c++) {
-        e->e.tune.promote[0][c] = promote_fp64 ? std::max(0, promote_fp64[c < 3 ? c : 2]) : 0;
-        e->e.tune.promote[1][c] = promote_fp32 ? std::max(0, promote_fp32[c < 3 ? c : 2]) : 0;
+    for (int c = 0; c < 5; c++) {
+        e->e.tune.promote[0][c] = (promote_fp64 && c < 4) ? std::max(0, promote_fp64[c]) : 0;
+        e->e.tune.promote[1][c] = (promote_fp32 && c < 4) ? std::max(0, promote_fp32[c]) : 0;
     }
-    e->e.tune.promote[0][3] = e->e.tune.promote[1][3] = 0;
     return e->e.rebuild();
 }
 

@@ -12,13 +12,15 @@
 
 namespace dns {
 
-// plane-size classes: (threads, max complex elements of the half plane); class 3 only fits
-// the LDS in FP32 (192 x 192)
+// plane-size classes: (threads, max complex elements of the half plane).  Class 0 is one
+// wavefront per plane (its barriers are nearly free; small planes of ensembles keep many planes
+// in flight per CU); class 4 only fits the LDS in FP32 (192 x 192).
 template <int C> struct Cls;
-template <> struct Cls<0> { static constexpr int NT = 256, MAXE = 2176; };
-template <> struct Cls<1> { static constexpr int NT = 512, MAXE = 4704; };
-template <> struct Cls<2> { static constexpr int NT = 1024, MAXE = 8320; };
-template <> struct Cls<3> { static constexpr int NT = 1024, MAXE = 18624; };
+template <> struct Cls<0> { static constexpr int NT = 64, MAXE = 1088; };
+template <> struct Cls<1> { static constexpr int NT = 256, MAXE = 2176; };
+template <> struct Cls<2> { static constexpr int NT = 512, MAXE = 4704; };
+template <> struct Cls<3> { static constexpr int NT = 1024, MAXE = 8320; };
+template <> struct Cls<4> { static constexpr int NT = 1024, MAXE = 18624; };
 
 template <class T> struct Vec;
 template <> struct Vec<double> { using type = double2; };
@@ -226,7 +228,8 @@ static int size_class(int H, bool f32) {
     if (H <= Cls<0>::MAXE) return 0;
     if (H <= Cls<1>::MAXE) return 1;
     if (H <= Cls<2>::MAXE) return 2;
-    if (f32 && H <= Cls<3>::MAXE) return 3;
+    if (H <= Cls<3>::MAXE) return 3;
+    if (f32 && H <= Cls<4>::MAXE) return 4;
     return -1;
 }
 
@@ -442,6 +445,7 @@ hipError_t fft_prepare(const FFTGeom& g) {
         case 0: e = prepare_ct<0, double>(); break;
         case 1: e = prepare_ct<1, double>(); break;
         case 2: e = prepare_ct<2, double>(); break;
+        case 3: e = prepare_ct<3, double>(); break;
         default: break;
     }
     if (e != hipSuccess) return e;
@@ -450,6 +454,7 @@ hipError_t fft_prepare(const FFTGeom& g) {
         case 1: return prepare_ct<1, float>();
         case 2: return prepare_ct<2, float>();
         case 3: return prepare_ct<3, float>();
+        case 4: return prepare_ct<4, float>();
         default: return hipErrorInvalidValue;
     }
 }
@@ -458,9 +463,10 @@ hipError_t fft_prepare_all() {
     hipError_t e;
     if ((e = prepare_two_pass()) != hipSuccess) return e;
     if ((e = prepare_ct<0, double>()) != hipSuccess || (e = prepare_ct<1, double>()) != hipSuccess ||
-        (e = prepare_ct<2, double>()) != hipSuccess || (e = prepare_ct<0, float>()) != hipSuccess ||
-        (e = prepare_ct<1, float>()) != hipSuccess || (e = prepare_ct<2, float>()) != hipSuccess ||
-        (e = prepare_ct<3, float>()) != hipSuccess)
+        (e = prepare_ct<2, double>()) != hipSuccess || (e = prepare_ct<3, double>()) != hipSuccess ||
+        (e = prepare_ct<0, float>()) != hipSuccess || (e = prepare_ct<1, float>()) != hipSuccess ||
+        (e = prepare_ct<2, float>()) != hipSuccess || (e = prepare_ct<3, float>()) != hipSuccess ||
+        (e = prepare_ct<4, float>()) != hipSuccess)
         return e;
     return hipSuccess;
 }
@@ -470,7 +476,8 @@ int fft_class_capacity(int c, bool f32) {
         case 0: return Cls<0>::MAXE;
         case 1: return Cls<1>::MAXE;
         case 2: return Cls<2>::MAXE;
-        case 3: return f32 ? Cls<3>::MAXE : -1;
+        case 3: return Cls<3>::MAXE;
+        case 4: return f32 ? Cls<4>::MAXE : -1;
         default: return -1;
     }
 }
@@ -506,6 +513,7 @@ static hipError_t dispatch(const FFTGeom& g, bool c2r, int op, FFTArgs a, int np
             case 1: return run<1, float>(c2r, op, a, np, nf, s);
             case 2: return run<2, float>(c2r, op, a, np, nf, s);
             case 3: return run<3, float>(c2r, op, a, np, nf, s);
+            case 4: return run<4, float>(c2r, op, a, np, nf, s);
             default: return hipErrorInvalidValue;
         }
     }
@@ -513,6 +521,7 @@ static hipError_t dispatch(const FFTGeom& g, bool c2r, int op, FFTArgs a, int np
         case 0: return run<0, double>(c2r, op, a, np, nf, s);
         case 1: return run<1, double>(c2r, op, a, np, nf, s);
         case 2: return run<2, double>(c2r, op, a, np, nf, s);
+        case 3: return run<3, double>(c2r, op, a, np, nf, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -543,6 +552,7 @@ static hipError_t dispatch_rag(bool c2r, int cls, int op, bool f32, const FFTArg
             case 1: return run_rag<1, float>(c2r, d, map, nt, nf, lds, s);
             case 2: return run_rag<2, float>(c2r, d, map, nt, nf, lds, s);
             case 3: return run_rag<3, float>(c2r, d, map, nt, nf, lds, s);
+            case 4: return run_rag<4, float>(c2r, d, map, nt, nf, lds, s);
             default: return hipErrorInvalidValue;
         }
     }
@@ -550,6 +560,7 @@ static hipError_t dispatch_rag(bool c2r, int cls, int op, bool f32, const FFTArg
         case 0: return run_rag<0, double>(c2r, d, map, nt, nf, lds, s);
         case 1: return run_rag<1, double>(c2r, d, map, nt, nf, lds, s);
         case 2: return run_rag<2, double>(c2r, d, map, nt, nf, lds, s);
+        case 3: return run_rag<3, double>(c2r, d, map, nt, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -92,10 +92,11 @@ struct FoldOp {
     const double* A;
     int rb0, rb1, ks, ksp, rows0, rows1;
     int extra;                   // 1: group-0 row rows0-1 is the dense row after the blocks
+    int pb;                      // blocks per group as stored (zero padded); group 1 starts at pb
 };
 // the dense extra row of a folded operator (valid when extra)
 __host__ __device__ inline const double* fold_xrow(const FoldOp& o) {
-    return o.A + size_t(o.rb0 + o.rb1) * o.ksp * 64;
+    return o.A + size_t(2 * o.pb) * o.ksp * 64;
 }
 
 struct ZArgs {
@@ -109,8 +110,14 @@ struct ZArgs {
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
 // pairs of row blocks per wave of a folded operator
-inline int fold_rbw(const FoldOp& o) { return ((o.rb0 > o.rb1 ? o.rb0 : o.rb1) + 3) / 4; }
+inline int fold_rbw(const FoldOp& o) { return o.pb / 4; }
+// fewer row blocks than waves: the kernels unfold their results through LDS so that every
+// thread stores
+inline bool fold_lx(const FoldOp& o) { return (o.rb0 > o.rb1 ? o.rb0 : o.rb1) < 4; }
+// kernel variant of a folded operator: 0 = one pair per wave with the LDS epilogue, else pairs
+inline int fold_var(const FoldOp& o) { return fold_lx(o) ? 0 : fold_rbw(o); }
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
+size_t zderiv_lds_bytes(const ZArgs& a);
 
 struct FlowState;
 struct HelmArgs {
@@ -192,11 +199,13 @@ struct FlowArgs {
 };
 hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
 
-// ragged variants of the step kernels (rbw = fold_rbw of the groups' operators; lds = dynamic
+// ragged variants of the step kernels (var = fold_var of the groups' operators; lds = dynamic
 // LDS bytes, the largest over the launch's groups)
-hipError_t launch_zderiv_rag(int rbw, int mode, const ZArgs* d, const Tile* map, int ntiles, int nf, hipStream_t s);
-hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int ntiles, size_t lds, hipStream_t s);
-hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int ntiles, size_t lds, hipStream_t s);
+hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int ntiles, int nf, size_t lds,
+                             hipStream_t s);
+hipError_t launch_helmholtz_rag(int var, bool sf32, const HelmArgs* d, const Tile* map, int ntiles, size_t lds,
+                                hipStream_t s);
+hipError_t launch_poisson_rag(int var, const PoisArgs* d, const Tile* map, int ntiles, size_t lds, hipStream_t s);
 hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int ntiles, hipStream_t s);
 hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int ntiles, hipStream_t s);
 

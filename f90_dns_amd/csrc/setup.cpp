@@ -242,9 +242,10 @@ FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector
     // dot product (stored after the blocks, 4 ksp entries, zero beyond K)
     f.extra = (rows0 % 16 == 1) ? 1 : 0;
     if (f.extra) f.rb0 = rows0 / 16;
-    f.data.assign(size_t(f.rb0 + f.rb1) * f.ksp * 64 + (f.extra ? size_t(4) * f.ksp : 0), 0.0);
+    f.pb = 4 * ((std::max(f.rb0, f.rb1) + 3) / 4);
+    f.data.assign(size_t(2 * f.pb) * f.ksp * 64 + (f.extra ? size_t(4) * f.ksp : 0), 0.0);
     if (f.extra)
-        for (int c = 0; c < K; c++) f.data[size_t(f.rb0 + f.rb1) * f.ksp * 64 + c] = M0[size_t(rows0 - 1) * K + c];
+        for (int c = 0; c < K; c++) f.data[size_t(2 * f.pb) * f.ksp * 64 + c] = M0[size_t(rows0 - 1) * K + c];
     auto put = [&](const std::vector<double>& M, int rows, int cb0, int nb) {
         for (int b = 0; b < nb; b++)
             for (int s = 0; s < f.ks; s++)
@@ -254,7 +255,7 @@ FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector
                 }
     };
     put(M0, f.extra ? rows0 - 1 : rows0, 0, f.rb0);
-    put(M1, rows1, f.rb0, f.rb1);
+    put(M1, rows1, f.pb, f.rb1);
     return f;
 }
 

@@ -47,14 +47,16 @@ SwizzledA swizzle_a(const std::vector<double>& rowmajor, int rows, int cols);
 // out_{n-1-i} = g0_i - g1_i (i < h), out_h = g0_h.
 //
 // FoldedA: the two row groups in the lane order of v_mfma_f64_16x16x4f64's A operand; group g
-// acts on its own K-row panel.  Block cb (group-0 blocks first, then rb1 group-1 blocks),
+// acts on its own K-row panel.  Block cb (group-0 blocks first, then group 1 from block pb),
 // k-step s: entry [(cb*ksp + s)*64 + lane] = M_g(16*(cb - first_g) + lane%16, 4s + lane/16);
 // ks = ceil(K/4) k-steps are used; ksp (a multiple of 4, >= ks + 1) are stored, the tail zero.
 // When rows0 = 16 k + 1 (the centre row or the extra even eigenvector of an odd column), that
 // last group-0 row is not in the blocks (rb0 = k) but stored dense after them (extra = 1,
 // 4 ksp entries) and computed as a dot product, so the row-block pairs stay k per group.
+// Both groups are padded with zero blocks to pb = 4 ceil(max(rb0, rb1) / 4) blocks, so each of
+// a block's 4 waves loads its pairs unconditionally; group 1 starts at block pb.
 struct FoldedA {
-    int rows0 = 0, rows1 = 0, K = 0, rb0 = 0, rb1 = 0, ks = 0, ksp = 0, extra = 0;
+    int rows0 = 0, rows1 = 0, K = 0, rb0 = 0, rb1 = 0, ks = 0, ksp = 0, extra = 0, pb = 0;
     std::vector<double> data;
 };
 FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector<double>& M1, int rows1, int K);

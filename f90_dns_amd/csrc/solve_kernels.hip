@@ -29,7 +29,7 @@ __device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx,
 // (DNS_pressure_BC_3D.f90:1077-1079).  Folded: s = D1 x_odd on X_O (rows up to the centre),
 // r = D1 x_even on X_E, out_k = s_k + r_k, out_{nz-1-k} = s_k - r_k.  One block = 16 modes.
 // -------------------------------------------------------------------------------------
-template <int RBW, int MODE>
+template <int RBW, int MODE, bool LX>
 __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int mb, double* lds) {
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
     const int wave = threadIdx.x >> 6;
@@ -42,16 +42,15 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     for (int q = 0; q < nfield; q++) {
         const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : by;   // vort: v first, then u
         const double2* x2 = reinterpret_cast<const double2*>(a.in[f] + mo) + (sval ? smode : 0);
+        // branch-free: mirrored rows (clamped in range), the centre row has na == nb
         auto fold = [&](int kf) -> FoldPair {
-            FoldPair r{make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-            if (!sval) return r;
-            if (kf < hF) {
-                const double2 xa = x2[kf * rs], xb = x2[(nz - 1 - kf) * rs];
-                r.p0 = make_double2(xa.x - xb.x, xa.y - xb.y);
-                r.p1 = make_double2(xa.x + xb.x, xa.y + xb.y);
-            } else if (kf < LE) {
-                r.p1 = x2[hF * rs];
-            }
+            const int na = min(kf, nz - 1), nb = max(nz - 1 - kf, 0);
+            const double2 xa = x2[na * rs], xb = x2[nb * rs];
+            const double keep = (sval && kf < LE) ? 1.0 : 0.0;
+            const double pair = kf < hF ? 1.0 : 0.0;   // 0 on the centre row
+            FoldPair r;
+            r.p0 = make_double2(keep * pair * (xa.x - xb.x), keep * pair * (xa.y - xb.y));
+            r.p1 = make_double2(keep * (xa.x + pair * xb.x), keep * (xa.y + pair * xb.y));
             return r;
         };
         auto emit = [&](int node, int col, double y) {
@@ -80,9 +79,22 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
         zero_acc2(acc1);
         double2 xacc = make_double2(0.0, 0.0);
         fold_gemm<RBW>(acc0, acc1, a.d1, fold, lds, xacc);
+        const double xs = a.d1.extra ? reduce_extra(xacc, lds) : 0.0;   // the centre row s_h
+        if constexpr (LX) {
+            // few row blocks: unfold through LDS so that every thread stores
+            double* G0 = lds;
+            double* G1 = lds + a.d1.rows0 * LDB;
+            store_groups<RBW>(acc0, acc1, a.d1, G0, G1);
+            if (a.d1.extra && threadIdx.x < BCOLS) G0[(a.d1.rows0 - 1) * LDB + threadIdx.x] = xs;
+            __syncthreads();
+            for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
+                const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+                if (col < H2) emit(node, col, unfold_lds(G0, G1, nz, node, c));
+            }
+            __syncthreads();
+            continue;
+        }
         if (a.d1.extra) {
-            // the centre row s_h (out_h = s_h)
-            const double xs = reduce_extra(xacc, lds);
             const int col = cb * BCOLS + int(threadIdx.x);
             if (threadIdx.x < BCOLS && col < H2) emit(hF, col, xs);
         }
@@ -108,17 +120,17 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     }
 }
 
-template <int RBW, int MODE>
+template <int RBW, int MODE, bool LX>
 __global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
-    __shared__ double lds[STAGE_LDS];
-    zderiv_tile<RBW, MODE>(a, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+    extern __shared__ double smem[];
+    zderiv_tile<RBW, MODE, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
-template <int RBW, int MODE>
+template <int RBW, int MODE, bool LX>
 __global__ __launch_bounds__(256) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
-    __shared__ double lds[STAGE_LDS];
+    extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
-    zderiv_tile<RBW, MODE>(a, t.x, blockIdx.y, t.z, lds);
+    zderiv_tile<RBW, MODE, LX>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // -------------------------------------------------------------------------------------
@@ -132,7 +144,7 @@ __global__ __launch_bounds__(256) void k_zderiv_rag(const ZArgs* __restrict__ d,
 // Folded: even eigen rows on [v_E ; u_E], odd rows on [v_O ; u_O] (v = alpha u_I + dt S'_I over
 // the interior, u over the whole column); u*_I unfolds from p = B_E z_E and q = B_O z_O.
 // -------------------------------------------------------------------------------------
-template <int RBW>
+template <int RBW, bool SF32, bool LX>
 __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int comp, int mb, double* smem) {
     const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2;
     const int hI = m / 2, L1 = hI + (m & 1), hF = nz / 2, LE = hF + (nz & 1);
@@ -160,36 +172,32 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const double2* u2 = reinterpret_cast<const double2*>(uh) + (sval ? smode : 0);
     const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.sh[comp]) + mo) + (sval ? smode : 0);
     const double2* sd2 = reinterpret_cast<const double2*>(a.sh[comp] + mo) + (sval ? smode : 0);
-    auto vval = [&](int node) -> double2 {
-        const double2 u = u2[node * rs];
-        double2 sv;
-        if (a.s_f32) {
+    auto sload = [&](int node) -> double2 {
+        if (SF32) {
             const float2 sf = sf2[node * rs];
-            sv = make_double2(double(sf.x), double(sf.y));
-        } else {
-            sv = sd2[node * rs];
+            return make_double2(double(sf.x), double(sf.y));
         }
-        return make_double2(alpha * u.x + a.dt * (sv.x + fr), alpha * u.y + a.dt * sv.y);
+        return sd2[node * rs];
     };
+    // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
+    // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
+    // two nodes coincide
     auto fold = [&](int kf) -> FoldPair {
-        FoldPair r{make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-        if (!sval) return r;
-        if (kf < hI) {
-            const double2 va = vval(kf + 1), vb = vval(m - kf);
-            r.p0 = make_double2(va.x + vb.x, va.y + vb.y);
-            r.p1 = make_double2(va.x - vb.x, va.y - vb.y);
-        } else if (kf < L1) {
-            r.p0 = vval(hI + 1);
-        } else {
-            const int c = kf - L1;
-            if (c < hF) {
-                const double2 ua = u2[c * rs], ub = u2[(nz - 1 - c) * rs];
-                r.p0 = make_double2(ua.x + ub.x, ua.y + ub.y);
-                r.p1 = make_double2(ua.x - ub.x, ua.y - ub.y);
-            } else if (c < LE) {
-                r.p0 = u2[hF * rs];
-            }
-        }
+        const bool inv = kf < L1;
+        const int c = inv ? kf : kf - L1;
+        const int half = inv ? hI : hF, lim = inv ? L1 : LE;
+        const int na = min(inv ? c + 1 : c, nz - 1), nb = max(inv ? m - c : nz - 1 - c, 0);
+        const double2 ua = u2[na * rs], ub = u2[nb * rs];
+        const double2 sa = sload(na), sb = sload(nb);
+        const double va_r = inv ? alpha * ua.x + a.dt * (sa.x + fr) : ua.x;
+        const double va_i = inv ? alpha * ua.y + a.dt * sa.y : ua.y;
+        const double vb_r = inv ? alpha * ub.x + a.dt * (sb.x + fr) : ub.x;
+        const double vb_i = inv ? alpha * ub.y + a.dt * sb.y : ub.y;
+        const double keep = (sval && c < lim) ? 1.0 : 0.0;
+        const double pair = c < half ? 1.0 : 0.0;
+        FoldPair r;
+        r.p0 = make_double2(keep * (va_r + pair * vb_r), keep * (va_i + pair * vb_i));
+        r.p1 = make_double2(keep * pair * (va_r - vb_r), keep * pair * (va_i - vb_i));
         return r;
     };
     // per-lane column constants for the two 16-column tiles
@@ -251,6 +259,31 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     zero_acc2(acc1);
     fold_gemm_lds<RBW>(acc0, acc1, a.hB, z0, z1);
     double* out = a.out[comp] + mo;
+    if constexpr (LX) {
+        // few row blocks: unfold through LDS (over the z panels, once every wave has read them)
+        const double xc = (a.hB.extra && threadIdx.x < BCOLS) ? extra_dot(a.hB, z0) : 0.0;
+        __syncthreads();
+        double* G0 = smem;
+        double* G1 = smem + a.hB.rows0 * LDB;
+        double* bcs = G1 + a.hB.rows1 * LDB;   // 2 x BCOLS wall values
+        store_groups<RBW>(acc0, acc1, a.hB, G0, G1);
+        if (a.hB.extra && threadIdx.x < BCOLS) G0[(a.hB.rows0 - 1) * LDB + threadIdx.x] = xc;
+        if (threadIdx.x < 16) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                bcs[ct * 16 + threadIdx.x] = bc1[ct];
+                bcs[BCOLS + ct * 16 + threadIdx.x] = bc2[ct];
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
+            const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+            if (col >= H2) continue;
+            const double v = node == 0 ? bcs[c] : node == nz - 1 ? bcs[BCOLS + c] : unfold_lds(G0, G1, m, node - 1, c);
+            out[size_t(node) * H2 + col] = v;
+        }
+        return;
+    }
     if (a.hB.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2)   // the interior centre row
         out[size_t(a.hB.rows0) * H2 + cb * BCOLS + threadIdx.x] = extra_dot(a.hB, z0);
 #pragma unroll
@@ -278,17 +311,17 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     }
 }
 
-template <int RBW>
+template <int RBW, bool SF32, bool LX>
 __global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
     extern __shared__ double smem[];
-    helmholtz_tile<RBW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    helmholtz_tile<RBW, SF32, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
-template <int RBW>
+template <int RBW, bool SF32, bool LX>
 __global__ __launch_bounds__(256) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
-    helmholtz_tile<RBW>(a, t.x, blockIdx.y, t.z, smem);
+    helmholtz_tile<RBW, SF32, LX>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // -------------------------------------------------------------------------------------
@@ -302,7 +335,7 @@ __global__ __launch_bounds__(256) void k_helmholtz_rag(const HelmArgs* __restric
 // (Bf_E z_E, Bf_O z_O), D1 phi from ((D1 Bf)_O z_O, (D1 Bf)_E z_E).  Mode (0,0) is left to
 // k_poisson_mode0.
 // -------------------------------------------------------------------------------------
-template <int RBW>
+template <int RBW, bool LX>
 __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
@@ -322,29 +355,24 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     const double2* us2 = reinterpret_cast<const double2*>(us) + (sval ? smode : 0);
     const double2* vs2 = reinterpret_cast<const double2*>(vs) + (sval ? smode : 0);
     const double2* ws2 = reinterpret_cast<const double2*>(ws) + (sval ? smode : 0);
-    auto div = [&](int node) -> double2 {
-        const double2 u = us2[node * rs], v = vs2[node * rs];
-        return make_double2(-skx * u.y - sky * v.y, skx * u.x + sky * v.x);
-    };
+    // branch-free: rows [0, LE) fold div (E: sum on panel 0, O: difference on panel 1), rows
+    // [LE, 2 LE) fold w* the other way round (w_O on panel 0, w_E on panel 1)
     auto fold = [&](int kf) -> FoldPair {
-        FoldPair r{make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
-        if (!sval) return r;
-        if (kf < hF) {
-            const double2 da = div(kf), db = div(nz - 1 - kf);
-            r.p0 = make_double2(da.x + db.x, da.y + db.y);
-            r.p1 = make_double2(da.x - db.x, da.y - db.y);
-        } else if (kf < LE) {
-            r.p0 = div(hF);
-        } else {
-            const int c = kf - LE;
-            if (c < hF) {
-                const double2 wa = ws2[c * rs], wb = ws2[(nz - 1 - c) * rs];
-                r.p0 = make_double2(wa.x - wb.x, wa.y - wb.y);
-                r.p1 = make_double2(wa.x + wb.x, wa.y + wb.y);
-            } else if (c < LE) {
-                r.p1 = ws2[hF * rs];
-            }
-        }
+        const bool ind = kf < LE;
+        const int c = ind ? kf : kf - LE;
+        const int na = min(c, nz - 1), nb = max(nz - 1 - c, 0);
+        const double2 ua = us2[na * rs], ub = us2[nb * rs];
+        const double2 va = vs2[na * rs], vb = vs2[nb * rs];
+        const double2 wa = ws2[na * rs], wb = ws2[nb * rs];
+        const double xa_r = ind ? -skx * ua.y - sky * va.y : wa.x, xa_i = ind ? skx * ua.x + sky * va.x : wa.y;
+        const double xb_r = ind ? -skx * ub.y - sky * vb.y : wb.x, xb_i = ind ? skx * ub.x + sky * vb.x : wb.y;
+        const double keep = (sval && c < LE) ? 1.0 : 0.0;
+        const double pair = c < hF ? 1.0 : 0.0;
+        const double er = keep * (xa_r + pair * xb_r), ei = keep * (xa_i + pair * xb_i);   // even fold
+        const double orr = keep * pair * (xa_r - xb_r), oi = keep * pair * (xa_i - xb_i);  // odd fold
+        FoldPair r;
+        r.p0 = ind ? make_double2(er, ei) : make_double2(orr, oi);
+        r.p1 = ind ? make_double2(orr, oi) : make_double2(er, ei);
         return r;
     };
     double k2c[CT], kxe[CT], kye[CT];
@@ -397,6 +425,59 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     zero_acc2(acc0);
     zero_acc2(acc1);
     fold_gemm_lds<RBW>(acc0, acc1, a.pB, z0, z1);
+    if constexpr (LX) {
+        // few row blocks: both projections unfold through LDS, after the z panels
+        double* G0 = smem + 2 * ZR * LDB;
+        double* G1 = G0 + a.pB.rows0 * LDB;
+        double* cc = G1 + a.pB.rows1 * LDB;   // per block column: kx_eff, ky_eff, live
+        if (threadIdx.x < 16) {
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                cc[ct * 16 + threadIdx.x] = kxe[ct];
+                cc[BCOLS + ct * 16 + threadIdx.x] = kye[ct];
+                cc[2 * BCOLS + ct * 16 + threadIdx.x] = live[ct] ? 1.0 : 0.0;
+            }
+        }
+        const double xph = (a.pB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pB, z0) : 0.0;
+        store_groups<RBW>(acc0, acc1, a.pB, G0, G1);
+        if (a.pB.extra && threadIdx.x < BCOLS) G0[(a.pB.rows0 - 1) * LDB + threadIdx.x] = xph;
+        __syncthreads();
+        for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
+            const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+            if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
+            const double ph = unfold_lds(G0, G1, nz, node, c), other = unfold_lds(G0, G1, nz, node, c ^ 1);
+            const size_t qn = size_t(node) * H2 + col;
+            if (!a.fused) {
+                phio[qn] = ph;
+                continue;
+            }
+            const int part = col & 1;
+            const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
+            const bool wall = node == 0 || node == nz - 1;
+            const double du = part == 0 ? -cc[c] * pim : cc[c] * pre;
+            const double dv = part == 0 ? -cc[BCOLS + c] * pim : cc[BCOLS + c] * pre;
+            uo[qn] = wall ? 0.0 : us[qn] - a.dt * du;
+            vo[qn] = wall ? 0.0 : vs[qn] - a.dt * dv;
+            po[qn] = po[qn] + ph;
+        }
+        __syncthreads();
+        zero_acc2(acc0);
+        zero_acc2(acc1);
+        fold_gemm_lds<RBW>(acc0, acc1, a.pDB, z1, z0);
+        const double xd = (a.pDB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pDB, z1) : 0.0;
+        store_groups<RBW>(acc0, acc1, a.pDB, G0, G1);
+        if (a.pDB.extra && threadIdx.x < BCOLS) G0[(a.pDB.rows0 - 1) * LDB + threadIdx.x] = xd;
+        __syncthreads();
+        for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
+            const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+            if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
+            const double dph = unfold_lds(G0, G1, nz, node, c);
+            const size_t qn = size_t(node) * H2 + col;
+            if (!a.fused) dphio[qn] = dph;
+            else wo[qn] = (node == 0 || node == nz - 1) ? 0.0 : ws[qn] - a.dt * dph;
+        }
+        return;
+    }
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb);
@@ -501,17 +582,17 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     }
 }
 
-template <int RBW>
+template <int RBW, bool LX>
 __global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
     extern __shared__ double smem[];
-    poisson_tile<RBW>(a, blockIdx.x, blockIdx.z, smem);
+    poisson_tile<RBW, LX>(a, blockIdx.x, blockIdx.z, smem);
 }
-template <int RBW>
+template <int RBW, bool LX>
 __global__ __launch_bounds__(256) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const PoisArgs& a = d[t.g];
-    poisson_tile<RBW>(a, t.x, t.z, smem);
+    poisson_tile<RBW, LX>(a, t.x, t.z, smem);
 }
 
 // mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
@@ -704,47 +785,79 @@ __global__ void k_reduce_final(ReduceArgs a) {
 // ---------------------------------------------------------------------------------------
 static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
 
+size_t zderiv_lds_bytes(const ZArgs& a) {
+    const size_t g = fold_lx(a.d1) ? size_t(a.d1.rows0 + a.d1.rows1) * LDB : 0;
+    return sizeof(double) * std::max<size_t>(STAGE_LDS, g);
+}
 size_t helm_lds_bytes(const HelmArgs& a) {
-    return sizeof(double) * std::max<size_t>(STAGE_LDS, size_t(2) * zpanel_rows(a.hB.ks) * LDB);
+    const size_t z = size_t(2) * zpanel_rows(a.hB.ks) * LDB;
+    const size_t g = fold_lx(a.hB) ? size_t(a.hB.rows0 + a.hB.rows1) * LDB + 2 * BCOLS : 0;
+    return sizeof(double) * std::max<size_t>(std::max<size_t>(STAGE_LDS, z), g);
 }
 size_t pois_lds_bytes(const PoisArgs& a) {
-    return sizeof(double) * std::max<size_t>(STAGE_LDS, size_t(2) * zpanel_rows(a.pB.ks) * LDB);
+    size_t z = size_t(2) * zpanel_rows(a.pB.ks) * LDB;
+    if (fold_lx(a.pB)) z += size_t(a.pB.rows0 + a.pB.rows1) * LDB + 3 * BCOLS;
+    return sizeof(double) * std::max<size_t>(STAGE_LDS, z);
 }
+
+// variant index: row-block pairs per wave (1, 2) x LDS epilogue (only with 1 pair)
+static inline int zvar(const FoldOp& o) { return fold_lx(o) ? 0 : fold_rbw(o); }
+
+#define ZD_CASES(K, ...)                                                                             \
+    switch (var * 2 + (mode == Z_VORT)) {                                                            \
+        case 0: hipLaunchKernelGGL((K<1, Z_PLAIN, true>), __VA_ARGS__); break;                        \
+        case 1: hipLaunchKernelGGL((K<1, Z_VORT, true>), __VA_ARGS__); break;                         \
+        case 2: hipLaunchKernelGGL((K<1, Z_PLAIN, false>), __VA_ARGS__); break;                       \
+        case 3: hipLaunchKernelGGL((K<1, Z_VORT, false>), __VA_ARGS__); break;                        \
+        case 4: hipLaunchKernelGGL((K<2, Z_PLAIN, false>), __VA_ARGS__); break;                       \
+        case 5: hipLaunchKernelGGL((K<2, Z_VORT, false>), __VA_ARGS__); break;                        \
+        default: return hipErrorInvalidValue;                                                        \
+    }
 
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
     g.z = nm;
-    switch (fold_rbw(a.d1) * 2 + (mode == Z_VORT)) {
-        case 2: hipLaunchKernelGGL((k_zderiv<1, Z_PLAIN>), g, dim3(256), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_zderiv<1, Z_VORT>), g, dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_zderiv<2, Z_PLAIN>), g, dim3(256), 0, s, a); break;
-        case 5: hipLaunchKernelGGL((k_zderiv<2, Z_VORT>), g, dim3(256), 0, s, a); break;
-        default: return hipErrorInvalidValue;
-    }
+    const int var = zvar(a.d1);
+    const size_t lds = zderiv_lds_bytes(a);
+    ZD_CASES(k_zderiv, g, dim3(256), lds, s, a)
     return hipGetLastError();
 }
+
+#define HM_CASES(K, ...)                                                                             \
+    switch (var * 2 + (sf32 ? 1 : 0)) {                                                              \
+        case 0: hipLaunchKernelGGL((K<1, false, true>), __VA_ARGS__); break;                          \
+        case 1: hipLaunchKernelGGL((K<1, true, true>), __VA_ARGS__); break;                           \
+        case 2: hipLaunchKernelGGL((K<1, false, false>), __VA_ARGS__); break;                         \
+        case 3: hipLaunchKernelGGL((K<1, true, false>), __VA_ARGS__); break;                          \
+        case 4: hipLaunchKernelGGL((K<2, false, false>), __VA_ARGS__); break;                         \
+        case 5: hipLaunchKernelGGL((K<2, true, false>), __VA_ARGS__); break;                          \
+        default: return hipErrorInvalidValue;                                                        \
+    }
 
 hipError_t launch_helmholtz(const HelmArgs& a, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, 3);
     g.z = nm;
     const size_t lds = helm_lds_bytes(a);
-    switch (fold_rbw(a.hF)) {
-        case 1: hipLaunchKernelGGL((k_helmholtz<1>), g, dim3(256), lds, s, a); break;
-        case 2: hipLaunchKernelGGL((k_helmholtz<2>), g, dim3(256), lds, s, a); break;
-        default: return hipErrorInvalidValue;
-    }
+    const int var = zvar(a.hF);
+    const bool sf32 = a.s_f32 != 0;
+    HM_CASES(k_helmholtz, g, dim3(256), lds, s, a)
     return hipGetLastError();
 }
+
+#define PS_CASES(K, ...)                                                                             \
+    switch (var) {                                                                                   \
+        case 0: hipLaunchKernelGGL((K<1, true>), __VA_ARGS__); break;                                 \
+        case 1: hipLaunchKernelGGL((K<1, false>), __VA_ARGS__); break;                                \
+        case 2: hipLaunchKernelGGL((K<2, false>), __VA_ARGS__); break;                                \
+        default: return hipErrorInvalidValue;                                                        \
+    }
 
 hipError_t launch_poisson(const PoisArgs& a, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, 1);
     g.z = nm;
     const size_t lds = pois_lds_bytes(a);
-    switch (fold_rbw(a.pF)) {
-        case 1: hipLaunchKernelGGL((k_poisson<1>), g, dim3(256), lds, s, a); break;
-        case 2: hipLaunchKernelGGL((k_poisson<2>), g, dim3(256), lds, s, a); break;
-        default: return hipErrorInvalidValue;
-    }
+    const int var = zvar(a.pF);
+    PS_CASES(k_poisson, g, dim3(256), lds, s, a)
     return hipGetLastError();
 }
 hipError_t launch_poisson_mode0(const Mode0Args& a, int nm, hipStream_t s) {
@@ -766,35 +879,23 @@ hipError_t launch_flow_control(const FlowArgs& a, int nm, hipStream_t s) {
     hipLaunchKernelGGL(k_flow_control, dim3(nm), dim3(64), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_zderiv_rag(int rbw, int mode, const ZArgs* d, const Tile* map, int nt, int nf, hipStream_t s) {
+hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int nt, int nf, size_t lds,
+                             hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     const dim3 g(nt, mode == Z_VORT ? 1 : nf);
-    switch (rbw * 2 + (mode == Z_VORT)) {
-        case 2: hipLaunchKernelGGL((k_zderiv_rag<1, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
-        case 3: hipLaunchKernelGGL((k_zderiv_rag<1, Z_VORT>), g, dim3(256), 0, s, d, map); break;
-        case 4: hipLaunchKernelGGL((k_zderiv_rag<2, Z_PLAIN>), g, dim3(256), 0, s, d, map); break;
-        case 5: hipLaunchKernelGGL((k_zderiv_rag<2, Z_VORT>), g, dim3(256), 0, s, d, map); break;
-        default: return hipErrorInvalidValue;
-    }
+    ZD_CASES(k_zderiv_rag, g, dim3(256), lds, s, d, map)
     return hipGetLastError();
 }
-hipError_t launch_helmholtz_rag(int rbw, const HelmArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
+hipError_t launch_helmholtz_rag(int var, bool sf32, const HelmArgs* d, const Tile* map, int nt, size_t lds,
+                                hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     const dim3 g(nt, 3);
-    switch (rbw) {
-        case 1: hipLaunchKernelGGL((k_helmholtz_rag<1>), g, dim3(256), lds, s, d, map); break;
-        case 2: hipLaunchKernelGGL((k_helmholtz_rag<2>), g, dim3(256), lds, s, d, map); break;
-        default: return hipErrorInvalidValue;
-    }
+    HM_CASES(k_helmholtz_rag, g, dim3(256), lds, s, d, map)
     return hipGetLastError();
 }
-hipError_t launch_poisson_rag(int rbw, const PoisArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
+hipError_t launch_poisson_rag(int var, const PoisArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
     if (nt <= 0) return hipSuccess;
-    switch (rbw) {
-        case 1: hipLaunchKernelGGL((k_poisson_rag<1>), dim3(nt), dim3(256), lds, s, d, map); break;
-        case 2: hipLaunchKernelGGL((k_poisson_rag<2>), dim3(nt), dim3(256), lds, s, d, map); break;
-        default: return hipErrorInvalidValue;
-    }
+    PS_CASES(k_poisson_rag, dim3(nt), dim3(256), lds, s, d, map)
     return hipGetLastError();
 }
 hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int nt, hipStream_t s) {

@@ -48,8 +48,8 @@ int dns_ensemble_synchronize(dns_ensemble* e);
 int dns_ensemble_step_timing(dns_ensemble* e, double* out4);
 
 /* Launch-shape tuning: lanes = streams the step is split over (1, 2); interleave = z-kernel
- * tile order; promote_fp64[3] / promote_fp32[3] = extra size-class levels (more threads per
- * plane) for the plane kernels of size classes 0..2 (NULL = none). */
+ * tile order; promote_fp64[4] / promote_fp32[4] = extra size-class levels (more threads per
+ * plane: 64, 256, 512, 1024) for the plane kernels of size classes 0..3 (NULL = none). */
 int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const int* promote_fp64,
                             const int* promote_fp32);
 /* Autotuner: times steps_per_trial graph replays of each candidate tuning (coordinate
