@@ -59,6 +59,19 @@ def kernel_work(kw):
     }
 
 
+def profiled_traffic(profile, kernel):
+    """HBM bytes per dispatch of a step kernel from a committed rocprof summary (FETCH_SIZE and
+    WRITE_SIZE runs scaled by the calibration copy kernel); None when not profiled."""
+    path = os.path.join(ROOT, "profiles", profile)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline(kw, steps):
     """The oracle (C port of the reference algorithm as written: per-mode matrix rebuild and
     LU every step, serial mode loops, OpenMP over planes/columns) on the host cores."""
@@ -191,15 +204,20 @@ def main():
         name = max(prof, key=prof.get)
         ms = prof[name]
         wk = work[name]
+        profile = "step_128x128x129_fp64_kernels.md" if args.workload == "channel_128x128x129_fp64" else None
+        traffic = profiled_traffic(profile, name) if profile else None
         if wk["flops"] > 0:
             ach = wk["flops"] / (ms * 1e-3) / 1e12
             roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "ms": ms,
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic, "ms": ms,
                     "hbm_gbs": wk["bytes"] / (ms * 1e-3) / 1e9}
         else:
             ach = wk["bytes"] / (ms * 1e-3) / 1e9
             roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS, "traffic": None, "ms": ms}
+                    "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "ms": ms}
+        roof["algorithmic_bytes"] = wk["bytes"]
+        if traffic is not None:
+            roof["traffic_source"] = "profiles/" + profile
         roof["kernels_ms"] = prof
     print("[bench] kernel ms: " + json.dumps(prof), file=sys.stderr, flush=True)
 
