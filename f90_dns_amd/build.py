@@ -13,8 +13,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libf90dns.so")
-SOURCES = ["setup.cpp", "fft_kernels.hip", "solve_kernels.hip", "dns_api.hip", "fft_plan.hip", "ensemble.hip", "server.hip"]
-HEADERS = ["setup.h", "device_common.h", "kernels.h", "contract.h", "context.h", "ensemble.h"]
+SOURCES = ["setup.cpp", "fft_kernels.hip", "fft_c2r64.hip", "fft_c2r32.hip", "fft_r2c64.hip", "fft_r2c32.hip",
+           "solve_kernels.hip", "dns_api.hip", "fft_plan.hip", "ensemble.hip", "server.hip"]
+HEADERS = ["setup.h", "device_common.h", "kernels.h", "fft_plane.h", "contract.h", "context.h", "ensemble.h"]
 ARCH = os.environ.get("F90DNS_ARCH", "gfx950")
 # packed FP32 instructions are disabled: the device code relies on scalar/vector FP64 and
 # plain FP32 arithmetic only.

@@ -98,7 +98,7 @@ static int destroy_impl(dns_ctx* c) {
 }
 
 static std::string check_params(const dns_params* p) {
-    if (p->nx < 4 || p->ny < 4 || (p->nx & 1) || (p->ny & 1)) return "nx and ny must be even and >= 4";
+    if (p->nx < 8 || p->ny < 4 || (p->nx & 1) || (p->ny & 1)) return "nx must be even and >= 8, ny even and >= 4";
     if (p->nz < 5 || p->nz > 144) return "nz must be in [5, 144]";
     if (!(p->dt > 0) || !(p->re > 0) || !(p->xlen > 0) || !(p->ylen > 0)) return "dt, re, xlen, ylen must be positive";
     if (p->precision != DNS_PREC_FP64 && p->precision != DNS_PREC_MIXED)
@@ -162,8 +162,8 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     };
 
     if (!fft_geom(c->nx, c->ny, c->g) || (!c->mixed && c->g.cls64 < 0)) {
-        g_create_error = "plane too large for the LDS-resident transform: (nx/2+1)*ny must be <= 8320 in FP64, "
-                         "<= 18624 in mixed precision";
+        g_create_error = "plane too large for the LDS-resident transform: (nx/2)*ny must be <= 8192 in FP64, "
+                         "<= 18432 in mixed precision";
         return bail(DNS_EUNSUPPORTED);
     }
     if (!dns_make_factors(c->ny, c->g.fy) || !dns_make_factors(c->nx / 2, c->g.fx)) {

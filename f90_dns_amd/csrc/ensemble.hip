@@ -43,7 +43,9 @@ bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) 
     b += 15 * d(NS * B) + 6 * d(N * B);
     b += std::max<size_t>(B, 1) * (sizeof(FlowState) + sizeof(FlowMember));
     b += d(512 * 5) + d(5);
-    if (nxh * p.ny > 8320) b += d(NS);   // two-pass scratch (plane beyond the FP64 LDS transform)
+    FFTGeom g;
+    fft_geom(p.nx, p.ny, g);
+    if (g.cls64 < 0) b += d(NS);   // two-pass scratch (plane beyond the FP64 LDS transform)
     bytes = b;
     return true;
 }
@@ -231,7 +233,7 @@ int Ensemble::rebuild() {
                     const int B = int(act.size());
                     if (kind == K_C2R || kind == K_R2C) {
                         L.nf = kind == K_C2R ? 6 : 3;
-                        L.lds = std::max(L.lds, size_t(c->nxh) * c->ny * (c->mixed ? sizeof(float2) : sizeof(double2)));
+                        L.lds = std::max(L.lds, plane_lds_bytes(c->g, c->mixed));
                         for (int z = 0; z < B; z++)
                             if (act[z])
                                 for (int k = 0; k < c->nz; k++) tiles.push_back(Tile{g, z * c->nz + k, z});

@@ -1,0 +1,67 @@
+// fft_c2r32.hip -- FP32 instances of the c2r plane kernels: the step's C2R_NL in every
+// FP32 size class (direct and ragged launches) and C2R_PLAIN in the largest class, which
+// runs any plane of the precision (its plans fit every larger class).
+#include "fft_plane.h"
+
+namespace dns {
+
+template <int C>
+static hipError_t hot(const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_c2r<C, C2R_NL, float>), dim3(np, nf), dim3(Cls<C>::NT), lds, s, a);
+    return hipGetLastError();
+}
+template <int C>
+static hipError_t hot_rag(const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_c2r_rag<C, C2R_NL, float>), dim3(nt, nf), dim3(Cls<C>::NT), lds, s, d, map);
+    return hipGetLastError();
+}
+
+hipError_t c2r_launch32(int cls, int op, const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
+    if (op == C2R_PLAIN) {
+        hipLaunchKernelGGL((k_c2r<4, C2R_PLAIN, float>), dim3(np, nf), dim3(Cls<4>::NT), lds, s, a);
+        return hipGetLastError();
+    }
+    if (op != C2R_NL) return hipErrorInvalidValue;
+    switch (cls) {
+        case 0: return hot<0>(a, np, nf, lds, s);
+        case 1: return hot<1>(a, np, nf, lds, s);
+        case 2: return hot<2>(a, np, nf, lds, s);
+        case 3: return hot<3>(a, np, nf, lds, s);
+        case 4: return hot<4>(a, np, nf, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t c2r_launch_rag32(int cls, const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds, hipStream_t s) {
+    switch (cls) {
+        case 0: return hot_rag<0>(d, map, nt, nf, lds, s);
+        case 1: return hot_rag<1>(d, map, nt, nf, lds, s);
+        case 2: return hot_rag<2>(d, map, nt, nf, lds, s);
+        case 3: return hot_rag<3>(d, map, nt, nf, lds, s);
+        case 4: return hot_rag<4>(d, map, nt, nf, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t c2r_prepare32() {
+    const void* fns[] = {
+        reinterpret_cast<const void*>(&k_c2r<0, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<0, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<1, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<1, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<2, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<2, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<3, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<3, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<4, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<4, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<4, C2R_PLAIN, float>),
+    };
+    for (const void* f : fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace dns

@@ -1,249 +1,89 @@
-// fft_kernels.hip -- 2D plane transforms real(nx,ny) <-> complex(nx/2+1,ny) with the whole
-// half-complex plane resident in LDS (one workgroup per plane and field), in FP64 or FP32.
-//
-// c2r: complex inverse along y for every kx column, then the x direction as a half-length
-// complex inverse of packed pairs z_n = x_{2n} + i x_{2n+1}; the packing uses only Re of the
-// kx = 0 and kx = nx/2 entries, which is FFTW's c2r result for non-Hermitian input
-// (fftw3_dns_module.f90:116-132).  Output divided by nx*ny.
-// r2c: the mirror image, unnormalised (fftw3_dns_module.f90:95-108).
+// fft_kernels.hip -- host side of the plane transforms (size classes, stage plans, launch
+// dispatch to the instantiating units fft_c2r64/32.hip and fft_r2c64/32.hip), and the two-pass
+// FP64 transforms for planes beyond the LDS.
 #include <algorithm>
+#include <vector>
 
-#include "kernels.h"
+#include "fft_plane.h"
 
 namespace dns {
 
-// plane-size classes: (threads, max complex elements of the half plane).  Class 0 is one
-// wavefront per plane (its barriers are nearly free; small planes of ensembles keep many planes
-// in flight per CU); class 4 only fits the LDS in FP32 (192 x 192).
-template <int C> struct Cls;
-template <> struct Cls<0> { static constexpr int NT = 64, MAXE = 1088; };
-template <> struct Cls<1> { static constexpr int NT = 256, MAXE = 2176; };
-template <> struct Cls<2> { static constexpr int NT = 512, MAXE = 4704; };
-template <> struct Cls<3> { static constexpr int NT = 1024, MAXE = 8320; };
-template <> struct Cls<4> { static constexpr int NT = 1024, MAXE = 18624; };
-
-template <class T> struct Vec;
-template <> struct Vec<double> { using type = double2; };
-template <> struct Vec<float> { using type = float2; };
-
-// load the complex (or real pair) at element index idx2 (in pairs) of a double or float array
-template <class V>
-__device__ __forceinline__ V ld2(const void* p, size_t idx2, bool f32) {
-    if (f32) {
-        const float2 x = reinterpret_cast<const float2*>(p)[idx2];
-        return mk2(typename Scalar<V>::type(x.x), typename Scalar<V>::type(x.y));
-    }
-    const double2 x = reinterpret_cast<const double2*>(p)[idx2];
-    return mk2(typename Scalar<V>::type(x.x), typename Scalar<V>::type(x.y));
-}
-template <class V>
-__device__ __forceinline__ void st2(void* p, size_t idx2, V v, bool f32) {
-    if (f32) reinterpret_cast<float2*>(p)[idx2] = make_float2(float(v.x), float(v.y));
-    else reinterpret_cast<double2*>(p)[idx2] = make_double2(double(v.x), double(v.y));
+// ---- host: classes and plans ----------------------------------------------------------------
+// plane (nx/2) x ny, twiddles ny + nx, column-0 buffer ny (complex); kx, ky (real)
+static size_t lds_bytes(int nx, int ny, bool f32) {
+    const size_t t = f32 ? sizeof(float) : sizeof(double);
+    return 2 * t * (size_t(nx / 2) * ny + 2 * size_t(ny) + size_t(nx)) + t * (size_t(nx / 2 + 1) + ny);
 }
 
-// spectral input of c2r field f at plane k, mode (i, j), flat index e = i + nxh*j
-template <int OP, class V>
-__device__ __forceinline__ V c2r_input(const FFTArgs& a, int f, int k, int e, int i, int j) {
-    using S = typename Scalar<V>::type;
-    const size_t off2 = (size_t(k) * a.sstride) / 2 + size_t(e);
-    if (OP == C2R_PLAIN) {
-        return ld2<V>(a.src[f], off2, (a.src_f32 >> f) & 1);
-    } else if (OP == C2R_NL) {
-        // 0 u, 1 v, 2 w, 3 omega_x, 4 omega_y (precomputed), 5 omega_z = i kx v - i ky u
-        if (f < 5) return ld2<V>(a.src[f], off2, (a.src_f32 >> f) & 1);
-        const V u = ld2<V>(a.src[0], off2, a.src_f32 & 1);
-        const V v = ld2<V>(a.src[1], off2, (a.src_f32 >> 1) & 1);
-        const S kx = S(a.xw[i]), ky = S(a.yw[j]);
-        // (i kx v) - (i ky u)
-        return mk2(-kx * v.y - (-ky * u.y), kx * v.x - ky * u.x);
-    } else {
-        // divergence: i kx u + i ky v + (D1 w)
-        const V u = ld2<V>(a.src[0], off2, a.src_f32 & 1);
-        const V v = ld2<V>(a.src[1], off2, (a.src_f32 >> 1) & 1);
-        const V dw = ld2<V>(a.src[2], off2, (a.src_f32 >> 2) & 1);
-        const S kx = S(a.xw[i]), ky = S(a.yw[j]);
-        return mk2(-kx * u.y + -ky * v.y + dw.x, kx * u.x + ky * v.x + dw.y);
-    }
-}
-
-// plane k of field f
-template <int C, int OP, class T>
-__device__ __forceinline__ void c2r_plane(const FFTArgs& a, int k, int f) {
-    using V = typename Vec<T>::type;
-    constexpr int NT = Cls<C>::NT, MAXE = Cls<C>::MAXE;
-    extern __shared__ char smem_raw[];
-    V* lds = reinterpret_cast<V*>(smem_raw);
-    const V* twx = reinterpret_cast<const V*>(a.twx);
-    const V* twy = reinterpret_cast<const V*>(a.twy);
-    const int nxh = a.nxh, ny = a.ny, M = a.M, nx = a.nx;
-    const int H = nxh * ny;
-    for (int e = threadIdx.x; e < H; e += NT) {
-        const int j = e / nxh, i = e - j * nxh;
-        lds[e] = c2r_input<OP, V>(a, f, k, e, i, j);
-    }
-    __syncthreads();
-    // inverse along y: nxh transforms of length ny, element stride nxh
-    fft_lds<NT, MAXE, true, V>(lds, a.fy, nxh, nxh, 1, twy, ny, +1);
-    // pack each row for the half-length inverse along x
-    const int half = M / 2 + 1;
-    for (int e = threadIdx.x; e < ny * half; e += NT) {
-        const int y = e / half, kk = e - y * half;
-        V* row = lds + y * nxh;
-        if (kk == 0) {
-            const T x0 = row[0].x, xm = row[M].x;
-            row[0] = mk2(x0 + xm, x0 - xm);
-        } else {
-            const int k2 = M - kk;
-            const V A = row[kk], B = row[k2];
-            const V tw = twx[kk];
-            const V w = mk2(tw.x, -tw.y);                    // exp(+2 pi i kk/nx)
-            const V sm = mk2(A.x + B.x, A.y - B.y);          // A + conj B
-            const V d = mk2(A.x - B.x, A.y + B.y);           // A - conj B
-            const V wd = cmul(w, d);
-            const V Zk = mk2(sm.x - wd.y, sm.y + wd.x);      // s + i w d
-            if (k2 != kk) {
-                const V w2 = mk2(-w.x, w.y);                 // exp(+2 pi i (M-kk)/nx)
-                const V s2 = mk2(B.x + A.x, B.y - A.y);
-                const V d2 = mk2(B.x - A.x, B.y + A.y);
-                const V wd2 = cmul(w2, d2);
-                row[k2] = mk2(s2.x - wd2.y, s2.y + wd2.x);
-            }
-            row[kk] = Zk;
-        }
-    }
-    __syncthreads();
-    // inverse along x: ny transforms of length M, contiguous, row stride nxh
-    fft_lds<NT, MAXE, false, V>(lds, a.fx, ny, 1, nxh, twx, nx, +1);
-    const T norm = T(nx) * T(ny);
-    void* out = a.dst[f];
-    const size_t ob2 = (size_t(k) * a.dstride) / 2;
-    const bool of32 = a.dst_f32 != 0;
-    for (int e = threadIdx.x; e < M * ny; e += NT) {
-        const int y = e / M, n = e - y * M;
-        const V z = lds[y * nxh + n];
-        st2<V>(out, ob2 + size_t(y) * M + n, mk2(z.x / norm, z.y / norm), of32);
-    }
-}
-
-template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_c2r(FFTArgs a) {
-    c2r_plane<C, OP, T>(a, blockIdx.x, blockIdx.y);
-}
-template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map) {
-    const Tile t = map[blockIdx.x];
-    const FFTArgs& a = d[t.g];
-    c2r_plane<C, OP, T>(a, t.x, blockIdx.y);
-}
-
-template <int C, int OP, class T>
-__device__ __forceinline__ void r2c_plane(const FFTArgs& a, int k, int f) {
-    using V = typename Vec<T>::type;
-    constexpr int NT = Cls<C>::NT, MAXE = Cls<C>::MAXE;
-    extern __shared__ char smem_raw[];
-    V* lds = reinterpret_cast<V*>(smem_raw);
-    const V* twx = reinterpret_cast<const V*>(a.twx);
-    const V* twy = reinterpret_cast<const V*>(a.twy);
-    const int nxh = a.nxh, ny = a.ny, M = a.M, nx = a.nx;
-    const size_t po2 = (size_t(k) * a.sstride) / 2;
-    for (int e = threadIdx.x; e < M * ny; e += NT) {
-        const int y = e / M, n = e - y * M;
-        const size_t q2 = po2 + size_t(y) * M + n;
-        V z;
-        if (OP == R2C_PLAIN) {
-            z = ld2<V>(a.src[f], q2, (a.src_f32 >> f) & 1);
-        } else {
-            // rotational product S = u x omega (DNS_pressure_BC_3D.f90:1089-1091);
-            // src = u, v, w, omega_x, omega_y, omega_z
-            int ia, ib, ic, id;
-            if (f == 0) { ia = 1; ib = 5; ic = 2; id = 4; }        // v*oz - w*oy
-            else if (f == 1) { ia = 2; ib = 3; ic = 0; id = 5; }   // w*ox - u*oz
-            else { ia = 0; ib = 4; ic = 1; id = 3; }               // u*oy - v*ox
-            const V A = ld2<V>(a.src[ia], q2, (a.src_f32 >> ia) & 1);
-            const V B = ld2<V>(a.src[ib], q2, (a.src_f32 >> ib) & 1);
-            const V Cc = ld2<V>(a.src[ic], q2, (a.src_f32 >> ic) & 1);
-            const V D = ld2<V>(a.src[id], q2, (a.src_f32 >> id) & 1);
-            z = mk2(A.x * B.x - Cc.x * D.x, A.y * B.y - Cc.y * D.y);
-        }
-        lds[y * nxh + n] = z;
-    }
-    __syncthreads();
-    // forward along x: half-length transforms of the packed rows
-    fft_lds<NT, MAXE, false, V>(lds, a.fx, ny, 1, nxh, twx, nx, -1);
-    const int half = M / 2 + 1;
-    for (int e = threadIdx.x; e < ny * half; e += NT) {
-        const int y = e / half, kk = e - y * half;
-        V* row = lds + y * nxh;
-        if (kk == 0) {
-            const V Z0 = row[0];
-            row[0] = mk2(Z0.x + Z0.y, T(0));
-            row[M] = mk2(Z0.x - Z0.y, T(0));
-        } else {
-            const int k2 = M - kk;
-            const V A = row[kk], B = row[k2];
-            const V w = twx[kk];                                 // exp(-2 pi i kk/nx)
-            const V sm = mk2(A.x + B.x, A.y - B.y);
-            const V d = mk2(A.x - B.x, A.y + B.y);
-            const V wd = cmul(w, d);
-            const T h = T(0.5);
-            const V Xk = mk2(h * (sm.x + wd.y), h * (sm.y - wd.x));   // (s - i w d)/2
-            if (k2 != kk) {
-                const V w2 = mk2(-w.x, w.y);                     // exp(-2 pi i (M-kk)/nx)
-                const V s2 = mk2(B.x + A.x, B.y - A.y);
-                const V d2 = mk2(B.x - A.x, B.y + A.y);
-                const V wd2 = cmul(w2, d2);
-                row[k2] = mk2(h * (s2.x + wd2.y), h * (s2.y - wd2.x));
-            }
-            row[kk] = Xk;
-        }
-    }
-    __syncthreads();
-    // forward along y
-    fft_lds<NT, MAXE, true, V>(lds, a.fy, nxh, nxh, 1, twy, ny, -1);
-    const int H = nxh * ny;
-    void* out = a.dst[f];
-    const size_t ob2 = (size_t(k) * a.dstride) / 2;
-    const bool of32 = a.dst_f32 != 0;
-    const unsigned char* mask = a.mask;
-    for (int e = threadIdx.x; e < H; e += NT) {
-        V x = lds[e];
-        if (mask && !mask[e]) x = mk2(T(0), T(0));
-        st2<V>(out, ob2 + e, x, of32);
-    }
-}
-
-template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_r2c(FFTArgs a) {
-    r2c_plane<C, OP, T>(a, blockIdx.x, blockIdx.y);
-}
-template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_r2c_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map) {
-    const Tile t = map[blockIdx.x];
-    const FFTArgs& a = d[t.g];
-    r2c_plane<C, OP, T>(a, t.x, blockIdx.y);
-}
-
-static int size_class(int H, bool f32) {
-    if (H <= Cls<0>::MAXE) return 0;
-    if (H <= Cls<1>::MAXE) return 1;
-    if (H <= Cls<2>::MAXE) return 2;
-    if (H <= Cls<3>::MAXE) return 3;
-    if (f32 && H <= Cls<4>::MAXE) return 4;
+static int size_class(int nx, int ny, bool f32) {
+    const int H = (nx / 2) * ny;
+    if (lds_bytes(nx, ny, f32) > 163840) return -1;
+    for (int c = 0; c < FFT_CLASSES; c++)
+        if (H <= CLS_MAXE[c] && (c < 4 || f32)) return c;
     return -1;
+}
+
+// Stages for length n over nb transforms in class cls: at least two (the first reads the
+// source, the last writes the result), fewest, then the smallest largest radix, every stage
+// within the class's per-thread register budget (radix 2 and 3 always are, so every 2^a 3^b
+// length n >= 4 has a plan).  Stages run in increasing radix: the first stage, which reads
+// HBM, then has the most butterflies in flight.
+static bool make_plan(int n, int nb, int cls, bool f32, Plan& p) {
+    p.n = n; p.ns = 0;
+    if (cls < 0 || n < 1) return false;
+    auto fits = [&](int R) {
+        const long long items = (long long)nb * (n / R);
+        return (items + CLS_NT[cls] - 1) / CLS_NT[cls] <= stage_maxb(cls, R, f32);
+    };
+    std::vector<int> chosen;
+    std::vector<std::pair<int, std::vector<int>>> stack;   // (remaining length, radices so far)
+    stack.push_back({n, {}});
+    while (!stack.empty()) {
+        const int m = stack.back().first;
+        const std::vector<int> seq = stack.back().second;
+        stack.pop_back();
+        if (m == 1) {
+            if (seq.size() < 2) continue;
+            if (chosen.empty() || seq.size() < chosen.size() ||
+                (seq.size() == chosen.size() && seq.front() < chosen.front()))
+                chosen = seq;
+            continue;
+        }
+        const int maxr = seq.empty() ? 8 : seq.back();    // non-increasing: each multiset once
+        for (int R : {8, 4, 3, 2}) {
+            if (R > maxr || m % R || !fits(R)) continue;
+            std::vector<int> s2 = seq;
+            s2.push_back(R);
+            stack.push_back({m / R, s2});
+        }
+    }
+    if (chosen.empty() || chosen.size() > 8) return false;
+    std::reverse(chosen.begin(), chosen.end());
+    p.ns = int(chosen.size());
+    for (int i = 0; i < p.ns; i++) p.r[i] = chosen[i];
+    return true;
 }
 
 bool fft_geom(int nx, int ny, FFTGeom& g) {
     g.nx = nx; g.ny = ny; g.nxh = nx / 2 + 1; g.M = nx / 2;
-    const int H = g.nxh * ny;
-    g.cls64 = size_class(H, false);
-    g.cls32 = size_class(H, true);
+    g.cls64 = size_class(nx, ny, false);
+    g.cls32 = size_class(nx, ny, true);
+    // plans stay empty (ns = 0) when nx/2 or ny is not 2^a 3^b; the plane launchers refuse them
+    for (int f = 0; f < 2; f++) {
+        const int cls = f ? g.cls32 : g.cls64;
+        if (!(make_plan(ny, g.M, cls, f != 0, g.py[f]) && make_plan(g.M, ny, cls, f != 0, g.px[f])))
+            g.py[f].ns = g.px[f].ns = 0;
+    }
     return g.cls32 >= 0;
+}
+
+size_t plane_lds_bytes(const FFTGeom& g, bool f32) {
+    return lds_bytes(g.nx, g.ny, f32);
 }
 
 // ---------------------------------------------------------------------------------------
 // Two-pass FP64 plane transforms, for planes whose half-complex plane does not fit the LDS
-// (nx/2+1)*ny > 8320 (192 x 192).  Field input/output and diagnostics use them; the step
+// (nx/2)*ny > 8192 in FP64 (192 x 192).  Field input/output and diagnostics use them; the step
 // itself runs such planes in FP32.  Pass "rows": LROWS x-lines per block (packed half-length
 // complex FFT plus the real/half-complex split); pass "cols": LCOLS kx-columns per block
 // (complex FFT along y), both staged in LDS, through a scratch plane in HBM.
@@ -412,22 +252,6 @@ hipError_t launch_two_pass(const FFTGeom& g, bool forward, const double* src, do
     return hipGetLastError();
 }
 
-template <int C, class T>
-static hipError_t prepare_ct() {
-    const void* fns[] = {reinterpret_cast<const void*>(&k_c2r<C, C2R_PLAIN, T>),
-                         reinterpret_cast<const void*>(&k_c2r<C, C2R_NL, T>),
-                         reinterpret_cast<const void*>(&k_c2r<C, C2R_DIV, T>),
-                         reinterpret_cast<const void*>(&k_r2c<C, R2C_PLAIN, T>),
-                         reinterpret_cast<const void*>(&k_r2c<C, R2C_CROSS, T>),
-                         reinterpret_cast<const void*>(&k_c2r_rag<C, C2R_NL, T>),
-                         reinterpret_cast<const void*>(&k_r2c_rag<C, R2C_CROSS, T>)};
-    for (const void* f : fns) {
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-}
-
 static hipError_t prepare_two_pass() {
     const void* fns[] = {reinterpret_cast<const void*>(&k_tp_rows<-1>), reinterpret_cast<const void*>(&k_tp_rows<1>),
                          reinterpret_cast<const void*>(&k_tp_cols<-1>), reinterpret_cast<const void*>(&k_tp_cols<1>)};
@@ -438,92 +262,62 @@ static hipError_t prepare_two_pass() {
     return hipSuccess;
 }
 
-hipError_t fft_prepare(const FFTGeom& g) {
-    hipError_t e = prepare_two_pass();
-    if (e != hipSuccess) return e;
-    switch (g.cls64) {
-        case 0: e = prepare_ct<0, double>(); break;
-        case 1: e = prepare_ct<1, double>(); break;
-        case 2: e = prepare_ct<2, double>(); break;
-        case 3: e = prepare_ct<3, double>(); break;
-        default: break;
-    }
-    if (e != hipSuccess) return e;
-    switch (g.cls32) {
-        case 0: return prepare_ct<0, float>();
-        case 1: return prepare_ct<1, float>();
-        case 2: return prepare_ct<2, float>();
-        case 3: return prepare_ct<3, float>();
-        case 4: return prepare_ct<4, float>();
-        default: return hipErrorInvalidValue;
-    }
-}
+hipError_t fft_prepare(const FFTGeom&) { return fft_prepare_all(); }
 
 hipError_t fft_prepare_all() {
-    hipError_t e;
-    if ((e = prepare_two_pass()) != hipSuccess) return e;
-    if ((e = prepare_ct<0, double>()) != hipSuccess || (e = prepare_ct<1, double>()) != hipSuccess ||
-        (e = prepare_ct<2, double>()) != hipSuccess || (e = prepare_ct<3, double>()) != hipSuccess ||
-        (e = prepare_ct<0, float>()) != hipSuccess || (e = prepare_ct<1, float>()) != hipSuccess ||
-        (e = prepare_ct<2, float>()) != hipSuccess || (e = prepare_ct<3, float>()) != hipSuccess ||
-        (e = prepare_ct<4, float>()) != hipSuccess)
-        return e;
-    return hipSuccess;
+    static hipError_t done = [] {
+        hipError_t e;
+        if ((e = prepare_two_pass()) != hipSuccess || (e = c2r_prepare64()) != hipSuccess ||
+            (e = c2r_prepare32()) != hipSuccess || (e = r2c_prepare64()) != hipSuccess ||
+            (e = r2c_prepare32()) != hipSuccess)
+            return e;
+        return hipSuccess;
+    }();
+    return done;
 }
 
 int fft_class_capacity(int c, bool f32) {
-    switch (c) {
-        case 0: return Cls<0>::MAXE;
-        case 1: return Cls<1>::MAXE;
-        case 2: return Cls<2>::MAXE;
-        case 3: return Cls<3>::MAXE;
-        case 4: return f32 ? Cls<4>::MAXE : -1;
-        default: return -1;
-    }
+    if (c < 0 || c >= FFT_CLASSES || (c == 4 && !f32)) return -1;
+    return CLS_MAXE[c];
 }
 
 void fft_fill(const FFTGeom& g, FFTArgs& a, bool f32) {
     a.nx = g.nx; a.ny = g.ny; a.nxh = g.nxh; a.M = g.M;
-    a.fx = g.fx; a.fy = g.fy;
+    a.py = g.py[f32 ? 1 : 0]; a.px = g.px[f32 ? 1 : 0];
     if (f32) { a.twx = g.twx32; a.twy = g.twy32; }
     else { a.twx = g.twx; a.twy = g.twy; }
 }
 
-template <int C, class T>
-static hipError_t run(bool c2r, int op, const FFTArgs& a, int np, int nf, hipStream_t s) {
-    const size_t lds = size_t(a.nxh) * a.ny * 2 * sizeof(T);
-    const dim3 grid(np, nf), block(Cls<C>::NT);
+// Operand precisions the instances are built for: FP64 compute FP64 in and out; FP32 compute
+// of the step NL (u, v, w FP64, omega_x, omega_y FP64 or FP32 per field), FP32 out; FP32
+// CROSS FP32 in and out; FP32 PLAIN FP64 in and out; DIV FP64 only.
+static bool operands_ok(bool c2r, int op, const FFTArgs& a, bool f32) {
+    if (!f32) return a.src_f32 == 0 && a.dst_f32 == 0;
     if (c2r) {
-        if (op == C2R_PLAIN) hipLaunchKernelGGL((k_c2r<C, C2R_PLAIN, T>), grid, block, lds, s, a);
-        else if (op == C2R_NL) hipLaunchKernelGGL((k_c2r<C, C2R_NL, T>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_c2r<C, C2R_DIV, T>), grid, block, lds, s, a);
-    } else {
-        if (op == R2C_PLAIN) hipLaunchKernelGGL((k_r2c<C, R2C_PLAIN, T>), grid, block, lds, s, a);
-        else hipLaunchKernelGGL((k_r2c<C, R2C_CROSS, T>), grid, block, lds, s, a);
+        if (op == C2R_NL) return a.dst_f32 && (a.src_f32 & 7) == 0;
+        if (op == C2R_PLAIN) return a.src_f32 == 0 && a.dst_f32 == 0;
+        return false;
     }
-    return hipGetLastError();
+    if (op == R2C_CROSS) return (a.src_f32 & 63) == 63 && a.dst_f32;
+    return a.src_f32 == 0 && a.dst_f32 == 0;
 }
 
 static hipError_t dispatch(const FFTGeom& g, bool c2r, int op, FFTArgs a, int np, int nf, bool f32, hipStream_t s) {
     if (np <= 0 || nf <= 0) return hipSuccess;
     fft_fill(g, a, f32);
-    if (f32) {
-        switch (g.cls32) {
-            case 0: return run<0, float>(c2r, op, a, np, nf, s);
-            case 1: return run<1, float>(c2r, op, a, np, nf, s);
-            case 2: return run<2, float>(c2r, op, a, np, nf, s);
-            case 3: return run<3, float>(c2r, op, a, np, nf, s);
-            case 4: return run<4, float>(c2r, op, a, np, nf, s);
-            default: return hipErrorInvalidValue;
-        }
-    }
-    switch (g.cls64) {
-        case 0: return run<0, double>(c2r, op, a, np, nf, s);
-        case 1: return run<1, double>(c2r, op, a, np, nf, s);
-        case 2: return run<2, double>(c2r, op, a, np, nf, s);
-        case 3: return run<3, double>(c2r, op, a, np, nf, s);
-        default: return hipErrorInvalidValue;
-    }
+    const int cls = f32 ? g.cls32 : g.cls64;
+    if (cls < 0 || a.py.ns < 2 || a.px.ns < 2 || !operands_ok(c2r, op, a, f32)) return hipErrorInvalidValue;
+    // omega_z and the divergence read the wavenumber tables; every source and destination is read
+    // or written
+    if (c2r && op != C2R_PLAIN && (!a.xw || !a.yw)) return hipErrorInvalidValue;
+    const int nsrc = c2r ? (op == C2R_NL ? 5 : op == C2R_DIV ? 3 : nf) : (op == R2C_CROSS ? 6 : nf);
+    for (int q = 0; q < nsrc; q++)
+        if (!a.src[q]) return hipErrorInvalidValue;
+    for (int q = 0; q < nf; q++)
+        if (!a.dst[q]) return hipErrorInvalidValue;
+    const size_t lds = plane_lds_bytes(g, f32);
+    if (c2r) return f32 ? c2r_launch32(cls, op, a, np, nf, lds, s) : c2r_launch64(cls, op, a, np, nf, lds, s);
+    return f32 ? r2c_launch32(cls, op, a, np, nf, lds, s) : r2c_launch64(cls, op, a, np, nf, lds, s);
 }
 
 hipError_t launch_c2r(const FFTGeom& g, int op, FFTArgs a, int np, int nf, bool f32, hipStream_t s) {
@@ -533,45 +327,19 @@ hipError_t launch_r2c(const FFTGeom& g, int op, FFTArgs a, int np, int nf, bool 
     return dispatch(g, false, op, a, np, nf, f32, s);
 }
 
-// The ragged launches are used by the fused step only (C2R_NL, R2C_CROSS).
-template <int C, class T>
-static hipError_t run_rag(bool c2r, const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds, hipStream_t s) {
-    const dim3 grid(nt, nf), block(Cls<C>::NT);
-    if (c2r) hipLaunchKernelGGL((k_c2r_rag<C, C2R_NL, T>), grid, block, lds, s, d, map);
-    else hipLaunchKernelGGL((k_r2c_rag<C, R2C_CROSS, T>), grid, block, lds, s, d, map);
-    return hipGetLastError();
-}
-
-static hipError_t dispatch_rag(bool c2r, int cls, int op, bool f32, const FFTArgs* d, const Tile* map, int nt, int nf,
-                               size_t lds, hipStream_t s) {
-    if (nt <= 0 || nf <= 0) return hipSuccess;
-    if (op != (c2r ? int(C2R_NL) : int(R2C_CROSS))) return hipErrorInvalidValue;
-    if (f32) {
-        switch (cls) {
-            case 0: return run_rag<0, float>(c2r, d, map, nt, nf, lds, s);
-            case 1: return run_rag<1, float>(c2r, d, map, nt, nf, lds, s);
-            case 2: return run_rag<2, float>(c2r, d, map, nt, nf, lds, s);
-            case 3: return run_rag<3, float>(c2r, d, map, nt, nf, lds, s);
-            case 4: return run_rag<4, float>(c2r, d, map, nt, nf, lds, s);
-            default: return hipErrorInvalidValue;
-        }
-    }
-    switch (cls) {
-        case 0: return run_rag<0, double>(c2r, d, map, nt, nf, lds, s);
-        case 1: return run_rag<1, double>(c2r, d, map, nt, nf, lds, s);
-        case 2: return run_rag<2, double>(c2r, d, map, nt, nf, lds, s);
-        case 3: return run_rag<3, double>(c2r, d, map, nt, nf, lds, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
+// The ragged launches are used by the fused step only (C2R_NL, R2C_CROSS); the descriptors
+// are built by the caller with fft_fill and the step's operand precisions.
 hipError_t launch_c2r_rag(int cls, int op, bool f32, const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds,
                           hipStream_t s) {
-    return dispatch_rag(true, cls, op, f32, d, map, nt, nf, lds, s);
+    if (nt <= 0 || nf <= 0) return hipSuccess;
+    if (op != C2R_NL) return hipErrorInvalidValue;
+    return f32 ? c2r_launch_rag32(cls, d, map, nt, nf, lds, s) : c2r_launch_rag64(cls, d, map, nt, nf, lds, s);
 }
 hipError_t launch_r2c_rag(int cls, int op, bool f32, const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds,
                           hipStream_t s) {
-    return dispatch_rag(false, cls, op, f32, d, map, nt, nf, lds, s);
+    if (nt <= 0 || nf <= 0) return hipSuccess;
+    if (op != R2C_CROSS) return hipErrorInvalidValue;
+    return f32 ? r2c_launch_rag32(cls, d, map, nt, nf, lds, s) : r2c_launch_rag64(cls, d, map, nt, nf, lds, s);
 }
 
 }  // namespace dns

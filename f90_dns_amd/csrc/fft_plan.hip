@@ -44,7 +44,7 @@ int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
     if (!out) return DNS_EINVAL;
     *out = nullptr;
     if (precision != DNS_PREC_FP64 && precision != DNS_PREC_MIXED) { dns_set_create_error("bad precision"); return DNS_EINVAL; }
-    if (nx < 4 || ny < 4 || (nx & 1) || (ny & 1)) { dns_set_create_error("nx, ny must be even and >= 4"); return DNS_EINVAL; }
+    if (nx < 8 || ny < 4 || (nx & 1) || (ny & 1)) { dns_set_create_error("nx must be even and >= 8, ny even and >= 4"); return DNS_EINVAL; }
     std::string why;
     if (!dns_check_gfx950(why)) { dns_set_create_error(why); return DNS_EDEVICE; }
     dns_fft_plan* p = new dns_fft_plan();

@@ -18,9 +18,34 @@ struct Tile {
 enum C2ROp { C2R_PLAIN = 0, C2R_NL = 1, C2R_DIV = 2 };
 enum R2COp { R2C_PLAIN = 0, R2C_CROSS = 1 };
 
+// Plane-size classes of the plane kernels: threads per block and the largest plane (nx/2)*ny
+// complex values they take (eight per thread; eighteen in class 4, which fits the LDS only in
+// FP32).  Class 0 is one wavefront.
+constexpr int FFT_CLASSES = 5;
+constexpr int CLS_NT[FFT_CLASSES] = {64, 256, 512, 1024, 1024};
+constexpr int CLS_MAXE[FFT_CLASSES] = {512, 2048, 4096, 8192, 18432};
+// complex values a thread may hold in one transform stage (registers)
+constexpr int stage_budget(bool f32) { return f32 ? 36 : 18; }
+// butterflies per thread a radix-R stage of class cls is compiled for
+constexpr int stage_maxb(int cls, int R, bool f32) {
+    return ((CLS_MAXE[cls] / R + CLS_NT[cls] - 1) / CLS_NT[cls]) < (stage_budget(f32) / R > 1 ? stage_budget(f32) / R : 1)
+               ? (CLS_MAXE[cls] / R + CLS_NT[cls] - 1) / CLS_NT[cls]
+               : (stage_budget(f32) / R > 1 ? stage_budget(f32) / R : 1);
+}
+
+// Radices of the Stockham stages of one direction of a plane transform (16, 8, 4, 3, 2),
+// chosen per plane shape and precision so that every stage fits the registers of the plane's
+// smallest size class; larger classes run the same stages, so a plane's result does not depend
+// on the class it runs in.
+struct Plan {
+    int n, ns;
+    int r[8];
+};
+
 struct FFTGeom {
     int nx, ny, nxh, M;          // M = nx/2
-    Factors fy, fx;              // factors of ny and of M
+    Factors fy, fx;              // radix-4/2/3 factors of ny and M (two-pass transforms)
+    Plan py[2], px[2];           // [f32]: stages along y (M columns) and x (ny rows, length M)
     const double2* twx;          // exp(-2 pi i k/nx), k < nx (FP64)
     const double2* twy;          // exp(-2 pi i k/ny), k < ny
     const float2* twx32;         // the same tables rounded to FP32
@@ -32,7 +57,7 @@ struct FFTGeom {
 // src[f] as float; dst_f32 marks all destinations as float.  Strides are in elements.
 struct FFTArgs {
     int nx, ny, nxh, M;
-    Factors fy, fx;
+    Plan py, px;
     const void* twx;
     const void* twy;
     const double* xw;
@@ -45,7 +70,11 @@ struct FFTArgs {
     int dst_f32;
 };
 
+// shape, size classes and stage plans (plans need nx/2 and ny of the form 2^a 3^b)
 bool fft_geom(int nx, int ny, FFTGeom& g);
+// dynamic LDS bytes of a plane kernel: the plane as M columns x ny rows, the y and x twiddle
+// tables, and the c2r side buffers (combined column 0, kx, ky)
+size_t plane_lds_bytes(const FFTGeom& g, bool f32);
 // fill the geometry fields of a (shape, factors, twiddles of the compute precision)
 void fft_fill(const FFTGeom& g, FFTArgs& a, bool f32);
 // max complex elements of the half plane for size class c (-1: no such class for f32)
