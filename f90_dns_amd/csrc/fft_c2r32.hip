@@ -7,18 +7,18 @@ namespace dns {
 
 template <int C>
 static hipError_t hot(const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_c2r<C, C2R_NL, float>), dim3(np, nf), dim3(Cls<C>::NT), lds, s, a);
+    hipLaunchKernelGGL((k_c2r<C, C2R_NL, float>), dim3(xcd_grid(np, nf)), dim3(Cls<C>::NT), lds, s, a, np, nf);
     return hipGetLastError();
 }
 template <int C>
 static hipError_t hot_rag(const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_c2r_rag<C, C2R_NL, float>), dim3(nt, nf), dim3(Cls<C>::NT), lds, s, d, map);
+    hipLaunchKernelGGL((k_c2r_rag<C, C2R_NL, float>), dim3(xcd_grid(nt, nf)), dim3(Cls<C>::NT), lds, s, d, map, nt, nf);
     return hipGetLastError();
 }
 
 hipError_t c2r_launch32(int cls, int op, const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
     if (op == C2R_PLAIN) {
-        hipLaunchKernelGGL((k_c2r<4, C2R_PLAIN, float>), dim3(np, nf), dim3(Cls<4>::NT), lds, s, a);
+        hipLaunchKernelGGL((k_c2r<4, C2R_PLAIN, float>), dim3(xcd_grid(np, nf)), dim3(Cls<4>::NT), lds, s, a, np, nf);
         return hipGetLastError();
     }
     if (op != C2R_NL) return hipErrorInvalidValue;

@@ -9,10 +9,11 @@
 namespace dns {
 
 // ---- host: classes and plans ----------------------------------------------------------------
-// plane (nx/2) x ny, twiddles ny + nx, column-0 buffer ny (complex); kx, ky (real)
+// plane (nx/2) x ny in whole swizzle blocks, twiddles ny + nx, column-0 buffer ny (complex);
+// kx, ky (real)
 static size_t lds_bytes(int nx, int ny, bool f32) {
     const size_t t = f32 ? sizeof(float) : sizeof(double);
-    return 2 * t * (size_t(nx / 2) * ny + 2 * size_t(ny) + size_t(nx)) + t * (size_t(nx / 2 + 1) + ny);
+    return 2 * t * (size_t(plane_region(nx / 2, ny)) + 2 * size_t(ny) + size_t(nx)) + t * (size_t(nx / 2 + 1) + ny);
 }
 
 static int size_class(int nx, int ny, bool f32) {

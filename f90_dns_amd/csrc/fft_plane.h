@@ -50,6 +50,31 @@ __device__ __forceinline__ int fdiv(int x, const FastDiv& f) {
     return f.d > 1 ? int(__umulhi(unsigned(x), f.m)) : x;
 }
 
+// LDS bank swizzle of the flat plane index: x ^ ((x >> 3) & mask) permutes each aligned
+// block of 16 (FP64) or 32 (FP32) elements, so that the stride-R writes of a first Stockham stage
+// (and the stride-Ns runs of later ones) spread over the banks, while runs of consecutive
+// indices stay conflict free.  The plane region is a whole number of such blocks.
+template <class V> struct Swz;
+template <> struct Swz<double2> { static constexpr int MASK = 15; };
+template <> struct Swz<float2> { static constexpr int MASK = 31; };
+template <class V> __device__ __forceinline__ int swz(int x) { return x ^ ((x >> 3) & Swz<V>::MASK); }
+// complex elements of the plane region (M * ny rounded up to a whole swizzle block)
+__host__ __device__ inline int plane_region(int M, int ny) { return (M * ny + 31) & ~31; }
+
+// XCD-aware block order.  Workgroups go to the 8 XCDs round-robin by linear id, so block
+// 8 * slot + x runs on XCD x; the slots of one XCD take the nf fields of one plane (or ragged
+// tile) one after another.  Blocks that read the same planes (omega_z reads u and v; each cross
+// product reads four of the six physical fields) then share an L2 and run close in time.
+constexpr int XCDS = 8;
+__device__ __forceinline__ bool xcd_item(int n, int nf, int& item, int& f) {
+    const int L = blockIdx.x, x = L % XCDS, slot = L / XCDS;
+    const int t = slot / nf;
+    f = slot - t * nf;
+    item = t * XCDS + x;
+    return item < n;
+}
+inline int xcd_grid(int n, int nf) { return XCDS * nf * ((n + XCDS - 1) / XCDS); }
+
 // complex value of type V from a complex element of type S
 template <class V, class S>
 __device__ __forceinline__ V ldc(const void* p, size_t idx2) {
@@ -196,15 +221,16 @@ template <class V, bool YDIR> struct LdsLoad {
     static constexpr bool kLds = true;
     using Raw = V;
     const V* lds; int pitch;
-    __device__ __forceinline__ V fetch(int b, int e) const { return YDIR ? lds[e * pitch + b] : lds[b * pitch + e]; }
+    __device__ __forceinline__ V fetch(int b, int e) const {
+        return lds[swz<V>(YDIR ? e * pitch + b : b * pitch + e)];
+    }
     __device__ __forceinline__ V finish(int, int, V x) const { return x; }
 };
 template <class V, bool YDIR> struct LdsStore {
     static constexpr bool kLds = true;
     V* lds; int pitch;
     __device__ __forceinline__ void operator()(int b, int e, V x) const {
-        if (YDIR) lds[e * pitch + b] = x;
-        else lds[b * pitch + e] = x;
+        lds[swz<V>(YDIR ? e * pitch + b : b * pitch + e)] = x;
     }
 };
 
@@ -364,8 +390,8 @@ template <class V> struct PackLoad {
     const V* lds; const V* twx; int M;
     __device__ __forceinline__ V finish(int, int, V x) const { return x; }
     __device__ __forceinline__ V fetch(int y, int n) const {
-        const V* row = lds + y * M;
-        const V A = row[n], B = row[n == 0 ? 0 : M - n];
+        const int r0 = y * M;
+        const V A = lds[swz<V>(r0 + n)], B = lds[swz<V>(r0 + (n == 0 ? 0 : M - n))];
         const V tw = twx[n];
         const V w = mk2(tw.x, -tw.y);                    // exp(+2 pi i n/nx)
         const V sm = mk2(A.x + B.x, A.y - B.y);          // A + conj B
@@ -392,7 +418,7 @@ __device__ __forceinline__ void c2r_body(const FFTArgs& a, int k, int f, typenam
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int NT = Cls<C>::NT;
     const int M = a.M, ny = a.ny, nx = a.nx;
-    V* twy = lds + M * ny;
+    V* twy = lds + plane_region(M, ny);
     V* twx = twy + ny;
     V* c0 = twx + nx;
     T* kx = reinterpret_cast<T*>(c0 + ny);
@@ -430,15 +456,20 @@ __device__ __forceinline__ void c2r_plane(const FFTArgs& a, int k, int f) {
     }
 }
 
+// grid of xcd_grid(np, nf) blocks over np planes (or ragged tiles) and nf fields
 template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_c2r(FFTArgs a) {
-    c2r_plane<C, OP, T>(a, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(Cls<C>::NT) void k_c2r(FFTArgs a, int np, int nf) {
+    int k, f;
+    if (xcd_item(np, nf, k, f)) c2r_plane<C, OP, T>(a, k, f);
 }
 template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map) {
-    const Tile t = map[blockIdx.x];
+__global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map,
+                                                        int nt, int nf) {
+    int i, f;
+    if (!xcd_item(nt, nf, i, f)) return;
+    const Tile t = map[i];
     const FFTArgs& a = d[t.g];
-    c2r_plane<C, OP, T>(a, t.x, blockIdx.y);
+    c2r_plane<C, OP, T>(a, t.x, f);
 }
 
 // ---- r2c ------------------------------------------------------------------------------------
@@ -474,8 +505,8 @@ template <class V> struct SplitLoad {
     __device__ __forceinline__ V finish(int, int, V x) const { return x; }
     __device__ __forceinline__ V fetch(int b, int y) const {
         using U = typename Scalar<V>::type;
-        const V* row = lds + y * M;
-        const V A = row[b], B = row[b == 0 ? 0 : M - b];
+        const int r0 = y * M;
+        const V A = lds[swz<V>(r0 + b)], B = lds[swz<V>(r0 + (b == 0 ? 0 : M - b))];
         const V w = twx[b];                                  // exp(-2 pi i b/nx)
         const V sm = mk2(A.x + B.x, A.y - B.y);
         const V d = mk2(A.x - B.x, A.y + B.y);
@@ -493,7 +524,7 @@ template <class V, class D> struct R2CStore {
     __device__ __forceinline__ void operator()(int b, int y, V x) const {
         using U = typename Scalar<V>::type;
         if (b == 0) {
-            lds[y * M] = x;
+            lds[swz<V>(y * M)] = x;
         } else {
             const size_t e = size_t(y) * nxh + b;
             if (mask && !mask[e]) x = mk2(U(0), U(0));
@@ -509,7 +540,7 @@ __device__ __forceinline__ void r2c_body(const FFTArgs& a, int k, int f, typenam
     constexpr bool F32 = sizeof(T) == 4;
     constexpr int NT = Cls<C>::NT;
     const int M = a.M, ny = a.ny, nxh = a.nxh;
-    V* twy = lds + M * ny;
+    V* twy = lds + plane_region(M, ny);
     V* twx = twy + ny;
     run_direction<C, F32, -1, false>(a.px, ny, lds, M, twx, a.nx,
                                      R2CLoad<OP, V, SRC>{&a, f, (size_t(k) * a.sstride) / 2},
@@ -521,7 +552,7 @@ __device__ __forceinline__ void r2c_body(const FFTArgs& a, int k, int f, typenam
     // separate column 0: C_y = F0_y + i FM_y with F0, FM Hermitian in y
     const unsigned char* mask = a.mask;
     for (int y = threadIdx.x; y < ny; y += NT) {
-        const V c = lds[y * M], cr = lds[(y == 0 ? 0 : ny - y) * M];
+        const V c = lds[swz<V>(y * M)], cr = lds[swz<V>((y == 0 ? 0 : ny - y) * M)];
         const U h = U(0.5);
         V x0 = mk2(h * (c.x + cr.x), h * (c.y - cr.y));
         V xm = mk2(h * (c.y + cr.y), -h * (c.x - cr.x));
@@ -540,21 +571,25 @@ __device__ __forceinline__ void r2c_plane(const FFTArgs& a, int k, int f) {
     using V = typename Vec<T>::type;
     extern __shared__ char smem_raw[];
     V* lds = reinterpret_cast<V*>(smem_raw);
-    load_twiddles<Cls<C>::NT>(a, lds + a.M * a.ny);
+    load_twiddles<Cls<C>::NT>(a, lds + plane_region(a.M, a.ny));
     __syncthreads();
     if constexpr (sizeof(T) == 4 && OP == R2C_CROSS) r2c_body<C, OP, T, float, float>(a, k, f, lds);
     else r2c_body<C, OP, T, double, double>(a, k, f, lds);
 }
 
 template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_r2c(FFTArgs a) {
-    r2c_plane<C, OP, T>(a, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(Cls<C>::NT) void k_r2c(FFTArgs a, int np, int nf) {
+    int k, f;
+    if (xcd_item(np, nf, k, f)) r2c_plane<C, OP, T>(a, k, f);
 }
 template <int C, int OP, class T>
-__global__ __launch_bounds__(Cls<C>::NT) void k_r2c_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map) {
-    const Tile t = map[blockIdx.x];
+__global__ __launch_bounds__(Cls<C>::NT) void k_r2c_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map,
+                                                        int nt, int nf) {
+    int i, f;
+    if (!xcd_item(nt, nf, i, f)) return;
+    const Tile t = map[i];
     const FFTArgs& a = d[t.g];
-    r2c_plane<C, OP, T>(a, t.x, blockIdx.y);
+    r2c_plane<C, OP, T>(a, t.x, f);
 }
 
 // ---- launchers of the instantiating translation units ----------------------------------------

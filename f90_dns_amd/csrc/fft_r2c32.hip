@@ -7,18 +7,18 @@ namespace dns {
 
 template <int C>
 static hipError_t hot(const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_r2c<C, R2C_CROSS, float>), dim3(np, nf), dim3(Cls<C>::NT), lds, s, a);
+    hipLaunchKernelGGL((k_r2c<C, R2C_CROSS, float>), dim3(xcd_grid(np, nf)), dim3(Cls<C>::NT), lds, s, a, np, nf);
     return hipGetLastError();
 }
 template <int C>
 static hipError_t hot_rag(const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_r2c_rag<C, R2C_CROSS, float>), dim3(nt, nf), dim3(Cls<C>::NT), lds, s, d, map);
+    hipLaunchKernelGGL((k_r2c_rag<C, R2C_CROSS, float>), dim3(xcd_grid(nt, nf)), dim3(Cls<C>::NT), lds, s, d, map, nt, nf);
     return hipGetLastError();
 }
 
 hipError_t r2c_launch32(int cls, int op, const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
     if (op == R2C_PLAIN) {
-        hipLaunchKernelGGL((k_r2c<4, R2C_PLAIN, float>), dim3(np, nf), dim3(Cls<4>::NT), lds, s, a);
+        hipLaunchKernelGGL((k_r2c<4, R2C_PLAIN, float>), dim3(xcd_grid(np, nf)), dim3(Cls<4>::NT), lds, s, a, np, nf);
         return hipGetLastError();
     }
     if (op != R2C_CROSS) return hipErrorInvalidValue;
