@@ -6,7 +6,7 @@
 // w takes the row-block pairs b = w, w + 4, ...: group-0 block b and group-1 block b, so the two
 // results of one output row pair (g0_i, g1_i) sit in the same lane and item and unfold in
 // registers.  The panels (the spectral data, folded and prologue applied) are produced once per
-// element by one thread and staged through double-buffered LDS chunks of KC rows; the next
+// element by one thread and staged through double-buffered LDS chunks of rows; the next
 // chunk and the wave's A operands for it are fetched while the current one is consumed.
 #pragma once
 #include "device_common.h"
@@ -17,27 +17,36 @@ namespace dns {
 constexpr int CT = 2;              // 16-column MFMA tiles per block
 constexpr int BCOLS = 16 * CT;     // real columns per block
 constexpr int BMODES = BCOLS / 2;  // complex modes per block
-constexpr int KC = 16;             // panel rows per LDS chunk (4 MFMA k-steps)
+constexpr int KC = 16;             // panel rows per chunk of an LDS-resident contraction (4 k-steps)
 constexpr int LDB = BCOLS + 2;     // padded LDS row stride, doubles
-constexpr int STAGE_LDS = 4 * KC * LDB;   // doubles of the staging buffers (2 buffers x 2 panels)
+// k-steps per staged chunk of fold_gemm (with 8, the prefetch registers cost occupancy and the
+// kernels measured slower)
+__host__ __device__ constexpr int fold_chunk_ks(int) { return 4; }
+constexpr int STAGE_LDS = 4 * 4 * 4 * LDB;   // doubles of the staging buffers (2 buffers x 2 panels x 16 rows)
 
 struct FoldPair {
     double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
 };
 
-// acc0[i][ct] += A0[block b] x P0, acc1[i][ct] += A1[block b] x P1 for b = wave + 4i;
-// fold(kf) gives panel row kf for the staging thread's mode (thread t stages row t/16 of a
-// chunk for mode t%16), zero beyond K.  lds: STAGE_LDS doubles.
+// acc0[i][ct] += A0[block b] x P0, acc1[i][ct] += A1[block b] x P1 for b = wave + 4i.
+// Panel row kf for the staging thread's mode (thread t stages rows t/16 + 16 r of a chunk for
+// mode t%16) is finish(kf, fetch(kf)), zero beyond K: fetch issues the memory reads (branch
+// free), finish does the arithmetic, so the next chunk's reads are in flight during the
+// current chunk's MFMA chain.  lds: STAGE_LDS doubles.
 // With op.extra, xacc accumulates the staging thread's share of the dense extra row times
 // panel 0 (reduce_extra sums the shares of a column).
-template <int RBW, class FOLD>
-__device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FOLD&& fold,
-                                          double* lds, double2& xacc) {
+template <int RBW, class FETCH, class FINISH>
+__device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FETCH&& fetch,
+                                          FINISH&& finish, double* lds, double2& xacc) {
+    using Raw = decltype(fetch(0));
+    constexpr int KS = fold_chunk_ks(RBW);   // k-steps per chunk
+    constexpr int KR = 4 * KS;               // panel rows per chunk
+    constexpr int RPT = KR / 16;             // rows staged per thread
     const double* xrow = fold_xrow(op);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, cl = lane & 15;
     const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
-    const int nch = (op.ks + 3) / 4;
+    const int nch = (op.ks + KS - 1) / KS;
     const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
     const double* a0p[RBW];
     const double* a1p[RBW];
@@ -47,68 +56,80 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
         a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
     }
-    double ar0[RBW][4], ar1[RBW][4], an0[RBW][4], an1[RBW][4];
-    auto load = [&](double (&d0)[RBW][4], double (&d1)[RBW][4], int ch) {
+    // A operands of the current chunk; each is refilled with the next chunk's k-step as soon as
+    // its MFMAs are issued (a rolling one-chunk prefetch in KS registers per group)
+    double ar0[RBW][KS], ar1[RBW][KS];
 #pragma unroll
-        for (int i = 0; i < RBW; i++) {
+    for (int i = 0; i < RBW; i++)
 #pragma unroll
-            for (int sl = 0; sl < 4; sl++) {
-                d0[i][sl] = a0p[i][(ch * 4 + sl) * 64];
-                d1[i][sl] = a1p[i][(ch * 4 + sl) * 64];
-            }
+        for (int sl = 0; sl < KS; sl++) {
+            ar0[i][sl] = a0p[i][sl * 64];
+            ar1[i][sl] = a1p[i][sl * 64];
         }
+    // the extra-row coefficient of a staged row is read with the row's fold loads (a read at
+    // staging time would wait for every load issued before it, the prefetches included)
+    auto stage = [&](double* buf, const FoldPair& v, int row, double c) {
+        xacc.x += c * v.p0.x;
+        xacc.y += c * v.p0.y;
+        buf[row * LDB + 2 * mt] = v.p0.x;
+        buf[row * LDB + 2 * mt + 1] = v.p0.y;
+        buf[KR * LDB + row * LDB + 2 * mt] = v.p1.x;
+        buf[KR * LDB + row * LDB + 2 * mt + 1] = v.p1.y;
     };
-    auto stage = [&](double* buf, const FoldPair& v, int kf) {
-        if (op.extra) {
-            const double c = xrow[kf];
-            xacc.x += c * v.p0.x;
-            xacc.y += c * v.p0.y;
-        }
-        buf[rt * LDB + 2 * mt] = v.p0.x;
-        buf[rt * LDB + 2 * mt + 1] = v.p0.y;
-        buf[KC * LDB + rt * LDB + 2 * mt] = v.p1.x;
-        buf[KC * LDB + rt * LDB + 2 * mt + 1] = v.p1.y;
-    };
-    load(ar0, ar1, 0);
-    stage(lds, fold(rt), rt);
+    auto xcoef = [&](int kf) { return op.extra ? xrow[kf] : 0.0; };
+    {
+        Raw r0[RPT];
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) r0[rr] = fetch(rt + 16 * rr);
+#pragma unroll
+        for (int rr = 0; rr < RPT; rr++) stage(lds, finish(rt + 16 * rr, r0[rr]), rt + 16 * rr, xcoef(rt + 16 * rr));
+    }
     __syncthreads();
     for (int ch = 0; ch < nch; ch++) {
-        const double* cur = lds + (ch & 1) * 2 * KC * LDB;
+        const double* cur = lds + (ch & 1) * 2 * KR * LDB;
         const bool more = ch + 1 < nch;
-        FoldPair nxt;
+        Raw nraw[RPT];
+        double nxc[RPT];
         if (more) {
-            nxt = fold((ch + 1) * KC + rt);
-            load(an0, an1, ch + 1);
+#pragma unroll
+            for (int rr = 0; rr < RPT; rr++) {
+                nraw[rr] = fetch((ch + 1) * KR + rt + 16 * rr);
+                nxc[rr] = xcoef((ch + 1) * KR + rt + 16 * rr);
+            }
         }
+        // the next chunk's A operands (k-steps past the block's ksp are never read: ksp is a
+        // multiple of 8 >= nch KS)
+        const int nxa = (more ? ch + 1 : ch) * KS;
         // straight-line MFMA chain: absent blocks have zero A operands, the k-steps past ks zero A
         // columns and zero panel rows; a wave without blocks skips it (scalar branch)
         if (active) {
 #pragma unroll
-        for (int sl = 0; sl < 4; sl++) {
+        for (int sl = 0; sl < KS; sl++) {
             double b0[CT], b1[CT];
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
                 b0[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
-                b1[ct] = cur[KC * LDB + (sl * 4 + kk) * LDB + ct * 16 + cl];
+                b1[ct] = cur[KR * LDB + (sl * 4 + kk) * LDB + ct * 16 + cl];
             }
 #pragma unroll
-            for (int i = 0; i < RBW; i++)
+            for (int i = 0; i < RBW; i++) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++) {
                     acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
                     acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
                 }
+                ar0[i][sl] = a0p[i][(nxa + sl) * 64];
+                ar1[i][sl] = a1p[i][(nxa + sl) * 64];
+            }
         }
         }
         if (more) {
-            stage(lds + ((ch + 1) & 1) * 2 * KC * LDB, nxt, (ch + 1) * KC + rt);
+            double* nb = lds + ((ch + 1) & 1) * 2 * KR * LDB;
 #pragma unroll
-            for (int i = 0; i < RBW; i++)
-#pragma unroll
-                for (int sl = 0; sl < 4; sl++) {
-                    ar0[i][sl] = an0[i][sl];
-                    ar1[i][sl] = an1[i][sl];
-                }
+            for (int rr = 0; rr < RPT; rr++) {
+                const int kf = (ch + 1) * KR + rt + 16 * rr;
+                stage(nb, finish(kf, nraw[rr]), rt + 16 * rr, nxc[rr]);
+            }
         }
         __syncthreads();
     }

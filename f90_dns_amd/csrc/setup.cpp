@@ -237,7 +237,9 @@ FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector
     FoldedA f;
     f.rows0 = rows0; f.rows1 = rows1; f.K = K;
     f.rb0 = (rows0 + 15) / 16; f.rb1 = (rows1 + 15) / 16;
-    f.ks = (K + 3) / 4; f.ksp = (f.ks + 4) / 4 * 4;   // >= ks + 1 (the prefetch of k-step ks stays in the block)
+    // ksp: a multiple of 8 >= ks + 1, so the chunks of 8 (or 4) k-steps of the contractions stay
+    // inside the block
+    f.ks = (K + 3) / 4; f.ksp = (f.ks + 8) / 8 * 8;
     // a group 0 of 16 k + 1 rows keeps its last row out of the MFMA blocks: it is computed as a
     // dot product (stored after the blocks, 4 ksp entries, zero beyond K)
     f.extra = (rows0 % 16 == 1) ? 1 : 0;

@@ -6,6 +6,7 @@
 // Spectral fields are complex(nx/2+1, ny, nz), plane-major; seen as real matrices they are
 // nz rows x H2 = 2 (nx/2+1) ny columns, column c = 2*mode + part (part 0 = Re, 1 = Im).
 #include <algorithm>
+#include <type_traits>
 
 #include "contract.h"
 #include "kernels.h"
@@ -14,6 +15,10 @@ namespace dns {
 
 static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to 256 threads");
 static_assert(KC == 16, "a chunk is 16 panel rows");
+
+// waves per SIMD the contraction kernels are compiled for: three with one row-block pair per
+// wave (at most 168 VGPRs), two with two pairs
+#define Z_WAVES(rbw) ((rbw) == 1 ? 3 : 2)
 
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
 // the x Nyquist column gives an anti-Hermitian column (zero after the round trip), and so
@@ -43,9 +48,13 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
         const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : by;   // vort: v first, then u
         const double2* x2 = reinterpret_cast<const double2*>(a.in[f] + mo) + (sval ? smode : 0);
         // branch-free: mirrored rows (clamped in range), the centre row has na == nb
-        auto fold = [&](int kf) -> FoldPair {
+        struct Raw { double2 xa, xb; };
+        auto fetch = [&](int kf) -> Raw {
             const int na = min(kf, nz - 1), nb = max(nz - 1 - kf, 0);
-            const double2 xa = x2[na * rs], xb = x2[nb * rs];
+            return Raw{x2[na * rs], x2[nb * rs]};
+        };
+        auto finish = [&](int kf, const Raw& rw) -> FoldPair {
+            const double2 xa = rw.xa, xb = rw.xb;
             const double keep = (sval && kf < LE) ? 1.0 : 0.0;
             const double pair = kf < hF ? 1.0 : 0.0;   // 0 on the centre row
             FoldPair r;
@@ -78,7 +87,7 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
         zero_acc2(acc0);
         zero_acc2(acc1);
         double2 xacc = make_double2(0.0, 0.0);
-        fold_gemm<RBW>(acc0, acc1, a.d1, fold, lds, xacc);
+        fold_gemm<RBW>(acc0, acc1, a.d1, fetch, finish, lds, xacc);
         const double xs = a.d1.extra ? reduce_extra(xacc, lds) : 0.0;   // the centre row s_h
         if constexpr (LX) {
             // few row blocks: unfold through LDS so that every thread stores
@@ -121,12 +130,12 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
 }
 
 template <int RBW, int MODE, bool LX>
-__global__ __launch_bounds__(256) void k_zderiv(ZArgs a) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_zderiv(ZArgs a) {
     extern __shared__ double smem[];
     zderiv_tile<RBW, MODE, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, int MODE, bool LX>
-__global__ __launch_bounds__(256) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
@@ -172,23 +181,33 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const double2* u2 = reinterpret_cast<const double2*>(uh) + (sval ? smode : 0);
     const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.sh[comp]) + mo) + (sval ? smode : 0);
     const double2* sd2 = reinterpret_cast<const double2*>(a.sh[comp] + mo) + (sval ? smode : 0);
-    auto sload = [&](int node) -> double2 {
-        if (SF32) {
-            const float2 sf = sf2[node * rs];
-            return make_double2(double(sf.x), double(sf.y));
-        }
-        return sd2[node * rs];
-    };
     // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
     // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
     // two nodes coincide
-    auto fold = [&](int kf) -> FoldPair {
+    // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
+    // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
+    // two nodes coincide.  fetch reads u and S at both nodes (S unused on u rows).
+    using SV = typename std::conditional<SF32, float2, double2>::type;
+    const SV* sv2 = SF32 ? reinterpret_cast<const SV*>(sf2) : reinterpret_cast<const SV*>(sd2);
+    struct Raw { double2 ua, ub; SV sa, sb; };
+    auto nodes = [&](int kf, int& na, int& nb) {
+        const bool inv = kf < L1;
+        const int c = inv ? kf : kf - L1;
+        na = min(inv ? c + 1 : c, nz - 1);
+        nb = max(inv ? m - c : nz - 1 - c, 0);
+    };
+    auto fetch = [&](int kf) -> Raw {
+        int na, nb;
+        nodes(kf, na, nb);
+        return Raw{u2[na * rs], u2[nb * rs], sv2[na * rs], sv2[nb * rs]};
+    };
+    auto finish = [&](int kf, const Raw& rw) -> FoldPair {
         const bool inv = kf < L1;
         const int c = inv ? kf : kf - L1;
         const int half = inv ? hI : hF, lim = inv ? L1 : LE;
-        const int na = min(inv ? c + 1 : c, nz - 1), nb = max(inv ? m - c : nz - 1 - c, 0);
-        const double2 ua = u2[na * rs], ub = u2[nb * rs];
-        const double2 sa = sload(na), sb = sload(nb);
+        const double2 ua = rw.ua, ub = rw.ub;
+        const double2 sa = make_double2(double(rw.sa.x), double(rw.sa.y));
+        const double2 sb = make_double2(double(rw.sb.x), double(rw.sb.y));
         const double va_r = inv ? alpha * ua.x + a.dt * (sa.x + fr) : ua.x;
         const double va_i = inv ? alpha * ua.y + a.dt * sa.y : ua.y;
         const double vb_r = inv ? alpha * ub.x + a.dt * (sb.x + fr) : ub.x;
@@ -223,7 +242,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     zero_acc2(acc0);
     zero_acc2(acc1);
     double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW>(acc0, acc1, F, fold, smem, xacc);
+    fold_gemm<RBW>(acc0, acc1, F, fetch, finish, smem, xacc);
     const double xv = F.extra ? reduce_extra(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.hB.ks);
     double* z0 = smem;
@@ -312,12 +331,12 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
 }
 
 template <int RBW, bool SF32, bool LX>
-__global__ __launch_bounds__(256) void k_helmholtz(HelmArgs a) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_helmholtz(HelmArgs a) {
     extern __shared__ double smem[];
     helmholtz_tile<RBW, SF32, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, bool SF32, bool LX>
-__global__ __launch_bounds__(256) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
@@ -356,16 +375,23 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     const double2* vs2 = reinterpret_cast<const double2*>(vs) + (sval ? smode : 0);
     const double2* ws2 = reinterpret_cast<const double2*>(ws) + (sval ? smode : 0);
     // branch-free: rows [0, LE) fold div (E: sum on panel 0, O: difference on panel 1), rows
-    // [LE, 2 LE) fold w* the other way round (w_O on panel 0, w_E on panel 1)
-    auto fold = [&](int kf) -> FoldPair {
+    // [LE, 2 LE) fold w* the other way round (w_O on panel 0, w_E on panel 1).  fetch reads
+    // (u*, v*) on div rows and (w*, w*) on w rows at both nodes.
+    struct Raw { double2 pa, pb, qa, qb; };
+    auto fetch = [&](int kf) -> Raw {
         const bool ind = kf < LE;
         const int c = ind ? kf : kf - LE;
         const int na = min(c, nz - 1), nb = max(nz - 1 - c, 0);
-        const double2 ua = us2[na * rs], ub = us2[nb * rs];
-        const double2 va = vs2[na * rs], vb = vs2[nb * rs];
-        const double2 wa = ws2[na * rs], wb = ws2[nb * rs];
-        const double xa_r = ind ? -skx * ua.y - sky * va.y : wa.x, xa_i = ind ? skx * ua.x + sky * va.x : wa.y;
-        const double xb_r = ind ? -skx * ub.y - sky * vb.y : wb.x, xb_i = ind ? skx * ub.x + sky * vb.x : wb.y;
+        const double2* p2 = ind ? us2 : ws2;
+        const double2* q2 = ind ? vs2 : ws2;
+        return Raw{p2[na * rs], p2[nb * rs], q2[na * rs], q2[nb * rs]};
+    };
+    auto finish = [&](int kf, const Raw& rw) -> FoldPair {
+        const bool ind = kf < LE;
+        const int c = ind ? kf : kf - LE;
+        const double2 ua = rw.pa, ub = rw.pb, va = rw.qa, vb = rw.qb;   // w rows: pa, pb = w*
+        const double xa_r = ind ? -skx * ua.y - sky * va.y : ua.x, xa_i = ind ? skx * ua.x + sky * va.x : ua.y;
+        const double xb_r = ind ? -skx * ub.y - sky * vb.y : ub.x, xb_i = ind ? skx * ub.x + sky * vb.x : ub.y;
         const double keep = (sval && c < LE) ? 1.0 : 0.0;
         const double pair = c < hF ? 1.0 : 0.0;
         const double er = keep * (xa_r + pair * xb_r), ei = keep * (xa_i + pair * xb_i);   // even fold
@@ -394,7 +420,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     zero_acc2(acc0);
     zero_acc2(acc1);
     double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW>(acc0, acc1, a.pF, fold, smem, xacc);
+    fold_gemm<RBW>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
     const double xv = a.pF.extra ? reduce_extra(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.pB.ks);
     double* z0 = smem;
@@ -583,12 +609,12 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
 }
 
 template <int RBW, bool LX>
-__global__ __launch_bounds__(256) void k_poisson(PoisArgs a) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_poisson(PoisArgs a) {
     extern __shared__ double smem[];
     poisson_tile<RBW, LX>(a, blockIdx.x, blockIdx.z, smem);
 }
 template <int RBW, bool LX>
-__global__ __launch_bounds__(256) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const PoisArgs& a = d[t.g];
