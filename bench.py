@@ -36,12 +36,21 @@ WORKLOADS = {
 }
 
 
+def fold_macs(rows, cols):
+    """Multiply-adds of an even/odd-folded rows x cols operator on one column: the even half
+    ceil(rows/2) x ceil(cols/2) plus the odd half floor(rows/2) x floor(cols/2)."""
+    return -(-rows // 2) * -(-cols // 2) + (rows // 2) * (cols // 2)
+
+
 def kernel_work(kw):
     """Algorithmic HBM bytes and flops per launch of each step kernel (FP64).
 
     Spectral field: 16 B x (nx/2+1) ny nz; physical field: 8 B x nx ny nz.  Each kernel is
-    charged one read of its inputs and one write of its outputs; flops count the useful
-    multiply-adds of the wall-normal contractions (2 per MAC), none for transforms.
+    charged one read of its inputs and one write of its outputs.  Flops count the useful
+    multiply-adds (2 flops each) of the folded wall-normal contractions as computed: D1 on two
+    fields; per component the Helmholtz forward [T | f T D2_I] (m x m on v, m x nz on u) and
+    back B (m x m); the Poisson forward [Tf | Tf D1] and back [Bf ; D1 Bf] (four nz x nz).
+    Transforms count none.
     """
     nx, ny, nz = kw["nx"], kw["ny"], kw["nz"]
     H2 = 2 * (nx // 2 + 1) * ny
@@ -49,12 +58,12 @@ def kernel_work(kw):
     P = 8.0 * nx * ny * nz       # bytes of one physical field
     m = nz - 2
     return {
-        "zderiv_vort": dict(bytes=5 * S, flops=2 * 2.0 * nz * nz * H2),
+        "zderiv_vort": dict(bytes=5 * S, flops=2.0 * 2 * fold_macs(nz, nz) * H2),
         "c2r_nl": dict(bytes=5 * S + 6 * P, flops=0.0),
         "r2c_cross": dict(bytes=6 * P + 3 * S, flops=0.0),
         "flow_control": dict(bytes=16.0 * nz, flops=0.0),
-        "helmholtz": dict(bytes=9 * S, flops=3 * 2.0 * (m * (m + nz) + m * m) * H2),
-        "poisson_fused": dict(bytes=8 * S, flops=2.0 * (nz * 2 * nz + 2 * nz * nz) * H2),
+        "helmholtz": dict(bytes=9 * S, flops=2.0 * 3 * (2 * fold_macs(m, m) + fold_macs(m, nz)) * H2),
+        "poisson_fused": dict(bytes=8 * S, flops=2.0 * 4 * fold_macs(nz, nz) * H2),
         "poisson_mode0": dict(bytes=0.0, flops=0.0),
     }
 
