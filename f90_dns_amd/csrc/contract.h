@@ -214,11 +214,20 @@ __device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds)
 }
 
 // Column t < BCOLS of the extra row applied to an LDS-resident panel (rows 4 ks, stride LDB).
+// The coefficient reads go out eight at a time (the sum keeps its order).
 __device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) {
     const double* x = fold_xrow(op);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, n = 4 * op.ks;
     double v = 0.0;
-    for (int j = 0; j < 4 * op.ks; j++) v += x[j] * z0[j * LDB + t];
+    int j = 0;
+    for (; j + 8 <= n; j += 8) {
+        double c[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) c[u] = x[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v += c[u] * z0[(j + u) * LDB + t];
+    }
+    for (; j < n; j++) v += x[j] * z0[j * LDB + t];
     return v;
 }
 

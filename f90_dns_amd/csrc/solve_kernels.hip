@@ -107,23 +107,57 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             const int col = cb * BCOLS + int(threadIdx.x);
             if (threadIdx.x < BCOLS && col < H2) emit(hF, col, xs);
         }
+        // per tile and row-block pair: the vorticity's w reads for all of the lane's nodes go out
+        // together (indices clamped into range), then the results are combined and stored
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
-            const int col = out_col(ct, cb);
-            if (col >= H2) continue;
+            const int col = out_col(ct, cb), colc = min(col, H2 - 1);
+            const int mode = colc >> 1, part = colc & 1, j = mode / a.nxh, i = mode - j * a.nxh;
+            const double kk = MODE == Z_VORT ? (f == 1 ? a.yw[j] : a.xw[i]) : 0.0;
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
                 const int b = wave + 4 * ii;
-                if (b >= a.d1.rb0) continue;
+                double yv[4][2], wc[4][2], wo[4][2];
+                size_t qn[4][2];
+                bool ok[4][2];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int row = out_row(ii, r);
-                    if (row >= LE) continue;
                     const double sv = acc0[ii][ct][r];
                     const double rv = (row < hF && b < a.d1.rb1) ? acc1[ii][ct][r] : 0.0;
-                    emit(row, col, sv + rv);
-                    if (row < hF) emit(nz - 1 - row, col, sv - rv);
+                    yv[r][0] = sv + rv;
+                    yv[r][1] = sv - rv;
+#pragma unroll
+                    for (int side = 0; side < 2; side++) {
+                        const int node = max(min(side == 0 ? row : nz - 1 - row, nz - 1), 0);
+                        qn[r][side] = size_t(node) * H2;
+                        ok[r][side] = col < H2 && b < a.d1.rb0 && row < LE && (side == 0 || row < hF);
+                        if (MODE == Z_VORT) {
+                            wc[r][side] = w_in[qn[r][side] + colc];
+                            wo[r][side] = w_in[qn[r][side] + (colc ^ 1)];   // the other part of w
+                        }
+                    }
                 }
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int side = 0; side < 2; side++) {
+                        if (!ok[r][side]) continue;
+                        const size_t q = mo + qn[r][side] + col;
+                        const double y = yv[r][side];
+                        if (MODE == Z_PLAIN) {
+                            a.out[f][q] = y;
+                            continue;
+                        }
+                        const double wr = part == 0 ? wc[r][side] : wo[r][side];
+                        const double wi = part == 0 ? wo[r][side] : wc[r][side];
+                        double res;
+                        if (f == 1) res = part == 0 ? (-kk * wi - y) : (kk * wr - y);   // omega_x = i ky w - D1 v
+                        else res = part == 0 ? (y + kk * wi) : (y - kk * wr);          // omega_y = D1 u - i kx w
+                        const int o = f == 1 ? 0 : 1;
+                        if (a.out_f32) reinterpret_cast<float*>(a.out[o])[q] = float(res);
+                        else a.out[o][q] = res;
+                    }
             }
         }
     }
@@ -219,24 +253,21 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         r.p1 = make_double2(keep * pair * (va_r - vb_r), keep * pair * (va_i - vb_i));
         return r;
     };
-    // per-lane column constants for the two 16-column tiles
+    // per-lane column constants for the two 16-column tiles (reads clamped into range and
+    // issued together; a column past the plane gets zeros)
     double k2c[CT], bc1[CT], bc2[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct, cb);
-        k2c[ct] = 0.0; bc1[ct] = 0.0; bc2[ct] = 0.0;
-        if (col < H2) {
-            const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
-            const double kx = a.xw[i], ky = a.yw[j];
-            k2c[ct] = kx * kx + ky * ky;
-            if (comp < 2) {
-                const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
-                const double2 p0 = reinterpret_cast<const double2*>(ph)[mode];
-                const double2 p1 = reinterpret_cast<const double2*>(ph + size_t(nz - 1) * H2)[mode];
-                bc1[ct] = part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x);    // -dt * (i ke p)
-                bc2[ct] = part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x);
-            }
-        }
+        const int col = out_col(ct, cb), colc = min(col, H2 - 1);
+        const int mode = colc >> 1, part = colc & 1, j = mode / a.nxh, i = mode - j * a.nxh;
+        const double kx = a.xw[i], ky = a.yw[j];
+        const double2 p0 = reinterpret_cast<const double2*>(ph)[mode];
+        const double2 p1 = reinterpret_cast<const double2*>(ph + size_t(nz - 1) * H2)[mode];
+        const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
+        const bool in = col < H2, lift = in && comp < 2;
+        k2c[ct] = in ? kx * kx + ky * ky : 0.0;
+        bc1[ct] = lift ? (part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x)) : 0.0;   // -dt * (i ke p)
+        bc2[ct] = lift ? (part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x)) : 0.0;
     }
     d4 acc0[RBW][CT], acc1[RBW][CT];
     zero_acc2(acc0);
@@ -253,22 +284,33 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         const int row = F.rows0 - 1, ct = threadIdx.x >> 4;
         z0[row * LDB + threadIdx.x] = (xv + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
     }
+    {
+        // the lifting sums and eigenvalues of the lane's rows, read together (clamped) first
+        double e1[2][RBW][4], e2[2][RBW][4], el[2][RBW][4];
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-        const bool cv = out_col(ct, cb) < H2;
-#pragma unroll
-        for (int ii = 0; ii < RBW; ii++) {
-            const int b = wave + 4 * ii;
+        for (int ii = 0; ii < RBW; ii++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                if (cv && b < F.rb0 && row < F.rows0)
-                    z0[row * LDB + ct * 16 + cl] =
-                        (acc0[ii][ct][r] + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
-                if (cv && b < F.rb1 && row < F.rows1) {
-                    const int g = F.rows0 + row;
-                    z1[row * LDB + ct * 16 + cl] =
-                        (acc1[ii][ct][r] + c1v[g] * bc1[ct] + c2v[g] * bc2[ct]) / (hlv[g] + fco * k2c[ct]);
+                const int g0 = min(row, F.rows0 - 1), g1 = F.rows0 + min(row, F.rows1 - 1);
+                e1[0][ii][r] = c1v[g0]; e2[0][ii][r] = c2v[g0]; el[0][ii][r] = hlv[g0];
+                e1[1][ii][r] = c1v[g1]; e2[1][ii][r] = c2v[g1]; el[1][ii][r] = hlv[g1];
+            }
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            const bool cv = out_col(ct, cb) < H2;
+#pragma unroll
+            for (int ii = 0; ii < RBW; ii++) {
+                const int b = wave + 4 * ii;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = out_row(ii, r);
+                    if (cv && b < F.rb0 && row < F.rows0)
+                        z0[row * LDB + ct * 16 + cl] = (acc0[ii][ct][r] + e1[0][ii][r] * bc1[ct] + e2[0][ii][r] * bc2[ct]) /
+                                                       (el[0][ii][r] + fco * k2c[ct]);
+                    if (cv && b < F.rb1 && row < F.rows1)
+                        z1[row * LDB + ct * 16 + cl] = (acc1[ii][ct][r] + e1[1][ii][r] * bc1[ct] + e2[1][ii][r] * bc2[ct]) /
+                                                       (el[1][ii][r] + fco * k2c[ct]);
                 }
             }
         }
@@ -431,18 +473,30 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         const int row = a.pF.rows0 - 1, ct = threadIdx.x >> 4;
         if (live[ct]) z0[row * LDB + threadIdx.x] = -xv / (a.dt * (a.pl[row] + k2c[ct]));
     }
+    {
+        // eigenvalues of the lane's rows, read together (clamped) before the stores
+        double l0[RBW][4], l1[RBW][4];
 #pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-#pragma unroll
-        for (int ii = 0; ii < RBW; ii++) {
-            const int b = wave + 4 * ii;
+        for (int ii = 0; ii < RBW; ii++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                if (live[ct] && b < a.pF.rb0 && row < a.pF.rows0)
-                    z0[row * LDB + ct * 16 + cl] = -acc0[ii][ct][r] / (a.dt * (a.pl[row] + k2c[ct]));
-                if (live[ct] && b < a.pF.rb1 && row < a.pF.rows1)
-                    z1[row * LDB + ct * 16 + cl] = -acc1[ii][ct][r] / (a.dt * (a.pl[a.pF.rows0 + row] + k2c[ct]));
+                l0[ii][r] = a.pl[min(row, a.pF.rows0 - 1)];
+                l1[ii][r] = a.pl[a.pF.rows0 + min(row, a.pF.rows1 - 1)];
+            }
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+#pragma unroll
+            for (int ii = 0; ii < RBW; ii++) {
+                const int b = wave + 4 * ii;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int row = out_row(ii, r);
+                    if (live[ct] && b < a.pF.rb0 && row < a.pF.rows0)
+                        z0[row * LDB + ct * 16 + cl] = -acc0[ii][ct][r] / (a.dt * (l0[ii][r] + k2c[ct]));
+                    if (live[ct] && b < a.pF.rb1 && row < a.pF.rows1)
+                        z1[row * LDB + ct * 16 + cl] = -acc1[ii][ct][r] / (a.dt * (l1[ii][r] + k2c[ct]));
+                }
             }
         }
     }
@@ -504,42 +558,69 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         }
         return;
     }
+    // phi epilogue, per 16-column tile: the outputs' old values (u*, v*, p) are read for all of
+    // the lane's nodes first (indices clamped into range), then combined and stored
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct, cb);
+        const int col = out_col(ct, cb), colc = min(col, H2 - 1);
         const int part = col & 1;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
             const int b = wave + 4 * ii;
-            if (b >= a.pB.rb0) continue;
+            double pv[4][2], ov[4][2];
+            size_t qn[4][2];
+            bool ok[4][2];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
                 const double p = acc0[ii][ct][r];
                 const double q = (row < hF && b < a.pB.rb1) ? acc1[ii][ct][r] : 0.0;
-                const double pha = p + q, phb = p - q;
-                const double oa = __shfl_xor(pha, 1), ob = __shfl_xor(phb, 1);   // the other part of the mode
-                if (!live[ct] || row >= LE) continue;
+                pv[r][0] = p + q;
+                pv[r][1] = p - q;
+                ov[r][0] = __shfl_xor(pv[r][0], 1);   // the other part of the mode
+                ov[r][1] = __shfl_xor(pv[r][1], 1);
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
-                    if (side == 1 && row >= hF) continue;
+                    const int node = min(side == 0 ? row : nz - 1 - row, nz - 1);
+                    qn[r][side] = size_t(max(node, 0)) * H2 + colc;
+                    ok[r][side] = b < a.pB.rb0 && live[ct] && row < LE && (side == 0 || row < hF);
+                }
+            }
+            if (!a.fused) {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int side = 0; side < 2; side++)
+                        if (ok[r][side]) phio[qn[r][side]] = pv[r][side];
+                continue;
+            }
+            double uu[4][2], vv[4][2], pp[4][2];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int side = 0; side < 2; side++) {
+                    uu[r][side] = us[qn[r][side]];
+                    vv[r][side] = vs[qn[r][side]];
+                    pp[r][side] = po[qn[r][side]];
+                }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int side = 0; side < 2; side++) {
+                    if (!ok[r][side]) continue;
+                    const int row = out_row(ii, r);
                     const int node = side == 0 ? row : nz - 1 - row;
-                    const double ph = side == 0 ? pha : phb, other = side == 0 ? oa : ob;
-                    const size_t qn = size_t(node) * H2 + col;
-                    if (!a.fused) {
-                        phio[qn] = ph;
-                        continue;
-                    }
+                    const double ph = pv[r][side], other = ov[r][side];
                     const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
                     const bool wall = node == 0 || node == nz - 1;
                     // u - dt*(i k phi): Re = u.re - dt*(-k phi.im), Im = u.im - dt*(k phi.re)
                     const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
                     const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
-                    uo[qn] = wall ? 0.0 : us[qn] - a.dt * du;
-                    vo[qn] = wall ? 0.0 : vs[qn] - a.dt * dv;
-                    po[qn] = po[qn] + ph;
+                    const size_t q = qn[r][side];
+                    uo[q] = wall ? 0.0 : uu[r][side] - a.dt * du;
+                    vo[q] = wall ? 0.0 : vv[r][side] - a.dt * dv;
+                    po[q] = pp[r][side] + ph;
                 }
-            }
         }
     }
     if (a.pB.extra && wave == 0) {
@@ -569,31 +650,43 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     fold_gemm_lds<RBW>(acc0, acc1, a.pDB, z1, z0);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct, cb);
+        const int col = out_col(ct, cb), colc = min(col, H2 - 1);
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
             const int b = wave + 4 * ii;
-            if (b >= a.pDB.rb0) continue;
+            double dv[4][2], wv[4][2];
+            size_t qn[4][2];
+            bool ok[4][2];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row(ii, r);
-                if (!live[ct] || row >= LE) continue;
-                const double s = acc0[ii][ct][r];
+                const double sv = acc0[ii][ct][r];
                 const double rv = (row < hF && b < a.pDB.rb1) ? acc1[ii][ct][r] : 0.0;
+                dv[r][0] = sv + rv;
+                dv[r][1] = sv - rv;
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
-                    if (side == 1 && row >= hF) continue;
+                    const int node = min(side == 0 ? row : nz - 1 - row, nz - 1);
+                    qn[r][side] = size_t(max(node, 0)) * H2 + colc;
+                    ok[r][side] = b < a.pDB.rb0 && live[ct] && row < LE && (side == 0 || row < hF);
+                    wv[r][side] = a.fused ? ws[qn[r][side]] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int side = 0; side < 2; side++) {
+                    if (!ok[r][side]) continue;
+                    const int row = out_row(ii, r);
                     const int node = side == 0 ? row : nz - 1 - row;
-                    const double dph = side == 0 ? s + rv : s - rv;
-                    const size_t qn = size_t(node) * H2 + col;
+                    const size_t q = qn[r][side];
                     if (!a.fused) {
-                        dphio[qn] = dph;
+                        dphio[q] = dv[r][side];
                         continue;
                     }
                     const bool wall = node == 0 || node == nz - 1;
-                    wo[qn] = wall ? 0.0 : ws[qn] - a.dt * dph;
+                    wo[q] = wall ? 0.0 : wv[r][side] - a.dt * dv[r][side];
                 }
-            }
         }
     }
     if (a.pDB.extra && threadIdx.x < BCOLS) {
