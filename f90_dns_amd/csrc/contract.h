@@ -43,11 +43,11 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     constexpr int KR = 4 * KS;               // panel rows per chunk
     constexpr int RPT = KR / 16;             // rows staged per thread
     const double* xrow = fold_xrow(op);
+    const int xlast = 4 * op.ksp - 1;        // the extra row's stored entries (zero beyond K)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, cl = lane & 15;
     const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
     const int nch = (op.ks + KS - 1) / KS;
-    const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
     const double* a0p[RBW];
     const double* a1p[RBW];
 #pragma unroll
@@ -66,8 +66,8 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
             ar0[i][sl] = a0p[i][sl * 64];
             ar1[i][sl] = a1p[i][sl * 64];
         }
-    // the extra-row coefficient of a staged row is read with the row's fold loads (a read at
-    // staging time would wait for every load issued before it, the prefetches included)
+    // the extra-row coefficient of a staged row (clamped: rows past K have zero panel entries)
+    auto xcoef = [&](int kf) { return op.extra ? xrow[min(kf, xlast)] : 0.0; };
     auto stage = [&](double* buf, const FoldPair& v, int row, double c) {
         xacc.x += c * v.p0.x;
         xacc.y += c * v.p0.y;
@@ -76,33 +76,38 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         buf[KR * LDB + row * LDB + 2 * mt] = v.p1.x;
         buf[KR * LDB + row * LDB + 2 * mt + 1] = v.p1.y;
     };
-    auto xcoef = [&](int kf) { return op.extra ? xrow[kf] : 0.0; };
+    // Panel reads run two chunks ahead of the MFMAs and the arithmetic of the next chunk's fold
+    // runs beside the current chunk's MFMA chain: the loop body is one straight-line block (a
+    // chunk past the end folds rows past K, which are zero, into the idle buffer; every wave
+    // issues the chain, absent row blocks having zero A operands).
+    Raw r1[RPT];
+    double c1[RPT];
     {
         Raw r0[RPT];
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) r0[rr] = fetch(rt + 16 * rr);
+        for (int rr = 0; rr < RPT; rr++) {
+            r0[rr] = fetch(rt + 16 * rr);
+            r1[rr] = fetch(KR + rt + 16 * rr);
+            c1[rr] = xcoef(KR + rt + 16 * rr);
+        }
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) stage(lds, finish(rt + 16 * rr, r0[rr]), rt + 16 * rr, xcoef(rt + 16 * rr));
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ch++) {
         const double* cur = lds + (ch & 1) * 2 * KR * LDB;
-        const bool more = ch + 1 < nch;
-        Raw nraw[RPT];
-        double nxc[RPT];
-        if (more) {
+        double* nb = lds + ((ch + 1) & 1) * 2 * KR * LDB;
+        Raw r2[RPT];
+        double c2[RPT];
+        FoldPair f1[RPT];
 #pragma unroll
-            for (int rr = 0; rr < RPT; rr++) {
-                nraw[rr] = fetch((ch + 1) * KR + rt + 16 * rr);
-                nxc[rr] = xcoef((ch + 1) * KR + rt + 16 * rr);
-            }
+        for (int rr = 0; rr < RPT; rr++) {
+            const int kf2 = (ch + 2) * KR + rt + 16 * rr;
+            r2[rr] = fetch(kf2);
+            c2[rr] = xcoef(kf2);
+            f1[rr] = finish((ch + 1) * KR + rt + 16 * rr, r1[rr]);
         }
-        // the next chunk's A operands (k-steps past the block's ksp are never read: ksp is a
-        // multiple of 8 >= nch KS)
-        const int nxa = (more ? ch + 1 : ch) * KS;
-        // straight-line MFMA chain: absent blocks have zero A operands, the k-steps past ks zero A
-        // columns and zero panel rows; a wave without blocks skips it (scalar branch)
-        if (active) {
+        const int nxa = min(ch + 1, nch - 1) * KS;   // the next chunk's A operands (the last chunk reloads its own)
 #pragma unroll
         for (int sl = 0; sl < KS; sl++) {
             double b0[CT], b1[CT];
@@ -122,14 +127,11 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
                 ar1[i][sl] = a1p[i][(nxa + sl) * 64];
             }
         }
-        }
-        if (more) {
-            double* nb = lds + ((ch + 1) & 1) * 2 * KR * LDB;
 #pragma unroll
-            for (int rr = 0; rr < RPT; rr++) {
-                const int kf = (ch + 1) * KR + rt + 16 * rr;
-                stage(nb, finish(kf, nraw[rr]), rt + 16 * rr, nxc[rr]);
-            }
+        for (int rr = 0; rr < RPT; rr++) {
+            stage(nb, f1[rr], rt + 16 * rr, c1[rr]);
+            r1[rr] = r2[rr];
+            c1[rr] = c2[rr];
         }
         __syncthreads();
     }
