@@ -22,6 +22,10 @@ constexpr int LDB = BCOLS + 2;     // padded LDS row stride, doubles
 // k-steps per staged chunk of fold_gemm (with 8, the prefetch registers cost occupancy and the
 // kernels measured slower)
 __host__ __device__ constexpr int fold_chunk_ks(int) { return 4; }
+// chunks by which the panel reads of fold_gemm run ahead of the MFMAs
+#ifndef FOLD_PREFETCH
+#define FOLD_PREFETCH 1
+#endif
 constexpr int STAGE_LDS = 4 * 4 * 4 * LDB;   // doubles of the staging buffers (2 buffers x 2 panels x 16 rows)
 
 struct FoldPair {
@@ -76,19 +80,24 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         buf[KR * LDB + row * LDB + 2 * mt] = v.p1.x;
         buf[KR * LDB + row * LDB + 2 * mt + 1] = v.p1.y;
     };
-    // Panel reads run two chunks ahead of the MFMAs and the arithmetic of the next chunk's fold
+    // Panel reads run PD chunks ahead of the MFMAs and the arithmetic of the next chunk's fold
     // runs beside the current chunk's MFMA chain: the loop body is one straight-line block (a
     // chunk past the end folds rows past K, which are zero, into the idle buffer; every wave
-    // issues the chain, absent row blocks having zero A operands).
-    Raw r1[RPT];
-    double c1[RPT];
+    // issues the chain, absent row blocks having zero A operands).  q[d] holds the reads of
+    // chunk ch + 1 + d.
+    constexpr int PD = FOLD_PREFETCH;
+    Raw q[PD][RPT];
+    double qc[PD][RPT];
     {
         Raw r0[RPT];
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
             r0[rr] = fetch(rt + 16 * rr);
-            r1[rr] = fetch(KR + rt + 16 * rr);
-            c1[rr] = xcoef(KR + rt + 16 * rr);
+#pragma unroll
+            for (int d = 0; d < PD; d++) {
+                q[d][rr] = fetch((1 + d) * KR + rt + 16 * rr);
+                qc[d][rr] = xcoef((1 + d) * KR + rt + 16 * rr);
+            }
         }
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) stage(lds, finish(rt + 16 * rr, r0[rr]), rt + 16 * rr, xcoef(rt + 16 * rr));
@@ -102,10 +111,10 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         FoldPair f1[RPT];
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
-            const int kf2 = (ch + 2) * KR + rt + 16 * rr;
+            const int kf2 = (ch + 1 + PD) * KR + rt + 16 * rr;
             r2[rr] = fetch(kf2);
             c2[rr] = xcoef(kf2);
-            f1[rr] = finish((ch + 1) * KR + rt + 16 * rr, r1[rr]);
+            f1[rr] = finish((ch + 1) * KR + rt + 16 * rr, q[0][rr]);
         }
         const int nxa = min(ch + 1, nch - 1) * KS;   // the next chunk's A operands (the last chunk reloads its own)
 #pragma unroll
@@ -129,9 +138,14 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         }
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
-            stage(nb, f1[rr], rt + 16 * rr, c1[rr]);
-            r1[rr] = r2[rr];
-            c1[rr] = c2[rr];
+            stage(nb, f1[rr], rt + 16 * rr, qc[0][rr]);
+#pragma unroll
+            for (int d = 0; d + 1 < PD; d++) {
+                q[d][rr] = q[d + 1][rr];
+                qc[d][rr] = qc[d + 1][rr];
+            }
+            q[PD - 1][rr] = r2[rr];
+            qc[PD - 1][rr] = c2[rr];
         }
         __syncthreads();
     }
@@ -277,6 +291,12 @@ __device__ __forceinline__ int out_row(int i, int r) {
     return (wave + 4 * i) * 16 + (lane >> 4) + 4 * r;
 }
 __device__ __forceinline__ int out_col(int ct, int cb) { return cb * BCOLS + ct * 16 + (threadIdx.x & 15); }
+
+// zero rows [rows, zr) of an LDS panel (the k rows past an operator's K that a chunked
+// contraction still reads; its A columns there are zero, so the rows must be finite)
+__device__ __forceinline__ void zero_panel_tail(double* z, int rows, int zr) {
+    for (int e = threadIdx.x; e < (zr - rows) * BCOLS; e += 256) z[(rows + e / BCOLS) * LDB + e % BCOLS] = 0.0;
+}
 
 // panel rows of the LDS-resident z panels of a back contraction with k-steps ks: whole chunks
 // of 16 rows (the chunked contraction reads them all; rows past 4 ks are zero)

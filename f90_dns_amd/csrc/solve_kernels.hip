@@ -17,8 +17,11 @@ static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to 256 threads")
 static_assert(KC == 16, "a chunk is 16 panel rows");
 
 // waves per SIMD the contraction kernels are compiled for: three with one row-block pair per
-// wave (at most 168 VGPRs), two with two pairs
-#define Z_WAVES(rbw) ((rbw) == 1 ? 3 : 2)
+// wave (at most 168 VGPRs; four spilled and measured no faster), two with two pairs per wave
+#define ZW_D1 3
+#define ZW_HELM 3
+#define ZW_POIS 3
+#define Z_WAVES(rbw, w) ((rbw) == 1 ? (w) : 2)
 
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
 // the x Nyquist column gives an anti-Hermitian column (zero after the round trip), and so
@@ -164,12 +167,12 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
 }
 
 template <int RBW, int MODE, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_zderiv(ZArgs a) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_D1)) void k_zderiv(ZArgs a) {
     extern __shared__ double smem[];
     zderiv_tile<RBW, MODE, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, int MODE, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
@@ -278,8 +281,11 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const int ZR = zpanel_rows(a.hB.ks);
     double* z0 = smem;
     double* z1 = smem + ZR * LDB;
-    for (int e = threadIdx.x; e < 2 * ZR * LDB; e += 256) smem[e] = 0.0;
-    __syncthreads();
+    // the back contraction reads whole 16-row chunks: zero the panel rows past each group's rows
+    // (every other row of a live column is written below; the barrier before the back
+    // contraction orders these writes)
+    zero_panel_tail(z0, F.rows0, ZR);
+    zero_panel_tail(z1, F.rows1, ZR);
     if (F.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2) {
         const int row = F.rows0 - 1, ct = threadIdx.x >> 4;
         z0[row * LDB + threadIdx.x] = (xv + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
@@ -373,12 +379,12 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
 }
 
 template <int RBW, bool SF32, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_helmholtz(HelmArgs a) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz(HelmArgs a) {
     extern __shared__ double smem[];
     helmholtz_tile<RBW, SF32, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, bool SF32, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
@@ -467,8 +473,8 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     const int ZR = zpanel_rows(a.pB.ks);
     double* z0 = smem;
     double* z1 = smem + ZR * LDB;
-    for (int e = threadIdx.x; e < 2 * ZR * LDB; e += 256) smem[e] = 0.0;
-    __syncthreads();
+    zero_panel_tail(z0, a.pF.rows0, ZR);
+    zero_panel_tail(z1, a.pF.rows1, ZR);
     if (a.pF.extra && threadIdx.x < BCOLS) {
         const int row = a.pF.rows0 - 1, ct = threadIdx.x >> 4;
         if (live[ct]) z0[row * LDB + threadIdx.x] = -xv / (a.dt * (a.pl[row] + k2c[ct]));
@@ -702,12 +708,12 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
 }
 
 template <int RBW, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_poisson(PoisArgs a) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_POIS)) void k_poisson(PoisArgs a) {
     extern __shared__ double smem[];
     poisson_tile<RBW, LX>(a, blockIdx.x, blockIdx.z, smem);
 }
 template <int RBW, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW)) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
+__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_POIS)) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const PoisArgs& a = d[t.g];
