@@ -223,7 +223,8 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     // two nodes coincide
     // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
     // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
-    // two nodes coincide.  fetch reads u and S at both nodes (S unused on u rows).
+    // two nodes coincide.  fetch reads u and S at both nodes; on u rows, where S is not used, the
+    // S reads go to the u elements just read (the same cache lines, no HBM traffic).
     using SV = typename std::conditional<SF32, float2, double2>::type;
     const SV* sv2 = SF32 ? reinterpret_cast<const SV*>(sf2) : reinterpret_cast<const SV*>(sd2);
     struct Raw { double2 ua, ub; SV sa, sb; };
@@ -236,7 +237,10 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     auto fetch = [&](int kf) -> Raw {
         int na, nb;
         nodes(kf, na, nb);
-        return Raw{u2[na * rs], u2[nb * rs], sv2[na * rs], sv2[nb * rs]};
+        const bool inv = kf < L1;
+        const SV* sa = inv ? &sv2[na * rs] : reinterpret_cast<const SV*>(&u2[na * rs]);
+        const SV* sb = inv ? &sv2[nb * rs] : reinterpret_cast<const SV*>(&u2[nb * rs]);
+        return Raw{u2[na * rs], u2[nb * rs], *sa, *sb};
     };
     auto finish = [&](int kf, const Raw& rw) -> FoldPair {
         const bool inv = kf < L1;
