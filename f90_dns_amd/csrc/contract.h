@@ -39,7 +39,10 @@ struct FoldPair {
 // current chunk's MFMA chain.  lds: STAGE_LDS doubles.
 // With op.extra, xacc accumulates the staging thread's share of the dense extra row times
 // panel 0 (reduce_extra sums the shares of a column).
-template <int RBW, class FETCH, class FINISH>
+// SKIP: waves without row blocks skip the MFMA chain (operators with fewer row-block pairs
+// than waves, the small-nz variants); otherwise every wave has blocks and the loop body stays
+// one straight-line block.
+template <int RBW, bool SKIP, class FETCH, class FINISH>
 __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FETCH&& fetch,
                                           FINISH&& finish, double* lds, double2& xacc) {
     using Raw = decltype(fetch(0));
@@ -52,6 +55,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     const int kk = lane >> 4, cl = lane & 15;
     const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
     const int nch = (op.ks + KS - 1) / KS;
+    const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
     const double* a0p[RBW];
     const double* a1p[RBW];
 #pragma unroll
@@ -117,6 +121,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
             f1[rr] = finish((ch + 1) * KR + rt + 16 * rr, q[0][rr]);
         }
         const int nxa = min(ch + 1, nch - 1) * KS;   // the next chunk's A operands (the last chunk reloads its own)
+        if (!SKIP || active)
 #pragma unroll
         for (int sl = 0; sl < KS; sl++) {
             double b0[CT], b1[CT];

@@ -90,7 +90,7 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
         zero_acc2(acc0);
         zero_acc2(acc1);
         double2 xacc = make_double2(0.0, 0.0);
-        fold_gemm<RBW>(acc0, acc1, a.d1, fetch, finish, lds, xacc);
+        fold_gemm<RBW, LX>(acc0, acc1, a.d1, fetch, finish, lds, xacc);
         const double xs = a.d1.extra ? reduce_extra(xacc, lds) : 0.0;   // the centre row s_h
         if constexpr (LX) {
             // few row blocks: unfold through LDS so that every thread stores
@@ -99,9 +99,45 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             store_groups<RBW>(acc0, acc1, a.d1, G0, G1);
             if (a.d1.extra && threadIdx.x < BCOLS) G0[(a.d1.rows0 - 1) * LDB + threadIdx.x] = xs;
             __syncthreads();
-            for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
-                const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-                if (col < H2) emit(node, col, unfold_lds(G0, G1, nz, node, c));
+            // four outputs per thread per pass, their w and wavenumber reads issued first
+            // (indices clamped into range)
+            const int ne = nz * BCOLS;
+            for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+                double y[4], wc[4], wo[4], kk[4];
+                int nd[4], cl4[4];
+                bool ok[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int e = e0 + 256 * u, ec = min(e, ne - 1);
+                    const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c, colc = min(col, H2 - 1);
+                    const int mode = colc >> 1, j = mode / a.nxh, i = mode - j * a.nxh;
+                    ok[u] = e < ne && col < H2;
+                    nd[u] = node;
+                    cl4[u] = col;
+                    y[u] = unfold_lds(G0, G1, nz, node, c);
+                    if (MODE == Z_VORT) {
+                        wc[u] = w_in[size_t(node) * H2 + colc];
+                        wo[u] = w_in[size_t(node) * H2 + (colc ^ 1)];
+                        kk[u] = f == 1 ? a.yw[j] : a.xw[i];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (!ok[u]) continue;
+                    const size_t q = mo + size_t(nd[u]) * H2 + cl4[u];
+                    if (MODE == Z_PLAIN) {
+                        a.out[f][q] = y[u];
+                        continue;
+                    }
+                    const int part = cl4[u] & 1;
+                    const double wr = part == 0 ? wc[u] : wo[u], wi = part == 0 ? wo[u] : wc[u];
+                    double res;
+                    if (f == 1) res = part == 0 ? (-kk[u] * wi - y[u]) : (kk[u] * wr - y[u]);   // omega_x = i ky w - D1 v
+                    else res = part == 0 ? (y[u] + kk[u] * wi) : (y[u] - kk[u] * wr);          // omega_y = D1 u - i kx w
+                    const int o = f == 1 ? 0 : 1;
+                    if (a.out_f32) reinterpret_cast<float*>(a.out[o])[q] = float(res);
+                    else a.out[o][q] = res;
+                }
             }
             __syncthreads();
             continue;
@@ -280,7 +316,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     zero_acc2(acc0);
     zero_acc2(acc1);
     double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW>(acc0, acc1, F, fetch, finish, smem, xacc);
+    fold_gemm<RBW, LX>(acc0, acc1, F, fetch, finish, smem, xacc);
     const double xv = F.extra ? reduce_extra(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.hB.ks);
     double* z0 = smem;
@@ -472,7 +508,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     zero_acc2(acc0);
     zero_acc2(acc1);
     double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
+    fold_gemm<RBW, LX>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
     const double xv = a.pF.extra ? reduce_extra(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.pB.ks);
     double* z0 = smem;
@@ -532,23 +568,42 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         store_groups<RBW>(acc0, acc1, a.pB, G0, G1);
         if (a.pB.extra && threadIdx.x < BCOLS) G0[(a.pB.rows0 - 1) * LDB + threadIdx.x] = xph;
         __syncthreads();
-        for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
-            const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-            if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
-            const double ph = unfold_lds(G0, G1, nz, node, c), other = unfold_lds(G0, G1, nz, node, c ^ 1);
-            const size_t qn = size_t(node) * H2 + col;
-            if (!a.fused) {
-                phio[qn] = ph;
-                continue;
+        // four outputs per thread per pass, their reads of u*, v*, p issued first (clamped)
+        const int ne = nz * BCOLS;
+        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+            double uu[4], vv[4], pp[4];
+            size_t qn[4];
+            bool ok[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + 256 * u, ec = min(e, ne - 1);
+                const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c;
+                ok[u] = e < ne && col < H2 && cc[2 * BCOLS + c] != 0.0;
+                qn[u] = size_t(node) * H2 + min(col, H2 - 1);
+                if (a.fused) {
+                    uu[u] = us[qn[u]];
+                    vv[u] = vs[qn[u]];
+                    pp[u] = po[qn[u]];
+                }
             }
-            const int part = col & 1;
-            const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
-            const bool wall = node == 0 || node == nz - 1;
-            const double du = part == 0 ? -cc[c] * pim : cc[c] * pre;
-            const double dv = part == 0 ? -cc[BCOLS + c] * pim : cc[BCOLS + c] * pre;
-            uo[qn] = wall ? 0.0 : us[qn] - a.dt * du;
-            vo[qn] = wall ? 0.0 : vs[qn] - a.dt * dv;
-            po[qn] = po[qn] + ph;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (!ok[u]) continue;
+                const int e = e0 + 256 * u, node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+                const double ph = unfold_lds(G0, G1, nz, node, c), other = unfold_lds(G0, G1, nz, node, c ^ 1);
+                if (!a.fused) {
+                    phio[qn[u]] = ph;
+                    continue;
+                }
+                const int part = col & 1;
+                const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
+                const bool wall = node == 0 || node == nz - 1;
+                const double du = part == 0 ? -cc[c] * pim : cc[c] * pre;
+                const double dv = part == 0 ? -cc[BCOLS + c] * pim : cc[BCOLS + c] * pre;
+                uo[qn[u]] = wall ? 0.0 : uu[u] - a.dt * du;
+                vo[qn[u]] = wall ? 0.0 : vv[u] - a.dt * dv;
+                po[qn[u]] = pp[u] + ph;
+            }
         }
         __syncthreads();
         zero_acc2(acc0);
@@ -558,13 +613,25 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         store_groups<RBW>(acc0, acc1, a.pDB, G0, G1);
         if (a.pDB.extra && threadIdx.x < BCOLS) G0[(a.pDB.rows0 - 1) * LDB + threadIdx.x] = xd;
         __syncthreads();
-        for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
-            const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-            if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
-            const double dph = unfold_lds(G0, G1, nz, node, c);
-            const size_t qn = size_t(node) * H2 + col;
-            if (!a.fused) dphio[qn] = dph;
-            else wo[qn] = (node == 0 || node == nz - 1) ? 0.0 : ws[qn] - a.dt * dph;
+        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+            double ww[4];
+            size_t qn[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = min(e0 + 256 * u, ne - 1), node = e / BCOLS, c = e - node * BCOLS;
+                qn[u] = size_t(node) * H2 + min(cb * BCOLS + c, H2 - 1);
+                ww[u] = a.fused ? ws[qn[u]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = e0 + 256 * u;
+                if (e >= ne) continue;
+                const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+                if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
+                const double dph = unfold_lds(G0, G1, nz, node, c);
+                if (!a.fused) dphio[qn[u]] = dph;
+                else wo[qn[u]] = (node == 0 || node == nz - 1) ? 0.0 : ww[u] - a.dt * dph;
+            }
         }
         return;
     }
