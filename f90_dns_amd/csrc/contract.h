@@ -1,9 +1,10 @@
 // contract.h -- block-level FP64 MFMA contractions of the folded wall-normal operators over a
 // 32-column tile of spectral columns.
 //
-// A block of 4 wavefronts owns 32 real columns (16 complex modes).  A folded operator has two
+// A block of NW wavefronts (4; at small nz one per row-block pair) owns 32 real columns (16
+// complex modes).  A folded operator has two
 // row groups (setup.h FoldedA): group 0 acts on panel 0, group 1 on panel 1, both K rows.  Wave
-// w takes the row-block pairs b = w, w + 4, ...: group-0 block b and group-1 block b, so the two
+// w takes the row-block pairs b = w, w + NW, ...: group-0 block b and group-1 block b, so the two
 // results of one output row pair (g0_i, g1_i) sit in the same lane and item and unfold in
 // registers.  The panels (the spectral data, folded and prologue applied) are produced once per
 // element by one thread and staged through double-buffered LDS chunks of rows; the next
@@ -19,12 +20,31 @@ constexpr int BCOLS = 16 * CT;     // real columns per block
 constexpr int BMODES = BCOLS / 2;  // complex modes per block
 constexpr int KC = 16;             // panel rows per chunk of an LDS-resident contraction (4 k-steps)
 constexpr int LDB = BCOLS + 2;     // padded LDS row stride, doubles
-// k-steps per staged chunk of fold_gemm (with 8, the prefetch registers cost occupancy and the
-// kernels measured slower)
-__host__ __device__ constexpr int fold_chunk_ks(int) { return 4; }
-// chunks by which the panel reads of fold_gemm run ahead of the MFMAs
+// k-steps per staged chunk of fold_gemm in a block of NW waves (with 8, the prefetch registers
+// cost occupancy and the kernels measured slower; blocks of one or two waves take shorter chunks
+// so that each thread stages one row of a chunk).  Each must divide 8: ksp is a multiple of 8,
+// so the A operands of the chunk after the last one stay inside the block.
+#ifndef FOLD_KS_NW1
+#define FOLD_KS_NW1 1
+#endif
+#ifndef FOLD_KS_NW2
+#define FOLD_KS_NW2 2
+#endif
+#ifndef FOLD_KS_NW3
+#define FOLD_KS_NW3 2
+#endif
+__host__ __device__ constexpr int fold_chunk_ks(int nw) {
+    return nw == 1 ? FOLD_KS_NW1 : nw == 2 ? FOLD_KS_NW2 : nw == 3 ? FOLD_KS_NW3 : 4;
+}
+static_assert(8 % fold_chunk_ks(1) == 0 && 8 % fold_chunk_ks(2) == 0 && 8 % fold_chunk_ks(3) == 0,
+              "chunk k-steps must divide 8");
+// chunks by which the panel reads of fold_gemm run ahead of the MFMAs (four waves;
+// FOLD_PREFETCH_SMALL for fewer)
 #ifndef FOLD_PREFETCH
 #define FOLD_PREFETCH 1
+#endif
+#ifndef FOLD_PREFETCH_SMALL
+#define FOLD_PREFETCH_SMALL 1
 #endif
 constexpr int STAGE_LDS = 4 * 4 * 4 * LDB;   // doubles of the staging buffers (2 buffers x 2 panels x 16 rows)
 
@@ -32,9 +52,10 @@ struct FoldPair {
     double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
 };
 
-// acc0[i][ct] += A0[block b] x P0, acc1[i][ct] += A1[block b] x P1 for b = wave + 4i.
-// Panel row kf for the staging thread's mode (thread t stages rows t/16 + 16 r of a chunk for
-// mode t%16) is finish(kf, fetch(kf)), zero beyond K: fetch issues the memory reads (branch
+// acc0[i][ct] += A0[block b] x P0, acc1[i][ct] += A1[block b] x P1 for b = wave + NW i (a
+// block of NW waves).
+// Panel row kf for the staging thread's mode (thread t of the block's NW waves stages rows
+// t/16 + 4 NW r of a chunk for mode t%16) is finish(kf, fetch(kf)), zero beyond K: fetch issues the memory reads (branch
 // free), finish does the arithmetic, so the next chunk's reads are in flight during the
 // current chunk's MFMA chain.  lds: STAGE_LDS doubles.
 // With op.extra, xacc accumulates the staging thread's share of the dense extra row times
@@ -42,13 +63,14 @@ struct FoldPair {
 // SKIP: waves without row blocks skip the MFMA chain (operators with fewer row-block pairs
 // than waves, the small-nz variants); otherwise every wave has blocks and the loop body stays
 // one straight-line block.
-template <int RBW, bool SKIP, class FETCH, class FINISH>
+template <int RBW, bool SKIP, int NW, class FETCH, class FINISH>
 __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FETCH&& fetch,
                                           FINISH&& finish, double* lds, double2& xacc) {
     using Raw = decltype(fetch(0));
-    constexpr int KS = fold_chunk_ks(RBW);   // k-steps per chunk
+    constexpr int KS = fold_chunk_ks(NW);    // k-steps per chunk
     constexpr int KR = 4 * KS;               // panel rows per chunk
-    constexpr int RPT = KR / 16;             // rows staged per thread
+    constexpr int RS = 4 * NW;               // rows one pass of the block's threads stages
+    constexpr int RPT = (KR + RS - 1) / RS;  // rows staged per thread (the last pass may be partial)
     const double* xrow = fold_xrow(op);
     const int xlast = 4 * op.ksp - 1;        // the extra row's stored entries (zero beyond K)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -60,7 +82,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     const double* a1p[RBW];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
-        const int b = wave + 4 * i;
+        const int b = wave + NW * i;
         a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
         a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
     }
@@ -77,6 +99,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     // the extra-row coefficient of a staged row (clamped: rows past K have zero panel entries)
     auto xcoef = [&](int kf) { return op.extra ? xrow[min(kf, xlast)] : 0.0; };
     auto stage = [&](double* buf, const FoldPair& v, int row, double c) {
+        if (RS * RPT > KR && row >= KR) return;   // a partial last pass (wave-uniform)
         xacc.x += c * v.p0.x;
         xacc.y += c * v.p0.y;
         buf[row * LDB + 2 * mt] = v.p0.x;
@@ -89,22 +112,22 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     // chunk past the end folds rows past K, which are zero, into the idle buffer; every wave
     // issues the chain, absent row blocks having zero A operands).  q[d] holds the reads of
     // chunk ch + 1 + d.
-    constexpr int PD = FOLD_PREFETCH;
+    constexpr int PD = NW < 4 ? FOLD_PREFETCH_SMALL : FOLD_PREFETCH;
     Raw q[PD][RPT];
     double qc[PD][RPT];
     {
         Raw r0[RPT];
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
-            r0[rr] = fetch(rt + 16 * rr);
+            r0[rr] = fetch(rt + RS * rr);
 #pragma unroll
             for (int d = 0; d < PD; d++) {
-                q[d][rr] = fetch((1 + d) * KR + rt + 16 * rr);
-                qc[d][rr] = xcoef((1 + d) * KR + rt + 16 * rr);
+                q[d][rr] = fetch((1 + d) * KR + rt + RS * rr);
+                qc[d][rr] = xcoef((1 + d) * KR + rt + RS * rr);
             }
         }
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) stage(lds, finish(rt + 16 * rr, r0[rr]), rt + 16 * rr, xcoef(rt + 16 * rr));
+        for (int rr = 0; rr < RPT; rr++) stage(lds, finish(rt + RS * rr, r0[rr]), rt + RS * rr, xcoef(rt + RS * rr));
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ch++) {
@@ -115,10 +138,10 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         FoldPair f1[RPT];
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
-            const int kf2 = (ch + 1 + PD) * KR + rt + 16 * rr;
+            const int kf2 = (ch + 1 + PD) * KR + rt + RS * rr;
             r2[rr] = fetch(kf2);
             c2[rr] = xcoef(kf2);
-            f1[rr] = finish((ch + 1) * KR + rt + 16 * rr, q[0][rr]);
+            f1[rr] = finish((ch + 1) * KR + rt + RS * rr, q[0][rr]);
         }
         const int nxa = min(ch + 1, nch - 1) * KS;   // the next chunk's A operands (the last chunk reloads its own)
         if (!SKIP || active)
@@ -143,7 +166,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         }
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
-            stage(nb, f1[rr], rt + 16 * rr, qc[0][rr]);
+            stage(nb, f1[rr], rt + RS * rr, qc[0][rr]);
 #pragma unroll
             for (int d = 0; d + 1 < PD; d++) {
                 q[d][rr] = q[d + 1][rr];
@@ -159,7 +182,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
 // The same contraction with both panels resident in LDS (z0, z1: >= 4*ks rows, row stride LDB);
 // the A operands are fetched one chunk of 4 k-steps ahead (ksp is a multiple of 4 >= ks + 1,
 // so the chunk after the last one stays inside the block).
-template <int RBW>
+template <int RBW, int NW>
 __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op,
                                               const double* z0, const double* z1) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -172,7 +195,7 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
     double ar0[RBW][4], ar1[RBW][4], an0[RBW][4], an1[RBW][4];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
-        const int b = wave + 4 * i;
+        const int b = wave + NW * i;
         a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
         a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
 #pragma unroll
@@ -218,8 +241,9 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
     }
 }
 
-// Column t < BCOLS of the extra row from the staging threads' shares (mode t/2, part t%2);
-// every thread calls it (it synchronises); uses 2 * 256 doubles of lds.
+// Column t < BCOLS of the extra row from the staging threads' shares (mode t/2, part t%2) of a
+// block of NW waves; every thread calls it (it synchronises); uses 2 * 64 NW doubles of lds.
+template <int NW>
 __device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds) {
     const int t = threadIdx.x;
     lds[2 * t] = xacc.x;
@@ -228,7 +252,7 @@ __device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds)
     double v = 0.0;
     if (t < BCOLS) {
         const int ml = t >> 1, part = t & 1;
-        for (int rt = 0; rt < 16; rt++) v += lds[2 * (rt * 16 + ml) + part];
+        for (int rt = 0; rt < 4 * NW; rt++) v += lds[2 * (rt * 16 + ml) + part];
     }
     __syncthreads();
     return v;
@@ -254,18 +278,18 @@ __device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) 
 
 // The two row groups of a folded result into LDS panels G0 (rows0 rows) and G1 (rows1 rows),
 // row stride LDB; the extra group-0 row (computed elsewhere) is written by the caller.
-template <int RBW>
+template <int RBW, int NW>
 __device__ __forceinline__ void store_groups(const d4 (&acc0)[RBW][CT], const d4 (&acc1)[RBW][CT], const FoldOp& op,
                                              double* G0, double* G1) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
 #pragma unroll
     for (int ii = 0; ii < RBW; ii++) {
-        const int b = wave + 4 * ii;
+        const int b = wave + NW * ii;
 #pragma unroll
         for (int ct = 0; ct < CT; ct++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = (wave + 4 * ii) * 16 + (lane >> 4) + 4 * r;
+                const int row = (wave + NW * ii) * 16 + (lane >> 4) + 4 * r;
                 if (b < op.rb0 && row < op.rows0) G0[row * LDB + ct * 16 + cl] = acc0[ii][ct][r];
                 if (b < op.rb1 && row < op.rows1) G1[row * LDB + ct * 16 + cl] = acc1[ii][ct][r];
             }
@@ -289,18 +313,21 @@ __device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
         for (int ct = 0; ct < CT; ct++) acc[i][ct] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
-// Coordinates of acc[i][ct][r]: row within the group = (wave + 4i)*16 + lane/16 + 4r (the
+// Coordinates of acc[i][ct][r] in a block of NW waves: row within the group =
+// (wave + NW i)*16 + lane/16 + 4r (the
 // result layout of the instruction), column = cb*BCOLS + ct*16 + lane%16.
+template <int NW>
 __device__ __forceinline__ int out_row(int i, int r) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    return (wave + 4 * i) * 16 + (lane >> 4) + 4 * r;
+    return (wave + NW * i) * 16 + (lane >> 4) + 4 * r;
 }
 __device__ __forceinline__ int out_col(int ct, int cb) { return cb * BCOLS + ct * 16 + (threadIdx.x & 15); }
 
 // zero rows [rows, zr) of an LDS panel (the k rows past an operator's K that a chunked
 // contraction still reads; its A columns there are zero, so the rows must be finite)
+template <int NW>
 __device__ __forceinline__ void zero_panel_tail(double* z, int rows, int zr) {
-    for (int e = threadIdx.x; e < (zr - rows) * BCOLS; e += 256) z[(rows + e / BCOLS) * LDB + e % BCOLS] = 0.0;
+    for (int e = threadIdx.x; e < (zr - rows) * BCOLS; e += 64 * NW) z[(rows + e / BCOLS) * LDB + e % BCOLS] = 0.0;
 }
 
 // panel rows of the LDS-resident z panels of a back contraction with k-steps ks: whole chunks

@@ -213,11 +213,11 @@ int Ensemble::rebuild() {
                     while (k > base && fft_class_capacity(k, c->mixed) < 0) k--;
                     key = k;
                 } else if (kind == K_ZVORT) {
-                    key = fold_var(hz[g].d1); f32 = 0;
+                    key = zd_variant(hz[g]); f32 = 0;
                 } else if (kind == K_POIS) {
-                    key = fold_var(hp[g].pF); f32 = 0;
+                    key = pois_variant(hp[g]); f32 = 0;
                 } else if (kind == K_HELM) {
-                    key = fold_var(hh[g].hF); f32 = hh[g].s_f32 ? 1 : 0;
+                    key = helm_variant(hh[g]); f32 = hh[g].s_f32 ? 1 : 0;
                 } else {
                     f32 = 0;
                 }

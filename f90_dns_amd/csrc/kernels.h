@@ -138,13 +138,26 @@ struct ZArgs {
     int out_f32;
 };
 enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
-// pairs of row blocks per wave of a folded operator
-inline int fold_rbw(const FoldOp& o) { return o.pb / 4; }
-// fewer row blocks than waves: the kernels unfold their results through LDS so that every
-// thread stores
-inline bool fold_lx(const FoldOp& o) { return (o.rb0 > o.rb1 ? o.rb0 : o.rb1) < 4; }
-// kernel variant of a folded operator: 0 = one pair per wave with the LDS epilogue, else pairs
-inline int fold_var(const FoldOp& o) { return fold_lx(o) ? 0 : fold_rbw(o); }
+// row-block pairs of a folded operator
+inline int fold_nrb(const FoldOp& o) { return o.rb0 > o.rb1 ? o.rb0 : o.rb1; }
+// Kernel variant of a contraction kernel whose operators have at most nrb row-block pairs:
+// 0, 1, 2 = one wave per pair (1 to 3 pairs; results unfold through LDS so that every thread
+// stores), 3 = four waves with one pair each, 4 = four waves with two pairs each (the stored
+// operators are zero padded to 4 ceil(nrb / 4) blocks per group); -1 = more than 8 pairs.
+inline int z_variant(int nrb) { return nrb < 4 ? nrb - 1 : nrb <= 8 ? 2 + (nrb + 3) / 4 : -1; }
+inline bool z_variant_lx(int var) { return var >= 0 && var < 3; }
+// the variant of a kernel applying the operators o[0..n): -1 unless all are padded alike
+inline int z_variant_of(const FoldOp* const* o, int n) {
+    int m = 0;
+    for (int i = 0; i < n; i++) m = fold_nrb(*o[i]) > m ? fold_nrb(*o[i]) : m;
+    for (int i = 0; i < n; i++)
+        if (o[i]->pb != 4 * ((m + 3) / 4)) return -1;
+    return z_variant(m);
+}
+inline int zd_variant(const ZArgs& a) {
+    const FoldOp* o[1] = {&a.d1};
+    return z_variant_of(o, 1);
+}
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
 size_t zderiv_lds_bytes(const ZArgs& a);
 
@@ -165,6 +178,10 @@ struct HelmArgs {
     const FlowState* flow;       // per member: current pressure gradient G
 };
 hipError_t launch_helmholtz(const HelmArgs& a, int nmembers, hipStream_t s);
+inline int helm_variant(const HelmArgs& a) {
+    const FoldOp* o[2] = {&a.hF, &a.hB};
+    return z_variant_of(o, 2);
+}
 // dynamic LDS bytes of the Helmholtz / Poisson kernels (staging, then the two z panels)
 size_t helm_lds_bytes(const HelmArgs& a);
 
@@ -181,6 +198,10 @@ struct PoisArgs {
     double* phi; double* dphi;                              // stage outputs
 };
 hipError_t launch_poisson(const PoisArgs& a, int nmembers, hipStream_t s);
+inline int pois_variant(const PoisArgs& a) {
+    const FoldOp* o[3] = {&a.pF, &a.pB, &a.pDB};
+    return z_variant_of(o, 3);
+}
 size_t pois_lds_bytes(const PoisArgs& a);
 
 struct Mode0Args {
@@ -228,7 +249,7 @@ struct FlowArgs {
 };
 hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
 
-// ragged variants of the step kernels (var = fold_var of the groups' operators; lds = dynamic
+// ragged variants of the step kernels (var = zd/helm/pois_variant of the groups; lds = dynamic
 // LDS bytes, the largest over the launch's groups)
 hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int ntiles, int nf, size_t lds,
                              hipStream_t s);

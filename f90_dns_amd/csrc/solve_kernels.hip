@@ -13,7 +13,7 @@
 
 namespace dns {
 
-static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to 256 threads");
+static_assert(BMODES == 16, "staging assigns 16 modes x 16 rows to a block's threads");
 static_assert(KC == 16, "a chunk is 16 panel rows");
 
 // waves per SIMD the contraction kernels are compiled for: three with one row-block pair per
@@ -37,8 +37,10 @@ __device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx,
 // (DNS_pressure_BC_3D.f90:1077-1079).  Folded: s = D1 x_odd on X_O (rows up to the centre),
 // r = D1 x_even on X_E, out_k = s_k + r_k, out_{nz-1-k} = s_k - r_k.  One block = 16 modes.
 // -------------------------------------------------------------------------------------
-template <int RBW, int MODE, bool LX>
+template <int RBW, int MODE, int NW>
 __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int mb, double* lds) {
+    constexpr bool LX = NW < 4;   // fewer row-block pairs than four: one wave per pair, LDS unfold
+    constexpr int NT = 64 * NW;
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
     const int wave = threadIdx.x >> 6;
     const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
@@ -90,25 +92,25 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
         zero_acc2(acc0);
         zero_acc2(acc1);
         double2 xacc = make_double2(0.0, 0.0);
-        fold_gemm<RBW, LX>(acc0, acc1, a.d1, fetch, finish, lds, xacc);
-        const double xs = a.d1.extra ? reduce_extra(xacc, lds) : 0.0;   // the centre row s_h
+        fold_gemm<RBW, false, NW>(acc0, acc1, a.d1, fetch, finish, lds, xacc);
+        const double xs = a.d1.extra ? reduce_extra<NW>(xacc, lds) : 0.0;   // the centre row s_h
         if constexpr (LX) {
             // few row blocks: unfold through LDS so that every thread stores
             double* G0 = lds;
             double* G1 = lds + a.d1.rows0 * LDB;
-            store_groups<RBW>(acc0, acc1, a.d1, G0, G1);
+            store_groups<RBW, NW>(acc0, acc1, a.d1, G0, G1);
             if (a.d1.extra && threadIdx.x < BCOLS) G0[(a.d1.rows0 - 1) * LDB + threadIdx.x] = xs;
             __syncthreads();
             // four outputs per thread per pass, their w and wavenumber reads issued first
             // (indices clamped into range)
             const int ne = nz * BCOLS;
-            for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+            for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
                 double y[4], wc[4], wo[4], kk[4];
                 int nd[4], cl4[4];
                 bool ok[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    const int e = e0 + 256 * u, ec = min(e, ne - 1);
+                    const int e = e0 + NT * u, ec = min(e, ne - 1);
                     const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c, colc = min(col, H2 - 1);
                     const int mode = colc >> 1, j = mode / a.nxh, i = mode - j * a.nxh;
                     ok[u] = e < ne && col < H2;
@@ -155,13 +157,13 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             const double kk = MODE == Z_VORT ? (f == 1 ? a.yw[j] : a.xw[i]) : 0.0;
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
-                const int b = wave + 4 * ii;
+                const int b = wave + NW * ii;
                 double yv[4][2], wc[4][2], wo[4][2];
                 size_t qn[4][2];
                 bool ok[4][2];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int row = out_row(ii, r);
+                    const int row = out_row<NW>(ii, r);
                     const double sv = acc0[ii][ct][r];
                     const double rv = (row < hF && b < a.d1.rb1) ? acc1[ii][ct][r] : 0.0;
                     yv[r][0] = sv + rv;
@@ -202,17 +204,17 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     }
 }
 
-template <int RBW, int MODE, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_D1)) void k_zderiv(ZArgs a) {
+template <int RBW, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv(ZArgs a) {
     extern __shared__ double smem[];
-    zderiv_tile<RBW, MODE, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    zderiv_tile<RBW, MODE, NW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
-template <int RBW, int MODE, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
+template <int RBW, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
-    zderiv_tile<RBW, MODE, LX>(a, t.x, blockIdx.y, t.z, smem);
+    zderiv_tile<RBW, MODE, NW>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // -------------------------------------------------------------------------------------
@@ -226,8 +228,10 @@ __global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(const Z
 // Folded: even eigen rows on [v_E ; u_E], odd rows on [v_O ; u_O] (v = alpha u_I + dt S'_I over
 // the interior, u over the whole column); u*_I unfolds from p = B_E z_E and q = B_O z_O.
 // -------------------------------------------------------------------------------------
-template <int RBW, bool SF32, bool LX>
+template <int RBW, bool SF32, int NW>
 __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int comp, int mb, double* smem) {
+    constexpr bool LX = NW < 4;
+    constexpr int NT = 64 * NW;
     const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2;
     const int hI = m / 2, L1 = hI + (m & 1), hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
@@ -316,16 +320,16 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     zero_acc2(acc0);
     zero_acc2(acc1);
     double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW, LX>(acc0, acc1, F, fetch, finish, smem, xacc);
-    const double xv = F.extra ? reduce_extra(xacc, smem) : 0.0;
+    fold_gemm<RBW, false, NW>(acc0, acc1, F, fetch, finish, smem, xacc);
+    const double xv = F.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.hB.ks);
     double* z0 = smem;
     double* z1 = smem + ZR * LDB;
     // the back contraction reads whole 16-row chunks: zero the panel rows past each group's rows
     // (every other row of a live column is written below; the barrier before the back
     // contraction orders these writes)
-    zero_panel_tail(z0, F.rows0, ZR);
-    zero_panel_tail(z1, F.rows1, ZR);
+    zero_panel_tail<NW>(z0, F.rows0, ZR);
+    zero_panel_tail<NW>(z1, F.rows1, ZR);
     if (F.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2) {
         const int row = F.rows0 - 1, ct = threadIdx.x >> 4;
         z0[row * LDB + threadIdx.x] = (xv + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
@@ -337,7 +341,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         for (int ii = 0; ii < RBW; ii++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = out_row(ii, r);
+                const int row = out_row<NW>(ii, r);
                 const int g0 = min(row, F.rows0 - 1), g1 = F.rows0 + min(row, F.rows1 - 1);
                 e1[0][ii][r] = c1v[g0]; e2[0][ii][r] = c2v[g0]; el[0][ii][r] = hlv[g0];
                 e1[1][ii][r] = c1v[g1]; e2[1][ii][r] = c2v[g1]; el[1][ii][r] = hlv[g1];
@@ -347,10 +351,10 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
             const bool cv = out_col(ct, cb) < H2;
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
-                const int b = wave + 4 * ii;
+                const int b = wave + NW * ii;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int row = out_row(ii, r);
+                    const int row = out_row<NW>(ii, r);
                     if (cv && b < F.rb0 && row < F.rows0)
                         z0[row * LDB + ct * 16 + cl] = (acc0[ii][ct][r] + e1[0][ii][r] * bc1[ct] + e2[0][ii][r] * bc2[ct]) /
                                                        (el[0][ii][r] + fco * k2c[ct]);
@@ -364,7 +368,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     __syncthreads();
     zero_acc2(acc0);
     zero_acc2(acc1);
-    fold_gemm_lds<RBW>(acc0, acc1, a.hB, z0, z1);
+    fold_gemm_lds<RBW, NW>(acc0, acc1, a.hB, z0, z1);
     double* out = a.out[comp] + mo;
     if constexpr (LX) {
         // few row blocks: unfold through LDS (over the z panels, once every wave has read them)
@@ -373,7 +377,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         double* G0 = smem;
         double* G1 = smem + a.hB.rows0 * LDB;
         double* bcs = G1 + a.hB.rows1 * LDB;   // 2 x BCOLS wall values
-        store_groups<RBW>(acc0, acc1, a.hB, G0, G1);
+        store_groups<RBW, NW>(acc0, acc1, a.hB, G0, G1);
         if (a.hB.extra && threadIdx.x < BCOLS) G0[(a.hB.rows0 - 1) * LDB + threadIdx.x] = xc;
         if (threadIdx.x < 16) {
 #pragma unroll
@@ -383,7 +387,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
             }
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < nz * BCOLS; e += 256) {
+        for (int e = threadIdx.x; e < nz * BCOLS; e += NT) {
             const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
             if (col >= H2) continue;
             const double v = node == 0 ? bcs[c] : node == nz - 1 ? bcs[BCOLS + c] : unfold_lds(G0, G1, m, node - 1, c);
@@ -399,11 +403,11 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         if (col >= H2) continue;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
-            const int b = wave + 4 * ii;
+            const int b = wave + NW * ii;
             if (b >= a.hB.rb0) continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = out_row(ii, r);
+                const int row = out_row<NW>(ii, r);
                 if (row >= L1) continue;
                 const double p = acc0[ii][ct][r];
                 const double q = (row < hI && b < a.hB.rb1) ? acc1[ii][ct][r] : 0.0;
@@ -418,17 +422,17 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     }
 }
 
-template <int RBW, bool SF32, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz(HelmArgs a) {
+template <int RBW, bool SF32, int NW>
+__global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz(HelmArgs a) {
     extern __shared__ double smem[];
-    helmholtz_tile<RBW, SF32, LX>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    helmholtz_tile<RBW, SF32, NW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
-template <int RBW, bool SF32, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
+template <int RBW, bool SF32, int NW>
+__global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
-    helmholtz_tile<RBW, SF32, LX>(a, t.x, blockIdx.y, t.z, smem);
+    helmholtz_tile<RBW, SF32, NW>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // -------------------------------------------------------------------------------------
@@ -442,8 +446,10 @@ __global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_rag(co
 // (Bf_E z_E, Bf_O z_O), D1 phi from ((D1 Bf)_O z_O, (D1 Bf)_E z_E).  Mode (0,0) is left to
 // k_poisson_mode0.
 // -------------------------------------------------------------------------------------
-template <int RBW, bool LX>
+template <int RBW, int NW>
 __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
+    constexpr bool LX = NW < 4;
+    constexpr int NT = 64 * NW;
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
     const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
@@ -508,13 +514,13 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     zero_acc2(acc0);
     zero_acc2(acc1);
     double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW, LX>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
-    const double xv = a.pF.extra ? reduce_extra(xacc, smem) : 0.0;
+    fold_gemm<RBW, false, NW>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
+    const double xv = a.pF.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.pB.ks);
     double* z0 = smem;
     double* z1 = smem + ZR * LDB;
-    zero_panel_tail(z0, a.pF.rows0, ZR);
-    zero_panel_tail(z1, a.pF.rows1, ZR);
+    zero_panel_tail<NW>(z0, a.pF.rows0, ZR);
+    zero_panel_tail<NW>(z1, a.pF.rows1, ZR);
     if (a.pF.extra && threadIdx.x < BCOLS) {
         const int row = a.pF.rows0 - 1, ct = threadIdx.x >> 4;
         if (live[ct]) z0[row * LDB + threadIdx.x] = -xv / (a.dt * (a.pl[row] + k2c[ct]));
@@ -526,7 +532,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         for (int ii = 0; ii < RBW; ii++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = out_row(ii, r);
+                const int row = out_row<NW>(ii, r);
                 l0[ii][r] = a.pl[min(row, a.pF.rows0 - 1)];
                 l1[ii][r] = a.pl[a.pF.rows0 + min(row, a.pF.rows1 - 1)];
             }
@@ -534,10 +540,10 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         for (int ct = 0; ct < CT; ct++) {
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
-                const int b = wave + 4 * ii;
+                const int b = wave + NW * ii;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int row = out_row(ii, r);
+                    const int row = out_row<NW>(ii, r);
                     if (live[ct] && b < a.pF.rb0 && row < a.pF.rows0)
                         z0[row * LDB + ct * 16 + cl] = -acc0[ii][ct][r] / (a.dt * (l0[ii][r] + k2c[ct]));
                     if (live[ct] && b < a.pF.rb1 && row < a.pF.rows1)
@@ -550,7 +556,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     // phi = Bf z: pressure increment and the horizontal projection
     zero_acc2(acc0);
     zero_acc2(acc1);
-    fold_gemm_lds<RBW>(acc0, acc1, a.pB, z0, z1);
+    fold_gemm_lds<RBW, NW>(acc0, acc1, a.pB, z0, z1);
     if constexpr (LX) {
         // few row blocks: both projections unfold through LDS, after the z panels
         double* G0 = smem + 2 * ZR * LDB;
@@ -565,18 +571,18 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
             }
         }
         const double xph = (a.pB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pB, z0) : 0.0;
-        store_groups<RBW>(acc0, acc1, a.pB, G0, G1);
+        store_groups<RBW, NW>(acc0, acc1, a.pB, G0, G1);
         if (a.pB.extra && threadIdx.x < BCOLS) G0[(a.pB.rows0 - 1) * LDB + threadIdx.x] = xph;
         __syncthreads();
         // four outputs per thread per pass, their reads of u*, v*, p issued first (clamped)
         const int ne = nz * BCOLS;
-        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
             double uu[4], vv[4], pp[4];
             size_t qn[4];
             bool ok[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int e = e0 + 256 * u, ec = min(e, ne - 1);
+                const int e = e0 + NT * u, ec = min(e, ne - 1);
                 const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c;
                 ok[u] = e < ne && col < H2 && cc[2 * BCOLS + c] != 0.0;
                 qn[u] = size_t(node) * H2 + min(col, H2 - 1);
@@ -589,7 +595,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 if (!ok[u]) continue;
-                const int e = e0 + 256 * u, node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
+                const int e = e0 + NT * u, node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
                 const double ph = unfold_lds(G0, G1, nz, node, c), other = unfold_lds(G0, G1, nz, node, c ^ 1);
                 if (!a.fused) {
                     phio[qn[u]] = ph;
@@ -608,23 +614,23 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         __syncthreads();
         zero_acc2(acc0);
         zero_acc2(acc1);
-        fold_gemm_lds<RBW>(acc0, acc1, a.pDB, z1, z0);
+        fold_gemm_lds<RBW, NW>(acc0, acc1, a.pDB, z1, z0);
         const double xd = (a.pDB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pDB, z1) : 0.0;
-        store_groups<RBW>(acc0, acc1, a.pDB, G0, G1);
+        store_groups<RBW, NW>(acc0, acc1, a.pDB, G0, G1);
         if (a.pDB.extra && threadIdx.x < BCOLS) G0[(a.pDB.rows0 - 1) * LDB + threadIdx.x] = xd;
         __syncthreads();
-        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
             double ww[4];
             size_t qn[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int e = min(e0 + 256 * u, ne - 1), node = e / BCOLS, c = e - node * BCOLS;
+                const int e = min(e0 + NT * u, ne - 1), node = e / BCOLS, c = e - node * BCOLS;
                 qn[u] = size_t(node) * H2 + min(cb * BCOLS + c, H2 - 1);
                 ww[u] = a.fused ? ws[qn[u]] : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int e = e0 + 256 * u;
+                const int e = e0 + NT * u;
                 if (e >= ne) continue;
                 const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
                 if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
@@ -643,13 +649,13 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
         const int part = col & 1;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
-            const int b = wave + 4 * ii;
+            const int b = wave + NW * ii;
             double pv[4][2], ov[4][2];
             size_t qn[4][2];
             bool ok[4][2];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = out_row(ii, r);
+                const int row = out_row<NW>(ii, r);
                 const double p = acc0[ii][ct][r];
                 const double q = (row < hF && b < a.pB.rb1) ? acc1[ii][ct][r] : 0.0;
                 pv[r][0] = p + q;
@@ -685,7 +691,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
                     if (!ok[r][side]) continue;
-                    const int row = out_row(ii, r);
+                    const int row = out_row<NW>(ii, r);
                     const int node = side == 0 ? row : nz - 1 - row;
                     const double ph = pv[r][side], other = ov[r][side];
                     const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
@@ -724,19 +730,19 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     // D1 phi = (D1 Bf) z: the wall-normal projection (s on z_O, r on z_E)
     zero_acc2(acc0);
     zero_acc2(acc1);
-    fold_gemm_lds<RBW>(acc0, acc1, a.pDB, z1, z0);
+    fold_gemm_lds<RBW, NW>(acc0, acc1, a.pDB, z1, z0);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb), colc = min(col, H2 - 1);
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
-            const int b = wave + 4 * ii;
+            const int b = wave + NW * ii;
             double dv[4][2], wv[4][2];
             size_t qn[4][2];
             bool ok[4][2];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = out_row(ii, r);
+                const int row = out_row<NW>(ii, r);
                 const double sv = acc0[ii][ct][r];
                 const double rv = (row < hF && b < a.pDB.rb1) ? acc1[ii][ct][r] : 0.0;
                 dv[r][0] = sv + rv;
@@ -754,7 +760,7 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
                     if (!ok[r][side]) continue;
-                    const int row = out_row(ii, r);
+                    const int row = out_row<NW>(ii, r);
                     const int node = side == 0 ? row : nz - 1 - row;
                     const size_t q = qn[r][side];
                     if (!a.fused) {
@@ -778,17 +784,17 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     }
 }
 
-template <int RBW, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_POIS)) void k_poisson(PoisArgs a) {
+template <int RBW, int NW>
+__global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_POIS)) void k_poisson(PoisArgs a) {
     extern __shared__ double smem[];
-    poisson_tile<RBW, LX>(a, blockIdx.x, blockIdx.z, smem);
+    poisson_tile<RBW, NW>(a, blockIdx.x, blockIdx.z, smem);
 }
-template <int RBW, bool LX>
-__global__ __launch_bounds__(256, Z_WAVES(RBW, ZW_POIS)) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
+template <int RBW, int NW>
+__global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_POIS)) void k_poisson_rag(const PoisArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const PoisArgs& a = d[t.g];
-    poisson_tile<RBW, LX>(a, t.x, t.z, smem);
+    poisson_tile<RBW, NW>(a, t.x, t.z, smem);
 }
 
 // mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
@@ -982,51 +988,58 @@ __global__ void k_reduce_final(ReduceArgs a) {
 static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
 
 size_t zderiv_lds_bytes(const ZArgs& a) {
-    const size_t g = fold_lx(a.d1) ? size_t(a.d1.rows0 + a.d1.rows1) * LDB : 0;
+    const size_t g = z_variant_lx(zd_variant(a)) ? size_t(a.d1.rows0 + a.d1.rows1) * LDB : 0;
     return sizeof(double) * std::max<size_t>(STAGE_LDS, g);
 }
 size_t helm_lds_bytes(const HelmArgs& a) {
     const size_t z = size_t(2) * zpanel_rows(a.hB.ks) * LDB;
-    const size_t g = fold_lx(a.hB) ? size_t(a.hB.rows0 + a.hB.rows1) * LDB + 2 * BCOLS : 0;
+    const size_t g = z_variant_lx(helm_variant(a)) ? size_t(a.hB.rows0 + a.hB.rows1) * LDB + 2 * BCOLS : 0;
     return sizeof(double) * std::max<size_t>(std::max<size_t>(STAGE_LDS, z), g);
 }
 size_t pois_lds_bytes(const PoisArgs& a) {
     size_t z = size_t(2) * zpanel_rows(a.pB.ks) * LDB;
-    if (fold_lx(a.pB)) z += size_t(a.pB.rows0 + a.pB.rows1) * LDB + 3 * BCOLS;
+    if (z_variant_lx(pois_variant(a))) z += size_t(a.pB.rows0 + a.pB.rows1) * LDB + 3 * BCOLS;
     return sizeof(double) * std::max<size_t>(STAGE_LDS, z);
 }
 
-// variant index: row-block pairs per wave (1, 2) x LDS epilogue (only with 1 pair)
-static inline int zvar(const FoldOp& o) { return fold_lx(o) ? 0 : fold_rbw(o); }
-
-#define ZD_CASES(K, ...)                                                                             \
+// variants (z_variant): (pairs per wave, waves per block) = (1, 1), (1, 2), (1, 3), (1, 4), (2, 4)
+#define Z_BLOCK(var) dim3((var) < 3 ? 64 * ((var) + 1) : 256)
+#define ZD_CASES(K, G, ...)                                                                          \
     switch (var * 2 + (mode == Z_VORT)) {                                                            \
-        case 0: hipLaunchKernelGGL((K<1, Z_PLAIN, true>), __VA_ARGS__); break;                        \
-        case 1: hipLaunchKernelGGL((K<1, Z_VORT, true>), __VA_ARGS__); break;                         \
-        case 2: hipLaunchKernelGGL((K<1, Z_PLAIN, false>), __VA_ARGS__); break;                       \
-        case 3: hipLaunchKernelGGL((K<1, Z_VORT, false>), __VA_ARGS__); break;                        \
-        case 4: hipLaunchKernelGGL((K<2, Z_PLAIN, false>), __VA_ARGS__); break;                       \
-        case 5: hipLaunchKernelGGL((K<2, Z_VORT, false>), __VA_ARGS__); break;                        \
+        case 0: hipLaunchKernelGGL((K<1, Z_PLAIN, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;          \
+        case 1: hipLaunchKernelGGL((K<1, Z_VORT, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;           \
+        case 2: hipLaunchKernelGGL((K<1, Z_PLAIN, 2>), G, Z_BLOCK(var), __VA_ARGS__); break;          \
+        case 3: hipLaunchKernelGGL((K<1, Z_VORT, 2>), G, Z_BLOCK(var), __VA_ARGS__); break;           \
+        case 4: hipLaunchKernelGGL((K<1, Z_PLAIN, 3>), G, Z_BLOCK(var), __VA_ARGS__); break;          \
+        case 5: hipLaunchKernelGGL((K<1, Z_VORT, 3>), G, Z_BLOCK(var), __VA_ARGS__); break;           \
+        case 6: hipLaunchKernelGGL((K<1, Z_PLAIN, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;          \
+        case 7: hipLaunchKernelGGL((K<1, Z_VORT, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;           \
+        case 8: hipLaunchKernelGGL((K<2, Z_PLAIN, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;          \
+        case 9: hipLaunchKernelGGL((K<2, Z_VORT, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;           \
         default: return hipErrorInvalidValue;                                                        \
     }
 
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
     g.z = nm;
-    const int var = zvar(a.d1);
+    const int var = zd_variant(a);
     const size_t lds = zderiv_lds_bytes(a);
-    ZD_CASES(k_zderiv, g, dim3(256), lds, s, a)
+    ZD_CASES(k_zderiv, g, lds, s, a)
     return hipGetLastError();
 }
 
-#define HM_CASES(K, ...)                                                                             \
+#define HM_CASES(K, G, ...)                                                                          \
     switch (var * 2 + (sf32 ? 1 : 0)) {                                                              \
-        case 0: hipLaunchKernelGGL((K<1, false, true>), __VA_ARGS__); break;                          \
-        case 1: hipLaunchKernelGGL((K<1, true, true>), __VA_ARGS__); break;                           \
-        case 2: hipLaunchKernelGGL((K<1, false, false>), __VA_ARGS__); break;                         \
-        case 3: hipLaunchKernelGGL((K<1, true, false>), __VA_ARGS__); break;                          \
-        case 4: hipLaunchKernelGGL((K<2, false, false>), __VA_ARGS__); break;                         \
-        case 5: hipLaunchKernelGGL((K<2, true, false>), __VA_ARGS__); break;                          \
+        case 0: hipLaunchKernelGGL((K<1, false, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;            \
+        case 1: hipLaunchKernelGGL((K<1, true, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;             \
+        case 2: hipLaunchKernelGGL((K<1, false, 2>), G, Z_BLOCK(var), __VA_ARGS__); break;            \
+        case 3: hipLaunchKernelGGL((K<1, true, 2>), G, Z_BLOCK(var), __VA_ARGS__); break;             \
+        case 4: hipLaunchKernelGGL((K<1, false, 3>), G, Z_BLOCK(var), __VA_ARGS__); break;            \
+        case 5: hipLaunchKernelGGL((K<1, true, 3>), G, Z_BLOCK(var), __VA_ARGS__); break;             \
+        case 6: hipLaunchKernelGGL((K<1, false, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;            \
+        case 7: hipLaunchKernelGGL((K<1, true, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;             \
+        case 8: hipLaunchKernelGGL((K<2, false, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;            \
+        case 9: hipLaunchKernelGGL((K<2, true, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;             \
         default: return hipErrorInvalidValue;                                                        \
     }
 
@@ -1034,17 +1047,19 @@ hipError_t launch_helmholtz(const HelmArgs& a, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, 3);
     g.z = nm;
     const size_t lds = helm_lds_bytes(a);
-    const int var = zvar(a.hF);
+    const int var = helm_variant(a);
     const bool sf32 = a.s_f32 != 0;
-    HM_CASES(k_helmholtz, g, dim3(256), lds, s, a)
+    HM_CASES(k_helmholtz, g, lds, s, a)
     return hipGetLastError();
 }
 
-#define PS_CASES(K, ...)                                                                             \
+#define PS_CASES(K, G, ...)                                                                          \
     switch (var) {                                                                                   \
-        case 0: hipLaunchKernelGGL((K<1, true>), __VA_ARGS__); break;                                 \
-        case 1: hipLaunchKernelGGL((K<1, false>), __VA_ARGS__); break;                                \
-        case 2: hipLaunchKernelGGL((K<2, false>), __VA_ARGS__); break;                                \
+        case 0: hipLaunchKernelGGL((K<1, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;                   \
+        case 1: hipLaunchKernelGGL((K<1, 2>), G, Z_BLOCK(var), __VA_ARGS__); break;                   \
+        case 2: hipLaunchKernelGGL((K<1, 3>), G, Z_BLOCK(var), __VA_ARGS__); break;                   \
+        case 3: hipLaunchKernelGGL((K<1, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;                   \
+        case 4: hipLaunchKernelGGL((K<2, 4>), G, Z_BLOCK(var), __VA_ARGS__); break;                   \
         default: return hipErrorInvalidValue;                                                        \
     }
 
@@ -1052,8 +1067,8 @@ hipError_t launch_poisson(const PoisArgs& a, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, 1);
     g.z = nm;
     const size_t lds = pois_lds_bytes(a);
-    const int var = zvar(a.pF);
-    PS_CASES(k_poisson, g, dim3(256), lds, s, a)
+    const int var = pois_variant(a);
+    PS_CASES(k_poisson, g, lds, s, a)
     return hipGetLastError();
 }
 hipError_t launch_poisson_mode0(const Mode0Args& a, int nm, hipStream_t s) {
@@ -1079,19 +1094,19 @@ hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map,
                              hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     const dim3 g(nt, mode == Z_VORT ? 1 : nf);
-    ZD_CASES(k_zderiv_rag, g, dim3(256), lds, s, d, map)
+    ZD_CASES(k_zderiv_rag, g, lds, s, d, map)
     return hipGetLastError();
 }
 hipError_t launch_helmholtz_rag(int var, bool sf32, const HelmArgs* d, const Tile* map, int nt, size_t lds,
                                 hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     const dim3 g(nt, 3);
-    HM_CASES(k_helmholtz_rag, g, dim3(256), lds, s, d, map)
+    HM_CASES(k_helmholtz_rag, g, lds, s, d, map)
     return hipGetLastError();
 }
 hipError_t launch_poisson_rag(int var, const PoisArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
     if (nt <= 0) return hipSuccess;
-    PS_CASES(k_poisson_rag, dim3(nt), dim3(256), lds, s, d, map)
+    PS_CASES(k_poisson_rag, dim3(nt), lds, s, d, map)
     return hipGetLastError();
 }
 hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int nt, hipStream_t s) {
