@@ -499,8 +499,14 @@ extern "C++" FFTArgs dns::ctx_c2r_nl_args(dns_ctx* c) {
     return a;
 }
 
-static int nl_c2r(dns_ctx* c) {
-    HIPCHECK(c, launch_c2r(c->g, C2R_NL, ctx_c2r_nl_args(c), c->nz * c->B, 6, c->mixed, c->stream));
+// The fused step transforms the interior planes only: the update uses S = u x omega at the
+// interior nodes alone (the wall rows of the Helmholtz solve are the boundary values), so the
+// wall planes of the six physical fields and of S are not needed.  The stage API
+// (dns_compute_source_terms) computes every plane, as the reference does.
+static int nl_c2r(dns_ctx* c, bool walls) {
+    FFTArgs a = ctx_c2r_nl_args(c);
+    a.zin = walls ? 0 : c->nz - 2;
+    HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, (walls ? c->nz : c->nz - 2) * c->B, 6, c->mixed, c->stream));
     return 0;
 }
 
@@ -515,8 +521,10 @@ extern "C++" FFTArgs dns::ctx_r2c_cross_args(dns_ctx* c) {
     return b;
 }
 
-static int nl_r2c(dns_ctx* c) {
-    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, ctx_r2c_cross_args(c), c->nz * c->B, 3, c->mixed, c->stream));
+static int nl_r2c(dns_ctx* c, bool walls) {
+    FFTArgs a = ctx_r2c_cross_args(c);
+    a.zin = walls ? 0 : c->nz - 2;
+    HIPCHECK(c, launch_r2c(c->g, R2C_CROSS, a, (walls ? c->nz : c->nz - 2) * c->B, 3, c->mixed, c->stream));
     return 0;
 }
 
@@ -525,11 +533,15 @@ static int flow_control(dns_ctx* c) {
     return 0;
 }
 
+static int source_terms(dns_ctx* c, bool walls) {
+    int r;
+    if ((r = nl_zderiv(c)) || (r = nl_c2r(c, walls)) || (r = nl_r2c(c, walls)) || (r = flow_control(c))) return r;
+    return 0;
+}
+
 int dns_compute_source_terms(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    int r;
-    if ((r = nl_zderiv(c)) || (r = nl_c2r(c)) || (r = nl_r2c(c)) || (r = flow_control(c))) return r;
-    return 0;
+    return source_terms(c, true);
 }
 
 extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
@@ -611,7 +623,7 @@ int dns_apply_boundary_conditions(dns_ctx* c) {
 
 static int one_step_fused(dns_ctx* c) {
     int r;
-    if ((r = dns_compute_source_terms(c))) return r;
+    if ((r = source_terms(c, false))) return r;
     if ((r = dns_viscous_step(c))) return r;
     return pressure(c, 1);
 }
@@ -853,9 +865,9 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
         HIPCHECK(c, hipEventRecord(c->ev[0], c->stream));
         if ((r = nl_zderiv(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[1], c->stream));
-        if ((r = nl_c2r(c))) return r;
+        if ((r = nl_c2r(c, false))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
-        if ((r = nl_r2c(c))) return r;
+        if ((r = nl_r2c(c, false))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
         if ((r = flow_control(c))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));

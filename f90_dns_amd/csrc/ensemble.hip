@@ -236,7 +236,7 @@ int Ensemble::rebuild() {
                         L.lds = std::max(L.lds, plane_lds_bytes(c->g, c->mixed));
                         for (int z = 0; z < B; z++)
                             if (act[z])
-                                for (int k = 0; k < c->nz; k++) tiles.push_back(Tile{g, z * c->nz + k, z});
+                                for (int k = 1; k < c->nz - 1; k++) tiles.push_back(Tile{g, z * c->nz + k, z});
                     } else if (kind == K_HELM || kind == K_POIS || kind == K_ZVORT) {
                         L.lds = std::max(L.lds, kind == K_HELM   ? helm_lds_bytes(hh[g])
                                                 : kind == K_POIS ? pois_lds_bytes(hp[g])

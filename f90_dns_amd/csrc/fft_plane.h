@@ -148,9 +148,15 @@ template <int S, class V> struct PDFT<8, S, V> {
 // twiddle table exp(-2 pi i k/L); twstep = L / (Ns R).  BFAST: consecutive threads take
 // consecutive transforms (y direction), else consecutive butterflies of one transform.
 // SYNC_MID: the loads read LDS that the stores overwrite; SYNC_END: the stores write LDS.
-template <int R, int NT, int MAXB, int S, bool BFAST, bool SYNC_MID, bool SYNC_END, class V, class LD, class ST>
+// mid() runs after the fetches are issued and before the first finish (work whose own reads
+// then overlap the stage's).
+struct NoMid {
+    __device__ __forceinline__ void operator()() const {}
+};
+template <int R, int NT, int MAXB, int S, bool BFAST, bool SYNC_MID, bool SYNC_END, class V, class LD, class ST,
+          class MID = NoMid>
 __device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, int twstep, const LD& ld,
-                                            const ST& st) {
+                                            const ST& st, const MID& mid = MID{}) {
     const int m = N / R;
     const int total = nb * m;
     const FastDiv dq = fastdiv(BFAST ? nb : m), dn = fastdiv(Ns);
@@ -169,6 +175,7 @@ __device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, 
             for (int r = 0; r < R; r++) raw[q][r] = ld.fetch(b, j + r * m);
         }
     }
+    mid();
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
         const int t = threadIdx.x + q * NT;
@@ -201,19 +208,19 @@ __device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, 
     if (SYNC_END) __syncthreads();
 }
 
-template <int C, bool F32, int S, bool BFAST, class V, class LD, class ST>
+template <int C, bool F32, int S, bool BFAST, class V, class LD, class ST, class MF = NoMid>
 __device__ __forceinline__ void stage_r(int R, int N, int Ns, int nb, const V* tw, int twstep, const LD& ld,
-                                        const ST& st) {
+                                        const ST& st, const MF& mid = MF{}) {
     constexpr int NT = Cls<C>::NT;
     constexpr bool MID = LD::kLds && ST::kLds, END = ST::kLds;
     if (R == 8)
-        plane_stage<8, NT, stage_maxb(C, 8, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st);
+        plane_stage<8, NT, stage_maxb(C, 8, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st, mid);
     else if (R == 4)
-        plane_stage<4, NT, stage_maxb(C, 4, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st);
+        plane_stage<4, NT, stage_maxb(C, 4, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st, mid);
     else if (R == 3)
-        plane_stage<3, NT, stage_maxb(C, 3, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st);
+        plane_stage<3, NT, stage_maxb(C, 3, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st, mid);
     else
-        plane_stage<2, NT, stage_maxb(C, 2, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st);
+        plane_stage<2, NT, stage_maxb(C, 2, F32), S, BFAST, MID, END>(N, Ns, nb, tw, twstep, ld, st, mid);
 }
 
 // element (b, e) of the LDS plane: y direction (b = column, e = row) or x (b = row, e = column)
@@ -235,14 +242,14 @@ template <class V, bool YDIR> struct LdsStore {
 };
 
 // all stages of one direction (plans have at least two): first-stage loads from ld0, last-stage
-// stores to stl, LDS in between
-template <int C, bool F32, int S, bool BFAST, class V, class LD0, class STL>
+// stores to stl, LDS in between; mid as in plane_stage, for the first stage
+template <int C, bool F32, int S, bool BFAST, class V, class LD0, class STL, class MF = NoMid>
 __device__ __forceinline__ void run_direction(const Plan& P, int nb, V* lds, int pitch, const V* tw, int L,
-                                              const LD0& ld0, const STL& stl) {
+                                              const LD0& ld0, const STL& stl, const MF& mid = MF{}) {
     const LdsLoad<V, BFAST> ldl{lds, pitch};
     const LdsStore<V, BFAST> stm{lds, pitch};
     int Ns = P.r[0];
-    stage_r<C, F32, S, BFAST>(Ns, P.n, 1, nb, tw, L / Ns, ld0, stm);
+    stage_r<C, F32, S, BFAST>(Ns, P.n, 1, nb, tw, L / Ns, ld0, stm, mid);
     for (int s = 1; s < P.ns - 1; s++) {
         const int R = P.r[s];
         stage_r<C, F32, S, BFAST>(R, P.n, Ns, nb, tw, L / (Ns * R), ldl, stm);
@@ -424,10 +431,22 @@ __device__ __forceinline__ void c2r_body(const FFTArgs& a, int k, int f, typenam
     T* kx = reinterpret_cast<T*>(c0 + ny);
     const C2RSide<V> sd{c0, kx, kx + a.nxh};
     const size_t p2 = (size_t(k) * a.sstride) / 2;
-    c2r_prologue<NT, KIND, V, SRC>(a, f, p2, twy, sd);
-    __syncthreads();
-    run_direction<C, F32, +1, true>(a.py, M, lds, M, twy, ny, C2RLoad<KIND, V, SRC>{&a, f, p2, sd},
-                                    LdsStore<V, true>{lds, M});
+    // A plain transform runs the prologue inside the first y stage, after the stage's reads are
+    // issued, so that both sets of reads are in flight together (the first stage uses no
+    // twiddles; its finish takes column 0 from the side buffer the prologue writes).  omega_z
+    // and the divergence hold two or three reads per element: their prologue runs first (the
+    // held reads would spill).
+    auto prologue = [&]() {
+        c2r_prologue<NT, KIND, V, SRC>(a, f, p2, twy, sd);
+        __syncthreads();
+    };
+    const C2RLoad<KIND, V, SRC> ld0{&a, f, p2, sd};
+    if constexpr (KIND == K_PLAIN) {
+        run_direction<C, F32, +1, true>(a.py, M, lds, M, twy, ny, ld0, LdsStore<V, true>{lds, M}, prologue);
+    } else {
+        prologue();
+        run_direction<C, F32, +1, true>(a.py, M, lds, M, twy, ny, ld0, LdsStore<V, true>{lds, M});
+    }
     const T inv = T(1) / (T(nx) * T(ny));
     run_direction<C, F32, +1, false>(a.px, ny, lds, M, twx, nx, PackLoad<V>{lds, twx, M},
                                      C2RStore<V, DST>{a.dst[f], (size_t(k) * a.dstride) / 2, M, inv});
@@ -456,11 +475,18 @@ __device__ __forceinline__ void c2r_plane(const FFTArgs& a, int k, int f) {
     }
 }
 
-// grid of xcd_grid(np, nf) blocks over np planes (or ragged tiles) and nf fields
+// plane of item k of a direct launch (FFTArgs::zin)
+__device__ __forceinline__ int item_plane(const FFTArgs& a, int k) {
+    if (a.zin == 0) return k;
+    const int m = k / a.zin;
+    return m * (a.zin + 2) + 1 + (k - m * a.zin);
+}
+
+// grid of xcd_grid(np, nf) blocks over np items (planes, or ragged tiles) and nf fields
 template <int C, int OP, class T>
 __global__ __launch_bounds__(Cls<C>::NT) void k_c2r(FFTArgs a, int np, int nf) {
     int k, f;
-    if (xcd_item(np, nf, k, f)) c2r_plane<C, OP, T>(a, k, f);
+    if (xcd_item(np, nf, k, f)) c2r_plane<C, OP, T>(a, item_plane(a, k), f);
 }
 template <int C, int OP, class T>
 __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map,
@@ -517,17 +543,23 @@ template <class V> struct SplitLoad {
     }
 };
 
-// last y stage of r2c: columns b > 0 to HBM (dealias mask applied), column 0 back to LDS
+// the 2/3 dealias rule (FFTArgs::kxm, kym) when on: is kx index i, ky index j kept
+struct Dealias {
+    bool on; int kxm, kym, ny;
+    __device__ __forceinline__ bool keep(int i, int j) const { return !on || (i <= kxm && min(j, ny - j) <= kym); }
+};
+
+// last y stage of r2c: columns b > 0 to HBM (dealias rule applied), column 0 back to LDS
 template <class V, class D> struct R2CStore {
     static constexpr bool kLds = true;
-    V* lds; void* out; size_t ob2; const unsigned char* mask; int M, nxh;
+    V* lds; void* out; size_t ob2; Dealias dl; int M, nxh;
     __device__ __forceinline__ void operator()(int b, int y, V x) const {
         using U = typename Scalar<V>::type;
         if (b == 0) {
             lds[swz<V>(y * M)] = x;
         } else {
             const size_t e = size_t(y) * nxh + b;
-            if (mask && !mask[e]) x = mk2(U(0), U(0));
+            if (!dl.keep(b, y)) x = mk2(U(0), U(0));
             stc<D>(out, ob2 + e, x);
         }
     }
@@ -547,18 +579,18 @@ __device__ __forceinline__ void r2c_body(const FFTArgs& a, int k, int f, typenam
                                      LdsStore<V, false>{lds, M});
     const size_t ob2 = (size_t(k) * a.dstride) / 2;
     void* out = a.dst[f];
+    const Dealias dl{a.mask != nullptr, a.kxm, a.kym, ny};
     run_direction<C, F32, -1, true>(a.py, M, lds, M, twy, ny, SplitLoad<V>{lds, twx, M},
-                                    R2CStore<V, DST>{lds, out, ob2, a.mask, M, nxh});
+                                    R2CStore<V, DST>{lds, out, ob2, dl, M, nxh});
     // separate column 0: C_y = F0_y + i FM_y with F0, FM Hermitian in y
-    const unsigned char* mask = a.mask;
     for (int y = threadIdx.x; y < ny; y += NT) {
         const V c = lds[swz<V>(y * M)], cr = lds[swz<V>((y == 0 ? 0 : ny - y) * M)];
         const U h = U(0.5);
         V x0 = mk2(h * (c.x + cr.x), h * (c.y - cr.y));
         V xm = mk2(h * (c.y + cr.y), -h * (c.x - cr.x));
         const size_t e = size_t(y) * nxh;
-        if (mask && !mask[e]) x0 = mk2(U(0), U(0));
-        if (mask && !mask[e + M]) xm = mk2(U(0), U(0));
+        if (!dl.keep(0, y)) x0 = mk2(U(0), U(0));
+        if (!dl.keep(M, y)) xm = mk2(U(0), U(0));
         stc<DST>(out, ob2 + e, x0);
         stc<DST>(out, ob2 + e + M, xm);
     }
@@ -580,7 +612,7 @@ __device__ __forceinline__ void r2c_plane(const FFTArgs& a, int k, int f) {
 template <int C, int OP, class T>
 __global__ __launch_bounds__(Cls<C>::NT) void k_r2c(FFTArgs a, int np, int nf) {
     int k, f;
-    if (xcd_item(np, nf, k, f)) r2c_plane<C, OP, T>(a, k, f);
+    if (xcd_item(np, nf, k, f)) r2c_plane<C, OP, T>(a, item_plane(a, k), f);
 }
 template <int C, int OP, class T>
 __global__ __launch_bounds__(Cls<C>::NT) void k_r2c_rag(const FFTArgs* __restrict__ d, const Tile* __restrict__ map,

@@ -62,12 +62,19 @@ struct FFTArgs {
     const void* twy;
     const double* xw;
     const double* yw;
-    const unsigned char* mask;   // r2c: dealias mask (null = none)
+    // r2c: the 2/3 dealias rule, on when mask is set: the table (kx index i + nxh * ky index j,
+    // 1 = kept) for the two-pass kernels; the plane kernels apply the same rule from its bounds
+    // (kept: i <= kxm and min(j, ny - j) <= kym), set by fft_fill
+    const unsigned char* mask;
+    int kxm, kym;
     const void* src[6];
     void* dst[6];
     long long sstride, dstride;
     unsigned src_f32;
     int dst_f32;
+    // zin > 0: item k of a direct launch is plane (k / zin)(zin + 2) + 1 + k % zin, the interior
+    // planes of members of zin + 2 planes (the walls skipped); zin = 0: item k is plane k
+    int zin;
 };
 
 // shape, size classes and stage plans (plans need nx/2 and ny of the form 2^a 3^b)
