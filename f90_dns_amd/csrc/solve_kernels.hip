@@ -18,9 +18,15 @@ static_assert(KC == 16, "a chunk is 16 panel rows");
 
 // waves per SIMD the contraction kernels are compiled for: three with one row-block pair per
 // wave (at most 168 VGPRs; four spilled and measured no faster), two with two pairs per wave
+#ifndef ZW_D1
 #define ZW_D1 3
+#endif
+#ifndef ZW_HELM
 #define ZW_HELM 3
+#endif
+#ifndef ZW_POIS
 #define ZW_POIS 3
+#endif
 #define Z_WAVES(rbw, w) ((rbw) == 1 ? (w) : 2)
 
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
