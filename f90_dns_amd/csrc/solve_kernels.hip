@@ -52,11 +52,12 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
     const bool sval = smode < H;
     const size_t rs = size_t(H2) / 2;
-    const int nfield = MODE == Z_VORT ? 2 : 1;
+    // vorticity: one component per block, by = 0: omega_x from v, 1: omega_y from u (two blocks
+    // per column block fill the CUs: one block doing both left a third of the slots empty)
     const size_t mo = size_t(mb) * a.mstride;
     const double* w_in = a.in[2] + mo;
-    for (int q = 0; q < nfield; q++) {
-        const int f = MODE == Z_VORT ? (q == 0 ? 1 : 0) : by;   // vort: v first, then u
+    {
+        const int f = MODE == Z_VORT ? (by == 0 ? 1 : 0) : by;
         const double2* x2 = reinterpret_cast<const double2*>(a.in[f] + mo) + (sval ? smode : 0);
         // branch-free: mirrored rows (clamped in range), the centre row has na == nb
         struct Raw { double2 xa, xb; };
@@ -147,8 +148,7 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
                     else a.out[o][q] = res;
                 }
             }
-            __syncthreads();
-            continue;
+            return;
         }
         if (a.d1.extra) {
             const int col = cb * BCOLS + int(threadIdx.x);
@@ -1026,7 +1026,7 @@ size_t pois_lds_bytes(const PoisArgs& a) {
     }
 
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
-    dim3 g = col_grid(a.H2, mode == Z_VORT ? 1 : nf);
+    dim3 g = col_grid(a.H2, mode == Z_VORT ? 2 : nf);
     g.z = nm;
     const int var = zd_variant(a);
     const size_t lds = zderiv_lds_bytes(a);
@@ -1099,7 +1099,7 @@ hipError_t launch_flow_control(const FlowArgs& a, int nm, hipStream_t s) {
 hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int nt, int nf, size_t lds,
                              hipStream_t s) {
     if (nt <= 0) return hipSuccess;
-    const dim3 g(nt, mode == Z_VORT ? 1 : nf);
+    const dim3 g(nt, mode == Z_VORT ? 2 : nf);
     ZD_CASES(k_zderiv_rag, g, lds, s, d, map)
     return hipGetLastError();
 }
