@@ -533,15 +533,17 @@ static int flow_control(dns_ctx* c) {
     return 0;
 }
 
-static int source_terms(dns_ctx* c, bool walls) {
+// flow: launch the flow controller here (the stage API); the fused step runs it inside the
+// Helmholtz launch instead
+static int source_terms(dns_ctx* c, bool walls, bool flow) {
     int r;
-    if ((r = nl_zderiv(c)) || (r = nl_c2r(c, walls)) || (r = nl_r2c(c, walls)) || (r = flow_control(c))) return r;
-    return 0;
+    if ((r = nl_zderiv(c)) || (r = nl_c2r(c, walls)) || (r = nl_r2c(c, walls))) return r;
+    return flow ? flow_control(c) : 0;
 }
 
 int dns_compute_source_terms(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    return source_terms(c, true);
+    return source_terms(c, true, true);
 }
 
 extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
@@ -562,10 +564,20 @@ extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
     return h;
 }
 
+// with_flow: the Helmholtz launch also runs the flow controller (the fused step)
+static int viscous(dns_ctx* c, bool with_flow) {
+    HelmArgs h = ctx_helm_args(c);
+    if (with_flow) {
+        h.fl = ctx_flow_args(c);
+        h.flow_fused = 1;
+    }
+    HIPCHECK(c, launch_helmholtz(h, c->B, c->stream));
+    return 0;
+}
+
 int dns_viscous_step(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    HIPCHECK(c, launch_helmholtz(ctx_helm_args(c), c->B, c->stream));
-    return 0;
+    return viscous(c, false);
 }
 
 extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
@@ -577,6 +589,8 @@ extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
     a.us = c->us; a.vs = c->vs; a.ws = c->ws;
     a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph;
     a.phi = c->phi; a.dphi = c->dphi;
+    a.m0 = ctx_mode0_args(c, fused);   // mode (0,0) solved by the Poisson launch
+    a.mode0 = 1;
     return a;
 }
 
@@ -593,8 +607,6 @@ extern "C++" Mode0Args dns::ctx_mode0_args(dns_ctx* c, int fused) {
 static int pressure(dns_ctx* c, int fused) {
     PoisArgs a = ctx_pois_args(c, fused);
     HIPCHECK(c, launch_poisson(a, c->B, c->stream));
-    Mode0Args m = ctx_mode0_args(c, fused);
-    HIPCHECK(c, launch_poisson_mode0(m, c->B, c->stream));
     return 0;
 }
 
@@ -623,8 +635,8 @@ int dns_apply_boundary_conditions(dns_ctx* c) {
 
 static int one_step_fused(dns_ctx* c) {
     int r;
-    if ((r = source_terms(c, false))) return r;
-    if ((r = dns_viscous_step(c))) return r;
+    if ((r = source_terms(c, false, false))) return r;
+    if ((r = viscous(c, true))) return r;
     return pressure(c, 1);
 }
 
@@ -869,15 +881,13 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
         HIPCHECK(c, hipEventRecord(c->ev[2], c->stream));
         if ((r = nl_r2c(c, false))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[3], c->stream));
-        if ((r = flow_control(c))) return r;
+        // the fused step runs the flow controller inside the Helmholtz launch and mode (0,0)
+        // inside the Poisson launch: their slots stay empty
         HIPCHECK(c, hipEventRecord(c->ev[4], c->stream));
-        if ((r = dns_viscous_step(c))) return r;
+        if ((r = viscous(c, true))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
-        PoisArgs pa = ctx_pois_args(c, 1);
-        HIPCHECK(c, launch_poisson(pa, c->B, c->stream));
+        if ((r = pressure(c, 1))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
-        Mode0Args ma = ctx_mode0_args(c, 1);
-        HIPCHECK(c, launch_poisson_mode0(ma, c->B, c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));
         HIPCHECK(c, hipEventSynchronize(c->ev[7]));
         for (int i = 0; i < NK; i++) {

@@ -19,7 +19,8 @@
 
 namespace dns {
 
-enum StageKind { K_ZVORT = 0, K_C2R, K_R2C, K_FLOW, K_HELM, K_POIS, K_MODE0, K_COUNT };
+// the flow controller runs inside K_HELM and mode (0,0) inside K_POIS
+enum StageKind { K_ZVORT = 0, K_C2R, K_R2C, K_HELM, K_POIS, K_COUNT };
 
 // Tunable launch shapes (chosen by Ensemble::autotune or set by hand).
 struct EnsTuning {
@@ -70,10 +71,8 @@ private:
     hipEvent_t fork = nullptr, join = nullptr;
     FFTArgs *d_c2r = nullptr, *d_r2c = nullptr;
     ZArgs* d_z = nullptr;
-    FlowArgs* d_fl = nullptr;
     HelmArgs* d_h = nullptr;
     PoisArgs* d_p = nullptr;
-    Mode0Args* d_m0 = nullptr;
     Tile* d_tiles = nullptr;
     size_t cap_groups = 0, cap_tiles = 0;
     std::vector<Launch> launches;

@@ -57,7 +57,7 @@ Ensemble::~Ensemble() {
     for (auto e : ev) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
     if (join) (void)hipEventDestroy(join);
-    void* bufs[] = {d_c2r, d_r2c, d_z, d_fl, d_h, d_p, d_m0, d_tiles};
+    void* bufs[] = {d_c2r, d_r2c, d_z, d_h, d_p, d_tiles};
     for (void* q : bufs)
         if (q) (void)hipFree(q);
     for (auto& g : groups)
@@ -155,10 +155,8 @@ int Ensemble::rebuild() {
     const int G = int(groups.size());
     std::vector<FFTArgs> hc2r(G), hr2c(G);
     std::vector<ZArgs> hz(G);
-    std::vector<FlowArgs> hfl(G);
     std::vector<HelmArgs> hh(G);
     std::vector<PoisArgs> hp(G);
-    std::vector<Mode0Args> hm(G);
     for (int g = 0; g < G; g++) {
         std::memset(&hc2r[g], 0, sizeof(FFTArgs));
         std::memset(&hr2c[g], 0, sizeof(FFTArgs));
@@ -169,10 +167,10 @@ int Ensemble::rebuild() {
         hr2c[g] = ctx_r2c_cross_args(c);
         fft_fill(c->g, hr2c[g], c->mixed);
         hz[g] = ctx_vort_args(c);
-        hfl[g] = ctx_flow_args(c);
         hh[g] = ctx_helm_args(c);
-        hp[g] = ctx_pois_args(c, 1);
-        hm[g] = ctx_mode0_args(c, 1);
+        hh[g].fl = ctx_flow_args(c);
+        hh[g].flow_fused = 1;   // the flow controller runs in the Helmholtz launch
+        hp[g] = ctx_pois_args(c, 1);   // mode (0,0) is solved in the Poisson launch
     }
     // lane assignment: balance an estimate of the work (points x (log2 plane + nz/8))
     std::vector<double> load(2, 0.0);
@@ -251,24 +249,6 @@ int Ensemble::rebuild() {
                                 if (act[z])
                                     for (int x = 0; x < ncb; x++) tiles.push_back(Tile{g, x, z});
                         }
-                    } else if (kind == K_FLOW) {
-                        for (int z = 0; z < B; z++)
-                            if (act[z]) tiles.push_back(Tile{g, 0, z});
-                    } else if (kind == K_MODE0) {
-                        for (int z = 0; z < B; z++)
-                            if (act[z])
-                                for (int k = 0; k < c->nz; k++) tiles.push_back(Tile{g, k, z});
-                    } else {
-                        const int ncb = (c->H2 + BCOLS - 1) / BCOLS;
-                        if (tune.interleave) {
-                            for (int x = 0; x < ncb; x++)
-                                for (int z = 0; z < B; z++)
-                                    if (act[z]) tiles.push_back(Tile{g, x, z});
-                        } else {
-                            for (int z = 0; z < B; z++)
-                                if (act[z])
-                                    for (int x = 0; x < ncb; x++) tiles.push_back(Tile{g, x, z});
-                        }
                     }
                 }
                 L.count = int(tiles.size()) - L.off;
@@ -281,17 +261,13 @@ int Ensemble::rebuild() {
         size_t cg = 0;
         ECHECK(grow(&d_r2c, cg, G)); cg = 0;
         ECHECK(grow(&d_z, cg, G)); cg = 0;
-        ECHECK(grow(&d_fl, cg, G)); cg = 0;
         ECHECK(grow(&d_h, cg, G)); cg = 0;
-        ECHECK(grow(&d_p, cg, G)); cg = 0;
-        ECHECK(grow(&d_m0, cg, G));
+        ECHECK(grow(&d_p, cg, G));
         ECHECK(hipMemcpy(d_c2r, hc2r.data(), G * sizeof(FFTArgs), hipMemcpyHostToDevice));
         ECHECK(hipMemcpy(d_r2c, hr2c.data(), G * sizeof(FFTArgs), hipMemcpyHostToDevice));
         ECHECK(hipMemcpy(d_z, hz.data(), G * sizeof(ZArgs), hipMemcpyHostToDevice));
-        ECHECK(hipMemcpy(d_fl, hfl.data(), G * sizeof(FlowArgs), hipMemcpyHostToDevice));
         ECHECK(hipMemcpy(d_h, hh.data(), G * sizeof(HelmArgs), hipMemcpyHostToDevice));
         ECHECK(hipMemcpy(d_p, hp.data(), G * sizeof(PoisArgs), hipMemcpyHostToDevice));
-        ECHECK(hipMemcpy(d_m0, hm.data(), G * sizeof(Mode0Args), hipMemcpyHostToDevice));
     }
     if (!tiles.empty()) {
         ECHECK(grow(&d_tiles, cap_tiles, tiles.size()));
@@ -308,10 +284,8 @@ int Ensemble::launch(const Launch& L, hipStream_t s) {
         case K_ZVORT: e = launch_zderiv_rag(L.cls, Z_VORT, d_z, map, L.count, 1, L.lds, s); break;
         case K_C2R: e = launch_c2r_rag(L.cls, C2R_NL, L.f32 != 0, d_c2r, map, L.count, L.nf, L.lds, s); break;
         case K_R2C: e = launch_r2c_rag(L.cls, R2C_CROSS, L.f32 != 0, d_r2c, map, L.count, L.nf, L.lds, s); break;
-        case K_FLOW: e = launch_flow_control_rag(d_fl, map, L.count, s); break;
         case K_HELM: e = launch_helmholtz_rag(L.cls, L.f32 != 0, d_h, map, L.count, L.lds, s); break;
         case K_POIS: e = launch_poisson_rag(L.cls, d_p, map, L.count, L.lds, s); break;
-        case K_MODE0: e = launch_poisson_mode0_rag(d_m0, map, L.count, s); break;
         default: e = hipErrorInvalidValue;
     }
     if (e != hipSuccess) {

@@ -169,6 +169,28 @@ hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStre
 size_t zderiv_lds_bytes(const ZArgs& a);
 
 struct FlowState;
+struct FlowMember;
+struct FlowArgs {
+    int nz;
+    double dt, nxny, wsum;
+    const double* uh;            // u-hat: reads Re u(0,0,k) at stride H2
+    long long H2, mstride;
+    const double* wz;
+    const FlowMember* mem;
+    FlowState* st;
+};
+
+struct Mode0Args {
+    long long mstride;
+    int nz, H2;
+    const double* e0;            // row-major nz x nz: phi(0,0,:) = E0 w*(0,0,:)
+    const double* f0;            // row-major nz x nz: D1 phi(0,0,:) = F0 w*(0,0,:)
+    double dt;
+    int fused;
+    const double* us; const double* vs; const double* ws;
+    double* u; double* v; double* w; double* p;
+    double* phi; double* dphi;
+};
 struct HelmArgs {
     long long mstride, fstride;  // member strides of the fields and of hF.A (per-member Re)
     int nz, m, H2, nxh, ny, nx;
@@ -183,6 +205,10 @@ struct HelmArgs {
     const double* ph;
     int s_f32;
     const FlowState* flow;       // per member: current pressure gradient G
+    // flow_fused: the block of column block 0, component 0 runs the flow controller (fl) for its
+    // member before reading G (the fused step; the stage API launches it separately)
+    FlowArgs fl;
+    int flow_fused;
 };
 hipError_t launch_helmholtz(const HelmArgs& a, int nmembers, hipStream_t s);
 inline int helm_variant(const HelmArgs& a) {
@@ -203,6 +229,10 @@ struct PoisArgs {
     const double* us; const double* vs; const double* ws;   // u*, v*, w* spectral
     double* u; double* v; double* w; double* p;             // fused outputs
     double* phi; double* dphi;                              // stage outputs
+    // mode0: wave 0 of the block of column block cb also solves mode (0,0) at the nodes
+    // k = cb, cb + ncb, ... of its member (m0; the Poisson tiles skip that mode)
+    Mode0Args m0;
+    int mode0;
 };
 hipError_t launch_poisson(const PoisArgs& a, int nmembers, hipStream_t s);
 inline int pois_variant(const PoisArgs& a) {
@@ -211,18 +241,6 @@ inline int pois_variant(const PoisArgs& a) {
 }
 size_t pois_lds_bytes(const PoisArgs& a);
 
-struct Mode0Args {
-    long long mstride;
-    int nz, H2;
-    const double* e0;            // row-major nz x nz: phi(0,0,:) = E0 w*(0,0,:)
-    const double* f0;            // row-major nz x nz: D1 phi(0,0,:) = F0 w*(0,0,:)
-    double dt;
-    int fused;
-    const double* us; const double* vs; const double* ws;
-    double* u; double* v; double* w; double* p;
-    double* phi; double* dphi;
-};
-hipError_t launch_poisson_mode0(const Mode0Args& a, int nmembers, hipStream_t s);
 
 struct CorrArgs {
     int nz, H2, nxh, ny, nx;
@@ -245,15 +263,6 @@ struct FlowMember {
     double target_G, target_Ub, gain, re;
     int method, freq, istart, pad;
 };
-struct FlowArgs {
-    int nz;
-    double dt, nxny, wsum;
-    const double* uh;            // u-hat: reads Re u(0,0,k) at stride H2
-    long long H2, mstride;
-    const double* wz;
-    const FlowMember* mem;
-    FlowState* st;
-};
 hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
 
 // ragged variants of the step kernels (var = zd/helm/pois_variant of the groups; lds = dynamic
@@ -263,8 +272,6 @@ hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map,
 hipError_t launch_helmholtz_rag(int var, bool sf32, const HelmArgs* d, const Tile* map, int ntiles, size_t lds,
                                 hipStream_t s);
 hipError_t launch_poisson_rag(int var, const PoisArgs* d, const Tile* map, int ntiles, size_t lds, hipStream_t s);
-hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int ntiles, hipStream_t s);
-hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int ntiles, hipStream_t s);
 
 // physical initial fields of one member from the separable form (setup.h PlanarInit):
 // u = U[k] + scale(u2 wd[k]), v = scale(v2 wd[k]), w = 0, walls zero

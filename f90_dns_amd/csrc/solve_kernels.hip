@@ -223,6 +223,83 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(con
     zderiv_tile<RBW, MODE, NW>(a, t.x, blockIdx.y, t.z, smem);
 }
 
+// apply_flow_control_forcing / update_pressure_gradient_3d / calculate_bulk_velocity_3d
+// (DNS_pressure_BC_3D.f90:1107-1148, 2434-2549), called with istart as in :1118.
+// U_b = sum_k w_k Re u_hat(0,0,k) / (nx ny sum w): the (0,0) coefficient of the
+// unnormalised forward transform is the plane sum.  One wavefront (threads 0-63 of the block).
+__device__ __forceinline__ void flow_tile(const FlowArgs& a, int mb) {
+    const int lane = threadIdx.x;
+    const FlowMember P = a.mem[mb];
+    const double* uh = a.uh + size_t(mb) * a.mstride;
+    // the bulk integral is also formed under constant forcing: a non-finite value there flags
+    // a member whose fields have gone non-finite (any such point reaches mode (0,0) of u
+    // through the product's forward transform within one step)
+    double part = 0.0;
+    for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * a.wz[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if (lane != 0) return;
+    FlowState s = a.st[mb];
+    if (P.method == 1) {
+        s.G = P.target_G;
+        s.Ub = part / (a.nxny * a.wsum);
+    } else if (P.istart % P.freq == 0) {
+        s.Ub = part / (a.nxny * a.wsum);
+        s.err = P.target_Ub - s.Ub;
+        const double kp = P.gain, ki = 0.05 * P.gain;
+        const double gu = 3.0 / P.re + kp * s.err + ki * s.err_int;
+        const double gc = fmax(0.0, fmin(10.0, gu));
+        if (fabs(gc - gu) < 1.0e-12) {
+            if (P.istart <= P.freq) s.err_int = s.err_int + s.err * a.dt;
+            else s.err_int = s.err_int + 0.5 * a.dt * (s.err_prev + s.err);
+        }
+        s.err_prev = s.err;
+        s.G = gc;
+        s.gsum += s.G;
+        s.gcount += 1;
+        s.gavg = s.gsum / double(s.gcount);
+    }
+    a.st[mb] = s;
+}
+
+// mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
+// (DNS_pressure_BC_3D.f90:1702-1711).  Its solution is linear in w*(0,0,:):
+// phi = E0 w*, D1 phi = F0 w*, with E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0
+// precomputed on the host.  Row k by one wavefront (threads 0-63 of the block).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ void mode0_tile(const Mode0Args& a, int k, int mb) {
+    const int nz = a.nz, lane = threadIdx.x;
+    const size_t mo = size_t(mb) * a.mstride;
+    const double* ws = a.ws + mo;
+    double s0 = 0.0, s1 = 0.0, d0 = 0.0, d1 = 0.0;
+    for (int q = lane; q < nz; q += 64) {
+        const double w0 = ws[size_t(q) * a.H2], w1 = ws[size_t(q) * a.H2 + 1];
+        const double e = a.e0[size_t(k) * nz + q], f = a.f0[size_t(k) * nz + q];
+        s0 += e * w0; s1 += e * w1; d0 += f * w0; d1 += f * w1;
+    }
+    s0 = wave_sum(s0); s1 = wave_sum(s1); d0 = wave_sum(d0); d1 = wave_sum(d1);
+    if (lane < 2) {
+        const int part = lane;
+        const double xv = part ? s1 : s0, dx = part ? d1 : d0;
+        const size_t o = mo + size_t(k) * a.H2 + part;
+        if (!a.fused) {
+            a.phi[o] = xv;
+            a.dphi[o] = dx;
+        } else {
+            const bool wall = k == 0 || k == nz - 1;
+            a.u[o] = wall ? 0.0 : a.us[o];
+            a.v[o] = wall ? 0.0 : a.vs[o];
+            a.w[o] = wall ? 0.0 : a.ws[o] - a.dt * dx;
+            a.p[o] = a.p[o] + xv;
+        }
+    }
+}
+
 // -------------------------------------------------------------------------------------
 // Helmholtz, one velocity component per grid.y (solve_viscous_helmholtz,
 // DNS_pressure_BC_3D.f90:1425-1521, with the explicit half of Crank-Nicolson from
@@ -250,7 +327,14 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const double* c2v = a.c2 + size_t(mb) * m;
     const double* hlv = a.hl + size_t(mb) * m;
     const double fco = a.fvec[mb];
-    const double force = comp == 0 ? a.flow[mb].G * a.nxny : 0.0;
+    // the forcing enters mode (0,0) of u only, in column block 0, component 0: that block runs
+    // the flow controller first when it is fused
+    const bool forced = comp == 0 && cb == 0;
+    if (a.flow_fused && forced) {
+        if (threadIdx.x < 64) flow_tile(a.fl, mb);
+        __syncthreads();
+    }
+    const double force = forced ? a.flow[mb].G * a.nxny : 0.0;
     // staging thread: its mode's constants, once
     const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
     const bool sval = smode < H;
@@ -449,11 +533,11 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_ra
 //   z = -y / (dt (lf + k^2)), phi = Bf z, D1 phi = (D1 Bf) z,
 //   u = u* - dt i kx_eff phi, v = v* - dt i ky_eff phi, w = w* - dt D1 phi, p += phi.
 // Folded: even rows on [div_E ; w_O], odd rows on [div_O ; w_E]; phi unfolds from
-// (Bf_E z_E, Bf_O z_O), D1 phi from ((D1 Bf)_O z_O, (D1 Bf)_E z_E).  Mode (0,0) is left to
-// k_poisson_mode0.
+// (Bf_E z_E, Bf_O z_O), D1 phi from ((D1 Bf)_O z_O, (D1 Bf)_E z_E).  Mode (0,0) is solved by
+// mode0_tile (poisson_tile runs it after the tile).
 // -------------------------------------------------------------------------------------
 template <int RBW, int NW>
-__device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
+__device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, double* smem) {
     constexpr bool LX = NW < 4;
     constexpr int NT = 64 * NW;
     const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
@@ -790,6 +874,16 @@ __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, 
     }
 }
 
+// the Poisson tile, then (when fused) wave 0 solves mode (0,0) at nodes cb, cb + ncb, ...
+template <int RBW, int NW>
+__device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
+    poisson_body<RBW, NW>(a, cb, mb, smem);
+    if (a.mode0 && threadIdx.x < 64) {
+        const int ncb = (a.H2 + BCOLS - 1) / BCOLS;
+        for (int k = cb; k < a.nz; k += ncb) mode0_tile(a.m0, k, mb);
+    }
+}
+
 template <int RBW, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_POIS)) void k_poisson(PoisArgs a) {
     extern __shared__ double smem[];
@@ -801,51 +895,6 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_POIS)) void k_poisson_rag(
     const Tile t = map[blockIdx.x];
     const PoisArgs& a = d[t.g];
     poisson_tile<RBW, NW>(a, t.x, t.z, smem);
-}
-
-// mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
-// (DNS_pressure_BC_3D.f90:1702-1711).  Its solution is linear in w*(0,0,:):
-// phi = E0 w*, D1 phi = F0 w*, with E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0
-// precomputed on the host.  One block per row k, one wavefront.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
-}
-
-__device__ __forceinline__ void mode0_tile(const Mode0Args& a, int k, int mb) {
-    const int nz = a.nz, lane = threadIdx.x;
-    const size_t mo = size_t(mb) * a.mstride;
-    const double* ws = a.ws + mo;
-    double s0 = 0.0, s1 = 0.0, d0 = 0.0, d1 = 0.0;
-    for (int q = lane; q < nz; q += 64) {
-        const double w0 = ws[size_t(q) * a.H2], w1 = ws[size_t(q) * a.H2 + 1];
-        const double e = a.e0[size_t(k) * nz + q], f = a.f0[size_t(k) * nz + q];
-        s0 += e * w0; s1 += e * w1; d0 += f * w0; d1 += f * w1;
-    }
-    s0 = wave_sum(s0); s1 = wave_sum(s1); d0 = wave_sum(d0); d1 = wave_sum(d1);
-    if (lane < 2) {
-        const int part = lane;
-        const double xv = part ? s1 : s0, dx = part ? d1 : d0;
-        const size_t o = mo + size_t(k) * a.H2 + part;
-        if (!a.fused) {
-            a.phi[o] = xv;
-            a.dphi[o] = dx;
-        } else {
-            const bool wall = k == 0 || k == nz - 1;
-            a.u[o] = wall ? 0.0 : a.us[o];
-            a.v[o] = wall ? 0.0 : a.vs[o];
-            a.w[o] = wall ? 0.0 : a.ws[o] - a.dt * dx;
-            a.p[o] = a.p[o] + xv;
-        }
-    }
-}
-
-__global__ __launch_bounds__(64) void k_poisson_mode0(Mode0Args a) { mode0_tile(a, blockIdx.x, blockIdx.y); }
-__global__ __launch_bounds__(64) void k_poisson_mode0_rag(const Mode0Args* __restrict__ d, const Tile* __restrict__ map) {
-    const Tile t = map[blockIdx.x];
-    const Mode0Args& a = d[t.g];
-    mode0_tile(a, t.x, t.z);
 }
 
 // projection of the stage API (pressure_correction_3d :1903-1929), from stored phi, D1 phi
@@ -880,51 +929,7 @@ __global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, long long ms
     }
 }
 
-// apply_flow_control_forcing / update_pressure_gradient_3d / calculate_bulk_velocity_3d
-// (DNS_pressure_BC_3D.f90:1107-1148, 2434-2549), called with istart as in :1118.
-// U_b = sum_k w_k Re u_hat(0,0,k) / (nx ny sum w): the (0,0) coefficient of the
-// unnormalised forward transform is the plane sum.
-__device__ __forceinline__ void flow_tile(const FlowArgs& a, int mb) {
-    const int lane = threadIdx.x;
-    const FlowMember P = a.mem[mb];
-    const double* uh = a.uh + size_t(mb) * a.mstride;
-    // the bulk integral is also formed under constant forcing: a non-finite value there flags
-    // a member whose fields have gone non-finite (any such point reaches mode (0,0) of u
-    // through the product's forward transform within one step)
-    double part = 0.0;
-    for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * a.wz[k];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-    if (lane != 0) return;
-    FlowState s = a.st[mb];
-    if (P.method == 1) {
-        s.G = P.target_G;
-        s.Ub = part / (a.nxny * a.wsum);
-    } else if (P.istart % P.freq == 0) {
-        s.Ub = part / (a.nxny * a.wsum);
-        s.err = P.target_Ub - s.Ub;
-        const double kp = P.gain, ki = 0.05 * P.gain;
-        const double gu = 3.0 / P.re + kp * s.err + ki * s.err_int;
-        const double gc = fmax(0.0, fmin(10.0, gu));
-        if (fabs(gc - gu) < 1.0e-12) {
-            if (P.istart <= P.freq) s.err_int = s.err_int + s.err * a.dt;
-            else s.err_int = s.err_int + 0.5 * a.dt * (s.err_prev + s.err);
-        }
-        s.err_prev = s.err;
-        s.G = gc;
-        s.gsum += s.G;
-        s.gcount += 1;
-        s.gavg = s.gsum / double(s.gcount);
-    }
-    a.st[mb] = s;
-}
-
 __global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) { flow_tile(a, blockIdx.x); }
-__global__ __launch_bounds__(64) void k_flow_control_rag(const FlowArgs* __restrict__ d, const Tile* __restrict__ map) {
-    const Tile t = map[blockIdx.x];
-    const FlowArgs& a = d[t.g];
-    flow_tile(a, t.z);
-}
 
 // explicit roundings (no contraction into fma) reproduce the host generator bit for bit
 __global__ __launch_bounds__(256) void k_init_fields(InitArgs a) {
@@ -1077,10 +1082,6 @@ hipError_t launch_poisson(const PoisArgs& a, int nm, hipStream_t s) {
     PS_CASES(k_poisson, g, lds, s, a)
     return hipGetLastError();
 }
-hipError_t launch_poisson_mode0(const Mode0Args& a, int nm, hipStream_t s) {
-    hipLaunchKernelGGL(k_poisson_mode0, dim3(a.nz, nm), dim3(64), 0, s, a);
-    return hipGetLastError();
-}
 hipError_t launch_correction(const CorrArgs& a, hipStream_t s) {
     const size_t total = size_t(a.nz) * (a.H2 / 2);
     int blocks = int(std::min<size_t>((total + 255) / 256, 4096));
@@ -1113,16 +1114,6 @@ hipError_t launch_helmholtz_rag(int var, bool sf32, const HelmArgs* d, const Til
 hipError_t launch_poisson_rag(int var, const PoisArgs* d, const Tile* map, int nt, size_t lds, hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     PS_CASES(k_poisson_rag, dim3(nt), lds, s, d, map)
-    return hipGetLastError();
-}
-hipError_t launch_poisson_mode0_rag(const Mode0Args* d, const Tile* map, int nt, hipStream_t s) {
-    if (nt <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_poisson_mode0_rag, dim3(nt), dim3(64), 0, s, d, map);
-    return hipGetLastError();
-}
-hipError_t launch_flow_control_rag(const FlowArgs* d, const Tile* map, int nt, hipStream_t s) {
-    if (nt <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_flow_control_rag, dim3(nt), dim3(64), 0, s, d, map);
     return hipGetLastError();
 }
 // plain copy of n doubles (calibration of the HBM byte counters)

@@ -23,7 +23,6 @@ STEP_NAMES = [
     (r"k_r2c(_rag)?<\d+, 1,", "r2c_cross"),
     (r"k_flow_control", "flow_control"),
     (r"k_helmholtz", "helmholtz"),
-    (r"k_poisson_mode0", "poisson_mode0"),
     (r"k_poisson(_rag)?<", "poisson_fused"),
 ]
 

@@ -15,8 +15,10 @@ int dns_debug_spectral(dns_ctx* ctx, int which, double* out);
 /* out = D1 in on one spectral field (host arrays), through the MFMA z-contraction. */
 int dns_debug_zderiv(dns_ctx* ctx, const double* spec_in, double* spec_out);
 /* Run nsteps eager fused steps with HIP events between the kernel launches on the context
- * stream; out[i] = mean duration in ms of kernel i (7 kernels, names from
- * dns_profile_kernel_name).  Advances the simulation like dns_step. */
+ * stream; out[i] = mean duration in ms of kernel i (7 slots, names from
+ * dns_profile_kernel_name).  The fused step runs the flow controller inside the Helmholtz
+ * launch and mode (0,0) inside the Poisson launch, so those two slots read ~0.  Advances the
+ * simulation like dns_step. */
 int dns_profile_step(dns_ctx* ctx, int nsteps, double* out);
 const char* dns_profile_kernel_name(int i);
 /* Lane layout of the FP64 MFMA used by the contractions: 4096 single products with one-hot
