@@ -153,15 +153,32 @@ template <int S, class V> struct PDFT<8, S, V> {
 struct NoMid {
     __device__ __forceinline__ void operator()() const {}
 };
-template <int R, int NT, int MAXB, int S, bool BFAST, bool SYNC_MID, bool SYNC_END, class V, class LD, class ST,
-          class MID = NoMid>
-__device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, int twstep, const LD& ld,
-                                            const ST& st, const MID& mid = MID{}) {
+// the fetch half of a stage: the reads of the thread's MAXB butterflies
+template <int R, int NT, int MAXB, bool BFAST, class LD>
+__device__ __forceinline__ void stage_fetch(int N, int nb, const LD& ld, typename LD::Raw (&raw)[MAXB][R]) {
+    const int m = N / R;
+    const int total = nb * m;
+    const FastDiv dq = fastdiv(BFAST ? nb : m);
+#pragma unroll
+    for (int q = 0; q < MAXB; q++) {
+        const int t = threadIdx.x + q * NT;
+        int b, j;
+        if (BFAST) { j = fdiv(t, dq); b = t - j * nb; }
+        else { b = fdiv(t, dq); j = t - b * m; }
+        if (t < total) {
+#pragma unroll
+            for (int r = 0; r < R; r++) raw[q][r] = ld.fetch(b, j + r * m);
+        }
+    }
+}
+// the rest of a stage from its reads: finish, twiddles, butterflies, stores
+template <int R, int NT, int MAXB, int S, bool BFAST, bool SYNC_MID, bool SYNC_END, class V, class LD, class ST>
+__device__ __forceinline__ void stage_rest(int N, int Ns, int nb, const V* tw, int twstep, const LD& ld,
+                                           const ST& st, const typename LD::Raw (&raw)[MAXB][R]) {
     const int m = N / R;
     const int total = nb * m;
     const FastDiv dq = fastdiv(BFAST ? nb : m), dn = fastdiv(Ns);
     V v[MAXB][R];
-    typename LD::Raw raw[MAXB][R];
     int bi[MAXB], ji[MAXB];
 #pragma unroll
     for (int q = 0; q < MAXB; q++) {
@@ -172,16 +189,7 @@ __device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, 
         bi[q] = b; ji[q] = j;
         if (t < total) {
 #pragma unroll
-            for (int r = 0; r < R; r++) raw[q][r] = ld.fetch(b, j + r * m);
-        }
-    }
-    mid();
-#pragma unroll
-    for (int q = 0; q < MAXB; q++) {
-        const int t = threadIdx.x + q * NT;
-        if (t < total) {
-#pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = ld.finish(bi[q], ji[q] + r * m, raw[q][r]);
+            for (int r = 0; r < R; r++) v[q][r] = ld.finish(b, j + r * m, raw[q][r]);
         }
     }
     if (SYNC_MID) __syncthreads();
@@ -206,6 +214,15 @@ __device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, 
         }
     }
     if (SYNC_END) __syncthreads();
+}
+template <int R, int NT, int MAXB, int S, bool BFAST, bool SYNC_MID, bool SYNC_END, class V, class LD, class ST,
+          class MID = NoMid>
+__device__ __forceinline__ void plane_stage(int N, int Ns, int nb, const V* tw, int twstep, const LD& ld,
+                                            const ST& st, const MID& mid = MID{}) {
+    typename LD::Raw raw[MAXB][R];
+    stage_fetch<R, NT, MAXB, BFAST>(N, nb, ld, raw);
+    mid();
+    stage_rest<R, NT, MAXB, S, BFAST, SYNC_MID, SYNC_END>(N, Ns, nb, tw, twstep, ld, st, raw);
 }
 
 template <int C, bool F32, int S, bool BFAST, class V, class LD, class ST, class MF = NoMid>
@@ -243,13 +260,36 @@ template <class V, bool YDIR> struct LdsStore {
 
 // all stages of one direction (plans have at least two): first-stage loads from ld0, last-stage
 // stores to stl, LDS in between; mid as in plane_stage, for the first stage
-template <int C, bool F32, int S, bool BFAST, class V, class LD0, class STL, class MF = NoMid>
+template <int C, bool F32, int S, bool BFAST, class V, class LD0, class STL, class MF = NoMid, class AF = NoMid>
 __device__ __forceinline__ void run_direction(const Plan& P, int nb, V* lds, int pitch, const V* tw, int L,
-                                              const LD0& ld0, const STL& stl, const MF& mid = MF{}) {
+                                              const LD0& ld0, const STL& stl, const MF& mid = MF{},
+                                              const AF& after_first = AF{}) {
     const LdsLoad<V, BFAST> ldl{lds, pitch};
     const LdsStore<V, BFAST> stm{lds, pitch};
     int Ns = P.r[0];
     stage_r<C, F32, S, BFAST>(Ns, P.n, 1, nb, tw, L / Ns, ld0, stm, mid);
+    after_first();
+    for (int s = 1; s < P.ns - 1; s++) {
+        const int R = P.r[s];
+        stage_r<C, F32, S, BFAST>(R, P.n, Ns, nb, tw, L / (Ns * R), ldl, stm);
+        Ns *= R;
+    }
+    const int R = P.r[P.ns - 1];
+    stage_r<C, F32, S, BFAST>(R, P.n, Ns, nb, tw, L / (Ns * R), ldl, stl);
+}
+
+// run_direction whose first stage, of radix R0 = P.r[0], already has its reads in raw;
+// after_first() runs once that stage is done (raw may then be refilled)
+template <int C, bool F32, int S, bool BFAST, int R0, class V, class LD0, class STL, class AF>
+__device__ __forceinline__ void run_direction_pre(const Plan& P, int nb, V* lds, int pitch, const V* tw, int L,
+                                                  const LD0& ld0,
+                                                  const typename LD0::Raw (&raw)[stage_maxb(C, R0, F32)][R0],
+                                                  const STL& stl, const AF& after_first) {
+    const LdsLoad<V, BFAST> ldl{lds, pitch};
+    const LdsStore<V, BFAST> stm{lds, pitch};
+    stage_rest<R0, Cls<C>::NT, stage_maxb(C, R0, F32), S, BFAST, false, true>(P.n, 1, nb, tw, L / R0, ld0, stm, raw);
+    after_first();
+    int Ns = R0;
     for (int s = 1; s < P.ns - 1; s++) {
         const int R = P.r[s];
         stage_r<C, F32, S, BFAST>(R, P.n, Ns, nb, tw, L / (Ns * R), ldl, stm);
@@ -496,6 +536,120 @@ __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restric
     const Tile t = map[i];
     const FFTArgs& a = d[t.g];
     c2r_plane<C, OP, T>(a, t.x, f);
+}
+
+// ---- c2r of the step, persistent ---------------------------------------------------------------
+// A block per CU walks the step's c2r items (plane, field) in the XCD-aware order of the direct
+// launch: virtual block L = blockIdx.x, blockIdx.x + gridDim.x, ... (gridDim.x a multiple of 8,
+// so item L still runs on XCD L % 8).  The twiddle and wavenumber tables are loaded once.  Once
+// an item's first y stage is done, the reads of the next item's first stage (when it is a plain
+// field) are issued into registers, so they travel while this item's remaining stages run in
+// LDS and its real rows are written.
+
+// wavenumber tables of the side buffer (kx: nx/2 + 1, ky: ny) of the compute precision
+template <int NT, class V>
+__device__ __forceinline__ void c2r_wavenumbers(const FFTArgs& a, const C2RSide<V>& sd) {
+    using U = typename Scalar<V>::type;
+    for (int i = threadIdx.x; i < a.nxh; i += NT) sd.kx[i] = U(a.xw[i]);
+    for (int y = threadIdx.x; y < a.ny; y += NT) sd.ky[y] = U(a.yw[y]);
+}
+
+// column 0 of the combined plane, herm(X_0) + i herm(X_M), into the side buffer (the item part
+// of c2r_prologue; the wavenumbers come from the side buffer)
+template <int NT, int KIND, class V, class S>
+__device__ __forceinline__ void c2r_column0(const FFTArgs& a, int f, size_t p2, const C2RSide<V>& sd) {
+    using U = typename Scalar<V>::type;
+    const int ny = a.ny, M = a.M;
+    const int src0 = KIND == K_PLAIN ? f : 0;
+    for (int y0 = 0; y0 < ny; y0 += NT) {
+        const int y = min(y0 + int(threadIdx.x), ny - 1), yr = y == 0 ? 0 : ny - y;
+        const size_t o[4] = {p2 + size_t(y) * a.nxh, p2 + size_t(yr) * a.nxh, p2 + size_t(y) * a.nxh + M,
+                             p2 + size_t(yr) * a.nxh + M};
+        V uu[4], vv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uu[q] = ldc<V, S>(a.src[src0], o[q]);
+            if (KIND == K_OMZ) vv[q] = ldc<V, S>(a.src[1], o[q]);
+        }
+        V x[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if constexpr (KIND == K_PLAIN) {
+                x[q] = uu[q];
+            } else {
+                const U kx = q < 2 ? sd.kx[0] : sd.kx[M], kyq = (q & 1) ? sd.ky[yr] : sd.ky[y];
+                x[q] = mk2(-kx * vv[q].y - (-kyq * uu[q].y), kx * vv[q].x - kyq * uu[q].x);
+            }
+        }
+        const U h = U(0.5);
+        const V h0 = mk2(h * (x[0].x + x[1].x), h * (x[0].y - x[1].y));
+        const V hm = mk2(h * (x[2].x + x[3].x), h * (x[2].y - x[3].y));
+        sd.c0[y] = mk2(h0.x - hm.y, h0.y + hm.x);
+    }
+}
+
+// virtual block L of xcd_grid(n, nf): its item and field (xcd_item for a given L)
+__device__ __forceinline__ bool xcd_decode(int L, int n, int nf, int& item, int& f) {
+    const int x = L % XCDS, slot = L / XCDS;
+    const int t = slot / nf;
+    f = slot - t * nf;
+    item = t * XCDS + x;
+    return item < n;
+}
+
+// FP64 C2R_NL (u, v, w, omega_x, omega_y plain, omega_z from u, v) over np items, first y stage of
+// radix R0; nv = xcd_grid(np, 6) virtual blocks
+template <int C, int R0>
+__global__ __launch_bounds__(Cls<C>::NT) void k_c2r_nl_pers(FFTArgs a, int np, int nv) {
+    using T = double;
+    using V = typename Vec<T>::type;
+    constexpr int NT = Cls<C>::NT, MB0 = stage_maxb(C, R0, false), NF = 6;
+    extern __shared__ char smem_raw[];
+    V* lds = reinterpret_cast<V*>(smem_raw);
+    const int M = a.M, ny = a.ny, nx = a.nx;
+    V* twy = lds + plane_region(M, ny);
+    V* twx = twy + ny;
+    V* c0 = twx + nx;
+    T* kx = reinterpret_cast<T*>(c0 + ny);
+    const C2RSide<V> sd{c0, kx, kx + a.nxh};
+    load_twiddles<NT>(a, twy);
+    c2r_wavenumbers<NT>(a, sd);
+    __syncthreads();
+    using LdP = C2RLoad<K_PLAIN, V, double>;
+    typename LdP::Raw raw[MB0][R0];
+    bool have = false;   // raw holds the reads of this item's first stage
+    const T inv = T(1) / (T(nx) * T(ny));
+    for (int L = blockIdx.x; L < nv; L += gridDim.x) {
+        int k, f;
+        if (!xcd_decode(L, np, NF, k, f)) continue;
+        const int kp = item_plane(a, k);
+        const size_t p2 = (size_t(kp) * a.sstride) / 2;
+        // the next item's first-stage reads, when it is a plain field
+        auto prefetch = [&]() {
+            int kn, fn;
+            have = xcd_decode(L + gridDim.x, np, NF, kn, fn) && fn < 5;
+            if (have) {
+                const LdP ldn{&a, fn, (size_t(item_plane(a, kn)) * a.sstride) / 2, sd};
+                stage_fetch<R0, NT, MB0, true>(a.py.n, M, ldn, raw);
+            }
+        };
+        if (f < 5) {
+            const LdP ld0{&a, f, p2, sd};
+            if (!have) stage_fetch<R0, NT, MB0, true>(a.py.n, M, ld0, raw);
+            c2r_column0<NT, K_PLAIN, V, double>(a, f, p2, sd);
+            __syncthreads();
+            run_direction_pre<C, false, +1, true, R0>(a.py, M, lds, M, twy, ny, ld0, raw, LdsStore<V, true>{lds, M},
+                                                      prefetch);
+        } else {
+            c2r_column0<NT, K_OMZ, V, double>(a, f, p2, sd);
+            __syncthreads();
+            run_direction<C, false, +1, true>(a.py, M, lds, M, twy, ny, C2RLoad<K_OMZ, V, double>{&a, f, p2, sd},
+                                              LdsStore<V, true>{lds, M}, NoMid{}, prefetch);
+        }
+        run_direction<C, false, +1, false>(a.px, ny, lds, M, twx, nx, PackLoad<V>{lds, twx, M},
+                                           C2RStore<V, double>{a.dst[f], (size_t(kp) * a.dstride) / 2, M, inv});
+        __syncthreads();   // the plane and column 0 are rewritten by the next item
+    }
 }
 
 // ---- r2c ------------------------------------------------------------------------------------

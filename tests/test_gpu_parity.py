@@ -244,6 +244,20 @@ def test_seeded_noise_production_shape_short(dns):
     assert max_err(errs) < 1e-10, errs
 
 
+@pytest.mark.parametrize("nx,ny", [(96, 96), (128, 96), (192, 48)])
+def test_largest_class_step_shapes(dns, nx, ny):
+    """FP64 steps on planes of the largest transform class with other first-stage radices (the
+    step's c2r runs one persistent block per CU there): 4 steps against the oracle."""
+    _say(f"{nx}x{ny}x17 4 steps")
+    kw = dict(CONFIG1, nx=nx, ny=ny, nz=17, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2,
+              perturbation_kind=1, seed=7, noise_kmax=6, perturbation_amplitude=0.05)
+    o, g = pair(**kw)
+    o.step(4); g.step(4)
+    errs = field_errors(g.get_fields(), o.get_fields())
+    _say(f"  {nx}x{ny}x17 after 4 steps {errs}")
+    assert max_err(errs) < 1e-10, errs
+
+
 @pytest.mark.parametrize("kw", [dict(CONFIG1),
                                 dict(CONFIG1, nx=48, ny=36, nz=33, flow_control_method=2, perturbation_kind=1,
                                      seed=4, noise_kmax=5, perturbation_amplitude=0.05, use_dealias_23_rule=0),
