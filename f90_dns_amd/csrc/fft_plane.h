@@ -260,15 +260,13 @@ template <class V, bool YDIR> struct LdsStore {
 
 // all stages of one direction (plans have at least two): first-stage loads from ld0, last-stage
 // stores to stl, LDS in between; mid as in plane_stage, for the first stage
-template <int C, bool F32, int S, bool BFAST, class V, class LD0, class STL, class MF = NoMid, class AF = NoMid>
+template <int C, bool F32, int S, bool BFAST, class V, class LD0, class STL, class MF = NoMid>
 __device__ __forceinline__ void run_direction(const Plan& P, int nb, V* lds, int pitch, const V* tw, int L,
-                                              const LD0& ld0, const STL& stl, const MF& mid = MF{},
-                                              const AF& after_first = AF{}) {
+                                              const LD0& ld0, const STL& stl, const MF& mid = MF{}) {
     const LdsLoad<V, BFAST> ldl{lds, pitch};
     const LdsStore<V, BFAST> stm{lds, pitch};
     int Ns = P.r[0];
     stage_r<C, F32, S, BFAST>(Ns, P.n, 1, nb, tw, L / Ns, ld0, stm, mid);
-    after_first();
     for (int s = 1; s < P.ns - 1; s++) {
         const int R = P.r[s];
         stage_r<C, F32, S, BFAST>(R, P.n, Ns, nb, tw, L / (Ns * R), ldl, stm);
@@ -278,17 +276,15 @@ __device__ __forceinline__ void run_direction(const Plan& P, int nb, V* lds, int
     stage_r<C, F32, S, BFAST>(R, P.n, Ns, nb, tw, L / (Ns * R), ldl, stl);
 }
 
-// run_direction whose first stage, of radix R0 = P.r[0], already has its reads in raw;
-// after_first() runs once that stage is done (raw may then be refilled)
-template <int C, bool F32, int S, bool BFAST, int R0, class V, class LD0, class STL, class AF>
+// run_direction whose first stage, of radix R0 = P.r[0], already has its reads in raw
+template <int C, bool F32, int S, bool BFAST, int R0, class V, class LD0, class STL>
 __device__ __forceinline__ void run_direction_pre(const Plan& P, int nb, V* lds, int pitch, const V* tw, int L,
                                                   const LD0& ld0,
                                                   const typename LD0::Raw (&raw)[stage_maxb(C, R0, F32)][R0],
-                                                  const STL& stl, const AF& after_first) {
+                                                  const STL& stl) {
     const LdsLoad<V, BFAST> ldl{lds, pitch};
     const LdsStore<V, BFAST> stm{lds, pitch};
     stage_rest<R0, Cls<C>::NT, stage_maxb(C, R0, F32), S, BFAST, false, true>(P.n, 1, nb, tw, L / R0, ld0, stm, raw);
-    after_first();
     int Ns = R0;
     for (int s = 1; s < P.ns - 1; s++) {
         const int R = P.r[s];
@@ -542,9 +538,11 @@ __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restric
 // A block per CU walks the step's c2r items (plane, field) in the XCD-aware order of the direct
 // launch: virtual block L = blockIdx.x, blockIdx.x + gridDim.x, ... (gridDim.x a multiple of 8,
 // so item L still runs on XCD L % 8).  The twiddle and wavenumber tables are loaded once.  Once
-// an item's first y stage is done, the reads of the next item's first stage (when it is a plain
-// field) are issued into registers, so they travel while this item's remaining stages run in
-// LDS and its real rows are written.
+// an item's y direction is done, the reads of the next item's first stage (when it is a plain
+// field) are issued into registers, so they travel while this item's x direction runs in LDS
+// and its real rows are written.  (Issued right after the first y stage they stalled the y
+// stages: the kernel spills a few registers, and a spill reload waits for every read issued
+// before it.)
 
 // wavenumber tables of the side buffer (kx: nx/2 + 1, ky: ny) of the compute precision
 template <int NT, class V>
@@ -638,14 +636,14 @@ __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_nl_pers(FFTArgs a, int np, i
             if (!have) stage_fetch<R0, NT, MB0, true>(a.py.n, M, ld0, raw);
             c2r_column0<NT, K_PLAIN, V, double>(a, f, p2, sd);
             __syncthreads();
-            run_direction_pre<C, false, +1, true, R0>(a.py, M, lds, M, twy, ny, ld0, raw, LdsStore<V, true>{lds, M},
-                                                      prefetch);
+            run_direction_pre<C, false, +1, true, R0>(a.py, M, lds, M, twy, ny, ld0, raw, LdsStore<V, true>{lds, M});
         } else {
             c2r_column0<NT, K_OMZ, V, double>(a, f, p2, sd);
             __syncthreads();
             run_direction<C, false, +1, true>(a.py, M, lds, M, twy, ny, C2RLoad<K_OMZ, V, double>{&a, f, p2, sd},
-                                              LdsStore<V, true>{lds, M}, NoMid{}, prefetch);
+                                              LdsStore<V, true>{lds, M});
         }
+        prefetch();
         run_direction<C, false, +1, false>(a.px, ny, lds, M, twx, nx, PackLoad<V>{lds, twx, M},
                                            C2RStore<V, double>{a.dst[f], (size_t(kp) * a.dstride) / 2, M, inv});
         __syncthreads();   // the plane and column 0 are rewritten by the next item
