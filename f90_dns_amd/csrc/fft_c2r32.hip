@@ -1,6 +1,8 @@
 // fft_c2r32.hip -- FP32 instances of the c2r plane kernels: the step's C2R_NL in every
-// FP32 size class (direct and ragged launches) and C2R_PLAIN in the largest class, which
-// runs any plane of the precision (its plans fit every larger class).
+// FP32 size class (direct and ragged launches, persistent in class 3) and C2R_PLAIN in the
+// largest class, which runs any plane of the precision (its plans fit every larger class).
+#include <algorithm>
+
 #include "fft_plane.h"
 
 namespace dns {
@@ -9,6 +11,31 @@ template <int C>
 static hipError_t hot(const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_c2r<C, C2R_NL, float>), dim3(xcd_grid(np, nf)), dim3(Cls<C>::NT), lds, s, a, np, nf);
     return hipGetLastError();
+}
+// The step's c2r of class 3 runs persistent, one block per CU (k_c2r_nl_pers), when its first y
+// stage has a radix the kernel is built for; cus is set by c2r_prepare32.
+#ifndef C2R_PERSIST
+#define C2R_PERSIST 1
+#endif
+static int g_cus = 0;
+template <int R0>
+static hipError_t pers(const FFTArgs& a, int np, size_t lds, hipStream_t s) {
+    const int nv = xcd_grid(np, 6);
+    const int g = std::min(nv, (g_cus / XCDS) * XCDS);
+    hipLaunchKernelGGL((k_c2r_nl_pers<3, R0, float>), dim3(g), dim3(Cls<3>::NT), lds, s, a, np, nv);
+    return hipGetLastError();
+}
+static hipError_t hot3(const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
+    if (C2R_PERSIST && nf == 6 && g_cus >= XCDS) {
+        switch (a.py.r[0]) {
+            case 8: return pers<8>(a, np, lds, s);
+            case 4: return pers<4>(a, np, lds, s);
+            case 3: return pers<3>(a, np, lds, s);
+            case 2: return pers<2>(a, np, lds, s);
+            default: break;
+        }
+    }
+    return hot<3>(a, np, nf, lds, s);
 }
 template <int C>
 static hipError_t hot_rag(const FFTArgs* d, const Tile* map, int nt, int nf, size_t lds, hipStream_t s) {
@@ -26,7 +53,7 @@ hipError_t c2r_launch32(int cls, int op, const FFTArgs& a, int np, int nf, size_
         case 0: return hot<0>(a, np, nf, lds, s);
         case 1: return hot<1>(a, np, nf, lds, s);
         case 2: return hot<2>(a, np, nf, lds, s);
-        case 3: return hot<3>(a, np, nf, lds, s);
+        case 3: return hot3(a, np, nf, lds, s);
         case 4: return hot<4>(a, np, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
@@ -56,7 +83,15 @@ hipError_t c2r_prepare32() {
         reinterpret_cast<const void*>(&k_c2r<4, C2R_NL, float>),
         reinterpret_cast<const void*>(&k_c2r_rag<4, C2R_NL, float>),
         reinterpret_cast<const void*>(&k_c2r<4, C2R_PLAIN, float>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 8, float>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 4, float>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 3, float>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 2, float>),
     };
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&g_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
     for (const void* f : fns) {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
         if (e != hipSuccess) return e;

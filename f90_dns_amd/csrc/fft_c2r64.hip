@@ -22,7 +22,7 @@ template <int R0>
 static hipError_t pers(const FFTArgs& a, int np, size_t lds, hipStream_t s) {
     const int nv = xcd_grid(np, 6);
     const int g = std::min(nv, (g_cus / XCDS) * XCDS);
-    hipLaunchKernelGGL((k_c2r_nl_pers<3, R0>), dim3(g), dim3(Cls<3>::NT), lds, s, a, np, nv);
+    hipLaunchKernelGGL((k_c2r_nl_pers<3, R0, double>), dim3(g), dim3(Cls<3>::NT), lds, s, a, np, nv);
     return hipGetLastError();
 }
 static hipError_t hot3(const FFTArgs& a, int np, int nf, size_t lds, hipStream_t s) {
@@ -84,10 +84,10 @@ hipError_t c2r_prepare64() {
         reinterpret_cast<const void*>(&k_c2r_rag<3, C2R_NL, double>),
         reinterpret_cast<const void*>(&k_c2r<3, C2R_PLAIN, double>),
         reinterpret_cast<const void*>(&k_c2r<3, C2R_DIV, double>),
-        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 8>),
-        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 4>),
-        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 3>),
-        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 2>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 8, double>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 4, double>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 3, double>),
+        reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 2, double>),
     };
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);

@@ -389,13 +389,13 @@ __device__ __forceinline__ void c2r_prologue(const FFTArgs& a, int f, size_t p2,
 // first y stage of c2r: column b, row y of the combined plane; columns b > 0 from HBM (the
 // reads are issued by fetch, the arithmetic of OMZ / DIV is done by finish), column 0 from the
 // side buffer
-template <int KIND, class V, class S> struct C2RRaw { V u; };
-template <class V, class S> struct C2RRaw<K_OMZ, V, S> { V u, v; };
-template <class V, class S> struct C2RRaw<K_DIV, V, S> { V u, v, dw; };
+template <int KIND, class V> struct C2RRaw { V u; };   // converted to the compute type on reading
+template <class V> struct C2RRaw<K_OMZ, V> { V u, v; };
+template <class V> struct C2RRaw<K_DIV, V> { V u, v, dw; };
 
 template <int KIND, class V, class S> struct C2RLoad {
     static constexpr bool kLds = false;
-    using Raw = C2RRaw<KIND, V, S>;
+    using Raw = C2RRaw<KIND, V>;
     const FFTArgs* a; int f; size_t p2; C2RSide<V> sd;
     // branch free: every lane reads its HBM element (column 0 lanes read X_0, unused); finish
     // takes column 0 from the side buffer (the lanes of a row read one address)
@@ -595,13 +595,14 @@ __device__ __forceinline__ bool xcd_decode(int L, int n, int nf, int& item, int&
     return item < n;
 }
 
-// FP64 C2R_NL (u, v, w, omega_x, omega_y plain, omega_z from u, v) over np items, first y stage of
-// radix R0; nv = xcd_grid(np, 6) virtual blocks
-template <int C, int R0>
+// C2R_NL (u, v, w, omega_x, omega_y plain, omega_z from u, v) over np items, first y stage of
+// radix R0; nv = xcd_grid(np, 6) virtual blocks.  T = double: FP64 in and out; T = float: the
+// storage of the mixed step (u, v, w FP64, omega_x, omega_y as src_f32 marks them, FP32 out).
+template <int C, int R0, class T>
 __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_nl_pers(FFTArgs a, int np, int nv) {
-    using T = double;
     using V = typename Vec<T>::type;
-    constexpr int NT = Cls<C>::NT, MB0 = stage_maxb(C, R0, false), NF = 6;
+    constexpr bool F32 = sizeof(T) == 4;
+    constexpr int NT = Cls<C>::NT, MB0 = stage_maxb(C, R0, F32), NF = 6;
     extern __shared__ char smem_raw[];
     V* lds = reinterpret_cast<V*>(smem_raw);
     const int M = a.M, ny = a.ny, nx = a.nx;
@@ -613,39 +614,49 @@ __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_nl_pers(FFTArgs a, int np, i
     load_twiddles<NT>(a, twy);
     c2r_wavenumbers<NT>(a, sd);
     __syncthreads();
-    using LdP = C2RLoad<K_PLAIN, V, double>;
-    typename LdP::Raw raw[MB0][R0];
+    using LdD = C2RLoad<K_PLAIN, V, double>;
+    using LdF = C2RLoad<K_PLAIN, V, float>;
+    C2RRaw<K_PLAIN, V> raw[MB0][R0];
     bool have = false;   // raw holds the reads of this item's first stage
     const T inv = T(1) / (T(nx) * T(ny));
+    // plain field f stored as float (only with FP32 compute)
+    auto src_float = [&](int f) { return F32 && ((a.src_f32 >> f) & 1u) != 0; };
     for (int L = blockIdx.x; L < nv; L += gridDim.x) {
         int k, f;
         if (!xcd_decode(L, np, NF, k, f)) continue;
         const int kp = item_plane(a, k);
         const size_t p2 = (size_t(kp) * a.sstride) / 2;
-        // the next item's first-stage reads, when it is a plain field
-        auto prefetch = [&]() {
-            int kn, fn;
-            have = xcd_decode(L + gridDim.x, np, NF, kn, fn) && fn < 5;
-            if (have) {
-                const LdP ldn{&a, fn, (size_t(item_plane(a, kn)) * a.sstride) / 2, sd};
-                stage_fetch<R0, NT, MB0, true>(a.py.n, M, ldn, raw);
-            }
-        };
         if (f < 5) {
-            const LdP ld0{&a, f, p2, sd};
-            if (!have) stage_fetch<R0, NT, MB0, true>(a.py.n, M, ld0, raw);
-            c2r_column0<NT, K_PLAIN, V, double>(a, f, p2, sd);
+            const LdD ldd{&a, f, p2, sd};
+            const LdF ldf{&a, f, p2, sd};
+            const bool sf = src_float(f);
+            if (!have) {
+                if (sf) stage_fetch<R0, NT, MB0, true>(a.py.n, M, ldf, raw);
+                else stage_fetch<R0, NT, MB0, true>(a.py.n, M, ldd, raw);
+            }
+            if (sf) c2r_column0<NT, K_PLAIN, V, float>(a, f, p2, sd);
+            else c2r_column0<NT, K_PLAIN, V, double>(a, f, p2, sd);
             __syncthreads();
-            run_direction_pre<C, false, +1, true, R0>(a.py, M, lds, M, twy, ny, ld0, raw, LdsStore<V, true>{lds, M});
+            // finish of a plain load converts nothing (the read did): either loader will do
+            run_direction_pre<C, F32, +1, true, R0>(a.py, M, lds, M, twy, ny, ldd, raw, LdsStore<V, true>{lds, M});
         } else {
             c2r_column0<NT, K_OMZ, V, double>(a, f, p2, sd);
             __syncthreads();
-            run_direction<C, false, +1, true>(a.py, M, lds, M, twy, ny, C2RLoad<K_OMZ, V, double>{&a, f, p2, sd},
-                                              LdsStore<V, true>{lds, M});
+            run_direction<C, F32, +1, true>(a.py, M, lds, M, twy, ny, C2RLoad<K_OMZ, V, double>{&a, f, p2, sd},
+                                            LdsStore<V, true>{lds, M});
         }
-        prefetch();
-        run_direction<C, false, +1, false>(a.px, ny, lds, M, twx, nx, PackLoad<V>{lds, twx, M},
-                                           C2RStore<V, double>{a.dst[f], (size_t(kp) * a.dstride) / 2, M, inv});
+        // the next item's first-stage reads, when it is a plain field
+        {
+            int kn, fn;
+            have = xcd_decode(L + gridDim.x, np, NF, kn, fn) && fn < 5;
+            if (have) {
+                const size_t pn = (size_t(item_plane(a, kn)) * a.sstride) / 2;
+                if (src_float(fn)) stage_fetch<R0, NT, MB0, true>(a.py.n, M, LdF{&a, fn, pn, sd}, raw);
+                else stage_fetch<R0, NT, MB0, true>(a.py.n, M, LdD{&a, fn, pn, sd}, raw);
+            }
+        }
+        run_direction<C, F32, +1, false>(a.px, ny, lds, M, twx, nx, PackLoad<V>{lds, twx, M},
+                                         C2RStore<V, T>{a.dst[f], (size_t(kp) * a.dstride) / 2, M, inv});
         __syncthreads();   // the plane and column 0 are rewritten by the next item
     }
 }
