@@ -20,6 +20,7 @@ from f90_dns_amd.profile_step import COPY_DOUBLES
 STEP_NAMES = [
     (r"k_zderiv(_rag)?<\d+, 1", "zderiv_vort"),
     (r"k_c2r(_rag)?<\d+, 1,", "c2r_nl"),
+    (r"k_c2r_nl_pers<", "c2r_nl"),
     (r"k_r2c(_rag)?<\d+, 1,", "r2c_cross"),
     (r"k_flow_control", "flow_control"),
     (r"k_helmholtz", "helmholtz"),
