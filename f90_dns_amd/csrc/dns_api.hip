@@ -833,6 +833,13 @@ int dns_member_divergence(dns_ctx* c, int b, double* max_div) {
 
 int dns_check_divergence(dns_ctx* c, double* max_div) { return dns_member_divergence(c, 0, max_div); }
 
+int dns_grid_nodes(const dns_ctx* c, double* z, double* w) {
+    if (!c) return DNS_EINVAL;
+    if (z) std::copy(c->op.z.begin(), c->op.z.end(), z);
+    if (w) std::copy(c->op.wz.begin(), c->op.wz.end(), w);
+    return 0;
+}
+
 // ---- debug hooks (include/f90dns_debug.h) -------------------------------------------------
 int dns_debug_spectral(dns_ctx* c, int which, double* out) {
     if (!c || !out) return DNS_EINVAL;

@@ -4,7 +4,11 @@
 ! the same order, plus an optional &gpu namelist) and runs the time loop with the reference's
 ! stage subroutine names and call order (DNS_pressure_BC_3D.f90:286-361).  Each stage is a thin
 ! wrapper over the C ABI of include/f90dns.h through ISO_C_BINDING; the fields stay on the GPU and
-! come back only for diagnostics and the final restart file.
+! come back only for diagnostics, the output files and the final restart file.  Every nwrt
+! steps it prints the diagnostic line and writes output_time_series.dat and flow_statistics.dat,
+! every 10 nwrt steps a velocity snapshot and the plane-mean profile, in a directory named after
+! the input file and the start time, as the reference does (DNS_pressure_BC_3D.f90:328-338,
+! 2557-2586, 2664-2694, 2702-2948).
 !
 ! Usage: dns_3d_gpu [input.dat]
 program dns_3d_gpu
@@ -56,6 +60,8 @@ program dns_3d_gpu
     real(wp) :: time, max_divergence
     integer(int64) :: t0, t1, rate
     real(wp), allocatable, target :: u(:,:,:), v(:,:,:), w(:,:,:), p_total(:,:,:), un(:,:,:), vn(:,:,:), wn(:,:,:)
+    real(wp), allocatable, target :: zpts(:), zwts(:)
+    character(len=256) :: output_directory
 
     input_filename = 'input.dat'
     if (command_argument_count() >= 1) call get_command_argument(1, input_filename)
@@ -96,6 +102,9 @@ program dns_3d_gpu
     end if
     call check(c_dns_init_fields(ctx), 'dns_init_fields')
     write(*,'(A,I0,A,I0,A,I0)') ' Grid: ', nx, 'x', ny, 'x', nz
+    allocate(zpts(nz), zwts(nz))
+    call check(c_dns_grid_nodes(ctx, c_loc(zpts), c_loc(zwts)), 'dns_grid_nodes')
+    call initialize_output_files()
     call check_divergence_3d(0)
 
     allocate(u(nx,ny,nz), v(nx,ny,nz), w(nx,ny,nz), p_total(nx,ny,nz))
@@ -118,7 +127,15 @@ program dns_3d_gpu
             call pressure_correction_3d()
             call apply_boundary_conditions_3d()
         end if
-        if (mod(istep, nwrt) == 0) call output_diagnostics_3d(istep)
+        if (mod(istep, nwrt) == 0) then
+            call output_diagnostics_3d(istep)
+            call write_time_series_data(istep)
+            call write_flow_statistics(istep)
+            if (mod(istep, 10 * nwrt) == 0) then
+                call write_velocity_field_snapshot(istep)
+                call write_spanwise_averaged_profile(istep)
+            end if
+        end if
         if (mod(istep, 50) == 0) call check_divergence_3d(istep)
     end do
     call check(c_dns_synchronize(ctx), 'dns_synchronize')
@@ -224,6 +241,136 @@ contains
                 ', dP/dx=', d%pressure_gradient
         end if
     end subroutine output_diagnostics_3d
+
+    ! create_output_directory + initialize_output_files (DNS_pressure_BC_3D.f90:2557-2586,
+    ! 2664-2694): <input file without .dat>_<YYYYMMDD_HHMMSS>, and the two file headers
+    subroutine initialize_output_files()
+        character(len=256) :: base_name
+        character(len=10) :: date_str, time_str
+        integer :: un_
+        call date_and_time(date_str, time_str)
+        base_name = trim(input_filename)
+        if (index(base_name, '.dat') > 0) base_name = base_name(1:index(base_name, '.dat') - 1)
+        output_directory = trim(base_name) // '_' // date_str(1:4) // date_str(5:6) // date_str(7:8) // '_' // &
+                           time_str(1:2) // time_str(3:4) // time_str(5:6)
+        call execute_command_line('mkdir -p ' // trim(output_directory))
+        write(*,'(A)') ' Output directory created:'
+        write(*,'(A,A)') '   ', trim(output_directory)
+        open(newunit=un_, file=trim(output_directory) // '/output_time_series.dat', status='replace')
+        write(un_,'(A)') '# DNS 3D Time Series Data'
+        write(un_,'(A)') '# Columns: step, time, u_max, v_max, w_max, energy, pressure_gradient, bulk_velocity'
+        close(un_)
+        open(newunit=un_, file=trim(output_directory) // '/flow_statistics.dat', status='replace')
+        write(un_,'(A)') '# DNS 3D Flow Statistics Summary'
+        write(un_,'(A)') '# Updated periodically during simulation'
+        close(un_)
+    end subroutine initialize_output_files
+
+    ! write_time_series_data (DNS_pressure_BC_3D.f90:2702-2748): one line per call, the averaged
+    ! G and U_b under method 2, the current G under method 1
+    subroutine write_time_series_data(step)
+        integer, intent(in) :: step
+        type(dns_diag_t) :: d
+        integer :: un_
+        call check(c_dns_diagnostics(ctx, d), 'write_time_series_data')
+        open(newunit=un_, file=trim(output_directory) // '/output_time_series.dat', status='old', position='append')
+        if (flow_control_method == 2) then
+            write(un_,'(I8,7(ES15.6))') step, time, d%umax, d%vmax, d%wmax, d%energy, d%avg_pressure_gradient, &
+                                        d%bulk_velocity
+        else
+            write(un_,'(I8,6(ES15.6))') step, time, d%umax, d%vmax, d%wmax, d%energy, d%pressure_gradient
+        end if
+        close(un_)
+    end subroutine write_time_series_data
+
+    ! write_flow_statistics (DNS_pressure_BC_3D.f90:2849-2948): the file is rewritten each call.
+    ! The wall shear comes from the plane means of u next to each wall (one-sided differences over
+    ! the LGL nodes, as the reference); the bulk velocity is the LGL-weighted plane-mean profile
+    ! integral of the library (the reference's method-1 branch divides the same sum by 2 instead
+    ! of by the sum of the weights, which is 2).
+    subroutine write_flow_statistics(step)
+        integer, intent(in) :: step
+        type(dns_diag_t) :: d
+        real(wp) :: nu, du_dz_lower, du_dz_upper, tau_lower, tau_upper, ut
+        integer :: un_
+        call check(c_dns_diagnostics(ctx, d), 'write_flow_statistics')
+        call check(c_dns_get_fields(ctx, c_loc(u), c_null_ptr, c_null_ptr, c_null_ptr), 'dns_get_fields')
+        nu = 1.0_wp / re
+        du_dz_lower = sum((u(:,:,2) - u(:,:,1)) / (zpts(2) - zpts(1))) / real(nx * ny, wp)
+        du_dz_upper = sum((u(:,:,nz) - u(:,:,nz-1)) / (zpts(nz) - zpts(nz-1))) / real(nx * ny, wp)
+        tau_lower = nu * du_dz_lower
+        tau_upper = nu * du_dz_upper
+        ut = sqrt((abs(tau_lower) + abs(tau_upper)) / 2.0_wp)
+        open(newunit=un_, file=trim(output_directory) // '/flow_statistics.dat', status='replace')
+        write(un_,'(A)') '# DNS 3D Flow Statistics Summary'
+        write(un_,'(A,I0)') '# Time step: ', step
+        write(un_,'(A,F12.6)') '# Time: ', time
+        write(un_,'(A)')
+        write(un_,'(A)') '# Maximum velocities:'
+        write(un_,'(A,ES15.6)') 'u_max = ', d%umax
+        write(un_,'(A,ES15.6)') 'v_max = ', d%vmax
+        write(un_,'(A,ES15.6)') 'w_max = ', d%wmax
+        write(un_,'(A)')
+        write(un_,'(A)') '# Energy and bulk flow:'
+        write(un_,'(A,ES15.6)') 'kinetic_energy = ', d%energy
+        write(un_,'(A,ES15.6)') 'bulk_velocity = ', d%bulk_velocity
+        if (flow_control_method == 2) then
+            write(un_,'(A,ES15.6)') 'pressure_gradient = ', d%avg_pressure_gradient
+        else
+            write(un_,'(A,ES15.6)') 'pressure_gradient = ', d%pressure_gradient
+        end if
+        write(un_,'(A)')
+        write(un_,'(A)') '# Wall shear stress:'
+        write(un_,'(A,ES15.6)') 'tau_wall_lower = ', tau_lower
+        write(un_,'(A,ES15.6)') 'tau_wall_upper = ', tau_upper
+        write(un_,'(A,ES15.6)') 'friction_velocity_avg = ', ut
+        write(un_,'(A,ES15.6)') 're_tau = ', ut * re
+        close(un_)
+    end subroutine write_flow_statistics
+
+    ! write_velocity_field_snapshot (DNS_pressure_BC_3D.f90:2756-2791): every point as text
+    subroutine write_velocity_field_snapshot(step)
+        integer, intent(in) :: step
+        character(len=32) :: filename
+        integer :: i, j, k, un_
+        call check(c_dns_get_fields(ctx, c_loc(u), c_loc(v), c_loc(w), c_null_ptr), 'dns_get_fields')
+        write(filename, '(A,I5.5,A)') 'velocity_field_', step, '.dat'
+        open(newunit=un_, file=trim(output_directory) // '/' // trim(filename), status='replace')
+        write(un_,'(A)') '# DNS 3D Velocity Field Snapshot'
+        write(un_,'(A,I0)') '# Time step: ', step
+        write(un_,'(A,F12.6)') '# Time: ', time
+        write(un_,'(A)') '# Columns: i, j, k, x, y, z, u, v, w'
+        do k = 1, nz
+            do j = 1, ny
+                do i = 1, nx
+                    write(un_,'(3I6,6ES15.6)') i, j, k, real(i - 1, wp) * xlen / real(nx, wp), &
+                        real(j - 1, wp) * ylen / real(ny, wp), zpts(k), u(i,j,k), v(i,j,k), w(i,j,k)
+                end do
+            end do
+        end do
+        close(un_)
+        write(*,'(A,A)') ' Velocity field snapshot written: ', trim(filename)
+    end subroutine write_velocity_field_snapshot
+
+    ! write_spanwise_averaged_profile (DNS_pressure_BC_3D.f90:2799-2841): plane means per node
+    ! (u, v, w as fetched by the snapshot written just before)
+    subroutine write_spanwise_averaged_profile(step)
+        integer, intent(in) :: step
+        character(len=32) :: filename
+        integer :: k, un_
+        write(filename, '(A,I5.5,A)') 'spanwise_profile_', step, '.dat'
+        open(newunit=un_, file=trim(output_directory) // '/' // trim(filename), status='replace')
+        write(un_,'(A)') '# DNS 3D Spanwise-Averaged Velocity Profile'
+        write(un_,'(A,I0)') '# Time step: ', step
+        write(un_,'(A,F12.6)') '# Time: ', time
+        write(un_,'(A)') '# Columns: k, z, u_avg, v_avg, w_avg'
+        do k = 1, nz
+            write(un_,'(I6,4ES15.6)') k, zpts(k), sum(u(:,:,k)) / real(nx * ny, wp), &
+                sum(v(:,:,k)) / real(nx * ny, wp), sum(w(:,:,k)) / real(nx * ny, wp)
+        end do
+        close(un_)
+        write(*,'(A,A)') ' Spanwise profile written: ', trim(filename)
+    end subroutine write_spanwise_averaged_profile
 
     ! check_divergence_3d (DNS_pressure_BC_3D.f90:2171-2190)
     subroutine check_divergence_3d(step)

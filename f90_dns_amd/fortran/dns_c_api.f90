@@ -9,6 +9,7 @@ module dns_c_api
               c_dns_get_fields, c_dns_begin_step, c_dns_compute_source_terms, c_dns_viscous_step, &
               c_dns_solve_pressure, c_dns_pressure_correction, c_dns_apply_boundary_conditions, &
               c_dns_step, c_dns_step_graph, c_dns_synchronize, c_dns_diagnostics, c_dns_check_divergence, &
+              c_dns_grid_nodes, &
               c_dns_fft_plan_create, c_dns_fft_plan_destroy, c_dns_fft_forward, c_dns_fft_backward, &
               dns_error_message
 
@@ -106,6 +107,10 @@ module dns_c_api
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: mx
+        end function
+        integer(c_int) function c_dns_grid_nodes(ctx, z, w) bind(C, name="dns_grid_nodes")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, z, w
         end function
         integer(c_int) function c_dns_fft_plan_create(nx, ny, prec, plan) bind(C, name="dns_fft_plan_create")
             import :: c_int, c_ptr

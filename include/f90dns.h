@@ -118,6 +118,11 @@ int dns_synchronize(dns_ctx* ctx);
 
 int dns_diagnostics(dns_ctx* ctx, dns_diag* d);
 int dns_check_divergence(dns_ctx* ctx, double* max_div);
+/* The wall-normal grid of the context: LGL nodes zpts(nz) on [-1, 1] and weights zwts(nz)
+ * (lgl_nodes_weights, lgl_module.f90:30-80, as setup_spectral_3d stores them,
+ * DNS_pressure_BC_3D.f90:799); either pointer may be NULL.  The driver's flow statistics
+ * (write_flow_statistics, :2903-2922) take wall differences over them. */
+int dns_grid_nodes(const dns_ctx* ctx, double* z, double* w);
 
 /* Per-step device time of the last dns_step / dns_step_graph call, from HIP events
  * recorded on the context stream: min, p50, p99, mean in milliseconds. */

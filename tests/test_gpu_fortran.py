@@ -52,3 +52,65 @@ def test_fortran_driver_reproduces_reference_trace(tmp_path, fused):
     restart = tmp_path / "restart.dat"
     n = 32 * 32 * 17
     assert restart.stat().st_size == 18 + 4 + 16 + 12 + 24 + 7 * 8 * n + 32
+
+
+def test_fortran_driver_output_files(tmp_path):
+    """output_time_series.dat and flow_statistics.dat in <input>_<YYYYMMDD_HHMMSS>/, with the
+    reference's layouts (DNS_pressure_BC_3D.f90:2664-2694, 2702-2748, 2849-2948): the time
+    series carries the printed diagnostics of every nwrt step, the statistics the final state,
+    its wall shear from the restart file's u over the LGL nodes."""
+    import glob
+
+    import numpy as np
+
+    from oracle import oracle as orc
+    exe = os.path.join(BIN, "dns_3d_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    inp = tmp_path / "input.dat"
+    inp.write_text(open(os.path.join(ROOT, "tests", "golden", "trace_input.dat")).read())
+    r = _run([exe, str(inp)], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    dirs = glob.glob(str(tmp_path / "input_????????_??????"))
+    assert len(dirs) == 1, os.listdir(tmp_path)
+    ts = open(os.path.join(dirs[0], "output_time_series.dat")).read().splitlines()
+    assert ts[0] == "# DNS 3D Time Series Data"
+    assert ts[1].startswith("# Columns: step, time, u_max")
+    rows = [l.split() for l in ts[2:]]
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "standard_output_trace.json")))
+    assert len(rows) == len(fx["rows"])
+    for row, want in zip(rows, fx["rows"]):
+        assert len(row) == 7 and int(row[0]) == int(want["step"])
+        got = [float(x) for x in row[2:]]
+        ref = [float(want[k]) for k in ("umax", "vmax", "wmax", "E", "dPdx")]
+        for g, w_ in zip(got, ref):
+            assert abs(g - w_) <= 5e-4 * abs(w_), (row, want)   # the trace holds 5 digits
+    st = {}
+    for l in open(os.path.join(dirs[0], "flow_statistics.dat")).read().splitlines():
+        if " = " in l:
+            k, v = l.split(" = ")
+            st[k.strip()] = float(v)
+    assert st["u_max"] == float(rows[-1][2]) and st["kinetic_energy"] == float(rows[-1][5])
+    n = 32 * 32 * 17
+    raw = (tmp_path / "restart.dat").read_bytes()
+    u = np.frombuffer(raw, dtype="<f8", count=n, offset=18 + 4 + 16 + 12 + 24).reshape(17, 32, 32)
+    z, _ = orc.lgl(17)
+    nu = 1.0 / 500.0
+    lower = nu * np.mean((u[1] - u[0]) / (z[1] - z[0]))
+    upper = nu * np.mean((u[16] - u[15]) / (z[16] - z[15]))
+    assert abs(st["tau_wall_lower"] - lower) <= 1e-6 * abs(lower), (st, lower)
+    assert abs(st["tau_wall_upper"] - upper) <= 1e-6 * abs(upper), (st, upper)
+    ut = np.sqrt((abs(lower) + abs(upper)) / 2)
+    assert abs(st["re_tau"] - ut * 500.0) <= 1e-6 * ut * 500.0
+    # every 10 nwrt steps (here step 10): a snapshot of every point and the plane-mean profile
+    snap = np.loadtxt(os.path.join(dirs[0], "velocity_field_00010.dat"), comments="#")
+    prof = np.loadtxt(os.path.join(dirs[0], "spanwise_profile_00010.dat"), comments="#")
+    assert snap.shape == (n, 9) and prof.shape == (17, 5)
+    i, j, k = snap[:, 0].astype(int), snap[:, 1].astype(int), snap[:, 2].astype(int)
+    assert (i[:32] == np.arange(1, 33)).all() and k[0] == 1 and k[-1] == 17 and j[-1] == 32
+    assert np.allclose(snap[:, 3], (i - 1) * 6.283185307179586 / 32, rtol=1e-6, atol=1e-12)
+    assert np.allclose(snap[:, 5], z[k - 1], rtol=1e-6, atol=1e-12)
+    assert np.allclose(prof[:, 1], z, rtol=1e-6, atol=1e-12)
+    for c in range(3):   # plane means of the snapshot's own (rounded) values
+        means = snap[:, 6 + c].reshape(17, 32 * 32).mean(axis=1)
+        assert np.allclose(prof[:, 2 + c], means, rtol=1e-5, atol=1e-12), c
