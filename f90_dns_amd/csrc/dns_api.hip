@@ -132,8 +132,6 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     for (int b = 0; b < nmembers; b++) {
         const std::string e = check_params(&params[b]);
         if (!e.empty()) return fail(nullptr, DNS_EINVAL, nmembers > 1 ? "member " + std::to_string(b) + ": " + e : e);
-        if (params[b].use_weak_form_divergence)
-            return fail(nullptr, DNS_EUNSUPPORTED, "weak-form pressure RHS is not implemented");
         if (!same_shape(params[0], params[b]))
             return fail(nullptr, DNS_EINVAL,
                         "member " + std::to_string(b) + " differs from member 0 in grid, dt, domain or scheme flags");
@@ -180,7 +178,7 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     for (int b = 0; b < c->B; b++) {
         Operators ob;
         if (!build_operators(c->nz, p->dt, params[b].re, c->theta, p->use_crank_nicolson != 0,
-                             p->lgl_derivative != 0, ob, err)) {
+                             p->lgl_derivative != 0, p->use_weak_form_divergence != 0, ob, err)) {
             g_create_error = err;
             return bail(DNS_ESOLVER);
         }
@@ -392,8 +390,8 @@ int dns_set_member_params(dns_ctx* c, int b, const dns_params* q) {
     if (!same_shape(c->p, *q)) return fail(c, DNS_EINVAL, "member parameters differ in grid, dt, domain or scheme flags");
     Operators ob;
     std::string err;
-    if (!build_operators(c->nz, c->p.dt, q->re, c->theta, c->p.use_crank_nicolson != 0, c->p.lgl_derivative != 0, ob,
-                         err))
+    if (!build_operators(c->nz, c->p.dt, q->re, c->theta, c->p.use_crank_nicolson != 0, c->p.lgl_derivative != 0,
+                         c->p.use_weak_form_divergence != 0, ob, err))
         return fail(c, DNS_ESOLVER, err);
     const int m = c->nz - 2;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -585,7 +583,7 @@ extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
     a.mstride = (long long)c->NS;
     a.nz = c->nz; a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
     a.pF = fold_op(c->op.pF, c->pF); a.pB = fold_op(c->op.pB, c->pB); a.pDB = fold_op(c->op.pDB, c->pDB);
-    a.pl = c->pl; a.dt = c->p.dt; a.fused = fused; a.xw = c->xw; a.yw = c->yw;
+    a.pl = c->pl; a.dt = c->p.dt; a.fused = fused; a.weak = c->p.use_weak_form_divergence != 0; a.xw = c->xw; a.yw = c->yw;
     a.us = c->us; a.vs = c->vs; a.ws = c->ws;
     a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph;
     a.phi = c->phi; a.dphi = c->dphi;
@@ -982,7 +980,7 @@ static void unswizzle(const FoldedA& a, std::vector<double>& m) {
 int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols) {
     Operators op;
     std::string err;
-    if (!build_operators(nz, dt, re, cn ? 0.5 : 1.0, cn != 0, exact != 0, op, err)) {
+    if (!build_operators(nz, dt, re, cn ? 0.5 : 1.0, cn != 0, exact != 0, false, op, err)) {
         g_create_error = err;
         return DNS_ESOLVER;
     }

@@ -28,7 +28,7 @@ namespace dns {
 bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) {
     Operators o;
     if (!build_operators(p.nz, p.dt, p.re, p.use_crank_nicolson ? 0.5 : 1.0, p.use_crank_nicolson != 0,
-                         p.lgl_derivative != 0, o, err))
+                         p.lgl_derivative != 0, p.use_weak_form_divergence != 0, o, err))
         return false;
     const size_t nxh = p.nx / 2 + 1, H2 = 2 * nxh * p.ny, nz = p.nz, m = nz - 2;
     const size_t NS = H2 * nz, N = size_t(p.nx) * p.ny * nz;

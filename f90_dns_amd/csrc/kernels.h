@@ -225,6 +225,7 @@ struct PoisArgs {
     const double* pl;            // even eigenvalues first
     double dt;
     int fused;                   // 1: apply correction + walls; 0: write phi, D1 phi only
+    int weak;                    // weak-form RHS: div = -(i xw u* + i yw v*) on the raw wavenumbers
     const double* xw; const double* yw;
     const double* us; const double* vs; const double* ws;   // u*, v*, w* spectral
     double* u; double* v; double* w; double* p;             // fused outputs

@@ -331,7 +331,7 @@ struct OpBase {
     int nE = 0, nO = 0, L1 = 0, L2 = 0;
 };
 
-static bool build_base(int nz, double dt, bool exact_d1, OpBase& base, std::string& err) {
+static bool build_base(int nz, double dt, bool exact_d1, bool weak, OpBase& base, std::string& err) {
     Operators& op = base.op;
     if (nz < 5) { err = "nz must be >= 5"; return false; }
     op.nz = nz; op.m = nz - 2;
@@ -359,6 +359,14 @@ static bool build_base(int nz, double dt, bool exact_d1, OpBase& base, std::stri
     auto A = [&](int r, int c) -> ld { return (ld)op.stiff[r + size_t(nz) * c]; };
     auto D1 = [&](int r, int c) -> ld { return (ld)op.d1[r + size_t(nz) * c]; };
     auto D2 = [&](int r, int c) -> ld { return (ld)op.d2[r + size_t(nz) * c]; };
+    // operator on w* in the pressure RHS, b = W (div_h + Gw w*) / dt: D1 for the strong-form
+    // divergence (compute_divergence_3d, DNS_pressure_BC_3D.f90:1858-1879); for the weak form,
+    // whose z term is -sum_m D1(m,k) w_m w*_m (solve_pressure_3d_weak_form,
+    // DNS_pressure_BC_3D.f90:1792-1803), Gw = -W^{-1} D1^T W.  Both are centro-antisymmetric
+    // (D1 is, and w is symmetric), so the same even/odd folding applies.
+    auto Gw = [&](int r, int c) -> ld {
+        return weak ? -D1(c, r) * (ld)op.wz[c] / (ld)op.wz[r] : D1(r, c);
+    };
     std::vector<ld> sw(nz), isw(nz);
     for (int k = 0; k < nz; k++) { sw[k] = std::sqrt((ld)op.wz[k]); isw[k] = 1.0L / sw[k]; }
     const int hF = nz / 2, rF = nz % 2, LE = hF + rF;   // full-column fold
@@ -444,10 +452,10 @@ static bool build_base(int nz, double dt, bool exact_d1, OpBase& base, std::stri
             const size_t row = size_t(j) * K;
             for (int k = 0; k < hF; k++) F[row + k] = double(q(k) * sw[k]);
             if (rF && even) F[row + hF] = double(q(hF) * sw[hF]);
-            // (Tf D1)(j, c): odd in c for even j (acts on w_O), even for odd j (acts on w_E)
+            // (Tf Gw)(j, c): odd in c for even j (acts on w_O), even for odd j (acts on w_E)
             for (int c = 0; c < hF + (even ? 0 : rF); c++) {
                 ld acc = 0;
-                for (int i = 0; i < nz; i++) acc += q(i) * sw[i] * D1(i, c);
+                for (int i = 0; i < nz; i++) acc += q(i) * sw[i] * Gw(i, c);
                 F[row + LE + c] = double(acc);
             }
         }
@@ -506,7 +514,7 @@ static bool build_base(int nz, double dt, bool exact_d1, OpBase& base, std::stri
             for (int q = 0; q < nz; q++) {
                 ld acc = 0;
                 for (int j = 1; j < nz; j++)
-                    acc += -inv[size_t(k - 1) * (nz - 1) + (j - 1)] * (ld)op.wz[j] * D1(j, q);
+                    acc += -inv[size_t(k - 1) * (nz - 1) + (j - 1)] * (ld)op.wz[j] * Gw(j, q);
                 e0[size_t(k) * nz + q] = acc / (ld)dt;
             }
         op.e0.resize(e0.size());
@@ -523,14 +531,14 @@ static bool build_base(int nz, double dt, bool exact_d1, OpBase& base, std::stri
 }
 
 static std::mutex g_base_mu;
-static std::map<std::tuple<int, uint64_t, int>, std::shared_ptr<const OpBase>> g_base;
+static std::map<std::tuple<int, uint64_t, int, int>, std::shared_ptr<const OpBase>> g_base;
 
-bool build_operators(int nz, double dt, double re, double theta, bool cn, bool exact_d1, Operators& op,
+bool build_operators(int nz, double dt, double re, double theta, bool cn, bool exact_d1, bool weak, Operators& op,
                      std::string& err) {
     if (nz < 5) { err = "nz must be >= 5"; return false; }
     uint64_t dtbits;
     std::memcpy(&dtbits, &dt, sizeof(dt));
-    const auto key = std::make_tuple(nz, dtbits, exact_d1 ? 1 : 0);
+    const auto key = std::make_tuple(nz, dtbits, exact_d1 ? 1 : 0, weak ? 1 : 0);
     std::shared_ptr<const OpBase> base;
     {
         std::lock_guard<std::mutex> lk(g_base_mu);
@@ -539,7 +547,7 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
     }
     if (!base) {
         auto b = std::make_shared<OpBase>();
-        if (!build_base(nz, dt, exact_d1, *b, err)) return false;
+        if (!build_base(nz, dt, exact_d1, weak, *b, err)) return false;
         std::lock_guard<std::mutex> lk(g_base_mu);
         g_base[key] = b;
         base = b;

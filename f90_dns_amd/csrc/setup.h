@@ -75,7 +75,9 @@ FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector
 //    on [div_O ; w_E]; phi unfolds from (p = Bf_E z_E, q = Bf_O z_O), D1 phi from
 //    (s = (D1 Bf)_O z_O, r = (D1 Bf)_E z_E).  Mode (0,0): phi_0 = 0, phi_{1:} = M0 b_{1:} with
 //    M0 = -(A_{1:,1:})^{-1} and b = w * (D1 w*) / dt, i.e. phi = E0 w*, D1 phi = F0 w* with
-//    E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0.
+//    E0 = [0; M0 W_{1:} D1_{1:,:}] / dt and F0 = D1 E0.  The weak-form RHS replaces D1 on w*
+//    by Gw = -W^{-1} D1^T W (in Tf D1 and E0) and div by -(i kx u* + i ky v*) on the raw
+//    wavenumbers.
 //  z derivative: D1 x unfolds from (s = D1 x_odd on X_O, r = D1 x_even on X_E).
 struct Operators {
     int nz = 0, m = 0;
@@ -93,7 +95,8 @@ struct Operators {
     std::vector<double> e0, f0;  // nz x nz row-major: mode-(0,0) phi = E0 w*, D1 phi = F0 w*
     double helm_recon_err = 0, pois_recon_err = 0;   // max |reconstruction - operator|, relative
 };
-bool build_operators(int nz, double dt, double re, double theta, bool crank_nicolson, bool exact_d1,
+// weak: the pressure RHS of the weak form (w* enters through -W^{-1} D1^T W instead of D1)
+bool build_operators(int nz, double dt, double re, double theta, bool crank_nicolson, bool exact_d1, bool weak,
                      Operators& op, std::string& err);
 
 // exp(-2 pi i k / n), k = 0..n-1, as interleaved (re, im)

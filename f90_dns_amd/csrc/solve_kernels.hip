@@ -529,7 +529,8 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_ra
 // Pressure (solve_pressure_3d_standard :1654-1733 with compute_divergence_3d :1858-1879)
 // and, when fused, the projection + wall values (pressure_correction_3d :1903-1929,
 // apply_boundary_conditions_3d :1953-1972):
-//   y = Tf div + (Tf D1) w*,  div = i kx_eff u* + i ky_eff v*,
+//   y = Tf div + (Tf D1) w*,  div = i kx_eff u* + i ky_eff v*  (the weak form,
+//   solve_pressure_3d_weak_form :1792-1803: div = -(i xw u* + i yw v*), D1 -> -W^{-1} D1^T W),
 //   z = -y / (dt (lf + k^2)), phi = Bf z, D1 phi = (D1 Bf) z,
 //   u = u* - dt i kx_eff phi, v = v* - dt i ky_eff phi, w = w* - dt D1 phi, p += phi.
 // Folded: even rows on [div_E ; w_O], odd rows on [div_O ; w_E]; phi unfolds from
@@ -552,8 +553,8 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     double skx = 0.0, sky = 0.0;
     if (sval) {
         const int j = smode / a.nxh, i = smode - j * a.nxh;
-        skx = kx_eff(a.xw, i, a.nx);
-        sky = ky_eff(a.yw, i, j, a.nx, a.ny);
+        skx = a.weak ? -a.xw[i] : kx_eff(a.xw, i, a.nx);
+        sky = a.weak ? -a.yw[j] : ky_eff(a.yw, i, j, a.nx, a.ny);
     }
     const double2* us2 = reinterpret_cast<const double2*>(us) + (sval ? smode : 0);
     const double2* vs2 = reinterpret_cast<const double2*>(vs) + (sval ? smode : 0);

@@ -46,7 +46,7 @@ typedef struct dns_params {
     double xlen, ylen, re, dt;          /* &simulation xlen, ylen, re; &time_control dt */
     int use_crank_nicolson;             /* &simulation: theta = 0.5 if nonzero, else 1 */
     int use_dealias_23_rule;            /* &computational */
-    int use_weak_form_divergence;       /* &pressure_solver: must be 0 (DNS_EUNSUPPORTED) */
+    int use_weak_form_divergence;       /* &pressure_solver: 1 = weak-form pressure RHS (:1749-1835) */
     int flow_control_method;            /* &flow_control: 1 = constant G, 2 = PI on U_bulk */
     double target_pressure_gradient;
     double target_bulk_velocity;

@@ -674,13 +674,25 @@ static void divergence(orc_ctx* c, const double* fx, const double* fy, const dou
     for (size_t n = 0; n < c->N; n++) div[n] = div[n] + dyf[n] + dzf[n];
 }
 
-/* solve_pressure_3d_standard (DNS_pressure_BC_3D.f90:1654-1733) */
+/* solve_pressure_3d_standard (DNS_pressure_BC_3D.f90:1654-1733), or with
+ * use_weak_form_divergence the RHS of solve_pressure_3d_weak_form (:1749-1835): per mode
+ * rhs_k = [-(i xw u^ + i yw v^)_k w_k ybar/2 - sum_m D1(m,k) w^_m w_m 2/ybar] / dt from the
+ * transformed u*, v*, w* (:1789-1802), with the same matrix and mode-(1,1) treatment. */
 void orc_solve_pressure(orc_ctx* c) {
     int nz = c->nz, nxhp = c->nxhp, ny = c->ny, nspl = c->nspl;
+    const int weak = c->p.use_weak_form_divergence;
     double* div = c->f7;
-    divergence(c, c->ustar, c->vstar, c->wstar, div);
-    for (size_t n = 0; n < c->N; n++) div[n] = div[n] / c->p.dt;
-    r2c_field(c, div, c->spec);
+    cpx *vh = NULL, *wh = NULL;
+    if (weak) {
+        vh = (cpx*)calloc(c->NS, sizeof(cpx)); wh = (cpx*)calloc(c->NS, sizeof(cpx));
+        r2c_field(c, c->ustar, c->spec);
+        r2c_field(c, c->vstar, vh);
+        r2c_field(c, c->wstar, wh);
+    } else {
+        divergence(c, c->ustar, c->vstar, c->wstar, div);
+        for (size_t n = 0; n < c->N; n++) div[n] = div[n] / c->p.dt;
+        r2c_field(c, div, c->spec);
+    }
     int cached = !c->p.refactor_every_step;
     if (cached && !c->pready) {
         c->plu = dalloc((size_t)nz * nz * nspl); c->ppiv = (int*)calloc((size_t)nz * nspl, sizeof(int));
@@ -710,10 +722,30 @@ void orc_solve_pressure(orc_ctx* c) {
                 if (cached) memcpy(a, pm, sizeof(double) * (size_t)nz * nz);
                 { int info = lu_factor(nz, a, pv); if (info && !c->status) c->status = info; /* the reference stops here */ }
             }
-            for (int k = 0; k < nz; k++) {
-                cpx r = c->spec[m0 + (size_t)nspl * k];
-                double sc = 0.5 * c->ybar * c->wz[k];
-                br[k] = r.re * sc; bi[k] = r.im * sc;
+            if (weak) {
+                const double kx = c->xw[i], ky = c->yw[j];
+                for (int k = 0; k < nz; k++) {
+                    const cpx u = c->spec[m0 + (size_t)nspl * k], v = vh[m0 + (size_t)nspl * k];
+                    /* cmplx(0, xw) u^ + cmplx(0, yw) v^, negated, then times w_k and ybar/2 */
+                    const double hr = (0.0 * u.re - kx * u.im) + (0.0 * v.re - ky * v.im);
+                    const double hi = (0.0 * u.im + kx * u.re) + (0.0 * v.im + ky * v.re);
+                    double rr = (-hr) * c->wz[k] * (c->ybar / 2.0), ri = (-hi) * c->wz[k] * (c->ybar / 2.0);
+                    double sr = 0.0, si = 0.0;
+                    for (int m = 0; m < nz; m++) {
+                        const cpx wm = wh[m0 + (size_t)nspl * m];
+                        sr += c->d1[m + nz * k] * wm.re * c->wz[m];
+                        si += c->d1[m + nz * k] * wm.im * c->wz[m];
+                    }
+                    rr = rr - sr * (2.0 / c->ybar);
+                    ri = ri - si * (2.0 / c->ybar);
+                    br[k] = rr / c->p.dt; bi[k] = ri / c->p.dt;
+                }
+            } else {
+                for (int k = 0; k < nz; k++) {
+                    cpx r = c->spec[m0 + (size_t)nspl * k];
+                    double sc = 0.5 * c->ybar * c->wz[k];
+                    br[k] = r.re * sc; bi[k] = r.im * sc;
+                }
             }
             if (i == 0 && j == 0) { br[0] = 0.0; bi[0] = 0.0; }
             lu_solve(nz, a, pv, br); lu_solve(nz, a, pv, bi);
@@ -722,7 +754,7 @@ void orc_solve_pressure(orc_ctx* c) {
     }
     if (cached) c->pready = 1;
     c2r_field(c, c->spec2, c->phi);
-    free(pm); free(piv); free(br); free(bi);
+    free(pm); free(piv); free(br); free(bi); free(vh); free(wh);
 }
 
 /* pressure_correction_3d (DNS_pressure_BC_3D.f90:1903-1929); grad p_total is kept

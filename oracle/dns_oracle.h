@@ -43,6 +43,7 @@ typedef struct orc_params {
     /* oracle execution mode */
     int refactor_every_step;       /* 1: rebuild + LU every mode every step, as written */
     int lgl_derivative;            /* 0: reference D1 (1e-15 denominator threshold), 1: exact LGL D1 */
+    int use_weak_form_divergence;  /* 1: solve_pressure_3d_weak_form (DNS_pressure_BC_3D.f90:1749-1835) */
 } orc_params;
 
 typedef struct orc_diag {

@@ -27,6 +27,7 @@ class OrcParams(ctypes.Structure):
         ("analytical_mode_x", ctypes.c_int), ("analytical_mode_y", ctypes.c_int),
         ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
         ("refactor_every_step", ctypes.c_int), ("lgl_derivative", ctypes.c_int),
+        ("use_weak_form_divergence", ctypes.c_int),
     ]
 
 
@@ -103,7 +104,7 @@ DEFAULTS = dict(
     controller_update_freq=7, istart=0, u_centerline=1.0,
     enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1, analytical_mode_y=1,
     perturbation_amplitude=0.01, seed=0, noise_kmax=8, refactor_every_step=0,
-    lgl_derivative=0,
+    lgl_derivative=0, use_weak_form_divergence=0,
 )
 
 

@@ -25,14 +25,19 @@ def test_fortran_fft_module_known_answers(tmp_path):
     assert r.returncode == 0 and "FFT MODULE PASSED" in r.stdout
 
 
+@pytest.mark.parametrize("weak", [False, True])
 @pytest.mark.parametrize("fused", [False, True])
-def test_fortran_driver_reproduces_reference_trace(tmp_path, fused):
-    """Same input as the reference run of standard_output.txt; every printed diagnostic row must
-    equal the reference's (standard_output.txt:88-104)."""
+def test_fortran_driver_reproduces_reference_trace(tmp_path, fused, weak):
+    """Same input as the reference runs of standard_output.txt and weak_form_output.txt (the
+    same namelist with use_weak_form_divergence = .true.); every printed diagnostic row must
+    equal the reference's (:88-104 of each)."""
     exe = os.path.join(BIN, "dns_3d_gpu")
     if not os.path.exists(exe):
         pytest.skip("Fortran driver not built")
     src = open(os.path.join(ROOT, "tests", "golden", "trace_input.dat")).read()
+    if weak:
+        assert "use_weak_form_divergence = .false." in src
+        src = src.replace("use_weak_form_divergence = .false.", "use_weak_form_divergence = .true.")
     if fused:
         src += "&gpu\n  fused_step = .true.\n/\n"
     inp = tmp_path / "input.dat"
@@ -40,7 +45,8 @@ def test_fortran_driver_reproduces_reference_trace(tmp_path, fused):
     r = _run([exe, str(inp)], tmp_path)
     print(r.stdout[-3000:])
     assert r.returncode == 0, r.stdout + r.stderr
-    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "standard_output_trace.json")))
+    trace = "weak_form_output_trace.json" if weak else "standard_output_trace.json"
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", trace)))
     lines = [l for l in r.stdout.splitlines() if re.match(r"\s*Step\s+\d+, t=", l)]
     assert len(lines) == len(fx["rows"])
     for line, row in zip(lines, fx["rows"]):
