@@ -120,6 +120,7 @@ def lib():
         L.dns_debug_mfma_probe.argtypes = [_dp]
         L.dns_debug_mfma_probe_f32.argtypes = [_dp]
         L.dns_debug_operator_error.argtypes = [_vp, _dp]
+        L.dns_debug_zmap.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp]
         L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_profile_kernel_name.argtypes = [ctypes.c_int]
         L.dns_profile_kernel_name.restype = ctypes.c_char_p
@@ -349,6 +350,20 @@ def host_initial_field(**kw):
     if rc != 0:
         raise DnsError(rc, lib().dns_last_error(None).decode())
     return out
+
+
+def zmap(nx, ny):
+    """The mode space of the per-mode solves (include/f90dns_debug.h dns_debug_zmap; host
+    only): (so, s2o, s2s) arrays over its real columns."""
+    H2 = 2 * (nx // 2 + 1) * ny
+    so = np.empty(H2, dtype=np.int32)
+    s2o = np.empty(H2, dtype=np.int32)
+    s2s = np.empty(H2)
+    n = ctypes.c_int()
+    rc = lib().dns_debug_zmap(nx, ny, ctypes.byref(n), so.ctypes.data, s2o.ctypes.data, s2s.ctypes.data)
+    if rc != 0:
+        raise DnsError(rc, "dns_debug_zmap")
+    return so[:n.value], s2o[:n.value], s2s[:n.value]
 
 
 def estimate_bytes(nmembers=1, **kw):

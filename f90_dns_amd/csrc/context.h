@@ -29,6 +29,8 @@ struct dns_ctx {
     unsigned char* mask = nullptr;
     double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr, *fvec = nullptr;
     double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
+    int4* ztab = nullptr;         // mode space of the per-mode solves: per column (zmap_build)
+    int* zbfl = nullptr;          //   and per column block
     // fields, B members each
     double *uh = nullptr, *vh = nullptr, *wh = nullptr, *ph = nullptr;
     double *us = nullptr, *vs = nullptr, *ws = nullptr;

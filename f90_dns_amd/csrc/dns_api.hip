@@ -216,6 +216,16 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
         (r = upload(c, &c->fvec, fv)) || (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) ||
         (r = upload(c, &c->pDB, o.pDB.data)) || (r = upload(c, &c->pl, o.pl)))
         return bail(r);
+    {
+        const ZMap z = zmap_make(c->nx, c->ny);
+        std::vector<int4> zt(size_t(2) * z.P);
+        std::vector<int> zb(zmap_blocks(z));
+        if (!zmap_build(z, reinterpret_cast<int*>(zt.data()), zb.data())) {
+            g_create_error = "mode space of the per-mode solves does not come out even";
+            return bail(DNS_EINVAL);
+        }
+        if ((r = upload(c, &c->ztab, zt)) || (r = upload(c, &c->zbfl, zb))) return bail(r);
+    }
     const size_t NSB = c->NS * c->B, NB = c->N * c->B;
     double** specs[] = {&c->uh, &c->vh, &c->wh, &c->ph, &c->us, &c->vs, &c->ws, &c->ox, &c->oy,
                         &c->sh[0], &c->sh[1], &c->sh[2], &c->phi, &c->dphi, &c->tmp};
@@ -559,6 +569,9 @@ extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
     h.ph = c->ph;
     h.s_f32 = c->mixed ? 1 : 0;
     h.flow = c->flow;
+    h.zm = zmap_make(c->nx, c->ny);
+    h.zm.tab = c->ztab;
+    h.zm.bfl = c->zbfl;
     return h;
 }
 
@@ -589,6 +602,9 @@ extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
     a.phi = c->phi; a.dphi = c->dphi;
     a.m0 = ctx_mode0_args(c, fused);   // mode (0,0) solved by the Poisson launch
     a.mode0 = 1;
+    a.zm = zmap_make(c->nx, c->ny);
+    a.zm.tab = c->ztab;
+    a.zm.bfl = c->zbfl;
     return a;
 }
 
@@ -975,6 +991,20 @@ static void unswizzle(const FoldedA& a, std::vector<double>& m) {
                                                                    : get(r / 16, r, c);
     for (int r = 0; r < a.rows1; r++)
         for (int c = 0; c < a.K; c++) m[size_t(a.rows0 + r) * a.K + c] = get(a.pb + r / 16, r, c);
+}
+
+int dns_debug_zmap(int nx, int ny, int* ncols, int* so, int* s2o, double* s2s) {
+    if (nx < 2 || ny < 1 || nx % 2 || !ncols || !so || !s2o || !s2s) return DNS_EINVAL;
+    const ZMap z = zmap_make(nx, ny);
+    std::vector<int> t(size_t(8) * z.P), b(zmap_blocks(z));
+    if (!zmap_build(z, t.data(), b.data())) return DNS_EINVAL;
+    *ncols = 2 * z.P;
+    for (int c = 0; c < 2 * z.P; c++) {
+        so[c] = t[4 * c];
+        s2o[c] = t[4 * c + 1];
+        s2s[c] = s2o[c] < 0 ? 0.0 : s2o[c] == so[c] + 1 ? 0.0 : (so[c] & 1) ? -1.0 : 1.0;
+    }
+    return 0;
 }
 
 int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int which, double* out, int* rows, int* cols) {

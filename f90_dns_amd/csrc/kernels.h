@@ -168,6 +168,108 @@ inline int zd_variant(const ZArgs& a) {
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
 size_t zderiv_lds_bytes(const ZArgs& a);
 
+// Mode space walked by the Helmholtz and Poisson tiles.  The fields are transforms of real
+// planes, and the step keeps them so: on the kx = 0 and kx = nx/2 columns the modes
+// ky > ny/2 are the conjugates of ky < ny/2, and the four self-conjugate modes (0,0),
+// (0,ny/2), (nx/2,0), (nx/2,ny/2) are real.  With on (ny even), the tiles walk P = (nx/2) ny
+// complex slots, the independent coefficients only.  Slot q is storage mode q wherever that
+// mode is independent and not self-conjugate, so a block's slots keep the storage alignment;
+// the other slots below P take, in order, two paired slots (the real parts of (0,0) with
+// (0,ny/2), and of (nx/2,0) with (nx/2,ny/2)), then the independent modes at or beyond P.  A
+// tile stores each mirrored mode as the conjugate of its slot and zeroes the imaginary parts
+// of the self-conjugate modes.  At 128 x 128 that is 512 column blocks instead of 520.
+// Without on, P = (nx/2+1) ny and slot q is storage mode q.
+constexpr int Z_BLOCK_MODES = 16;   // complex slots per column block
+struct ZMap {
+    int on, nxh, ny, P;
+    const int4* tab;             // device: per column of the 2 P, (so, s2o, i, j) (zmap_build)
+    const int* bfl;              // device: per column block, ZB_SECOND | ZB_PAIRED | ZB_IDENT
+};
+// block flags: a column with a second storage column / a paired slot / every column is its own
+// storage column with no second one (the kernels then skip the table)
+enum { ZB_SECOND = 1, ZB_PAIRED = 2, ZB_IDENT = 4 };
+inline ZMap zmap_make(int nx, int ny) {
+    ZMap z{};
+    z.nxh = nx / 2 + 1;
+    z.ny = ny;
+    z.on = (ny % 2 == 0 && nx >= 4) ? 1 : 0;
+    z.P = z.on ? (nx / 2) * ny : z.nxh * ny;
+    return z;
+}
+__host__ __device__ inline int zmap_blocks(const ZMap& z) { return (z.P + Z_BLOCK_MODES - 1) / Z_BLOCK_MODES; }
+
+// Host: the table of the 2 P columns, int4 (so, s2o, i, j) each: so = the column's storage
+// real column, s2o = -1 or a second storage column (the mirrored mode's part, which receives
+// +v for a real part and -v for an imaginary part, or s2o = so + 1: the zeroed imaginary part
+// of a self-conjugate mode), (i, j) = the column's mode; and the flags of the column blocks.
+// Returns false if the slots do not come out even (not expected for any shape).
+inline bool zmap_build(const ZMap& z, int* tab4, int* bfl) {
+    const int W = z.nxh, ny = z.ny, P = z.P, nb = zmap_blocks(z);
+    for (int cb = 0; cb < nb; cb++) bfl[cb] = 0;
+    auto put = [&](int c, int i, int j, int part, int s2o) {
+        const int so = 2 * (i + W * j) + part;
+        tab4[4 * c] = so; tab4[4 * c + 1] = s2o; tab4[4 * c + 2] = i; tab4[4 * c + 3] = j;
+        if (s2o >= 0) bfl[c / (2 * Z_BLOCK_MODES)] |= ZB_SECOND;
+    };
+    auto ident_flags = [&]() {
+        for (int cb = 0; cb < nb; cb++) {
+            bool id = true;
+            for (int c = cb * 2 * Z_BLOCK_MODES; c < (cb + 1) * 2 * Z_BLOCK_MODES && c < 2 * P; c++)
+                id = id && tab4[4 * c] == c && tab4[4 * c + 1] < 0;
+            if (id) bfl[cb] |= ZB_IDENT;
+        }
+    };
+    if (!z.on) {
+        for (int q = 0; q < P; q++)
+            for (int part = 0; part < 2; part++) put(2 * q + part, q % W, q / W, part, -1);
+        ident_flags();
+        return true;
+    }
+    const int h = ny / 2;
+    // 0: independent, 1: mirror of (i, ny - j), 2: self-conjugate
+    auto kind = [&](int i, int j) {
+        const bool edge = i == 0 || i == W - 1;
+        return (edge && (j == 0 || j == h)) ? 2 : (edge && j > h) ? 1 : 0;
+    };
+    auto plain = [&](int q, int i, int j) {
+        const bool mir = (i == 0 || i == W - 1) && j > 0 && j < h;
+        for (int part = 0; part < 2; part++)
+            put(2 * q + part, i, j, part, mir ? 2 * (i + W * (ny - j)) + part : -1);
+    };
+    int nfree = 0, nhome = 0;
+    for (int q = 0; q < P; q++) nfree += kind(q % W, q / W) != 0;
+    for (int M = P; M < W * ny; M++) nhome += kind(M % W, M / W) == 0;
+    if (nfree != nhome + 2) return false;
+    int next = P, paired = 0;   // next storage mode beyond P to place
+    for (int q = 0; q < P; q++) {
+        const int i = q % W, j = q / W;
+        if (kind(i, j) == 0) { plain(q, i, j); continue; }
+        if (paired < 2) {
+            const int pi = paired == 0 ? 0 : W - 1;
+            put(2 * q, pi, 0, 0, 2 * pi + 1);
+            put(2 * q + 1, pi, h, 0, 2 * (pi + W * h) + 1);
+            bfl[q / Z_BLOCK_MODES] |= ZB_PAIRED;
+            paired++;
+            continue;
+        }
+        while (kind(next % W, next / W) != 0) next++;
+        plain(q, next % W, next / W);
+        next++;
+    }
+    ident_flags();
+    return true;
+}
+struct ZCol { int so, s2o, i, j; };
+// device: column col from the table, or (ident: its block has ZB_IDENT) from its index
+__device__ __forceinline__ ZCol zcol_t(const ZMap& z, int col, bool ident) {
+    if (ident) {
+        const int q = col >> 1, j = q / z.nxh;
+        return ZCol{col, -1, q - j * z.nxh, j};
+    }
+    const int4 t = z.tab[col];
+    return ZCol{t.x, t.y, t.z, t.w};
+}
+
 struct FlowState;
 struct FlowMember;
 struct FlowArgs {
@@ -209,6 +311,7 @@ struct HelmArgs {
     // member before reading G (the fused step; the stage API launches it separately)
     FlowArgs fl;
     int flow_fused;
+    ZMap zm;                     // the tiles' mode space
 };
 hipError_t launch_helmholtz(const HelmArgs& a, int nmembers, hipStream_t s);
 inline int helm_variant(const HelmArgs& a) {
@@ -234,6 +337,7 @@ struct PoisArgs {
     // k = cb, cb + ncb, ... of its member (m0; the Poisson tiles skip that mode)
     Mode0Args m0;
     int mode0;
+    ZMap zm;                     // the tiles' mode space
 };
 hipError_t launch_poisson(const PoisArgs& a, int nmembers, hipStream_t s);
 inline int pois_variant(const PoisArgs& a) {

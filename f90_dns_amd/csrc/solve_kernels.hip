@@ -36,6 +36,23 @@ __device__ __forceinline__ double kx_eff(const double* xw, int i, int nx) { retu
 __device__ __forceinline__ double ky_eff(const double* yw, int i, int j, int nx, int ny) {
     return (j == ny / 2 && (i == 0 || i == nx / 2)) ? 0.0 : yw[j];
 }
+static_assert(BMODES == Z_BLOCK_MODES, "column blocks of the tile mode space");
+
+// storage of a tile column kept for its stores: x = so, y = s2o (zcol_t)
+__device__ __forceinline__ int2 zst(const ZCol& z) { return make_int2(z.so, z.s2o); }
+// store v at the storage of a tile column (at row offset ro), and at its second column: the
+// mirrored mode's part (+v real, -v imaginary) or the zeroed imaginary part (s2o = so + 1)
+// (sec: the block holds such columns, block-uniform)
+__device__ __forceinline__ void zstore(double* out, size_t ro, int2 z, double v, bool sec) {
+    out[ro + z.x] = v;
+    if (sec && z.y >= 0) out[ro + z.y] = z.y == z.x + 1 ? 0.0 : ((z.x & 1) ? -v : v);
+}
+// the same at element q = ro + z.x
+__device__ __forceinline__ void zstore_q(double* out, size_t q, int2 z, double v, bool sec) {
+    zstore(out, q - z.x, z, v, sec);
+}
+// the flags of column block cb (ZB_SECOND, ZB_PAIRED), as a wave-uniform value
+__device__ __forceinline__ int zflags(const ZMap& z, int cb) { return __builtin_amdgcn_readfirstlane(z.bfl[cb]); }
 
 // -------------------------------------------------------------------------------------
 // z derivative: out = D1 in  (compute_derivatives_3d dz, DNS_pressure_BC_3D.f90:2348-2358),
@@ -311,11 +328,12 @@ __device__ __forceinline__ void mode0_tile(const Mode0Args& a, int k, int mb) {
 // Folded: even eigen rows on [v_E ; u_E], odd rows on [v_O ; u_O] (v = alpha u_I + dt S'_I over
 // the interior, u over the whole column); u*_I unfolds from p = B_E z_E and q = B_O z_O.
 // -------------------------------------------------------------------------------------
-template <int RBW, bool SF32, int NW>
+// PS: the block holds a paired slot (its two parts are the real parts of two modes)
+template <int RBW, bool SF32, int NW, bool PS>
 __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int comp, int mb, double* smem) {
     constexpr bool LX = NW < 4;
     constexpr int NT = 64 * NW;
-    const int H2 = a.H2, nz = a.nz, m = a.m, H = H2 / 2;
+    const int H2 = a.H2, nz = a.nz, m = a.m, P = a.zm.P, NC = 2 * P;
     const int hI = m / 2, L1 = hI + (m & 1), hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
     const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
@@ -327,27 +345,33 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const double* c2v = a.c2 + size_t(mb) * m;
     const double* hlv = a.hl + size_t(mb) * m;
     const double fco = a.fvec[mb];
-    // the forcing enters mode (0,0) of u only, in column block 0, component 0: that block runs
-    // the flow controller first when it is fused
-    const bool forced = comp == 0 && cb == 0;
+    // the forcing enters mode (0,0) of u only, in the column block holding that mode,
+    // component 0: that block runs the flow controller first when it is fused
+    const bool forced = comp == 0 && cb == 0;   // slot 0 holds mode (0,0)
+    const int bfl = zflags(a.zm, cb);
+    const bool sec = (bfl & ZB_SECOND) != 0, ident = (bfl & ZB_IDENT) != 0;
     if (a.flow_fused && forced) {
         if (threadIdx.x < 64) flow_tile(a.fl, mb);
         __syncthreads();
     }
     const double force = forced ? a.flow[mb].G * a.nxny : 0.0;
-    // staging thread: its mode's constants, once
-    const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
-    const bool sval = smode < H;
-    double alpha = 1.0, fr = 0.0;
+    // staging thread: its slot's constants, once (alpha per part: the parts of a paired slot
+    // are different modes)
+    const int mt = threadIdx.x & 15, slot = cb * BMODES + mt;
+    const bool sval = slot < P;
+    const ZCol zx = zcol_t(a.zm, 2 * min(slot, P - 1), ident), zy = zcol_t(a.zm, 2 * min(slot, P - 1) + 1, ident);
+    double ax = 1.0, ay = 1.0, fr = 0.0;
     if (sval) {
-        const int j = smode / a.nxh, i = smode - j * a.nxh;
-        const double kx = a.xw[i], ky = a.yw[j];
-        alpha = a.cn ? 1.0 - fco * (kx * kx + ky * ky) : 1.0;
-        fr = smode == 0 ? force : 0.0;
+        const double kxx = a.xw[zx.i], kyx = a.yw[zx.j], kxy = a.xw[zy.i], kyy = a.yw[zy.j];
+        ax = a.cn ? 1.0 - fco * (kxx * kxx + kyx * kyx) : 1.0;
+        ay = a.cn ? 1.0 - fco * (kxy * kxy + kyy * kyy) : 1.0;
+        fr = zx.so == 0 ? force : 0.0;
     }
-    const double2* u2 = reinterpret_cast<const double2*>(uh) + (sval ? smode : 0);
-    const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.sh[comp]) + mo) + (sval ? smode : 0);
-    const double2* sd2 = reinterpret_cast<const double2*>(a.sh[comp] + mo) + (sval ? smode : 0);
+    const int smode = sval ? (zx.so >> 1) : 0;   // storage mode of a plain slot
+    const int ox = sval ? zx.so : 0, oy = sval ? zy.so : 0;   // storage columns of the two parts
+    const double2* u2 = reinterpret_cast<const double2*>(uh) + smode;
+    const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.sh[comp]) + mo) + smode;
+    const double2* sd2 = reinterpret_cast<const double2*>(a.sh[comp] + mo) + smode;
     // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
     // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
     // two nodes coincide
@@ -368,9 +392,24 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         int na, nb;
         nodes(kf, na, nb);
         const bool inv = kf < L1;
-        const SV* sa = inv ? &sv2[na * rs] : reinterpret_cast<const SV*>(&u2[na * rs]);
-        const SV* sb = inv ? &sv2[nb * rs] : reinterpret_cast<const SV*>(&u2[nb * rs]);
-        return Raw{u2[na * rs], u2[nb * rs], *sa, *sb};
+        if constexpr (PS) {
+            // two scalar reads per value: a paired slot's parts are the real parts of two modes
+            using ST = typename std::conditional<SF32, float, double>::type;
+            const ST* s1 = reinterpret_cast<const ST*>(a.sh[comp]) + mo;
+            auto ldu = [&](int n) { return make_double2(uh[size_t(n) * H2 + ox], uh[size_t(n) * H2 + oy]); };
+            auto lsv = [&](int n) -> SV {
+                if (!inv) return SV{};
+                SV v;
+                v.x = s1[size_t(n) * H2 + ox];
+                v.y = s1[size_t(n) * H2 + oy];
+                return v;
+            };
+            return Raw{ldu(na), ldu(nb), lsv(na), lsv(nb)};
+        } else {
+            const SV* sa = inv ? &sv2[na * rs] : reinterpret_cast<const SV*>(&u2[na * rs]);
+            const SV* sb = inv ? &sv2[nb * rs] : reinterpret_cast<const SV*>(&u2[nb * rs]);
+            return Raw{u2[na * rs], u2[nb * rs], *sa, *sb};
+        }
     };
     auto finish = [&](int kf, const Raw& rw) -> FoldPair {
         const bool inv = kf < L1;
@@ -379,10 +418,10 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         const double2 ua = rw.ua, ub = rw.ub;
         const double2 sa = make_double2(double(rw.sa.x), double(rw.sa.y));
         const double2 sb = make_double2(double(rw.sb.x), double(rw.sb.y));
-        const double va_r = inv ? alpha * ua.x + a.dt * (sa.x + fr) : ua.x;
-        const double va_i = inv ? alpha * ua.y + a.dt * sa.y : ua.y;
-        const double vb_r = inv ? alpha * ub.x + a.dt * (sb.x + fr) : ub.x;
-        const double vb_i = inv ? alpha * ub.y + a.dt * sb.y : ub.y;
+        const double va_r = inv ? ax * ua.x + a.dt * (sa.x + fr) : ua.x;
+        const double va_i = inv ? ay * ua.y + a.dt * sa.y : ua.y;
+        const double vb_r = inv ? ax * ub.x + a.dt * (sb.x + fr) : ub.x;
+        const double vb_i = inv ? ay * ub.y + a.dt * sb.y : ub.y;
         const double keep = (sval && c < lim) ? 1.0 : 0.0;
         const double pair = c < half ? 1.0 : 0.0;
         FoldPair r;
@@ -395,13 +434,14 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     double k2c[CT], bc1[CT], bc2[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct, cb), colc = min(col, H2 - 1);
-        const int mode = colc >> 1, part = colc & 1, j = mode / a.nxh, i = mode - j * a.nxh;
+        const int col = out_col(ct, cb), colc = min(col, NC - 1);
+        const ZCol zc = zcol_t(a.zm, colc, ident);
+        const int mode = zc.so >> 1, part = zc.so & 1, j = zc.j, i = zc.i;
         const double kx = a.xw[i], ky = a.yw[j];
         const double2 p0 = reinterpret_cast<const double2*>(ph)[mode];
         const double2 p1 = reinterpret_cast<const double2*>(ph + size_t(nz - 1) * H2)[mode];
         const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
-        const bool in = col < H2, lift = in && comp < 2;
+        const bool in = col < NC, lift = in && comp < 2;
         k2c[ct] = in ? kx * kx + ky * ky : 0.0;
         bc1[ct] = lift ? (part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x)) : 0.0;   // -dt * (i ke p)
         bc2[ct] = lift ? (part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x)) : 0.0;
@@ -420,7 +460,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     // contraction orders these writes)
     zero_panel_tail<NW>(z0, F.rows0, ZR);
     zero_panel_tail<NW>(z1, F.rows1, ZR);
-    if (F.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2) {
+    if (F.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < NC) {
         const int row = F.rows0 - 1, ct = threadIdx.x >> 4;
         z0[row * LDB + threadIdx.x] = (xv + c1v[row] * bc1[ct] + c2v[row] * bc2[ct]) / (hlv[row] + fco * k2c[ct]);
     }
@@ -438,7 +478,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
             }
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
-            const bool cv = out_col(ct, cb) < H2;
+            const bool cv = out_col(ct, cb) < NC;
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
                 const int b = wave + NW * ii;
@@ -477,20 +517,26 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
             }
         }
         __syncthreads();
+        // the thread's column is the same for all its elements (NT is a multiple of BCOLS)
+        const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c;
+        const int2 zt = zst(zcol_t(a.zm, min(col, NC - 1), ident));
         for (int e = threadIdx.x; e < nz * BCOLS; e += NT) {
-            const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-            if (col >= H2) continue;
+            const int node = e / BCOLS;
+            if (col >= NC) continue;
             const double v = node == 0 ? bcs[c] : node == nz - 1 ? bcs[BCOLS + c] : unfold_lds(G0, G1, m, node - 1, c);
-            out[size_t(node) * H2 + col] = v;
+            zstore(out, size_t(node) * H2, zt, v, sec);
         }
         return;
     }
-    if (a.hB.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < H2)   // the interior centre row
-        out[size_t(a.hB.rows0) * H2 + cb * BCOLS + threadIdx.x] = extra_dot(a.hB, z0);
+    if (a.hB.extra && threadIdx.x < BCOLS && cb * BCOLS + int(threadIdx.x) < NC)   // the interior centre row
+        zstore(out, size_t(a.hB.rows0) * H2, zst(zcol_t(a.zm, cb * BCOLS + threadIdx.x, ident)), extra_dot(a.hB, z0), sec);
+    int2 zo[CT];   // storage of the lane's output columns (read here, after the contractions)
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) zo[ct] = zst(zcol_t(a.zm, min(out_col(ct, cb), NC - 1), ident));
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb);
-        if (col >= H2) continue;
+        if (col >= NC) continue;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
             const int b = wave + NW * ii;
@@ -501,13 +547,13 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
                 if (row >= L1) continue;
                 const double p = acc0[ii][ct][r];
                 const double q = (row < hI && b < a.hB.rb1) ? acc1[ii][ct][r] : 0.0;
-                out[size_t(row + 1) * H2 + col] = p + q;
-                if (row < hI) out[size_t(m - row) * H2 + col] = p - q;
+                zstore(out, size_t(row + 1) * H2, zo[ct], p + q, sec);
+                if (row < hI) zstore(out, size_t(m - row) * H2, zo[ct], p - q, sec);
             }
         }
         if (wave == 0 && (lane >> 4) == 0) {
-            out[col] = bc1[ct];
-            out[size_t(nz - 1) * H2 + col] = bc2[ct];
+            zstore(out, 0, zo[ct], bc1[ct], sec);
+            zstore(out, size_t(nz - 1) * H2, zo[ct], bc2[ct], sec);
         }
     }
 }
@@ -515,14 +561,16 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
 template <int RBW, bool SF32, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz(HelmArgs a) {
     extern __shared__ double smem[];
-    helmholtz_tile<RBW, SF32, NW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    if (zflags(a.zm, blockIdx.x) & ZB_PAIRED) helmholtz_tile<RBW, SF32, NW, true>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    else helmholtz_tile<RBW, SF32, NW, false>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, bool SF32, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_rag(const HelmArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
-    helmholtz_tile<RBW, SF32, NW>(a, t.x, blockIdx.y, t.z, smem);
+    if (zflags(a.zm, t.x) & ZB_PAIRED) helmholtz_tile<RBW, SF32, NW, true>(a, t.x, blockIdx.y, t.z, smem);
+    else helmholtz_tile<RBW, SF32, NW, false>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // -------------------------------------------------------------------------------------
@@ -537,28 +585,38 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_ra
 // (Bf_E z_E, Bf_O z_O), D1 phi from ((D1 Bf)_O z_O, (D1 Bf)_E z_E).  Mode (0,0) is solved by
 // mode0_tile (poisson_tile runs it after the tile).
 // -------------------------------------------------------------------------------------
-template <int RBW, int NW>
+template <int RBW, int NW, bool PS>
 __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, double* smem) {
     constexpr bool LX = NW < 4;
     constexpr int NT = 64 * NW;
-    const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
+    const int H2 = a.H2, nz = a.nz, P = a.zm.P, NC = 2 * P, hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
     const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
     const double* us = a.us + mo; const double* vs = a.vs + mo; const double* ws = a.ws + mo;
     double* uo = a.u + mo; double* vo = a.v + mo; double* wo = a.w + mo; double* po = a.p + mo;
     double* phio = a.phi + mo; double* dphio = a.dphi + mo;
-    // staging thread's mode
-    const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
-    const bool sval = smode < H && smode != 0;
+    // staging thread's slot; storage mode (0,0) (mode0_tile solves it) is left out part by part
+    // (in a paired slot it is the first part only).  The weak form's horizontal term
+    // -(i xw u* + i yw v*) is taken on the effective wavenumbers: they differ from the raw ones
+    // only where the term is anti-Hermitian in a real field's transform (the kx = nx/2 column,
+    // and ky = ny/2 on the kx = 0 and nx/2 columns), which the reference's c2r of phi drops.
+    const int bfl = zflags(a.zm, cb);
+    const bool sec = (bfl & ZB_SECOND) != 0, ident = (bfl & ZB_IDENT) != 0;
+    const int mt = threadIdx.x & 15, slot = cb * BMODES + mt;
+    const bool sval = slot < P;
+    const ZCol zx = zcol_t(a.zm, 2 * min(slot, P - 1), ident), zy = zcol_t(a.zm, 2 * min(slot, P - 1) + 1, ident);
+    const int kpm = ((sval && zx.so != 0) ? 1 : 0) | ((sval && zy.so != 1) ? 2 : 0);   // parts kept
     double skx = 0.0, sky = 0.0;
     if (sval) {
-        const int j = smode / a.nxh, i = smode - j * a.nxh;
-        skx = a.weak ? -a.xw[i] : kx_eff(a.xw, i, a.nx);
-        sky = a.weak ? -a.yw[j] : ky_eff(a.yw, i, j, a.nx, a.ny);
+        const double sg = a.weak ? -1.0 : 1.0;
+        skx = sg * kx_eff(a.xw, zx.i, a.nx);
+        sky = sg * ky_eff(a.yw, zx.i, zx.j, a.nx, a.ny);
     }
-    const double2* us2 = reinterpret_cast<const double2*>(us) + (sval ? smode : 0);
-    const double2* vs2 = reinterpret_cast<const double2*>(vs) + (sval ? smode : 0);
-    const double2* ws2 = reinterpret_cast<const double2*>(ws) + (sval ? smode : 0);
+    const int smode = sval ? (zx.so >> 1) : 0;
+    const int ox = sval ? zx.so : 0, oy = sval ? zy.so : 0;
+    const double2* us2 = reinterpret_cast<const double2*>(us) + smode;
+    const double2* vs2 = reinterpret_cast<const double2*>(vs) + smode;
+    const double2* ws2 = reinterpret_cast<const double2*>(ws) + smode;
     // branch-free: rows [0, LE) fold div (E: sum on panel 0, O: difference on panel 1), rows
     // [LE, 2 LE) fold w* the other way round (w_O on panel 0, w_E on panel 1).  fetch reads
     // (u*, v*) on div rows and (w*, w*) on w rows at both nodes.
@@ -567,9 +625,17 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         const bool ind = kf < LE;
         const int c = ind ? kf : kf - LE;
         const int na = min(c, nz - 1), nb = max(nz - 1 - c, 0);
-        const double2* p2 = ind ? us2 : ws2;
-        const double2* q2 = ind ? vs2 : ws2;
-        return Raw{p2[na * rs], p2[nb * rs], q2[na * rs], q2[nb * rs]};
+        if constexpr (PS) {
+            // a paired slot's parts are the real parts of two modes: two reads per value
+            const double* p1 = ind ? us : ws;
+            const double* q1 = ind ? vs : ws;
+            auto ld = [&](const double* b, int n) { return make_double2(b[size_t(n) * H2 + ox], b[size_t(n) * H2 + oy]); };
+            return Raw{ld(p1, na), ld(p1, nb), ld(q1, na), ld(q1, nb)};
+        } else {
+            const double2* p2 = ind ? us2 : ws2;
+            const double2* q2 = ind ? vs2 : ws2;
+            return Raw{p2[na * rs], p2[nb * rs], q2[na * rs], q2[nb * rs]};
+        }
     };
     auto finish = [&](int kf, const Raw& rw) -> FoldPair {
         const bool ind = kf < LE;
@@ -577,10 +643,11 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         const double2 ua = rw.pa, ub = rw.pb, va = rw.qa, vb = rw.qb;   // w rows: pa, pb = w*
         const double xa_r = ind ? -skx * ua.y - sky * va.y : ua.x, xa_i = ind ? skx * ua.x + sky * va.x : ua.y;
         const double xb_r = ind ? -skx * ub.y - sky * vb.y : ub.x, xb_i = ind ? skx * ub.x + sky * vb.x : ub.y;
-        const double keep = (sval && c < LE) ? 1.0 : 0.0;
+        // the parts differ only in a paired slot
+        const double kr = (c < LE && (kpm & 1)) ? 1.0 : 0.0, ki = PS ? ((c < LE && (kpm & 2)) ? 1.0 : 0.0) : kr;
         const double pair = c < hF ? 1.0 : 0.0;
-        const double er = keep * (xa_r + pair * xb_r), ei = keep * (xa_i + pair * xb_i);   // even fold
-        const double orr = keep * pair * (xa_r - xb_r), oi = keep * pair * (xa_i - xb_i);  // odd fold
+        const double er = kr * (xa_r + pair * xb_r), ei = ki * (xa_i + pair * xb_i);   // even fold
+        const double orr = kr * pair * (xa_r - xb_r), oi = ki * pair * (xa_i - xb_i);  // odd fold
         FoldPair r;
         r.p0 = ind ? make_double2(er, ei) : make_double2(orr, oi);
         r.p1 = ind ? make_double2(orr, oi) : make_double2(er, ei);
@@ -591,10 +658,11 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb);
-        const int mode = col >> 1, j = mode / a.nxh, i = mode - j * a.nxh;
-        live[ct] = col < H2 && mode != 0;
+        const ZCol zc = zcol_t(a.zm, min(col, NC - 1), ident);
+        const int j = zc.j, i = zc.i;
+        live[ct] = col < NC && zc.so >= 2;   // not storage mode (0,0)
         k2c[ct] = 1.0; kxe[ct] = 0.0; kye[ct] = 0.0;
-        if (col < H2) {
+        if (col < NC) {
             const double kx = a.xw[i], ky = a.yw[j];
             k2c[ct] = kx * kx + ky * ky;
             kxe[ct] = kx_eff(a.xw, i, a.nx);
@@ -665,8 +733,10 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         store_groups<RBW, NW>(acc0, acc1, a.pB, G0, G1);
         if (a.pB.extra && threadIdx.x < BCOLS) G0[(a.pB.rows0 - 1) * LDB + threadIdx.x] = xph;
         __syncthreads();
-        // four outputs per thread per pass, their reads of u*, v*, p issued first (clamped)
+        // four outputs per thread per pass, their reads of u*, v*, p issued first (clamped); the
+        // thread's column is the same for all its elements (NT is a multiple of BCOLS)
         const int ne = nz * BCOLS;
+        const int2 zt = zst(zcol_t(a.zm, min(cb * BCOLS + int(threadIdx.x % BCOLS), NC - 1), ident));
         for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
             double uu[4], vv[4], pp[4];
             size_t qn[4];
@@ -675,8 +745,8 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
             for (int u = 0; u < 4; u++) {
                 const int e = e0 + NT * u, ec = min(e, ne - 1);
                 const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c;
-                ok[u] = e < ne && col < H2 && cc[2 * BCOLS + c] != 0.0;
-                qn[u] = size_t(node) * H2 + min(col, H2 - 1);
+                ok[u] = e < ne && col < NC && cc[2 * BCOLS + c] != 0.0;
+                qn[u] = size_t(node) * H2 + zt.x;
                 if (a.fused) {
                     uu[u] = us[qn[u]];
                     vv[u] = vs[qn[u]];
@@ -688,8 +758,9 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                 if (!ok[u]) continue;
                 const int e = e0 + NT * u, node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
                 const double ph = unfold_lds(G0, G1, nz, node, c), other = unfold_lds(G0, G1, nz, node, c ^ 1);
+                const size_t ro = size_t(node) * H2;
                 if (!a.fused) {
-                    phio[qn[u]] = ph;
+                    zstore(phio, ro, zt, ph, sec);
                     continue;
                 }
                 const int part = col & 1;
@@ -697,9 +768,9 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                 const bool wall = node == 0 || node == nz - 1;
                 const double du = part == 0 ? -cc[c] * pim : cc[c] * pre;
                 const double dv = part == 0 ? -cc[BCOLS + c] * pim : cc[BCOLS + c] * pre;
-                uo[qn[u]] = wall ? 0.0 : uu[u] - a.dt * du;
-                vo[qn[u]] = wall ? 0.0 : vv[u] - a.dt * dv;
-                po[qn[u]] = pp[u] + ph;
+                zstore(uo, ro, zt, wall ? 0.0 : uu[u] - a.dt * du, sec);
+                zstore(vo, ro, zt, wall ? 0.0 : vv[u] - a.dt * dv, sec);
+                zstore(po, ro, zt, pp[u] + ph, sec);
             }
         }
         __syncthreads();
@@ -715,8 +786,8 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
             size_t qn[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int e = min(e0 + NT * u, ne - 1), node = e / BCOLS, c = e - node * BCOLS;
-                qn[u] = size_t(node) * H2 + min(cb * BCOLS + c, H2 - 1);
+                const int e = min(e0 + NT * u, ne - 1), node = e / BCOLS;
+                qn[u] = size_t(node) * H2 + zt.x;
                 ww[u] = a.fused ? ws[qn[u]] : 0.0;
             }
 #pragma unroll
@@ -724,19 +795,23 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                 const int e = e0 + NT * u;
                 if (e >= ne) continue;
                 const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-                if (col >= H2 || cc[2 * BCOLS + c] == 0.0) continue;
+                if (col >= NC || cc[2 * BCOLS + c] == 0.0) continue;
                 const double dph = unfold_lds(G0, G1, nz, node, c);
-                if (!a.fused) dphio[qn[u]] = dph;
-                else wo[qn[u]] = (node == 0 || node == nz - 1) ? 0.0 : ww[u] - a.dt * dph;
+                const size_t ro = size_t(node) * H2;
+                if (!a.fused) zstore(dphio, ro, zt, dph, sec);
+                else zstore(wo, ro, zt, (node == 0 || node == nz - 1) ? 0.0 : ww[u] - a.dt * dph, sec);
             }
         }
         return;
     }
     // phi epilogue, per 16-column tile: the outputs' old values (u*, v*, p) are read for all of
     // the lane's nodes first (indices clamped into range), then combined and stored
+    int2 zo[CT];   // storage of the lane's output columns (read here, after the contractions)
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) zo[ct] = zst(zcol_t(a.zm, min(out_col(ct, cb), NC - 1), ident));
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct, cb), colc = min(col, H2 - 1);
+        const int col = out_col(ct, cb);
         const int part = col & 1;
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
@@ -756,7 +831,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
                     const int node = min(side == 0 ? row : nz - 1 - row, nz - 1);
-                    qn[r][side] = size_t(max(node, 0)) * H2 + colc;
+                    qn[r][side] = size_t(max(node, 0)) * H2 + zo[ct].x;
                     ok[r][side] = b < a.pB.rb0 && live[ct] && row < LE && (side == 0 || row < hF);
                 }
             }
@@ -765,7 +840,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                 for (int r = 0; r < 4; r++)
 #pragma unroll
                     for (int side = 0; side < 2; side++)
-                        if (ok[r][side]) phio[qn[r][side]] = pv[r][side];
+                        if (ok[r][side]) zstore_q(phio, qn[r][side], zo[ct], pv[r][side], sec);
                 continue;
             }
             double uu[4][2], vv[4][2], pp[4][2];
@@ -791,9 +866,9 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                     const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
                     const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
                     const size_t q = qn[r][side];
-                    uo[q] = wall ? 0.0 : uu[r][side] - a.dt * du;
-                    vo[q] = wall ? 0.0 : vv[r][side] - a.dt * dv;
-                    po[q] = pp[r][side] + ph;
+                    zstore_q(uo, q, zo[ct], wall ? 0.0 : uu[r][side] - a.dt * du, sec);
+                    zstore_q(vo, q, zo[ct], wall ? 0.0 : vv[r][side] - a.dt * dv, sec);
+                    zstore_q(po, q, zo[ct], pp[r][side] + ph, sec);
                 }
         }
     }
@@ -803,18 +878,18 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         const double ph = t < BCOLS ? extra_dot(a.pB, z0) : 0.0;
         const double other = __shfl_xor(ph, 1);
         const int col = cb * BCOLS + t, node = a.pB.rows0 - 1;
-        if (t < BCOLS && live[ct]) {
-            const size_t qn = size_t(node) * H2 + col;
+        if (t < BCOLS && live[ct]) {   // lane t's column of tile ct is col: zo[ct] is its storage
+            const size_t qn = size_t(node) * H2 + zo[ct].x;
             if (!a.fused) {
-                phio[qn] = ph;
+                zstore_q(phio, qn, zo[ct], ph, sec);
             } else {
                 const int part = col & 1;
                 const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
                 const double du = part == 0 ? -kxe[ct] * pim : kxe[ct] * pre;
                 const double dv = part == 0 ? -kye[ct] * pim : kye[ct] * pre;
-                uo[qn] = us[qn] - a.dt * du;
-                vo[qn] = vs[qn] - a.dt * dv;
-                po[qn] = po[qn] + ph;
+                zstore_q(uo, qn, zo[ct], us[qn] - a.dt * du, sec);
+                zstore_q(vo, qn, zo[ct], vs[qn] - a.dt * dv, sec);
+                zstore_q(po, qn, zo[ct], po[qn] + ph, sec);
             }
         }
     }
@@ -824,7 +899,6 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     fold_gemm_lds<RBW, NW>(acc0, acc1, a.pDB, z1, z0);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-        const int col = out_col(ct, cb), colc = min(col, H2 - 1);
 #pragma unroll
         for (int ii = 0; ii < RBW; ii++) {
             const int b = wave + NW * ii;
@@ -841,7 +915,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
                 for (int side = 0; side < 2; side++) {
                     const int node = min(side == 0 ? row : nz - 1 - row, nz - 1);
-                    qn[r][side] = size_t(max(node, 0)) * H2 + colc;
+                    qn[r][side] = size_t(max(node, 0)) * H2 + zo[ct].x;
                     ok[r][side] = b < a.pDB.rb0 && live[ct] && row < LE && (side == 0 || row < hF);
                     wv[r][side] = a.fused ? ws[qn[r][side]] : 0.0;
                 }
@@ -855,22 +929,22 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                     const int node = side == 0 ? row : nz - 1 - row;
                     const size_t q = qn[r][side];
                     if (!a.fused) {
-                        dphio[q] = dv[r][side];
+                        zstore_q(dphio, q, zo[ct], dv[r][side], sec);
                         continue;
                     }
                     const bool wall = node == 0 || node == nz - 1;
-                    wo[q] = wall ? 0.0 : wv[r][side] - a.dt * dv[r][side];
+                    zstore_q(wo, q, zo[ct], wall ? 0.0 : wv[r][side] - a.dt * dv[r][side], sec);
                 }
         }
     }
     if (a.pDB.extra && threadIdx.x < BCOLS) {
         // the centre row of D1 phi (s_h, on z_O)
-        const int t = threadIdx.x, ct = t >> 4, col = cb * BCOLS + t, node = a.pDB.rows0 - 1;
-        if (live[ct]) {
+        const int t = threadIdx.x, ct = t >> 4, node = a.pDB.rows0 - 1;
+        if (live[ct]) {   // lane t's column of tile ct: zo[ct] is its storage
             const double dph = extra_dot(a.pDB, z1);
-            const size_t qn = size_t(node) * H2 + col;
-            if (!a.fused) dphio[qn] = dph;
-            else wo[qn] = ws[qn] - a.dt * dph;
+            const size_t qn = size_t(node) * H2 + zo[ct].x;
+            if (!a.fused) zstore_q(dphio, qn, zo[ct], dph, sec);
+            else zstore_q(wo, qn, zo[ct], ws[qn] - a.dt * dph, sec);
         }
     }
 }
@@ -878,9 +952,10 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
 // the Poisson tile, then (when fused) wave 0 solves mode (0,0) at nodes cb, cb + ncb, ...
 template <int RBW, int NW>
 __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
-    poisson_body<RBW, NW>(a, cb, mb, smem);
+    if (zflags(a.zm, cb) & ZB_PAIRED) poisson_body<RBW, NW, true>(a, cb, mb, smem);
+    else poisson_body<RBW, NW, false>(a, cb, mb, smem);
     if (a.mode0 && threadIdx.x < 64) {
-        const int ncb = (a.H2 + BCOLS - 1) / BCOLS;
+        const int ncb = zmap_blocks(a.zm);
         for (int k = cb; k < a.nz; k += ncb) mode0_tile(a.m0, k, mb);
     }
 }
@@ -1056,7 +1131,7 @@ hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s
     }
 
 hipError_t launch_helmholtz(const HelmArgs& a, int nm, hipStream_t s) {
-    dim3 g = col_grid(a.H2, 3);
+    dim3 g(zmap_blocks(a.zm), 3);
     g.z = nm;
     const size_t lds = helm_lds_bytes(a);
     const int var = helm_variant(a);
@@ -1076,7 +1151,7 @@ hipError_t launch_helmholtz(const HelmArgs& a, int nm, hipStream_t s) {
     }
 
 hipError_t launch_poisson(const PoisArgs& a, int nm, hipStream_t s) {
-    dim3 g = col_grid(a.H2, 1);
+    dim3 g(zmap_blocks(a.zm), 1);
     g.z = nm;
     const size_t lds = pois_lds_bytes(a);
     const int var = pois_variant(a);

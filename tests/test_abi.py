@@ -116,3 +116,31 @@ def test_host_initial_field_matches_oracle_generator():
         o.init_fields()
         ou, ov, ow, _ = o.get_fields()
         assert np.max(np.abs(u - ou)) < 1e-14 and np.max(np.abs(v - ov)) < 1e-14 and np.max(np.abs(w - ow)) == 0
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 2), (8, 6), (32, 32), (48, 36), (64, 96), (128, 128), (192, 48), (32, 27),
+                                   (16, 9)])
+def test_solve_mode_space_covers_a_real_planes_transform(nx, ny):
+    """The per-mode solves walk the independent coefficients of a real plane's transform: every
+    storage column is written exactly once (by its slot, or as a slot's mirror or a zeroed
+    imaginary part), and the slots alone rebuild numpy's rfft2 of a random real plane."""
+    import numpy as np
+    import f90_dns_amd as dns
+    if not os.path.exists(dns.library_path()):
+        pytest.skip("libf90dns.so not built")
+    so, s2o, s2s = dns.zmap(nx, ny)
+    nxh = nx // 2 + 1
+    H2 = 2 * nxh * ny
+    assert len(so) == (nx * ny if ny % 2 == 0 else H2)
+    targets = np.concatenate([so, s2o[s2o >= 0]])
+    assert np.array_equal(np.sort(targets), np.arange(H2))
+    if ny % 2:
+        assert np.array_equal(so, np.arange(H2)) and np.all(s2o < 0)
+    X = np.fft.rfft2(np.random.default_rng(nx * ny).standard_normal((ny, nx)))   # [j, i]: mode i + nxh j
+    R = np.empty(H2)
+    R[0::2], R[1::2] = X.real.ravel(), X.imag.ravel()
+    Y = np.full(H2, np.nan)
+    Y[so] = R[so]
+    m = s2o >= 0
+    Y[s2o[m]] = s2s[m] * R[so[m]]
+    np.testing.assert_allclose(Y, R, rtol=0, atol=1e-12 * np.abs(R).max())
