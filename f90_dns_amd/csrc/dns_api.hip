@@ -489,6 +489,10 @@ extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
     z.out[0] = c->ox; z.out[1] = c->oy;
     z.xw = c->xw; z.yw = c->yw;
     z.out_f32 = c->mixed ? 1 : 0;
+    z.packed = 1;
+    z.zm = zmap_make(c->nx, c->ny);
+    z.zm.tab = c->ztab;
+    z.zm.bfl = c->zbfl;
     return z;
 }
 

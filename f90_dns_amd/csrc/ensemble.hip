@@ -242,8 +242,7 @@ int Ensemble::rebuild() {
                                                 : kind == K_POIS ? pois_lds_bytes(hp[g])
                                                                  : zderiv_lds_bytes(hz[g]));
                         // the Helmholtz and Poisson tiles walk the independent slots only
-                        const int ncb = kind == K_ZVORT ? (c->H2 + BCOLS - 1) / BCOLS
-                                                        : zmap_blocks(zmap_make(c->nx, c->ny));
+                        const int ncb = zmap_blocks(zmap_make(c->nx, c->ny));
                         if (tune.interleave) {
                             for (int x = 0; x < ncb; x++)
                                 for (int z = 0; z < B; z++)

@@ -135,39 +135,6 @@ __host__ __device__ inline const double* fold_xrow(const FoldOp& o) {
     return o.A + size_t(2 * o.pb) * o.ksp * 64;
 }
 
-struct ZArgs {
-    long long mstride;
-    int nz, H2, nxh, ny, nx;     // H2 = 2 * (nx/2+1) * ny real columns per plane
-    FoldOp d1;                   // folded D1: s (centre group) on X_O | r on X_E
-    const double* in[3];
-    double* out[3];              // float when out_f32
-    const double* xw; const double* yw;
-    int out_f32;
-};
-enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
-// row-block pairs of a folded operator
-inline int fold_nrb(const FoldOp& o) { return o.rb0 > o.rb1 ? o.rb0 : o.rb1; }
-// Kernel variant of a contraction kernel whose operators have at most nrb row-block pairs:
-// 0, 1, 2 = one wave per pair (1 to 3 pairs; results unfold through LDS so that every thread
-// stores), 3 = four waves with one pair each, 4 = four waves with two pairs each (the stored
-// operators are zero padded to 4 ceil(nrb / 4) blocks per group); -1 = more than 8 pairs.
-inline int z_variant(int nrb) { return nrb < 4 ? nrb - 1 : nrb <= 8 ? 2 + (nrb + 3) / 4 : -1; }
-inline bool z_variant_lx(int var) { return var >= 0 && var < 3; }
-// the variant of a kernel applying the operators o[0..n): -1 unless all are padded alike
-inline int z_variant_of(const FoldOp* const* o, int n) {
-    int m = 0;
-    for (int i = 0; i < n; i++) m = fold_nrb(*o[i]) > m ? fold_nrb(*o[i]) : m;
-    for (int i = 0; i < n; i++)
-        if (o[i]->pb != 4 * ((m + 3) / 4)) return -1;
-    return z_variant(m);
-}
-inline int zd_variant(const ZArgs& a) {
-    const FoldOp* o[1] = {&a.d1};
-    return z_variant_of(o, 1);
-}
-hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
-size_t zderiv_lds_bytes(const ZArgs& a);
-
 // Mode space walked by the Helmholtz and Poisson tiles.  The fields are transforms of real
 // planes, and the step keeps them so: on the kx = 0 and kx = nx/2 columns the modes
 // ky > ny/2 are the conjugates of ky < ny/2, and the four self-conjugate modes (0,0),
@@ -269,6 +236,43 @@ __device__ __forceinline__ ZCol zcol_t(const ZMap& z, int col, bool ident) {
     const int4 t = z.tab[col];
     return ZCol{t.x, t.y, t.z, t.w};
 }
+
+struct ZArgs {
+    long long mstride;
+    int nz, H2, nxh, ny, nx;     // H2 = 2 * (nx/2+1) * ny real columns per plane
+    FoldOp d1;                   // folded D1: s (centre group) on X_O | r on X_E
+    const double* in[3];
+    double* out[3];              // float when out_f32
+    const double* xw; const double* yw;
+    int out_f32;
+    // packed: the columns walked are the per-mode solves' slots (ZMap; the vorticity of the step),
+    // else the storage columns
+    int packed;
+    ZMap zm;
+};
+enum ZMode { Z_PLAIN = 0, Z_VORT = 1 };
+// row-block pairs of a folded operator
+inline int fold_nrb(const FoldOp& o) { return o.rb0 > o.rb1 ? o.rb0 : o.rb1; }
+// Kernel variant of a contraction kernel whose operators have at most nrb row-block pairs:
+// 0, 1, 2 = one wave per pair (1 to 3 pairs; results unfold through LDS so that every thread
+// stores), 3 = four waves with one pair each, 4 = four waves with two pairs each (the stored
+// operators are zero padded to 4 ceil(nrb / 4) blocks per group); -1 = more than 8 pairs.
+inline int z_variant(int nrb) { return nrb < 4 ? nrb - 1 : nrb <= 8 ? 2 + (nrb + 3) / 4 : -1; }
+inline bool z_variant_lx(int var) { return var >= 0 && var < 3; }
+// the variant of a kernel applying the operators o[0..n): -1 unless all are padded alike
+inline int z_variant_of(const FoldOp* const* o, int n) {
+    int m = 0;
+    for (int i = 0; i < n; i++) m = fold_nrb(*o[i]) > m ? fold_nrb(*o[i]) : m;
+    for (int i = 0; i < n; i++)
+        if (o[i]->pb != 4 * ((m + 3) / 4)) return -1;
+    return z_variant(m);
+}
+inline int zd_variant(const ZArgs& a) {
+    const FoldOp* o[1] = {&a.d1};
+    return z_variant_of(o, 1);
+}
+hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
+size_t zderiv_lds_bytes(const ZArgs& a);
 
 struct FlowState;
 struct FlowMember;

@@ -47,6 +47,10 @@ __device__ __forceinline__ void zstore(double* out, size_t ro, int2 z, double v,
     out[ro + z.x] = v;
     if (sec && z.y >= 0) out[ro + z.y] = z.y == z.x + 1 ? 0.0 : ((z.x & 1) ? -v : v);
 }
+__device__ __forceinline__ void zstore_f(float* out, size_t ro, int2 z, double v, bool sec) {
+    out[ro + z.x] = float(v);
+    if (sec && z.y >= 0) out[ro + z.y] = z.y == z.x + 1 ? 0.0f : float((z.x & 1) ? -v : v);
+}
 // the same at element q = ro + z.x
 __device__ __forceinline__ void zstore_q(double* out, size_t q, int2 z, double v, bool sec) {
     zstore(out, q - z.x, z, v, sec);
@@ -60,27 +64,50 @@ __device__ __forceinline__ int zflags(const ZMap& z, int cb) { return __builtin_
 // (DNS_pressure_BC_3D.f90:1077-1079).  Folded: s = D1 x_odd on X_O (rows up to the centre),
 // r = D1 x_even on X_E, out_k = s_k + r_k, out_{nz-1-k} = s_k - r_k.  One block = 16 modes.
 // -------------------------------------------------------------------------------------
-template <int RBW, int MODE, int NW>
+// PS: the block holds a paired slot (its two parts are the real parts of two modes)
+template <int RBW, int MODE, int NW, bool PS>
 __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int mb, double* lds) {
     constexpr bool LX = NW < 4;   // fewer row-block pairs than four: one wave per pair, LDS unfold
     constexpr int NT = 64 * NW;
-    const int H2 = a.H2, nz = a.nz, H = H2 / 2, hF = nz / 2, LE = hF + (nz & 1);
+    const int H2 = a.H2, nz = a.nz, hF = nz / 2, LE = hF + (nz & 1);
     const int wave = threadIdx.x >> 6;
-    const int mt = threadIdx.x & 15, smode = cb * BMODES + mt;
-    const bool sval = smode < H;
+    // mode space: the per-mode solves' slots for the vorticity of the step (packed), else the
+    // storage modes
+    const int P = a.packed ? a.zm.P : H2 / 2, NC = 2 * P;
+    const int bfl = a.packed ? zflags(a.zm, cb) : ZB_IDENT;
+    const bool sec = (bfl & ZB_SECOND) != 0, ident = (bfl & ZB_IDENT) != 0;
+    auto zc_of = [&](int col) -> ZCol {
+        if (ident) {
+            const int q = col >> 1, j = q / a.nxh;
+            return ZCol{col, -1, q - j * a.nxh, j};
+        }
+        return zcol_t(a.zm, col, false);
+    };
+    const int mt = threadIdx.x & 15, slot = cb * BMODES + mt;
+    const bool sval = slot < P;
+    const ZCol zx = zc_of(2 * min(slot, P - 1)), zy = zc_of(2 * min(slot, P - 1) + 1);
+    const int smode = sval ? (zx.so >> 1) : 0, ox = sval ? zx.so : 0, oy = sval ? zy.so : 0;
     const size_t rs = size_t(H2) / 2;
     // vorticity: one component per block, by = 0: omega_x from v, 1: omega_y from u (two blocks
-    // per column block fill the CUs: one block doing both left a third of the slots empty)
+    // per column block fill the CUs: one block doing both left a third of the slots empty).  Its
+    // i k w terms take the effective wavenumbers: they differ from the raw ones only where the
+    // term is anti-Hermitian, which the c2r that reads omega drops.
     const size_t mo = size_t(mb) * a.mstride;
     const double* w_in = a.in[2] + mo;
     {
         const int f = MODE == Z_VORT ? (by == 0 ? 1 : 0) : by;
-        const double2* x2 = reinterpret_cast<const double2*>(a.in[f] + mo) + (sval ? smode : 0);
+        const double* x1 = a.in[f] + mo;
+        const double2* x2 = reinterpret_cast<const double2*>(x1) + smode;
         // branch-free: mirrored rows (clamped in range), the centre row has na == nb
         struct Raw { double2 xa, xb; };
         auto fetch = [&](int kf) -> Raw {
             const int na = min(kf, nz - 1), nb = max(nz - 1 - kf, 0);
-            return Raw{x2[na * rs], x2[nb * rs]};
+            if constexpr (PS) {
+                return Raw{make_double2(x1[size_t(na) * H2 + ox], x1[size_t(na) * H2 + oy]),
+                           make_double2(x1[size_t(nb) * H2 + ox], x1[size_t(nb) * H2 + oy])};
+            } else {
+                return Raw{x2[na * rs], x2[nb * rs]};
+            }
         };
         auto finish = [&](int kf, const Raw& rw) -> FoldPair {
             const double2 xa = rw.xa, xb = rw.xb;
@@ -91,26 +118,21 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             r.p1 = make_double2(keep * (xa.x + pair * xb.x), keep * (xa.y + pair * xb.y));
             return r;
         };
-        auto emit = [&](int node, int col, double y) {
-            if (MODE == Z_PLAIN) {
-                a.out[f][mo + size_t(node) * H2 + col] = y;
-                return;
-            }
-            const int mode = col >> 1, part = col & 1, j = mode / a.nxh, i = mode - j * a.nxh;
-            const double wo = w_in[size_t(node) * H2 + (col ^ 1)];   // the other part of w
-            const double wc = w_in[size_t(node) * H2 + col];
+        const int o = f == 1 ? 0 : 1;
+        // the vorticity from y = D1 x at row offset ro of a column: w's two parts of the mode
+        // (wc this column's, wo the other), the column's effective wavenumber kk and part
+        auto vort = [&](double y, double wc, double wo, double kk, int part) {
             const double wr = part == 0 ? wc : wo, wi = part == 0 ? wo : wc;
-            double res;
-            if (f == 1) {        // omega_x = i ky w - D1 v
-                const double ky = a.yw[j];
-                res = part == 0 ? (-ky * wi - y) : (ky * wr - y);
-            } else {             // omega_y = D1 u - i kx w
-                const double kx = a.xw[i];
-                res = part == 0 ? (y + kx * wi) : (y - kx * wr);
-            }
-            const int o = f == 1 ? 0 : 1;
-            if (a.out_f32) reinterpret_cast<float*>(a.out[o])[mo + size_t(node) * H2 + col] = float(res);
-            else a.out[o][mo + size_t(node) * H2 + col] = res;
+            if (f == 1) return part == 0 ? (-kk * wi - y) : (kk * wr - y);   // omega_x = i ky w - D1 v
+            return part == 0 ? (y + kk * wi) : (y - kk * wr);                // omega_y = D1 u - i kx w
+        };
+        auto kk_of = [&](const ZCol& z) {
+            return f == 1 ? ky_eff(a.yw, z.i, z.j, a.nx, a.ny) : kx_eff(a.xw, z.i, a.nx);
+        };
+        auto put = [&](size_t ro, int2 z, double v) {
+            if (MODE == Z_PLAIN) zstore(a.out[f] + mo, ro, z, v, sec);
+            else if (a.out_f32) zstore_f(reinterpret_cast<float*>(a.out[o]) + mo, ro, z, v, sec);
+            else zstore(a.out[o] + mo, ro, z, v, sec);
         };
         d4 acc0[RBW][CT], acc1[RBW][CT];
         zero_acc2(acc0);
@@ -125,59 +147,56 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             store_groups<RBW, NW>(acc0, acc1, a.d1, G0, G1);
             if (a.d1.extra && threadIdx.x < BCOLS) G0[(a.d1.rows0 - 1) * LDB + threadIdx.x] = xs;
             __syncthreads();
-            // four outputs per thread per pass, their w and wavenumber reads issued first
-            // (indices clamped into range)
+            // four outputs per thread per pass, their w reads issued first (indices clamped into
+            // range); the thread's column is the same for all its elements (NT is a multiple of BCOLS)
             const int ne = nz * BCOLS;
+            const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c;
+            const ZCol zc = zc_of(min(col, NC - 1));
+            const int2 zt = zst(zc);
+            const double kk = MODE == Z_VORT ? kk_of(zc) : 0.0;
             for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
-                double y[4], wc[4], wo[4], kk[4];
-                int nd[4], cl4[4];
+                double y[4], wc[4], wo[4];
+                int nd[4];
                 bool ok[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int e = e0 + NT * u, ec = min(e, ne - 1);
-                    const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c, colc = min(col, H2 - 1);
-                    const int mode = colc >> 1, j = mode / a.nxh, i = mode - j * a.nxh;
-                    ok[u] = e < ne && col < H2;
+                    const int node = ec / BCOLS;
+                    ok[u] = e < ne && col < NC;
                     nd[u] = node;
-                    cl4[u] = col;
                     y[u] = unfold_lds(G0, G1, nz, node, c);
                     if (MODE == Z_VORT) {
-                        wc[u] = w_in[size_t(node) * H2 + colc];
-                        wo[u] = w_in[size_t(node) * H2 + (colc ^ 1)];
-                        kk[u] = f == 1 ? a.yw[j] : a.xw[i];
+                        wc[u] = w_in[size_t(node) * H2 + zc.so];
+                        wo[u] = w_in[size_t(node) * H2 + (zc.so ^ 1)];
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     if (!ok[u]) continue;
-                    const size_t q = mo + size_t(nd[u]) * H2 + cl4[u];
-                    if (MODE == Z_PLAIN) {
-                        a.out[f][q] = y[u];
-                        continue;
-                    }
-                    const int part = cl4[u] & 1;
-                    const double wr = part == 0 ? wc[u] : wo[u], wi = part == 0 ? wo[u] : wc[u];
-                    double res;
-                    if (f == 1) res = part == 0 ? (-kk[u] * wi - y[u]) : (kk[u] * wr - y[u]);   // omega_x = i ky w - D1 v
-                    else res = part == 0 ? (y[u] + kk[u] * wi) : (y[u] - kk[u] * wr);          // omega_y = D1 u - i kx w
-                    const int o = f == 1 ? 0 : 1;
-                    if (a.out_f32) reinterpret_cast<float*>(a.out[o])[q] = float(res);
-                    else a.out[o][q] = res;
+                    const size_t ro = size_t(nd[u]) * H2;
+                    put(ro, zt, MODE == Z_PLAIN ? y[u] : vort(y[u], wc[u], wo[u], kk, col & 1));
                 }
             }
             return;
         }
         if (a.d1.extra) {
             const int col = cb * BCOLS + int(threadIdx.x);
-            if (threadIdx.x < BCOLS && col < H2) emit(hF, col, xs);
+            if (threadIdx.x < BCOLS && col < NC) {
+                const ZCol zc = zc_of(col);
+                const size_t ro = size_t(hF) * H2;
+                const double v = MODE == Z_PLAIN ? xs
+                                 : vort(xs, w_in[ro + zc.so], w_in[ro + (zc.so ^ 1)], kk_of(zc), col & 1);
+                put(ro, zst(zc), v);
+            }
         }
         // per tile and row-block pair: the vorticity's w reads for all of the lane's nodes go out
         // together (indices clamped into range), then the results are combined and stored
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
-            const int col = out_col(ct, cb), colc = min(col, H2 - 1);
-            const int mode = colc >> 1, part = colc & 1, j = mode / a.nxh, i = mode - j * a.nxh;
-            const double kk = MODE == Z_VORT ? (f == 1 ? a.yw[j] : a.xw[i]) : 0.0;
+            const int col = out_col(ct, cb), part = col & 1;
+            const ZCol zc = zc_of(min(col, NC - 1));
+            const int2 zt = zst(zc);
+            const double kk = MODE == Z_VORT ? kk_of(zc) : 0.0;
 #pragma unroll
             for (int ii = 0; ii < RBW; ii++) {
                 const int b = wave + NW * ii;
@@ -195,10 +214,10 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
                     for (int side = 0; side < 2; side++) {
                         const int node = max(min(side == 0 ? row : nz - 1 - row, nz - 1), 0);
                         qn[r][side] = size_t(node) * H2;
-                        ok[r][side] = col < H2 && b < a.d1.rb0 && row < LE && (side == 0 || row < hF);
+                        ok[r][side] = col < NC && b < a.d1.rb0 && row < LE && (side == 0 || row < hF);
                         if (MODE == Z_VORT) {
-                            wc[r][side] = w_in[qn[r][side] + colc];
-                            wo[r][side] = w_in[qn[r][side] + (colc ^ 1)];   // the other part of w
+                            wc[r][side] = w_in[qn[r][side] + zc.so];
+                            wo[r][side] = w_in[qn[r][side] + (zc.so ^ 1)];   // the other part of w
                         }
                     }
                 }
@@ -207,20 +226,8 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
 #pragma unroll
                     for (int side = 0; side < 2; side++) {
                         if (!ok[r][side]) continue;
-                        const size_t q = mo + qn[r][side] + col;
                         const double y = yv[r][side];
-                        if (MODE == Z_PLAIN) {
-                            a.out[f][q] = y;
-                            continue;
-                        }
-                        const double wr = part == 0 ? wc[r][side] : wo[r][side];
-                        const double wi = part == 0 ? wo[r][side] : wc[r][side];
-                        double res;
-                        if (f == 1) res = part == 0 ? (-kk * wi - y) : (kk * wr - y);   // omega_x = i ky w - D1 v
-                        else res = part == 0 ? (y + kk * wi) : (y - kk * wr);          // omega_y = D1 u - i kx w
-                        const int o = f == 1 ? 0 : 1;
-                        if (a.out_f32) reinterpret_cast<float*>(a.out[o])[q] = float(res);
-                        else a.out[o][q] = res;
+                        put(qn[r][side], zt, MODE == Z_PLAIN ? y : vort(y, wc[r][side], wo[r][side], kk, part));
                     }
             }
         }
@@ -230,14 +237,16 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
 template <int RBW, int MODE, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv(ZArgs a) {
     extern __shared__ double smem[];
-    zderiv_tile<RBW, MODE, NW>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    if (a.packed && (zflags(a.zm, blockIdx.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, true>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    else zderiv_tile<RBW, MODE, NW, false>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, int MODE, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
-    zderiv_tile<RBW, MODE, NW>(a, t.x, blockIdx.y, t.z, smem);
+    if (a.packed && (zflags(a.zm, t.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, true>(a, t.x, blockIdx.y, t.z, smem);
+    else zderiv_tile<RBW, MODE, NW, false>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // apply_flow_control_forcing / update_pressure_gradient_3d / calculate_bulk_velocity_3d
@@ -1108,6 +1117,7 @@ size_t pois_lds_bytes(const PoisArgs& a) {
 
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s) {
     dim3 g = col_grid(a.H2, mode == Z_VORT ? 2 : nf);
+    if (a.packed) g.x = zmap_blocks(a.zm);
     g.z = nm;
     const int var = zd_variant(a);
     const size_t lds = zderiv_lds_bytes(a);
