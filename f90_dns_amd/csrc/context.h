@@ -31,6 +31,7 @@ struct dns_ctx {
     double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
     int4* ztab = nullptr;         // mode space of the per-mode solves: per column (zmap_build)
     int* zbfl = nullptr;          //   and per column block
+    dns::ZMap zm{};               //   its shape (tab, bfl set)
     // fields, B members each
     double *uh = nullptr, *vh = nullptr, *wh = nullptr, *ph = nullptr;
     double *us = nullptr, *vs = nullptr, *ws = nullptr;
@@ -56,6 +57,9 @@ struct dns_ctx {
 };
 
 namespace dns {
+// the contraction kernels of these operators all run their four-wave variants (the per-mode
+// solves' packed mode space is on only then)
+bool ops_wide(const Operators& o);
 // Argument blocks of the fused step's kernels over all members of c (plane-kernel blocks
 // without their geometry: fft_fill adds it).
 FFTArgs ctx_c2r_nl_args(dns_ctx* c);

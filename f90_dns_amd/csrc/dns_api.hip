@@ -217,7 +217,7 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
         (r = upload(c, &c->pDB, o.pDB.data)) || (r = upload(c, &c->pl, o.pl)))
         return bail(r);
     {
-        const ZMap z = zmap_make(c->nx, c->ny);
+        const ZMap z = zmap_make(c->nx, c->ny, ops_wide(c->op));
         std::vector<int4> zt(size_t(2) * z.P);
         std::vector<int> zb(zmap_blocks(z));
         if (!zmap_build(z, reinterpret_cast<int*>(zt.data()), zb.data())) {
@@ -225,6 +225,9 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
             return bail(DNS_EINVAL);
         }
         if ((r = upload(c, &c->ztab, zt)) || (r = upload(c, &c->zbfl, zb))) return bail(r);
+        c->zm = z;
+        c->zm.tab = c->ztab;
+        c->zm.bfl = c->zbfl;
     }
     const size_t NSB = c->NS * c->B, NB = c->N * c->B;
     double** specs[] = {&c->uh, &c->vh, &c->wh, &c->ph, &c->us, &c->vs, &c->ws, &c->ox, &c->oy,
@@ -273,6 +276,18 @@ static FoldOp fold_op(const FoldedA& f, const double* dev) {
     o.extra = f.extra;
     o.pb = f.pb;
     return o;
+}
+
+extern "C++" bool dns::ops_wide(const Operators& o) {
+    const FoldOp zd = fold_op(o.zD1, nullptr), hf = fold_op(o.hF, nullptr), hb = fold_op(o.hB, nullptr);
+    const FoldOp pf = fold_op(o.pF, nullptr), pb = fold_op(o.pB, nullptr), pdb = fold_op(o.pDB, nullptr);
+    const FoldOp* z1[1] = {&zd};
+    const FoldOp* h2[2] = {&hf, &hb};
+    const FoldOp* p3[3] = {&pf, &pb, &pdb};
+    const int v[3] = {z_variant_of(z1, 1), z_variant_of(h2, 2), z_variant_of(p3, 3)};
+    for (int x : v)
+        if (x < 3) return false;
+    return true;
 }
 
 static FFTArgs fft_args(dns_ctx* c) {
@@ -490,9 +505,7 @@ extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
     z.xw = c->xw; z.yw = c->yw;
     z.out_f32 = c->mixed ? 1 : 0;
     z.packed = 1;
-    z.zm = zmap_make(c->nx, c->ny);
-    z.zm.tab = c->ztab;
-    z.zm.bfl = c->zbfl;
+    z.zm = c->zm;
     return z;
 }
 
@@ -573,9 +586,7 @@ extern "C++" HelmArgs dns::ctx_helm_args(dns_ctx* c) {
     h.ph = c->ph;
     h.s_f32 = c->mixed ? 1 : 0;
     h.flow = c->flow;
-    h.zm = zmap_make(c->nx, c->ny);
-    h.zm.tab = c->ztab;
-    h.zm.bfl = c->zbfl;
+    h.zm = c->zm;
     return h;
 }
 
@@ -606,9 +617,7 @@ extern "C++" PoisArgs dns::ctx_pois_args(dns_ctx* c, int fused) {
     a.phi = c->phi; a.dphi = c->dphi;
     a.m0 = ctx_mode0_args(c, fused);   // mode (0,0) solved by the Poisson launch
     a.mode0 = 1;
-    a.zm = zmap_make(c->nx, c->ny);
-    a.zm.tab = c->ztab;
-    a.zm.bfl = c->zbfl;
+    a.zm = c->zm;
     return a;
 }
 
@@ -999,7 +1008,7 @@ static void unswizzle(const FoldedA& a, std::vector<double>& m) {
 
 int dns_debug_zmap(int nx, int ny, int* ncols, int* so, int* s2o, double* s2s) {
     if (nx < 2 || ny < 1 || nx % 2 || !ncols || !so || !s2o || !s2s) return DNS_EINVAL;
-    const ZMap z = zmap_make(nx, ny);
+    const ZMap z = zmap_make(nx, ny, true);
     std::vector<int> t(size_t(8) * z.P), b(zmap_blocks(z));
     if (!zmap_build(z, t.data(), b.data())) return DNS_EINVAL;
     *ncols = 2 * z.P;

@@ -40,7 +40,7 @@ bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) 
     b += d(nz) + d(nz * nz) * 2 + d(o.zD1.data.size());
     b += d(o.hF.data.size() * B) + d(o.hB.data.size()) + 3 * d(m * B) + d(B);
     b += d(o.pF.data.size()) + d(o.pB.data.size()) + d(o.pDB.data.size()) + d(nz);
-    const ZMap zm = zmap_make(p.nx, p.ny);
+    const ZMap zm = zmap_make(p.nx, p.ny, ops_wide(o));
     b += size_t(2) * zm.P * sizeof(int4) + std::max<size_t>(zmap_blocks(zm), 1) * sizeof(int);   // mode-space tables
     b += 15 * d(NS * B) + 6 * d(N * B);
     b += std::max<size_t>(B, 1) * (sizeof(FlowState) + sizeof(FlowMember));
@@ -242,7 +242,7 @@ int Ensemble::rebuild() {
                                                 : kind == K_POIS ? pois_lds_bytes(hp[g])
                                                                  : zderiv_lds_bytes(hz[g]));
                         // the Helmholtz and Poisson tiles walk the independent slots only
-                        const int ncb = zmap_blocks(zmap_make(c->nx, c->ny));
+                        const int ncb = zmap_blocks(c->zm);
                         if (tune.interleave) {
                             for (int x = 0; x < ncb; x++)
                                 for (int z = 0; z < B; z++)

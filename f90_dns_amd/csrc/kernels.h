@@ -145,7 +145,9 @@ __host__ __device__ inline const double* fold_xrow(const FoldOp& o) {
 // (0,ny/2), and of (nx/2,0) with (nx/2,ny/2)), then the independent modes at or beyond P.  A
 // tile stores each mirrored mode as the conjugate of its slot and zeroes the imaginary parts
 // of the self-conjugate modes.  At 128 x 128 that is 512 column blocks instead of 520.
-// Without on, P = (nx/2+1) ny and slot q is storage mode q.
+// Without on, P = (nx/2+1) ny and slot q is storage mode q.  It is on only where the
+// contraction kernels run their four-wave variants (wide: nz of about 98 and more); the one- to
+// three-wave variants are compiled without it (their registers would not fit four waves per SIMD).
 constexpr int Z_BLOCK_MODES = 16;   // complex slots per column block
 struct ZMap {
     int on, nxh, ny, P;
@@ -155,11 +157,11 @@ struct ZMap {
 // block flags: a column with a second storage column / a paired slot / every column is its own
 // storage column with no second one (the kernels then skip the table)
 enum { ZB_SECOND = 1, ZB_PAIRED = 2, ZB_IDENT = 4 };
-inline ZMap zmap_make(int nx, int ny) {
+inline ZMap zmap_make(int nx, int ny, bool wide) {
     ZMap z{};
     z.nxh = nx / 2 + 1;
     z.ny = ny;
-    z.on = (ny % 2 == 0 && nx >= 4) ? 1 : 0;
+    z.on = (wide && ny % 2 == 0 && nx >= 4) ? 1 : 0;
     z.P = z.on ? (nx / 2) * ny : z.nxh * ny;
     return z;
 }

@@ -74,7 +74,8 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     // mode space: the per-mode solves' slots for the vorticity of the step (packed), else the
     // storage modes
     const int P = a.packed ? a.zm.P : H2 / 2, NC = 2 * P;
-    const int bfl = a.packed ? zflags(a.zm, cb) : ZB_IDENT;
+    // (the one- to three-wave variants are compiled for the storage columns only)
+    const int bfl = (NW == 4 && a.packed) ? zflags(a.zm, cb) : ZB_IDENT;
     const bool sec = (bfl & ZB_SECOND) != 0, ident = (bfl & ZB_IDENT) != 0;
     auto zc_of = [&](int col) -> ZCol {
         if (ident) {
@@ -237,7 +238,7 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
 template <int RBW, int MODE, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv(ZArgs a) {
     extern __shared__ double smem[];
-    if (a.packed && (zflags(a.zm, blockIdx.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, true>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    if (NW == 4 && a.packed && (zflags(a.zm, blockIdx.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, NW == 4>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
     else zderiv_tile<RBW, MODE, NW, false>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, int MODE, int NW>
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(con
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
-    if (a.packed && (zflags(a.zm, t.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, true>(a, t.x, blockIdx.y, t.z, smem);
+    if (NW == 4 && a.packed && (zflags(a.zm, t.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, NW == 4>(a, t.x, blockIdx.y, t.z, smem);
     else zderiv_tile<RBW, MODE, NW, false>(a, t.x, blockIdx.y, t.z, smem);
 }
 
@@ -357,7 +358,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     // the forcing enters mode (0,0) of u only, in the column block holding that mode,
     // component 0: that block runs the flow controller first when it is fused
     const bool forced = comp == 0 && cb == 0;   // slot 0 holds mode (0,0)
-    const int bfl = zflags(a.zm, cb);
+    const int bfl = NW == 4 ? zflags(a.zm, cb) : ZB_IDENT;   // (four-wave variants only)
     const bool sec = (bfl & ZB_SECOND) != 0, ident = (bfl & ZB_IDENT) != 0;
     if (a.flow_fused && forced) {
         if (threadIdx.x < 64) flow_tile(a.fl, mb);
@@ -570,7 +571,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
 template <int RBW, bool SF32, int NW>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz(HelmArgs a) {
     extern __shared__ double smem[];
-    if (zflags(a.zm, blockIdx.x) & ZB_PAIRED) helmholtz_tile<RBW, SF32, NW, true>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    if (NW == 4 && (zflags(a.zm, blockIdx.x) & ZB_PAIRED)) helmholtz_tile<RBW, SF32, NW, NW == 4>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
     else helmholtz_tile<RBW, SF32, NW, false>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
 template <int RBW, bool SF32, int NW>
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_HELM)) void k_helmholtz_ra
     extern __shared__ double smem[];
     const Tile t = map[blockIdx.x];
     const HelmArgs& a = d[t.g];
-    if (zflags(a.zm, t.x) & ZB_PAIRED) helmholtz_tile<RBW, SF32, NW, true>(a, t.x, blockIdx.y, t.z, smem);
+    if (NW == 4 && (zflags(a.zm, t.x) & ZB_PAIRED)) helmholtz_tile<RBW, SF32, NW, NW == 4>(a, t.x, blockIdx.y, t.z, smem);
     else helmholtz_tile<RBW, SF32, NW, false>(a, t.x, blockIdx.y, t.z, smem);
 }
 
@@ -609,7 +610,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     // -(i xw u* + i yw v*) is taken on the effective wavenumbers: they differ from the raw ones
     // only where the term is anti-Hermitian in a real field's transform (the kx = nx/2 column,
     // and ky = ny/2 on the kx = 0 and nx/2 columns), which the reference's c2r of phi drops.
-    const int bfl = zflags(a.zm, cb);
+    const int bfl = NW == 4 ? zflags(a.zm, cb) : ZB_IDENT;   // (four-wave variants only)
     const bool sec = (bfl & ZB_SECOND) != 0, ident = (bfl & ZB_IDENT) != 0;
     const int mt = threadIdx.x & 15, slot = cb * BMODES + mt;
     const bool sval = slot < P;
@@ -961,7 +962,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
 // the Poisson tile, then (when fused) wave 0 solves mode (0,0) at nodes cb, cb + ncb, ...
 template <int RBW, int NW>
 __device__ __forceinline__ void poisson_tile(const PoisArgs& a, int cb, int mb, double* smem) {
-    if (zflags(a.zm, cb) & ZB_PAIRED) poisson_body<RBW, NW, true>(a, cb, mb, smem);
+    if (NW == 4 && (zflags(a.zm, cb) & ZB_PAIRED)) poisson_body<RBW, NW, NW == 4>(a, cb, mb, smem);
     else poisson_body<RBW, NW, false>(a, cb, mb, smem);
     if (a.mode0 && threadIdx.x < 64) {
         const int ncb = zmap_blocks(a.zm);

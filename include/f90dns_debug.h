@@ -39,7 +39,9 @@ int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int whic
 int dns_debug_copy(size_t n, int repeats, double* ms_out);
 /* max relative reconstruction error of the Helmholtz and Poisson eigen bases. */
 int dns_debug_operator_error(dns_ctx* ctx, double* out2);
-/* The mode space of the per-mode solves for an nx x ny plane (host only): *ncols real columns
+/* The mode space of the per-mode solves for an nx x ny plane where nz puts them on their
+ * four-wave kernels (nz of about 98 and more; below that they walk the storage columns), host
+ * only: *ncols real columns
  * (at most 2 (nx/2+1) ny); for column c, so[c] = its storage real column (re/im interleaved,
  * mode i + (nx/2+1) j), s2o[c] = -1 or a second storage column that receives s2s[c] times the
  * value (a mirrored mode's part, or 0 for the imaginary part of a self-conjugate mode).  Arrays

@@ -280,3 +280,67 @@ def test_gpu_matches_cpu_emulation_of_same_algorithm(dns, kw):
     errs = field_errors(g.get_fields(), e.get_fields())
     _say(f"  {errs}")
     assert max_err(errs) < 1e-12, errs
+
+
+def test_packed_mode_space_stages_against_oracle(dns):
+    """nz = 129 puts the per-mode solves on their four-wave kernels, which walk only the
+    independent coefficients and store the mirrored modes as conjugates: every stage's spectral
+    output (all modes, mirrored and self-conjugate ones included) against the oracle's r2c."""
+    _say("packed mode space 48x36x129 stages")
+    kw = dict(CONFIG1, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
+              flow_control_method=2, perturbation_kind=1, seed=21, noise_kmax=6, perturbation_amplitude=0.05)
+    o, g = pair(**kw)
+    o.step(2)
+    g.step(2)
+    g.begin_step()
+    o.compute_source_terms(); g.compute_source_terms()
+    G = o.diagnostics()["pressure_gradient"]
+    want = [orc.fft_r2c(s) for s in o.source_terms()]
+    want[0][:, 0, 0] -= G * kw["nx"] * kw["ny"]
+    scale = np.sqrt(sum(np.linalg.norm(w) ** 2 for w in want))
+    for c in range(3):
+        err = np.linalg.norm(g.spectral(7 + c) - want[c]) / scale
+        _say(f"  S_{c}: {err:.2e}")
+        assert err < 1e-11, c
+    o.viscous_step(); g.viscous_step()
+    want = [orc.fft_r2c(s) for s in o.star_fields()]
+    scale = np.sqrt(sum(np.linalg.norm(w) ** 2 for w in want))
+    for c in range(3):
+        got = g.spectral(4 + c)
+        err = np.linalg.norm(got - want[c]) / scale
+        _say(f"  star_{c}: {err:.2e}")
+        assert err < 1e-11, c
+        # mirrored modes are exact conjugates, self-conjugate modes exactly real
+        ny = kw["ny"]
+        for i in (0, kw["nx"] // 2):
+            assert np.array_equal(got[:, ny - 1:ny // 2:-1, i], np.conj(got[:, 1:ny // 2, i]))
+            assert np.all(got[:, 0, i].imag == 0) and np.all(got[:, ny // 2, i].imag == 0)
+    o.solve_pressure(); g.solve_pressure()
+    err = rel_l2(g.spectral(12), orc.fft_r2c(o.phi()))
+    _say(f"  phi: {err:.2e}")
+    assert err < 1e-10
+    o.pressure_correction(); g.pressure_correction()
+    o.apply_boundary_conditions(); g.apply_boundary_conditions()
+    e = field_errors(g.get_fields(), o.get_fields())
+    _say(f"  after stages {e}")
+    assert max_err(e) < 1e-10, e
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_packed_mode_space_batch_member_equals_alone(dns, precision):
+    """Members of a batch at nz = 129 (packed mode space) are bitwise the runs alone."""
+    base = dict(CONFIG1, nx=32, ny=24, nz=129, lgl_derivative=1, precision=precision, perturbation_kind=1,
+                noise_kmax=4, perturbation_amplitude=0.04)
+    mem = [dict(re=180.0, seed=3), dict(re=260.0, seed=4, flow_control_method=2)]
+    _say(f"packed batch precision {precision}")
+    batch = dns.Simulation(members=mem, **base)
+    batch.init_fields()
+    batch.step(6, graph=True)
+    for b, m in enumerate(mem):
+        one = dns.Simulation(**{**base, **m})
+        one.init_fields()
+        one.step(6, graph=True)
+        for f, (x, y) in enumerate(zip(batch.get_fields(member=b), one.get_fields())):
+            assert np.array_equal(x, y), (b, f)
+        one.close()
+    batch.close()
