@@ -61,6 +61,10 @@ WEAK_CASES = {
     "exact64x32x65": (dict(CONFIG1, nx=64, ny=32, nz=65, lgl_derivative=1, use_crank_nicolson=0,
                            perturbation_kind=1, seed=8, noise_kmax=6, perturbation_amplitude=0.05,
                            use_weak_form_divergence=1), 8),
+    # nz = 129: the per-mode solves walk the packed mode space
+    "packed48x36x129": (dict(CONFIG1, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
+                             perturbation_kind=1, seed=9, noise_kmax=6, perturbation_amplitude=0.05,
+                             use_weak_form_divergence=1), 5),
 }
 
 
