@@ -328,7 +328,8 @@ def test_packed_mode_space_stages_against_oracle(dns):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_packed_mode_space_batch_member_equals_alone(dns, precision):
-    """Members of a batch at nz = 129 (packed mode space) are bitwise the runs alone."""
+    """Members of a batch at nz = 129 (packed mode space) are bitwise the runs alone, and graph
+    replay is bitwise the eager launches."""
     base = dict(CONFIG1, nx=32, ny=24, nz=129, lgl_derivative=1, precision=precision, perturbation_kind=1,
                 noise_kmax=4, perturbation_amplitude=0.04)
     mem = [dict(re=180.0, seed=3), dict(re=260.0, seed=4, flow_control_method=2)]
@@ -336,6 +337,13 @@ def test_packed_mode_space_batch_member_equals_alone(dns, precision):
     batch = dns.Simulation(members=mem, **base)
     batch.init_fields()
     batch.step(6, graph=True)
+    eager = dns.Simulation(members=mem, **base)
+    eager.init_fields()
+    eager.step(6, graph=False)
+    for b in range(len(mem)):
+        for x, y in zip(batch.get_fields(member=b), eager.get_fields(member=b)):
+            assert np.array_equal(x, y), ("graph vs eager", b)
+    eager.close()
     for b, m in enumerate(mem):
         one = dns.Simulation(**{**base, **m})
         one.init_fields()
