@@ -335,6 +335,8 @@ def test_packed_mode_space_batch_member_equals_alone(dns, precision):
     mem = [dict(re=180.0, seed=3), dict(re=260.0, seed=4, flow_control_method=2)]
     _say(f"packed batch precision {precision}")
     batch = dns.Simulation(members=mem, **base)
+    # the packed mode space's tables are counted by the estimate the serving loop admits by
+    assert batch.device_bytes == dns.estimate_bytes(nmembers=len(mem), **{**base, **mem[0]})
     batch.init_fields()
     batch.step(6, graph=True)
     eager = dns.Simulation(members=mem, **base)
