@@ -172,6 +172,10 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
         g_create_error = "hipFuncSetAttribute failed for the plane kernels";
         return bail(DNS_EDEVICE);
     }
+    if (z_prepare() != hipSuccess) {
+        g_create_error = "hipDeviceGetAttribute failed";
+        return bail(DNS_EDEVICE);
+    }
     // operators: the Helmholtz parts depend on Re and are stacked per member
     std::string err;
     std::vector<double> hF, c1, c2, hl, fv;

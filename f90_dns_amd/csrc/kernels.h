@@ -346,6 +346,8 @@ struct PoisArgs {
     ZMap zm;                     // the tiles' mode space
 };
 hipError_t launch_poisson(const PoisArgs& a, int nmembers, hipStream_t s);
+// device facts the z-kernel launchers use (compute units); once per process, before any launch
+hipError_t z_prepare();
 inline int pois_variant(const PoisArgs& a) {
     const FoldOp* o[3] = {&a.pF, &a.pB, &a.pDB};
     return z_variant_of(o, 3);

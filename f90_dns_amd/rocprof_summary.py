@@ -24,7 +24,7 @@ STEP_NAMES = [
     (r"k_r2c(_rag)?<\d+, 1,", "r2c_cross"),
     (r"k_flow_control", "flow_control"),
     (r"k_helmholtz", "helmholtz"),
-    (r"k_poisson(_rag)?<", "poisson_fused"),
+    (r"k_poisson(_rag|_w2)?<", "poisson_fused"),
 ]
 
 
