@@ -142,3 +142,21 @@ def test_config4_ensemble_64_members(dns):
     _check_bitwise(dns, ens, mem, 8)
     for i in range(len(mem)):
         assert np.isfinite(ens.diagnostics(i)["energy"])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ensemble_with_packed_members_equals_alone(dns, precision):
+    """Members at nz = 129 (their per-mode solves walk the packed mode space, through the ragged
+    kernels' tile map) next to narrow members: each bitwise its run alone."""
+    from f90_dns_amd.workloads import BASE
+    mem = [dict(BASE, nx=32, ny=24, nz=129, lgl_derivative=1, re=180.0, seed=31, precision=precision),
+           dict(BASE, nx=32, ny=32, nz=33, re=150.0, seed=32, precision=precision),
+           dict(BASE, nx=48, ny=36, nz=129, lgl_derivative=1, re=240.0, seed=33, flow_control_method=2,
+                target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, controller_update_freq=7, precision=precision)]
+    ens = dns.Ensemble(mem)
+    ens.init_fields()
+    ens.step(6, graph=True)
+    ens.synchronize()
+    _say(f"ensemble with packed members, precision {precision}: 6 steps")
+    _check_bitwise(dns, ens, mem, 6)
+    ens.close()
