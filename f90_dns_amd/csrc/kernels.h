@@ -135,7 +135,8 @@ __host__ __device__ inline const double* fold_xrow(const FoldOp& o) {
     return o.A + size_t(2 * o.pb) * o.ksp * 64;
 }
 
-// Mode space walked by the Helmholtz and Poisson tiles.  The fields are transforms of real
+// Mode space walked by the Helmholtz and Poisson tiles and by the step's vorticity
+// z-derivative.  The fields are transforms of real
 // planes, and the step keeps them so: on the kx = 0 and kx = nx/2 columns the modes
 // ky > ny/2 are the conjugates of ky < ny/2, and the four self-conjugate modes (0,0),
 // (0,ny/2), (nx/2,0), (nx/2,ny/2) are real.  With on (ny even), the tiles walk P = (nx/2) ny
