@@ -121,6 +121,9 @@ def lib():
         L.dns_debug_mfma_probe_f32.argtypes = [_dp]
         L.dns_debug_operator_error.argtypes = [_vp, _dp]
         L.dns_debug_zmap.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp, _vp, _vp]
+        L.dns_debug_wavenumbers.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
+        L.dns_debug_dealias_mask.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+        L.dns_debug_context_wavenumbers.argtypes = [_vp, _vp, _vp]
         L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_profile_kernel_name.argtypes = [ctypes.c_int]
         L.dns_profile_kernel_name.restype = ctypes.c_char_p
@@ -341,6 +344,12 @@ class Simulation:
         self._check(lib().dns_debug_operator_error(self._h, out))
         return out[0], out[1]
 
+    def device_wavenumbers(self):
+        """The context's device-resident xw, yw tables, copied back."""
+        xw = np.empty(self.nx // 2 + 1); yw = np.empty(self.ny)
+        self._check(lib().dns_debug_context_wavenumbers(self._h, xw.ctypes.data, yw.ctypes.data))
+        return xw, yw
+
 
 def host_initial_field(**kw):
     """initialize_fields_3d evaluated point by point on the host (include/f90dns_debug.h)."""
@@ -364,6 +373,25 @@ def zmap(nx, ny):
     if rc != 0:
         raise DnsError(rc, "dns_debug_zmap")
     return so[:n.value], s2o[:n.value], s2s[:n.value]
+
+
+def wavenumbers(nx, ny, xlen, ylen):
+    """xw(nx/2+1), yw(ny) as the library computes and uploads them (host only)."""
+    xw = np.empty(nx // 2 + 1); yw = np.empty(ny)
+    rc = lib().dns_debug_wavenumbers(nx, ny, xlen, ylen, xw.ctypes.data, yw.ctypes.data)
+    if rc != 0:
+        raise DnsError(rc, "dns_debug_wavenumbers")
+    return xw, yw
+
+
+def dealias_mask(nx, ny, which=1):
+    """The 2/3 mask (ny, nx/2+1), 1 = kept: which 0 = the two-pass transforms' table, 1 = the
+    plane kernels' bounds rule (host only)."""
+    m = np.empty((ny, nx // 2 + 1), dtype=np.uint8)
+    rc = lib().dns_debug_dealias_mask(nx, ny, which, m.ctypes.data)
+    if rc != 0:
+        raise DnsError(rc, "dns_debug_dealias_mask")
+    return m
 
 
 def estimate_bytes(nmembers=1, **kw):

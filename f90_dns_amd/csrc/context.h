@@ -41,6 +41,9 @@ struct dns_ctx {
     double* io = nullptr;         // one member's spectral scratch for the two-pass FP64 transforms
     dns::FlowState* flow = nullptr;
     dns::FlowMember* fmem = nullptr;
+    // members whose device FlowState::zero_wallgrad is set (fields set with p): cleared on the
+    // stream after the next pressure correction
+    std::vector<char> zwg;
     double *red_partial = nullptr, *red_result = nullptr;
     int red_blocks = 512;
     std::vector<void*> allocs;
@@ -69,6 +72,9 @@ FlowArgs ctx_flow_args(dns_ctx* c);
 HelmArgs ctx_helm_args(dns_ctx* c);
 PoisArgs ctx_pois_args(dns_ctx* c, int fused);
 Mode0Args ctx_mode0_args(dns_ctx* c, int fused);
+// after a step's pressure correction is enqueued on s: clears the zero-wall-gradient state of
+// members whose fields were set with p (no-op otherwise)
+int ctx_after_correction(dns_ctx* c, hipStream_t s);
 }  // namespace dns
 
 void dns_set_create_error(const std::string& msg);

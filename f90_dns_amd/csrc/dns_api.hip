@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -247,6 +248,7 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     for (double** q : specs) (void)hipMemset(*q, 0, NSB * sizeof(double));
     std::vector<FlowState> fs(c->B);
     std::vector<FlowMember> fm(c->B);
+    c->zwg.assign(c->B, 0);
     for (int b = 0; b < c->B; b++) {
         const dns_params& q = params[b];
         fs[b] = FlowState{};
@@ -328,6 +330,29 @@ static int c2r_planes(dns_ctx* c, const double* src, double* dst, int nplanes) {
 
 static bool bad_member(const dns_ctx* c, int b) { return !c || b < 0 || b >= c->B; }
 
+// A pressure field set from outside (a restart read) leaves the wall pressure gradient of the
+// Helmholtz boundary values at zero until the next pressure correction, as grad_p_prev is in
+// the reference (zeroed by initialize_fields_3d :957 and written only by pressure_correction_3d
+// :1923; read_restart_file_3d does not restore it).
+static int mark_zero_wallgrad(dns_ctx* c, int b) {
+    const int one = 1;
+    HIPCHECK(c, hipMemcpyAsync(reinterpret_cast<char*>(c->flow + b) + offsetof(FlowState, zero_wallgrad), &one,
+                               sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    c->zwg[b] = 1;
+    return 0;
+}
+
+// after a pressure correction on the stream: the wall gradients follow p again
+extern "C++" int dns::ctx_after_correction(dns_ctx* c, hipStream_t s) {
+    bool any = false;
+    for (char f : c->zwg) any = any || f;
+    if (!any) return 0;
+    HIPCHECK(c, launch_clear_wallgrad(c->flow, c->B, s));
+    c->zwg.assign(c->B, 0);
+    return 0;
+}
+
 int dns_set_member_fields(dns_ctx* c, int b, const double* u, const double* v, const double* w, const double* p) {
     if (bad_member(c, b)) return DNS_EINVAL;
     const double* in[4] = {u, v, w, p};
@@ -336,6 +361,10 @@ int dns_set_member_fields(dns_ctx* c, int b, const double* u, const double* v, c
         if (!in[f]) continue;
         HIPCHECK(c, hipMemcpyAsync(c->phys[0], in[f], c->N * sizeof(double), hipMemcpyHostToDevice, c->stream));
         int r = r2c_planes(c, c->phys[0], outs[f] + size_t(b) * c->NS, c->nz);
+        if (r) return r;
+    }
+    if (p) {
+        int r = mark_zero_wallgrad(c, b);
         if (r) return r;
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -373,6 +402,10 @@ int dns_set_fields_device(dns_ctx* c, const double* u, const double* v, const do
         int r = r2c_planes(c, in[f], outs[f], c->nz);
         if (r) return r;
     }
+    if (p) {
+        int r = mark_zero_wallgrad(c, 0);
+        if (r) return r;
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -407,6 +440,7 @@ int dns_set_member_controller(dns_ctx* c, int b, double G, double integral, doub
     FlowState fs{};
     fs.G = G; fs.err_int = integral; fs.err_prev = prev; fs.gsum = gsum; fs.gcount = gcount;
     fs.gavg = gcount > 0 ? gsum / double(gcount) : G;
+    fs.zero_wallgrad = c->zwg[b];
     HIPCHECK(c, hipMemcpyAsync(c->flow + b, &fs, sizeof(fs), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -433,6 +467,7 @@ int dns_set_member_params(dns_ctx* c, int b, const dns_params* q) {
     FlowState fs{};
     fs.G = q->target_pressure_gradient;
     fs.gavg = q->target_pressure_gradient;
+    c->zwg[b] = 0;
     FlowMember fm{q->target_pressure_gradient, q->target_bulk_velocity, q->controller_gain, q->re,
                   q->flow_control_method, q->controller_update_freq, q->istart, 0};
     HIPCHECK(c, hipMemcpy(c->flow + b, &fs, sizeof(fs), hipMemcpyHostToDevice));
@@ -655,7 +690,7 @@ int dns_pressure_correction(dns_ctx* c) {
     a.us = c->us; a.vs = c->vs; a.ws = c->ws; a.phi = c->phi; a.dphi = c->dphi;
     a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph; a.dt = c->p.dt;
     HIPCHECK(c, launch_correction(a, c->stream));
-    return 0;
+    return ctx_after_correction(c, c->stream);
 }
 
 int dns_apply_boundary_conditions(dns_ctx* c) {
@@ -664,11 +699,16 @@ int dns_apply_boundary_conditions(dns_ctx* c) {
     return 0;
 }
 
-static int one_step_fused(dns_ctx* c) {
+// the captured step: no host logic after the launches
+static int one_step_kernels(dns_ctx* c) {
     int r;
     if ((r = source_terms(c, false, false))) return r;
     if ((r = viscous(c, true))) return r;
     return pressure(c, 1);
+}
+static int one_step_fused(dns_ctx* c) {
+    int r = one_step_kernels(c);
+    return r ? r : ctx_after_correction(c, c->stream);
 }
 
 static int ensure_events(dns_ctx* c, int n) {
@@ -710,7 +750,7 @@ int dns_step(dns_ctx* c, int nsteps) {
 static int ensure_graph(dns_ctx* c) {
     if (c->gexec) return 0;
     HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int r = one_step_fused(c);
+    const int r = one_step_kernels(c);
     hipGraph_t gph = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &gph);
     if (r) {
@@ -732,6 +772,7 @@ int dns_step_graph(dns_ctx* c, int nsteps) {
     for (int s = 0; s < nsteps; s++) {
         dns_begin_step(c);
         HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
+        if ((r = ctx_after_correction(c, c->stream))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[s + 1], c->stream));
     }
     c->step_ms.clear();
@@ -746,7 +787,7 @@ int dns_enqueue_step(dns_ctx* c, int use_graph) {
     dns_begin_step(c);
     if (use_graph) {
         HIPCHECK(c, hipGraphLaunch(c->gexec, c->stream));
-        return 0;
+        return ctx_after_correction(c, c->stream);
     }
     return one_step_fused(c);
 }
@@ -925,6 +966,7 @@ int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
         if ((r = viscous(c, true))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[5], c->stream));
         if ((r = pressure(c, 1))) return r;
+        if ((r = ctx_after_correction(c, c->stream))) return r;
         HIPCHECK(c, hipEventRecord(c->ev[6], c->stream));
         HIPCHECK(c, hipEventRecord(c->ev[7], c->stream));
         HIPCHECK(c, hipEventSynchronize(c->ev[7]));
@@ -1008,6 +1050,39 @@ static void unswizzle(const FoldedA& a, std::vector<double>& m) {
                                                                    : get(r / 16, r, c);
     for (int r = 0; r < a.rows1; r++)
         for (int c = 0; c < a.K; c++) m[size_t(a.rows0 + r) * a.K + c] = get(a.pb + r / 16, r, c);
+}
+
+int dns_debug_wavenumbers(int nx, int ny, double xlen, double ylen, double* xw, double* yw) {
+    if (nx < 2 || ny < 2 || !xw || !yw || !(xlen > 0.0) || !(ylen > 0.0)) return DNS_EINVAL;
+    std::vector<double> x, y;
+    wavenumbers(nx, ny, xlen, ylen, x, y);
+    std::memcpy(xw, x.data(), x.size() * sizeof(double));
+    std::memcpy(yw, y.data(), y.size() * sizeof(double));
+    return 0;
+}
+
+int dns_debug_dealias_mask(int nx, int ny, int which, unsigned char* mask) {
+    if (nx < 2 || ny < 2 || !mask || (which != 0 && which != 1)) return DNS_EINVAL;
+    const int nxh = nx / 2 + 1;
+    if (which == 0) {
+        std::vector<unsigned char> m;
+        dealias_mask(nx, ny, m);
+        std::memcpy(mask, m.data(), m.size());
+        return 0;
+    }
+    Dealias d{true, 0, 0, ny};
+    dealias_bounds(nx, ny, d.kxm, d.kym);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nxh; i++) mask[i + size_t(nxh) * j] = d.keep(i, j) ? 1 : 0;
+    return 0;
+}
+
+int dns_debug_context_wavenumbers(dns_ctx* c, double* xw, double* yw) {
+    if (!c || !xw || !yw) return DNS_EINVAL;
+    HIPCHECK(c, hipMemcpyAsync(xw, c->xw, size_t(c->nxh) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(yw, c->yw, size_t(c->ny) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 int dns_debug_zmap(int nx, int ny, int* ncols, int* so, int* s2o, double* s2s) {

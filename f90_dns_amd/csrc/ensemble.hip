@@ -360,6 +360,8 @@ int Ensemble::step(int n, bool use_graph) {
         } else if ((r = enqueue_step())) {
             return r;
         }
+        for (auto& g : groups)
+            if (g.c && (r = ctx_after_correction(g.c, s0))) return r;
         ECHECK(hipEventRecord(ev[s + 1], s0));
         for (auto& g : groups)
             if (g.c) dns_begin_step(g.c);

@@ -284,8 +284,8 @@ int fft_class_capacity(int c, bool f32) {
 
 void fft_fill(const FFTGeom& g, FFTArgs& a, bool f32) {
     a.nx = g.nx; a.ny = g.ny; a.nxh = g.nxh; a.M = g.M;
-    // bounds of the 2/3 rule (setup.cpp dealias_mask)
-    a.kxm = (g.nx / 2) * 2 / 3; a.kym = (g.ny / 2) * 2 / 3;
+    // bounds of the 2/3 rule (the same as setup.cpp dealias_mask's table)
+    dealias_bounds(g.nx, g.ny, a.kxm, a.kym);
     a.py = g.py[f32 ? 1 : 0]; a.px = g.px[f32 ? 1 : 0];
     if (f32) { a.twx = g.twx32; a.twy = g.twy32; }
     else { a.twx = g.twx; a.twy = g.twy; }

@@ -706,12 +706,6 @@ template <class V> struct SplitLoad {
     }
 };
 
-// the 2/3 dealias rule (FFTArgs::kxm, kym) when on: is kx index i, ky index j kept
-struct Dealias {
-    bool on; int kxm, kym, ny;
-    __device__ __forceinline__ bool keep(int i, int j) const { return !on || (i <= kxm && min(j, ny - j) <= kym); }
-};
-
 // last y stage of r2c: columns b > 0 to HBM (dealias rule applied), column 0 back to LDS
 template <class V, class D> struct R2CStore {
     static constexpr bool kLds = true;

@@ -55,6 +55,23 @@ struct FFTGeom {
 
 // Operands of the plane kernels.  src/dst hold double or float data: bit f of src_f32 marks
 // src[f] as float; dst_f32 marks all destinations as float.  Strides are in elements.
+// The 2/3 rule of apply_dealias_truncation (DNS_pressure_BC_3D.f90:1163-1193): integer bounds
+// kx_max = (nx/2)*2/3, ky_max = (ny/2)*2/3 (:1169-1170); kx index i is kept when i <= kx_max,
+// ky index j when j <= ky_max for j <= ny/2 and |j - ny| <= ky_max above (:1176-1190), i.e.
+// min(j, ny - j) <= ky_max.
+__host__ __device__ inline void dealias_bounds(int nx, int ny, int& kxm, int& kym) {
+    kxm = (nx / 2) * 2 / 3;
+    kym = (ny / 2) * 2 / 3;
+}
+// the rule as the plane kernels apply it (FFTArgs::kxm, kym) when on
+struct Dealias {
+    bool on; int kxm, kym, ny;
+    __host__ __device__ __forceinline__ bool keep(int i, int j) const {
+        const int jj = j < ny - j ? j : ny - j;
+        return !on || (i <= kxm && jj <= kym);
+    }
+};
+
 struct FFTArgs {
     int nx, ny, nxh, M;
     Plan py, px;
@@ -368,16 +385,22 @@ hipError_t launch_correction(const CorrArgs& a, hipStream_t s);
 hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nmembers, double* u, double* v, double* w,
                              hipStream_t s);
 
-// PI / constant forcing state, device resident (update_pressure_gradient_3d), one per member
+// PI / constant forcing state, device resident (update_pressure_gradient_3d), one per member.
+// zero_wallgrad: the Helmholtz wall values of the next step use a zero pressure gradient (the
+// reference's grad_p_prev after initialize_fields_3d or a restart read, which is set only by
+// pressure_correction_3d, DNS_pressure_BC_3D.f90:957,1923); 0 once a pressure correction has
+// run, when the wall gradient is that of the resident p.
 struct FlowState {
     double G, Ub, err, err_int, err_prev, gsum, gavg;
-    int gcount, pad;
+    int gcount, zero_wallgrad;
 };
 struct FlowMember {
     double target_G, target_Ub, gain, re;
     int method, freq, istart, pad;
 };
 hipError_t launch_flow_control(const FlowArgs& a, int nmembers, hipStream_t s);
+// zero_wallgrad = 0 for members [0, nmembers) (a small kernel: no memset node in captured graphs)
+hipError_t launch_clear_wallgrad(FlowState* st, int nmembers, hipStream_t s);
 
 // ragged variants of the step kernels (var = zd/helm/pois_variant of the groups; lds = dynamic
 // LDS bytes, the largest over the launch's groups)

@@ -442,6 +442,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     // per-lane column constants for the two 16-column tiles (reads clamped into range and
     // issued together; a column past the plane gets zeros)
     double k2c[CT], bc1[CT], bc2[CT];
+    const bool zwg = a.flow[mb].zero_wallgrad != 0;   // grad p of the walls still zero (restart)
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb), colc = min(col, NC - 1);
@@ -451,7 +452,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         const double2 p0 = reinterpret_cast<const double2*>(ph)[mode];
         const double2 p1 = reinterpret_cast<const double2*>(ph + size_t(nz - 1) * H2)[mode];
         const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
-        const bool in = col < NC, lift = in && comp < 2;
+        const bool in = col < NC, lift = in && comp < 2 && !zwg;
         k2c[ct] = in ? kx * kx + ky * ky : 0.0;
         bc1[ct] = lift ? (part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x)) : 0.0;   // -dt * (i ke p)
         bc2[ct] = lift ? (part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x)) : 0.0;
@@ -1026,6 +1027,10 @@ __global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, long long ms
 }
 
 __global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) { flow_tile(a, blockIdx.x); }
+__global__ __launch_bounds__(64) void k_clear_wallgrad(FlowState* st, int n) {
+    const int b = blockIdx.x * 64 + threadIdx.x;
+    if (b < n) st[b].zero_wallgrad = 0;
+}
 
 // explicit roundings (no contraction into fma) reproduce the host generator bit for bit
 __global__ __launch_bounds__(256) void k_init_fields(InitArgs a) {
@@ -1202,6 +1207,11 @@ hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nm, double* 
 }
 hipError_t launch_flow_control(const FlowArgs& a, int nm, hipStream_t s) {
     hipLaunchKernelGGL(k_flow_control, dim3(nm), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_clear_wallgrad(FlowState* st, int nm, hipStream_t s) {
+    if (nm <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_wallgrad, dim3((nm + 63) / 64), dim3(64), 0, s, st, nm);
     return hipGetLastError();
 }
 hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int nt, int nf, size_t lds,

@@ -48,6 +48,16 @@ int dns_debug_operator_error(dns_ctx* ctx, double* out2);
  * hold 2 (nx/2+1) ny entries.  No counterpart in the reference (a layout of this library). */
 int dns_debug_zmap(int nx, int ny, int* ncols, int* so, int* s2o, double* s2s);
 
+/* The wavenumbers of setup_spectral_3d (DNS_pressure_BC_3D.f90:780-796) as the library computes
+ * and uploads them: xw(nx/2+1), yw(ny); host only. */
+int dns_debug_wavenumbers(int nx, int ny, double xlen, double ylen, double* xw, double* yw);
+/* The 2/3 dealias mask of apply_dealias_truncation (DNS_pressure_BC_3D.f90:1163-1193) as
+ * (nx/2+1, ny) bytes, kx fastest, 1 = kept: which 0 = the table the two-pass transforms read,
+ * 1 = the bounds and test the plane kernels apply; host only. */
+int dns_debug_dealias_mask(int nx, int ny, int which, unsigned char* mask);
+/* The device-resident wavenumber tables of a context, copied back: xw(nx/2+1), yw(ny). */
+int dns_debug_context_wavenumbers(dns_ctx* ctx, double* xw, double* yw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,13 +37,15 @@ static const double PI = 3.141592653589793115997963468544185161590576171875; /* 
 /* ------------------------------------------------------------------------- */
 typedef struct { int n, nf, f[48]; cpx* tw; } fplan;   /* tw[k] = exp(-2 pi i k/n) */
 
-static void fplan_init(fplan* p, int n) {
+static void fplan_init(fplan* p, int n, int variant) {
     int m = n, q;
     p->n = n; p->nf = 0;
-    while (m % 4 == 0) { p->f[p->nf++] = 4; m /= 4; }
+    if (variant != 1) while (m % 4 == 0) { p->f[p->nf++] = 4; m /= 4; }
     while (m % 2 == 0) { p->f[p->nf++] = 2; m /= 2; }
     for (q = 3; q * q <= m; q += 2) while (m % q == 0) { p->f[p->nf++] = q; m /= q; }
     if (m > 1) p->f[p->nf++] = m;
+    if (variant == 1)               /* radix 2 only, odd factors first: other rounding */
+        for (int a = 0, b = p->nf - 1; a < b; a++, b--) { int t = p->f[a]; p->f[a] = p->f[b]; p->f[b] = t; }
     p->tw = (cpx*)malloc(sizeof(cpx) * (size_t)n);
     for (int k = 0; k < n; k++) {
         double ang = -2.0 * PI * (double)k / (double)n;
@@ -114,9 +116,9 @@ static void fft_exec(const fplan* p, cpx* data, cpx* work, int sign) {
 /* 2D plane transforms on real(nx,ny) <-> complex(nx/2+1,ny), FFTW conventions */
 typedef struct { int nx, ny, nxhp; fplan px, py; } plan2d;
 
-static void plan2d_init(plan2d* p, int nx, int ny) {
+static void plan2d_init(plan2d* p, int nx, int ny, int variant) {
     p->nx = nx; p->ny = ny; p->nxhp = nx / 2 + 1;
-    fplan_init(&p->px, nx); fplan_init(&p->py, ny);
+    fplan_init(&p->px, nx, variant); fplan_init(&p->py, ny, variant);
 }
 static void plan2d_free(plan2d* p) { fplan_free(&p->px); fplan_free(&p->py); }
 
@@ -321,7 +323,7 @@ struct orc_ctx {
     /* state (DNS_pressure_BC_3D.f90:113-120) */
     double *u, *v, *w, *p_total, *ustar, *vstar, *wstar, *phi, *su, *sv, *sw;
     double *gpx, *gpy;              /* grad p_total, wall planes k = 1 and nz only */
-    double G, Ub, err, err_int, err_prev, gsum, gavg; int gcount;
+    orc_pi_state pi;
     double time; int step;
     /* scratch */
     double *f1, *f2, *f3, *f4, *f5, *f6, *f7, *f8, *f9;
@@ -368,7 +370,7 @@ orc_ctx* orc_create(const orc_params* p) {
     }
     c->mask = (unsigned char*)malloc((size_t)c->nspl);
     orc_dealias_mask(nx, ny, c->mask);
-    plan2d_init(&c->plan, nx, ny);
+    plan2d_init(&c->plan, nx, ny, p->fft_variant);
 #ifdef _OPENMP
     c->nthreads = omp_get_max_threads();
 #else
@@ -383,7 +385,7 @@ orc_ctx* orc_create(const orc_params* p) {
     c->f6 = dalloc(N); c->f7 = dalloc(N); c->f8 = dalloc(N); c->f9 = dalloc(N);
     c->spec = (cpx*)calloc(c->NS, sizeof(cpx)); c->spec2 = (cpx*)calloc(c->NS, sizeof(cpx));
     /* read_input_3d (DNS_pressure_BC_3D.f90:629-641) */
-    c->G = p->target_pressure_gradient; c->gavg = p->target_pressure_gradient;
+    c->pi.G = p->target_pressure_gradient; c->pi.gavg = p->target_pressure_gradient;
     return c;
 }
 
@@ -478,35 +480,51 @@ static void derivatives(orc_ctx* c, const double* f, double* dx, double* dy, dou
 }
 
 /* calculate_bulk_velocity_3d (DNS_pressure_BC_3D.f90:2434-2464) */
-static double bulk_velocity(const orc_ctx* c) {
+static double bulk_of(int nx, int ny, int nz, const double* u, const double* wz) {
     double vol = 0.0, ws = 0.0;
-    for (int k = 0; k < c->nz; k++) {
-        for (int j = 0; j < c->ny; j++)
-            for (int i = 0; i < c->nx; i++)
-                vol += c->u[(size_t)i + (size_t)c->nx * ((size_t)j + (size_t)c->ny * k)] * c->wz[k];
-        ws += c->wz[k];
+    for (int k = 0; k < nz; k++) {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+                vol += u[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] * wz[k];
+        ws += wz[k];
     }
-    return vol / ((double)(c->nx * c->ny) * ws);
+    return vol / ((double)(nx * ny) * ws);
+}
+static double bulk_velocity(const orc_ctx* c) { return bulk_of(c->nx, c->ny, c->nz, c->u, c->wz); }
+
+double orc_bulk_velocity(int nx, int ny, int nz, const double* u) {
+    double* z = dalloc(nz); double* wz = dalloc(nz);
+    orc_lgl(nz, z, wz);
+    double ub = bulk_of(nx, ny, nz, u, wz);
+    free(z); free(wz);
+    return ub;
 }
 
-/* update_pressure_gradient_3d (DNS_pressure_BC_3D.f90:2488-2549), called with istart (:1118) */
-static void update_pressure_gradient(orc_ctx* c, int istep) {
-    if (istep % c->p.controller_update_freq != 0) return;
-    c->Ub = bulk_velocity(c);
-    c->err = c->p.target_bulk_velocity - c->Ub;
-    double kp = c->p.controller_gain, ki = 0.05 * c->p.controller_gain;
-    double base = 3.0 / c->p.re;
-    double prop = kp * c->err, integ = ki * c->err_int;
+/* update_pressure_gradient_3d (DNS_pressure_BC_3D.f90:2488-2549) */
+int orc_pi_update(const orc_params* p, orc_pi_state* s, int istep, double ub) {
+    if (istep % p->controller_update_freq != 0) return 0;
+    s->Ub = ub;
+    s->err = p->target_bulk_velocity - s->Ub;
+    double kp = p->controller_gain, ki = 0.05 * p->controller_gain;
+    double base = 3.0 / p->re;
+    double prop = kp * s->err, integ = ki * s->err_int;
     double gu = base + prop + integ;
     double gc = fmax(0.0, fmin(10.0, gu));
-    if (fabs(gc - gu) < 1.0e-12) {
-        if (istep <= c->p.controller_update_freq) c->err_int = c->err_int + c->err * c->p.dt;
-        else c->err_int = c->err_int + 0.5 * c->p.dt * (c->err_prev + c->err);
+    if (fabs(gc - gu) < 1.0e-12) {                       /* anti-windup: frozen while clamped */
+        if (istep <= p->controller_update_freq) s->err_int = s->err_int + s->err * p->dt;   /* Euler */
+        else s->err_int = s->err_int + 0.5 * p->dt * (s->err_prev + s->err);               /* trapezoid */
     }
-    c->err_prev = c->err;
-    c->G = gc;
-    c->gsum += c->G; c->gcount += 1;
-    c->gavg = c->gsum / (double)c->gcount;
+    s->err_prev = s->err;
+    s->G = gc;
+    s->gsum += s->G; s->gcount += 1;
+    s->gavg = s->gsum / (double)s->gcount;
+    return 1;
+}
+
+/* called with istart, not the step number (DNS_pressure_BC_3D.f90:1118) */
+static void update_pressure_gradient(orc_ctx* c, int istep) {
+    if (istep % c->p.controller_update_freq != 0) return;
+    orc_pi_update(&c->p, &c->pi, istep, bulk_velocity(c));
 }
 
 static void dealias_field(orc_ctx* c, double* f) {
@@ -546,9 +564,9 @@ void orc_compute_source_terms(orc_ctx* c) {
         dealias_field(c, c->su); dealias_field(c, c->sv); dealias_field(c, c->sw);
     }
     /* apply_flow_control_forcing (:1107-1148) */
-    if (c->p.flow_control_method == 1) c->G = c->p.target_pressure_gradient;
+    if (c->p.flow_control_method == 1) c->pi.G = c->p.target_pressure_gradient;
     else update_pressure_gradient(c, c->p.istart);
-    double G = c->G;
+    double G = c->pi.G;
     for (size_t n = 0; n < N; n++) c->su[n] = c->su[n] + G;
 }
 
@@ -590,7 +608,7 @@ static void helmholtz_matrix(const orc_ctx* c, double factor, int i, int j, doub
 
 /* solve_viscous_helmholtz (DNS_pressure_BC_3D.f90:1425-1521) */
 static void helmholtz(orc_ctx* c, double* fout, const double* rhs, double theta, const double* bc1, const double* bc2) {
-    int nz = c->nz, nxhp = c->nxhp, ny = c->ny, nspl = c->nspl;
+    int nz = c->nz, nxhp = c->nxhp, nspl = c->nspl;
     double factor = theta * c->p.dt / c->p.re;
     r2c_field(c, rhs, c->spec);
     cpx* bh1 = (cpx*)malloc(sizeof(cpx) * (size_t)nspl);
@@ -605,11 +623,18 @@ static void helmholtz(orc_ctx* c, double* fout, const double* rhs, double theta,
     if (cached && !c->hready) {
         c->hlu = dalloc((size_t)nz * nz * nspl); c->hpiv = (int*)calloc((size_t)nz * nspl, sizeof(int));
     }
+    /* the reference's mode loop is serial (:1475); with cached factors the modes are
+     * independent, so they run in parallel (bitwise the same per mode) */
+    const int was_ready = c->hready;
+    #pragma omp parallel if (cached)
+    {
     double* hm = dalloc((size_t)nz * nz);
     int* piv = (int*)calloc(nz, sizeof(int));
     double* br = dalloc(nz); double* bi = dalloc(nz);
-    for (int j = 0; j < ny; j++) {           /* serial mode loop, as in the reference */
-        for (int i = 0; i < nxhp; i++) {
+    #pragma omp for schedule(dynamic, 8)
+    for (int mi = 0; mi < nspl; mi++) {
+        const int i = mi % nxhp, j = mi / nxhp;
+        {
             size_t m0 = (size_t)i + (size_t)nxhp * j;
             double* a = cached ? c->hlu + (size_t)nz * nz * m0 : hm;
             int* pv = cached ? c->hpiv + (size_t)nz * m0 : piv;
@@ -628,19 +653,25 @@ static void helmholtz(orc_ctx* c, double* fout, const double* rhs, double theta,
             }
             br[0] = bh1[m0].re; bi[0] = bh1[m0].im;
             br[nz - 1] = bh2[m0].re; bi[nz - 1] = bh2[m0].im;
-            if (!cached || !c->hready) {
+            if (!cached || !was_ready) {
                 for (int n = 0; n < nz; n++) { hm[0 + nz * n] = 0.0; hm[(nz - 1) + nz * n] = 0.0; }
                 hm[0] = 1.0; hm[(nz - 1) + nz * (nz - 1)] = 1.0;
                 if (cached) memcpy(a, hm, sizeof(double) * (size_t)nz * nz);
-                { int info = lu_factor(nz, a, pv); if (info && !c->status) c->status = info; /* the reference stops here */ }
+                int info = lu_factor(nz, a, pv);
+                if (info) {
+                    #pragma omp critical
+                    { if (!c->status) c->status = info; }   /* the reference stops here */
+                }
             }
             lu_solve(nz, a, pv, br); lu_solve(nz, a, pv, bi);
             for (int k = 0; k < nz; k++) { c->spec2[m0 + (size_t)nspl * k].re = br[k]; c->spec2[m0 + (size_t)nspl * k].im = bi[k]; }
         }
     }
+    free(hm); free(piv); free(br); free(bi);
+    }
     if (cached) c->hready = 1;
     c2r_field(c, c->spec2, fout);
-    free(hm); free(piv); free(br); free(bi); free(bh1); free(bh2);
+    free(bh1); free(bh2);
 }
 
 /* viscous_step_3d (DNS_pressure_BC_3D.f90:1285-1332) */
@@ -679,7 +710,7 @@ static void divergence(orc_ctx* c, const double* fx, const double* fy, const dou
  * rhs_k = [-(i xw u^ + i yw v^)_k w_k ybar/2 - sum_m D1(m,k) w^_m w_m 2/ybar] / dt from the
  * transformed u*, v*, w* (:1789-1802), with the same matrix and mode-(1,1) treatment. */
 void orc_solve_pressure(orc_ctx* c) {
-    int nz = c->nz, nxhp = c->nxhp, ny = c->ny, nspl = c->nspl;
+    int nz = c->nz, nxhp = c->nxhp, nspl = c->nspl;
     const int weak = c->p.use_weak_form_divergence;
     double* div = c->f7;
     cpx *vh = NULL, *wh = NULL;
@@ -697,15 +728,21 @@ void orc_solve_pressure(orc_ctx* c) {
     if (cached && !c->pready) {
         c->plu = dalloc((size_t)nz * nz * nspl); c->ppiv = (int*)calloc((size_t)nz * nspl, sizeof(int));
     }
+    /* serial in the reference (:1680); parallel over independent modes with cached factors */
+    const int was_ready = c->pready;
+    #pragma omp parallel if (cached)
+    {
     double* pm = dalloc((size_t)nz * nz);
     int* piv = (int*)calloc(nz, sizeof(int));
     double* br = dalloc(nz); double* bi = dalloc(nz);
-    for (int j = 0; j < ny; j++) {
-        for (int i = 0; i < nxhp; i++) {
+    #pragma omp for schedule(dynamic, 8)
+    for (int mi = 0; mi < nspl; mi++) {
+        const int i = mi % nxhp, j = mi / nxhp;
+        {
             size_t m0 = (size_t)i + (size_t)nxhp * j;
             double* a = cached ? c->plu + (size_t)nz * nz * m0 : pm;
             int* pv = cached ? c->ppiv + (size_t)nz * m0 : piv;
-            if (!cached || !c->pready) {
+            if (!cached || !was_ready) {
                 /* the O(nz^3) rebuild per mode (:1684-1692) */
                 for (int k = 0; k < nz; k++) {
                     for (int n = 0; n < nz; n++) {
@@ -720,7 +757,11 @@ void orc_solve_pressure(orc_ctx* c) {
                     pm[0] = 1.0;
                 }
                 if (cached) memcpy(a, pm, sizeof(double) * (size_t)nz * nz);
-                { int info = lu_factor(nz, a, pv); if (info && !c->status) c->status = info; /* the reference stops here */ }
+                int info = lu_factor(nz, a, pv);
+                if (info) {
+                    #pragma omp critical
+                    { if (!c->status) c->status = info; }   /* the reference stops here */
+                }
             }
             if (weak) {
                 const double kx = c->xw[i], ky = c->yw[j];
@@ -752,9 +793,11 @@ void orc_solve_pressure(orc_ctx* c) {
             for (int k = 0; k < nz; k++) { c->spec2[m0 + (size_t)nspl * k].re = br[k]; c->spec2[m0 + (size_t)nspl * k].im = bi[k]; }
         }
     }
+    free(pm); free(piv); free(br); free(bi);
+    }
     if (cached) c->pready = 1;
     c2r_field(c, c->spec2, c->phi);
-    free(pm); free(piv); free(br); free(bi); free(vh); free(wh);
+    free(vh); free(wh);
 }
 
 /* pressure_correction_3d (DNS_pressure_BC_3D.f90:1903-1929); grad p_total is kept
@@ -829,11 +872,11 @@ void orc_diagnostics(orc_ctx* c, orc_diag* d) {
     }
     e = e / (c->p.xlen * c->p.ylen * c->ybar);
     d->umax = um; d->vmax = vm; d->wmax = wm; d->energy = e;
-    d->pressure_gradient = c->G;
+    d->pressure_gradient = c->pi.G;
     d->bulk_velocity = bulk_velocity(c);
-    if (c->p.flow_control_method == 2) c->Ub = d->bulk_velocity;
-    d->avg_pressure_gradient = c->gavg;
-    d->bulk_error_integral = c->err_int; d->bulk_error_prev = c->err_prev;
+    if (c->p.flow_control_method == 2) c->pi.Ub = d->bulk_velocity;
+    d->avg_pressure_gradient = c->pi.gavg;
+    d->bulk_error_integral = c->pi.err_int; d->bulk_error_prev = c->pi.err_prev;
     d->time = c->time; d->step = c->step;
 }
 
@@ -880,8 +923,8 @@ int orc_get_wall_gradients(const orc_ctx* c, double* gpx, double* gpy) {
     return 0;
 }
 int orc_set_controller(orc_ctx* c, double G, double integral, double prev, double gsum, int gcount) {
-    c->G = G; c->err_int = integral; c->err_prev = prev; c->gsum = gsum; c->gcount = gcount;
-    c->gavg = gcount > 0 ? gsum / (double)gcount : G;
+    c->pi.G = G; c->pi.err_int = integral; c->pi.err_prev = prev; c->pi.gsum = gsum; c->pi.gcount = gcount;
+    c->pi.gavg = gcount > 0 ? gsum / (double)gcount : G;
     return 0;
 }
 
@@ -987,7 +1030,7 @@ int orc_init_fields(orc_ctx* c) {
 /* ------------------------------------------------------------------------- */
 int orc_fft_r2c(int nx, int ny, int nplanes, const double* in, double* out) {
     if ((nx & 1) || (ny & 1)) return 1;
-    plan2d p; plan2d_init(&p, nx, ny);
+    plan2d p; plan2d_init(&p, nx, ny, 0);
     int m = nx > ny ? nx : ny;
     cpx* buf = (cpx*)malloc(sizeof(cpx) * m); cpx* work = (cpx*)malloc(sizeof(cpx) * m);
     for (int k = 0; k < nplanes; k++)
@@ -997,7 +1040,7 @@ int orc_fft_r2c(int nx, int ny, int nplanes, const double* in, double* out) {
 }
 int orc_fft_c2r(int nx, int ny, int nplanes, const double* in, double* out) {
     if ((nx & 1) || (ny & 1)) return 1;
-    plan2d p; plan2d_init(&p, nx, ny);
+    plan2d p; plan2d_init(&p, nx, ny, 0);
     int m = nx > ny ? nx : ny;
     cpx* buf = (cpx*)malloc(sizeof(cpx) * m); cpx* work = (cpx*)malloc(sizeof(cpx) * m);
     cpx* tmp = (cpx*)malloc(sizeof(cpx) * (size_t)(nx / 2 + 1) * ny);

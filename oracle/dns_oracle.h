@@ -44,7 +44,17 @@ typedef struct orc_params {
     int refactor_every_step;       /* 1: rebuild + LU every mode every step, as written */
     int lgl_derivative;            /* 0: reference D1 (1e-15 denominator threshold), 1: exact LGL D1 */
     int use_weak_form_divergence;  /* 1: solve_pressure_3d_weak_form (DNS_pressure_BC_3D.f90:1749-1835) */
+    int fft_variant;               /* 0: Stockham factors 4s, 2s, odd primes; 1: radix 2 in place of 4,
+                                      odd factors first (an equivalent transform, other rounding) */
 } orc_params;
+
+/* PI controller state (DNS_pressure_BC_3D.f90:145-152): current_pressure_gradient,
+ * current_bulk_velocity, bulk_velocity_error, _integral, _prev, pressure_gradient_sum,
+ * pressure_gradient_count, avg_pressure_gradient. */
+typedef struct orc_pi_state {
+    double G, Ub, err, err_int, err_prev, gsum, gavg;
+    int gcount;
+} orc_pi_state;
 
 typedef struct orc_diag {
     double umax, vmax, wmax, energy;
@@ -81,6 +91,14 @@ double orc_max_divergence(orc_ctx* c);
 const double* orc_source_terms(const orc_ctx* c, int comp);   /* su, sv, sw after stage 1 */
 const double* orc_star_fields(const orc_ctx* c, int comp);    /* u*, v*, w* after stage 2 */
 const double* orc_phi(const orc_ctx* c);                      /* phi after stage 3 */
+
+/* update_pressure_gradient_3d (DNS_pressure_BC_3D.f90:2488-2549) for a given bulk velocity ub
+ * (the value calculate_bulk_velocity_3d would return): returns 1 when the controller ran, 0 when
+ * mod(istep, controller_update_freq) /= 0 left the state untouched. */
+int orc_pi_update(const orc_params* p, orc_pi_state* s, int istep, double ub);
+/* calculate_bulk_velocity_3d (DNS_pressure_BC_3D.f90:2434-2464) of a field u(nx,ny,nz) over the
+ * LGL weights of nz nodes. */
+double orc_bulk_velocity(int nx, int ny, int nz, const double* u);
 
 /* component-level entry points (unit parity) */
 void orc_lgl(int n, double* nodes, double* weights);

@@ -27,8 +27,14 @@ class OrcParams(ctypes.Structure):
         ("analytical_mode_x", ctypes.c_int), ("analytical_mode_y", ctypes.c_int),
         ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
         ("refactor_every_step", ctypes.c_int), ("lgl_derivative", ctypes.c_int),
-        ("use_weak_form_divergence", ctypes.c_int),
+        ("use_weak_form_divergence", ctypes.c_int), ("fft_variant", ctypes.c_int),
     ]
+
+
+class OrcPiState(ctypes.Structure):
+    _fields_ = [("G", ctypes.c_double), ("Ub", ctypes.c_double), ("err", ctypes.c_double),
+                ("err_int", ctypes.c_double), ("err_prev", ctypes.c_double), ("gsum", ctypes.c_double),
+                ("gavg", ctypes.c_double), ("gcount", ctypes.c_int)]
 
 
 class OrcDiag(ctypes.Structure):
@@ -88,6 +94,11 @@ def lib():
         L.orc_fft_r2c.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_fft_c2r.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_make_initial_field.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp, _dp]
+        L.orc_pi_update.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcPiState), ctypes.c_int,
+                                    ctypes.c_double]
+        L.orc_pi_update.restype = ctypes.c_int
+        L.orc_bulk_velocity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_bulk_velocity.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -104,7 +115,7 @@ DEFAULTS = dict(
     controller_update_freq=7, istart=0, u_centerline=1.0,
     enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1, analytical_mode_y=1,
     perturbation_amplitude=0.01, seed=0, noise_kmax=8, refactor_every_step=0,
-    lgl_derivative=0, use_weak_form_divergence=0,
+    lgl_derivative=0, use_weak_form_divergence=0, fft_variant=0,
 )
 
 
@@ -266,3 +277,29 @@ def initial_field(**kw):
     u, v, w = (np.empty(shape) for _ in range(3))
     lib().orc_make_initial_field(ctypes.byref(p), _ptr(u), _ptr(v), _ptr(w))
     return u, v, w
+
+
+class PIController:
+    """update_pressure_gradient_3d on a prescribed bulk-velocity sequence (no flow field):
+    the state starts as read_input_3d leaves it (G = average = target_pressure_gradient)."""
+
+    def __init__(self, **kw):
+        self.params = make_params(**kw)
+        self.state = OrcPiState(G=self.params.target_pressure_gradient, gavg=self.params.target_pressure_gradient)
+
+    def update(self, istep, ub):
+        ran = lib().orc_pi_update(ctypes.byref(self.params), ctypes.byref(self.state), istep, ub)
+        return bool(ran)
+
+    def __getattr__(self, name):
+        st = self.__dict__.get("state")
+        if st is not None and name in dict(OrcPiState._fields_):
+            return getattr(st, name)
+        raise AttributeError(name)
+
+
+def bulk_velocity(u):
+    """calculate_bulk_velocity_3d of u with shape (nz, ny, nx)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    nz, ny, nx = u.shape
+    return lib().orc_bulk_velocity(nx, ny, nz, _ptr(u))

@@ -28,6 +28,26 @@ def max_err(e):
     return max(v for k, v in e.items() if k != "w_self")
 
 
+# Bounds on w's error relative to its own norm.  w is ~1e-6 of u in these runs, so rounding
+# moves it at ~1e-10 of its own norm under any reordering of the arithmetic: the oracle against
+# its own twin with other plane-transform factorisations disagrees by 1.2e-10 in w after 100
+# config-1 steps (tests/test_oracle_pins.py::test_w_rounding_floor_of_the_oracle_itself) while
+# u, v, p agree to 1e-13.  FP64 paths are held to W_SELF_FP64 (a few times the largest such
+# floor seen, 2e-9 at 128^2 x 129 after 3 steps, where w is younger and smaller still); the
+# long runs compare against the floor measured for that very run.  Mixed precision rounds the
+# product in FP32 (6e-8 of |u x omega|), which is ~1e-1 of w's own source term: W_SELF_MIXED
+# records that known miss against the 1e-6 target (u, v, p and w on the velocity scale meet it).
+W_SELF_FP64 = 5e-9
+W_SELF_MIXED = 1e-3
+
+
+def check_parity(e, tol, w_self_tol):
+    """u, v, p relative L2, w on the velocity scale below tol; w's own relative L2 below
+    w_self_tol."""
+    assert max_err(e) < tol, e
+    assert e["w_self"] < w_self_tol, e
+
+
 # config 1 (32x32x33, Lx=2pi, Ly=pi, Re=180, dt=0.002, G=2/Re, analytical mx=my=1 amp 0.01, dealias)
 CONFIG1 = dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
                use_dealias_23_rule=1, flow_control_method=1, target_pressure_gradient=2.0 / 180.0,

@@ -110,12 +110,20 @@ def test_host_initial_field_matches_oracle_generator():
     for kw in [dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, enable_perturbations=1, perturbation_kind=0,
                     u_centerline=1.0, perturbation_amplitude=0.01),
                dict(nx=48, ny=36, nz=17, xlen=4 * np.pi, ylen=np.pi, enable_perturbations=1, perturbation_kind=1,
-                    seed=123, noise_kmax=5, perturbation_amplitude=0.05, u_centerline=1.5)]:
+                    seed=123, noise_kmax=5, perturbation_amplitude=0.05, u_centerline=1.5),
+               dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, enable_perturbations=1,
+                    perturbation_kind=1, seed=2024, noise_kmax=8, perturbation_amplitude=0.05, u_centerline=1.0),
+               dict(nx=96, ny=64, nz=65, xlen=2 * np.pi, ylen=np.pi, enable_perturbations=1, perturbation_kind=0,
+                    analytical_mode_x=2, analytical_mode_y=3, perturbation_amplitude=0.03, u_centerline=1.0),
+               dict(nx=16, ny=24, nz=9, enable_perturbations=0, u_centerline=1.0)]:
         u, v, w = dns.host_initial_field(**kw)
         o = orc.Oracle(**kw)
         o.init_fields()
         ou, ov, ow, _ = o.get_fields()
-        assert np.max(np.abs(u - ou)) < 1e-14 and np.max(np.abs(v - ov)) < 1e-14 and np.max(np.abs(w - ow)) == 0
+        # bit-identical: the same operations in the same order (initialize_fields_3d)
+        np.testing.assert_array_equal(u, ou)
+        np.testing.assert_array_equal(v, ov)
+        np.testing.assert_array_equal(w, ow)
 
 
 @pytest.mark.parametrize("nx,ny", [(4, 2), (8, 6), (32, 32), (48, 36), (64, 96), (128, 128), (192, 48), (32, 27),
@@ -144,3 +152,36 @@ def test_solve_mode_space_covers_a_real_planes_transform(nx, ny):
     m = s2o >= 0
     Y[s2o[m]] = s2s[m] * R[so[m]]
     np.testing.assert_allclose(Y, R, rtol=0, atol=1e-12 * np.abs(R).max())
+
+
+@pytest.mark.parametrize("nx,ny,xlen,ylen", [(32, 24, 6.283185307179586, 3.141592653589793),
+                                             (128, 96, 4 * 3.141592653589793, 2 * 3.141592653589793),
+                                             (48, 36, 1.7, 0.9), (192, 64, 12.566370614359172, 3.0)])
+def test_library_wavenumbers_bitwise_equal_oracle(nx, ny, xlen, ylen):
+    """xw, yw of setup_spectral_3d (DNS_pressure_BC_3D.f90:780-796) as the library uploads them,
+    bit for bit against the oracle's (ny != nx, so an x/y swap fails)."""
+    import numpy as np
+    import f90_dns_amd as dns
+    from oracle import oracle as orc
+    if not os.path.exists(dns.library_path()):
+        pytest.skip("libf90dns.so not built")
+    xw, yw = dns.wavenumbers(nx, ny, xlen, ylen)
+    oxw, oyw = orc.wavenumbers(nx, ny, xlen, ylen)
+    np.testing.assert_array_equal(xw, oxw)
+    np.testing.assert_array_equal(yw, oyw)
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 24), (24, 32), (128, 96), (96, 64), (192, 128), (64, 192), (48, 36), (34, 22),
+                                   (8, 6)])
+def test_library_dealias_mask_bitwise_equal_oracle(nx, ny):
+    """The 2/3 mask (apply_dealias_truncation, DNS_pressure_BC_3D.f90:1163-1193): the two-pass
+    transforms' table and the plane kernels' bounds rule, both bit for bit against the oracle,
+    with ny != nx (a kx/ky bound swap fails)."""
+    import numpy as np
+    import f90_dns_amd as dns
+    from oracle import oracle as orc
+    if not os.path.exists(dns.library_path()):
+        pytest.skip("libf90dns.so not built")
+    want = orc.dealias_mask(nx, ny)
+    for which in (0, 1):
+        np.testing.assert_array_equal(dns.dealias_mask(nx, ny, which), want)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import field_errors, max_err
+from tests.gpu_util import W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err
 
 pytestmark = pytest.mark.gpu
 
@@ -118,7 +118,7 @@ def test_ensemble_members_match_fp64_oracle(dns):
         o.step(30)
         e = field_errors(ens.get_fields(i), o.get_fields())
         _say(f"member {i} {ens.shape(i)} vs oracle after 30 steps: {e}")
-        assert max_err(e) < 1e-10, e
+        check_parity(e, 1e-10, W_SELF_FP64)
 
 
 def test_estimate_bytes_matches_allocation(dns):

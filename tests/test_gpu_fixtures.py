@@ -1,0 +1,140 @@
+"""Long runs at the production and served sizes against the FP64 oracle's fixtures.
+
+The oracle needs about a second per 128 x 128 x 129 step on the build container's cores, so
+these runs are compared with the compact fixtures tests/golden/make_oracle_fixtures.py wrote
+there (tests/oracle_fixtures.py: whole-field sketches, per-plane norms, one full plane, the
+diagnostics trace and the oracle's own rounding floor per field).  Every run starts from the
+device's initial field, which is bit for bit the oracle's (tests/test_abi.py)."""
+import sys
+
+import numpy as np
+import pytest
+
+from tests import oracle_fixtures as fx
+from tests.gpu_util import W_SELF_FP64, W_SELF_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dns():
+    import f90_dns_amd
+    f90_dns_amd.lib()
+    return f90_dns_amd
+
+
+def _check(F, fields, step, tol, w_self_tol, label):
+    e = F.compare(fields, step)
+    floor = F.floor(step)
+    _say(f"{label} after {step} steps: " + ", ".join(f"{k} {v:.2e}" for k, v in e.items()))
+    _say(f"  oracle rounding floor: {floor}")
+    for c in ("u", "v", "p"):
+        assert e[c] < tol, (c, e)                     # whole field (sketch estimate)
+        assert e[f"{c}_plane"] < tol, (c, e)          # one z-plane, exact
+        assert e[f"{c}_planes"] < tol, (c, e)         # every plane's norm
+    assert e["w"] < tol and e["w_plane"] < max(tol, w_self_tol), e
+    assert e["w_self"] < w_self_tol, e
+    return e
+
+
+def _trace_check(F, sim, step, rtol):
+    row = next(r for r in F.trace() if r["step"] == step)
+    d = sim.diagnostics()
+    for k in ("energy", "umax", "vmax", "pressure_gradient", "bulk_velocity", "avg_pressure_gradient"):
+        assert abs(d[k] - row[k]) <= rtol * max(abs(row[k]), 1e-300), (step, k, d[k], row[k])
+    assert abs(d["bulk_error_integral"] - row["bulk_error_integral"]) <= rtol * max(abs(row["bulk_error_integral"]),
+                                                                                     1e-12), step
+
+
+def test_config2_fp64_100_and_500_steps(dns):
+    """128 x 128 x 129, PI control, seeded noise: FP64 fields within 1e-10 of the oracle after
+    100 steps (w's own error within 4x the oracle's own floor), the energy / PI trace every 10
+    steps, and still within 1e-9 after 500 steps."""
+    F = fx.Fixture("config2")
+    g = dns.Simulation(**F.params)
+    g.init_fields()
+    done = 0
+    for step in F.steps():
+        while done < step:
+            g.step(10, graph=True)
+            done += 10
+            _trace_check(F, g, done, 1e-10)
+        fl = F.floor(step)
+        tol = 1e-10 if step <= 100 else 1e-9
+        _check(F, g.get_fields(), step, tol, max(4 * fl["w"], W_SELF_FP64 if step > 100 else 0.0), "FP64")
+    _say(f"  max divergence gpu {g.max_divergence():.3e} oracle {F.floor(F.steps()[-1])['max_divergence']:.3e}")
+    g.close()
+
+
+def test_config2_mixed_500_steps(dns):
+    """The same run in mixed precision: u, v, p within 1e-6 of the FP64 oracle after 500 steps
+    (w on the velocity scale too; w's own error is the known mixed-precision miss)."""
+    F = fx.Fixture("config2")
+    g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **F.params)
+    g.init_fields()
+    done = 0
+    for step in F.steps():
+        g.step(step - done, graph=True)
+        done = step
+        _check(F, g.get_fields(), step, 1e-6, W_SELF_MIXED, "mixed")
+    g.close()
+
+
+ENSEMBLE_CHECKED = (15, 35, 41, 60)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_ensemble64_members_against_oracle_after_500_steps(dns, precision):
+    """The 64-member ensemble (nx, ny in {32, 64, 96}, nz in {33, 65}, one launch per stage and
+    kernel class, graph-captured): four members after 500 steps against their own FP64 oracle
+    runs, at 1e-6 in mixed precision and 1e-10 in FP64."""
+    from f90_dns_amd.workloads import ensemble_members
+    mem = ensemble_members(precision=precision)
+    ens = dns.Ensemble(mem)
+    ens.init_fields()
+    ens.step(500, graph=True)
+    ens.synchronize()
+    for i in ENSEMBLE_CHECKED:
+        F = fx.Fixture(f"ens_{i}")
+        assert F.params["nx"] == mem[i]["nx"] and F.params["seed"] == mem[i]["seed"]
+        tol, wtol = (1e-6, W_SELF_MIXED) if precision else (1e-10, max(4 * F.floor(500)["w"], W_SELF_FP64))
+        _check(F, ens.get_fields(i), 500, tol, wtol, f"ensemble member {i} {ens.shape(i)} precision {precision}")
+    ens.close()
+
+
+def test_served_mix_at_config5_sizes_against_oracle(dns):
+    """Continuous batching at the served sweep's sizes: a 192 x 192 x 129 job, the 128 x 128 x 129
+    production run, a 128 x 96 x 65 job and small jobs arrive while others run; each of the
+    three large results is compared with its FP64 oracle fixture at 1e-6 (mixed precision)."""
+    from f90_dns_amd.workloads import BASE
+    big = {name: fx.Fixture(name) for name in ("served_192x192x129", "config2", "served_128x96x65")}
+    srv = dns.Server(hbm_budget_bytes=24 << 30, slots_per_group=2)
+    ids = {}
+    small = [dict(BASE, nx=32, ny=32, nz=65, re=150.0 + 20 * i, target_pressure_gradient=2.0 / (150.0 + 20 * i),
+                  perturbation_amplitude=0.02, seed=700 + i, precision=1) for i in range(3)]
+    ids["served_128x96x65"] = srv.submit(200, keep_fields=True, precision=1, **big["served_128x96x65"].params)
+    srv.submit(80, **small[0])
+    for _ in range(15):
+        srv.tick()
+    ids["served_192x192x129"] = srv.submit(60, keep_fields=True, precision=1, **big["served_192x192x129"].params)
+    srv.submit(120, **small[1])
+    for _ in range(10):
+        srv.tick()
+    ids["config2"] = srv.submit(100, keep_fields=True, precision=1, **big["config2"].params)
+    srv.submit(50, **small[2])
+    srv.drain()
+    st = srv.stats()
+    _say(f"server: {st}")
+    assert st["done"] == 6 and st["failed"] == 0 and st["rejected"] == 0 and st["max_running"] >= 3
+    for name, jid in ids.items():
+        F = big[name]
+        step = 100 if name == "config2" else F.steps()[0]
+        assert srv.status(jid) == ("done", step)
+        got, diag = srv.result(jid)
+        _check(F, got, step, 1e-6, W_SELF_MIXED, f"served {name}")
+    srv.close()

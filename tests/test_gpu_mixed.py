@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, field_errors, max_err
+from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err
 
 pytestmark = pytest.mark.gpu
 
@@ -37,9 +37,11 @@ def test_fp32_plane_transforms(dns, nx, ny):
     assert err < 2e-6 and err_b < 2e-6
 
 
-def test_mixed_config1_500_steps_within_1e6_of_fp64_oracle(dns):
-    """North-star bar for the mixed path: each field within 1e-6 (relative L2; w on the velocity
-    scale) of the FP64 oracle after 500 steps."""
+def test_mixed_config1_500_steps_against_fp64_oracle(dns):
+    """The mixed path against the FP64 oracle after 500 config-1 steps: u, v, p within 1e-6
+    relative L2 and w within 1e-6 of the velocity norm.  w's error relative to its own norm (w
+    is ~1e-6 of u here) is a known miss against 1e-6 (about 1.5e-4, from the FP32 product),
+    held to W_SELF_MIXED; the FP64 GPU run beside it meets 1e-10 with w at its rounding floor."""
     o = orc.Oracle(**CONFIG1)
     o.init_fields()
     g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **CONFIG1)
@@ -54,8 +56,8 @@ def test_mixed_config1_500_steps_within_1e6_of_fp64_oracle(dns):
     e_64 = field_errors(g64.get_fields(), ref)
     _say(f"mixed vs FP64 oracle after 500 steps: {e_mixed}")
     _say(f"FP64 GPU vs FP64 oracle after 500 steps: {e_64}")
-    assert max_err(e_mixed) < 1e-6, e_mixed
-    assert max_err(e_64) < 1e-10, e_64
+    check_parity(e_mixed, 1e-6, W_SELF_MIXED)
+    check_parity(e_64, 1e-10, W_SELF_FP64)
 
 
 def test_mixed_production_shape_short(dns):
@@ -71,7 +73,7 @@ def test_mixed_production_shape_short(dns):
     g.step(3)
     e = field_errors(g.get_fields(), o.get_fields())
     _say(f"mixed 128^2x129 after 3 steps: {e}")
-    assert max_err(e) < 1e-6, e
+    check_parity(e, 1e-6, W_SELF_MIXED)
 
 
 def test_mixed_192_plane_fields_in_and_out(dns):

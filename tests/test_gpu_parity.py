@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, field_errors, max_err, pair, rel_l2
+from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err, pair, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -119,7 +119,7 @@ def test_stage_parity_config1(dns):
     g.synchronize()
     e = field_errors(g.get_fields(), o.get_fields())
     _say(f"  after 3 steps {e}")
-    assert max_err(e) < 1e-12, e
+    check_parity(e, 1e-12, W_SELF_FP64)
     g.begin_step()
     o.compute_source_terms(); g.compute_source_terms()
     G = o.diagnostics()["pressure_gradient"]
@@ -146,7 +146,7 @@ def test_stage_parity_config1(dns):
     o.apply_boundary_conditions(); g.apply_boundary_conditions()
     e = field_errors(g.get_fields(), o.get_fields())
     _say(f"  after stages {e}")
-    assert max_err(e) < 1e-11, e
+    check_parity(e, 1e-11, W_SELF_FP64)
 
 
 def test_fused_step_equals_stages(dns):
@@ -180,8 +180,12 @@ def test_config1_100_steps(dns, dealias):
     _say(f"config1 100 steps dealias={dealias}")
     kw = dict(CONFIG1, use_dealias_23_rule=dealias)
     o, g = pair(**kw)
+    # the oracle's own twin (plane transforms factored otherwise): w's rounding floor of this run
+    t = orc.Oracle(fft_variant=1, **kw)
+    t.init_fields()
     for block in range(10):
         o.step(10)
+        t.step(10)
         g.step(10)
         do, dg = o.diagnostics(), g.diagnostics()
         assert abs(do["energy"] - dg["energy"]) < 1e-12 * do["energy"], block
@@ -190,17 +194,25 @@ def test_config1_100_steps(dns, dealias):
         assert abs(do["vmax"] - dg["vmax"]) < 1e-10 * do["umax"]
     fo = o.get_fields(); fg = g.get_fields()
     errs = field_errors(fg, fo)
+    floor = field_errors(t.get_fields(), fo)
     _say(f"  parity after 100 steps {errs}")
-    assert max_err(errs) < 1e-10, errs
+    _say(f"  oracle vs its twin (rounding floor) {floor}")
+    # u, v, p and w (velocity scale) at 1e-10; w's own error within 4x the oracle's own floor
+    check_parity(errs, 1e-10, 4 * floor["w_self"])
+    # same max-divergence order: both at the level of the collocation projection's residue
     do_, dg_ = o.max_divergence(), g.max_divergence()
-    assert dg_ < 10 * do_ + 1e-13
+    _say(f"  max divergence oracle {do_:.3e} gpu {dg_:.3e}")
+    assert dg_ < 2 * do_ + 1e-13 and do_ < 2 * dg_ + 1e-13
 
 
-def test_pi_controller_trace(dns):
-    """Method 2 (constant bulk velocity), controller every step via istart = 0."""
-    _say("PI controller")
+@pytest.mark.parametrize("istart", [0, 14, 3])
+def test_pi_controller_trace(dns, istart):
+    """Method 2 (constant bulk velocity), the controller called with istart (freq 7): istart = 0
+    runs it every step on its Euler branch, 14 every step on the trapezoid branch, 3 never
+    (G stays the target)."""
+    _say(f"PI controller istart={istart}")
     kw = dict(CONFIG1, flow_control_method=2, perturbation_kind=1, seed=11, noise_kmax=4,
-              perturbation_amplitude=0.05)
+              perturbation_amplitude=0.05, istart=istart, controller_update_freq=7)
     o, g = pair(**kw)
     for _ in range(20):
         o.step(1); g.step(1)
@@ -209,7 +221,7 @@ def test_pi_controller_trace(dns):
             assert abs(do[key] - dg[key]) < 1e-12 * max(1.0, abs(do[key])), key
     e = field_errors(g.get_fields(), o.get_fields())
     _say(f"  PI run {e}")
-    assert max_err(e) < 1e-10, e
+    check_parity(e, 1e-10, W_SELF_FP64)
 
 
 def test_trace_config_matches_reference_stdout(dns):
@@ -241,7 +253,7 @@ def test_seeded_noise_production_shape_short(dns):
     fo, fg = o.get_fields(), g.get_fields()
     errs = field_errors(fg, fo)
     _say(f"  128^2x129 after 3 steps {errs}")
-    assert max_err(errs) < 1e-10, errs
+    check_parity(errs, 1e-10, W_SELF_FP64)
 
 
 @pytest.mark.parametrize("nx,ny", [(96, 96), (128, 96), (192, 48)])
@@ -255,7 +267,7 @@ def test_largest_class_step_shapes(dns, nx, ny):
     o.step(4); g.step(4)
     errs = field_errors(g.get_fields(), o.get_fields())
     _say(f"  {nx}x{ny}x17 after 4 steps {errs}")
-    assert max_err(errs) < 1e-10, errs
+    check_parity(errs, 1e-10, W_SELF_FP64)
 
 
 @pytest.mark.parametrize("kw", [dict(CONFIG1),
@@ -279,7 +291,7 @@ def test_gpu_matches_cpu_emulation_of_same_algorithm(dns, kw):
     g.step(5)
     errs = field_errors(g.get_fields(), e.get_fields())
     _say(f"  {errs}")
-    assert max_err(errs) < 1e-12, errs
+    check_parity(errs, 1e-12, W_SELF_FP64)
 
 
 def test_packed_mode_space_stages_against_oracle(dns):
@@ -323,7 +335,7 @@ def test_packed_mode_space_stages_against_oracle(dns):
     o.apply_boundary_conditions(); g.apply_boundary_conditions()
     e = field_errors(g.get_fields(), o.get_fields())
     _say(f"  after stages {e}")
-    assert max_err(e) < 1e-10, e
+    check_parity(e, 1e-10, W_SELF_FP64)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -354,3 +366,57 @@ def test_packed_mode_space_batch_member_equals_alone(dns, precision):
             assert np.array_equal(x, y), (b, f)
         one.close()
     batch.close()
+
+
+def test_context_wavenumber_tables_bitwise_equal_oracle(dns):
+    """The device-resident xw, yw of a context (ny != nx) bit for bit against the oracle."""
+    g = dns.Simulation(nx=48, ny=36, nz=17, xlen=4 * np.pi, ylen=2 * np.pi)
+    xw, yw = g.device_wavenumbers()
+    oxw, oyw = orc.wavenumbers(48, 36, 4 * np.pi, 2 * np.pi)
+    np.testing.assert_array_equal(xw, oxw)
+    np.testing.assert_array_equal(yw, oyw)
+
+
+@pytest.mark.parametrize("nz", [17, 33, 129])
+def test_laminar_bulk_velocity_on_device(dns, nz):
+    """calculate_bulk_velocity_3d of the Poiseuille profile: 2 uc / 3 (exact LGL quadrature)."""
+    g = dns.Simulation(nx=32, ny=24, nz=nz, u_centerline=1.5, enable_perturbations=0, lgl_derivative=1)
+    g.init_fields()
+    assert abs(g.diagnostics()["bulk_velocity"] - 1.0) < 1e-12
+
+
+def test_restart_with_pressure_keeps_zero_wall_gradient_for_first_step(dns):
+    """Fields set with a nonzero p (a restart read): the reference's grad_p_prev stays zero until
+    the first pressure correction (DNS_pressure_BC_3D.f90:957,1923), so the first step's
+    Helmholtz wall values are zero.  The oracle from the same fields with zero wall gradients
+    is the reference; one whose wall gradients follow p differs, so the test can tell."""
+    _say("restart with p")
+    kw = dict(CONFIG1, flow_control_method=2, perturbation_kind=1, seed=19, noise_kmax=4,
+              perturbation_amplitude=0.05)
+    src = orc.Oracle(**kw)
+    src.init_fields()
+    src.step(4)
+    f = src.get_fields()
+    gx, gy = src.get_wall_gradients()
+    assert np.abs(f[3]).max() > 0 and np.abs(gx).max() > 0
+    o = orc.Oracle(**kw)
+    o.set_fields(*f)                   # wall gradients stay zero, as after read_restart_file_3d
+    alt = orc.Oracle(**kw)
+    alt.set_fields(*f)
+    alt.set_wall_gradients(gx, gy)     # what a step with gradients from p would do
+    g = dns.Simulation(**kw)
+    g.set_fields(*f)
+    for _ in range(5):
+        o.step(1); alt.step(1); g.step(1)
+    e = field_errors(g.get_fields(), o.get_fields())
+    e_alt = field_errors(alt.get_fields(), o.get_fields())
+    _say(f"  restart parity {e}; gradient-from-p oracle differs by {e_alt}")
+    assert max(e_alt["u"], e_alt["v"]) > 1e-8
+    check_parity(e, 1e-10, W_SELF_FP64)
+    # a second set_fields with p and graph replay: the flag applies to the first replay only
+    g2 = dns.Simulation(**kw)
+    g2.set_fields(*f)
+    g2.step(2, graph=True)
+    g2.step(3, graph=True)
+    for a, b in zip(g2.get_fields(), g.get_fields()):
+        np.testing.assert_array_equal(a, b)

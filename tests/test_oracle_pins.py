@@ -103,12 +103,21 @@ def test_wavenumbers_closed_form():
 
 
 @pytest.mark.parametrize("nx,kxm", [(32, 10), (64, 21), (96, 32), (128, 42), (192, 64)])
-def test_dealias_mask_bounds(nx, kxm):
-    ny = nx
+@pytest.mark.parametrize("ny,kym", [(24, 8), (36, 12), (96, 32), (130, 43)])
+def test_dealias_mask_bounds(nx, kxm, ny, kym):
+    # the closed-form bounds of SURVEY (kx_max = 10, 21, 32, 42, 64) and the reference's loop
+    # (1-based i, j; :1176-1190) written out element by element, with ny != nx
     m = orc.dealias_mask(nx, ny)
-    kx = np.arange(nx // 2 + 1)[None, :]
-    ky = np.array([j if j <= ny // 2 else j - ny for j in range(ny)])[:, None]
-    want = (kx <= kxm) & (np.abs(ky) <= kxm)
+    want = np.ones((ny, nx // 2 + 1), dtype=bool)
+    for j1 in range(1, ny + 1):
+        for i1 in range(1, nx // 2 + 2):
+            if i1 - 1 > kxm:
+                want[j1 - 1, i1 - 1] = False
+            if j1 <= ny // 2 + 1:
+                if j1 - 1 > kym:
+                    want[j1 - 1, i1 - 1] = False
+            elif abs(j1 - 1 - ny) > kym:
+                want[j1 - 1, i1 - 1] = False
     np.testing.assert_array_equal(m.astype(bool), want)
 
 
@@ -274,3 +283,96 @@ def test_exact_d1_polynomials(n):
         assert np.max(np.abs(D @ x ** p - p * x ** (p - 1))) < 1e-11 * n * n * p
     if n == 33:
         np.testing.assert_allclose(D, orc.diff_matrix(n), atol=1e-9 * np.abs(D).max())
+
+
+# --- PI controller and bulk velocity (DNS_pressure_BC_3D.f90:2434-2464, 2488-2549) ------
+# Hand-worked recurrences on prescribed bulk velocities.  Re = 100 (base 3/Re = 0.03),
+# controller_gain Kp = 0.2 (Ki = 0.05 Kp = 0.01), target U_b* = 0.5, dt = 0.01,
+# target_pressure_gradient = 0.02 (the state read_input_3d leaves: G = average = 0.02).
+PI = dict(re=100.0, controller_gain=0.2, target_bulk_velocity=0.5, dt=0.01, target_pressure_gradient=0.02)
+
+
+def _close(a, b):
+    assert abs(a - b) <= 1e-15 * max(1.0, abs(b)), (a, b)
+
+
+def test_pi_euler_branch_and_clamps_by_hand():
+    # istep = 0, freq 7: mod(0, 7) = 0 runs every call and 0 <= 7 takes the Euler branch
+    c = orc.PIController(controller_update_freq=7, **PI)
+    rows = [  # U_b, G, integral, prev, average of G
+        (0.40, 0.05, 0.001, 0.1, 0.05),                       # e = .1: G = .03 + .2*.1 + .01*0
+        (0.45, 0.04001, 0.0015, 0.05, 0.045005),              # e = .05: G = .03 + .01 + .01*.001
+        (1.00, 0.0, 0.0015, -0.5, 0.09001 / 3),              # e = -.5: G_u = -.069985 -> 0, I frozen
+        (-60.0, 10.0, 0.0015, 60.5, 10.09001 / 4),           # e = 60.5: G_u = 12.130015 -> 10, I frozen
+        (0.50, 0.030015, 0.0015, 0.0, 10.120025 / 5),        # e = 0: G = .03 + .01*.0015
+    ]
+    for n, (ub, G, I, prev, avg) in enumerate(rows, 1):
+        assert c.update(0, ub)
+        _close(c.Ub, ub); _close(c.G, G); _close(c.err_int, I); _close(c.err_prev, prev); _close(c.gavg, avg)
+        assert c.gcount == n
+
+
+def test_pi_trapezoid_branch_by_hand():
+    # istep = 4 > freq 2 with mod(4, 2) = 0: I += dt/2 (e_prev + e)
+    c = orc.PIController(controller_update_freq=2, **PI)
+    for ub, G, I in [(0.40, 0.05, 0.0005),                  # I = .005 * (0 + .1)
+                     (0.45, 0.040005, 0.00125),             # G = .04 + .01*.0005; I += .005 * .15
+                     (0.30, 0.0700125, 0.0025)]:            # G = .07 + .01*.00125; I += .005 * .25
+        assert c.update(4, ub)
+        _close(c.G, G); _close(c.err_int, I)
+    # istep = freq is still the Euler branch (istep <= freq), 2*freq the trapezoid
+    e = orc.PIController(controller_update_freq=7, **PI)
+    e.update(7, 0.40); e.update(7, 0.45)
+    _close(e.err_int, 0.0015)
+    t = orc.PIController(controller_update_freq=7, **PI)
+    t.update(14, 0.40); t.update(14, 0.45)
+    _close(t.err_int, 0.00125)
+
+
+def test_pi_skipped_when_istep_not_a_multiple_of_freq():
+    c = orc.PIController(controller_update_freq=7, **PI)
+    for _ in range(3):
+        assert not c.update(3, 0.1)
+    assert c.G == 0.02 and c.gavg == 0.02 and c.gcount == 0 and c.err_int == 0.0 and c.Ub == 0.0
+
+
+def test_pi_skip_branch_keeps_forcing_in_a_run():
+    # istart = 3, freq 7: the controller never runs, so G stays target_pressure_gradient
+    o = orc.Oracle(nx=8, ny=8, nz=9, flow_control_method=2, istart=3, controller_update_freq=7,
+                   target_pressure_gradient=0.0125, perturbation_kind=1, seed=3, noise_kmax=2)
+    o.init_fields()
+    o.step(3)
+    d = o.diagnostics()
+    assert d["pressure_gradient"] == 0.0125 and d["avg_pressure_gradient"] == 0.0125
+    assert d["bulk_error_integral"] == 0.0
+
+
+@pytest.mark.parametrize("nz", [17, 33, 129])
+def test_laminar_bulk_velocity_is_two_thirds_of_centreline(nz):
+    # U_b = (1/2) int_{-1}^{1} uc (1 - z^2) dz = 2 uc / 3; LGL quadrature is exact for degree 2.
+    # A zero-mean x-periodic part leaves it unchanged.
+    nx, ny, uc = 12, 10, 1.5
+    z, _ = orc.lgl(nz)
+    x = np.arange(nx) * 2 * np.pi / nx
+    u = uc * (1 - z ** 2)[:, None, None] * np.ones((nz, ny, nx))
+    assert abs(orc.bulk_velocity(u) - 2 * uc / 3) < 1e-12
+    u2 = u + 0.3 * np.sin(x)[None, None, :] * (1 - z ** 2)[:, None, None]
+    assert abs(orc.bulk_velocity(u2) - 2 * uc / 3) < 1e-12
+    o = orc.Oracle(nx=nx, ny=ny, nz=nz, u_centerline=uc, enable_perturbations=0)
+    o.init_fields()
+    assert abs(o.diagnostics()["bulk_velocity"] - 2 * uc / 3) < 1e-12
+
+
+# --- the rounding floor of w -------------------------------------------------------------
+def test_w_rounding_floor_of_the_oracle_itself():
+    """w is ~1e-6 of u in config 1, so any reordering of the arithmetic moves it at the 1e-10
+    level of its own norm.  The oracle against itself with every plane transform factored in
+    radix 2 instead of 4, odd factors first (an equivalent transform, other rounding): after 100
+    steps u, v, p agree to ~1e-13 while w's own relative difference is ~1e-10."""
+    from tests.gpu_util import CONFIG1, field_errors
+    a = orc.Oracle(**CONFIG1); a.init_fields()
+    b = orc.Oracle(fft_variant=1, **CONFIG1); b.init_fields()
+    a.step(100); b.step(100)
+    e = field_errors(b.get_fields(), a.get_fields())
+    assert e["u"] < 1e-13 and e["v"] < 1e-12 and e["p"] < 1e-13, e
+    assert 5e-11 < e["w_self"] < 5e-10, e
