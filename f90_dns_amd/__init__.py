@@ -38,6 +38,7 @@ class DnsParams(ctypes.Structure):
         ("analytical_mode_x", ctypes.c_int), ("analytical_mode_y", ctypes.c_int),
         ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
         ("precision", ctypes.c_int), ("lgl_derivative", ctypes.c_int),
+        ("convection_scheme", ctypes.c_int), ("pressure_solver", ctypes.c_int), ("cgstol", ctypes.c_double),
     ]
 
 
@@ -70,7 +71,7 @@ DEFAULTS = dict(
     target_pressure_gradient=0.03, target_bulk_velocity=1.0, controller_gain=0.15, controller_update_freq=7,
     istart=0, u_centerline=1.5, enable_perturbations=0, perturbation_kind=0, analytical_mode_x=1,
     analytical_mode_y=1, perturbation_amplitude=0.01, seed=0, noise_kmax=8, precision=DNS_PREC_FP64,
-    lgl_derivative=0,
+    lgl_derivative=0, convection_scheme=0, pressure_solver=0, cgstol=1e-6,
 )
 
 _lib = None
@@ -124,6 +125,7 @@ def lib():
         L.dns_debug_wavenumbers.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp, _vp]
         L.dns_debug_dealias_mask.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.dns_debug_context_wavenumbers.argtypes = [_vp, _vp, _vp]
+        L.dns_debug_jacgs_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_int)]
         L.dns_profile_step.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_profile_kernel_name.argtypes = [ctypes.c_int]
         L.dns_profile_kernel_name.restype = ctypes.c_char_p
@@ -343,6 +345,12 @@ class Simulation:
         out = (ctypes.c_double * 2)()
         self._check(lib().dns_debug_operator_error(self._h, out))
         return out[0], out[1]
+
+    def jacgs_stats(self):
+        """(total iterations, non-converged systems) of the last jacgs pressure solve."""
+        it = ctypes.c_ulonglong(); nf = ctypes.c_int()
+        self._check(lib().dns_debug_jacgs_stats(self._h, ctypes.byref(it), ctypes.byref(nf)))
+        return it.value, nf.value
 
     def device_wavenumbers(self):
         """The context's device-resident xw, yw tables, copied back."""

@@ -45,6 +45,10 @@ struct dns_ctx {
     // stream after the next pressure correction
     std::vector<char> zwg;
     double *red_partial = nullptr, *red_result = nullptr;
+    // pressure_solver 1 (jacgs): the symmetric stiffness matrix and the solve counters
+    double* stiff = nullptr;
+    unsigned long long* jac_iters = nullptr;
+    int* jac_nonconv = nullptr;
     int red_blocks = 512;
     std::vector<void*> allocs;
     size_t bytes = 0;

@@ -108,6 +108,13 @@ static std::string check_params(const dns_params* p) {
     if (p->flow_control_method == 2 &&
         (p->target_bulk_velocity <= 0 || p->controller_gain <= 0 || p->controller_update_freq <= 0))
         return "method 2 needs positive target_bulk_velocity, controller_gain, controller_update_freq";
+    if (p->convection_scheme != 0 && p->convection_scheme != 1) return "convection_scheme must be 0 or 1";
+    if (p->pressure_solver != 0 && p->pressure_solver != 1) return "pressure_solver must be 0 or 1";
+    if (p->pressure_solver == 1) {
+        if (!(p->cgstol >= 0)) return "cgstol must be >= 0";
+        if (p->use_weak_form_divergence) return "the jacgs pressure solver is not available with the weak form";
+        if (p->precision != DNS_PREC_FP64) return "the jacgs pressure solver runs in FP64 only";
+    }
     return "";
 }
 
@@ -116,7 +123,8 @@ static bool same_shape(const dns_params& a, const dns_params& b) {
     return a.nx == b.nx && a.ny == b.ny && a.nz == b.nz && a.dt == b.dt && a.xlen == b.xlen && a.ylen == b.ylen &&
            a.use_crank_nicolson == b.use_crank_nicolson && a.use_dealias_23_rule == b.use_dealias_23_rule &&
            a.use_weak_form_divergence == b.use_weak_form_divergence && a.precision == b.precision &&
-           a.lgl_derivative == b.lgl_derivative;
+           a.lgl_derivative == b.lgl_derivative && a.convection_scheme == b.convection_scheme &&
+           a.pressure_solver == b.pressure_solver && (a.pressure_solver == 0 || a.cgstol == b.cgstol);
 }
 
 extern "C" {
@@ -242,6 +250,11 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     for (int f = 0; f < 6; f++)
         if ((r = dalloc(c, &c->phys[f], NB))) return bail(r);
     if (c->g.cls64 < 0 && (r = dalloc(c, &c->io, c->NS))) return bail(r);
+    if (p->pressure_solver == 1) {
+        if ((r = upload(c, &c->stiff, o.stiff)) || (r = dalloc(c, &c->jac_iters, 1)) ||
+            (r = dalloc(c, &c->jac_nonconv, 1)))
+            return bail(r);
+    }
     if ((r = dalloc(c, &c->flow, c->B)) || (r = dalloc(c, &c->fmem, c->B)) ||
         (r = dalloc(c, &c->red_partial, size_t(c->red_blocks) * 5)) || (r = dalloc(c, &c->red_result, 5)))
         return bail(r);
@@ -548,8 +561,17 @@ extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
     return z;
 }
 
-static int nl_zderiv(dns_ctx* c) {
-    HIPCHECK(c, launch_zderiv(Z_VORT, ctx_vort_args(c), 1, c->B, c->stream));
+// the nonlinear term's inputs: the state (u, v, w), or a convection stage's velocities
+struct NLInputs { const double* f[3]; };
+static NLInputs nl_inputs(const dns_ctx* c, bool stage) {
+    return stage ? NLInputs{{c->us, c->vs, c->ws}} : NLInputs{{c->uh, c->vh, c->wh}};
+}
+
+static int nl_zderiv(dns_ctx* c, bool stage = false) {
+    ZArgs z = ctx_vort_args(c);
+    const NLInputs in = nl_inputs(c, stage);
+    for (int f = 0; f < 3; f++) z.in[f] = in.f[f];
+    HIPCHECK(c, launch_zderiv(Z_VORT, z, 1, c->B, c->stream));
     return 0;
 }
 
@@ -567,8 +589,10 @@ extern "C++" FFTArgs dns::ctx_c2r_nl_args(dns_ctx* c) {
 // interior nodes alone (the wall rows of the Helmholtz solve are the boundary values), so the
 // wall planes of the six physical fields and of S are not needed.  The stage API
 // (dns_compute_source_terms) computes every plane, as the reference does.
-static int nl_c2r(dns_ctx* c, bool walls) {
+static int nl_c2r(dns_ctx* c, bool walls, bool stage = false) {
     FFTArgs a = ctx_c2r_nl_args(c);
+    const NLInputs in = nl_inputs(c, stage);
+    for (int f = 0; f < 3; f++) a.src[f] = in.f[f];
     a.zin = walls ? 0 : c->nz - 2;
     HIPCHECK(c, launch_c2r(c->g, C2R_NL, a, (walls ? c->nz : c->nz - 2) * c->B, 6, c->mixed, c->stream));
     return 0;
@@ -597,11 +621,38 @@ static int flow_control(dns_ctx* c) {
     return 0;
 }
 
+// The 4-stage 1/irk convection of runge_kutta_convection (DNS_pressure_BC_2D.f90:931-1014) on
+// the spectral state: stage irk = 4 takes S of (u, v, w); each later stage takes S of
+// un + (dt/(irk+1)) S_prev, held in the star buffers (written by the Helmholtz launch only
+// afterwards), walls zero; the S of the last stage stays in the S buffers.  The state itself is
+// not touched, which is the reference's reset to un.
+static int rk_convection(dns_ctx* c, bool walls) {
+    int r;
+    for (int irk = 4; irk >= 1; irk--) {
+        const bool stage = irk < 4;
+        if ((r = nl_zderiv(c, stage)) || (r = nl_c2r(c, walls, stage)) || (r = nl_r2c(c, walls))) return r;
+        if (irk == 1) break;
+        RKStageArgs a{};
+        a.n = (long long)c->NS * c->B; a.H2 = c->H2; a.nz = c->nz;
+        a.un[0] = c->uh; a.un[1] = c->vh; a.un[2] = c->wh;
+        for (int f = 0; f < 3; f++) a.s[f] = c->sh[f];
+        a.s_f32 = c->mixed ? 1 : 0;
+        a.out[0] = c->us; a.out[1] = c->vs; a.out[2] = c->ws;
+        a.a = (1.0 / double(irk)) * c->p.dt;   // alf * dt of this stage (:985-991)
+        HIPCHECK(c, launch_rk_stage(a, c->stream));
+    }
+    return 0;
+}
+
 // flow: launch the flow controller here (the stage API); the fused step runs it inside the
 // Helmholtz launch instead
 static int source_terms(dns_ctx* c, bool walls, bool flow) {
     int r;
-    if ((r = nl_zderiv(c)) || (r = nl_c2r(c, walls)) || (r = nl_r2c(c, walls))) return r;
+    if (c->p.convection_scheme == 1) {
+        if ((r = rk_convection(c, walls))) return r;
+    } else if ((r = nl_zderiv(c)) || (r = nl_c2r(c, walls)) || (r = nl_r2c(c, walls))) {
+        return r;
+    }
     return flow ? flow_control(c) : 0;
 }
 
@@ -670,7 +721,51 @@ extern "C++" Mode0Args dns::ctx_mode0_args(dns_ctx* c, int fused) {
     return m;
 }
 
+// plain D1 contraction of nf = 1 spectral field over all members: out = D1 in
+static int zderiv_plain(dns_ctx* c, const double* in, double* out) {
+    ZArgs z{};
+    z.mstride = (long long)c->NS;
+    z.nz = c->nz; z.H2 = c->H2; z.nxh = c->nxh; z.ny = c->ny; z.nx = c->nx;
+    z.d1 = fold_op(c->op.zD1, c->D1sw);
+    z.in[0] = in; z.out[0] = out; z.xw = c->xw; z.yw = c->yw;
+    HIPCHECK(c, launch_zderiv(Z_PLAIN, z, 1, c->B, c->stream));
+    return 0;
+}
+
+static int correction(dns_ctx* c) {
+    CorrArgs a{};
+    a.nz = c->nz * c->B;   // members are consecutive planes and the update is per element
+    a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
+    a.xw = c->xw; a.yw = c->yw;
+    a.us = c->us; a.vs = c->vs; a.ws = c->ws; a.phi = c->phi; a.dphi = c->dphi;
+    a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph; a.dt = c->p.dt;
+    HIPCHECK(c, launch_correction(a, c->stream));
+    return 0;
+}
+
+// pressure_solver 1: phi from jacgs per real column (RHS from u*, v* and D1 w*), D1 phi; when
+// fused also the projection, p += phi and the walls
+static int pressure_jacgs(dns_ctx* c, int fused) {
+    int r;
+    if ((r = zderiv_plain(c, c->ws, c->tmp))) return r;
+    HIPCHECK(c, launch_jacgs_clear(c->jac_iters, c->jac_nonconv, c->stream));
+    JacgsArgs j{};
+    j.mstride = (long long)c->NS;
+    j.nz = c->nz; j.H2 = c->H2; j.nxh = c->nxh; j.nx = c->nx; j.ny = c->ny; j.niter = 500;
+    j.st = c->stiff; j.wz = c->wz; j.xw = c->xw; j.yw = c->yw;
+    j.us = c->us; j.vs = c->vs; j.dws = c->tmp; j.phi = c->phi;
+    j.dt = c->p.dt; j.ybar = c->ybar; j.tol = c->p.cgstol;
+    j.iters = c->jac_iters; j.nonconv = c->jac_nonconv;
+    HIPCHECK(c, launch_jacgs(j, c->B, c->stream));
+    if ((r = zderiv_plain(c, c->phi, c->dphi))) return r;
+    if (!fused) return 0;
+    if ((r = correction(c))) return r;
+    HIPCHECK(c, launch_zero_walls(c->nz, c->H2, (long long)c->NS, c->B, c->uh, c->vh, c->wh, c->stream));
+    return 0;
+}
+
 static int pressure(dns_ctx* c, int fused) {
+    if (c->p.pressure_solver == 1) return pressure_jacgs(c, fused);
     PoisArgs a = ctx_pois_args(c, fused);
     HIPCHECK(c, launch_poisson(a, c->B, c->stream));
     return 0;
@@ -683,14 +778,8 @@ int dns_solve_pressure(dns_ctx* c) {
 
 int dns_pressure_correction(dns_ctx* c) {
     if (!c) return DNS_EINVAL;
-    CorrArgs a{};
-    a.nz = c->nz * c->B;   // members are consecutive planes and the update is per element
-    a.H2 = c->H2; a.nxh = c->nxh; a.ny = c->ny; a.nx = c->nx;
-    a.xw = c->xw; a.yw = c->yw;
-    a.us = c->us; a.vs = c->vs; a.ws = c->ws; a.phi = c->phi; a.dphi = c->dphi;
-    a.u = c->uh; a.v = c->vh; a.w = c->wh; a.p = c->ph; a.dt = c->p.dt;
-    HIPCHECK(c, launch_correction(a, c->stream));
-    return ctx_after_correction(c, c->stream);
+    int r = correction(c);
+    return r ? r : ctx_after_correction(c, c->stream);
 }
 
 int dns_apply_boundary_conditions(dns_ctx* c) {
@@ -947,6 +1036,8 @@ const char* dns_profile_kernel_name(int i) {
 
 int dns_profile_step(dns_ctx* c, int nsteps, double* out) {
     if (!c || !out || nsteps <= 0) return DNS_EINVAL;
+    if (c->p.convection_scheme != 0 || c->p.pressure_solver != 0)
+        return fail(c, DNS_EUNSUPPORTED, "dns_profile_step times the plain step's kernels only");
     const int NK = 7;
     int r = ensure_events(c, NK + 1);
     if (r) return r;
@@ -1074,6 +1165,15 @@ int dns_debug_dealias_mask(int nx, int ny, int which, unsigned char* mask) {
     dealias_bounds(nx, ny, d.kxm, d.kym);
     for (int j = 0; j < ny; j++)
         for (int i = 0; i < nxh; i++) mask[i + size_t(nxh) * j] = d.keep(i, j) ? 1 : 0;
+    return 0;
+}
+
+int dns_debug_jacgs_stats(dns_ctx* c, unsigned long long* iterations, int* nonconverged) {
+    if (!c || !iterations || !nonconverged) return DNS_EINVAL;
+    if (c->p.pressure_solver != 1) return fail(c, DNS_EINVAL, "the context does not use the jacgs pressure solver");
+    HIPCHECK(c, hipMemcpyAsync(iterations, c->jac_iters, sizeof(*iterations), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(nonconverged, c->jac_nonconv, sizeof(*nonconverged), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 

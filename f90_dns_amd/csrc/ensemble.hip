@@ -45,6 +45,8 @@ bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) 
     b += 15 * d(NS * B) + 6 * d(N * B);
     b += std::max<size_t>(B, 1) * (sizeof(FlowState) + sizeof(FlowMember));
     b += d(512 * 5) + d(5);
+    if (p.pressure_solver == 1)   // jacgs: stiffness and the two solve counters
+        b += d(nz * nz) + sizeof(unsigned long long) + sizeof(int);
     FFTGeom g;
     fft_geom(p.nx, p.ny, g);
     if (g.cls64 < 0) b += d(NS);   // two-pass scratch (plane beyond the FP64 LDS transform)
@@ -569,6 +571,11 @@ const char* dns_ensemble_last_error(const dns_ensemble* e) { return e ? e->e.err
 int dns_ensemble_create(const dns_params* members, int n, dns_ensemble** out) {
     if (!members || n < 1 || !out) return efail(nullptr, DNS_EINVAL, "null argument or no members");
     *out = nullptr;
+    for (int i = 0; i < n; i++)
+        if (members[i].convection_scheme != 0 || members[i].pressure_solver != 0)
+            return efail(nullptr, DNS_EUNSUPPORTED,
+                         "member " + std::to_string(i) + ": the convection and pressure-solver variants run in "
+                         "single contexts and batches only");
     dns_ensemble* e = new dns_ensemble();
     int r = e->e.init();
     if (r) { g_ens_error = e->e.err; delete e; return r; }

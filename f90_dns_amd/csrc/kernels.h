@@ -385,6 +385,39 @@ hipError_t launch_correction(const CorrArgs& a, hipStream_t s);
 hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nmembers, double* u, double* v, double* w,
                              hipStream_t s);
 
+// One stage of the 4-stage 1/irk convection (runge_kutta_convection, DNS_pressure_BC_2D.f90:
+// 931-1014) on the spectral state: out_c = un_c + a S_c for the three components (a = dt / irk),
+// zero on the wall planes (apply_wall_bc after each stage)
+struct RKStageArgs {
+    long long n;                 // doubles per field over all members (members * nz * H2)
+    long long H2; int nz;
+    const double* un[3]; const void* s[3]; int s_f32;
+    double* out[3];
+    double a;
+};
+hipError_t launch_rk_stage(const RKStageArgs& a, hipStream_t s);
+
+// jacgs (DNS_pressure_BC_2D.f90:2361-2476) on each real column (a mode's real or imaginary part)
+// of the 3D Poisson system of solve_pressure_3d_standard (DNS_pressure_BC_3D.f90:1680-1711):
+// A = -St - diag((kx^2 + ky^2) w (ybar/2)), mode (0,0) with row and column 1 replaced by the
+// identity; b = (div/dt) (ybar/2) w with div = i kx_eff u* + i ky_eff v* + D1 w*; x0 = 0; Jacobi
+// preconditioner; stop when ||r||_2 <= tol or after niter iterations.  One wavefront per column.
+struct JacgsArgs {
+    long long mstride;
+    int nz, H2, nxh, nx, ny, niter;
+    const double* st;            // nz x nz, symmetric: (2/ybar) sum_m D1(m,k) D1(m,n) w_m
+    const double* wz;
+    const double* xw; const double* yw;
+    const double* us; const double* vs; const double* dws;   // u*, v*, D1 w* (spectral)
+    double* phi;
+    double dt, ybar, tol;
+    unsigned long long* iters;   // total iterations (atomic)
+    int* nonconv;                // systems that met niter without converging (atomic)
+};
+hipError_t launch_jacgs(const JacgsArgs& a, int nmembers, hipStream_t s);
+// zero the two counters of launch_jacgs (a kernel: no memset node in captured graphs)
+hipError_t launch_jacgs_clear(unsigned long long* iters, int* nonconv, hipStream_t s);
+
 // PI / constant forcing state, device resident (update_pressure_gradient_3d), one per member.
 // zero_wallgrad: the Helmholtz wall values of the next step use a zero pressure gradient (the
 // reference's grad_p_prev after initialize_fields_3d or a restart read, which is set only by

@@ -265,6 +265,8 @@ int dns_server_destroy(dns_server* s) {
 
 int dns_server_submit(dns_server* s, const dns_params* p, int nsteps, int keep_fields, int* id) {
     if (!s || !p || nsteps < 1) return DNS_EINVAL;
+    if (p->convection_scheme != 0 || p->pressure_solver != 0)
+        return sfail(s, DNS_EUNSUPPORTED, "the convection and pressure-solver variants are not served");
     size_t b1 = 0, b2 = 0;
     std::string e;
     if (!ctx_bytes_for(*p, 1, b1, e) || !ctx_bytes_for(*p, 2, b2, e)) return sfail(s, DNS_ESOLVER, e);

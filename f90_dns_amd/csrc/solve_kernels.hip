@@ -1027,6 +1027,122 @@ __global__ __launch_bounds__(256) void k_zero_walls(int nz, int H2, long long ms
 }
 
 __global__ __launch_bounds__(64) void k_flow_control(FlowArgs a) { flow_tile(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_rk_stage(RKStageArgs a) {
+    for (long long e = blockIdx.x * 256LL + threadIdx.x; e < a.n; e += 256LL * gridDim.x) {
+        const long long plane = (e / a.H2) % a.nz;
+        const bool wall = plane == 0 || plane == a.nz - 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double sv = a.s_f32 ? double(static_cast<const float*>(a.s[c])[e]) : static_cast<const double*>(a.s[c])[e];
+            a.out[c][e] = wall ? 0.0 : a.un[c][e] + a.a * sv;
+        }
+    }
+}
+
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// one wavefront per real column; lane owns rows lane + 64 r (nz <= 192)
+__global__ __launch_bounds__(256) void k_jacgs(JacgsArgs a) {
+    constexpr int R = 3;
+    constexpr double small = 1.0e-30;
+    const int lane = threadIdx.x & 63, c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= a.H2) return;
+    const int nz = a.nz, mode = c >> 1, part = c & 1;
+    const int j = mode / a.nxh, i = mode - j * a.nxh;
+    const bool m0 = mode == 0;
+    const size_t mo = size_t(blockIdx.y) * a.mstride;
+    const double kxe = kx_eff(a.xw, i, a.nx), kye = ky_eff(a.yw, i, j, a.nx, a.ny);
+    const double ksq = a.xw[i] * a.xw[i] + a.yw[j] * a.yw[j];
+    double x[R], r[R], p[R], q[R], dg[R], ad[R], ap[R];
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+        const int row = lane + 64 * rr;
+        x[rr] = r[rr] = p[rr] = q[rr] = dg[rr] = ad[rr] = 0.0;
+        if (row >= nz) continue;
+        const double wk = a.wz[row];
+        // diagonal of A, as assembled (:1688-1691; mode (0,0): unit entry in row 1)
+        ad[rr] = (m0 && row == 0) ? 1.0 : -a.st[size_t(row) * nz + row] - ksq * wk * (a.ybar / 2.0);
+        dg[rr] = fabs(ad[rr]) > small ? 1.0 / ad[rr] : 1.0;
+        const size_t o = mo + size_t(row) * a.H2 + 2 * size_t(mode);
+        const double2 u = *reinterpret_cast<const double2*>(a.us + o);
+        const double2 v = *reinterpret_cast<const double2*>(a.vs + o);
+        const double2 dw = *reinterpret_cast<const double2*>(a.dws + o);
+        const double div = part == 0 ? (-kxe * u.y) + (-kye * v.y) + dw.x : (kxe * u.x) + (kye * v.x) + dw.y;
+        const double b = (m0 && row == 0) ? 0.0 : (div / a.dt) * (0.5 * a.ybar * wk);
+        r[rr] = b;                       // b - A x0 with x0 = 0
+        q[rr] = r[rr] * dg[rr];
+        p[rr] = q[rr];
+    }
+    auto rsum = [&](const double (&u)[R], const double (&v)[R]) {
+        double t = 0.0;
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) t += u[rr] * v[rr];
+        return wave_sum(t);
+    };
+    // A p: rows owned by the lane, columns in order (apn(i) = sum_j a(i,j) p(j))
+    auto matvec = [&]() {
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) ap[rr] = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < R; jj++) {
+            const int lim = min(64, nz - 64 * jj);
+            for (int l = 0; l < lim; l++) {
+                const int col = 64 * jj + l;
+                const double pj = read_lane(p[jj], l);
+                const double* sc = a.st + size_t(col) * nz;   // column col (= row col: symmetric)
+#pragma unroll
+                for (int rr = 0; rr < R; rr++) {
+                    const int row = lane + 64 * rr;
+                    if (row >= nz) continue;
+                    double aij = row == col ? ad[rr] : -sc[row];
+                    if (m0 && (row == 0 || col == 0) && row != col) aij = 0.0;
+                    ap[rr] = __dadd_rn(ap[rr], __dmul_rn(aij, pj));
+                }
+            }
+        }
+    };
+    const double res0 = sqrt(rsum(r, r));
+    int it = 0;
+    bool conv = res0 <= a.tol;
+    if (!conv) {
+        for (it = 1; it <= a.niter; it++) {
+            matvec();
+            const double qdr = rsum(q, r), pdapn = rsum(p, ap);
+            const double alfa = qdr / (pdapn + small);
+#pragma unroll
+            for (int rr = 0; rr < R; rr++) {
+                x[rr] = x[rr] + alfa * p[rr];
+                r[rr] = r[rr] - alfa * ap[rr];
+                q[rr] = r[rr] * dg[rr];
+            }
+            const double qdrnew = rsum(q, r);
+            const double beta = qdrnew / (qdr + small);
+#pragma unroll
+            for (int rr = 0; rr < R; rr++) p[rr] = r[rr] * dg[rr] + beta * p[rr];
+            const double res1 = sqrt(rsum(r, r));
+            if (res1 <= a.tol) { conv = true; break; }
+        }
+        if (it > a.niter) it = a.niter;
+    }
+    if (lane == 0) {
+        atomicAdd(a.iters, (unsigned long long)it);
+        if (!conv) atomicAdd(a.nonconv, 1);
+    }
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+        const int row = lane + 64 * rr;
+        if (row < nz) a.phi[mo + size_t(row) * a.H2 + c] = x[rr];
+    }
+}
+
+__global__ void k_jacgs_clear(unsigned long long* iters, int* nonconv) {
+    if (threadIdx.x == 0) { *iters = 0ULL; *nonconv = 0; }
+}
+
 __global__ __launch_bounds__(64) void k_clear_wallgrad(FlowState* st, int n) {
     const int b = blockIdx.x * 64 + threadIdx.x;
     if (b < n) st[b].zero_wallgrad = 0;
@@ -1207,6 +1323,20 @@ hipError_t launch_zero_walls(int nz, int H2, long long mstride, int nm, double* 
 }
 hipError_t launch_flow_control(const FlowArgs& a, int nm, hipStream_t s) {
     hipLaunchKernelGGL(k_flow_control, dim3(nm), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_rk_stage(const RKStageArgs& a, hipStream_t s) {
+    const int blocks = int(std::min<long long>((a.n + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_rk_stage, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_jacgs(const JacgsArgs& a, int nm, hipStream_t s) {
+    if (a.nz > 192) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_jacgs, dim3((a.H2 + 3) / 4, nm), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_jacgs_clear(unsigned long long* iters, int* nonconv, hipStream_t s) {
+    hipLaunchKernelGGL(k_jacgs_clear, dim3(1), dim3(64), 0, s, iters, nonconv);
     return hipGetLastError();
 }
 hipError_t launch_clear_wallgrad(FlowState* st, int nm, hipStream_t s) {

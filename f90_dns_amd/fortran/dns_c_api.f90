@@ -9,7 +9,7 @@ module dns_c_api
               c_dns_get_fields, c_dns_begin_step, c_dns_compute_source_terms, c_dns_viscous_step, &
               c_dns_solve_pressure, c_dns_pressure_correction, c_dns_apply_boundary_conditions, &
               c_dns_step, c_dns_step_graph, c_dns_synchronize, c_dns_diagnostics, c_dns_check_divergence, &
-              c_dns_grid_nodes, &
+              c_dns_grid_nodes, c_dns_set_controller, &
               c_dns_fft_plan_create, c_dns_fft_plan_destroy, c_dns_fft_forward, c_dns_fft_backward, &
               dns_error_message
 
@@ -26,6 +26,8 @@ module dns_c_api
         real(c_double) :: perturbation_amplitude
         integer(c_long_long) :: seed
         integer(c_int) :: noise_kmax, precision, lgl_derivative
+        integer(c_int) :: convection_scheme, pressure_solver
+        real(c_double) :: cgstol
     end type dns_params_t
 
     type, bind(C) :: dns_diag_t
@@ -59,6 +61,13 @@ module dns_c_api
         integer(c_int) function c_dns_get_fields(ctx, u, v, w, p) bind(C, name="dns_get_fields")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, u, v, w, p
+        end function
+        integer(c_int) function c_dns_set_controller(ctx, g, integral, prev, gsum, gcount) &
+                bind(C, name="dns_set_controller")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: g, integral, prev, gsum
+            integer(c_int), value :: gcount
         end function
         integer(c_int) function c_dns_begin_step(ctx) bind(C, name="dns_begin_step")
             import :: c_int, c_ptr

@@ -64,6 +64,17 @@ typedef struct dns_params {
     int lgl_derivative;                 /* 0: D1 as lgl_module.f90:211-263 (entries whose Lagrange
                                            denominator is <= 1e-15 set to 0: D1 is diagonal for
                                            nz >= 65); 1: exact LGL D1 (not in the reference) */
+    /* Optional numerics variants of the 2D solver applied to the 3D step (no 3D counterpart):
+     * convection_scheme 0 = the 3D step's single product u x omega; 1 = the 4-stage 1/irk
+     * scheme of runge_kutta_convection (DNS_pressure_BC_2D.f90:931-1014): S is re-evaluated
+     * at un + (dt/irk) S for irk = 4..1 (walls zero after each stage) and the last S is the
+     * source term.  pressure_solver 0 = direct per-mode solve (zgesv, DNS_pressure_BC_3D.f90:1714);
+     * 1 = jacgs, Jacobi-preconditioned CG (DNS_pressure_BC_2D.f90:2361-2476) on the real and
+     * imaginary parts, x0 = 0, at most 500 iterations, stop at ||r|| <= cgstol (&simulation
+     * cgstol); FP64 only, not with the weak form, nz <= 192.  Single contexts and batches only. */
+    int convection_scheme;
+    int pressure_solver;
+    double cgstol;
 } dns_params;
 
 /* Diagnostics of output_diagnostics_3d / check_divergence_3d

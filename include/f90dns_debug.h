@@ -55,6 +55,9 @@ int dns_debug_wavenumbers(int nx, int ny, double xlen, double ylen, double* xw, 
  * (nx/2+1, ny) bytes, kx fastest, 1 = kept: which 0 = the table the two-pass transforms read,
  * 1 = the bounds and test the plane kernels apply; host only. */
 int dns_debug_dealias_mask(int nx, int ny, int which, unsigned char* mask);
+/* Counters of the last jacgs pressure solve (pressure_solver 1): total iterations over all
+ * real systems, and systems that stopped at the iteration limit without converging. */
+int dns_debug_jacgs_stats(dns_ctx* ctx, unsigned long long* iterations, int* nonconverged);
 /* The device-resident wavenumber tables of a context, copied back: xw(nx/2+1), yw(ny). */
 int dns_debug_context_wavenumbers(dns_ctx* ctx, double* xw, double* yw);
 

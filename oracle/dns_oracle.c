@@ -328,6 +328,8 @@ struct orc_ctx {
     /* scratch */
     double *f1, *f2, *f3, *f4, *f5, *f6, *f7, *f8, *f9;
     cpx *spec, *spec2;
+    double *un, *vn, *wn;           /* runge_kutta_convection: velocities at the start of the step */
+    long long jac_iters; int jac_fail;
     /* cached LU factors (identical to refactoring: the matrices are step-invariant) */
     double* hlu; int* hpiv; int hready;
     double* plu; int* ppiv; int pready;
@@ -384,6 +386,7 @@ orc_ctx* orc_create(const orc_params* p) {
     c->f1 = dalloc(N); c->f2 = dalloc(N); c->f3 = dalloc(N); c->f4 = dalloc(N); c->f5 = dalloc(N);
     c->f6 = dalloc(N); c->f7 = dalloc(N); c->f8 = dalloc(N); c->f9 = dalloc(N);
     c->spec = (cpx*)calloc(c->NS, sizeof(cpx)); c->spec2 = (cpx*)calloc(c->NS, sizeof(cpx));
+    if (p->convection_scheme == 1) { c->un = dalloc(N); c->vn = dalloc(N); c->wn = dalloc(N); }
     /* read_input_3d (DNS_pressure_BC_3D.f90:629-641) */
     c->pi.G = p->target_pressure_gradient; c->pi.gavg = p->target_pressure_gradient;
     return c;
@@ -393,7 +396,8 @@ void orc_destroy(orc_ctx* c) {
     if (!c) return;
     double* fs[] = { c->xw, c->xsq, c->yw, c->ysq, c->z, c->wz, c->d1, c->d2, c->stiff, c->u, c->v, c->w,
                      c->p_total, c->ustar, c->vstar, c->wstar, c->phi, c->su, c->sv, c->sw, c->gpx, c->gpy,
-                     c->f1, c->f2, c->f3, c->f4, c->f5, c->f6, c->f7, c->f8, c->f9, c->hlu, c->plu };
+                     c->f1, c->f2, c->f3, c->f4, c->f5, c->f6, c->f7, c->f8, c->f9, c->hlu, c->plu,
+                     c->un, c->vn, c->wn };
     for (size_t i = 0; i < sizeof(fs) / sizeof(fs[0]); i++) free(fs[i]);
     free(c->mask); free(c->spec); free(c->spec2); free(c->hpiv); free(c->ppiv);
     plan2d_free(&c->plan);
@@ -543,15 +547,16 @@ static void dealias_field(orc_ctx* c, double* f) {
     }
 }
 
-/* compute_source_terms_3d (DNS_pressure_BC_3D.f90:1032-1148) */
-void orc_compute_source_terms(orc_ctx* c) {
+/* su, sv, sw = u x omega of c->u, c->v, c->w, dealiased when on
+ * (compute_source_terms_3d_standard :1054-1098 / _dealiased :1233-1261) */
+static void rotational_product(orc_ctx* c) {
     size_t N = c->N;
     double *ux = c->f1, *uy = c->f2, *uz = c->f3, *vx = c->f4, *vy = c->f5, *vz = c->f6;
     double *wx = c->f7, *wy = c->f8, *wz = c->f9;
     derivatives(c, c->u, ux, uy, uz);
     derivatives(c, c->v, vx, vy, vz);
     derivatives(c, c->w, wx, wy, wz);
-    /* omega (:1077-1079) stored over wy (ox), uz (oy), vx (oz) after use */
+    /* omega (:1077-1079) */
     #pragma omp parallel for schedule(static)
     for (size_t n = 0; n < N; n++) {
         double ox = wy[n] - vz[n], oy = uz[n] - wx[n], oz = vx[n] - uy[n];
@@ -562,6 +567,35 @@ void orc_compute_source_terms(orc_ctx* c) {
     }
     if (c->p.use_dealias_23_rule) {
         dealias_field(c, c->su); dealias_field(c, c->sv); dealias_field(c, c->sw);
+    }
+}
+
+/* compute_source_terms_3d (DNS_pressure_BC_3D.f90:1032-1148).  With convection_scheme 1 the
+ * product is staged as runge_kutta_convection does (DNS_pressure_BC_2D.f90:931-1014): from
+ * un = u, for irk = 4, 3, 2, 1: S = S(u), u = un + (1/irk) dt S, walls zero; the S of the last
+ * stage is the source term and u is reset to un.  The forcing follows, as in the 2D loop
+ * (apply_flow_control after convection_step, DNS_pressure_BC_2D.f90:217-223). */
+void orc_compute_source_terms(orc_ctx* c) {
+    size_t N = c->N;
+    if (c->p.convection_scheme == 1) {
+        size_t b = sizeof(double) * N, top = (size_t)c->npl * (c->nz - 1);
+        memcpy(c->un, c->u, b); memcpy(c->vn, c->v, b); memcpy(c->wn, c->w, b);
+        for (int irk = 4; irk >= 1; irk--) {
+            double alf = 1.0 / (double)irk, dt = c->p.dt;
+            rotational_product(c);
+            for (size_t n = 0; n < N; n++) {
+                c->u[n] = c->un[n] + alf * dt * c->su[n];
+                c->v[n] = c->vn[n] + alf * dt * c->sv[n];
+                c->w[n] = c->wn[n] + alf * dt * c->sw[n];
+            }
+            for (int m = 0; m < c->npl; m++) {      /* apply_wall_bc after each stage */
+                c->u[m] = 0.0; c->v[m] = 0.0; c->w[m] = 0.0;
+                c->u[top + m] = 0.0; c->v[top + m] = 0.0; c->w[top + m] = 0.0;
+            }
+        }
+        memcpy(c->u, c->un, b); memcpy(c->v, c->vn, b); memcpy(c->w, c->wn, b);
+    } else {
+        rotational_product(c);
     }
     /* apply_flow_control_forcing (:1107-1148) */
     if (c->p.flow_control_method == 1) c->pi.G = c->p.target_pressure_gradient;
@@ -705,6 +739,54 @@ static void divergence(orc_ctx* c, const double* fx, const double* fy, const dou
     for (size_t n = 0; n < c->N; n++) div[n] = div[n] + dyf[n] + dzf[n];
 }
 
+/* jacgs (DNS_pressure_BC_2D.f90:2361-2476): Jacobi-preconditioned conjugate gradients with the
+ * reference's loop order, small = 1e-30 guards and absolute stop ||r||_2 <= tol (checked on the
+ * initial residual and after each iteration). */
+int orc_jacgs(int n, const double* a, double* x, const double* b, int niter, double tol, int* iters) {
+    const double small = 1.0e-30;
+    double* r = dalloc(n); double* p = dalloc(n); double* q = dalloc(n);
+    double* apn = dalloc(n); double* diag = dalloc(n); double* sv = dalloc(n);
+    int info = 0, it = 0;
+    for (int i = 0; i < n; i++) {
+        diag[i] = fabs(a[i + (size_t)n * i]) > small ? 1.0 / a[i + (size_t)n * i] : 1.0;
+        sv[i] = 0.0;
+    }
+    for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) sv[i] = sv[i] + a[i + (size_t)n * j] * x[j];
+    for (int i = 0; i < n; i++) { r[i] = b[i] - sv[i]; q[i] = r[i] * diag[i]; p[i] = q[i]; }
+    double res0 = 0.0;
+    for (int i = 0; i < n; i++) res0 = res0 + r[i] * r[i];
+    res0 = sqrt(res0);
+    if (!(res0 <= tol)) {
+        double res1 = res0;
+        info = -1;
+        for (it = 1; it <= niter; it++) {
+            for (int i = 0; i < n; i++) apn[i] = 0.0;
+            for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) apn[i] = apn[i] + a[i + (size_t)n * j] * p[j];
+            double qdr = 0.0, pdapn = 0.0;
+            for (int i = 0; i < n; i++) { qdr = qdr + q[i] * r[i]; pdapn = pdapn + p[i] * apn[i]; }
+            double alfa = qdr / (pdapn + small);
+            for (int i = 0; i < n; i++) { x[i] = x[i] + alfa * p[i]; r[i] = r[i] - alfa * apn[i]; q[i] = r[i] * diag[i]; }
+            double qdrnew = 0.0;
+            for (int i = 0; i < n; i++) qdrnew = qdrnew + q[i] * r[i];
+            double beta = qdrnew / (qdr + small);
+            for (int i = 0; i < n; i++) p[i] = r[i] * diag[i] + beta * p[i];
+            res1 = 0.0;
+            for (int i = 0; i < n; i++) res1 = res1 + r[i] * r[i];
+            res1 = sqrt(res1);
+            if (res1 <= tol) { info = 0; break; }
+        }
+        if (it > niter) it = niter;
+    }
+    if (iters) *iters = it;
+    free(r); free(p); free(q); free(apn); free(diag); free(sv);
+    return info;
+}
+
+void orc_jacgs_stats(const orc_ctx* c, long long* iterations, int* nonconverged) {
+    if (iterations) *iterations = c->jac_iters;
+    if (nonconverged) *nonconverged = c->jac_fail;
+}
+
 /* solve_pressure_3d_standard (DNS_pressure_BC_3D.f90:1654-1733), or with
  * use_weak_form_divergence the RHS of solve_pressure_3d_weak_form (:1749-1835): per mode
  * rhs_k = [-(i xw u^ + i yw v^)_k w_k ybar/2 - sum_m D1(m,k) w^_m w_m 2/ybar] / dt from the
@@ -725,6 +807,8 @@ void orc_solve_pressure(orc_ctx* c) {
         r2c_field(c, div, c->spec);
     }
     int cached = !c->p.refactor_every_step;
+    const int jac = c->p.pressure_solver == 1;
+    c->jac_iters = 0; c->jac_fail = 0;
     if (cached && !c->pready) {
         c->plu = dalloc((size_t)nz * nz * nspl); c->ppiv = (int*)calloc((size_t)nz * nspl, sizeof(int));
     }
@@ -757,10 +841,12 @@ void orc_solve_pressure(orc_ctx* c) {
                     pm[0] = 1.0;
                 }
                 if (cached) memcpy(a, pm, sizeof(double) * (size_t)nz * nz);
-                int info = lu_factor(nz, a, pv);
-                if (info) {
-                    #pragma omp critical
-                    { if (!c->status) c->status = info; }   /* the reference stops here */
+                if (!jac) {
+                    int info = lu_factor(nz, a, pv);
+                    if (info) {
+                        #pragma omp critical
+                        { if (!c->status) c->status = info; }   /* the reference stops here */
+                    }
                 }
             }
             if (weak) {
@@ -789,7 +875,23 @@ void orc_solve_pressure(orc_ctx* c) {
                 }
             }
             if (i == 0 && j == 0) { br[0] = 0.0; bi[0] = 0.0; }
-            lu_solve(nz, a, pv, br); lu_solve(nz, a, pv, bi);
+            if (jac) {
+                /* jacgs on the real and the imaginary part, each from x = 0 (DNS_pressure_BC_2D.f90:1629) */
+                double* xr = dalloc(nz); double* xi = dalloc(nz);
+                int ir = 0, ii = 0;
+                int fr = orc_jacgs(nz, a, xr, br, 500, c->p.cgstol, &ir);
+                int fi = orc_jacgs(nz, a, xi, bi, 500, c->p.cgstol, &ii);
+                #pragma omp atomic
+                c->jac_iters += ir + ii;
+                if (fr || fi) {
+                    #pragma omp atomic
+                    c->jac_fail += (fr != 0) + (fi != 0);
+                }
+                memcpy(br, xr, sizeof(double) * nz); memcpy(bi, xi, sizeof(double) * nz);
+                free(xr); free(xi);
+            } else {
+                lu_solve(nz, a, pv, br); lu_solve(nz, a, pv, bi);
+            }
             for (int k = 0; k < nz; k++) { c->spec2[m0 + (size_t)nspl * k].re = br[k]; c->spec2[m0 + (size_t)nspl * k].im = bi[k]; }
         }
     }

@@ -46,6 +46,13 @@ typedef struct orc_params {
     int use_weak_form_divergence;  /* 1: solve_pressure_3d_weak_form (DNS_pressure_BC_3D.f90:1749-1835) */
     int fft_variant;               /* 0: Stockham factors 4s, 2s, odd primes; 1: radix 2 in place of 4,
                                       odd factors first (an equivalent transform, other rounding) */
+    /* optional numerics variants of the 2D solver, applied to the 3D step (parity unpinned) */
+    int convection_scheme;         /* 0: the 3D step's single evaluation of u x omega; 1: the 4-stage
+                                      1/irk scheme of runge_kutta_convection (DNS_pressure_BC_2D.f90:931-1014) */
+    int pressure_solver;           /* 0: direct per-mode solve (zgesv, :1714); 1: jacgs, Jacobi-
+                                      preconditioned CG (DNS_pressure_BC_2D.f90:2361-2476), real and
+                                      imaginary parts separately, x0 = 0, 500 iterations, tol = cgstol */
+    double cgstol;                 /* &simulation cgstol: absolute residual tolerance of jacgs */
 } orc_params;
 
 /* PI controller state (DNS_pressure_BC_3D.f90:145-152): current_pressure_gradient,
@@ -106,6 +113,11 @@ void orc_diff_matrix(int n, const double* nodes, double* d1 /* column-major n x 
 void orc_diff_matrix_exact(int n, const double* nodes, double* d1 /* column-major n x n */);
 int orc_status(const orc_ctx* c);   /* 0, or k+1 when a per-mode LU met a zero pivot at column k
                                        (the reference's zgesv info /= 0 stop) */
+/* jacgs (DNS_pressure_BC_2D.f90:2361-2476) on one real system a(n,n) column-major, x in/out;
+ * returns 0, or -1 when niter iterations did not reach ||r|| <= tol; *iters gets the count. */
+int orc_jacgs(int n, const double* a, double* x, const double* b, int niter, double tol, int* iters);
+/* solves of the last jacgs pressure stage: total iterations and non-converged systems */
+void orc_jacgs_stats(const orc_ctx* c, long long* iterations, int* nonconverged);
 void orc_wavenumbers(int nx, int ny, double xlen, double ylen, double* xw, double* yw);
 void orc_dealias_mask(int nx, int ny, unsigned char* mask /* (nx/2+1, ny) */);
 int  orc_fft_r2c(int nx, int ny, int nplanes, const double* in, double* out_interleaved);

@@ -28,6 +28,7 @@ class OrcParams(ctypes.Structure):
         ("perturbation_amplitude", ctypes.c_double), ("seed", ctypes.c_ulonglong), ("noise_kmax", ctypes.c_int),
         ("refactor_every_step", ctypes.c_int), ("lgl_derivative", ctypes.c_int),
         ("use_weak_form_divergence", ctypes.c_int), ("fft_variant", ctypes.c_int),
+        ("convection_scheme", ctypes.c_int), ("pressure_solver", ctypes.c_int), ("cgstol", ctypes.c_double),
     ]
 
 
@@ -99,6 +100,9 @@ def lib():
         L.orc_pi_update.restype = ctypes.c_int
         L.orc_bulk_velocity.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.orc_bulk_velocity.restype = ctypes.c_double
+        L.orc_jacgs.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+        L.orc_jacgs.restype = ctypes.c_int
+        L.orc_jacgs_stats.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -115,7 +119,8 @@ DEFAULTS = dict(
     controller_update_freq=7, istart=0, u_centerline=1.0,
     enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1, analytical_mode_y=1,
     perturbation_amplitude=0.01, seed=0, noise_kmax=8, refactor_every_step=0,
-    lgl_derivative=0, use_weak_form_divergence=0, fft_variant=0,
+    lgl_derivative=0, use_weak_form_divergence=0, fft_variant=0, convection_scheme=0, pressure_solver=0,
+    cgstol=1e-6,
 )
 
 
@@ -218,6 +223,11 @@ class Oracle:
     def max_divergence(self):
         return lib().orc_max_divergence(self._h)
 
+    def jacgs_stats(self):
+        it = ctypes.c_longlong(); nf = ctypes.c_int()
+        lib().orc_jacgs_stats(self._h, ctypes.byref(it), ctypes.byref(nf))
+        return it.value, nf.value
+
 
 def lgl(n):
     x = np.empty(n); w = np.empty(n)
@@ -303,3 +313,14 @@ def bulk_velocity(u):
     u = np.ascontiguousarray(u, dtype=np.float64)
     nz, ny, nx = u.shape
     return lib().orc_bulk_velocity(nx, ny, nz, _ptr(u))
+
+
+def jacgs(a, b, niter=500, tol=1e-6, x0=None):
+    """orc_jacgs on a real (n, n) system: (x, info, iterations)."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    n = a.shape[0]
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = ctypes.c_int()
+    info = lib().orc_jacgs(n, a.ctypes.data_as(_dp), _ptr(x), _ptr(b), niter, tol, ctypes.byref(it))
+    return x, info, it.value

@@ -185,3 +185,46 @@ def test_library_dealias_mask_bitwise_equal_oracle(nx, ny):
     want = orc.dealias_mask(nx, ny)
     for which in (0, 1):
         np.testing.assert_array_equal(dns.dealias_mask(nx, ny, which), want)
+
+
+def test_fortran_driver_command_line_without_gpu(tmp_path):
+    """process_command_line_args (DNS_pressure_BC_3D.f90:445-528) before any device work: --help
+    prints the usage and stops, an unknown argument stops with an error, no argument selects
+    input_3d.dat, --restart [file.dat] is accepted (and then reset by read_input_3d, :626-627)."""
+    import subprocess
+    from f90_dns_amd import build as b
+    if b.amdflang() is None:
+        pytest.skip("no Fortran compiler")
+    exe = os.path.join(b.BIN, "dns_3d_gpu")
+    if not os.path.exists(exe):
+        b.build_fortran()
+    run = lambda *a: subprocess.run([exe, *a], capture_output=True, text=True, cwd=tmp_path, timeout=60)
+    r = run("--help")
+    assert r.returncode == 0 and "USAGE:" in r.stdout and "--restart [file]" in r.stdout
+    assert "dns_create" not in r.stdout
+    r = run("-h")
+    assert "USAGE:" in r.stdout
+    r = run("bogus")
+    assert "ERROR: Unknown argument: bogus" in r.stdout and "dns_create" not in r.stdout
+    r = run()
+    assert "Using default input file: input_3d.dat" in r.stdout
+    assert "WARNING: Could not open input file: input_3d.dat" in r.stdout
+    r = run("--restart", "saved.dat", "case.dat")
+    assert "Restart requested from file: saved.dat" in r.stdout
+    assert "Using specified input file: case.dat" in r.stdout
+    assert "Command line restart mode enabled" in r.stdout
+    r = run("--restart")
+    assert "Restart requested from default file: restart.dat" in r.stdout
+
+
+def test_ensemble_rejects_the_numerics_variants_before_touching_a_device():
+    import ctypes
+    import f90_dns_amd as dns
+    if not os.path.exists(dns.library_path()):
+        pytest.skip("libf90dns.so not built")
+    L = dns.lib()
+    a = dns.make_params(nx=32, ny=32, nz=33, convection_scheme=1)
+    arr = (dns.DnsParams * 1)(a)
+    h = ctypes.c_void_p()
+    assert L.dns_ensemble_create(arr, 1, ctypes.byref(h)) == 6
+    assert b"single contexts and batches only" in L.dns_ensemble_last_error(None)

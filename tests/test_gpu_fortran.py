@@ -55,9 +55,16 @@ def test_fortran_driver_reproduces_reference_trace(tmp_path, fused, weak):
                 f"dP/dx={float(row['dPdx']):12.4E}")
         norm = lambda s: re.sub(r"\s+", " ", s.strip().rstrip(","))
         assert norm(line) == norm(want), (line, want)
-    restart = tmp_path / "restart.dat"
+    import glob
+    dirs = glob.glob(str(tmp_path / "input_????????_??????"))
+    assert len(dirs) == 1
+    restart = os.path.join(dirs[0], "restart.dat")
     n = 32 * 32 * 17
-    assert restart.stat().st_size == 18 + 4 + 16 + 12 + 24 + 7 * 8 * n + 32
+    assert os.path.getsize(restart) == 18 + 4 + 16 + 12 + 24 + 7 * 8 * n + 32
+    # the reference's restart detection finds nothing in the new directory and starts new
+    assert "WARNING: Restart file not found: " + dirs[0].split("/")[-1] + "/restart.dat" in r.stdout.replace(
+        str(tmp_path) + "/", "")
+    assert "STARTING NEW SIMULATION" in r.stdout
 
 
 def test_fortran_driver_output_files(tmp_path):
@@ -98,7 +105,7 @@ def test_fortran_driver_output_files(tmp_path):
             st[k.strip()] = float(v)
     assert st["u_max"] == float(rows[-1][2]) and st["kinetic_energy"] == float(rows[-1][5])
     n = 32 * 32 * 17
-    raw = (tmp_path / "restart.dat").read_bytes()
+    raw = open(os.path.join(dirs[0], "restart.dat"), "rb").read()
     u = np.frombuffer(raw, dtype="<f8", count=n, offset=18 + 4 + 16 + 12 + 24).reshape(17, 32, 32)
     z, _ = orc.lgl(17)
     nu = 1.0 / 500.0
@@ -120,3 +127,78 @@ def test_fortran_driver_output_files(tmp_path):
     for c in range(3):   # plane means of the snapshot's own (rounded) values
         means = snap[:, 6 + c].reshape(17, 32 * 32).mean(axis=1)
         assert np.allclose(prof[:, 2 + c], means, rtol=1e-5, atol=1e-12), c
+
+
+def _restart_fields(path, nx, ny, nz):
+    import numpy as np
+    raw = open(path, "rb").read()
+    n = nx * ny * nz
+    head = 18
+    assert raw[:head] == b"DNS3D_RESTART_v1.0"
+    step = int(np.frombuffer(raw, "<i4", 1, head)[0])
+    t, dt = np.frombuffer(raw, "<f8", 2, head + 4)
+    dims = np.frombuffer(raw, "<i4", 3, head + 20)
+    off = head + 4 + 16 + 12 + 24
+    f = [np.frombuffer(raw, "<f8", n, off + 8 * n * i).reshape(nz, ny, nx) for i in range(7)]
+    ctl = np.frombuffer(raw, "<f8", 4, off + 56 * n)
+    return step, t, tuple(int(x) for x in dims), f, ctl
+
+
+def test_fortran_driver_restart_files_summary_and_resume(tmp_path):
+    """restart.dat every restart_write_freq steps before the step body and after the last step
+    (DNS_pressure_BC_3D.f90:307-309, 367), simulation_summary.txt (:2594-2659), the timing
+    summary every 100 steps (:2207-2230), and a run resumed from a restart file (&gpu
+    restart_file) against the oracle resumed from the same fields with zero wall gradients."""
+    import glob
+
+    import numpy as np
+
+    from oracle import oracle as orc
+    exe = os.path.join(BIN, "dns_3d_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran driver not built")
+    base = open(os.path.join(ROOT, "tests", "golden", "trace_input.dat")).read()
+    assert "nsteps = 15" in base and "restart_write_freq" in base
+    src = re.sub(r"restart_write_freq\s*=\s*\d+", "restart_write_freq = 4", base).replace("nsteps = 15", "nsteps = 10")
+    a_dir = tmp_path / "a"
+    a_dir.mkdir()
+    (a_dir / "first.dat").write_text(src)
+    r = _run([exe, "first.dat"], a_dir)
+    assert r.returncode == 0, r.stdout + r.stderr
+    written = re.findall(r"Restart file written at step (\d+), time =\s*([0-9.]+)", r.stdout)
+    assert [int(s) for s, _ in written] == [4, 8, 10], written
+    # periodic files carry the post-increment time of their step (:296, 307)
+    assert abs(float(written[0][1]) - 4 * 0.002) < 1e-9 and abs(float(written[2][1]) - 10 * 0.002) < 1e-9
+    dirs = glob.glob(str(a_dir / "first_????????_??????"))
+    assert len(dirs) == 1
+    summ = open(os.path.join(dirs[0], "simulation_summary.txt"), encoding="utf-8").read().splitlines()
+    assert summ[1] == "   3D DNS SIMULATION SUMMARY" and " Input file: first.dat" in summ
+    assert "   Grid size: 32 \u00d7 32 \u00d7 17" in summ and "   Total steps: 10" in summ
+    assert "   Restart write frequency: 4" in summ and "   Restart mode: DISABLED (new simulation)" in summ
+    step, t, dims, f, ctl = _restart_fields(os.path.join(dirs[0], "restart.dat"), 32, 32, 17)
+    assert step == 10 and dims == (32, 32, 17) and abs(t - 0.02) < 1e-15
+    # resume for 5 more steps from that file
+    b_dir = tmp_path / "b"
+    b_dir.mkdir()
+    (b_dir / "second.dat").write_text(src.replace("nsteps = 10", "nsteps = 15") +
+                                      "&gpu\n  restart_file = '%s'\n/\n" % os.path.join(dirs[0], "restart.dat"))
+    r = _run([exe, "second.dat"], b_dir)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "RESTARTING FROM CHECKPOINT" in r.stdout and "Remaining steps: 5" in r.stdout
+    steps = [int(m) for m in re.findall(r"^ Step\s+(\d+), t=", r.stdout, flags=re.M)]
+    assert steps == [11, 12, 13, 14, 15]
+    d2 = glob.glob(str(b_dir / "second_????????_??????"))[0]
+    step2, t2, _, f2, _ = _restart_fields(os.path.join(d2, "restart.dat"), 32, 32, 17)
+    assert step2 == 15 and abs(t2 - 15 * 0.002) < 1e-12
+    kw = dict(nx=32, ny=32, nz=17, xlen=2 * np.pi, ylen=np.pi, re=500.0, dt=0.002, u_centerline=1.5,
+              flow_control_method=1, target_pressure_gradient=0.006, use_dealias_23_rule=1,
+              enable_perturbations=1, perturbation_kind=0, perturbation_amplitude=0.01)
+    o = orc.Oracle(**kw)
+    o.set_fields(f[0], f[1], f[2], f[6])          # wall gradients stay zero, as the reference's
+    o.set_controller(ctl[0], ctl[2], ctl[3])
+    o.step(5)
+    ref = o.get_fields()
+    for name, got, want in zip("uvwp", (f2[0], f2[1], f2[2], f2[6]), ref):
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        print(f"resumed {name}: {err:.2e}")
+        assert err < (1e-8 if name == "w" else 1e-10), name

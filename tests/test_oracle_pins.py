@@ -376,3 +376,79 @@ def test_w_rounding_floor_of_the_oracle_itself():
     e = field_errors(b.get_fields(), a.get_fields())
     assert e["u"] < 1e-13 and e["v"] < 1e-12 and e["p"] < 1e-13, e
     assert 5e-11 < e["w_self"] < 5e-10, e
+
+
+# --- optional numerics variants of the 2D solver on the 3D step (parity unpinned: no 3D
+# trace of the reference exists for them) ------------------------------------------------
+def test_jacgs_two_by_two_by_hand():
+    # A = [[4, 1], [1, 3]], b = [1, 2]: Jacobi-preconditioned CG reaches the exact solution
+    # x = A^-1 b = [1/11, 7/11] in n = 2 iterations (a wrong alfa, beta or preconditioner does not)
+    x, info, it = orc.jacgs(np.array([[4.0, 1.0], [1.0, 3.0]]), np.array([1.0, 2.0]), tol=1e-14)
+    assert info == 0 and it == 2
+    np.testing.assert_allclose(x, [1 / 11, 7 / 11], rtol=1e-14)
+    # first iterate by hand: r0 = b, q0 = p0 = D^-1 b = [1/4, 2/3], A p0 = [5/3, 9/4],
+    # alfa = (q0.r0)/(p0.Ap0) = (19/12)/(23/12) = 19/23, x1 = alfa p0
+    x1, _, it1 = orc.jacgs(np.array([[4.0, 1.0], [1.0, 3.0]]), np.array([1.0, 2.0]), niter=1, tol=1e-14)
+    assert it1 == 1
+    np.testing.assert_allclose(x1, [19 / 23 / 4, 19 / 23 * 2 / 3], rtol=1e-14)
+
+
+def test_jacgs_limits_and_definite_systems():
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((30, 30))
+    A = M @ M.T + 30 * np.eye(30)
+    b = rng.standard_normal(30)
+    x, info, it = orc.jacgs(A, b, tol=1e-10)
+    assert info == 0 and it < 100
+    assert np.linalg.norm(b - A @ x) <= 1e-10
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-8)
+    # negative definite (the Poisson operator's sign) converges the same way
+    xn, info, _ = orc.jacgs(-A, b, tol=1e-10)
+    assert info == 0
+    np.testing.assert_allclose(xn, -x, rtol=1e-8)
+    # ||b|| <= tol: no iteration, x stays 0; tol = 0: the iteration limit, info = -1
+    x0, info, it = orc.jacgs(A, 1e-9 * b / np.linalg.norm(b), tol=1e-8)
+    assert info == 0 and it == 0 and np.all(x0 == 0)
+    _, info, it = orc.jacgs(A, b, niter=7, tol=0.0)
+    assert info == -1 and it == 7
+
+
+def test_jacgs_pressure_stage_matches_direct_solve_when_tight():
+    kw = dict(nx=16, ny=12, nz=17, flow_control_method=2, perturbation_kind=1, seed=5, noise_kmax=3,
+              perturbation_amplitude=0.05)
+    a = orc.Oracle(**kw); a.init_fields(); a.step(3)
+    b = orc.Oracle(pressure_solver=1, cgstol=1e-12, **kw); b.init_fields(); b.step(3)
+    it, nf = b.jacgs_stats()
+    assert nf == 0 and it > 0
+    for name, x, y in zip("uvwp", b.get_fields(), a.get_fields()):
+        assert np.linalg.norm(x - y) <= (1e-9 if name == "w" else 1e-12) * np.linalg.norm(y), name
+
+
+def test_rk_convection_stages_compose_the_single_product():
+    """convection_scheme 1 against the stage recurrence run by hand on the oracle's own single
+    product (convection_scheme 0 on the stage fields): S4 = S(un), u3 = un + dt/4 S4, ...,
+    su = S(u1) + G with u1 = un + dt/2 S2, walls zero after each stage, fields back to un."""
+    kw = dict(nx=16, ny=12, nz=17, re=150.0, dt=0.004, flow_control_method=1, target_pressure_gradient=0.01,
+              perturbation_kind=1, seed=9, noise_kmax=3, perturbation_amplitude=0.08)
+    r = orc.Oracle(convection_scheme=1, **kw); r.init_fields()
+    un = r.get_fields()
+    r.compute_source_terms()
+    got = r.source_terms()
+    assert all(np.array_equal(x, y) for x, y in zip(r.get_fields(), un))
+    s = orc.Oracle(**kw)
+    f = [x.copy() for x in un[:3]]
+    for irk in (4, 3, 2, 1):
+        s.set_fields(*f, un[3])
+        s.compute_source_terms()
+        S = s.source_terms()
+        S[0] = S[0] - 0.01                       # the single product's forcing, added once at the end
+        f = [u0 + (1.0 / irk) * kw["dt"] * sc for u0, sc in zip(un[:3], S)]
+        for x in f:
+            x[0] = 0.0; x[-1] = 0.0
+    S[0] = S[0] + 0.01
+    for x, y in zip(got, S):
+        np.testing.assert_allclose(x, y, rtol=0, atol=1e-15 * max(1.0, np.abs(y).max()))
+    # and it is a different scheme from the single evaluation (O(dt) apart)
+    e = orc.Oracle(**kw); e.init_fields(); e.compute_source_terms()
+    d = np.abs(e.source_terms()[1] - got[1]).max()
+    assert 1e-8 < d < 1e-2
