@@ -116,6 +116,7 @@ def lib():
             getattr(L, n).argtypes = [_vp, _vp, _vp, ctypes.c_int]
         for n in ("dns_fft_forward_device", "dns_fft_backward_device"):
             getattr(L, n).argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.dns_fft_execute_device_noalloc.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp]
         L.dns_debug_spectral.argtypes = [_vp, ctypes.c_int, _vp]
         L.dns_debug_zderiv.argtypes = [_vp, _vp, _vp]
         L.dns_debug_mfma_probe.argtypes = [_dp]

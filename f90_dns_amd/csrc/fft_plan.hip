@@ -2,7 +2,9 @@
 // device arrays, the drop-in for fftw3_dns_module.f90:19-152.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,8 +20,38 @@ bool dns_check_gfx950(std::string& why);
 bool dns_make_factors(int n, Factors& f);
 
 
+// Scratch of dns_fft_execute_device_noalloc: the intermediate planes of the two-pass FP64
+// transforms (planes that do not fit the LDS in FP64), in device memory that exists for the
+// life of the process, one copy per device.  2 Mi doubles = 16 MiB: 15 planes of the largest
+// supported size (512 x 256), 56 of 192 x 192; larger batches run in chunks of that many planes.
+static constexpr size_t kScratchDoubles = size_t(2) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static constexpr int kMaxDevices = 64;
+static double* g_scratch_dev[kMaxDevices];   // device address of `scratch`, per device
+static std::mutex g_scratch_mu;
+
+// hipGetSymbolAddress of the scratch array on the current device, once per device (at plan
+// creation); host code uses only these addresses, never the array's name
+static int scratch_setup(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        dns_set_create_error("device index beyond the scratch table");
+        return DNS_EDEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev]) return 0;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        dns_set_create_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        return DNS_EDEVICE;
+    }
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+    return 0;
+}
+
 struct dns_fft_plan {
     FFTGeom g;
+    int dev = 0;                 // device the plan (its twiddles) lives on
     bool f32 = false;
     double *twx = nullptr, *twy = nullptr;
     float *twx32 = nullptr, *twy32 = nullptr;
@@ -49,6 +81,8 @@ int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
     if (!dns_check_gfx950(why)) { dns_set_create_error(why); return DNS_EDEVICE; }
     dns_fft_plan* p = new dns_fft_plan();
     p->f32 = precision == DNS_PREC_MIXED;
+    if (hipGetDevice(&p->dev) != hipSuccess) { delete p; dns_set_create_error("hipGetDevice failed"); return DNS_EDEVICE; }
+    if (int r = scratch_setup(p->dev)) { delete p; return r; }
     if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0 && (nx / 2 > 256 || ny > 256)) ||
         !dns_make_factors(ny, p->g.fy) ||
         !dns_make_factors(nx / 2, p->g.fx)) {
@@ -122,6 +156,50 @@ int dns_fft_backward_device(dns_fft_plan* p, const double* in, double* out, int 
                                                                                                         : DNS_EDEVICE;
     }
     return launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, p->f32, s) == hipSuccess ? 0 : DNS_EDEVICE;
+}
+
+int dns_fft_execute_device_noalloc(dns_fft_plan* p, int direction, const double* in, double* out, int nplanes,
+                                   void* stream) {
+    if (!p || !in || !out || nplanes < 0 || (direction != DNS_FFT_FORWARD && direction != DNS_FFT_BACKWARD)) {
+        dns_set_create_error("dns_fft_execute_device_noalloc: bad argument");
+        return DNS_EINVAL;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != p->dev) {
+        dns_set_create_error("dns_fft_execute_device_noalloc: the plan's device is not the current device");
+        return DNS_EINVAL;
+    }
+    const bool fwd = direction == DNS_FFT_FORWARD;
+    const long long R = (long long)p->g.nx * p->g.ny, C = 2LL * p->g.nxh * p->g.ny;
+    const long long sstride = fwd ? R : C, dstride = fwd ? C : R;
+    hipStream_t s = stream ? (hipStream_t)stream : p->stream;
+    if (p->f32 || p->g.cls64 >= 0) {   // LDS-resident planes: no scratch
+        FFTArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.src[0] = in; a.dst[0] = out;
+        a.sstride = sstride; a.dstride = dstride;
+        const hipError_t e = fwd ? launch_r2c(p->g, R2C_PLAIN, a, nplanes, 1, p->f32, s)
+                                 : launch_c2r(p->g, C2R_PLAIN, a, nplanes, 1, p->f32, s);
+        return e == hipSuccess ? 0 : DNS_EDEVICE;
+    }
+    // two passes through the scratch: one spectral plane (C doubles) per transformed plane
+    if (size_t(C) > kScratchDoubles) {
+        dns_set_create_error("dns_fft_execute_device_noalloc: one plane needs " + std::to_string(C) +
+                             " doubles of scratch, more than the " + std::to_string(kScratchDoubles) + " it has");
+        return DNS_EINVAL;
+    }
+    double* scr = g_scratch_dev[dev];
+    if (!scr) {
+        dns_set_create_error("dns_fft_execute_device_noalloc: scratch not set up on this device");
+        return DNS_EDEVICE;
+    }
+    const int chunk = int(kScratchDoubles / size_t(C));
+    for (int k0 = 0; k0 < nplanes; k0 += chunk) {
+        const int n = std::min(chunk, nplanes - k0);
+        if (launch_two_pass(p->g, fwd, in + sstride * k0, out + dstride * k0, scr, n, sstride, dstride, s) != hipSuccess)
+            return DNS_EDEVICE;
+    }
+    return 0;
 }
 
 int dns_fft_forward(dns_fft_plan* p, const double* in, double* out, int nplanes) {

@@ -181,6 +181,20 @@ int dns_fft_backward(dns_fft_plan* plan, const double* complex_in, double* real_
 int dns_fft_forward_device(dns_fft_plan* plan, const double* real_in, double* complex_out, int nplanes, void* stream);
 int dns_fft_backward_device(dns_fft_plan* plan, const double* complex_in, double* real_out, int nplanes, void* stream);
 
+/* Allocation-free plane transforms on HBM pointers (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  direction DNS_FFT_FORWARD is dns_fft_forward_device,
+ * DNS_FFT_BACKWARD dns_fft_backward_device, with the same results; the plan's device must be
+ * current.  Where the plane does not fit the LDS in FP64 (e.g. 192 x 192) the transform takes two
+ * passes through a scratch array that the library holds in device memory for the life of the
+ * process (16 MiB per device, set up by dns_fft_plan_create); batches larger than it holds run
+ * in chunks.  The call allocates nothing.  Every call on one device shares that scratch: calls
+ * that overlap on one device (other streams, other threads) race and must be serialised by the
+ * caller; calls on one stream are ordered. */
+enum { DNS_FFT_FORWARD = -1, DNS_FFT_BACKWARD = 1 };
+int dns_fft_execute_device_noalloc(dns_fft_plan* plan, int direction, const double* in, double* out, int nplanes,
+                                   void* stream);
+
 /* Library version string. */
 const char* dns_version(void);
 
