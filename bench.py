@@ -18,6 +18,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+METRIC = "grid-point RK4 steps/sec, FP64 plain vs mixed+batched; p50/p99 step latency"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 
@@ -68,6 +69,35 @@ def kernel_work(kw):
     }
 
 
+def kernel_fractions(prof, work):
+    """Per step kernel: measured ms, algorithmic GB/s and FP64 TFLOP/s, and their fractions of
+    the HBM and FP64 peaks (the larger is the kernel's roofline fraction)."""
+    out = {}
+    for name, ms in prof.items():
+        if ms <= 0 or name not in work:
+            continue
+        wk = work[name]
+        gbs = wk["bytes"] / (ms * 1e-3) / 1e9
+        tfs = wk["flops"] / (ms * 1e-3) / 1e12
+        out[name] = {"ms": ms, "algorithmic_gbs": gbs, "hbm_frac": gbs / PEAK_HBM_GBS, "fp64_tflops": tfs,
+                     "fp64_frac": tfs / PEAK_FP64_TFLOPS, "frac": max(gbs / PEAK_HBM_GBS, tfs / PEAK_FP64_TFLOPS)}
+    return out
+
+
+def step_roofline(work, ms_per_step):
+    """The fused step's own roof: the algorithmic bytes and FP64 flops of all its kernels (each
+    charged one read of its inputs and one write of its outputs), the time each would take at
+    peak, and the measured step against the larger of the two."""
+    b = sum(w["bytes"] for w in work.values())
+    f = sum(w["flops"] for w in work.values())
+    t_hbm = b / (PEAK_HBM_GBS * 1e9) * 1e3
+    t_fp64 = f / (PEAK_FP64_TFLOPS * 1e12) * 1e3
+    roof_ms = max(t_hbm, t_fp64)
+    return {"algorithmic_bytes": b, "fp64_flops": f, "hbm_bound_ms": t_hbm, "fp64_bound_ms": t_fp64,
+            "achieved_gbs": b / (ms_per_step * 1e-3) / 1e9, "achieved_fp64_tflops": f / (ms_per_step * 1e-3) / 1e12,
+            "frac": roof_ms / ms_per_step}
+
+
 def profiled_traffic(profile, kernel):
     """HBM bytes per dispatch of a step kernel from a committed rocprof summary (FETCH_SIZE and
     WRITE_SIZE runs scaled by the calibration copy kernel); None when not profiled."""
@@ -81,21 +111,77 @@ def profiled_traffic(profile, kernel):
     return None
 
 
-def cpu_baseline(kw, steps):
-    """The oracle (C port of the reference algorithm as written: per-mode matrix rebuild and
-    LU every step, serial mode loops, OpenMP over planes/columns) on the host cores."""
+def cpu_info():
+    """Host CPU model and physical core count (lscpu), and the threads this job may use."""
+    import subprocess
+    model, cores, sockets = "unknown", None, 1
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k == "Model name":
+                model = v
+            elif k == "Core(s) per socket":
+                cores = int(v)
+            elif k == "Socket(s)":
+                sockets = int(v)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    physical = cores * sockets if cores else None
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(x for x in (physical, allowed, env or None) if x)
+    return {"model": model, "physical_cores": physical, "threads_available": threads}
+
+
+def _set_threads(n):
+    import ctypes
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+    except OSError:
+        pass
+
+
+def _oracle_run(kw, steps, threads):
+    """Per-step wall times of the oracle in the reference's as-written mode (per-mode matrix
+    rebuild and LU every step, serial mode loops, OpenMP over planes/columns)."""
     from oracle import oracle as orc
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    _set_threads(threads)
     o = orc.Oracle(refactor_every_step=1, **kw)
     o.init_fields()
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.step(1)
+        times.append(time.perf_counter() - t0)
+    return np.array(times)
+
+
+def cpu_baseline(kw, steps, config1_steps):
+    """The reported CPU baseline: the oracle -- the C port of the reference
+    algorithm as written -- on this host, the production workload for a few steps on all cores
+    available, config 1 (32x32x33) for config1_steps steps on all cores and on one thread;
+    wall-clock per step, set-up excluded."""
+    info = cpu_info()
+    thr = info["threads_available"]
     n = kw["nx"] * kw["ny"] * kw["nz"]
-    return {"value": n * steps / dt, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} step(s) of the same {kw['nx']}x{kw['ny']}x{kw['nz']} workload, FP64, "
-                      f"per-mode rebuild + LU every step as the reference, {threads} OpenMP threads, "
-                      f"{dt:.2f} s wall"}
+    t = _oracle_run(kw, steps, thr)
+    out = {"value": n * steps / t.sum(), "unit": "grid-point-steps/s", "cores": thr, "kind": "port",
+           "sample": f"{steps} step(s) of the same {kw['nx']}x{kw['ny']}x{kw['nz']} workload, FP64, as written "
+                     f"(per-mode rebuild + LU every step), {thr} OpenMP threads, {t.sum():.1f} s wall",
+           "cpu_model": info["model"], "physical_cores": info["physical_cores"],
+           "step_s": {"p50": float(np.percentile(t, 50)), "p99": float(np.percentile(t, 99))}}
+    if config1_steps > 0:
+        k1 = WORKLOADS["channel_32x32x33_fp64"]
+        n1 = k1["nx"] * k1["ny"] * k1["nz"]
+        for label, th in (("config1_all_threads", thr), ("config1_1_thread", 1)):
+            t1 = _oracle_run(k1, config1_steps, th)
+            out[label] = {"value": n1 * config1_steps / t1.sum(), "unit": "grid-point-steps/s", "threads": th,
+                          "steps": config1_steps, "steps_per_s": config1_steps / t1.sum(),
+                          "step_s": {"p50": float(np.percentile(t1, 50)), "p99": float(np.percentile(t1, 99))}}
+        _set_threads(thr)
+    return out
 
 
 def ensemble_variant(dns, steps, warmup, precision, autotune=True):
@@ -171,7 +257,8 @@ def main():
     ap.add_argument("--workload", default="channel_128x128x129_fp64", choices=sorted(WORKLOADS))
     ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of graph replay")
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the production workload")
+    ap.add_argument("--cpu-config1-steps", type=int, default=100, help="oracle steps of config 1 (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=200,
                     help="steps for the mixed-precision line reported beside the FP64 value (0 = skip)")
@@ -228,11 +315,29 @@ def main():
         if traffic is not None:
             roof["traffic_source"] = "profiles/" + profile
         roof["kernels_ms"] = prof
+        roof["kernels"] = kernel_fractions(prof, work)
     print("[bench] kernel ms: " + json.dumps(prof), file=sys.stderr, flush=True)
 
     variants = {}
     if args.variant_steps > 0:
         sim.close()
+        # the 2D solver's 4-stage 1/irk convection on the same step (flagged variant)
+        r = dns.Simulation(convection_scheme=1, **kw)
+        r.init_fields()
+        r.step(args.warmup, graph=graph)
+        r.synchronize()
+        t0 = time.perf_counter()
+        r.step(args.variant_steps, graph=graph)
+        r.synchronize()
+        rw = time.perf_counter() - t0
+        rl = r.step_timing()
+        variants["rk4_convection_fp64"] = {
+            "value": npts * args.variant_steps / rw, "unit": "grid-point-steps/s", "steps": args.variant_steps,
+            "ms_per_step": rw / args.variant_steps * 1e3,
+            "step_latency_ms": {"p50": rl["p50_ms"], "p99": rl["p99_ms"]},
+            "scheme": "4-stage 1/irk convection (runge_kutta_convection, DNS_pressure_BC_2D.f90:931-1014) "
+                      "in the 3D fractional step"}
+        r.close()
         m = dns.Simulation(precision=dns.DNS_PREC_MIXED, **kw)
         m.init_fields()
         m.step(args.warmup, graph=graph)
@@ -264,11 +369,12 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:
-        print(f"[bench] cpu baseline: {args.cpu_steps} oracle step(s)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(kw, args.cpu_steps)
+        print(f"[bench] cpu baseline: {args.cpu_steps} oracle step(s), config 1 x {args.cpu_config1_steps}",
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(kw, args.cpu_steps, args.cpu_config1_steps)
 
     out = {
-        "metric": "grid-point RK4 steps/sec (one full fractional step of DNS_pressure_BC_3D.f90)",
+        "metric": METRIC,
         "value": value,
         "unit": "grid-point-steps/s",
         "n_gpus": 1,
@@ -281,9 +387,13 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded stream-function noise initial field)",
         "config": {"workload": args.workload, "nx": kw["nx"], "ny": kw["ny"], "nz": kw["nz"],
-                   "precision": "fp64", "graph": graph, "parallelism": "single GPU"},
+                   "precision": "fp64", "graph": graph, "parallelism": "single GPU",
+                   "step": "one full fractional step of DNS_pressure_BC_3D.f90:314-326 (explicit-Euler "
+                           "convection, Crank-Nicolson Helmholtz, Poisson projection); the 3D reference has no "
+                           "RK4 -- the 2D solver's 4-stage convection is variants.rk4_convection_fp64"},
         "step_latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "min": lat["min_ms"], "mean": lat["mean_ms"]},
         "roofline": roof,
+        "step_roofline": step_roofline(work, wall / args.steps * 1e3),
         "cpu_baseline": cpu,
         "variants": variants,
     }

@@ -47,6 +47,12 @@ static_assert(8 % fold_chunk_ks(1) == 0 && 8 % fold_chunk_ks(2) == 0 && 8 % fold
 #define FOLD_PREFETCH_SMALL 1
 #endif
 constexpr int STAGE_LDS = 4 * 4 * 4 * LDB;   // doubles of the staging buffers (2 buffers x 2 panels x 16 rows)
+// the staging buffers of fold_gemm in a block of nw waves: 2 buffers x 2 panels x 4 fold_chunk_ks(nw)
+// rows (blocks of fewer waves stage shorter chunks, so they need less LDS and more of them fit a CU)
+__host__ __device__ constexpr int stage_lds(int nw) { return 4 * 4 * fold_chunk_ks(nw) * LDB; }
+static_assert(stage_lds(4) == STAGE_LDS, "four-wave staging");
+// waves per block of a contraction kernel variant (z_variant): 1, 2, 3, then 4
+__host__ __device__ constexpr int z_variant_nw(int var) { return var < 3 ? var + 1 : 4; }
 
 struct FoldPair {
     double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
@@ -57,7 +63,7 @@ struct FoldPair {
 // Panel row kf for the staging thread's mode (thread t of the block's NW waves stages rows
 // t/16 + 4 NW r of a chunk for mode t%16) is finish(kf, fetch(kf)), zero beyond K: fetch issues the memory reads (branch
 // free), finish does the arithmetic, so the next chunk's reads are in flight during the
-// current chunk's MFMA chain.  lds: STAGE_LDS doubles.
+// current chunk's MFMA chain.  lds: stage_lds(NW) doubles.
 // With op.extra, xacc accumulates the staging thread's share of the dense extra row times
 // panel 0 (reduce_extra sums the shares of a column).
 // SKIP: waves without row blocks skip the MFMA chain (operators with fewer row-block pairs
@@ -180,8 +186,9 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
 }
 
 // The same contraction with both panels resident in LDS (z0, z1: >= 4*ks rows, row stride LDB);
-// the A operands are fetched one chunk of 4 k-steps ahead (ksp is a multiple of 4 >= ks + 1,
-// so the chunk after the last one stays inside the block).
+// the A operands are fetched one chunk of 4 k-steps ahead (ksp is a multiple of 8 >= ks + 1,
+// so the chunk after the last one stays inside the block).  The last chunk stops at k-step ks
+// (the k-steps past it would add exact zeros: zero A columns on zero panel rows).
 template <int RBW, int NW>
 __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op,
                                               const double* z0, const double* z1) {
@@ -217,6 +224,7 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
 #pragma unroll
         for (int sl = 0; sl < 4; sl++) {
             const int s = ch * 4 + sl;
+            if (s >= op.ks) break;    // wave-uniform: the partial last chunk
             double b0[CT], b1[CT];
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
@@ -330,8 +338,8 @@ __device__ __forceinline__ void zero_panel_tail(double* z, int rows, int zr) {
     for (int e = threadIdx.x; e < (zr - rows) * BCOLS; e += 64 * NW) z[(rows + e / BCOLS) * LDB + e % BCOLS] = 0.0;
 }
 
-// panel rows of the LDS-resident z panels of a back contraction with k-steps ks: whole chunks
-// of 16 rows (the chunked contraction reads them all; rows past 4 ks are zero)
-__host__ __device__ inline int zpanel_rows(int ks) { return 16 * ((ks + 3) / 4); }
+// panel rows of the LDS-resident z panels of a back contraction with k-steps ks: 4 rows per
+// k-step (fold_gemm_lds reads rows 4 s + 0..3 for s < ks; rows past the operator's K are zeroed)
+__host__ __device__ inline int zpanel_rows(int ks) { return 4 * ks; }
 
 }  // namespace dns

@@ -728,8 +728,17 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     zero_acc2(acc1);
     fold_gemm_lds<RBW, NW>(acc0, acc1, a.pB, z0, z1);
     if constexpr (LX) {
-        // few row blocks: both projections unfold through LDS, after the z panels
-        double* G0 = smem + 2 * ZR * LDB;
+        // few row blocks: both back contractions (phi and D1 phi) run while the z panels are
+        // live, then both projections unfold through LDS over the dead panels (which lets more
+        // blocks share a CU than unfold buffers of their own would)
+        d4 dacc0[RBW][CT], dacc1[RBW][CT];
+        zero_acc2(dacc0);
+        zero_acc2(dacc1);
+        fold_gemm_lds<RBW, NW>(dacc0, dacc1, a.pDB, z1, z0);
+        const double xph = (a.pB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pB, z0) : 0.0;
+        const double xd = (a.pDB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pDB, z1) : 0.0;
+        __syncthreads();
+        double* G0 = smem;
         double* G1 = G0 + a.pB.rows0 * LDB;
         double* cc = G1 + a.pB.rows1 * LDB;   // per block column: kx_eff, ky_eff, live
         if (threadIdx.x < 16) {
@@ -740,7 +749,6 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                 cc[2 * BCOLS + ct * 16 + threadIdx.x] = live[ct] ? 1.0 : 0.0;
             }
         }
-        const double xph = (a.pB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pB, z0) : 0.0;
         store_groups<RBW, NW>(acc0, acc1, a.pB, G0, G1);
         if (a.pB.extra && threadIdx.x < BCOLS) G0[(a.pB.rows0 - 1) * LDB + threadIdx.x] = xph;
         __syncthreads();
@@ -785,11 +793,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
             }
         }
         __syncthreads();
-        zero_acc2(acc0);
-        zero_acc2(acc1);
-        fold_gemm_lds<RBW, NW>(acc0, acc1, a.pDB, z1, z0);
-        const double xd = (a.pDB.extra && threadIdx.x < BCOLS) ? extra_dot(a.pDB, z1) : 0.0;
-        store_groups<RBW, NW>(acc0, acc1, a.pDB, G0, G1);
+        store_groups<RBW, NW>(dacc0, dacc1, a.pDB, G0, G1);
         if (a.pDB.extra && threadIdx.x < BCOLS) G0[(a.pDB.rows0 - 1) * LDB + threadIdx.x] = xd;
         __syncthreads();
         for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
@@ -1215,19 +1219,25 @@ __global__ void k_reduce_final(ReduceArgs a) {
 // ---------------------------------------------------------------------------------------
 static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
 
+// Dynamic LDS of the contraction kernels: fold_gemm's staging (stage_lds of the variant's waves,
+// which also holds reduce_extra's 2 * 64 NW doubles), the two z panels of the back contractions,
+// and the LDS unfold of the few-wave variants (G0, G1 over the dead panels, plus per-column values).
 size_t zderiv_lds_bytes(const ZArgs& a) {
-    const size_t g = z_variant_lx(zd_variant(a)) ? size_t(a.d1.rows0 + a.d1.rows1) * LDB : 0;
-    return sizeof(double) * std::max<size_t>(STAGE_LDS, g);
+    const int var = zd_variant(a);
+    const size_t g = z_variant_lx(var) ? size_t(a.d1.rows0 + a.d1.rows1) * LDB : 0;
+    return sizeof(double) * std::max<size_t>(stage_lds(z_variant_nw(var)), g);
 }
 size_t helm_lds_bytes(const HelmArgs& a) {
+    const int var = helm_variant(a);
     const size_t z = size_t(2) * zpanel_rows(a.hB.ks) * LDB;
-    const size_t g = z_variant_lx(helm_variant(a)) ? size_t(a.hB.rows0 + a.hB.rows1) * LDB + 2 * BCOLS : 0;
-    return sizeof(double) * std::max<size_t>(std::max<size_t>(STAGE_LDS, z), g);
+    const size_t g = z_variant_lx(var) ? size_t(a.hB.rows0 + a.hB.rows1) * LDB + 2 * BCOLS : 0;
+    return sizeof(double) * std::max<size_t>(std::max<size_t>(stage_lds(z_variant_nw(var)), z), g);
 }
 size_t pois_lds_bytes(const PoisArgs& a) {
-    size_t z = size_t(2) * zpanel_rows(a.pB.ks) * LDB;
-    if (z_variant_lx(pois_variant(a))) z += size_t(a.pB.rows0 + a.pB.rows1) * LDB + 3 * BCOLS;
-    return sizeof(double) * std::max<size_t>(STAGE_LDS, z);
+    const int var = pois_variant(a);
+    const size_t z = size_t(2) * zpanel_rows(a.pB.ks) * LDB;
+    const size_t g = z_variant_lx(var) ? size_t(a.pB.rows0 + a.pB.rows1) * LDB + 3 * BCOLS : 0;
+    return sizeof(double) * std::max<size_t>(std::max<size_t>(stage_lds(z_variant_nw(var)), z), g);
 }
 
 // variants (z_variant): (pairs per wave, waves per block) = (1, 1), (1, 2), (1, 3), (1, 4), (2, 4)
