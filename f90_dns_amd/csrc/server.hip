@@ -5,9 +5,10 @@
 // advances all running jobs by one step with one replay of the ensemble's captured step, so
 // jobs that arrive while others run join the next step (continuous batching at step
 // granularity).  Finished jobs are retired (diagnostics, optionally fields, copied out) and
-// their slots reused; a job whose fields go non-finite is evicted alone.  Ticks are pipelined
-// one step deep: the host waits for step t-1 after queueing step t, except when membership
-// changes, which needs an idle stream.
+// their slots reused; a job whose fields go non-finite is evicted alone (its flow state is
+// copied out beside every health_every-th step and read when that step is harvested).  Ticks
+// are pipelined one step deep: the host waits for step t-1 after queueing step t, except when
+// membership changes, which needs an idle stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,6 +57,9 @@ bool same_group(const dns_params& a, const dns_params& b) {
 struct Pending {
     hipEvent_t a, b;
     std::vector<int> jobs;     // running jobs of that step
+    // health check copied out with the step: (job id, index into the server's pinned
+    // FlowState buffer), read when the step is harvested
+    std::vector<std::pair<int, int>> health;
 };
 
 }  // namespace
@@ -78,6 +82,11 @@ struct dns_server {
     int max_running = 0;
     std::vector<double> all_lat;
     std::string err;
+    // pinned host copies of running jobs' flow states for the health checks (two halves, so a
+    // check's copies can be in flight while the previous check's are read)
+    FlowState* hflow = nullptr;
+    int hflow_cap = 0, hflow_half = 0;
+    std::vector<int> evict;    // jobs found non-finite, evicted at the next membership change
 };
 
 static int sfail(dns_server* s, int code, const std::string& m) {
@@ -101,6 +110,12 @@ static int harvest_one(dns_server* s) {
     Pending p = s->inflight.front();
     s->inflight.pop_front();
     if (hipEventSynchronize(p.b) != hipSuccess) return sfail(s, DNS_EDEVICE, "hipEventSynchronize failed");
+    // the bulk integral is formed every step from mode (0,0) of u: a non-finite value flags a
+    // job whose fields have gone non-finite
+    for (const auto& h : p.health) {
+        const FlowState& fs = s->hflow[h.second];
+        if (!std::isfinite(fs.Ub) || !std::isfinite(fs.G)) s->evict.push_back(h.first);
+    }
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) return sfail(s, DNS_EDEVICE, "hipEventElapsedTime failed");
     s->device_ms += ms;
@@ -150,18 +165,43 @@ static int retire(dns_server* s, Job& j, int state, const std::string& why) {
     return 0;
 }
 
-static int check_health(dns_server* s) {
-    for (auto& kv : s->jobs) {
-        Job& j = kv.second;
-        if (j.state != DNS_JOB_RUNNING) continue;
-        dns_ctx* c = s->ens.groups[j.group].c;
-        FlowState fs;
-        if (hipMemcpy(&fs, c->flow + j.slot, sizeof(fs), hipMemcpyDeviceToHost) != hipSuccess)
-            return sfail(s, DNS_EDEVICE, "flow-state copy failed");
-        if (!std::isfinite(fs.Ub) || !std::isfinite(fs.G)) {
-            int r = retire(s, j, DNS_JOB_FAILED, "non-finite fields (bulk velocity)");
-            if (r) return r;
+// evict the jobs that the asynchronous health checks found non-finite (stream idle)
+static int evict_flagged(dns_server* s) {
+    for (int id : s->evict) {
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end() || it->second.state != DNS_JOB_RUNNING) continue;
+        int r = retire(s, it->second, DNS_JOB_FAILED, "non-finite fields (bulk velocity)");
+        if (r) return r;
+    }
+    s->evict.clear();
+    return 0;
+}
+
+// queue copies of the running jobs' flow states after the step just enqueued (read when the
+// step is harvested: no wait here)
+static int queue_health(dns_server* s, Pending& pd, const std::vector<int>& run) {
+    const int need = int(run.size());
+    if (2 * need > s->hflow_cap) {
+        // grow only with the stream idle (a copy may still target the old buffer)
+        int r = drain(s);
+        if (r) return r;
+        if (s->hflow) (void)hipHostFree(s->hflow);
+        s->hflow = nullptr;
+        s->hflow_cap = 2 * std::max(need, 64);
+        if (hipHostMalloc(reinterpret_cast<void**>(&s->hflow), sizeof(FlowState) * s->hflow_cap) != hipSuccess) {
+            s->hflow_cap = 0;
+            return sfail(s, DNS_ENOMEM, "hipHostMalloc of the health-check buffer failed");
         }
+    }
+    const int base = s->hflow_half * (s->hflow_cap / 2);
+    s->hflow_half ^= 1;
+    for (int k = 0; k < need; k++) {
+        const Job& j = s->jobs[run[k]];
+        dns_ctx* c = s->ens.groups[j.group].c;
+        if (hipMemcpyAsync(s->hflow + base + k, c->flow + j.slot, sizeof(FlowState), hipMemcpyDeviceToHost,
+                           s->ens.stream()) != hipSuccess)
+            return sfail(s, DNS_EDEVICE, "flow-state copy failed");
+        pd.health.emplace_back(run[k], base + k);
     }
     return 0;
 }
@@ -258,6 +298,7 @@ int dns_server_create(size_t hbm_budget_bytes, int slots_per_group, int use_grap
 int dns_server_destroy(dns_server* s) {
     if (!s) return 0;
     (void)drain(s);
+    if (s->hflow) (void)hipHostFree(s->hflow);
     for (auto e : s->free_events) (void)hipEventDestroy(e);
     delete s;
     return 0;
@@ -291,11 +332,11 @@ int dns_server_tick(dns_server* s, int* running) {
     bool finish = false;
     for (auto& kv : s->jobs)
         if (kv.second.state == DNS_JOB_RUNNING && kv.second.launched >= kv.second.nsteps) finish = true;
-    bool health = s->health_every > 0 && s->ticks > 0 && s->ticks % s->health_every == 0;
+    const bool health = s->health_every > 0 && s->ticks > 0 && s->ticks % s->health_every == 0;
     bool maybe_admit = !s->queue.empty() && admissible(s);
-    if (finish || health || maybe_admit) {
+    if (finish || !s->evict.empty() || maybe_admit) {
         if ((r = drain(s))) return r;
-        if (health && (r = check_health(s))) return r;
+        if ((r = evict_flagged(s))) return r;
         for (auto& kv : s->jobs) {
             Job& j = kv.second;
             if (j.state == DNS_JOB_RUNNING && j.done >= j.nsteps && (r = retire(s, j, DNS_JOB_DONE, ""))) return r;
@@ -316,6 +357,7 @@ int dns_server_tick(dns_server* s, int* running) {
     pd.jobs = run;
     if (hipEventRecord(pd.a, s->ens.stream()) != hipSuccess) return sfail(s, DNS_EDEVICE, "hipEventRecord failed");
     if ((r = s->ens.step(1, s->graph))) return sfail(s, r, s->ens.err);
+    if (health && (r = queue_health(s, pd, run))) return r;
     if (hipEventRecord(pd.b, s->ens.stream()) != hipSuccess) return sfail(s, DNS_EDEVICE, "hipEventRecord failed");
     for (int id : run) {
         Job& j = s->jobs[id];
@@ -340,8 +382,8 @@ int dns_server_drain(dns_server* s) {
         for (auto& kv : s->jobs) left = left || kv.second.state == DNS_JOB_RUNNING;
         if (!left) break;
         if (running == 0 && s->queue.empty()) {
-            // every running job has all its steps queued: retire them
-            if ((r = drain(s))) return r;
+            // every running job has all its steps queued: retire them (the non-finite ones failed)
+            if ((r = drain(s)) || (r = evict_flagged(s))) return r;
             for (auto& kv : s->jobs)
                 if (kv.second.state == DNS_JOB_RUNNING && (r = retire(s, kv.second, DNS_JOB_DONE, ""))) return r;
         }

@@ -685,18 +685,27 @@ void initial_planar(const InitSpec& p, PlanarInit& o) {
     o.scaled = 0;
     o.sf = 1.0;
     if (!p.enable_perturbations) return;
+    // the sines and cosines depend on x or on y alone: they are evaluated once per grid line
+    // (the same calls on the same arguments as point by point, so the same values), and the
+    // products are formed in the point-by-point order
+    std::vector<double> sx(nx), cx(nx), sy(ny), cy(ny);
     if (p.kind == 0) {
         // -(ky sin cos wd) = (-(ky sin cos)) wd and kx cos sin wd = (kx cos sin) wd, exactly
         const double kx = 2.0 * kPi * double(p.mode_x) / p.xlen;
         const double ky = 2.0 * kPi * double(p.mode_y) / p.ylen;
+        for (int i = 0; i < nx; i++) {
+            const double x = double(i) * p.xlen / double(nx);
+            sx[i] = std::sin(kx * x); cx[i] = std::cos(kx * x);
+        }
         for (int j = 0; j < ny; j++) {
             const double y = double(j) * p.ylen / double(ny);
-            for (int i = 0; i < nx; i++) {
-                const double x = double(i) * p.xlen / double(nx);
-                o.u2[i + size_t(nx) * j] = -(ky * std::sin(kx * x) * std::cos(ky * y));
-                o.v2[i + size_t(nx) * j] = kx * std::cos(kx * x) * std::sin(ky * y);
-            }
+            sy[j] = std::sin(ky * y); cy[j] = std::cos(ky * y);
         }
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                o.u2[i + size_t(nx) * j] = -(ky * sx[i] * cy[j]);
+                o.v2[i + size_t(nx) * j] = kx * cx[i] * sy[j];
+            }
     } else {
         uint64_t st = p.seed;
         for (int mx = 0; mx <= p.kmax; mx++)
@@ -705,16 +714,19 @@ void initial_planar(const InitSpec& p, PlanarInit& o) {
                 const double ph1 = 2.0 * kPi * u01(st), ph2 = 2.0 * kPi * u01(st);
                 const double a = 1.0 / double(mx * mx + my * my);
                 const double kx = 2.0 * kPi * double(mx) / p.xlen, ky = 2.0 * kPi * double(my) / p.ylen;
+                for (int i = 0; i < nx; i++) {
+                    const double x = double(i) * p.xlen / double(nx);
+                    sx[i] = std::sin(kx * x + ph1); cx[i] = std::cos(kx * x + ph1);
+                }
                 for (int j = 0; j < ny; j++) {
                     const double y = double(j) * p.ylen / double(ny);
-                    const double sy = std::sin(ky * y + ph2), cy = std::cos(ky * y + ph2);
-                    for (int i = 0; i < nx; i++) {
-                        const double x = double(i) * p.xlen / double(nx);
-                        const double sx = std::sin(kx * x + ph1), cx = std::cos(kx * x + ph1);
-                        o.u2[i + size_t(nx) * j] += -(a * ky * sx * cy);
-                        o.v2[i + size_t(nx) * j] += a * kx * cx * sy;
-                    }
+                    sy[j] = std::sin(ky * y + ph2); cy[j] = std::cos(ky * y + ph2);
                 }
+                for (int j = 0; j < ny; j++)
+                    for (int i = 0; i < nx; i++) {
+                        o.u2[i + size_t(nx) * j] += -(a * ky * sx[i] * cy[j]);
+                        o.v2[i + size_t(nx) * j] += a * kx * cx[i] * sy[j];
+                    }
             }
     }
     // max over all points of |(u2 wd, v2 wd)|: every rounded operation is monotone in wd >= 0,
