@@ -77,7 +77,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     constexpr int KR = 4 * KS;               // panel rows per chunk
     constexpr int RS = 4 * NW;               // rows one pass of the block's threads stages
     constexpr int RPT = (KR + RS - 1) / RS;  // rows staged per thread (the last pass may be partial)
-    const double* xrow = fold_xrow(op);
+    const double* xrow = gmem(fold_xrow(op));
     const int xlast = 4 * op.ksp - 1;        // the extra row's stored entries (zero beyond K)
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int kk = lane >> 4, cl = lane & 15;
@@ -89,8 +89,8 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + NW * i;
-        a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
-        a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
+        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64 + lane;
+        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64 + lane;
     }
     // A operands of the current chunk; each is refilled with the next chunk's k-step as soon as
     // its MFMAs are issued (a rolling one-chunk prefetch in KS registers per group)
@@ -203,8 +203,8 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + NW * i;
-        a0p[i] = op.A + (size_t(b) * op.ksp) * 64 + lane;
-        a1p[i] = op.A + (size_t(op.pb + b) * op.ksp) * 64 + lane;
+        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64 + lane;
+        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64 + lane;
 #pragma unroll
         for (int sl = 0; sl < 4; sl++) {
             ar0[i][sl] = a0p[i][sl * 64];
@@ -269,7 +269,7 @@ __device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds)
 // Column t < BCOLS of the extra row applied to an LDS-resident panel (rows 4 ks, stride LDB).
 // The coefficient reads go out eight at a time (the sum keeps its order).
 __device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) {
-    const double* x = fold_xrow(op);
+    const double* x = gmem(fold_xrow(op));
     const int t = threadIdx.x, n = 4 * op.ks;
     double v = 0.0;
     int j = 0;

@@ -11,6 +11,16 @@ struct Factors {
     int f[14];
 };
 
+// A device pointer as a global-memory pointer.  A pointer the compiler cannot trace to a kernel
+// argument (one read from an argument block in memory, as the ragged launches do) is generic to
+// it, and it accesses memory through it with flat instructions; those count against the LDS
+// wait counter too, so every wait for LDS data would also wait for all global reads in flight.
+// The integer round trip starts the pointer afresh in the global address space (the argument
+// blocks hold device allocations only), and the accesses become global loads and stores.
+template <class T> __device__ __forceinline__ T* gmem(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)(__UINTPTR_TYPE__)p;
+}
+
 // complex arithmetic on double2 / float2
 template <class V> struct Scalar;
 template <> struct Scalar<double2> { using type = double; };

@@ -79,12 +79,12 @@ inline int xcd_grid(int n, int nf) { return XCDS * nf * ((n + XCDS - 1) / XCDS);
 template <class V, class S>
 __device__ __forceinline__ V ldc(const void* p, size_t idx2) {
     using U = typename Scalar<V>::type;
-    const typename Vec2<S>::type x = reinterpret_cast<const typename Vec2<S>::type*>(p)[idx2];
+    const typename Vec2<S>::type x = reinterpret_cast<const typename Vec2<S>::type*>(gmem(p))[idx2];
     return mk2(U(x.x), U(x.y));
 }
 template <class D, class V>
 __device__ __forceinline__ void stc(void* p, size_t idx2, V v) {
-    reinterpret_cast<typename Vec2<D>::type*>(p)[idx2] = mk2(D(v.x), D(v.y));
+    reinterpret_cast<typename Vec2<D>::type*>(gmem(p))[idx2] = mk2(D(v.x), D(v.y));
 }
 
 // ---- in-register DFTs: out_q = sum_r v_r exp(S 2 pi i q r / R), unnormalised -------------
@@ -299,8 +299,8 @@ __device__ __forceinline__ void run_direction_pre(const Plan& P, int nb, V* lds,
 // reads of a batch of four per thread are issued before their LDS writes
 template <int NT, class V>
 __device__ __forceinline__ void load_twiddles(const FFTArgs& a, V* twy) {
-    const V* gy = reinterpret_cast<const V*>(a.twy);
-    const V* gx = reinterpret_cast<const V*>(a.twx) - a.ny;
+    const V* gy = reinterpret_cast<const V*>(gmem(a.twy));
+    const V* gx = reinterpret_cast<const V*>(gmem(a.twx)) - a.ny;
     const int ny = a.ny, n = ny + a.nx;
     for (int base = 0; base < n; base += 4 * NT) {
         // branch free: lanes past the end read and write entry n - 1 again (same value)
@@ -337,8 +337,8 @@ template <int NT, int KIND, class V, class S>
 __device__ __forceinline__ void c2r_prologue(const FFTArgs& a, int f, size_t p2, V* twy, const C2RSide<V>& sd) {
     using U = typename Scalar<V>::type;
     const int ny = a.ny, M = a.M, ntw = ny + a.nx;
-    const V* gy = reinterpret_cast<const V*>(a.twy);
-    const V* gx = reinterpret_cast<const V*>(a.twx) - ny;
+    const V* gy = reinterpret_cast<const V*>(gmem(a.twy));
+    const V* gx = reinterpret_cast<const V*>(gmem(a.twx)) - ny;
     const int src0 = KIND == K_PLAIN ? f : 0;
     const int passes = max(max((ntw + 4 * NT - 1) / (4 * NT), (ny + NT - 1) / NT), (M + NT) / NT);
     for (int p = 0; p < passes; p++) {
@@ -363,7 +363,9 @@ __device__ __forceinline__ void c2r_prologue(const FFTArgs& a, int f, size_t p2,
         // wavenumbers only for OMZ / DIV (a plain transform's xw, yw may be null)
         U k0 = U(0), km = U(0), ky = U(0), kyr = U(0), kxi = U(0);
         if constexpr (KIND != K_PLAIN) {
-            k0 = U(a.xw[0]); km = U(a.xw[M]); ky = U(a.yw[y]); kyr = U(a.yw[yr]); kxi = U(a.xw[i]);
+            const double* xw = gmem(a.xw);
+            const double* yw = gmem(a.yw);
+            k0 = U(xw[0]); km = U(xw[M]); ky = U(yw[y]); kyr = U(yw[yr]); kxi = U(xw[i]);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) twy[min(t0 + u * NT, ntw - 1)] = t[u];
@@ -548,8 +550,10 @@ __global__ __launch_bounds__(Cls<C>::NT) void k_c2r_rag(const FFTArgs* __restric
 template <int NT, class V>
 __device__ __forceinline__ void c2r_wavenumbers(const FFTArgs& a, const C2RSide<V>& sd) {
     using U = typename Scalar<V>::type;
-    for (int i = threadIdx.x; i < a.nxh; i += NT) sd.kx[i] = U(a.xw[i]);
-    for (int y = threadIdx.x; y < a.ny; y += NT) sd.ky[y] = U(a.yw[y]);
+    const double* xw = gmem(a.xw);
+    const double* yw = gmem(a.yw);
+    for (int i = threadIdx.x; i < a.nxh; i += NT) sd.kx[i] = U(xw[i]);
+    for (int y = threadIdx.x; y < a.ny; y += NT) sd.ky[y] = U(yw[y]);
 }
 
 // column 0 of the combined plane, herm(X_0) + i herm(X_M), into the side buffer (the item part

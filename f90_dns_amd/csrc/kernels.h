@@ -253,7 +253,7 @@ __device__ __forceinline__ ZCol zcol_t(const ZMap& z, int col, bool ident) {
         const int q = col >> 1, j = q / z.nxh;
         return ZCol{col, -1, q - j * z.nxh, j};
     }
-    const int4 t = z.tab[col];
+    const int4 t = gmem(z.tab)[col];
     return ZCol{t.x, t.y, t.z, t.w};
 }
 

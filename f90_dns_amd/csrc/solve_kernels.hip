@@ -56,7 +56,7 @@ __device__ __forceinline__ void zstore_q(double* out, size_t q, int2 z, double v
     zstore(out, q - z.x, z, v, sec);
 }
 // the flags of column block cb (ZB_SECOND, ZB_PAIRED), as a wave-uniform value
-__device__ __forceinline__ int zflags(const ZMap& z, int cb) { return __builtin_amdgcn_readfirstlane(z.bfl[cb]); }
+__device__ __forceinline__ int zflags(const ZMap& z, int cb) { return __builtin_amdgcn_readfirstlane(gmem(z.bfl)[cb]); }
 
 // -------------------------------------------------------------------------------------
 // z derivative: out = D1 in  (compute_derivatives_3d dz, DNS_pressure_BC_3D.f90:2348-2358),
@@ -94,10 +94,10 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     // i k w terms take the effective wavenumbers: they differ from the raw ones only where the
     // term is anti-Hermitian, which the c2r that reads omega drops.
     const size_t mo = size_t(mb) * a.mstride;
-    const double* w_in = a.in[2] + mo;
+    const double* w_in = gmem(a.in[2]) + mo;
     {
         const int f = MODE == Z_VORT ? (by == 0 ? 1 : 0) : by;
-        const double* x1 = a.in[f] + mo;
+        const double* x1 = gmem(a.in[f]) + mo;
         const double2* x2 = reinterpret_cast<const double2*>(x1) + smode;
         // branch-free: mirrored rows (clamped in range), the centre row has na == nb
         struct Raw { double2 xa, xb; };
@@ -128,12 +128,12 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             return part == 0 ? (y + kk * wi) : (y - kk * wr);                // omega_y = D1 u - i kx w
         };
         auto kk_of = [&](const ZCol& z) {
-            return f == 1 ? ky_eff(a.yw, z.i, z.j, a.nx, a.ny) : kx_eff(a.xw, z.i, a.nx);
+            return f == 1 ? ky_eff(gmem(a.yw), z.i, z.j, a.nx, a.ny) : kx_eff(gmem(a.xw), z.i, a.nx);
         };
         auto put = [&](size_t ro, int2 z, double v) {
-            if (MODE == Z_PLAIN) zstore(a.out[f] + mo, ro, z, v, sec);
-            else if (a.out_f32) zstore_f(reinterpret_cast<float*>(a.out[o]) + mo, ro, z, v, sec);
-            else zstore(a.out[o] + mo, ro, z, v, sec);
+            if (MODE == Z_PLAIN) zstore(gmem(a.out[f]) + mo, ro, z, v, sec);
+            else if (a.out_f32) zstore_f(reinterpret_cast<float*>(gmem(a.out[o])) + mo, ro, z, v, sec);
+            else zstore(gmem(a.out[o]) + mo, ro, z, v, sec);
         };
         d4 acc0[RBW][CT], acc1[RBW][CT];
         zero_acc2(acc0);
@@ -256,17 +256,18 @@ __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(con
 // unnormalised forward transform is the plane sum.  One wavefront (threads 0-63 of the block).
 __device__ __forceinline__ void flow_tile(const FlowArgs& a, int mb) {
     const int lane = threadIdx.x;
-    const FlowMember P = a.mem[mb];
-    const double* uh = a.uh + size_t(mb) * a.mstride;
+    const FlowMember P = gmem(a.mem)[mb];
+    const double* uh = gmem(a.uh) + size_t(mb) * a.mstride;
+    const double* wz = gmem(a.wz);
     // the bulk integral is also formed under constant forcing: a non-finite value there flags
     // a member whose fields have gone non-finite (any such point reaches mode (0,0) of u
     // through the product's forward transform within one step)
     double part = 0.0;
-    for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * a.wz[k];
+    for (int k = lane; k < a.nz; k += 64) part += uh[size_t(k) * a.H2] * wz[k];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if (lane != 0) return;
-    FlowState s = a.st[mb];
+    FlowState s = gmem(a.st)[mb];
     if (P.method == 1) {
         s.G = P.target_G;
         s.Ub = part / (a.nxny * a.wsum);
@@ -286,7 +287,7 @@ __device__ __forceinline__ void flow_tile(const FlowArgs& a, int mb) {
         s.gcount += 1;
         s.gavg = s.gsum / double(s.gcount);
     }
-    a.st[mb] = s;
+    gmem(a.st)[mb] = s;
 }
 
 // mode (0,0): row/column 1 of the Poisson matrix replaced by the identity, rhs_1 = 0
@@ -302,11 +303,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ void mode0_tile(const Mode0Args& a, int k, int mb) {
     const int nz = a.nz, lane = threadIdx.x;
     const size_t mo = size_t(mb) * a.mstride;
-    const double* ws = a.ws + mo;
+    const double* ws = gmem(a.ws) + mo;
+    const double* e0 = gmem(a.e0);
+    const double* f0 = gmem(a.f0);
     double s0 = 0.0, s1 = 0.0, d0 = 0.0, d1 = 0.0;
     for (int q = lane; q < nz; q += 64) {
         const double w0 = ws[size_t(q) * a.H2], w1 = ws[size_t(q) * a.H2 + 1];
-        const double e = a.e0[size_t(k) * nz + q], f = a.f0[size_t(k) * nz + q];
+        const double e = e0[size_t(k) * nz + q], f = f0[size_t(k) * nz + q];
         s0 += e * w0; s1 += e * w1; d0 += f * w0; d1 += f * w1;
     }
     s0 = wave_sum(s0); s1 = wave_sum(s1); d0 = wave_sum(d0); d1 = wave_sum(d1);
@@ -315,14 +318,14 @@ __device__ __forceinline__ void mode0_tile(const Mode0Args& a, int k, int mb) {
         const double xv = part ? s1 : s0, dx = part ? d1 : d0;
         const size_t o = mo + size_t(k) * a.H2 + part;
         if (!a.fused) {
-            a.phi[o] = xv;
-            a.dphi[o] = dx;
+            gmem(a.phi)[o] = xv;
+            gmem(a.dphi)[o] = dx;
         } else {
             const bool wall = k == 0 || k == nz - 1;
-            a.u[o] = wall ? 0.0 : a.us[o];
-            a.v[o] = wall ? 0.0 : a.vs[o];
-            a.w[o] = wall ? 0.0 : a.ws[o] - a.dt * dx;
-            a.p[o] = a.p[o] + xv;
+            gmem(a.u)[o] = wall ? 0.0 : gmem(a.us)[o];
+            gmem(a.v)[o] = wall ? 0.0 : gmem(a.vs)[o];
+            gmem(a.w)[o] = wall ? 0.0 : ws[size_t(k) * a.H2 + part] - a.dt * dx;
+            gmem(a.p)[o] = gmem(a.p)[o] + xv;
         }
     }
 }
@@ -347,14 +350,16 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const int hI = m / 2, L1 = hI + (m & 1), hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
     const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
-    const double* uh = a.uh[comp] + mo;
-    const double* ph = a.ph + mo;
+    const double* uh = gmem(a.uh[comp]) + mo;
+    const double* ph = gmem(a.ph) + mo;
+    const double* xw = gmem(a.xw);
+    const double* yw = gmem(a.yw);
     FoldOp F = a.hF;
     F.A += size_t(mb) * a.fstride;
-    const double* c1v = a.c1 + size_t(mb) * m;
-    const double* c2v = a.c2 + size_t(mb) * m;
-    const double* hlv = a.hl + size_t(mb) * m;
-    const double fco = a.fvec[mb];
+    const double* c1v = gmem(a.c1) + size_t(mb) * m;
+    const double* c2v = gmem(a.c2) + size_t(mb) * m;
+    const double* hlv = gmem(a.hl) + size_t(mb) * m;
+    const double fco = gmem(a.fvec)[mb];
     // the forcing enters mode (0,0) of u only, in the column block holding that mode,
     // component 0: that block runs the flow controller first when it is fused
     const bool forced = comp == 0 && cb == 0;   // slot 0 holds mode (0,0)
@@ -364,7 +369,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         if (threadIdx.x < 64) flow_tile(a.fl, mb);
         __syncthreads();
     }
-    const double force = forced ? a.flow[mb].G * a.nxny : 0.0;
+    const double force = forced ? gmem(a.flow)[mb].G * a.nxny : 0.0;
     // staging thread: its slot's constants, once (alpha per part: the parts of a paired slot
     // are different modes)
     const int mt = threadIdx.x & 15, slot = cb * BMODES + mt;
@@ -372,7 +377,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const ZCol zx = zcol_t(a.zm, 2 * min(slot, P - 1), ident), zy = zcol_t(a.zm, 2 * min(slot, P - 1) + 1, ident);
     double ax = 1.0, ay = 1.0, fr = 0.0;
     if (sval) {
-        const double kxx = a.xw[zx.i], kyx = a.yw[zx.j], kxy = a.xw[zy.i], kyy = a.yw[zy.j];
+        const double kxx = xw[zx.i], kyx = yw[zx.j], kxy = xw[zy.i], kyy = yw[zy.j];
         ax = a.cn ? 1.0 - fco * (kxx * kxx + kyx * kyx) : 1.0;
         ay = a.cn ? 1.0 - fco * (kxy * kxy + kyy * kyy) : 1.0;
         fr = zx.so == 0 ? force : 0.0;
@@ -380,8 +385,8 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const int smode = sval ? (zx.so >> 1) : 0;   // storage mode of a plain slot
     const int ox = sval ? zx.so : 0, oy = sval ? zy.so : 0;   // storage columns of the two parts
     const double2* u2 = reinterpret_cast<const double2*>(uh) + smode;
-    const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(a.sh[comp]) + mo) + smode;
-    const double2* sd2 = reinterpret_cast<const double2*>(a.sh[comp] + mo) + smode;
+    const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(gmem(a.sh[comp])) + mo) + smode;
+    const double2* sd2 = reinterpret_cast<const double2*>(gmem(a.sh[comp]) + mo) + smode;
     // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
     // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
     // two nodes coincide
@@ -405,7 +410,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         if constexpr (PS) {
             // two scalar reads per value: a paired slot's parts are the real parts of two modes
             using ST = typename std::conditional<SF32, float, double>::type;
-            const ST* s1 = reinterpret_cast<const ST*>(a.sh[comp]) + mo;
+            const ST* s1 = reinterpret_cast<const ST*>(gmem(a.sh[comp])) + mo;
             auto ldu = [&](int n) { return make_double2(uh[size_t(n) * H2 + ox], uh[size_t(n) * H2 + oy]); };
             auto lsv = [&](int n) -> SV {
                 if (!inv) return SV{};
@@ -442,16 +447,16 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     // per-lane column constants for the two 16-column tiles (reads clamped into range and
     // issued together; a column past the plane gets zeros)
     double k2c[CT], bc1[CT], bc2[CT];
-    const bool zwg = a.flow[mb].zero_wallgrad != 0;   // grad p of the walls still zero (restart)
+    const bool zwg = gmem(a.flow)[mb].zero_wallgrad != 0;   // grad p of the walls still zero (restart)
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = out_col(ct, cb), colc = min(col, NC - 1);
         const ZCol zc = zcol_t(a.zm, colc, ident);
         const int mode = zc.so >> 1, part = zc.so & 1, j = zc.j, i = zc.i;
-        const double kx = a.xw[i], ky = a.yw[j];
+        const double kx = xw[i], ky = yw[j];
         const double2 p0 = reinterpret_cast<const double2*>(ph)[mode];
         const double2 p1 = reinterpret_cast<const double2*>(ph + size_t(nz - 1) * H2)[mode];
-        const double ke = comp == 0 ? kx_eff(a.xw, i, a.nx) : ky_eff(a.yw, i, j, a.nx, a.ny);
+        const double ke = comp == 0 ? kx_eff(xw, i, a.nx) : ky_eff(yw, i, j, a.nx, a.ny);
         const bool in = col < NC, lift = in && comp < 2 && !zwg;
         k2c[ct] = in ? kx * kx + ky * ky : 0.0;
         bc1[ct] = lift ? (part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x)) : 0.0;   // -dt * (i ke p)
@@ -510,7 +515,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     zero_acc2(acc0);
     zero_acc2(acc1);
     fold_gemm_lds<RBW, NW>(acc0, acc1, a.hB, z0, z1);
-    double* out = a.out[comp] + mo;
+    double* out = gmem(a.out[comp]) + mo;
     if constexpr (LX) {
         // few row blocks: unfold through LDS (over the z panels, once every wave has read them)
         const double xc = (a.hB.extra && threadIdx.x < BCOLS) ? extra_dot(a.hB, z0) : 0.0;
@@ -603,9 +608,10 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     const int H2 = a.H2, nz = a.nz, P = a.zm.P, NC = 2 * P, hF = nz / 2, LE = hF + (nz & 1);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
     const size_t mo = size_t(mb) * a.mstride, rs = size_t(H2) / 2;
-    const double* us = a.us + mo; const double* vs = a.vs + mo; const double* ws = a.ws + mo;
-    double* uo = a.u + mo; double* vo = a.v + mo; double* wo = a.w + mo; double* po = a.p + mo;
-    double* phio = a.phi + mo; double* dphio = a.dphi + mo;
+    const double* us = gmem(a.us) + mo; const double* vs = gmem(a.vs) + mo; const double* ws = gmem(a.ws) + mo;
+    double* uo = gmem(a.u) + mo; double* vo = gmem(a.v) + mo; double* wo = gmem(a.w) + mo; double* po = gmem(a.p) + mo;
+    double* phio = gmem(a.phi) + mo; double* dphio = gmem(a.dphi) + mo;
+    const double* xw = gmem(a.xw); const double* yw = gmem(a.yw); const double* pl = gmem(a.pl);
     // staging thread's slot; storage mode (0,0) (mode0_tile solves it) is left out part by part
     // (in a paired slot it is the first part only).  The weak form's horizontal term
     // -(i xw u* + i yw v*) is taken on the effective wavenumbers: they differ from the raw ones
@@ -620,8 +626,8 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     double skx = 0.0, sky = 0.0;
     if (sval) {
         const double sg = a.weak ? -1.0 : 1.0;
-        skx = sg * kx_eff(a.xw, zx.i, a.nx);
-        sky = sg * ky_eff(a.yw, zx.i, zx.j, a.nx, a.ny);
+        skx = sg * kx_eff(xw, zx.i, a.nx);
+        sky = sg * ky_eff(yw, zx.i, zx.j, a.nx, a.ny);
     }
     const int smode = sval ? (zx.so >> 1) : 0;
     const int ox = sval ? zx.so : 0, oy = sval ? zy.so : 0;
@@ -674,10 +680,10 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         live[ct] = col < NC && zc.so >= 2;   // not storage mode (0,0)
         k2c[ct] = 1.0; kxe[ct] = 0.0; kye[ct] = 0.0;
         if (col < NC) {
-            const double kx = a.xw[i], ky = a.yw[j];
+            const double kx = xw[i], ky = yw[j];
             k2c[ct] = kx * kx + ky * ky;
-            kxe[ct] = kx_eff(a.xw, i, a.nx);
-            kye[ct] = ky_eff(a.yw, i, j, a.nx, a.ny);
+            kxe[ct] = kx_eff(xw, i, a.nx);
+            kye[ct] = ky_eff(yw, i, j, a.nx, a.ny);
         }
     }
     d4 acc0[RBW][CT], acc1[RBW][CT];
@@ -693,7 +699,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     zero_panel_tail<NW>(z1, a.pF.rows1, ZR);
     if (a.pF.extra && threadIdx.x < BCOLS) {
         const int row = a.pF.rows0 - 1, ct = threadIdx.x >> 4;
-        if (live[ct]) z0[row * LDB + threadIdx.x] = -xv / (a.dt * (a.pl[row] + k2c[ct]));
+        if (live[ct]) z0[row * LDB + threadIdx.x] = -xv / (a.dt * (pl[row] + k2c[ct]));
     }
     {
         // eigenvalues of the lane's rows, read together (clamped) before the stores
@@ -703,8 +709,8 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = out_row<NW>(ii, r);
-                l0[ii][r] = a.pl[min(row, a.pF.rows0 - 1)];
-                l1[ii][r] = a.pl[a.pF.rows0 + min(row, a.pF.rows1 - 1)];
+                l0[ii][r] = pl[min(row, a.pF.rows0 - 1)];
+                l1[ii][r] = pl[a.pF.rows0 + min(row, a.pF.rows1 - 1)];
             }
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
