@@ -304,13 +304,15 @@ __device__ __forceinline__ void store_groups(const d4 (&acc0)[RBW][CT], const d4
     }
 }
 
-// Unfolded row `node` (of n) at panel column c from LDS groups G0, G1 (h = n/2 odd-group rows).
-__device__ __forceinline__ double unfold_lds(const double* G0, const double* G1, int n, int node, int c) {
+// Row pair i < n/2 + (n & 1) of a folded result in LDS groups G0, G1 (row stride LDB) at
+// column c: (out_i, out_{n-1-i}) = (G0_i + G1_i, G0_i - G1_i); on the centre row of odd n,
+// G0_h for both (one node).  Branch free: G1 is read at a clamped row and dropped there.
+__device__ __forceinline__ double2 unfold_pair(const double* G0, const double* G1, int n, int i, int c) {
     const int h = n / 2;
-    if (node < h) return G0[node * LDB + c] + G1[node * LDB + c];
-    if (node == h && (n & 1)) return G0[h * LDB + c];
-    const int i = n - 1 - node;
-    return G0[i * LDB + c] - G1[i * LDB + c];
+    const double g0 = G0[i * LDB + c];
+    const double g1r = G1[min(i, h - 1) * LDB + c];
+    const double g1 = i < h ? g1r : 0.0;
+    return make_double2(g0 + g1, g0 - g1);
 }
 
 template <int RBW>

@@ -148,34 +148,41 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             store_groups<RBW, NW>(acc0, acc1, a.d1, G0, G1);
             if (a.d1.extra && threadIdx.x < BCOLS) G0[(a.d1.rows0 - 1) * LDB + threadIdx.x] = xs;
             __syncthreads();
-            // four outputs per thread per pass, their w reads issued first (indices clamped into
-            // range); the thread's column is the same for all its elements (NT is a multiple of BCOLS)
-            const int ne = nz * BCOLS;
+            // each thread keeps one column (NT is a multiple of BCOLS) and walks row pairs
+            // i = t / BCOLS, + NT / BCOLS, ...: nodes i and nz-1-i from one pair of LDS reads; the
+            // vorticity's w reads for two row pairs go out first (rows clamped into range)
+            constexpr int NPR = NT / BCOLS;
             const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c;
             const ZCol zc = zc_of(min(col, NC - 1));
             const int2 zt = zst(zc);
             const double kk = MODE == Z_VORT ? kk_of(zc) : 0.0;
-            for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
-                double y[4], wc[4], wo[4];
-                int nd[4];
-                bool ok[4];
+            if (col >= NC) return;
+            for (int ib = threadIdx.x / BCOLS; ib < LE; ib += 2 * NPR) {
+                double wc[2][2], wo[2][2];
+                if (MODE == Z_VORT) {
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const int e = e0 + NT * u, ec = min(e, ne - 1);
-                    const int node = ec / BCOLS;
-                    ok[u] = e < ne && col < NC;
-                    nd[u] = node;
-                    y[u] = unfold_lds(G0, G1, nz, node, c);
-                    if (MODE == Z_VORT) {
-                        wc[u] = w_in[size_t(node) * H2 + zc.so];
-                        wo[u] = w_in[size_t(node) * H2 + (zc.so ^ 1)];
+                    for (int q = 0; q < 2; q++) {
+                        const int i = min(ib + q * NPR, LE - 1);
+#pragma unroll
+                        for (int sd = 0; sd < 2; sd++) {
+                            const size_t o = size_t(sd == 0 ? i : nz - 1 - i) * H2;
+                            wc[q][sd] = w_in[o + zc.so];
+                            wo[q][sd] = w_in[o + (zc.so ^ 1)];   // the other part of w
+                        }
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (!ok[u]) continue;
-                    const size_t ro = size_t(nd[u]) * H2;
-                    put(ro, zt, MODE == Z_PLAIN ? y[u] : vort(y[u], wc[u], wo[u], kk, col & 1));
+                for (int q = 0; q < 2; q++) {
+                    const int i = ib + q * NPR;
+                    if (i >= LE) continue;
+                    const double2 y = unfold_pair(G0, G1, nz, i, c);
+#pragma unroll
+                    for (int sd = 0; sd < 2; sd++) {
+                        if (sd == 1 && i >= hF) continue;   // the centre row of odd nz is one node
+                        const double yv = sd == 0 ? y.x : y.y;
+                        const size_t ro = size_t(sd == 0 ? i : nz - 1 - i) * H2;
+                        put(ro, zt, MODE == Z_PLAIN ? yv : vort(yv, wc[q][sd], wo[q][sd], kk, col & 1));
+                    }
                 }
             }
             return;
@@ -533,14 +540,21 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
             }
         }
         __syncthreads();
-        // the thread's column is the same for all its elements (NT is a multiple of BCOLS)
+        // each thread keeps one column (NT is a multiple of BCOLS) and walks the interior row
+        // pairs i = t / BCOLS, + NT / BCOLS, ...: nodes 1 + i and m - i from one pair of LDS reads;
+        // the first BCOLS threads also store the wall values
         const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c;
-        const int2 zt = zst(zcol_t(a.zm, min(col, NC - 1), ident));
-        for (int e = threadIdx.x; e < nz * BCOLS; e += NT) {
-            const int node = e / BCOLS;
-            if (col >= NC) continue;
-            const double v = node == 0 ? bcs[c] : node == nz - 1 ? bcs[BCOLS + c] : unfold_lds(G0, G1, m, node - 1, c);
-            zstore(out, size_t(node) * H2, zt, v, sec);
+        const int so = zcol_t(a.zm, min(col, NC - 1), ident).so;   // (storage column: ident here)
+        if (col < NC) {
+            if (threadIdx.x < BCOLS) {
+                out[so] = bcs[c];
+                out[size_t(nz - 1) * H2 + so] = bcs[BCOLS + c];
+            }
+            for (int i = threadIdx.x / BCOLS; i < L1; i += NT / BCOLS) {
+                const double2 v = unfold_pair(G0, G1, m, i, c);
+                out[size_t(1 + i) * H2 + so] = v.x;
+                if (i < hI) out[size_t(m - i) * H2 + so] = v.y;
+            }
         }
         return;
     }
@@ -758,69 +772,83 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         store_groups<RBW, NW>(acc0, acc1, a.pB, G0, G1);
         if (a.pB.extra && threadIdx.x < BCOLS) G0[(a.pB.rows0 - 1) * LDB + threadIdx.x] = xph;
         __syncthreads();
-        // four outputs per thread per pass, their reads of u*, v*, p issued first (clamped); the
-        // thread's column is the same for all its elements (NT is a multiple of BCOLS)
-        const int ne = nz * BCOLS;
-        const int2 zt = zst(zcol_t(a.zm, min(cb * BCOLS + int(threadIdx.x % BCOLS), NC - 1), ident));
-        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
-            double uu[4], vv[4], pp[4];
-            size_t qn[4];
-            bool ok[4];
+        // Each thread keeps one column and walks row pairs i = t / BCOLS, + NPR, ...: nodes i and
+        // nz-1-i unfold from one pair of LDS reads (branch free, unfold_pair), the other part of
+        // the mode comes from the neighbouring lane (columns 2q, 2q+1 are lanes t, t^1), and the
+        // reads of u*, v*, p for PP row pairs go out before any of their arithmetic and stores.
+        constexpr int NPR = NT / BCOLS, PP = 2;
+        const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c, part = col & 1;
+        const bool okc = col < NC && cc[2 * BCOLS + c] != 0.0;
+        const double kxc = cc[c], kyc = cc[BCOLS + c];
+        const int so = zcol_t(a.zm, min(col, NC - 1), ident).so;   // (storage column: ident here)
+        const int i0 = threadIdx.x / BCOLS;
+        for (int ib = i0; ib < LE; ib += PP * NPR) {
+            double uu[PP][2], vv[PP][2], pp[PP][2];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = e0 + NT * u, ec = min(e, ne - 1);
-                const int node = ec / BCOLS, c = ec - node * BCOLS, col = cb * BCOLS + c;
-                ok[u] = e < ne && col < NC && cc[2 * BCOLS + c] != 0.0;
-                qn[u] = size_t(node) * H2 + zt.x;
-                if (a.fused) {
-                    uu[u] = us[qn[u]];
-                    vv[u] = vs[qn[u]];
-                    pp[u] = po[qn[u]];
+            for (int q = 0; q < PP; q++) {
+                const int i = min(ib + q * NPR, LE - 1);
+#pragma unroll
+                for (int sd = 0; sd < 2; sd++) {
+                    const size_t o = size_t(sd == 0 ? i : nz - 1 - i) * H2 + so;
+                    if (a.fused) {
+                        uu[q][sd] = us[o];
+                        vv[q][sd] = vs[o];
+                        pp[q][sd] = po[o];
+                    }
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (!ok[u]) continue;
-                const int e = e0 + NT * u, node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-                const double ph = unfold_lds(G0, G1, nz, node, c), other = unfold_lds(G0, G1, nz, node, c ^ 1);
-                const size_t ro = size_t(node) * H2;
-                if (!a.fused) {
-                    zstore(phio, ro, zt, ph, sec);
-                    continue;
+            for (int q = 0; q < PP; q++) {
+                const int i = ib + q * NPR;
+                const double2 ph = unfold_pair(G0, G1, nz, min(i, LE - 1), c);
+                const double2 ot = make_double2(__shfl_xor(ph.x, 1), __shfl_xor(ph.y, 1));
+                if (i >= LE || !okc) continue;
+#pragma unroll
+                for (int sd = 0; sd < 2; sd++) {
+                    if (sd == 1 && i >= hF) continue;   // the centre row of odd nz is one node
+                    const int node = sd == 0 ? i : nz - 1 - i;
+                    const double phv = sd == 0 ? ph.x : ph.y, other = sd == 0 ? ot.x : ot.y;
+                    const size_t ro = size_t(node) * H2 + so;
+                    if (!a.fused) {
+                        phio[ro] = phv;
+                        continue;
+                    }
+                    const double pre = part == 0 ? phv : other, pim = part == 0 ? other : phv;
+                    const bool wall = node == 0 || node == nz - 1;
+                    const double du = part == 0 ? -kxc * pim : kxc * pre;
+                    const double dv = part == 0 ? -kyc * pim : kyc * pre;
+                    uo[ro] = wall ? 0.0 : uu[q][sd] - a.dt * du;
+                    vo[ro] = wall ? 0.0 : vv[q][sd] - a.dt * dv;
+                    po[ro] = pp[q][sd] + phv;
                 }
-                const int part = col & 1;
-                const double pre = part == 0 ? ph : other, pim = part == 0 ? other : ph;
-                const bool wall = node == 0 || node == nz - 1;
-                const double du = part == 0 ? -cc[c] * pim : cc[c] * pre;
-                const double dv = part == 0 ? -cc[BCOLS + c] * pim : cc[BCOLS + c] * pre;
-                zstore(uo, ro, zt, wall ? 0.0 : uu[u] - a.dt * du, sec);
-                zstore(vo, ro, zt, wall ? 0.0 : vv[u] - a.dt * dv, sec);
-                zstore(po, ro, zt, pp[u] + ph, sec);
             }
         }
         __syncthreads();
         store_groups<RBW, NW>(dacc0, dacc1, a.pDB, G0, G1);
         if (a.pDB.extra && threadIdx.x < BCOLS) G0[(a.pDB.rows0 - 1) * LDB + threadIdx.x] = xd;
         __syncthreads();
-        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * NT) {
-            double ww[4];
-            size_t qn[4];
+        for (int ib = i0; ib < LE; ib += PP * NPR) {
+            double ww[PP][2];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = min(e0 + NT * u, ne - 1), node = e / BCOLS;
-                qn[u] = size_t(node) * H2 + zt.x;
-                ww[u] = a.fused ? ws[qn[u]] : 0.0;
+            for (int q = 0; q < PP; q++) {
+                const int i = min(ib + q * NPR, LE - 1);
+#pragma unroll
+                for (int sd = 0; sd < 2; sd++) ww[q][sd] = a.fused ? ws[size_t(sd == 0 ? i : nz - 1 - i) * H2 + so] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = e0 + NT * u;
-                if (e >= ne) continue;
-                const int node = e / BCOLS, c = e - node * BCOLS, col = cb * BCOLS + c;
-                if (col >= NC || cc[2 * BCOLS + c] == 0.0) continue;
-                const double dph = unfold_lds(G0, G1, nz, node, c);
-                const size_t ro = size_t(node) * H2;
-                if (!a.fused) zstore(dphio, ro, zt, dph, sec);
-                else zstore(wo, ro, zt, (node == 0 || node == nz - 1) ? 0.0 : ww[u] - a.dt * dph, sec);
+            for (int q = 0; q < PP; q++) {
+                const int i = ib + q * NPR;
+                if (i >= LE || !okc) continue;
+                const double2 dph = unfold_pair(G0, G1, nz, i, c);
+#pragma unroll
+                for (int sd = 0; sd < 2; sd++) {
+                    if (sd == 1 && i >= hF) continue;
+                    const int node = sd == 0 ? i : nz - 1 - i;
+                    const double d = sd == 0 ? dph.x : dph.y;
+                    const size_t ro = size_t(node) * H2 + so;
+                    if (!a.fused) dphio[ro] = d;
+                    else wo[ro] = (node == 0 || node == nz - 1) ? 0.0 : ww[q][sd] - a.dt * d;
+                }
             }
         }
         return;
