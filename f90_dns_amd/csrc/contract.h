@@ -54,6 +54,12 @@ static_assert(stage_lds(4) == STAGE_LDS, "four-wave staging");
 // waves per block of a contraction kernel variant (z_variant): 1, 2, 3, then 4
 __host__ __device__ constexpr int z_variant_nw(int var) { return var < 3 ? var + 1 : 4; }
 
+// the double at byte offset boff of a uniform base: with the base in scalar registers and a
+// 32-bit offset the load takes the scalar-base form (no per-lane 64-bit address arithmetic)
+__device__ __forceinline__ double ld_at(const double* base, unsigned boff) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + boff);
+}
+
 struct FoldPair {
     double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
 };
@@ -84,14 +90,16 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
     const int nch = (op.ks + KS - 1) / KS;
     const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
+    // the wave's A blocks (uniform bases; element (k-step s, lane) at byte 8 (64 s + lane))
     const double* a0p[RBW];
     const double* a1p[RBW];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + NW * i;
-        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64 + lane;
-        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64 + lane;
+        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64;
+        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64;
     }
+    auto aoff = [&](int s) { return unsigned((s * 64 + lane) * 8); };
     // A operands of the current chunk; each is refilled with the next chunk's k-step as soon as
     // its MFMAs are issued (a rolling one-chunk prefetch in KS registers per group)
     double ar0[RBW][KS], ar1[RBW][KS];
@@ -99,8 +107,8 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     for (int i = 0; i < RBW; i++)
 #pragma unroll
         for (int sl = 0; sl < KS; sl++) {
-            ar0[i][sl] = a0p[i][sl * 64];
-            ar1[i][sl] = a1p[i][sl * 64];
+            ar0[i][sl] = ld_at(a0p[i], aoff(sl));
+            ar1[i][sl] = ld_at(a1p[i], aoff(sl));
         }
     // the extra-row coefficient of a staged row (clamped: rows past K have zero panel entries)
     auto xcoef = [&](int kf) { return op.extra ? xrow[min(kf, xlast)] : 0.0; };
@@ -166,8 +174,8 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
                     acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
                     acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
                 }
-                ar0[i][sl] = a0p[i][(nxa + sl) * 64];
-                ar1[i][sl] = a1p[i][(nxa + sl) * 64];
+                ar0[i][sl] = ld_at(a0p[i], aoff(nxa + sl));
+                ar1[i][sl] = ld_at(a1p[i], aoff(nxa + sl));
             }
         }
 #pragma unroll
@@ -199,16 +207,17 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
     const int nch = (op.ks + 3) / 4;
     const double* a0p[RBW];
     const double* a1p[RBW];
+    auto aoff = [&](int s) { return unsigned((s * 64 + lane) * 8); };
     double ar0[RBW][4], ar1[RBW][4], an0[RBW][4], an1[RBW][4];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + NW * i;
-        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64 + lane;
-        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64 + lane;
+        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64;
+        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64;
 #pragma unroll
         for (int sl = 0; sl < 4; sl++) {
-            ar0[i][sl] = a0p[i][sl * 64];
-            ar1[i][sl] = a1p[i][sl * 64];
+            ar0[i][sl] = ld_at(a0p[i], aoff(sl));
+            ar1[i][sl] = ld_at(a1p[i], aoff(sl));
         }
     }
     for (int ch = 0; ch < nch; ch++) {
@@ -217,8 +226,8 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
             for (int i = 0; i < RBW; i++)
 #pragma unroll
                 for (int sl = 0; sl < 4; sl++) {
-                    an0[i][sl] = a0p[i][((ch + 1) * 4 + sl) * 64];
-                    an1[i][sl] = a1p[i][((ch + 1) * 4 + sl) * 64];
+                    an0[i][sl] = ld_at(a0p[i], aoff((ch + 1) * 4 + sl));
+                    an1[i][sl] = ld_at(a1p[i], aoff((ch + 1) * 4 + sl));
                 }
         }
 #pragma unroll
@@ -266,21 +275,24 @@ __device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds)
     return v;
 }
 
-// Column t < BCOLS of the extra row applied to an LDS-resident panel (rows 4 ks, stride LDB).
-// The coefficient reads go out eight at a time (the sum keeps its order).
+// The dense extra row of a back operator is applied by extra_dot from its coefficients parked
+// in the padding column BCOLS of the LDS panel it acts on (rows below 4 ks; the contractions
+// read only columns below BCOLS).  xrow_fetch reads thread t's coefficient t once per block,
+// early (its latency hides behind the forward contraction); xrow_park stores it into the panel
+// with the panel's other writes.  Needs 4 ks <= the block's threads and the panel's rows.
+__device__ __forceinline__ double xrow_fetch(const FoldOp& op) {
+    const int t = threadIdx.x;
+    return (op.extra && t < 4 * op.ks) ? gmem(fold_xrow(op))[t] : 0.0;
+}
+__device__ __forceinline__ void xrow_park(const FoldOp& op, double c, double* z) {
+    if (op.extra && int(threadIdx.x) < 4 * op.ks) z[threadIdx.x * LDB + BCOLS] = c;
+}
+// Column t < BCOLS of the extra row applied to LDS panel z0 (rows 4 ks, stride LDB), its
+// coefficients parked in column BCOLS (the sum runs in row order)
 __device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) {
-    const double* x = gmem(fold_xrow(op));
     const int t = threadIdx.x, n = 4 * op.ks;
     double v = 0.0;
-    int j = 0;
-    for (; j + 8 <= n; j += 8) {
-        double c[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) c[u] = x[j + u];
-#pragma unroll
-        for (int u = 0; u < 8; u++) v += c[u] * z0[(j + u) * LDB + t];
-    }
-    for (; j < n; j++) v += x[j] * z0[j * LDB + t];
+    for (int j = 0; j < n; j++) v += z0[j * LDB + BCOLS] * z0[j * LDB + t];
     return v;
 }
 

@@ -451,8 +451,16 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         r.p1 = make_double2(keep * pair * (va_r - vb_r), keep * pair * (va_i - vb_i));
         return r;
     };
+    const double xb = xrow_fetch(a.hB);   // the back operator's extra-row coefficient (parked below)
+    d4 acc0[RBW][CT], acc1[RBW][CT];
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    double2 xacc = make_double2(0.0, 0.0);
+    fold_gemm<RBW, false, NW>(acc0, acc1, F, fetch, finish, smem, xacc);
+    const double xv = F.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     // per-lane column constants for the two 16-column tiles (reads clamped into range and
-    // issued together; a column past the plane gets zeros)
+    // issued together; a column past the plane gets zeros), formed after the forward contraction
+    // so that its prefetch registers are free of them
     double k2c[CT], bc1[CT], bc2[CT];
     const bool zwg = gmem(a.flow)[mb].zero_wallgrad != 0;   // grad p of the walls still zero (restart)
 #pragma unroll
@@ -469,15 +477,10 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         bc1[ct] = lift ? (part == 0 ? -a.dt * (-ke * p0.y) : -a.dt * (ke * p0.x)) : 0.0;   // -dt * (i ke p)
         bc2[ct] = lift ? (part == 0 ? -a.dt * (-ke * p1.y) : -a.dt * (ke * p1.x)) : 0.0;
     }
-    d4 acc0[RBW][CT], acc1[RBW][CT];
-    zero_acc2(acc0);
-    zero_acc2(acc1);
-    double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW, false, NW>(acc0, acc1, F, fetch, finish, smem, xacc);
-    const double xv = F.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.hB.ks);
     double* z0 = smem;
     double* z1 = smem + ZR * LDB;
+    xrow_park(a.hB, xb, z0);
     // the back contraction reads whole 16-row chunks: zero the panel rows past each group's rows
     // (every other row of a live column is written below; the barrier before the back
     // contraction orders these writes)
@@ -684,6 +687,15 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         r.p1 = ind ? make_double2(orr, oi) : make_double2(er, ei);
         return r;
     };
+    // the back operators' extra-row coefficients (parked below: pB's beside z0, pDB's beside z1)
+    const double xpb = xrow_fetch(a.pB), xpdb = xrow_fetch(a.pDB);
+    d4 acc0[RBW][CT], acc1[RBW][CT];
+    zero_acc2(acc0);
+    zero_acc2(acc1);
+    double2 xacc = make_double2(0.0, 0.0);
+    fold_gemm<RBW, false, NW>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
+    const double xv = a.pF.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
+    // per-lane column constants (after the forward contraction: its prefetch registers are free of them)
     double k2c[CT], kxe[CT], kye[CT];
     bool live[CT];
 #pragma unroll
@@ -700,15 +712,11 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
             kye[ct] = ky_eff(yw, i, j, a.nx, a.ny);
         }
     }
-    d4 acc0[RBW][CT], acc1[RBW][CT];
-    zero_acc2(acc0);
-    zero_acc2(acc1);
-    double2 xacc = make_double2(0.0, 0.0);
-    fold_gemm<RBW, false, NW>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
-    const double xv = a.pF.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     const int ZR = zpanel_rows(a.pB.ks);
     double* z0 = smem;
     double* z1 = smem + ZR * LDB;
+    xrow_park(a.pB, xpb, z0);
+    xrow_park(a.pDB, xpdb, z1);
     zero_panel_tail<NW>(z0, a.pF.rows0, ZR);
     zero_panel_tail<NW>(z1, a.pF.rows1, ZR);
     if (a.pF.extra && threadIdx.x < BCOLS) {
