@@ -338,22 +338,29 @@ def main():
             "scheme": "4-stage 1/irk convection (runge_kutta_convection, DNS_pressure_BC_2D.f90:931-1014) "
                       "in the 3D fractional step"}
         r.close()
-        m = dns.Simulation(precision=dns.DNS_PREC_MIXED, **kw)
-        m.init_fields()
-        m.step(args.warmup, graph=graph)
-        m.synchronize()
-        t0 = time.perf_counter()
-        m.step(args.variant_steps, graph=graph)
-        m.synchronize()
-        mw = time.perf_counter() - t0
-        ml = m.step_timing()
-        variants["mixed_precision"] = {
-            "value": npts * args.variant_steps / mw, "unit": "grid-point-steps/s", "steps": args.variant_steps,
-            "ms_per_step": mw / args.variant_steps * 1e3, "speedup_vs_fp64": (npts * args.variant_steps / mw) / value,
-            "step_latency_ms": {"p50": ml["p50_ms"], "p99": ml["p99_ms"]},
-            "kernels_ms": m.profile_step(args.profile_steps) if args.profile_steps > 0 else {},
-            "precision": "FP32 transforms, product, D1 output and S storage; FP64 solves"}
-        m.close()
+        mixed = ((dns.DNS_PREC_MIXED, "mixed_precision",
+                  "FP32 transforms, product, D1 output and S storage; FP64 D1 arithmetic and solves"),
+                 (dns.DNS_PREC_MIXED_F32D, "mixed_precision_f32_contraction",
+                  "as mixed_precision, and the vorticity's D1 contraction on FP32 MFMA (v_mfma_f32_16x16x4f32, "
+                  "FP32 accumulation)"))
+        for prec, name, what in mixed:
+            m = dns.Simulation(precision=prec, **kw)
+            m.init_fields()
+            m.step(args.warmup, graph=graph)
+            m.synchronize()
+            t0 = time.perf_counter()
+            m.step(args.variant_steps, graph=graph)
+            m.synchronize()
+            mw = time.perf_counter() - t0
+            ml = m.step_timing()
+            variants[name] = {
+                "value": npts * args.variant_steps / mw, "unit": "grid-point-steps/s", "steps": args.variant_steps,
+                "ms_per_step": mw / args.variant_steps * 1e3,
+                "speedup_vs_fp64": (npts * args.variant_steps / mw) / value,
+                "step_latency_ms": {"p50": ml["p50_ms"], "p99": ml["p99_ms"]},
+                "kernels_ms": m.profile_step(args.profile_steps) if args.profile_steps > 0 else {},
+                "precision": what}
+            m.close()
 
     if args.ensemble_steps > 0:
         for prec, name in ((1, "ensemble64_mixed"), (0, "ensemble64_fp64")):

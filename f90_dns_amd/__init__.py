@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libf90dns.so")
 
 DNS_PREC_FP64 = 0
 DNS_PREC_MIXED = 1
+DNS_PREC_MIXED_F32D = 2   # mixed, and the vorticity's D1 contraction on FP32 MFMA (FP32 accumulation)
 _ERRORS = {1: "EINVAL", 2: "ENOMEM", 3: "EDEVICE", 4: "ENONFINITE", 5: "ESOLVER", 6: "EUNSUPPORTED"}
 
 

@@ -19,6 +19,7 @@ struct dns_ctx {
     double theta, ybar;
     int rb, rbi;
     bool mixed = false;           // FP32 transforms, product and omega/S storage; FP64 solves
+    bool d1f32 = false;           // and the vorticity's D1 contraction on FP32 MFMA (D1sw32)
     hipStream_t stream = nullptr;
     dns::FFTGeom g;
     dns::Operators op;                 // member 0; the Re-independent parts are shared
@@ -27,6 +28,7 @@ struct dns_ctx {
     float *twx32 = nullptr, *twy32 = nullptr;
     double *twx = nullptr, *twy = nullptr, *xw = nullptr, *yw = nullptr, *wz = nullptr, *e0 = nullptr, *f0 = nullptr;
     unsigned char* mask = nullptr;
+    float* D1sw32 = nullptr;
     double *D1sw = nullptr, *hF = nullptr, *hB = nullptr, *hc1 = nullptr, *hc2 = nullptr, *hl = nullptr, *fvec = nullptr;
     double *pF = nullptr, *pB = nullptr, *pDB = nullptr, *pl = nullptr;
     int4* ztab = nullptr;         // mode space of the per-mode solves: per column (zmap_build)

@@ -1,5 +1,6 @@
-// contract.h -- block-level FP64 MFMA contractions of the folded wall-normal operators over a
-// 32-column tile of spectral columns.
+// contract.h -- block-level MFMA contractions (FP64; FP32 for the vorticity's D1 in the
+// FP32-contraction precision) of the folded wall-normal operators over a 32-column tile of
+// spectral columns.
 //
 // A block of NW wavefronts (4; at small nz one per row-block pair) owns 32 real columns (16
 // complex modes).  A folded operator has two
@@ -54,11 +55,37 @@ static_assert(stage_lds(4) == STAGE_LDS, "four-wave staging");
 // waves per block of a contraction kernel variant (z_variant): 1, 2, 3, then 4
 __host__ __device__ constexpr int z_variant_nw(int var) { return var < 3 ? var + 1 : 4; }
 
-// the double at byte offset boff of a uniform base: with the base in scalar registers and a
+// the element at byte offset boff of a uniform base: with the base in scalar registers and a
 // 32-bit offset the load takes the scalar-base form (no per-lane 64-bit address arithmetic)
-__device__ __forceinline__ double ld_at(const double* base, unsigned boff) {
-    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + boff);
+template <class T>
+__device__ __forceinline__ T ld_at(const T* base, unsigned boff) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
 }
+
+// Element type of a contraction.  FP64: v_mfma_f64_16x16x4f64; FP32: v_mfma_f32_16x16x4f32
+// with FP32 accumulation.  Both take A(i, k) at lane i + 16 k and B(k, j) at lane j + 16 k (the
+// stored operator layout); their results differ in layout (lane probes in the GPU tests): item
+// r of lane l holds row l/16 + 4 r (FP64) or row 4 (l/16) + r (FP32) of the 16-row block, at
+// column l % 16.
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <class T> struct Mfma;
+template <> struct Mfma<double> {
+    using acc = d4;
+    static __device__ __forceinline__ acc op(double a, double b, const acc& c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+    static __device__ __forceinline__ const double* A(const FoldOp& o) { return gmem(o.A); }
+};
+template <> struct Mfma<float> {
+    using acc = f4;
+    static __device__ __forceinline__ acc op(float a, float b, const acc& c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+    static __device__ __forceinline__ const float* A(const FoldOp& o) { return gmem(o.Af); }
+};
+template <class T> using acc_t = typename Mfma<T>::acc;
 
 struct FoldPair {
     double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
@@ -75,9 +102,11 @@ struct FoldPair {
 // SKIP: waves without row blocks skip the MFMA chain (operators with fewer row-block pairs
 // than waves, the small-nz variants); otherwise every wave has blocks and the loop body stays
 // one straight-line block.
-template <int RBW, bool SKIP, int NW, class FETCH, class FINISH>
-__device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][CT], const FoldOp& op, FETCH&& fetch,
-                                          FINISH&& finish, double* lds, double2& xacc) {
+// T: the element type of the panels, the A operands and the MFMAs (FP32 stages the folded
+// values rounded once to FP32 and reads the operator's FP32 copy op.Af; the extra row stays FP64).
+template <int RBW, bool SKIP, int NW, class T = double, class FETCH, class FINISH>
+__device__ __forceinline__ void fold_gemm(acc_t<T> (&acc0)[RBW][CT], acc_t<T> (&acc1)[RBW][CT], const FoldOp& op,
+                                          FETCH&& fetch, FINISH&& finish, T* lds, double2& xacc) {
     using Raw = decltype(fetch(0));
     constexpr int KS = fold_chunk_ks(NW);    // k-steps per chunk
     constexpr int KR = 4 * KS;               // panel rows per chunk
@@ -90,19 +119,19 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
     const int nch = (op.ks + KS - 1) / KS;
     const bool active = wave < (op.rb0 > op.rb1 ? op.rb0 : op.rb1);
-    // the wave's A blocks (uniform bases; element (k-step s, lane) at byte 8 (64 s + lane))
-    const double* a0p[RBW];
-    const double* a1p[RBW];
+    // the wave's A blocks (uniform bases; element (k-step s, lane) at element 64 s + lane)
+    const T* a0p[RBW];
+    const T* a1p[RBW];
 #pragma unroll
     for (int i = 0; i < RBW; i++) {
         const int b = wave + NW * i;
-        a0p[i] = gmem(op.A) + (size_t(b) * op.ksp) * 64;
-        a1p[i] = gmem(op.A) + (size_t(op.pb + b) * op.ksp) * 64;
+        a0p[i] = Mfma<T>::A(op) + (size_t(b) * op.ksp) * 64;
+        a1p[i] = Mfma<T>::A(op) + (size_t(op.pb + b) * op.ksp) * 64;
     }
-    auto aoff = [&](int s) { return unsigned((s * 64 + lane) * 8); };
+    auto aoff = [&](int s) { return unsigned((s * 64 + lane) * sizeof(T)); };
     // A operands of the current chunk; each is refilled with the next chunk's k-step as soon as
     // its MFMAs are issued (a rolling one-chunk prefetch in KS registers per group)
-    double ar0[RBW][KS], ar1[RBW][KS];
+    T ar0[RBW][KS], ar1[RBW][KS];
 #pragma unroll
     for (int i = 0; i < RBW; i++)
 #pragma unroll
@@ -112,14 +141,14 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         }
     // the extra-row coefficient of a staged row (clamped: rows past K have zero panel entries)
     auto xcoef = [&](int kf) { return op.extra ? xrow[min(kf, xlast)] : 0.0; };
-    auto stage = [&](double* buf, const FoldPair& v, int row, double c) {
+    auto stage = [&](T* buf, const FoldPair& v, int row, double c) {
         if (RS * RPT > KR && row >= KR) return;   // a partial last pass (wave-uniform)
         xacc.x += c * v.p0.x;
         xacc.y += c * v.p0.y;
-        buf[row * LDB + 2 * mt] = v.p0.x;
-        buf[row * LDB + 2 * mt + 1] = v.p0.y;
-        buf[KR * LDB + row * LDB + 2 * mt] = v.p1.x;
-        buf[KR * LDB + row * LDB + 2 * mt + 1] = v.p1.y;
+        buf[row * LDB + 2 * mt] = T(v.p0.x);
+        buf[row * LDB + 2 * mt + 1] = T(v.p0.y);
+        buf[KR * LDB + row * LDB + 2 * mt] = T(v.p1.x);
+        buf[KR * LDB + row * LDB + 2 * mt + 1] = T(v.p1.y);
     };
     // Panel reads run PD chunks ahead of the MFMAs and the arithmetic of the next chunk's fold
     // runs beside the current chunk's MFMA chain: the loop body is one straight-line block (a
@@ -145,8 +174,8 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ch++) {
-        const double* cur = lds + (ch & 1) * 2 * KR * LDB;
-        double* nb = lds + ((ch + 1) & 1) * 2 * KR * LDB;
+        const T* cur = lds + (ch & 1) * 2 * KR * LDB;
+        T* nb = lds + ((ch + 1) & 1) * 2 * KR * LDB;
         Raw r2[RPT];
         double c2[RPT];
         FoldPair f1[RPT];
@@ -161,7 +190,7 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
         if (!SKIP || active)
 #pragma unroll
         for (int sl = 0; sl < KS; sl++) {
-            double b0[CT], b1[CT];
+            T b0[CT], b1[CT];
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
                 b0[ct] = cur[(sl * 4 + kk) * LDB + ct * 16 + cl];
@@ -171,8 +200,8 @@ __device__ __forceinline__ void fold_gemm(d4 (&acc0)[RBW][CT], d4 (&acc1)[RBW][C
             for (int i = 0; i < RBW; i++) {
 #pragma unroll
                 for (int ct = 0; ct < CT; ct++) {
-                    acc0[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar0[i][sl], b0[ct], acc0[i][ct], 0, 0, 0);
-                    acc1[i][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar1[i][sl], b1[ct], acc1[i][ct], 0, 0, 0);
+                    acc0[i][ct] = Mfma<T>::op(ar0[i][sl], b0[ct], acc0[i][ct]);
+                    acc1[i][ct] = Mfma<T>::op(ar1[i][sl], b1[ct], acc1[i][ct]);
                 }
                 ar0[i][sl] = ld_at(a0p[i], aoff(nxa + sl));
                 ar1[i][sl] = ld_at(a1p[i], aoff(nxa + sl));
@@ -298,9 +327,9 @@ __device__ __forceinline__ double extra_dot(const FoldOp& op, const double* z0) 
 
 // The two row groups of a folded result into LDS panels G0 (rows0 rows) and G1 (rows1 rows),
 // row stride LDB; the extra group-0 row (computed elsewhere) is written by the caller.
-template <int RBW, int NW>
-__device__ __forceinline__ void store_groups(const d4 (&acc0)[RBW][CT], const d4 (&acc1)[RBW][CT], const FoldOp& op,
-                                             double* G0, double* G1) {
+template <int RBW, int NW, class T = double>
+__device__ __forceinline__ void store_groups(const acc_t<T> (&acc0)[RBW][CT], const acc_t<T> (&acc1)[RBW][CT],
+                                             const FoldOp& op, double* G0, double* G1) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, cl = lane & 15;
 #pragma unroll
     for (int ii = 0; ii < RBW; ii++) {
@@ -309,7 +338,7 @@ __device__ __forceinline__ void store_groups(const d4 (&acc0)[RBW][CT], const d4
         for (int ct = 0; ct < CT; ct++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = (wave + NW * ii) * 16 + (lane >> 4) + 4 * r;
+                const int row = (wave + NW * ii) * 16 + Mfma<T>::row(lane, r);
                 if (b < op.rb0 && row < op.rows0) G0[row * LDB + ct * 16 + cl] = acc0[ii][ct][r];
                 if (b < op.rb1 && row < op.rows1) G1[row * LDB + ct * 16 + cl] = acc1[ii][ct][r];
             }
@@ -327,21 +356,21 @@ __device__ __forceinline__ double2 unfold_pair(const double* G0, const double* G
     return make_double2(g0 + g1, g0 - g1);
 }
 
-template <int RBW>
-__device__ __forceinline__ void zero_acc2(d4 (&acc)[RBW][CT]) {
+template <int RBW, class A>
+__device__ __forceinline__ void zero_acc2(A (&acc)[RBW][CT]) {
 #pragma unroll
     for (int i = 0; i < RBW; i++)
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) acc[i][ct] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int ct = 0; ct < CT; ct++) acc[i][ct] = A{};
 }
 
 // Coordinates of acc[i][ct][r] in a block of NW waves: row within the group =
-// (wave + NW i)*16 + lane/16 + 4r (the
-// result layout of the instruction), column = cb*BCOLS + ct*16 + lane%16.
-template <int NW>
+// (wave + NW i)*16 + Mfma<T>::row(lane, r) (the result layout of the instruction),
+// column = cb*BCOLS + ct*16 + lane%16.
+template <int NW, class T = double>
 __device__ __forceinline__ int out_row(int i, int r) {
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    return (wave + NW * i) * 16 + (lane >> 4) + 4 * r;
+    return (wave + NW * i) * 16 + Mfma<T>::row(lane, r);
 }
 __device__ __forceinline__ int out_col(int ct, int cb) { return cb * BCOLS + ct * 16 + (threadIdx.x & 15); }
 

@@ -102,8 +102,8 @@ static std::string check_params(const dns_params* p) {
     if (p->nx < 8 || p->ny < 4 || (p->nx & 1) || (p->ny & 1)) return "nx must be even and >= 8, ny even and >= 4";
     if (p->nz < 5 || p->nz > 144) return "nz must be in [5, 144]";
     if (!(p->dt > 0) || !(p->re > 0) || !(p->xlen > 0) || !(p->ylen > 0)) return "dt, re, xlen, ylen must be positive";
-    if (p->precision != DNS_PREC_FP64 && p->precision != DNS_PREC_MIXED)
-        return "precision must be DNS_PREC_FP64 or DNS_PREC_MIXED";
+    if (p->precision != DNS_PREC_FP64 && p->precision != DNS_PREC_MIXED && p->precision != DNS_PREC_MIXED_F32D)
+        return "precision must be DNS_PREC_FP64, DNS_PREC_MIXED or DNS_PREC_MIXED_F32D";
     if (p->flow_control_method != 1 && p->flow_control_method != 2) return "flow_control_method must be 1 or 2";
     if (p->flow_control_method == 2 &&
         (p->target_bulk_velocity <= 0 || p->controller_gain <= 0 || p->controller_update_freq <= 0))
@@ -160,7 +160,8 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
     c->ybar = 2.0;
     c->rb = (c->nz + 15) / 16;
     c->rbi = (c->nz - 2 + 15) / 16;
-    c->mixed = p->precision == DNS_PREC_MIXED;
+    c->mixed = p->precision != DNS_PREC_FP64;
+    c->d1f32 = p->precision == DNS_PREC_MIXED_F32D;
     g_create_error.clear();
     auto bail = [&](int code) {
         if (g_create_error.empty()) g_create_error = c->err;
@@ -229,6 +230,10 @@ int dns_create_batch(const dns_params* params, int nmembers, dns_ctx** out) {
         (r = upload(c, &c->fvec, fv)) || (r = upload(c, &c->pF, o.pF.data)) || (r = upload(c, &c->pB, o.pB.data)) ||
         (r = upload(c, &c->pDB, o.pDB.data)) || (r = upload(c, &c->pl, o.pl)))
         return bail(r);
+    if (c->d1f32) {   // the D1 blocks rounded once to FP32 (the FP32 contraction's A operands)
+        const std::vector<float> d1f(o.zD1.data.begin(), o.zD1.data.end());
+        if ((r = upload(c, &c->D1sw32, d1f))) return bail(r);
+    }
     {
         const ZMap z = zmap_make(c->nx, c->ny, ops_wide(c->op));
         std::vector<int4> zt(size_t(2) * z.P);
@@ -291,7 +296,7 @@ size_t dns_device_bytes(const dns_ctx* c) { return c ? c->bytes : 0; }
 
 static FoldOp fold_op(const FoldedA& f, const double* dev) {
     FoldOp o;
-    o.A = dev; o.rb0 = f.rb0; o.rb1 = f.rb1; o.ks = f.ks; o.ksp = f.ksp; o.rows0 = f.rows0; o.rows1 = f.rows1;
+    o.A = dev; o.Af = nullptr; o.rb0 = f.rb0; o.rb1 = f.rb1; o.ks = f.ks; o.ksp = f.ksp; o.rows0 = f.rows0; o.rows1 = f.rows1;
     o.extra = f.extra;
     o.pb = f.pb;
     return o;
@@ -556,6 +561,8 @@ extern "C++" ZArgs dns::ctx_vort_args(dns_ctx* c) {
     z.out[0] = c->ox; z.out[1] = c->oy;
     z.xw = c->xw; z.yw = c->yw;
     z.out_f32 = c->mixed ? 1 : 0;
+    z.d1.Af = c->D1sw32;
+    z.d1_f32 = c->d1f32 ? 1 : 0;
     z.packed = 1;
     z.zm = c->zm;
     return z;

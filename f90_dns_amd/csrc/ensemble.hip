@@ -38,6 +38,7 @@ bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) 
     b += d(2 * p.ny) + std::max<size_t>(2 * p.ny, 1) * sizeof(float);   // y twiddles
     b += d(p.nx / 2 + 1) + d(p.ny) + std::max<size_t>(nxh * p.ny, 1);  // wavenumbers, mask
     b += d(nz) + d(nz * nz) * 2 + d(o.zD1.data.size());
+    if (p.precision == DNS_PREC_MIXED_F32D) b += std::max<size_t>(o.zD1.data.size(), 1) * sizeof(float);
     b += d(o.hF.data.size() * B) + d(o.hB.data.size()) + 3 * d(m * B) + d(B);
     b += d(o.pF.data.size()) + d(o.pB.data.size()) + d(o.pDB.data.size()) + d(nz);
     const ZMap zm = zmap_make(p.nx, p.ny, ops_wide(o));
@@ -215,7 +216,7 @@ int Ensemble::rebuild() {
                     while (k > base && fft_class_capacity(k, c->mixed) < 0) k--;
                     key = k;
                 } else if (kind == K_ZVORT) {
-                    key = zd_variant(hz[g]); f32 = 0;
+                    key = zd_variant(hz[g]); f32 = hz[g].d1_f32 ? 1 : 0;
                 } else if (kind == K_POIS) {
                     key = pois_variant(hp[g]); f32 = 0;
                 } else if (kind == K_HELM) {
@@ -286,7 +287,7 @@ int Ensemble::launch(const Launch& L, hipStream_t s) {
     const Tile* map = d_tiles + L.off;
     hipError_t e = hipSuccess;
     switch (L.kind) {
-        case K_ZVORT: e = launch_zderiv_rag(L.cls, Z_VORT, d_z, map, L.count, 1, L.lds, s); break;
+        case K_ZVORT: e = launch_zderiv_rag(L.cls, Z_VORT, L.f32 != 0, d_z, map, L.count, 1, L.lds, s); break;
         case K_C2R: e = launch_c2r_rag(L.cls, C2R_NL, L.f32 != 0, d_c2r, map, L.count, L.nf, L.lds, s); break;
         case K_R2C: e = launch_r2c_rag(L.cls, R2C_CROSS, L.f32 != 0, d_r2c, map, L.count, L.nf, L.lds, s); break;
         case K_HELM: e = launch_helmholtz_rag(L.cls, L.f32 != 0, d_h, map, L.count, L.lds, s); break;

@@ -75,12 +75,15 @@ extern "C" {
 int dns_fft_plan_create(int nx, int ny, int precision, dns_fft_plan** out) {
     if (!out) return DNS_EINVAL;
     *out = nullptr;
-    if (precision != DNS_PREC_FP64 && precision != DNS_PREC_MIXED) { dns_set_create_error("bad precision"); return DNS_EINVAL; }
+    if (precision != DNS_PREC_FP64 && precision != DNS_PREC_MIXED && precision != DNS_PREC_MIXED_F32D) {
+        dns_set_create_error("bad precision");
+        return DNS_EINVAL;
+    }
     if (nx < 8 || ny < 4 || (nx & 1) || (ny & 1)) { dns_set_create_error("nx must be even and >= 8, ny even and >= 4"); return DNS_EINVAL; }
     std::string why;
     if (!dns_check_gfx950(why)) { dns_set_create_error(why); return DNS_EDEVICE; }
     dns_fft_plan* p = new dns_fft_plan();
-    p->f32 = precision == DNS_PREC_MIXED;
+    p->f32 = precision != DNS_PREC_FP64;
     if (hipGetDevice(&p->dev) != hipSuccess) { delete p; dns_set_create_error("hipGetDevice failed"); return DNS_EDEVICE; }
     if (int r = scratch_setup(p->dev)) { delete p; return r; }
     if (!fft_geom(nx, ny, p->g) || (!p->f32 && p->g.cls64 < 0 && (nx / 2 > 256 || ny > 256)) ||

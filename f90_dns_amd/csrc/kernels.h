@@ -143,6 +143,7 @@ hipError_t launch_div_spec(const FFTGeom& g, const double* xw, const double* yw,
 // device view of a folded operator (setup.h FoldedA; ksp >= ks + 1 k-steps stored per block)
 struct FoldOp {
     const double* A;
+    const float* Af;             // the same blocks rounded to FP32 (the FP32 contraction), or null
     int rb0, rb1, ks, ksp, rows0, rows1;
     int extra;                   // 1: group-0 row rows0-1 is the dense row after the blocks
     int pb;                      // blocks per group as stored (zero padded); group 1 starts at pb
@@ -265,6 +266,7 @@ struct ZArgs {
     double* out[3];              // float when out_f32
     const double* xw; const double* yw;
     int out_f32;
+    int d1_f32;                  // the vorticity's D1 on FP32 MFMA (d1.Af), FP32 accumulation
     // packed: the columns walked are the per-mode solves' slots (ZMap; the vorticity of the step),
     // else the storage columns
     int packed;
@@ -437,7 +439,7 @@ hipError_t launch_clear_wallgrad(FlowState* st, int nmembers, hipStream_t s);
 
 // ragged variants of the step kernels (var = zd/helm/pois_variant of the groups; lds = dynamic
 // LDS bytes, the largest over the launch's groups)
-hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int ntiles, int nf, size_t lds,
+hipError_t launch_zderiv_rag(int var, int mode, bool f32, const ZArgs* d, const Tile* map, int ntiles, int nf, size_t lds,
                              hipStream_t s);
 hipError_t launch_helmholtz_rag(int var, bool sf32, const HelmArgs* d, const Tile* map, int ntiles, size_t lds,
                                 hipStream_t s);

@@ -64,8 +64,9 @@ __device__ __forceinline__ int zflags(const ZMap& z, int cb) { return __builtin_
 // (DNS_pressure_BC_3D.f90:1077-1079).  Folded: s = D1 x_odd on X_O (rows up to the centre),
 // r = D1 x_even on X_E, out_k = s_k + r_k, out_{nz-1-k} = s_k - r_k.  One block = 16 modes.
 // -------------------------------------------------------------------------------------
-// PS: the block holds a paired slot (its two parts are the real parts of two modes)
-template <int RBW, int MODE, int NW, bool PS>
+// PS: the block holds a paired slot (its two parts are the real parts of two modes).
+// T: the contraction's element type (float: FP32 panels, operator and MFMA, FP32 accumulation).
+template <int RBW, int MODE, int NW, bool PS, class T = double>
 __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int mb, double* lds) {
     constexpr bool LX = NW < 4;   // fewer row-block pairs than four: one wave per pair, LDS unfold
     constexpr int NT = 64 * NW;
@@ -135,17 +136,17 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             else if (a.out_f32) zstore_f(reinterpret_cast<float*>(gmem(a.out[o])) + mo, ro, z, v, sec);
             else zstore(gmem(a.out[o]) + mo, ro, z, v, sec);
         };
-        d4 acc0[RBW][CT], acc1[RBW][CT];
+        acc_t<T> acc0[RBW][CT], acc1[RBW][CT];
         zero_acc2(acc0);
         zero_acc2(acc1);
         double2 xacc = make_double2(0.0, 0.0);
-        fold_gemm<RBW, false, NW>(acc0, acc1, a.d1, fetch, finish, lds, xacc);
+        fold_gemm<RBW, false, NW, T>(acc0, acc1, a.d1, fetch, finish, reinterpret_cast<T*>(lds), xacc);
         const double xs = a.d1.extra ? reduce_extra<NW>(xacc, lds) : 0.0;   // the centre row s_h
         if constexpr (LX) {
             // few row blocks: unfold through LDS so that every thread stores
             double* G0 = lds;
             double* G1 = lds + a.d1.rows0 * LDB;
-            store_groups<RBW, NW>(acc0, acc1, a.d1, G0, G1);
+            store_groups<RBW, NW, T>(acc0, acc1, a.d1, G0, G1);
             if (a.d1.extra && threadIdx.x < BCOLS) G0[(a.d1.rows0 - 1) * LDB + threadIdx.x] = xs;
             __syncthreads();
             // each thread keeps one column (NT is a multiple of BCOLS) and walks row pairs
@@ -213,9 +214,9 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
                 bool ok[4][2];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int row = out_row<NW>(ii, r);
+                    const int row = out_row<NW, T>(ii, r);
                     const double sv = acc0[ii][ct][r];
-                    const double rv = (row < hF && b < a.d1.rb1) ? acc1[ii][ct][r] : 0.0;
+                    const double rv = (row < hF && b < a.d1.rb1) ? double(acc1[ii][ct][r]) : 0.0;
                     yv[r][0] = sv + rv;
                     yv[r][1] = sv - rv;
 #pragma unroll
@@ -242,19 +243,22 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
     }
 }
 
-template <int RBW, int MODE, int NW>
+// F32: the FP32 contraction (the vorticity of the FP32-contraction precision)
+template <int RBW, int MODE, int NW, bool F32 = false>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv(ZArgs a) {
     extern __shared__ double smem[];
-    if (NW == 4 && a.packed && (zflags(a.zm, blockIdx.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, NW == 4>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
-    else zderiv_tile<RBW, MODE, NW, false>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    using T = std::conditional_t<F32, float, double>;
+    if (NW == 4 && a.packed && (zflags(a.zm, blockIdx.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, NW == 4, T>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
+    else zderiv_tile<RBW, MODE, NW, false, T>(a, blockIdx.x, blockIdx.y, blockIdx.z, smem);
 }
-template <int RBW, int MODE, int NW>
+template <int RBW, int MODE, int NW, bool F32 = false>
 __global__ __launch_bounds__(64 * NW, Z_WAVES(RBW, ZW_D1)) void k_zderiv_rag(const ZArgs* __restrict__ d, const Tile* __restrict__ map) {
     extern __shared__ double smem[];
+    using T = std::conditional_t<F32, float, double>;
     const Tile t = map[blockIdx.x];
     const ZArgs& a = d[t.g];
-    if (NW == 4 && a.packed && (zflags(a.zm, t.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, NW == 4>(a, t.x, blockIdx.y, t.z, smem);
-    else zderiv_tile<RBW, MODE, NW, false>(a, t.x, blockIdx.y, t.z, smem);
+    if (NW == 4 && a.packed && (zflags(a.zm, t.x) & ZB_PAIRED)) zderiv_tile<RBW, MODE, NW, NW == 4, T>(a, t.x, blockIdx.y, t.z, smem);
+    else zderiv_tile<RBW, MODE, NW, false, T>(a, t.x, blockIdx.y, t.z, smem);
 }
 
 // apply_flow_control_forcing / update_pressure_gradient_3d / calculate_bulk_velocity_3d
@@ -1284,7 +1288,19 @@ size_t pois_lds_bytes(const PoisArgs& a) {
 
 // variants (z_variant): (pairs per wave, waves per block) = (1, 1), (1, 2), (1, 3), (1, 4), (2, 4)
 #define Z_BLOCK(var) dim3((var) < 3 ? 64 * ((var) + 1) : 256)
+// (with f32, the vorticity only: the FP32 contraction; the plain D1 stays FP64)
 #define ZD_CASES(K, G, ...)                                                                          \
+    if (f32) {                                                                                       \
+        if (mode != Z_VORT) return hipErrorInvalidValue;                                             \
+        switch (var) {                                                                               \
+            case 0: hipLaunchKernelGGL((K<1, Z_VORT, 1, true>), G, Z_BLOCK(var), __VA_ARGS__); break; \
+            case 1: hipLaunchKernelGGL((K<1, Z_VORT, 2, true>), G, Z_BLOCK(var), __VA_ARGS__); break; \
+            case 2: hipLaunchKernelGGL((K<1, Z_VORT, 3, true>), G, Z_BLOCK(var), __VA_ARGS__); break; \
+            case 3: hipLaunchKernelGGL((K<1, Z_VORT, 4, true>), G, Z_BLOCK(var), __VA_ARGS__); break; \
+            case 4: hipLaunchKernelGGL((K<2, Z_VORT, 4, true>), G, Z_BLOCK(var), __VA_ARGS__); break; \
+            default: return hipErrorInvalidValue;                                                    \
+        }                                                                                            \
+    } else                                                                                           \
     switch (var * 2 + (mode == Z_VORT)) {                                                            \
         case 0: hipLaunchKernelGGL((K<1, Z_PLAIN, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;          \
         case 1: hipLaunchKernelGGL((K<1, Z_VORT, 1>), G, Z_BLOCK(var), __VA_ARGS__); break;           \
@@ -1305,6 +1321,7 @@ hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nm, hipStream_t s
     g.z = nm;
     const int var = zd_variant(a);
     const size_t lds = zderiv_lds_bytes(a);
+    const bool f32 = mode == Z_VORT && a.d1_f32 != 0;
     ZD_CASES(k_zderiv, g, lds, s, a)
     return hipGetLastError();
 }
@@ -1396,7 +1413,7 @@ hipError_t launch_clear_wallgrad(FlowState* st, int nm, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_wallgrad, dim3((nm + 63) / 64), dim3(64), 0, s, st, nm);
     return hipGetLastError();
 }
-hipError_t launch_zderiv_rag(int var, int mode, const ZArgs* d, const Tile* map, int nt, int nf, size_t lds,
+hipError_t launch_zderiv_rag(int var, int mode, bool f32, const ZArgs* d, const Tile* map, int nt, int nf, size_t lds,
                              hipStream_t s) {
     if (nt <= 0) return hipSuccess;
     const dim3 g(nt, mode == Z_VORT ? 2 : nf);

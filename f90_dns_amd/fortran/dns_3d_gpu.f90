@@ -40,7 +40,8 @@ program dns_3d_gpu
     real(wp) :: perturbation_amplitude
     integer :: analytical_mode_x, analytical_mode_y
     logical :: use_weak_form_divergence, use_dealias_23_rule
-    ! &gpu (optional): precision 0 = FP64, 1 = mixed; lgl_derivative 0 = reference D1, 1 = exact;
+    ! &gpu (optional): precision 0 = FP64, 1 = mixed, 2 = mixed with the D1 contraction in FP32;
+    ! lgl_derivative 0 = reference D1, 1 = exact;
     ! seed/noise_kmax for the seeded noise used when use_analytical_perturbations is false;
     ! fused_step = .true. launches one fused step instead of the five stage calls; restart_file
     ! names a restart file to resume from.

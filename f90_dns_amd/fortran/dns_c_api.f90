@@ -4,7 +4,7 @@ module dns_c_api
     use iso_c_binding
     implicit none
     private
-    public :: dns_params_t, dns_diag_t, DNS_OK, DNS_PREC_FP64, DNS_PREC_MIXED
+    public :: dns_params_t, dns_diag_t, DNS_OK, DNS_PREC_FP64, DNS_PREC_MIXED, DNS_PREC_MIXED_F32D
     public :: c_dns_create, c_dns_destroy, c_dns_last_error, c_dns_init_fields, c_dns_set_fields, &
               c_dns_get_fields, c_dns_begin_step, c_dns_compute_source_terms, c_dns_viscous_step, &
               c_dns_solve_pressure, c_dns_pressure_correction, c_dns_apply_boundary_conditions, &
@@ -13,7 +13,8 @@ module dns_c_api
               c_dns_fft_plan_create, c_dns_fft_plan_destroy, c_dns_fft_forward, c_dns_fft_backward, &
               dns_error_message
 
-    integer(c_int), parameter :: DNS_OK = 0, DNS_PREC_FP64 = 0, DNS_PREC_MIXED = 1
+    integer(c_int), parameter :: DNS_OK = 0, DNS_PREC_FP64 = 0, DNS_PREC_MIXED = 1, &
+                                      DNS_PREC_MIXED_F32D = 2
 
     type, bind(C) :: dns_params_t
         integer(c_int) :: nx, ny, nz

@@ -37,7 +37,10 @@ enum {
     DNS_EUNSUPPORTED = 6  /* option not implemented on this path */
 };
 
-enum { DNS_PREC_FP64 = 0, DNS_PREC_MIXED = 1 };
+/* DNS_PREC_MIXED: FP32 plane transforms, nonlinear product and S / omega storage, FP64 solves
+ * and D1 arithmetic.  DNS_PREC_MIXED_F32D: the same, and the vorticity's wall-normal D1
+ * contraction on FP32 MFMA (v_mfma_f32_16x16x4f32) with FP32 accumulation. */
+enum { DNS_PREC_FP64 = 0, DNS_PREC_MIXED = 1, DNS_PREC_MIXED_F32D = 2 };
 
 /* Run parameters: the namelist variables of read_input_3d that the 3D step reads
  * (DNS_pressure_BC_3D.f90:559-568), plus the precision of the device path. */
@@ -60,7 +63,7 @@ typedef struct dns_params {
     double perturbation_amplitude;
     unsigned long long seed;            /* seeded noise: splitmix64 seed */
     int noise_kmax;                     /* seeded noise: modes 0 <= mx, my <= noise_kmax */
-    int precision;                      /* DNS_PREC_FP64 or DNS_PREC_MIXED */
+    int precision;                      /* DNS_PREC_FP64, DNS_PREC_MIXED or DNS_PREC_MIXED_F32D */
     int lgl_derivative;                 /* 0: D1 as lgl_module.f90:211-263 (entries whose Lagrange
                                            denominator is <= 1e-15 set to 0: D1 is diagonal for
                                            nz >= 65); 1: exact LGL D1 (not in the reference) */

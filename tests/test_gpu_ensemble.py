@@ -60,7 +60,7 @@ def _check_bitwise(dns, ens, members, steps):
             assert d[k] == dref[k], (i, k, d[k], dref[k])
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_ensemble_member_equals_member_alone(dns, precision):
     mem = _mixed_shapes(precision)
     ens = dns.Ensemble(mem)

@@ -76,17 +76,19 @@ def test_config2_fp64_100_and_500_steps(dns):
     g.close()
 
 
-def test_config2_mixed_500_steps(dns):
-    """The same run in mixed precision: u, v, p within 1e-6 of the FP64 oracle after 500 steps
-    (w on the velocity scale too; w's own error is the known mixed-precision miss)."""
+@pytest.mark.parametrize("precision", [1, 2], ids=["mixed", "mixed_f32d"])
+def test_config2_mixed_500_steps(dns, precision):
+    """The same run in mixed precision (and with the vorticity's D1 contraction on FP32 MFMA):
+    u, v, p within 1e-6 of the FP64 oracle after 500 steps (w on the velocity scale too; w's own
+    error is the known mixed-precision miss)."""
     F = fx.Fixture("config2")
-    g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **F.params)
+    g = dns.Simulation(precision=precision, **F.params)
     g.init_fields()
     done = 0
     for step in F.steps():
         g.step(step - done, graph=True)
         done = step
-        _check(F, g.get_fields(), step, 1e-6, W_SELF_MIXED, "mixed")
+        _check(F, g.get_fields(), step, 1e-6, W_SELF_MIXED, f"precision {precision}")
     g.close()
 
 

@@ -60,6 +60,53 @@ def test_mixed_config1_500_steps_against_fp64_oracle(dns):
     check_parity(e_64, 1e-10, W_SELF_FP64)
 
 
+def _f32_field(sim, which):
+    """A spectral buffer the mixed path stores in FP32 (omega_x, omega_y, S): its first half."""
+    raw = sim.spectral(which)
+    return np.frombuffer(raw.tobytes(), dtype=np.complex64, count=raw.size).reshape(raw.shape).astype(np.complex128)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(CONFIG1),
+    dict(nx=64, ny=32, nz=65, xlen=4 * np.pi, ylen=2 * np.pi, re=300.0, dt=0.002, use_dealias_23_rule=1,
+         flow_control_method=1, target_pressure_gradient=2.0 / 300.0, u_centerline=1.0, enable_perturbations=1,
+         perturbation_kind=1, seed=5, noise_kmax=6, perturbation_amplitude=0.05, lgl_derivative=1),
+    dict(nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_dealias_23_rule=1,
+         flow_control_method=1, target_pressure_gradient=2.0 / 180.0, u_centerline=1.0, enable_perturbations=1,
+         perturbation_kind=1, seed=6, noise_kmax=6, perturbation_amplitude=0.05, lgl_derivative=1)],
+    ids=["config1", "64x32x65", "48x36x129"])
+def test_fp32_contraction_vorticity(dns, kw):
+    """DNS_PREC_MIXED_F32D: the vorticity's D1 contraction on v_mfma_f32_16x16x4f32 with FP32
+    accumulation, against the same stage with FP64 arithmetic (DNS_PREC_MIXED, whose vorticity
+    matches the FP64 oracle to the FP32 output rounding).  The FP32 chain sums D1's large end
+    entries (up to N(N+1)/4) in FP32, so its error relative to the result's norm is ~1e-7..1e-6;
+    it must differ (the FP32 path runs) and stay within 5e-6 per component."""
+    ref = dns.Simulation(precision=dns.DNS_PREC_MIXED, **kw)
+    f32 = dns.Simulation(precision=dns.DNS_PREC_MIXED_F32D, **kw)
+    for s in (ref, f32):
+        s.init_fields()
+        s.compute_source_terms()
+    for which, name in ((10, "omega_x"), (11, "omega_y")):
+        a, b = _f32_field(ref, which), _f32_field(f32, which)
+        err = np.linalg.norm(b - a) / np.linalg.norm(a)
+        _say(f"FP32 contraction {kw['nx']}x{kw['ny']}x{kw['nz']} {name}: rel L2 vs FP64 arithmetic {err:.2e}")
+        assert 0 < err < 5e-6, (name, err)
+
+
+def test_mixed_f32d_config1_500_steps_against_fp64_oracle(dns):
+    """DNS_PREC_MIXED_F32D against the FP64 oracle after 500 config-1 steps: u, v, p and w on
+    the velocity scale within 1e-6 relative L2 (w's own error as for DNS_PREC_MIXED)."""
+    o = orc.Oracle(**CONFIG1)
+    o.init_fields()
+    g = dns.Simulation(precision=dns.DNS_PREC_MIXED_F32D, **CONFIG1)
+    g.set_fields(*o.get_fields())
+    o.step(500)
+    g.step(500, graph=True)
+    e = field_errors(g.get_fields(), o.get_fields())
+    _say(f"mixed + FP32 contraction vs FP64 oracle after 500 steps: {e}")
+    check_parity(e, 1e-6, W_SELF_MIXED)
+
+
 def test_mixed_production_shape_short(dns):
     kw = dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
               use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,

@@ -42,7 +42,7 @@ JOBS = [((32, 32, 33), 40), ((64, 32, 65), 25), ((32, 32, 33), 60), ((96, 64, 33
         ((64, 32, 65), 35), ((32, 24, 129), 20), ((32, 32, 33), 20)]
 
 
-@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("precision", [1, 0, 2])
 def test_result_alone_equals_result_in_the_mix(dns, precision):
     srv = dns.Server(hbm_budget_bytes=8 << 30, slots_per_group=2)
     ids = []
