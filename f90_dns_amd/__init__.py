@@ -160,6 +160,7 @@ def lib():
         L.dns_ensemble_step_timing.argtypes = [_vp, _dp]
         L.dns_ensemble_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
         L.dns_ensemble_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        L.dns_ensemble_set_tile_waves.argtypes = [_vp, _vp]
         L.dns_estimate_bytes.argtypes = [ctypes.POINTER(DnsParams), ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
         L.dns_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
         L.dns_server_destroy.argtypes = [_vp]
@@ -496,6 +497,11 @@ class Ensemble:
         a = (ctypes.c_int * 4)(*(list(promote_fp64) + [0] * 4)[:4])
         b = (ctypes.c_int * 4)(*(list(promote_fp32) + [0] * 4)[:4])
         self._check(lib().dns_ensemble_set_tuning(self._h, lanes, interleave, a, b))
+
+    def set_tile_waves(self, zderiv=0, helmholtz=0, poisson=0):
+        """Least waves per column block of the wall-normal launches (0 = natural, 2 or 4)."""
+        w = (ctypes.c_int * 3)(zderiv, helmholtz, poisson)
+        self._check(lib().dns_ensemble_set_tile_waves(self._h, w))
 
     def autotune(self, steps_per_trial=5):
         import json

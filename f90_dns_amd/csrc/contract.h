@@ -52,6 +52,7 @@ constexpr int STAGE_LDS = 4 * 4 * 4 * LDB;   // doubles of the staging buffers (
 // rows (blocks of fewer waves stage shorter chunks, so they need less LDS and more of them fit a CU)
 __host__ __device__ constexpr int stage_lds(int nw) { return 4 * 4 * fold_chunk_ks(nw) * LDB; }
 static_assert(stage_lds(4) == STAGE_LDS, "four-wave staging");
+static_assert(stage_lds(1) >= 512 && stage_lds(2) >= 512 && stage_lds(3) >= 512, "reduce_extra's 16 x 16 partials");
 // waves per block of a contraction kernel variant (z_variant): 1, 2, 3, then 4
 __host__ __device__ constexpr int z_variant_nw(int var) { return var < 3 ? var + 1 : 4; }
 
@@ -91,6 +92,26 @@ struct FoldPair {
     double2 p0, p1;   // (Re, Im) of the panel-0 and panel-1 entries of one row and mode
 };
 
+// A staging thread's partial sums of the dense extra row times panel 0 (fold_gemm), per mode.
+// Blocks of one, two or four waves stage RS = 4 NW rows per pass, a divisor of 16: the partials
+// are kept per row class kf mod 16 (16 / RS slots per thread, each summed in row order) and
+// reduce_extra adds the 16 classes in order, so the extra row's value does not depend on the
+// waves per block (a column block may run at any of those widths with the same results).  Three
+// waves keep one partial per thread (their rows of a class fall to different threads).  Slot j
+// holds class rt + RS ((base + j) mod N); fold_gemm rotates the slots between chunks so that
+// each staged row adds into a slot known at compile time.
+template <int NW> struct XAcc {
+    static constexpr int RS = 4 * NW;
+    static constexpr bool BYCLASS = 16 % RS == 0;
+    static constexpr int N = BYCLASS ? 16 / RS : 1;
+    double2 s[N];
+    int base;
+    __device__ __forceinline__ XAcc() : base(0) {
+#pragma unroll
+        for (int j = 0; j < N; j++) s[j] = make_double2(0.0, 0.0);
+    }
+};
+
 // acc0[i][ct] += A0[block b] x P0, acc1[i][ct] += A1[block b] x P1 for b = wave + NW i (a
 // block of NW waves).
 // Panel row kf for the staging thread's mode (thread t of the block's NW waves stages rows
@@ -98,7 +119,7 @@ struct FoldPair {
 // free), finish does the arithmetic, so the next chunk's reads are in flight during the
 // current chunk's MFMA chain.  lds: stage_lds(NW) doubles.
 // With op.extra, xacc accumulates the staging thread's share of the dense extra row times
-// panel 0 (reduce_extra sums the shares of a column).
+// panel 0 (XAcc; reduce_extra sums the shares of a column).
 // SKIP: waves without row blocks skip the MFMA chain (operators with fewer row-block pairs
 // than waves, the small-nz variants); otherwise every wave has blocks and the loop body stays
 // one straight-line block.
@@ -106,7 +127,7 @@ struct FoldPair {
 // values rounded once to FP32 and reads the operator's FP32 copy op.Af; the extra row stays FP64).
 template <int RBW, bool SKIP, int NW, class T = double, class FETCH, class FINISH>
 __device__ __forceinline__ void fold_gemm(acc_t<T> (&acc0)[RBW][CT], acc_t<T> (&acc1)[RBW][CT], const FoldOp& op,
-                                          FETCH&& fetch, FINISH&& finish, T* lds, double2& xacc) {
+                                          FETCH&& fetch, FINISH&& finish, T* lds, XAcc<NW>& xacc) {
     using Raw = decltype(fetch(0));
     constexpr int KS = fold_chunk_ks(NW);    // k-steps per chunk
     constexpr int KR = 4 * KS;               // panel rows per chunk
@@ -141,10 +162,25 @@ __device__ __forceinline__ void fold_gemm(acc_t<T> (&acc0)[RBW][CT], acc_t<T> (&
         }
     // the extra-row coefficient of a staged row (clamped: rows past K have zero panel entries)
     auto xcoef = [&](int kf) { return op.extra ? xrow[min(kf, xlast)] : 0.0; };
-    auto stage = [&](T* buf, const FoldPair& v, int row, double c) {
+    // the extra-row partials by row class: row rt + RS rr of a chunk adds into slot rr mod N;
+    // between chunks the slots rotate by the chunk's rows in units of RS
+    using XA = XAcc<NW>;
+    static_assert(!XA::BYCLASS || KR % RS == 0, "chunks of whole staging passes");
+    constexpr int XSH = XA::BYCLASS ? (KR / RS) % XA::N : 0;
+    auto xrotate = [&]() {
+        if constexpr (XSH != 0) {
+            double2 t[XA::N];
+#pragma unroll
+            for (int j = 0; j < XA::N; j++) t[j] = xacc.s[(j + XSH) % XA::N];
+#pragma unroll
+            for (int j = 0; j < XA::N; j++) xacc.s[j] = t[j];
+            xacc.base = (xacc.base + XSH) % XA::N;
+        }
+    };
+    auto stage = [&](T* buf, const FoldPair& v, int row, double c, double2& xs) {
         if (RS * RPT > KR && row >= KR) return;   // a partial last pass (wave-uniform)
-        xacc.x += c * v.p0.x;
-        xacc.y += c * v.p0.y;
+        xs.x += c * v.p0.x;
+        xs.y += c * v.p0.y;
         buf[row * LDB + 2 * mt] = T(v.p0.x);
         buf[row * LDB + 2 * mt + 1] = T(v.p0.y);
         buf[KR * LDB + row * LDB + 2 * mt] = T(v.p1.x);
@@ -170,7 +206,8 @@ __device__ __forceinline__ void fold_gemm(acc_t<T> (&acc0)[RBW][CT], acc_t<T> (&
             }
         }
 #pragma unroll
-        for (int rr = 0; rr < RPT; rr++) stage(lds, finish(rt + RS * rr, r0[rr]), rt + RS * rr, xcoef(rt + RS * rr));
+        for (int rr = 0; rr < RPT; rr++)
+            stage(lds, finish(rt + RS * rr, r0[rr]), rt + RS * rr, xcoef(rt + RS * rr), xacc.s[rr % XA::N]);
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ch++) {
@@ -207,9 +244,10 @@ __device__ __forceinline__ void fold_gemm(acc_t<T> (&acc0)[RBW][CT], acc_t<T> (&
                 ar1[i][sl] = ld_at(a1p[i], aoff(nxa + sl));
             }
         }
+        xrotate();
 #pragma unroll
         for (int rr = 0; rr < RPT; rr++) {
-            stage(nb, f1[rr], rt + RS * rr, qc[0][rr]);
+            stage(nb, f1[rr], rt + RS * rr, qc[0][rr], xacc.s[rr % XA::N]);
 #pragma unroll
             for (int d = 0; d + 1 < PD; d++) {
                 q[d][rr] = q[d + 1][rr];
@@ -287,18 +325,25 @@ __device__ __forceinline__ void fold_gemm_lds(d4 (&acc0)[RBW][CT], d4 (&acc1)[RB
     }
 }
 
-// Column t < BCOLS of the extra row from the staging threads' shares (mode t/2, part t%2) of a
-// block of NW waves; every thread calls it (it synchronises); uses 2 * 64 NW doubles of lds.
+// Column t < BCOLS of the extra row from the staging threads' partials (mode t/2, part t%2) of
+// a block of NW waves, added by row class 0..15 in order (one, two, four waves) or by staging
+// thread (three); every thread calls it (it synchronises); uses 2 * 256 doubles of lds.
 template <int NW>
-__device__ __forceinline__ double reduce_extra(const double2& xacc, double* lds) {
-    const int t = threadIdx.x;
-    lds[2 * t] = xacc.x;
-    lds[2 * t + 1] = xacc.y;
+__device__ __forceinline__ double reduce_extra(const XAcc<NW>& xacc, double* lds) {
+    using XA = XAcc<NW>;
+    const int t = threadIdx.x, rt = t >> 4, mt = t & 15;
+#pragma unroll
+    for (int j = 0; j < XA::N; j++) {
+        const int q = rt + XA::RS * ((xacc.base + j) % XA::N);   // the slot's row class
+        lds[2 * (q * 16 + mt)] = xacc.s[j].x;
+        lds[2 * (q * 16 + mt) + 1] = xacc.s[j].y;
+    }
     __syncthreads();
     double v = 0.0;
     if (t < BCOLS) {
         const int ml = t >> 1, part = t & 1;
-        for (int rt = 0; rt < 4 * NW; rt++) v += lds[2 * (rt * 16 + ml) + part];
+        constexpr int NQ = XA::BYCLASS ? 16 : XA::RS;
+        for (int q = 0; q < NQ; q++) v += lds[2 * (q * 16 + ml) + part];
     }
     __syncthreads();
     return v;

@@ -1,5 +1,6 @@
 // ensemble.h -- ragged ensembles: flows of different grids advanced together, one launch per
-// stage and kernel class for all of them, the step captured in one HIP graph.
+// stage and kernel class for all of them (the tile shapes the autotuner picks decide which
+// groups share a class), the step captured in one HIP graph.
 //
 // Members of equal shape form a group (a batched dns_ctx with a number of slots); a slot is
 // active or idle.  Per stage, the tiles of every active slot of every group that share a kernel
@@ -27,6 +28,10 @@ struct EnsTuning {
     int promote[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};   // [f32][size class]: extra class levels (more threads)
     int lanes = 1;                                       // streams the step is split over (1 or 2)
     int interleave = 0;                                  // 1: z-kernel tiles ordered column-block-major across members
+    // tile shapes of the wall-normal kernels (vorticity z-derivative, Helmholtz, Poisson): the
+    // least waves per column block (z_variant_widen: 0 = one per row-block pair, 2, 4); groups
+    // of different nz at one width share a launch
+    int zwaves[3] = {0, 0, 0};
 };
 
 class Ensemble {

@@ -176,6 +176,9 @@ int Ensemble::rebuild() {
         hh[g].fl = ctx_flow_args(c);
         hh[g].flow_fused = 1;   // the flow controller runs in the Helmholtz launch
         hp[g] = ctx_pois_args(c, 1);   // mode (0,0) is solved in the Poisson launch
+        hz[g].zwaves = tune.zwaves[0];
+        hh[g].zwaves = tune.zwaves[1];
+        hp[g].zwaves = tune.zwaves[2];
     }
     // lane assignment: balance an estimate of the work (points x (log2 plane + nz/8))
     std::vector<double> load(2, 0.0);
@@ -465,6 +468,17 @@ int Ensemble::time_steps(int n, double& ms) {
     return 0;
 }
 
+// the tuning as JSON members (closed with a brace when closed)
+static std::string tuning_json(const EnsTuning& t, bool closed) {
+    std::ostringstream o;
+    o << "{\"lanes\": " << t.lanes << ", \"interleave\": " << t.interleave << ", \"promote_fp64\": ["
+      << t.promote[0][0] << "," << t.promote[0][1] << "," << t.promote[0][2] << "," << t.promote[0][3]
+      << "], \"promote_fp32\": [" << t.promote[1][0] << "," << t.promote[1][1] << "," << t.promote[1][2] << ","
+      << t.promote[1][3] << "], \"tile_waves\": [" << t.zwaves[0] << "," << t.zwaves[1] << "," << t.zwaves[2]
+      << "]" << (closed ? "}" : "");
+    return o.str();
+}
+
 int Ensemble::autotune(int steps, std::string& report) {
     if (steps < 1) steps = 1;
     std::vector<void*> saved;
@@ -484,10 +498,7 @@ int Ensemble::autotune(int steps, std::string& report) {
         dirty = true;
         int rr = time_steps(steps, ms);
         if (rr) return rr;
-        rep << (first ? "" : ", ") << "{\"lanes\": " << t.lanes << ", \"interleave\": " << t.interleave
-            << ", \"promote_fp64\": [" << t.promote[0][0] << "," << t.promote[0][1] << "," << t.promote[0][2] << ","
-            << t.promote[0][3] << "], \"promote_fp32\": [" << t.promote[1][0] << "," << t.promote[1][1] << ","
-            << t.promote[1][2] << "," << t.promote[1][3] << "], \"ms_per_step\": " << ms << "}";
+        rep << (first ? "" : ", ") << tuning_json(t, false) << ", \"ms_per_step\": " << ms << "}";
         first = false;
         return 0;
     };
@@ -526,12 +537,28 @@ int Ensemble::autotune(int steps, std::string& report) {
             }
         }
     }
+    // tile shapes of the wall-normal kernels, one kernel kind at a time: widths that change
+    // some group's variant (a group whose operators have one or two row-block pairs)
+    for (int k = 0; k < 3; k++) {
+        for (int w : {2, 4}) {
+            bool changes = false;
+            for (auto& g : groups) {
+                if (!g.c) continue;
+                const int nat = k == 0 ? zd_variant(ctx_vort_args(g.c)) : k == 1 ? helm_variant(ctx_helm_args(g.c))
+                                                                                : pois_variant(ctx_pois_args(g.c, 1));
+                if (z_variant_widen(nat, w) != z_variant_widen(nat, best.zwaves[k])) changes = true;
+            }
+            if (!changes) continue;
+            EnsTuning t = best;
+            t.zwaves[k] = w;
+            double ms;
+            if ((r = trial(t, ms))) return r;
+            if (ms < best_ms) { best_ms = ms; best = t; }
+        }
+    }
     tune = best;
     dirty = true;
-    rep << "], \"chosen\": {\"lanes\": " << best.lanes << ", \"interleave\": " << best.interleave
-        << ", \"promote_fp64\": [" << best.promote[0][0] << "," << best.promote[0][1] << "," << best.promote[0][2]
-        << "," << best.promote[0][3] << "], \"promote_fp32\": [" << best.promote[1][0] << "," << best.promote[1][1]
-        << "," << best.promote[1][2] << "," << best.promote[1][3] << "]}, \"ms_per_step\": " << best_ms << "}";
+    rep << "], \"chosen\": " << tuning_json(best, true) << ", \"ms_per_step\": " << best_ms << "}";
     report = rep.str();
     if ((r = restore_state(saved))) return r;
     for (size_t g = 0; g < groups.size(); g++)
@@ -699,6 +726,16 @@ int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const in
         e->e.tune.promote[0][c] = (promote_fp64 && c < 4) ? std::max(0, promote_fp64[c]) : 0;
         e->e.tune.promote[1][c] = (promote_fp32 && c < 4) ? std::max(0, promote_fp32[c]) : 0;
     }
+    return e->e.rebuild();
+}
+
+int dns_ensemble_set_tile_waves(dns_ensemble* e, const int* zwaves) {
+    if (!e || !zwaves) return DNS_EINVAL;
+    for (int k = 0; k < 3; k++)
+        if (zwaves[k] != 0 && zwaves[k] != 2 && zwaves[k] != 4) return efail(e, DNS_EINVAL, "tile waves must be 0, 2 or 4");
+    int r = e->e.synchronize();
+    if (r) return r;
+    for (int k = 0; k < 3; k++) e->e.tune.zwaves[k] = zwaves[k];
     return e->e.rebuild();
 }
 

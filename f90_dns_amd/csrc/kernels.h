@@ -267,6 +267,7 @@ struct ZArgs {
     const double* xw; const double* yw;
     int out_f32;
     int d1_f32;                  // the vorticity's D1 on FP32 MFMA (d1.Af), FP32 accumulation
+    int zwaves;                  // least waves per column block (z_variant_widen; 0 = natural)
     // packed: the columns walked are the per-mode solves' slots (ZMap; the vorticity of the step),
     // else the storage columns
     int packed;
@@ -289,9 +290,18 @@ inline int z_variant_of(const FoldOp* const* o, int n) {
         if (o[i]->pb != 4 * ((m + 3) / 4)) return -1;
     return z_variant(m);
 }
+// The variant of a column block given the least waves per block asked for (zwaves, a tile
+// shape an ensemble's tuner picks; 0 = one wave per row-block pair).  One- and two-wave
+// variants widen to two or four waves (one row-block pair per wave, the extra waves idle in the
+// contractions); the results do not change (contract.h XAcc: the extra row is summed by row
+// class whatever the width).  Three waves keep their width.
+inline int z_variant_widen(int var, int zwaves) {
+    if (var < 0 || var > 1 || zwaves <= var + 1) return var;
+    return zwaves >= 4 ? 3 : 1;
+}
 inline int zd_variant(const ZArgs& a) {
     const FoldOp* o[1] = {&a.d1};
-    return z_variant_of(o, 1);
+    return z_variant_widen(z_variant_of(o, 1), a.zwaves);
 }
 hipError_t launch_zderiv(int mode, const ZArgs& a, int nf, int nmembers, hipStream_t s);
 size_t zderiv_lds_bytes(const ZArgs& a);
@@ -322,6 +332,7 @@ struct Mode0Args {
 struct HelmArgs {
     long long mstride, fstride;  // member strides of the fields and of hF.A (per-member Re)
     int nz, m, H2, nxh, ny, nx;
+    int zwaves;                  // least waves per column block (z_variant_widen; 0 = natural)
     FoldOp hF;                   // [T | f T D2_I] folded: even | odd eigen rows (member 0's A)
     FoldOp hB;                   // p | q
     const double* c1; const double* c2; const double* hl;   // m per member, even rows first
@@ -342,7 +353,7 @@ struct HelmArgs {
 hipError_t launch_helmholtz(const HelmArgs& a, int nmembers, hipStream_t s);
 inline int helm_variant(const HelmArgs& a) {
     const FoldOp* o[2] = {&a.hF, &a.hB};
-    return z_variant_of(o, 2);
+    return z_variant_widen(z_variant_of(o, 2), a.zwaves);
 }
 // dynamic LDS bytes of the Helmholtz / Poisson kernels (staging, then the two z panels)
 size_t helm_lds_bytes(const HelmArgs& a);
@@ -350,6 +361,7 @@ size_t helm_lds_bytes(const HelmArgs& a);
 struct PoisArgs {
     long long mstride;
     int nz, H2, nxh, ny, nx;
+    int zwaves;                  // least waves per column block (z_variant_widen; 0 = natural)
     FoldOp pF, pB, pDB;          // folded [Tf | Tf D1], Bf, D1 Bf
     const double* pl;            // even eigenvalues first
     double dt;
@@ -370,7 +382,7 @@ hipError_t launch_poisson(const PoisArgs& a, int nmembers, hipStream_t s);
 hipError_t z_prepare();
 inline int pois_variant(const PoisArgs& a) {
     const FoldOp* o[3] = {&a.pF, &a.pB, &a.pDB};
-    return z_variant_of(o, 3);
+    return z_variant_widen(z_variant_of(o, 3), a.zwaves);
 }
 size_t pois_lds_bytes(const PoisArgs& a);
 

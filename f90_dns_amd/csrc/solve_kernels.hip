@@ -28,6 +28,13 @@ static_assert(KC == 16, "a chunk is 16 panel rows");
 #define ZW_POIS 3
 #endif
 #define Z_WAVES(rbw, w) ((rbw) == 1 ? (w) : 2)
+// row pairs whose global reads a thread of the one- to three-wave epilogues issues together
+#ifndef ZD_PP
+#define ZD_PP 2
+#endif
+#ifndef POIS_PP
+#define POIS_PP 2
+#endif
 
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
 // the x Nyquist column gives an anti-Hermitian column (zero after the round trip), and so
@@ -139,7 +146,7 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
         acc_t<T> acc0[RBW][CT], acc1[RBW][CT];
         zero_acc2(acc0);
         zero_acc2(acc1);
-        double2 xacc = make_double2(0.0, 0.0);
+        XAcc<NW> xacc;
         fold_gemm<RBW, false, NW, T>(acc0, acc1, a.d1, fetch, finish, reinterpret_cast<T*>(lds), xacc);
         const double xs = a.d1.extra ? reduce_extra<NW>(xacc, lds) : 0.0;   // the centre row s_h
         if constexpr (LX) {
@@ -152,17 +159,17 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             // each thread keeps one column (NT is a multiple of BCOLS) and walks row pairs
             // i = t / BCOLS, + NT / BCOLS, ...: nodes i and nz-1-i from one pair of LDS reads; the
             // vorticity's w reads for two row pairs go out first (rows clamped into range)
-            constexpr int NPR = NT / BCOLS;
+            constexpr int NPR = NT / BCOLS, PP = ZD_PP;
             const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c;
             const ZCol zc = zc_of(min(col, NC - 1));
             const int2 zt = zst(zc);
             const double kk = MODE == Z_VORT ? kk_of(zc) : 0.0;
             if (col >= NC) return;
-            for (int ib = threadIdx.x / BCOLS; ib < LE; ib += 2 * NPR) {
-                double wc[2][2], wo[2][2];
+            for (int ib = threadIdx.x / BCOLS; ib < LE; ib += PP * NPR) {
+                double wc[PP][2], wo[PP][2];
                 if (MODE == Z_VORT) {
 #pragma unroll
-                    for (int q = 0; q < 2; q++) {
+                    for (int q = 0; q < PP; q++) {
                         const int i = min(ib + q * NPR, LE - 1);
 #pragma unroll
                         for (int sd = 0; sd < 2; sd++) {
@@ -173,7 +180,7 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < 2; q++) {
+                for (int q = 0; q < PP; q++) {
                     const int i = ib + q * NPR;
                     if (i >= LE) continue;
                     const double2 y = unfold_pair(G0, G1, nz, i, c);
@@ -459,7 +466,7 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     d4 acc0[RBW][CT], acc1[RBW][CT];
     zero_acc2(acc0);
     zero_acc2(acc1);
-    double2 xacc = make_double2(0.0, 0.0);
+    XAcc<NW> xacc;
     fold_gemm<RBW, false, NW>(acc0, acc1, F, fetch, finish, smem, xacc);
     const double xv = F.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     // per-lane column constants for the two 16-column tiles (reads clamped into range and
@@ -696,7 +703,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
     d4 acc0[RBW][CT], acc1[RBW][CT];
     zero_acc2(acc0);
     zero_acc2(acc1);
-    double2 xacc = make_double2(0.0, 0.0);
+    XAcc<NW> xacc;
     fold_gemm<RBW, false, NW>(acc0, acc1, a.pF, fetch, finish, smem, xacc);
     const double xv = a.pF.extra ? reduce_extra<NW>(xacc, smem) : 0.0;
     // per-lane column constants (after the forward contraction: its prefetch registers are free of them)
@@ -788,7 +795,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         // nz-1-i unfold from one pair of LDS reads (branch free, unfold_pair), the other part of
         // the mode comes from the neighbouring lane (columns 2q, 2q+1 are lanes t, t^1), and the
         // reads of u*, v*, p for PP row pairs go out before any of their arithmetic and stores.
-        constexpr int NPR = NT / BCOLS, PP = 2;
+        constexpr int NPR = NT / BCOLS, PP = POIS_PP;
         const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c, part = col & 1;
         const bool okc = col < NC && cc[2 * BCOLS + c] != 0.0;
         const double kxc = cc[c], kyc = cc[BCOLS + c];
@@ -1266,7 +1273,7 @@ __global__ void k_reduce_final(ReduceArgs a) {
 static inline dim3 col_grid(int H2, int y) { return dim3((H2 + BCOLS - 1) / BCOLS, y); }
 
 // Dynamic LDS of the contraction kernels: fold_gemm's staging (stage_lds of the variant's waves,
-// which also holds reduce_extra's 2 * 64 NW doubles), the two z panels of the back contractions,
+// which also holds reduce_extra's 512 doubles), the two z panels of the back contractions,
 // and the LDS unfold of the few-wave variants (G0, G1 over the dead panels, plus per-column values).
 size_t zderiv_lds_bytes(const ZArgs& a) {
     const int var = zd_variant(a);

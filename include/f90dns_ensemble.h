@@ -52,6 +52,13 @@ int dns_ensemble_step_timing(dns_ensemble* e, double* out4);
  * plane: 64, 256, 512, 1024) for the plane kernels of size classes 0..3 (NULL = none). */
 int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const int* promote_fp64,
                             const int* promote_fp32);
+/* Tile shapes of the wall-normal kernels: zwaves[3] = the least waves per column block of the
+ * vorticity z-derivative, Helmholtz and Poisson launches (0 = one wave per row-block pair of the
+ * group's operators, 2 or 4).  Groups whose operators have fewer row-block pairs run at that
+ * width (the extra waves stage and store, and idle in the contractions), so groups of different
+ * nz (e.g. 33 and 65 at two waves) share one launch per stage.  Results do not change with the
+ * width.  No counterpart in the reference (a launch shape of this library). */
+int dns_ensemble_set_tile_waves(dns_ensemble* e, const int* zwaves);
 /* Autotuner: times steps_per_trial graph replays of each candidate tuning (coordinate
  * search), keeps the fastest, restores every member's fields and controller state, and
  * writes a JSON report of the trials (truncated to report_len). */
