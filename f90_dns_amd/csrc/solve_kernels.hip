@@ -30,10 +30,10 @@ static_assert(KC == 16, "a chunk is 16 panel rows");
 #define Z_WAVES(rbw, w) ((rbw) == 1 ? (w) : 2)
 // row pairs whose global reads a thread of the one- to three-wave epilogues issues together
 #ifndef ZD_PP
-#define ZD_PP 2
+#define ZD_PP 1
 #endif
 #ifndef POIS_PP
-#define POIS_PP 2
+#define POIS_PP 1
 #endif
 
 // Wavenumber seen by a derivative whose result is projected by a c2r/r2c round trip:
@@ -158,27 +158,31 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
             __syncthreads();
             // each thread keeps one column (NT is a multiple of BCOLS) and walks row pairs
             // i = t / BCOLS, + NT / BCOLS, ...: nodes i and nz-1-i from one pair of LDS reads; the
-            // vorticity's w reads for two row pairs go out first (rows clamped into range)
+            // vorticity's w reads for PP row pairs go out first (rows clamped into range)
             constexpr int NPR = NT / BCOLS, PP = ZD_PP;
             const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c;
             const ZCol zc = zc_of(min(col, NC - 1));
             const int2 zt = zst(zc);
             const double kk = MODE == Z_VORT ? kk_of(zc) : 0.0;
             if (col >= NC) return;
-            for (int ib = threadIdx.x / BCOLS; ib < LE; ib += PP * NPR) {
-                double wc[PP][2], wo[PP][2];
-                if (MODE == Z_VORT) {
+            // software pipelined like the Poisson epilogue: the w reads of the next PP row pairs
+            // go out before the stores of the current ones
+            struct WRaw { double c[PP][2], o[PP][2]; };
+            auto w_load = [&](int ib) {
+                WRaw r;
 #pragma unroll
-                    for (int q = 0; q < PP; q++) {
-                        const int i = min(ib + q * NPR, LE - 1);
+                for (int q = 0; q < PP; q++) {
+                    const int i = min(ib + q * NPR, LE - 1);
 #pragma unroll
-                        for (int sd = 0; sd < 2; sd++) {
-                            const size_t o = size_t(sd == 0 ? i : nz - 1 - i) * H2;
-                            wc[q][sd] = w_in[o + zc.so];
-                            wo[q][sd] = w_in[o + (zc.so ^ 1)];   // the other part of w
-                        }
+                    for (int sd = 0; sd < 2; sd++) {
+                        const size_t o = size_t(sd == 0 ? i : nz - 1 - i) * H2;
+                        r.c[q][sd] = MODE == Z_VORT ? w_in[o + zc.so] : 0.0;
+                        r.o[q][sd] = MODE == Z_VORT ? w_in[o + (zc.so ^ 1)] : 0.0;   // the other part of w
                     }
                 }
+                return r;
+            };
+            auto y_store = [&](int ib, const WRaw& r) {
 #pragma unroll
                 for (int q = 0; q < PP; q++) {
                     const int i = ib + q * NPR;
@@ -189,9 +193,19 @@ __device__ __forceinline__ void zderiv_tile(const ZArgs& a, int cb, int by, int 
                         if (sd == 1 && i >= hF) continue;   // the centre row of odd nz is one node
                         const double yv = sd == 0 ? y.x : y.y;
                         const size_t ro = size_t(sd == 0 ? i : nz - 1 - i) * H2;
-                        put(ro, zt, MODE == Z_PLAIN ? yv : vort(yv, wc[q][sd], wo[q][sd], kk, col & 1));
+                        put(ro, zt, MODE == Z_PLAIN ? yv : vort(yv, r.c[q][sd], r.o[q][sd], kk, col & 1));
                     }
                 }
+            };
+            constexpr int STEP = PP * NPR;
+            const int i0 = threadIdx.x / BCOLS;
+            WRaw ra = w_load(i0);
+            for (int ib = i0; ib < LE; ib += 2 * STEP) {
+                const WRaw rb = w_load(ib + STEP);   // (rows clamped into range)
+                y_store(ib, ra);
+                if (ib + STEP >= LE) break;
+                ra = w_load(ib + 2 * STEP);
+                y_store(ib + STEP, rb);
             }
             return;
         }
@@ -796,26 +810,35 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
         // the mode comes from the neighbouring lane (columns 2q, 2q+1 are lanes t, t^1), and the
         // reads of u*, v*, p for PP row pairs go out before any of their arithmetic and stores.
         constexpr int NPR = NT / BCOLS, PP = POIS_PP;
+        const bool fused = a.fused != 0;
+        const double dt = a.dt;
         const int c = threadIdx.x % BCOLS, col = cb * BCOLS + c, part = col & 1;
         const bool okc = col < NC && cc[2 * BCOLS + c] != 0.0;
         const double kxc = cc[c], kyc = cc[BCOLS + c];
         const int so = zcol_t(a.zm, min(col, NC - 1), ident).so;   // (storage column: ident here)
         const int i0 = threadIdx.x / BCOLS;
-        for (int ib = i0; ib < LE; ib += PP * NPR) {
-            double uu[PP][2], vv[PP][2], pp[PP][2];
+        // Software pipelined: the reads of the next PP row pairs are issued before the stores of
+        // the current ones (vmcnt counts reads and stores in issue order, so a read issued after
+        // a store would wait for the store too): ragged Poisson at nz = 65 246 -> 227 us.
+        struct EpiRaw { double u[PP][2], v[PP][2], p[PP][2]; };
+        auto epi_load = [&](int ib) {
+            EpiRaw r;
 #pragma unroll
             for (int q = 0; q < PP; q++) {
                 const int i = min(ib + q * NPR, LE - 1);
 #pragma unroll
                 for (int sd = 0; sd < 2; sd++) {
+                    // (read unconditionally, also when only phi is written: a conditional read
+                    // would make the next set's reads wait for this set's)
                     const size_t o = size_t(sd == 0 ? i : nz - 1 - i) * H2 + so;
-                    if (a.fused) {
-                        uu[q][sd] = us[o];
-                        vv[q][sd] = vs[o];
-                        pp[q][sd] = po[o];
-                    }
+                    r.u[q][sd] = us[o];
+                    r.v[q][sd] = vs[o];
+                    r.p[q][sd] = po[o];
                 }
             }
+            return r;
+        };
+        auto epi_store = [&](int ib, const EpiRaw& r) {
 #pragma unroll
             for (int q = 0; q < PP; q++) {
                 const int i = ib + q * NPR;
@@ -828,7 +851,7 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                     const int node = sd == 0 ? i : nz - 1 - i;
                     const double phv = sd == 0 ? ph.x : ph.y, other = sd == 0 ? ot.x : ot.y;
                     const size_t ro = size_t(node) * H2 + so;
-                    if (!a.fused) {
+                    if (!fused) {
                         phio[ro] = phv;
                         continue;
                     }
@@ -836,24 +859,39 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                     const bool wall = node == 0 || node == nz - 1;
                     const double du = part == 0 ? -kxc * pim : kxc * pre;
                     const double dv = part == 0 ? -kyc * pim : kyc * pre;
-                    uo[ro] = wall ? 0.0 : uu[q][sd] - a.dt * du;
-                    vo[ro] = wall ? 0.0 : vv[q][sd] - a.dt * dv;
-                    po[ro] = pp[q][sd] + phv;
+                    uo[ro] = wall ? 0.0 : r.u[q][sd] - dt * du;
+                    vo[ro] = wall ? 0.0 : r.v[q][sd] - dt * dv;
+                    po[ro] = r.p[q][sd] + phv;
                 }
+            }
+        };
+        constexpr int STEP = PP * NPR;
+        {
+            EpiRaw ra = epi_load(i0);
+            for (int ib = i0; ib < LE; ib += 2 * STEP) {
+                const EpiRaw rb = epi_load(ib + STEP);   // (rows clamped into range)
+                epi_store(ib, ra);
+                if (ib + STEP >= LE) break;
+                ra = epi_load(ib + 2 * STEP);
+                epi_store(ib + STEP, rb);
             }
         }
         __syncthreads();
         store_groups<RBW, NW>(dacc0, dacc1, a.pDB, G0, G1);
         if (a.pDB.extra && threadIdx.x < BCOLS) G0[(a.pDB.rows0 - 1) * LDB + threadIdx.x] = xd;
         __syncthreads();
-        for (int ib = i0; ib < LE; ib += PP * NPR) {
-            double ww[PP][2];
+        struct WRaw { double w[PP][2]; };
+        auto w_load = [&](int ib) {
+            WRaw r;
 #pragma unroll
             for (int q = 0; q < PP; q++) {
                 const int i = min(ib + q * NPR, LE - 1);
 #pragma unroll
-                for (int sd = 0; sd < 2; sd++) ww[q][sd] = a.fused ? ws[size_t(sd == 0 ? i : nz - 1 - i) * H2 + so] : 0.0;
+                for (int sd = 0; sd < 2; sd++) r.w[q][sd] = ws[size_t(sd == 0 ? i : nz - 1 - i) * H2 + so];
             }
+            return r;
+        };
+        auto w_store = [&](int ib, const WRaw& r) {
 #pragma unroll
             for (int q = 0; q < PP; q++) {
                 const int i = ib + q * NPR;
@@ -865,9 +903,19 @@ __device__ __forceinline__ void poisson_body(const PoisArgs& a, int cb, int mb, 
                     const int node = sd == 0 ? i : nz - 1 - i;
                     const double d = sd == 0 ? dph.x : dph.y;
                     const size_t ro = size_t(node) * H2 + so;
-                    if (!a.fused) dphio[ro] = d;
-                    else wo[ro] = (node == 0 || node == nz - 1) ? 0.0 : ww[q][sd] - a.dt * d;
+                    if (!fused) dphio[ro] = d;
+                    else wo[ro] = (node == 0 || node == nz - 1) ? 0.0 : r.w[q][sd] - dt * d;
                 }
+            }
+        };
+        {
+            WRaw ra = w_load(i0);
+            for (int ib = i0; ib < LE; ib += 2 * STEP) {
+                const WRaw rb = w_load(ib + STEP);
+                w_store(ib, ra);
+                if (ib + STEP >= LE) break;
+                ra = w_load(ib + 2 * STEP);
+                w_store(ib + STEP, rb);
             }
         }
         return;
