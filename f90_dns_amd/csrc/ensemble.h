@@ -25,7 +25,7 @@ enum StageKind { K_ZVORT = 0, K_C2R, K_R2C, K_HELM, K_POIS, K_COUNT };
 
 // Tunable launch shapes (chosen by Ensemble::autotune or set by hand).
 struct EnsTuning {
-    int promote[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};   // [f32][size class]: extra class levels (more threads)
+    int promote[2][FFT_CLASSES] = {};   // [f32][size class]: levels up in capacity order (more threads)
     int lanes = 1;                                       // streams the step is split over (1 or 2)
     int interleave = 0;                                  // 1: z-kernel tiles ordered column-block-major across members
     // tile shapes of the wall-normal kernels (vorticity z-derivative, Helmholtz, Poisson): the

@@ -215,9 +215,9 @@ int Ensemble::rebuild() {
                 int key = 0, f32 = c->mixed ? 1 : 0;
                 if (kind == K_C2R || kind == K_R2C) {
                     const int base = c->mixed ? c->g.cls32 : c->g.cls64;
-                    int k = base + tune.promote[f32][base];
-                    while (k > base && fft_class_capacity(k, c->mixed) < 0) k--;
-                    key = k;
+                    int lvl = tune.promote[f32][base];
+                    while (lvl > 0 && fft_class_promote(base, lvl, c->mixed) < 0) lvl--;
+                    key = fft_class_promote(base, lvl, c->mixed);
                 } else if (kind == K_ZVORT) {
                     key = zd_variant(hz[g]); f32 = hz[g].d1_f32 ? 1 : 0;
                 } else if (kind == K_POIS) {
@@ -471,11 +471,13 @@ int Ensemble::time_steps(int n, double& ms) {
 // the tuning as JSON members (closed with a brace when closed)
 static std::string tuning_json(const EnsTuning& t, bool closed) {
     std::ostringstream o;
-    o << "{\"lanes\": " << t.lanes << ", \"interleave\": " << t.interleave << ", \"promote_fp64\": ["
-      << t.promote[0][0] << "," << t.promote[0][1] << "," << t.promote[0][2] << "," << t.promote[0][3]
-      << "], \"promote_fp32\": [" << t.promote[1][0] << "," << t.promote[1][1] << "," << t.promote[1][2] << ","
-      << t.promote[1][3] << "], \"tile_waves\": [" << t.zwaves[0] << "," << t.zwaves[1] << "," << t.zwaves[2]
-      << "]" << (closed ? "}" : "");
+    o << "{\"lanes\": " << t.lanes << ", \"interleave\": " << t.interleave;
+    for (int f = 0; f < 2; f++) {
+        o << (f ? ", \"promote_fp32\": [" : ", \"promote_fp64\": [");
+        for (int c = 0; c < FFT_CLASSES; c++) o << (c ? "," : "") << t.promote[f][c];
+        o << "]";
+    }
+    o << ", \"tile_waves\": [" << t.zwaves[0] << "," << t.zwaves[1] << "," << t.zwaves[2] << "]" << (closed ? "}" : "");
     return o.str();
 }
 
@@ -522,13 +524,13 @@ int Ensemble::autotune(int steps, std::string& report) {
         if (ms < best_ms) { best_ms = ms; best = t; }
     }
     for (int f32 = 0; f32 < 2; f32++) {
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < FFT_CLASSES; c++) {
             bool used = false;
             for (auto& g : groups)
                 if (g.c && (g.c->mixed ? 1 : 0) == f32 && (f32 ? g.c->g.cls32 : g.c->g.cls64) == c) used = true;
             if (!used) continue;
             for (int lvl = 1; lvl <= 2; lvl++) {
-                if (fft_class_capacity(c + lvl, f32 != 0) < 0) break;
+                if (fft_class_promote(c, lvl, f32 != 0) < 0) break;
                 EnsTuning t = best;
                 t.promote[f32][c] = lvl;
                 double ms;
@@ -722,7 +724,7 @@ int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const in
     if (r) return r;
     e->e.tune.lanes = lanes;
     e->e.tune.interleave = interleave ? 1 : 0;
-    for (int c = 0; c < 5; c++) {
+    for (int c = 0; c < FFT_CLASSES; c++) {
         e->e.tune.promote[0][c] = (promote_fp64 && c < 4) ? std::max(0, promote_fp64[c]) : 0;
         e->e.tune.promote[1][c] = (promote_fp32 && c < 4) ? std::max(0, promote_fp32[c]) : 0;
     }

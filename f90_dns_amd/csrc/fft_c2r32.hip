@@ -55,6 +55,8 @@ hipError_t c2r_launch32(int cls, int op, const FFTArgs& a, int np, int nf, size_
         case 2: return hot<2>(a, np, nf, lds, s);
         case 3: return hot3(a, np, nf, lds, s);
         case 4: return hot<4>(a, np, nf, lds, s);
+        case 5: return hot<5>(a, np, nf, lds, s);
+        case 6: return hot<6>(a, np, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -66,6 +68,8 @@ hipError_t c2r_launch_rag32(int cls, const FFTArgs* d, const Tile* map, int nt, 
         case 2: return hot_rag<2>(d, map, nt, nf, lds, s);
         case 3: return hot_rag<3>(d, map, nt, nf, lds, s);
         case 4: return hot_rag<4>(d, map, nt, nf, lds, s);
+        case 5: return hot_rag<5>(d, map, nt, nf, lds, s);
+        case 6: return hot_rag<6>(d, map, nt, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -82,6 +86,10 @@ hipError_t c2r_prepare32() {
         reinterpret_cast<const void*>(&k_c2r_rag<3, C2R_NL, float>),
         reinterpret_cast<const void*>(&k_c2r<4, C2R_NL, float>),
         reinterpret_cast<const void*>(&k_c2r_rag<4, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<5, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<5, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r<6, C2R_NL, float>),
+        reinterpret_cast<const void*>(&k_c2r_rag<6, C2R_NL, float>),
         reinterpret_cast<const void*>(&k_c2r<4, C2R_PLAIN, float>),
         reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 8, float>),
         reinterpret_cast<const void*>(&k_c2r_nl_pers<3, 4, float>),

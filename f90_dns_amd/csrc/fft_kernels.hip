@@ -16,11 +16,15 @@ static size_t lds_bytes(int nx, int ny, bool f32) {
     return 2 * t * (size_t(plane_region(nx / 2, ny)) + 2 * size_t(ny) + size_t(nx)) + t * (size_t(nx / 2 + 1) + ny);
 }
 
+// classes in order of capacity (and of threads per block)
+static const int CLS_ORDER[FFT_CLASSES] = {0, 1, 2, 5, 6, 3, 4};
+
+// the smallest class that holds the plane
 static int size_class(int nx, int ny, bool f32) {
     const int H = (nx / 2) * ny;
     if (lds_bytes(nx, ny, f32) > 163840) return -1;
-    for (int c = 0; c < FFT_CLASSES; c++)
-        if (H <= CLS_MAXE[c] && (c < 4 || f32)) return c;
+    for (int c : CLS_ORDER)
+        if (H <= CLS_MAXE[c] && (c != 4 || f32)) return c;
     return -1;
 }
 
@@ -280,6 +284,15 @@ hipError_t fft_prepare_all() {
 int fft_class_capacity(int c, bool f32) {
     if (c < 0 || c >= FFT_CLASSES || (c == 4 && !f32)) return -1;
     return CLS_MAXE[c];
+}
+
+int fft_class_promote(int c, int levels, bool f32) {
+    int i = 0;
+    while (i < FFT_CLASSES && CLS_ORDER[i] != c) i++;
+    if (i == FFT_CLASSES || fft_class_capacity(c, f32) < 0) return -1;
+    i += levels;
+    if (i >= FFT_CLASSES || fft_class_capacity(CLS_ORDER[i], f32) < 0) return -1;
+    return CLS_ORDER[i];
 }
 
 void fft_fill(const FFTGeom& g, FFTArgs& a, bool f32) {

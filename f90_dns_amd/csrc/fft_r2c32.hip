@@ -28,6 +28,8 @@ hipError_t r2c_launch32(int cls, int op, const FFTArgs& a, int np, int nf, size_
         case 2: return hot<2>(a, np, nf, lds, s);
         case 3: return hot<3>(a, np, nf, lds, s);
         case 4: return hot<4>(a, np, nf, lds, s);
+        case 5: return hot<5>(a, np, nf, lds, s);
+        case 6: return hot<6>(a, np, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -39,6 +41,8 @@ hipError_t r2c_launch_rag32(int cls, const FFTArgs* d, const Tile* map, int nt, 
         case 2: return hot_rag<2>(d, map, nt, nf, lds, s);
         case 3: return hot_rag<3>(d, map, nt, nf, lds, s);
         case 4: return hot_rag<4>(d, map, nt, nf, lds, s);
+        case 5: return hot_rag<5>(d, map, nt, nf, lds, s);
+        case 6: return hot_rag<6>(d, map, nt, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -55,6 +59,10 @@ hipError_t r2c_prepare32() {
         reinterpret_cast<const void*>(&k_r2c_rag<3, R2C_CROSS, float>),
         reinterpret_cast<const void*>(&k_r2c<4, R2C_CROSS, float>),
         reinterpret_cast<const void*>(&k_r2c_rag<4, R2C_CROSS, float>),
+        reinterpret_cast<const void*>(&k_r2c<5, R2C_CROSS, float>),
+        reinterpret_cast<const void*>(&k_r2c_rag<5, R2C_CROSS, float>),
+        reinterpret_cast<const void*>(&k_r2c<6, R2C_CROSS, float>),
+        reinterpret_cast<const void*>(&k_r2c_rag<6, R2C_CROSS, float>),
         reinterpret_cast<const void*>(&k_r2c<4, R2C_PLAIN, float>),
     };
     for (const void* f : fns) {

@@ -27,6 +27,8 @@ hipError_t r2c_launch64(int cls, int op, const FFTArgs& a, int np, int nf, size_
         case 1: return hot<1>(a, np, nf, lds, s);
         case 2: return hot<2>(a, np, nf, lds, s);
         case 3: return hot<3>(a, np, nf, lds, s);
+        case 5: return hot<5>(a, np, nf, lds, s);
+        case 6: return hot<6>(a, np, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -37,6 +39,8 @@ hipError_t r2c_launch_rag64(int cls, const FFTArgs* d, const Tile* map, int nt, 
         case 1: return hot_rag<1>(d, map, nt, nf, lds, s);
         case 2: return hot_rag<2>(d, map, nt, nf, lds, s);
         case 3: return hot_rag<3>(d, map, nt, nf, lds, s);
+        case 5: return hot_rag<5>(d, map, nt, nf, lds, s);
+        case 6: return hot_rag<6>(d, map, nt, nf, lds, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -51,6 +55,10 @@ hipError_t r2c_prepare64() {
         reinterpret_cast<const void*>(&k_r2c_rag<2, R2C_CROSS, double>),
         reinterpret_cast<const void*>(&k_r2c<3, R2C_CROSS, double>),
         reinterpret_cast<const void*>(&k_r2c_rag<3, R2C_CROSS, double>),
+        reinterpret_cast<const void*>(&k_r2c<5, R2C_CROSS, double>),
+        reinterpret_cast<const void*>(&k_r2c_rag<5, R2C_CROSS, double>),
+        reinterpret_cast<const void*>(&k_r2c<6, R2C_CROSS, double>),
+        reinterpret_cast<const void*>(&k_r2c_rag<6, R2C_CROSS, double>),
         reinterpret_cast<const void*>(&k_r2c<3, R2C_PLAIN, double>),
     };
     for (const void* f : fns) {

@@ -20,10 +20,13 @@ enum R2COp { R2C_PLAIN = 0, R2C_CROSS = 1 };
 
 // Plane-size classes of the plane kernels: threads per block and the largest plane (nx/2)*ny
 // complex values they take (eight per thread; eighteen in class 4, which fits the LDS only in
-// FP32).  Class 0 is one wavefront.
-constexpr int FFT_CLASSES = 5;
-constexpr int CLS_NT[FFT_CLASSES] = {64, 256, 512, 1024, 1024};
-constexpr int CLS_MAXE[FFT_CLASSES] = {512, 2048, 4096, 8192, 18432};
+// FP32).  Class 0 is one wavefront.  Classes 5 and 6 (nine and twelve waves) take the planes of
+// 4608 and 6144 values (e.g. 96 x 96, 128 x 96), which left a 1024-thread block a third or a
+// quarter idle in every radix-8 stage.  In order of capacity: 0, 1, 2, 5, 6, 3, 4
+// (fft_class_promote).
+constexpr int FFT_CLASSES = 7;
+constexpr int CLS_NT[FFT_CLASSES] = {64, 256, 512, 1024, 1024, 576, 768};
+constexpr int CLS_MAXE[FFT_CLASSES] = {512, 2048, 4096, 8192, 18432, 4608, 6144};
 // complex values a thread may hold in one transform stage (registers)
 constexpr int stage_budget(bool f32) { return f32 ? 36 : 18; }
 // butterflies per thread a radix-R stage of class cls is compiled for
@@ -103,6 +106,9 @@ size_t plane_lds_bytes(const FFTGeom& g, bool f32);
 void fft_fill(const FFTGeom& g, FFTArgs& a, bool f32);
 // max complex elements of the half plane for size class c (-1: no such class for f32)
 int fft_class_capacity(int c, bool f32);
+// the class `levels` steps above class c in order of capacity (more threads per plane; a plane's
+// plan fits every such class), or -1 past the largest class of the precision
+int fft_class_promote(int c, int levels, bool f32);
 // raise the dynamic-LDS limit of the plane kernels of g's size classes (once per process)
 hipError_t fft_prepare(const FFTGeom& g);
 // grid (nplanes, nfields); f32: compute in FP32 (else FP64)

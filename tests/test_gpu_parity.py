@@ -256,10 +256,11 @@ def test_seeded_noise_production_shape_short(dns):
     check_parity(errs, 1e-10, W_SELF_FP64)
 
 
-@pytest.mark.parametrize("nx,ny", [(96, 96), (128, 96), (192, 48)])
+@pytest.mark.parametrize("nx,ny", [(96, 96), (128, 96), (192, 48), (96, 144), (144, 96)])
 def test_largest_class_step_shapes(dns, nx, ny):
-    """FP64 steps on planes of the largest transform class with other first-stage radices (the
-    step's c2r runs one persistent block per CU there): 4 steps against the oracle."""
+    """FP64 steps on planes of the 576- and 768-thread transform classes (4608 and 6144 values:
+    96 x 96, 192 x 48, 128 x 96) and of the largest class with other first-stage radices (6912
+    values; the step's c2r runs one persistent block per CU there): 4 steps against the oracle."""
     _say(f"{nx}x{ny}x17 4 steps")
     kw = dict(CONFIG1, nx=nx, ny=ny, nz=17, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2,
               perturbation_kind=1, seed=7, noise_kmax=6, perturbation_amplitude=0.05)
