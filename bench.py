@@ -244,6 +244,8 @@ def serving_variant(dns, njobs, rate, budget_gib, slots):
            "job_step_latency_ms": {"p50": st["step_p50_ms"], "p99": st["step_p99_ms"]},
            "turnaround_s": {"p50": float(np.percentile(turn, 50)), "p99": float(np.percentile(turn, 99))},
            "hbm_budget_gib": budget_gib, "peak_hbm_gib": st["peak_bytes"] / 2 ** 30, "max_running": st["max_running"],
+           "membership_changes": st["changes"], "change_host_ms": st["change_ms"],
+           "graph_captures": st["captures"], "capture_host_ms": st["capture_ms"],
            "precision": "mixed"}
     srv.close()
     return out

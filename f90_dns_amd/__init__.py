@@ -58,7 +58,8 @@ class ServerStats(ctypes.Structure):
         ("device_ms", ctypes.c_double), ("step_p50_ms", ctypes.c_double), ("step_p99_ms", ctypes.c_double),
         ("bytes", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t), ("max_running", ctypes.c_int),
         ("queued", ctypes.c_int), ("running", ctypes.c_int), ("done", ctypes.c_int), ("failed", ctypes.c_int),
-        ("rejected", ctypes.c_int),
+        ("rejected", ctypes.c_int), ("changes", ctypes.c_longlong), ("captures", ctypes.c_longlong),
+        ("change_ms", ctypes.c_double), ("capture_ms", ctypes.c_double),
     ]
 
 
@@ -178,6 +179,7 @@ def lib():
         L.dns_server_job_times.argtypes = [_vp, ctypes.c_int, _dp]
         L.dns_server_get_stats.argtypes = [_vp, ctypes.POINTER(ServerStats)]
         L.dns_server_set_health_interval.argtypes = [_vp, ctypes.c_int]
+        L.dns_server_release_idle.argtypes = [_vp]
         L.dns_server_pending.argtypes = [_vp]
         L.dns_debug_copy.argtypes = [ctypes.c_size_t, ctypes.c_int, _dp]
         L.dns_debug_initial_field.argtypes = [ctypes.POINTER(DnsParams), _vp, _vp, _vp]
@@ -586,6 +588,10 @@ class Server:
 
     def set_health_interval(self, ticks):
         self._check(lib().dns_server_set_health_interval(self._h, ticks))
+
+    def release_idle(self):
+        """Free the HBM of shape groups whose jobs have all finished."""
+        self._check(lib().dns_server_release_idle(self._h))
 
 
 class FFTPlan:

@@ -60,11 +60,15 @@ public:
     int enqueue_step();
     // n steps, eager or replayed from the captured graph; per-step device time kept
     int step(int n, bool graph);
+    // the tables (and with graph the captured step) for the current membership, ahead of step
+    int prepare(bool graph);
     int synchronize();
     int timing(double* out4);
     // choose tuning by timing graph replays; restores every field and controller state
     int autotune(int steps, std::string& report);
     hipStream_t stream() const { return s0; }
+    // the next graph step rebuilds the tables and re-captures (membership or tuning changed)
+    bool stale() const { return dirty || !gexec; }
     long long steps_done = 0;
 
 private:

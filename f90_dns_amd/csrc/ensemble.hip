@@ -349,6 +349,11 @@ int Ensemble::ensure_graph() {
     return 0;
 }
 
+int Ensemble::prepare(bool use_graph) {
+    if (use_graph) return ensure_graph();
+    return dirty ? rebuild() : 0;
+}
+
 int Ensemble::step(int n, bool use_graph) {
     if (n < 0) return DNS_EINVAL;
     int r;

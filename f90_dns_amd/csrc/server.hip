@@ -1,11 +1,13 @@
 // server.hip -- a device-resident serving loop over a ragged ensemble (include/f90dns_server.h).
 //
 // Jobs (a run's parameters and a step count) are queued on submission and admitted when a
-// slot of their shape is free or a new group of slots fits the HBM budget.  Every tick
+// slot of their shape is free or a new group of slots fits the HBM budget (idle groups, whose
+// jobs have all finished, keep their memory for jobs of their shape and give it back, longest
+// idle first, when a new group needs it).  Every tick
 // advances all running jobs by one step with one replay of the ensemble's captured step, so
 // jobs that arrive while others run join the next step (continuous batching at step
 // granularity).  Finished jobs are retired (diagnostics, optionally fields, copied out) and
-// their slots reused; a job whose fields go non-finite is evicted alone (its flow state is
+// their slots reused; a job that opens a new group is set up while the previous step runs; a job whose fields go non-finite is evicted alone (its flow state is
 // copied out beside every health_every-th step and read when that step is harvested).  Ticks
 // are pipelined one step deep: the host waits for step t-1 after queueing step t, except when
 // membership changes, which needs an idle stream.
@@ -73,10 +75,14 @@ struct dns_server {
     std::map<int, Job> jobs;
     std::deque<int> queue;
     std::vector<std::vector<int>> slot_job;   // [group][slot] -> job id or -1
+    // [group]: the tick at which the group's last job retired, -1 while a job runs in it.  An
+    // idle group keeps its memory for jobs of its shape until a new group needs the budget.
+    std::vector<long long> idle_since;
     std::deque<Pending> inflight;
     std::vector<hipEvent_t> free_events;
     int next_id = 1;
-    long long ticks = 0, job_steps = 0;
+    long long ticks = 0, job_steps = 0, changes = 0, captures = 0;
+    double change_ms = 0, capture_ms = 0;   // host time of membership changes and re-captures
     double gp_steps = 0, device_ms = 0;
     size_t peak_bytes = 0;
     int max_running = 0;
@@ -157,12 +163,34 @@ static int retire(dns_server* s, Job& j, int state, const std::string& why) {
     j.t_finish = now_s();
     s->ens.set_active(j.group, j.slot, false);
     s->slot_job[j.group][j.slot] = -1;
-    // a group with no running job gives its memory back
+    // a group with no running job stays allocated, idle, for the next job of its shape
     bool empty = true;
     for (int x : s->slot_job[j.group]) empty = empty && x < 0;
-    if (empty) s->ens.remove_group(j.group);
+    if (empty) s->idle_since[j.group] = s->ticks;
     j.group = j.slot = -1;
     return 0;
+}
+
+static size_t idle_bytes(dns_server* s) {
+    size_t b = 0;
+    for (size_t g = 0; g < s->ens.groups.size(); g++)
+        if (s->ens.groups[g].c && s->idle_since[g] >= 0) b += dns_device_bytes(s->ens.groups[g].c);
+    return b;
+}
+
+// give back the memory of idle groups, the longest idle first, until `need` more bytes fit
+// the budget (all of them for need = SIZE_MAX); the stream must be idle
+static int release_idle(dns_server* s, size_t need) {
+    for (;;) {
+        if (need != SIZE_MAX && s->ens.bytes() + need <= s->budget) return 0;
+        int g = -1;
+        for (size_t gi = 0; gi < s->ens.groups.size(); gi++)
+            if (s->ens.groups[gi].c && s->idle_since[gi] >= 0 && (g < 0 || s->idle_since[gi] < s->idle_since[g]))
+                g = int(gi);
+        if (g < 0) return 0;
+        if (s->ens.remove_group(g)) return sfail(s, DNS_EDEVICE, "removing an idle group failed");
+        s->idle_since[g] = -1;
+    }
 }
 
 // evict the jobs that the asynchronous health checks found non-finite (stream idle)
@@ -212,10 +240,11 @@ static bool fits(dns_server* s, const Job& j, int nslots) {
 
 // whether the queue holds a job that can be placed now (host bookkeeping only)
 static bool admissible(dns_server* s) {
+    const size_t idle = idle_bytes(s);
     for (int id : s->queue) {
         const Job& j = s->jobs[id];
         if (j.fixed_bytes + j.slot_bytes > s->budget) return true;   // to be rejected
-        if (fits(s, j, 1)) return true;
+        if (s->ens.bytes() - idle + j.fixed_bytes + j.slot_bytes <= s->budget) return true;
         for (size_t gi = 0; gi < s->ens.groups.size(); gi++) {
             dns_ctx* c = s->ens.groups[gi].c;
             if (!c || !same_group(c->p, j.p)) continue;
@@ -227,7 +256,10 @@ static bool admissible(dns_server* s) {
 }
 
 // place queued jobs (first fit, in submission order); returns whether membership changed
-static int admit(dns_server* s, bool& changed) {
+// fresh: place only jobs that open a new group within the budget as it stands, whose set-up
+// touches no memory of a running group (so it may overlap the step in flight); otherwise the
+// stream is idle and idle groups may be released for a new one
+static int admit(dns_server* s, bool& changed, bool fresh) {
     for (auto it = s->queue.begin(); it != s->queue.end();) {
         Job& j = s->jobs[*it];
         int g = -1, slot = -1;
@@ -237,10 +269,22 @@ static int admit(dns_server* s, bool& changed) {
             for (size_t k = 0; k < s->slot_job[gi].size(); k++)
                 if (s->slot_job[gi][k] < 0) { g = int(gi); slot = int(k); break; }
         }
+        if (g >= 0 && fresh) {
+            ++it;   // a slot of a running group: placed once the stream is idle
+            continue;
+        }
         if (g < 0) {
             // a new group: as many slots as fit, up to the configured number
+            if (!fresh && j.fixed_bytes + j.slot_bytes <= s->budget) {
+                int r = release_idle(s, j.fixed_bytes + j.slot_bytes);
+                if (r) return r;
+            }
             int n = s->slots;
             while (n > 0 && !fits(s, j, n)) n--;
+            if (n == 0 && fresh) {
+                ++it;
+                continue;
+            }
             if (n == 0) {
                 if (j.fixed_bytes + j.slot_bytes > s->budget) {
                     j.state = DNS_JOB_REJECTED;
@@ -260,6 +304,7 @@ static int admit(dns_server* s, bool& changed) {
                 continue;
             }
             if (int(s->slot_job.size()) <= g) s->slot_job.resize(g + 1);
+            if (int(s->idle_since.size()) <= g) s->idle_since.resize(g + 1, -1);
             s->slot_job[g].assign(n, -1);
             slot = 0;
             s->peak_bytes = std::max(s->peak_bytes, s->ens.bytes());
@@ -269,6 +314,7 @@ static int admit(dns_server* s, bool& changed) {
             return sfail(s, DNS_EDEVICE, dns_last_error(c));
         s->ens.set_active(g, slot, true);
         s->slot_job[g][slot] = j.id;
+        s->idle_since[g] = -1;
         j.group = g; j.slot = slot;
         j.state = DNS_JOB_RUNNING;
         j.t_admit = now_s();
@@ -335,14 +381,19 @@ int dns_server_tick(dns_server* s, int* running) {
     const bool health = s->health_every > 0 && s->ticks > 0 && s->ticks % s->health_every == 0;
     bool maybe_admit = !s->queue.empty() && admissible(s);
     if (finish || !s->evict.empty() || maybe_admit) {
+        const double t0 = now_s();
+        bool changed = false;
+        // new groups are allocated and their jobs set up while the previous step runs
+        if (maybe_admit && (r = admit(s, changed, true))) return r;
         if ((r = drain(s))) return r;
         if ((r = evict_flagged(s))) return r;
         for (auto& kv : s->jobs) {
             Job& j = kv.second;
             if (j.state == DNS_JOB_RUNNING && j.done >= j.nsteps && (r = retire(s, j, DNS_JOB_DONE, ""))) return r;
         }
-        bool changed = false;
-        if ((r = admit(s, changed))) return r;
+        if ((r = admit(s, changed, false))) return r;
+        s->changes += 1;
+        s->change_ms += (now_s() - t0) * 1e3;
     }
     std::vector<int> run;
     for (auto& kv : s->jobs)
@@ -355,6 +406,14 @@ int dns_server_tick(dns_server* s, int* running) {
     pd.b = take_event(s);
     if (!pd.a || !pd.b) return sfail(s, DNS_EDEVICE, "hipEventCreate failed");
     pd.jobs = run;
+    // a changed membership rebuilds the tables and re-captures the step before the tick's
+    // start event, so the tick's device time holds only the step
+    if (s->ens.stale()) {
+        const double t1 = now_s();
+        if ((r = s->ens.prepare(s->graph))) return sfail(s, r, s->ens.err);
+        s->captures += 1;
+        s->capture_ms += (now_s() - t1) * 1e3;
+    }
     if (hipEventRecord(pd.a, s->ens.stream()) != hipSuccess) return sfail(s, DNS_EDEVICE, "hipEventRecord failed");
     if ((r = s->ens.step(1, s->graph))) return sfail(s, r, s->ens.err);
     if (health && (r = queue_health(s, pd, run))) return r;
@@ -389,6 +448,12 @@ int dns_server_drain(dns_server* s) {
         }
     }
     return drain(s);
+}
+
+int dns_server_release_idle(dns_server* s) {
+    if (!s) return DNS_EINVAL;
+    int r = drain(s);
+    return r ? r : release_idle(s, SIZE_MAX);
 }
 
 int dns_server_job_status(dns_server* s, int id, int* state, int* steps_done) {
@@ -453,6 +518,10 @@ int dns_server_get_stats(dns_server* s, dns_server_stats* st) {
     st->peak_bytes = s->peak_bytes;
     st->bytes = s->ens.bytes();
     st->max_running = s->max_running;
+    st->changes = s->changes;
+    st->captures = s->captures;
+    st->change_ms = s->change_ms;
+    st->capture_ms = s->capture_ms;
     for (auto& kv : s->jobs) {
         switch (kv.second.state) {
             case DNS_JOB_QUEUED: st->queued++; break;

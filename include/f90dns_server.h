@@ -5,8 +5,9 @@
  * by HBM budget into slots of shape groups of a ragged ensemble (include/f90dns_ensemble.h),
  * kept resident on the GPU between steps and advanced together: every tick runs one step
  * of every running job, so a job submitted while others run joins at the next tick
- * (continuous batching).  A finished job gives its slot back; a job whose fields go
- * non-finite is evicted alone.  A job's result is bitwise the result of the same run alone.
+ * (continuous batching).  A finished job gives its slot back (a group left without jobs
+ * stays allocated for jobs of its shape until a new group needs the budget); a job whose
+ * fields go non-finite is evicted alone.  A job's result is bitwise the result of the same run alone.
  */
 #ifndef F90DNS_SERVER_H
 #define F90DNS_SERVER_H
@@ -26,9 +27,13 @@ typedef struct dns_server_stats {
     double grid_point_steps;     /* sum over job steps of nx*ny*nz */
     double device_ms;            /* device time of all ticks */
     double step_p50_ms, step_p99_ms;   /* per job-step latency (device time of its tick) */
-    size_t bytes, peak_bytes;    /* HBM held by the slots now / at most */
+    size_t bytes, peak_bytes;    /* HBM held by the groups (idle ones included) now / at most */
     int max_running;
     int queued, running, done, failed, rejected;
+    /* host time of the serving loop: membership changes (drain, retire, admit) and re-captures
+       of the step graph after them, during which the device is idle */
+    long long changes, captures;
+    double change_ms, capture_ms;
 } dns_server_stats;
 
 typedef struct dns_server dns_server;
@@ -53,6 +58,10 @@ int dns_server_get_stats(dns_server* s, dns_server_stats* st);
 int dns_server_pending(const dns_server* s);
 /* ticks between non-finite checks (0 = never; default 16) */
 int dns_server_set_health_interval(dns_server* s, int ticks);
+/* free the HBM of idle groups (shape groups whose jobs have all finished).  Idle groups are
+   otherwise kept for jobs of their shape and released, longest idle first, only when a new
+   group needs the budget. */
+int dns_server_release_idle(dns_server* s);
 
 #ifdef __cplusplus
 }

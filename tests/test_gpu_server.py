@@ -90,6 +90,36 @@ def test_budget_admission_and_rejection(dns):
     srv.close()
 
 
+def test_idle_group_is_reused_then_released_for_a_new_shape(dns):
+    """A group whose jobs have finished stays allocated: a later job of its shape takes its
+    slot without a new allocation, and a job of another shape that needs the budget gets it
+    once the idle group is released.  Results stay bitwise those of the runs alone."""
+    a = dns.estimate_bytes(1, **_job(0, 64, 64, 65))
+    b = dns.estimate_bytes(1, **_job(1, 96, 64, 65))
+    budget = int(max(a, b) * 1.2)   # one group of one slot at a time
+    srv = dns.Server(hbm_budget_bytes=budget, slots_per_group=1)
+    first = srv.submit(8, keep_fields=True, **_job(0, 64, 64, 65))
+    srv.drain()
+    held = srv.stats()["bytes"]
+    assert held == a   # idle, still allocated
+    again = srv.submit(6, keep_fields=True, **_job(2, 64, 64, 65))
+    srv.drain()
+    assert srv.stats()["bytes"] == a   # the idle group's slot was reused
+    other = srv.submit(5, keep_fields=True, **_job(1, 96, 64, 65))
+    srv.drain()
+    st = srv.stats()
+    assert st["bytes"] == b and st["peak_bytes"] <= budget   # the idle group gave its memory back
+    for jid, i, shape, n in ((first, 0, (64, 64, 65), 8), (again, 2, (64, 64, 65), 6), (other, 1, (96, 64, 65), 5)):
+        assert srv.status(jid) == ("done", n)
+        got, _ = srv.result(jid)
+        ref = _alone(dns, _job(i, *shape), n)
+        for f in range(4):
+            assert np.array_equal(got[f], ref[f]), (jid, f)
+    srv.release_idle()
+    assert srv.stats()["bytes"] == 0
+    srv.close()
+
+
 def test_non_finite_job_is_evicted_alone(dns):
     """A job whose initial field is non-finite (infinite perturbation amplitude, set on the
     host) is evicted at the next health check; the others finish with their alone results."""
