@@ -162,6 +162,8 @@ def lib():
         L.dns_ensemble_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]
         L.dns_ensemble_autotune.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
         L.dns_ensemble_set_tile_waves.argtypes = [_vp, _vp]
+        L.dns_ensemble_set_promotion.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+        L.dns_ensemble_launch_counts.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
         L.dns_estimate_bytes.argtypes = [ctypes.POINTER(DnsParams), ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
         L.dns_server_create.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]
         L.dns_server_destroy.argtypes = [_vp]
@@ -499,6 +501,20 @@ class Ensemble:
         a = (ctypes.c_int * 4)(*(list(promote_fp64) + [0] * 4)[:4])
         b = (ctypes.c_int * 4)(*(list(promote_fp32) + [0] * 4)[:4])
         self._check(lib().dns_ensemble_set_tuning(self._h, lanes, interleave, a, b))
+
+    # plane-transform size classes in the order promotion walks (include/f90dns_ensemble.h)
+    SIZE_CLASS_ORDER = (0, 1, 2, 5, 6, 3, 4)
+
+    def set_promotion(self, fp32, levels):
+        """Promotion levels of the FP64 (fp32=0) or FP32 (fp32=1) plane kernels per size class."""
+        a = (ctypes.c_int * max(1, len(levels)))(*levels)
+        self._check(lib().dns_ensemble_set_promotion(self._h, int(fp32), a, len(levels)))
+
+    def launch_counts(self):
+        """Launches of one step per stage: vorticity, c2r, r2c, Helmholtz, Poisson."""
+        c = (ctypes.c_int * 5)()
+        self._check(lib().dns_ensemble_launch_counts(self._h, c))
+        return dict(zip(("zderiv", "c2r", "r2c", "helmholtz", "poisson"), list(c)))
 
     def set_tile_waves(self, zderiv=0, helmholtz=0, poisson=0):
         """Least waves per column block of the wall-normal launches (0 = natural, 2 or 4)."""

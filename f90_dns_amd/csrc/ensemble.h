@@ -62,6 +62,8 @@ public:
     int step(int n, bool graph);
     // the tables (and with graph the captured step) for the current membership, ahead of step
     int prepare(bool graph);
+    // launches of one step per stage kind (K_ZVORT .. K_POIS) for the current tables
+    void launch_counts(int* counts) const;
     int synchronize();
     int timing(double* out4);
     // choose tuning by timing graph replays; restores every field and controller state

@@ -349,6 +349,11 @@ int Ensemble::ensure_graph() {
     return 0;
 }
 
+void Ensemble::launch_counts(int* counts) const {
+    for (int k = 0; k < K_COUNT; k++) counts[k] = 0;
+    for (const Launch& L : launches) counts[L.kind] += 1;
+}
+
 int Ensemble::prepare(bool use_graph) {
     if (use_graph) return ensure_graph();
     return dirty ? rebuild() : 0;
@@ -734,6 +739,24 @@ int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const in
         e->e.tune.promote[1][c] = (promote_fp32 && c < 4) ? std::max(0, promote_fp32[c]) : 0;
     }
     return e->e.rebuild();
+}
+
+int dns_ensemble_set_promotion(dns_ensemble* e, int fp32, const int* levels, int nclasses) {
+    static_assert(DNS_ENSEMBLE_SIZE_CLASSES == FFT_CLASSES, "size classes of the ensemble ABI");
+    if (!e || (fp32 != 0 && fp32 != 1) || nclasses < 0 || nclasses > FFT_CLASSES || (nclasses > 0 && !levels))
+        return DNS_EINVAL;
+    int r = e->e.synchronize();
+    if (r) return r;
+    for (int c = 0; c < FFT_CLASSES; c++) e->e.tune.promote[fp32][c] = c < nclasses ? std::max(0, levels[c]) : 0;
+    return e->e.rebuild();
+}
+
+int dns_ensemble_launch_counts(dns_ensemble* e, int* counts) {
+    if (!e || !counts) return DNS_EINVAL;
+    int r = e->e.prepare(false);
+    if (r) return r;
+    e->e.launch_counts(counts);
+    return 0;
 }
 
 int dns_ensemble_set_tile_waves(dns_ensemble* e, const int* zwaves) {

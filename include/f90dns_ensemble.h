@@ -48,10 +48,24 @@ int dns_ensemble_synchronize(dns_ensemble* e);
 int dns_ensemble_step_timing(dns_ensemble* e, double* out4);
 
 /* Launch-shape tuning: lanes = streams the step is split over (1, 2); interleave = z-kernel
- * tile order; promote_fp64[4] / promote_fp32[4] = extra size-class levels (more threads per
- * plane: 64, 256, 512, 1024) for the plane kernels of size classes 0..3 (NULL = none). */
+ * tile order; promote_fp64[4] / promote_fp32[4] = levels of promotion (below) for the plane
+ * kernels of size classes 0..3 (NULL = none; classes 4..6 unpromoted). */
 int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const int* promote_fp64,
                             const int* promote_fp32);
+/* Plane-transform size classes, as (threads per plane, largest (nx/2)*ny): 0 (64, 512),
+ * 1 (256, 2048), 2 (512, 4096), 3 (1024, 8192), 4 (1024, 18432; FP32 only), 5 (576, 4608),
+ * 6 (768, 6144).  A plane runs in the smallest class that holds it, in the order
+ * 0, 1, 2, 5, 6, 3, 4; levels[c] (c < nclasses <= DNS_ENSEMBLE_SIZE_CLASSES) moves the planes of
+ * class c that many places up this order (past the end: as far as it goes), for the FP64
+ * (fp32 = 0) or FP32 (fp32 = 1) transforms; other classes are left unpromoted.  Results do not
+ * change.  Promoting every class present to one class gives one c2r and one r2c launch per
+ * step (with set_tile_waves 2, 2, 2 and one lane, one launch per stage for nz of 33 to 97).
+ * No counterpart in the reference. */
+enum { DNS_ENSEMBLE_SIZE_CLASSES = 7 };
+int dns_ensemble_set_promotion(dns_ensemble* e, int fp32, const int* levels, int nclasses);
+/* Launches of one step for the current members and tuning, per stage: counts[5] = vorticity
+ * z-derivative, c2r, r2c, Helmholtz, Poisson (over both lanes). */
+int dns_ensemble_launch_counts(dns_ensemble* e, int* counts);
 /* Tile shapes of the wall-normal kernels: zwaves[3] = the least waves per column block of the
  * vorticity z-derivative, Helmholtz and Poisson launches (0 = one wave per row-block pair of the
  * group's operators, 2 or 4).  Groups whose operators have fewer row-block pairs run at that

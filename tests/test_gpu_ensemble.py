@@ -93,6 +93,27 @@ def test_ensemble_tuning_does_not_change_results(dns, tuning, waves):
     _check_bitwise(dns, ens, mem, 10)
 
 
+def test_one_launch_per_stage(dns):
+    """Every plane class of the mix promoted to one class (the 96 x 96 planes' class 5) and the
+    wall-normal tiles at two waves: all members share one launch per stage, and each stays
+    bitwise its run alone."""
+    mem = _mixed_shapes(1)
+    ens = dns.Ensemble(mem)
+    natural = ens.launch_counts()
+    assert natural["c2r"] == natural["r2c"] == 4 and natural["zderiv"] >= 1
+    order = dns.Ensemble.SIZE_CLASS_ORDER
+    levels = [order.index(5) - order.index(c) if c in (0, 1, 2) else 0 for c in range(7)]
+    ens.set_promotion(1, levels)
+    ens.set_tile_waves(2, 2, 2)
+    counts = ens.launch_counts()
+    _say(f"launches per stage: natural {natural}, unified {counts}")
+    assert counts == dict(zderiv=1, c2r=1, r2c=1, helmholtz=1, poisson=1)
+    ens.init_fields()
+    ens.step(6, graph=True)
+    ens.step(2, graph=False)
+    _check_bitwise(dns, ens, mem, 8)
+
+
 def test_autotune_restores_state_and_keeps_results(dns):
     mem = _mixed_shapes(1)
     ens = dns.Ensemble(mem)
