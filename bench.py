@@ -159,7 +159,7 @@ def _oracle_run(kw, steps, threads):
 
 
 def cpu_baseline(kw, steps, config1_steps):
-    """The reported CPU baseline (BASELINE.md section 2): the oracle -- the C port of the reference
+    """The reported CPU baseline: the oracle -- the C port of the reference
     algorithm as written -- on this host, the production workload for a few steps on all cores
     available, config 1 (32x32x33) for config1_steps steps on all cores and on one thread;
     wall-clock per step, set-up excluded."""

@@ -49,12 +49,9 @@ def test_mfma_f64_lane_layout(dns):
     """v_mfma_f64_16x16x4f64 operand/result lane layout assumed by the contractions:
     A(i,k) at lane i + 16k, B(k,j) at lane j + 16k, C(i,j) at lane j + 16(i%4), item i/4."""
     import ctypes
-    import os
     out = np.empty(4096 * 256)
     assert dns.lib().dns_debug_mfma_probe(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     P = out.reshape(64, 64, 64, 4)          # [la, lb, lane, item]
-    os.makedirs("gpurun_out", exist_ok=True)
-    np.save("gpurun_out/mfma_probe.npy", P)
     want = np.zeros_like(P)
     for la in range(64):
         for lb in range(64):
@@ -67,15 +64,19 @@ def test_mfma_f64_lane_layout(dns):
 
 
 def test_mfma_f32_lane_layout_probe(dns):
-    """Records the v_mfma_f32_16x16x4f32 lane layout (used by the mixed-precision contraction)."""
+    """v_mfma_f32_16x16x4f32 lane layout assumed by the FP32 contraction: operands as in FP64
+    (A(i,k) at lane i + 16k, B(k,j) at lane j + 16k), C(i,j) at lane j + 16(i/4), item i%4."""
     import ctypes
-    import os
     out = np.empty(4096 * 256)
     assert dns.lib().dns_debug_mfma_probe_f32(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
     P = out.reshape(64, 64, 64, 4)
-    os.makedirs("gpurun_out", exist_ok=True)
-    np.save("gpurun_out/mfma_probe_f32.npy", P)
-    assert np.count_nonzero(P) == 1024 and set(np.unique(P)) == {0.0, 1.0}
+    want = np.zeros_like(P)
+    for la in range(64):
+        for lb in range(64):
+            if la // 16 == lb // 16:
+                i, j = la % 16, lb % 16
+                want[la, lb, j + 16 * (i // 4), i % 4] = 1.0
+    np.testing.assert_array_equal(P, want)
 
 
 @pytest.mark.parametrize("nz,exact", [(17, 0), (33, 0), (65, 1), (129, 1), (129, 0), (9, 0), (48, 1)])

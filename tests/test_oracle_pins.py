@@ -105,7 +105,7 @@ def test_wavenumbers_closed_form():
 @pytest.mark.parametrize("nx,kxm", [(32, 10), (64, 21), (96, 32), (128, 42), (192, 64)])
 @pytest.mark.parametrize("ny,kym", [(24, 8), (36, 12), (96, 32), (130, 43)])
 def test_dealias_mask_bounds(nx, kxm, ny, kym):
-    # the closed-form bounds of SURVEY (kx_max = 10, 21, 32, 42, 64) and the reference's loop
+    # the closed-form bounds (kx_max = (nx/2)*2/3 = 10, 21, 32, 42, 64) and the reference's loop
     # (1-based i, j; :1176-1190) written out element by element, with ny != nx
     m = orc.dealias_mask(nx, ny)
     want = np.ones((ny, nx // 2 + 1), dtype=bool)
