@@ -23,10 +23,13 @@ namespace dns {
 // the flow controller runs inside K_HELM and mode (0,0) inside K_POIS
 enum StageKind { K_ZVORT = 0, K_C2R, K_R2C, K_HELM, K_POIS, K_COUNT };
 
+// most streams a step is split over (GPU_MAX_HW_QUEUES is 4 by default)
+constexpr int ENS_LANES = 4;
+
 // Tunable launch shapes (chosen by Ensemble::autotune or set by hand).
 struct EnsTuning {
     int promote[2][FFT_CLASSES] = {};   // [f32][size class]: levels up in capacity order (more threads)
-    int lanes = 1;                                       // streams the step is split over (1 or 2)
+    int lanes = 1;                                       // streams the step is split over (1 to ENS_LANES)
     int interleave = 0;                                  // 1: z-kernel tiles ordered column-block-major across members
     // tile shapes of the wall-normal kernels (vorticity z-derivative, Helmholtz, Poisson): the
     // least waves per column block (z_variant_widen: 0 = one per row-block pair, 2, 4); groups
@@ -56,7 +59,7 @@ public:
     size_t bytes() const;
     // tile tables and argument blocks for the current membership and tuning (synchronises)
     int rebuild();
-    // one step of every active slot on stream s0 (and s1 when split), eager
+    // one step of every active slot on stream s0 (forked over the lanes' streams when split), eager
     int enqueue_step();
     // n steps, eager or replayed from the captured graph; per-step device time kept
     int step(int n, bool graph);
@@ -78,8 +81,10 @@ private:
         int kind, cls, f32, off, count, lane, nf;
         size_t lds;
     };
-    hipStream_t s0 = nullptr, s1 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t s0 = nullptr;                 // lane 0, the stream of the step
+    hipStream_t sl[ENS_LANES] = {};           // sl[0] = s0, the other lanes' streams
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[ENS_LANES] = {};
     FFTArgs *d_c2r = nullptr, *d_r2c = nullptr;
     ZArgs* d_z = nullptr;
     HelmArgs* d_h = nullptr;

@@ -56,28 +56,31 @@ bool ctx_bytes_for(const dns_params& p, int B, size_t& bytes, std::string& err) 
 }
 
 Ensemble::~Ensemble() {
-    if (s0) (void)hipStreamSynchronize(s0);
-    if (s1) (void)hipStreamSynchronize(s1);
+    for (hipStream_t s : sl)
+        if (s) (void)hipStreamSynchronize(s);
     drop_graph();
     for (auto e : ev) (void)hipEventDestroy(e);
     if (fork) (void)hipEventDestroy(fork);
-    if (join) (void)hipEventDestroy(join);
+    for (hipEvent_t e : join)
+        if (e) (void)hipEventDestroy(e);
     void* bufs[] = {d_c2r, d_r2c, d_z, d_h, d_p, d_tiles};
     for (void* q : bufs)
         if (q) (void)hipFree(q);
     for (auto& g : groups)
         if (g.c) dns_destroy(g.c);
-    if (s0) (void)hipStreamDestroy(s0);
-    if (s1) (void)hipStreamDestroy(s1);
+    for (hipStream_t s : sl)
+        if (s) (void)hipStreamDestroy(s);
 }
 
 int Ensemble::init() {
     std::string why;
     if (!dns_check_gfx950(why)) { err = why; return DNS_EDEVICE; }
-    ECHECK(hipStreamCreateWithFlags(&s0, hipStreamNonBlocking));
-    ECHECK(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+    for (int l = 0; l < ENS_LANES; l++) {
+        ECHECK(hipStreamCreateWithFlags(&sl[l], hipStreamNonBlocking));
+        ECHECK(hipEventCreateWithFlags(&join[l], hipEventDisableTiming));
+    }
+    s0 = sl[0];
     ECHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    ECHECK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     ECHECK(fft_prepare_all());
     return 0;
 }
@@ -99,8 +102,7 @@ int Ensemble::add_group(const dns_params* params, int n, bool active) {
 
 int Ensemble::remove_group(int g) {
     if (g < 0 || g >= int(groups.size()) || !groups[g].c) return DNS_EINVAL;
-    ECHECK(hipStreamSynchronize(s0));
-    ECHECK(hipStreamSynchronize(s1));
+    for (hipStream_t s : sl) ECHECK(hipStreamSynchronize(s));
     drop_graph();
     dns_destroy(groups[g].c);
     groups[g].c = nullptr;
@@ -154,8 +156,7 @@ static hipError_t grow(T** p, size_t& cap, size_t n) {
 }
 
 int Ensemble::rebuild() {
-    ECHECK(hipStreamSynchronize(s0));
-    ECHECK(hipStreamSynchronize(s1));
+    for (hipStream_t s : sl) ECHECK(hipStreamSynchronize(s));
     drop_graph();
     const int G = int(groups.size());
     std::vector<FFTArgs> hc2r(G), hr2c(G);
@@ -181,7 +182,7 @@ int Ensemble::rebuild() {
         hp[g].zwaves = tune.zwaves[2];
     }
     // lane assignment: balance an estimate of the work (points x (log2 plane + nz/8))
-    std::vector<double> load(2, 0.0);
+    std::vector<double> load(ENS_LANES, 0.0);
     std::vector<int> order;
     for (int g = 0; g < G; g++)
         if (groups[g].c) order.push_back(g);
@@ -192,16 +193,17 @@ int Ensemble::rebuild() {
         return double(act) * c->N * (std::log2(double(c->npl)) + c->nz / 8.0);
     };
     std::sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
-    for (int g : order) {
+    const int nlanes = std::max(1, std::min(tune.lanes, ENS_LANES));
+    for (int g : order) {   // largest first, each to the least loaded lane
         int l = 0;
-        if (tune.lanes > 1 && load[1] < load[0]) l = 1;
+        for (int k = 1; k < nlanes; k++)
+            if (load[k] < load[l]) l = k;
         groups[g].lane = l;
         load[l] += cost(g);
     }
     // tiles per (lane, kind, class)
     std::vector<Tile> tiles;
     launches.clear();
-    const int nlanes = std::max(1, std::min(tune.lanes, 2));
     for (int lane = 0; lane < nlanes; lane++) {
         for (int kind = 0; kind < K_COUNT; kind++) {
             // class key -> groups
@@ -309,19 +311,19 @@ int Ensemble::enqueue_step() {
         int r = rebuild();
         if (r) return r;
     }
-    const bool split = tune.lanes > 1;
-    if (split) {
+    const int nlanes = std::max(1, std::min(tune.lanes, ENS_LANES));
+    if (nlanes > 1) {
         ECHECK(hipEventRecord(fork, s0));
-        ECHECK(hipStreamWaitEvent(s1, fork, 0));
+        for (int l = 1; l < nlanes; l++) ECHECK(hipStreamWaitEvent(sl[l], fork, 0));
     }
     // launches are stored lane by lane, each lane in stage order
     for (const Launch& L : launches) {
-        int r = launch(L, L.lane == 0 ? s0 : s1);
+        int r = launch(L, sl[L.lane]);
         if (r) return r;
     }
-    if (split) {
-        ECHECK(hipEventRecord(join, s1));
-        ECHECK(hipStreamWaitEvent(s0, join, 0));
+    for (int l = 1; l < nlanes; l++) {
+        ECHECK(hipEventRecord(join[l], sl[l]));
+        ECHECK(hipStreamWaitEvent(s0, join[l], 0));
     }
     return 0;
 }
@@ -520,7 +522,10 @@ int Ensemble::autotune(int steps, std::string& report) {
     // coordinate search: stream split, tile order, then each plane-kernel size class
     std::vector<EnsTuning> cand;
     {
-        EnsTuning t = best; t.lanes = best.lanes == 1 ? 2 : 1; cand.push_back(t);
+        for (int l = 1; l <= ENS_LANES; l++)
+            if (l != best.lanes) {
+                EnsTuning t = best; t.lanes = l; cand.push_back(t);
+            }
     }
     for (auto& t0 : cand) {
         double ms;
@@ -549,10 +554,12 @@ int Ensemble::autotune(int steps, std::string& report) {
             }
         }
     }
-    // tile shapes of the wall-normal kernels, one kernel kind at a time: widths that change
-    // some group's variant (a group whose operators have one or two row-block pairs)
+    // tile shapes of the wall-normal kernels, one kernel kind at a time: widths (natural, two or
+    // four waves) that change some group's variant (a group whose operators have one or two
+    // row-block pairs)
     for (int k = 0; k < 3; k++) {
-        for (int w : {2, 4}) {
+        for (int w : {0, 2, 4}) {
+            if (w == best.zwaves[k]) continue;
             bool changes = false;
             for (auto& g : groups) {
                 if (!g.c) continue;
@@ -567,6 +574,15 @@ int Ensemble::autotune(int steps, std::string& report) {
             if ((r = trial(t, ms))) return r;
             if (ms < best_ms) { best_ms = ms; best = t; }
         }
+    }
+    // the stream split once more, for the tile shapes chosen
+    for (int l = 1; l <= ENS_LANES; l++) {
+        if (l == best.lanes) continue;
+        EnsTuning t = best;
+        t.lanes = l;
+        double ms;
+        if ((r = trial(t, ms))) return r;
+        if (ms < best_ms) { best_ms = ms; best = t; }
     }
     tune = best;
     dirty = true;
@@ -729,7 +745,7 @@ int dns_ensemble_step_timing(dns_ensemble* e, double* out4) {
 
 int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const int* promote_fp64,
                             const int* promote_fp32) {
-    if (!e || lanes < 1 || lanes > 2) return DNS_EINVAL;
+    if (!e || lanes < 1 || lanes > ENS_LANES) return DNS_EINVAL;
     int r = e->e.synchronize();
     if (r) return r;
     e->e.tune.lanes = lanes;

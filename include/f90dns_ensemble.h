@@ -47,7 +47,8 @@ int dns_ensemble_synchronize(dns_ensemble* e);
 /* min, p50, p99, mean device time per ensemble step (ms) of the last dns_ensemble_step */
 int dns_ensemble_step_timing(dns_ensemble* e, double* out4);
 
-/* Launch-shape tuning: lanes = streams the step is split over (1, 2); interleave = z-kernel
+/* Launch-shape tuning: lanes = streams the step is split over (1 to 4; groups are dealt to the
+ * least loaded lane, largest first); interleave = z-kernel
  * tile order; promote_fp64[4] / promote_fp32[4] = levels of promotion (below) for the plane
  * kernels of size classes 0..3 (NULL = none; classes 4..6 unpromoted). */
 int dns_ensemble_set_tuning(dns_ensemble* e, int lanes, int interleave, const int* promote_fp64,

@@ -76,8 +76,10 @@ def test_ensemble_member_equals_member_alone(dns, precision):
 @pytest.mark.parametrize("tuning,waves", [(dict(lanes=2), None), (dict(interleave=1), None),
                                           (dict(promote_fp64=(1, 1, 0)), None),
                                           (dict(lanes=2, promote_fp64=(2, 1, 0), interleave=1), None),
-                                          (dict(), (2, 2, 2)), (dict(), (4, 4, 4)), (dict(lanes=2), (2, 4, 2))],
-                         ids=["lanes2", "interleave", "promote", "all", "waves2", "waves4", "waves_mixed"])
+                                          (dict(), (2, 2, 2)), (dict(), (4, 4, 4)), (dict(lanes=2), (2, 4, 2)),
+                                          (dict(lanes=3), None), (dict(lanes=4), (0, 0, 2))],
+                         ids=["lanes2", "interleave", "promote", "all", "waves2", "waves4", "waves_mixed", "lanes3",
+                              "lanes4"])
 def test_ensemble_tuning_does_not_change_results(dns, tuning, waves):
     """Launch tuning, including the tile shapes of the wall-normal kernels (nz = 33 and 65 groups
     at two or four waves per column block, sharing one launch per stage), leaves every member
