@@ -38,7 +38,8 @@ typedef struct dns_server_stats {
 
 typedef struct dns_server dns_server;
 
-/* slots_per_group: slots a new shape group gets (fewer if the budget is short). */
+/* slots_per_group: slots a new shape group gets (fewer if the budget is short).  Each tick's
+ * step is split over two streams (include/f90dns_ensemble.h, lanes). */
 int dns_server_create(size_t hbm_budget_bytes, int slots_per_group, int use_graph, dns_server** out);
 int dns_server_destroy(dns_server* s);
 const char* dns_server_last_error(const dns_server* s);
