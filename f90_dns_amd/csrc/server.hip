@@ -1,16 +1,16 @@
 // server.hip -- a device-resident serving loop over a ragged ensemble (include/f90dns_server.h).
 //
 // Jobs (a run's parameters and a step count) are queued on submission and admitted when a
-// slot of their shape is free or a new group of slots fits the HBM budget (idle groups, whose
-// jobs have all finished, keep their memory for jobs of their shape and give it back, longest
-// idle first, when a new group needs it).  Every tick
-// advances all running jobs by one step with one replay of the ensemble's captured step, so
-// jobs that arrive while others run join the next step (continuous batching at step
-// granularity).  Finished jobs are retired (diagnostics, optionally fields, copied out) and
-// their slots reused; a job that opens a new group is set up while the previous step runs; a job whose fields go non-finite is evicted alone (its flow state is
-// copied out beside every health_every-th step and read when that step is harvested).  Ticks
-// are pipelined one step deep: the host waits for step t-1 after queueing step t, except when
-// membership changes, which needs an idle stream.
+// slot of their shape is free or a new group of slots fits the HBM budget.  Idle groups (whose
+// jobs have all finished) keep their memory for jobs of their shape and give it back, longest
+// idle first, when a new group needs it.  Every tick advances all running jobs by one step with
+// one replay of the ensemble's captured step, so jobs that arrive while others run join the next
+// step (continuous batching at step granularity).  Finished jobs are retired (diagnostics,
+// optionally fields, copied out) and their slots reused.  A job that opens a new group is set up
+// while the previous step runs.  A job whose fields go non-finite is evicted alone (its flow
+// state is copied out beside every health_every-th step and read when that step is harvested).
+// Ticks are pipelined one step deep: the host waits for step t-1 after queueing step t, except
+// when membership changes, which needs an idle stream.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
