@@ -1218,7 +1218,14 @@ int dns_debug_operator(int nz, double dt, double re, int cn, int exact, int whic
     auto vec = [&](const std::vector<double>& v) { m = v; r = int(v.size()); cc = 1; };
     auto mat = [&](const FoldedA& a) { unswizzle(a, m); r = a.rows0 + a.rows1; cc = a.K; };
     switch (which) {
-        case 0: mat(op.hF); break;
+        case 0: {   // back to the logical column order [T | f T D2_I] from the kernel's panel order
+            mat(op.hF);
+            const std::vector<double> st = m;
+            const int L1 = (cc - 1) / 2;
+            for (int i = 0; i < r; i++)
+                for (int p = 0; p < cc; p++) m[size_t(i) * cc + helm_panel_column(p, L1)] = st[size_t(i) * cc + p];
+            break;
+        }
         case 1: mat(op.hB); break;
         case 2: vec(op.hc1); break;
         case 3: vec(op.hc2); break;

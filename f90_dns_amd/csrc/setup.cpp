@@ -559,12 +559,13 @@ bool build_operators(int nz, double dt, double re, double theta, bool cn, bool e
     op.f = double(f);
     std::vector<double> F0(size_t(nE) * K, 0.0), F1(size_t(nO) * K, 0.0);
     op.hc1.assign(m, 0.0); op.hc2.assign(m, 0.0); op.hl.assign(m, 0.0);
+    std::vector<double> Fl(K);   // one row in the logical column order [T | f T D2_I]
     for (int jj = 0; jj < m; jj++) {
         std::vector<double>& F = jj < nE ? F0 : F1;
         const size_t row = size_t(jj < nE ? jj : jj - nE) * K;
-        for (int k = 0; k < L1; k++) F[row + k] = base->FT[size_t(jj) * L1 + k];
-        if (cn)
-            for (int c = 0; c < L2; c++) F[row + L1 + c] = double(f * base->TD2[size_t(jj) * L2 + c]);
+        for (int k = 0; k < L1; k++) Fl[k] = base->FT[size_t(jj) * L1 + k];
+        for (int c = 0; c < L2; c++) Fl[L1 + c] = cn ? double(f * base->TD2[size_t(jj) * L2 + c]) : 0.0;
+        for (int p = 0; p < K; p++) F[row + p] = Fl[helm_panel_column(p, L1)];   // the kernel's panel order
         op.hc1[jj] = double(-2.0L * f * base->c1[jj]);
         op.hc2[jj] = double(-2.0L * f * base->c2[jj]);
         op.hl[jj] = double(f * base->lam[jj] + 1.0L);

@@ -60,6 +60,9 @@ struct FoldedA {
     std::vector<double> data;
 };
 FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector<double>& M1, int rows1, int K);
+// Column of the Helmholtz forward operator [T | f T D2_I] (v rows 0..L1-1, then u rows 0..L1) that
+// panel row p holds in the interleaved order [v_0, u_1, v_1, u_2, ..., v_{L1-1}, u_L1, u_0].
+inline int helm_panel_column(int p, int L1) { return p < 2 * L1 ? ((p & 1) ? L1 + (p + 1) / 2 : p / 2) : L1; }
 
 // Operators of one grid and run (nz, dt, Re, theta):
 //  Helmholtz (interior unknowns I = 1..nz-2, m = nz-2), with Q L Q^T = W_I^{-1/2} A_II W_I^{-1/2}:
@@ -68,7 +71,9 @@ FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector
 //    T = Q^T W_I^{1/2}, B = W_I^{-1/2} Q, c_{1,2} = -2 f Q^T W_I^{-1/2} A_{I,{0,nz-1}}, hl = f lambda + 1.
 //    Folded: eigenvectors ordered even (group 0), then odd (group 1); the forward operator acts on
 //    the panels [v_E ; u_E] (group 0) and [v_O ; u_O] (group 1) with v = alpha u_I + dt S_I folded
-//    over the interior and u over all nz rows; B z unfolds (p from z_E, q from z_O).
+//    over the interior (L1 rows) and u over all nz rows (L1 + 1 rows); B z unfolds (p from z_E, q
+//    from z_O).  The stored K order interleaves the rows that fold the same two nodes (v row c and
+//    u row c+1): [v_0, u_1, v_1, u_2, ..., v_{L1-1}, u_L1, u_0] (helm_panel_column).
 //  Poisson (all nz unknowns), with Qf Lf Qf^T = W^{-1/2} A W^{-1/2}:
 //    y = Tf div + (Tf D1) w,  z = -y / (dt (lf + k^2)),  phi = Bf z,  D1 phi = (D1 Bf) z,
 //    Tf = Qf^T W^{1/2}, Bf = W^{-1/2} Qf.  Folded: group 0 (even) on [div_E ; w_O], group 1 (odd)

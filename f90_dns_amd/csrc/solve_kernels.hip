@@ -419,26 +419,33 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
     const double2* u2 = reinterpret_cast<const double2*>(uh) + smode;
     const float2* sf2 = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(gmem(a.sh[comp])) + mo) + smode;
     const double2* sd2 = reinterpret_cast<const double2*>(gmem(a.sh[comp]) + mo) + smode;
-    // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
-    // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
-    // two nodes coincide
-    // branch-free: rows [0, L1) fold v = alpha u_I + dt S'_I over the interior (nodes c+1 and
-    // m-c), rows [L1, L1+LE) fold u over the column (nodes c and nz-1-c); on a centre row the
-    // two nodes coincide.  fetch reads u and S at both nodes; on u rows, where S is not used, the
-    // S reads go to the u elements just read (the same cache lines, no HBM traffic).
+    // Panel rows, branch-free: v rows fold v = alpha u_I + dt S'_I over the interior (v row c:
+    // nodes c+1 and m-c), u rows fold u over the column (u row c: nodes c and nz-1-c); on a
+    // centre row the two nodes coincide.  v row c and u row c+1 fold the same two nodes, so the
+    // panel interleaves them, [v_0, u_1, v_1, u_2, ..., v_{L1-1}, u_L1, u_0] (LE = L1 + 1; the
+    // host stores the operator's columns in this order): the two rows of a pair are staged by
+    // lanes of one wave, whose load instruction then reads u once for both.  fetch reads u and S
+    // at both nodes; on u rows, where S is not used, the S reads go to the u elements just read
+    // (the same cache lines, no HBM traffic).
     using SV = typename std::conditional<SF32, float2, double2>::type;
     const SV* sv2 = SF32 ? reinterpret_cast<const SV*>(sf2) : reinterpret_cast<const SV*>(sd2);
     struct Raw { double2 ua, ub; SV sa, sb; };
+    auto prow = [&](int kf, bool& inv, int& c) {   // panel row kf: a v row (inv) or a u row, index c
+        const bool pz = kf < 2 * L1;
+        inv = pz && !(kf & 1);
+        c = pz ? (kf + 1) >> 1 : (kf == 2 * L1 ? 0 : LE);   // (rows past K: c = LE, kept zero)
+    };
     auto nodes = [&](int kf, int& na, int& nb) {
-        const bool inv = kf < L1;
-        const int c = inv ? kf : kf - L1;
+        bool inv;
+        int c;
+        prow(kf, inv, c);
         na = min(inv ? c + 1 : c, nz - 1);
         nb = max(inv ? m - c : nz - 1 - c, 0);
     };
     auto fetch = [&](int kf) -> Raw {
         int na, nb;
         nodes(kf, na, nb);
-        const bool inv = kf < L1;
+        const bool inv = kf < 2 * L1 && !(kf & 1);
         if constexpr (PS) {
             // two scalar reads per value: a paired slot's parts are the real parts of two modes
             using ST = typename std::conditional<SF32, float, double>::type;
@@ -459,8 +466,9 @@ __device__ __forceinline__ void helmholtz_tile(const HelmArgs& a, int cb, int co
         }
     };
     auto finish = [&](int kf, const Raw& rw) -> FoldPair {
-        const bool inv = kf < L1;
-        const int c = inv ? kf : kf - L1;
+        bool inv;
+        int c;
+        prow(kf, inv, c);
         const int half = inv ? hI : hF, lim = inv ? L1 : LE;
         const double2 ua = rw.ua, ub = rw.ub;
         const double2 sa = make_double2(double(rw.sa.x), double(rw.sa.y));
