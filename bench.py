@@ -158,10 +158,10 @@ def _oracle_run(kw, steps, threads):
     return np.array(times)
 
 
-def cpu_baseline(kw, steps, config1_steps):
+def cpu_baseline(kw, steps, channel32_steps):
     """The reported CPU baseline: the oracle -- the C port of the reference
     algorithm as written -- on this host, the production workload for a few steps on all cores
-    available, config 1 (32x32x33) for config1_steps steps on all cores and on one thread;
+    available, the 32x32x33 channel for channel32_steps steps on all cores and on one thread;
     wall-clock per step, set-up excluded."""
     info = cpu_info()
     thr = info["threads_available"]
@@ -172,13 +172,13 @@ def cpu_baseline(kw, steps, config1_steps):
                      f"(per-mode rebuild + LU every step), {thr} OpenMP threads, {t.sum():.1f} s wall",
            "cpu_model": info["model"], "physical_cores": info["physical_cores"],
            "step_s": {"p50": float(np.percentile(t, 50)), "p99": float(np.percentile(t, 99))}}
-    if config1_steps > 0:
+    if channel32_steps > 0:
         k1 = WORKLOADS["channel_32x32x33_fp64"]
         n1 = k1["nx"] * k1["ny"] * k1["nz"]
-        for label, th in (("config1_all_threads", thr), ("config1_1_thread", 1)):
-            t1 = _oracle_run(k1, config1_steps, th)
-            out[label] = {"value": n1 * config1_steps / t1.sum(), "unit": "grid-point-steps/s", "threads": th,
-                          "steps": config1_steps, "steps_per_s": config1_steps / t1.sum(),
+        for label, th in (("channel32_all_threads", thr), ("channel32_1_thread", 1)):
+            t1 = _oracle_run(k1, channel32_steps, th)
+            out[label] = {"value": n1 * channel32_steps / t1.sum(), "unit": "grid-point-steps/s", "threads": th,
+                          "steps": channel32_steps, "steps_per_s": channel32_steps / t1.sum(),
                           "step_s": {"p50": float(np.percentile(t1, 50)), "p99": float(np.percentile(t1, 99))}}
         _set_threads(thr)
     return out
@@ -260,7 +260,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch kernels eagerly instead of graph replay")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps of the production workload")
-    ap.add_argument("--cpu-config1-steps", type=int, default=100, help="oracle steps of config 1 (0 = skip)")
+    ap.add_argument("--cpu-channel32-steps", type=int, default=100, help="oracle steps of the 32x32x33 channel (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=200,
                     help="steps for the mixed-precision line reported beside the FP64 value (0 = skip)")
@@ -378,9 +378,9 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline and args.cpu_steps > 0:
-        print(f"[bench] cpu baseline: {args.cpu_steps} oracle step(s), config 1 x {args.cpu_config1_steps}",
+        print(f"[bench] cpu baseline: {args.cpu_steps} oracle step(s), 32x32x33 x {args.cpu_channel32_steps}",
               file=sys.stderr, flush=True)
-        cpu = cpu_baseline(kw, args.cpu_steps, args.cpu_config1_steps)
+        cpu = cpu_baseline(kw, args.cpu_steps, args.cpu_channel32_steps)
 
     out = {
         "metric": METRIC,

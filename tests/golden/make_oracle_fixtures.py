@@ -5,7 +5,7 @@ each case from its initial field, with a twin whose plane transforms use the rev
 factor order (fft_variant = 1) to record each field's rounding floor, and stores at each
 snapshot step the record of tests/oracle_fixtures.py.  Cases:
 
-- config2: 128 x 128 x 129, Lx = 4 pi, Ly = 2 pi, Re = 180, dt = 0.002, constant bulk velocity
+- channel128: 128 x 128 x 129, Lx = 4 pi, Ly = 2 pi, Re = 180, dt = 0.002, constant bulk velocity
   (PI, U_b = 2/3, gain 0.15), seeded stream-function noise |kx|, |ky| <= 8, amplitude 0.05,
   exact LGL D1; snapshots after 100 and 500 steps.
 - served_192x192x129, served_128x96x65: two jobs of the served-sweep test mix (Lx = 4 pi,
@@ -27,7 +27,7 @@ sys.path.insert(0, ROOT)
 from oracle import oracle as orc  # noqa: E402
 from tests import oracle_fixtures as fx  # noqa: E402
 
-CONFIG2 = dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+CHANNEL128 = dict(nx=128, ny=128, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
                use_dealias_23_rule=1, flow_control_method=2, target_pressure_gradient=2.0 / 180.0,
                target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, controller_update_freq=7, istart=0,
                u_centerline=1.0, enable_perturbations=1, perturbation_kind=1, seed=2024, noise_kmax=8,
@@ -46,7 +46,7 @@ def ensemble_member(i):
 
 
 CASES = {
-    "config2": (CONFIG2, [100, 500]),
+    "channel128": (CHANNEL128, [100, 500]),
     "served_192x192x129": (served(192, 192, 129, 250.0, 0.07, 5101), [60]),
     "served_128x96x65": (served(128, 96, 65, 320.0, 0.02, 5102), [200]),
 }

@@ -13,7 +13,7 @@ def field_errors(got, ref):
     """Per-field parity errors for (u, v, w, p): relative L2 for u, v, p; for w, the L2 error
     relative to the velocity-field norm ||(u, v, w)|| -- w is ~1e-6 of u in these channel runs,
     so its self-relative error sits on the rounding floor of the FFT round trips (~2e-10 at
-    config 1); that self-relative value is returned as 'w_self'."""
+    the 32x32x33 channel); that self-relative value is returned as 'w_self'."""
     u, v, w = ref[:3]
     p = ref[3] if len(ref) > 3 else None
     vel = np.sqrt(np.linalg.norm(u) ** 2 + np.linalg.norm(v) ** 2 + np.linalg.norm(w) ** 2)
@@ -31,7 +31,7 @@ def max_err(e):
 # Bounds on w's error relative to its own norm.  w is ~1e-6 of u in these runs, so rounding
 # moves it at ~1e-10 of its own norm under any reordering of the arithmetic: the oracle against
 # its own twin with other plane-transform factorisations disagrees by 1.2e-10 in w after 100
-# config-1 steps (tests/test_oracle_pins.py::test_w_rounding_floor_of_the_oracle_itself) while
+# 32x32x33 steps (tests/test_oracle_pins.py::test_w_rounding_floor_of_the_oracle_itself) while
 # u, v, p agree to 1e-13.  FP64 paths are held to W_SELF_FP64 (a few times the largest such
 # floor seen, 2e-9 at 128^2 x 129 after 3 steps, where w is younger and smaller still); the
 # long runs compare against the floor measured for that very run.  Mixed precision rounds the
@@ -48,8 +48,8 @@ def check_parity(e, tol, w_self_tol):
     assert e["w_self"] < w_self_tol, e
 
 
-# config 1 (32x32x33, Lx=2pi, Ly=pi, Re=180, dt=0.002, G=2/Re, analytical mx=my=1 amp 0.01, dealias)
-CONFIG1 = dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
+# the 32x32x33 channel (32x32x33, Lx=2pi, Ly=pi, Re=180, dt=0.002, G=2/Re, analytical mx=my=1 amp 0.01, dealias)
+CHANNEL32 = dict(nx=32, ny=32, nz=33, xlen=2 * np.pi, ylen=np.pi, re=180.0, dt=0.002, use_crank_nicolson=1,
                use_dealias_23_rule=1, flow_control_method=1, target_pressure_gradient=2.0 / 180.0,
                target_bulk_velocity=2.0 / 3.0, controller_gain=0.15, controller_update_freq=7, istart=0,
                u_centerline=1.0, enable_perturbations=1, perturbation_kind=0, analytical_mode_x=1,

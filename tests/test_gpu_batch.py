@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err
+from tests.gpu_util import CHANNEL32, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +46,7 @@ SHAPES = [
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("shape", SHAPES, ids=["32x32x33", "64x96x65"])
 def test_member_alone_equals_member_in_batch(dns, shape, precision):
-    base = {**CONFIG1, **shape, "precision": precision}
+    base = {**CHANNEL32, **shape, "precision": precision}
     mem = _members(shape["nz"])
     _say(f"batch of {len(mem)} at {shape}, precision {precision}")
     batch = dns.Simulation(members=mem, **base)
@@ -68,7 +68,7 @@ def test_member_alone_equals_member_in_batch(dns, shape, precision):
 
 
 def test_batch_eager_equals_graph(dns):
-    base = {**CONFIG1}
+    base = {**CHANNEL32}
     mem = _members(33)
     a = dns.Simulation(members=mem, **base)
     b = dns.Simulation(members=mem, **base)
@@ -82,13 +82,13 @@ def test_batch_eager_equals_graph(dns):
 
 
 def test_batch_members_match_fp64_oracle(dns):
-    """Each member of a batch against its own oracle run (config-1 grid, 40 steps)."""
+    """Each member of a batch against its own oracle run (32x32x33 grid, 40 steps)."""
     mem = _members(33)
-    batch = dns.Simulation(members=mem, **CONFIG1)
+    batch = dns.Simulation(members=mem, **CHANNEL32)
     batch.init_fields()
     oracles = []
     for m in mem:
-        o = orc.Oracle(**{**CONFIG1, **m})
+        o = orc.Oracle(**{**CHANNEL32, **m})
         o.init_fields()
         oracles.append(o)
     # identical initial fields: the oracle's own initial condition
@@ -105,7 +105,7 @@ def test_batch_members_match_fp64_oracle(dns):
 
 
 def test_member_index_checked(dns):
-    s = dns.Simulation(members=_members(33)[:2], **CONFIG1)
+    s = dns.Simulation(members=_members(33)[:2], **CHANNEL32)
     with pytest.raises(dns.DnsError):
         s.get_fields(member=2)
     with pytest.raises(dns.DnsError):

@@ -161,12 +161,12 @@ def test_estimate_bytes_matches_allocation(dns):
             s.close()
 
 
-def test_config4_ensemble_64_members(dns):
+def test_ensemble_of_64_ragged_members(dns):
     """The 64-member mixed-precision sweep: every member bitwise equal to its run alone."""
     from f90_dns_amd.workloads import ensemble_members
     mem = ensemble_members(64)
     ens = dns.Ensemble(mem)
-    _say(f"config-4 ensemble: {ens.groups} groups, {ens.device_bytes / 2**30:.2f} GiB")
+    _say(f"64-member ensemble: {ens.groups} groups, {ens.device_bytes / 2**30:.2f} GiB")
     ens.init_fields()
     ens.step(8, graph=True)
     _check_bitwise(dns, ens, mem, 8)

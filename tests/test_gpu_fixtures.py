@@ -54,11 +54,11 @@ def _trace_check(F, sim, step, rtol):
         step, d["bulk_error_integral"], row["bulk_error_integral"])
 
 
-def test_config2_fp64_100_and_500_steps(dns):
+def test_channel128_fp64_100_and_500_steps(dns):
     """128 x 128 x 129, PI control, seeded noise: FP64 fields within 1e-10 of the oracle after
     100 steps (w's own error within 10x the oracle's own floor), the energy / PI trace every 10
     steps, and still within 1e-9 after 500 steps."""
-    F = fx.Fixture("config2")
+    F = fx.Fixture("channel128")
     g = dns.Simulation(**F.params)
     g.init_fields()
     done = 0
@@ -77,11 +77,11 @@ def test_config2_fp64_100_and_500_steps(dns):
 
 
 @pytest.mark.parametrize("precision", [1, 2], ids=["mixed", "mixed_f32d"])
-def test_config2_mixed_500_steps(dns, precision):
+def test_channel128_mixed_500_steps(dns, precision):
     """The same run in mixed precision (and with the vorticity's D1 contraction on FP32 MFMA):
     u, v, p within 1e-6 of the FP64 oracle after 500 steps (w on the velocity scale too; w's own
     error is the known mixed-precision miss)."""
-    F = fx.Fixture("config2")
+    F = fx.Fixture("channel128")
     g = dns.Simulation(precision=precision, **F.params)
     g.init_fields()
     done = 0
@@ -122,13 +122,13 @@ def test_ensemble64_members_against_oracle_after_500_steps(dns, precision):
 
 
 @pytest.mark.parametrize("precision", [1, 2], ids=["mixed", "mixed_f32d"])
-def test_served_mix_at_config5_sizes_against_oracle(dns, precision):
+def test_served_mix_at_largest_sweep_sizes_against_oracle(dns, precision):
     """Continuous batching at the served sweep's sizes: a 192 x 192 x 129 job, the 128 x 128 x 129
     production run, a 128 x 96 x 65 job and small jobs arrive while others run; each of the
     three large results is compared with its FP64 oracle fixture at 1e-6 (mixed precision, and
     with the D1 contraction on FP32 MFMA)."""
     from f90_dns_amd.workloads import BASE
-    big = {name: fx.Fixture(name) for name in ("served_192x192x129", "config2", "served_128x96x65")}
+    big = {name: fx.Fixture(name) for name in ("served_192x192x129", "channel128", "served_128x96x65")}
     srv = dns.Server(hbm_budget_bytes=24 << 30, slots_per_group=2)
     ids = {}
     small = [dict(BASE, nx=32, ny=32, nz=65, re=150.0 + 20 * i, target_pressure_gradient=2.0 / (150.0 + 20 * i),
@@ -142,7 +142,7 @@ def test_served_mix_at_config5_sizes_against_oracle(dns, precision):
     srv.submit(120, **small[1])
     for _ in range(10):
         srv.tick()
-    ids["config2"] = srv.submit(100, keep_fields=True, precision=precision, **big["config2"].params)
+    ids["channel128"] = srv.submit(100, keep_fields=True, precision=precision, **big["channel128"].params)
     srv.submit(50, **small[2])
     srv.drain()
     st = srv.stats()
@@ -150,7 +150,7 @@ def test_served_mix_at_config5_sizes_against_oracle(dns, precision):
     assert st["done"] == 6 and st["failed"] == 0 and st["rejected"] == 0 and st["max_running"] >= 3
     for name, jid in ids.items():
         F = big[name]
-        step = 100 if name == "config2" else F.steps()[0]
+        step = 100 if name == "channel128" else F.steps()[0]
         assert srv.status(jid) == ("done", step)
         got, diag = srv.result(jid)
         _check(F, got, step, 1e-6, W_SELF_MIXED, f"served {name} precision {precision}")

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err
+from tests.gpu_util import CHANNEL32, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err
 
 pytestmark = pytest.mark.gpu
 
@@ -37,16 +37,16 @@ def test_fp32_plane_transforms(dns, nx, ny):
     assert err < 2e-6 and err_b < 2e-6
 
 
-def test_mixed_config1_500_steps_against_fp64_oracle(dns):
-    """The mixed path against the FP64 oracle after 500 config-1 steps: u, v, p within 1e-6
+def test_mixed_channel32_500_steps_against_fp64_oracle(dns):
+    """The mixed path against the FP64 oracle after 500 steps of the 32x32x33 channel: u, v, p within 1e-6
     relative L2 and w within 1e-6 of the velocity norm.  w's error relative to its own norm (w
     is ~1e-6 of u here) is a known miss against 1e-6 (about 1.5e-4, from the FP32 product),
     held to W_SELF_MIXED; the FP64 GPU run beside it meets 1e-10 with w at its rounding floor."""
-    o = orc.Oracle(**CONFIG1)
+    o = orc.Oracle(**CHANNEL32)
     o.init_fields()
-    g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **CONFIG1)
+    g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **CHANNEL32)
     g.set_fields(*o.get_fields())
-    g64 = dns.Simulation(**CONFIG1)
+    g64 = dns.Simulation(**CHANNEL32)
     g64.set_fields(*o.get_fields())
     o.step(500)
     g.step(500)
@@ -67,14 +67,14 @@ def _f32_field(sim, which):
 
 
 @pytest.mark.parametrize("kw", [
-    dict(CONFIG1),
+    dict(CHANNEL32),
     dict(nx=64, ny=32, nz=65, xlen=4 * np.pi, ylen=2 * np.pi, re=300.0, dt=0.002, use_dealias_23_rule=1,
          flow_control_method=1, target_pressure_gradient=2.0 / 300.0, u_centerline=1.0, enable_perturbations=1,
          perturbation_kind=1, seed=5, noise_kmax=6, perturbation_amplitude=0.05, lgl_derivative=1),
     dict(nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, re=180.0, dt=0.002, use_dealias_23_rule=1,
          flow_control_method=1, target_pressure_gradient=2.0 / 180.0, u_centerline=1.0, enable_perturbations=1,
          perturbation_kind=1, seed=6, noise_kmax=6, perturbation_amplitude=0.05, lgl_derivative=1)],
-    ids=["config1", "64x32x65", "48x36x129"])
+    ids=["channel32", "64x32x65", "48x36x129"])
 def test_fp32_contraction_vorticity(dns, kw):
     """DNS_PREC_MIXED_F32D: the vorticity's D1 contraction on v_mfma_f32_16x16x4f32 with FP32
     accumulation, against the same stage with FP64 arithmetic (DNS_PREC_MIXED, whose vorticity
@@ -93,12 +93,12 @@ def test_fp32_contraction_vorticity(dns, kw):
         assert 0 < err < 5e-6, (name, err)
 
 
-def test_mixed_f32d_config1_500_steps_against_fp64_oracle(dns):
-    """DNS_PREC_MIXED_F32D against the FP64 oracle after 500 config-1 steps: u, v, p and w on
+def test_mixed_f32d_channel32_500_steps_against_fp64_oracle(dns):
+    """DNS_PREC_MIXED_F32D against the FP64 oracle after 500 steps of the 32x32x33 channel: u, v, p and w on
     the velocity scale within 1e-6 relative L2 (w's own error as for DNS_PREC_MIXED)."""
-    o = orc.Oracle(**CONFIG1)
+    o = orc.Oracle(**CHANNEL32)
     o.init_fields()
-    g = dns.Simulation(precision=dns.DNS_PREC_MIXED_F32D, **CONFIG1)
+    g = dns.Simulation(precision=dns.DNS_PREC_MIXED_F32D, **CHANNEL32)
     g.set_fields(*o.get_fields())
     o.step(500)
     g.step(500, graph=True)

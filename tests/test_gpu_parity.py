@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err, pair, rel_l2
+from tests.gpu_util import CHANNEL32, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err, pair, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -108,11 +108,11 @@ def test_reference_d1_at_nz65_is_rejected_like_zgesv(dns):
     assert e.value.code == 5
 
 
-def test_stage_parity_config1(dns):
+def test_stage_parity_channel32(dns):
     """Each stage of one step against the oracle, on identical fields."""
-    _say("stage parity config 1")
-    o, g = pair(**CONFIG1)
-    nx, ny = CONFIG1["nx"], CONFIG1["ny"]
+    _say("stage parity 32x32x33")
+    o, g = pair(**CHANNEL32)
+    nx, ny = CHANNEL32["nx"], CHANNEL32["ny"]
     # warm both up 3 steps so p, wall gradients and all modes are nonzero
     o.step(3)
     for _ in range(3):
@@ -152,8 +152,8 @@ def test_stage_parity_config1(dns):
 
 def test_fused_step_equals_stages(dns):
     _say("fused vs stage API")
-    _, g1 = pair(**CONFIG1)
-    _, g2 = pair(**CONFIG1)
+    _, g1 = pair(**CHANNEL32)
+    _, g2 = pair(**CHANNEL32)
     for _ in range(4):
         g1.step(1)
         g2.begin_step(); g2.compute_source_terms(); g2.viscous_step(); g2.solve_pressure()
@@ -164,8 +164,8 @@ def test_fused_step_equals_stages(dns):
 
 def test_graph_step_equals_eager(dns):
     _say("graph vs eager")
-    _, g1 = pair(**CONFIG1)
-    _, g2 = pair(**CONFIG1)
+    _, g1 = pair(**CHANNEL32)
+    _, g2 = pair(**CHANNEL32)
     g1.step(5)
     g2.step(2, graph=True)
     g2.synchronize()
@@ -175,11 +175,11 @@ def test_graph_step_equals_eager(dns):
 
 
 @pytest.mark.parametrize("dealias", [1, 0])
-def test_config1_100_steps(dns, dealias):
-    """u, v, w, p within relative L2 1e-10 of the FP64 oracle after 100 steps,
+def test_channel32_100_steps(dns, dealias):
+    """The FP64 parity bar: u, v, w, p within relative L2 1e-10 of the FP64 oracle after 100 steps,
     with the same energy history and max divergence order."""
-    _say(f"config1 100 steps dealias={dealias}")
-    kw = dict(CONFIG1, use_dealias_23_rule=dealias)
+    _say(f"32x32x33 100 steps dealias={dealias}")
+    kw = dict(CHANNEL32, use_dealias_23_rule=dealias)
     o, g = pair(**kw)
     # the oracle's own twin (plane transforms factored otherwise): w's rounding floor of this run
     t = orc.Oracle(fft_variant=1, **kw)
@@ -212,7 +212,7 @@ def test_pi_controller_trace(dns, istart):
     runs it every step on its Euler branch, 14 every step on the trapezoid branch, 3 never
     (G stays the target)."""
     _say(f"PI controller istart={istart}")
-    kw = dict(CONFIG1, flow_control_method=2, perturbation_kind=1, seed=11, noise_kmax=4,
+    kw = dict(CHANNEL32, flow_control_method=2, perturbation_kind=1, seed=11, noise_kmax=4,
               perturbation_amplitude=0.05, istart=istart, controller_update_freq=7)
     o, g = pair(**kw)
     for _ in range(20):
@@ -263,7 +263,7 @@ def test_largest_class_step_shapes(dns, nx, ny):
     96 x 96, 192 x 48, 128 x 96) and of the largest class with other first-stage radices (6912
     values; the step's c2r runs one persistent block per CU there): 4 steps against the oracle."""
     _say(f"{nx}x{ny}x17 4 steps")
-    kw = dict(CONFIG1, nx=nx, ny=ny, nz=17, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2,
+    kw = dict(CHANNEL32, nx=nx, ny=ny, nz=17, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2,
               perturbation_kind=1, seed=7, noise_kmax=6, perturbation_amplitude=0.05)
     o, g = pair(**kw)
     o.step(4); g.step(4)
@@ -272,10 +272,10 @@ def test_largest_class_step_shapes(dns, nx, ny):
     check_parity(errs, 1e-10, W_SELF_FP64)
 
 
-@pytest.mark.parametrize("kw", [dict(CONFIG1),
-                                dict(CONFIG1, nx=48, ny=36, nz=33, flow_control_method=2, perturbation_kind=1,
+@pytest.mark.parametrize("kw", [dict(CHANNEL32),
+                                dict(CHANNEL32, nx=48, ny=36, nz=33, flow_control_method=2, perturbation_kind=1,
                                      seed=4, noise_kmax=5, perturbation_amplitude=0.05, use_dealias_23_rule=0),
-                                dict(CONFIG1, nx=64, ny=32, nz=65, lgl_derivative=1, use_crank_nicolson=0,
+                                dict(CHANNEL32, nx=64, ny=32, nz=65, lgl_derivative=1, use_crank_nicolson=0,
                                      perturbation_kind=1, seed=8, noise_kmax=6, perturbation_amplitude=0.05)])
 def test_gpu_matches_cpu_emulation_of_same_algorithm(dns, kw):
     """Same algebra on both sides (the library's operator matrices): implementation check."""
@@ -301,7 +301,7 @@ def test_packed_mode_space_stages_against_oracle(dns):
     independent coefficients and store the mirrored modes as conjugates: every stage's spectral
     output (all modes, mirrored and self-conjugate ones included) against the oracle's r2c."""
     _say("packed mode space 48x36x129 stages")
-    kw = dict(CONFIG1, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
+    kw = dict(CHANNEL32, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
               flow_control_method=2, perturbation_kind=1, seed=21, noise_kmax=6, perturbation_amplitude=0.05)
     o, g = pair(**kw)
     o.step(2)
@@ -344,7 +344,7 @@ def test_packed_mode_space_stages_against_oracle(dns):
 def test_packed_mode_space_batch_member_equals_alone(dns, precision):
     """Members of a batch at nz = 129 (packed mode space) are bitwise the runs alone, and graph
     replay is bitwise the eager launches."""
-    base = dict(CONFIG1, nx=32, ny=24, nz=129, lgl_derivative=1, precision=precision, perturbation_kind=1,
+    base = dict(CHANNEL32, nx=32, ny=24, nz=129, lgl_derivative=1, precision=precision, perturbation_kind=1,
                 noise_kmax=4, perturbation_amplitude=0.04)
     mem = [dict(re=180.0, seed=3), dict(re=260.0, seed=4, flow_control_method=2)]
     _say(f"packed batch precision {precision}")
@@ -393,7 +393,7 @@ def test_restart_with_pressure_keeps_zero_wall_gradient_for_first_step(dns):
     Helmholtz wall values are zero.  The oracle from the same fields with zero wall gradients
     is the reference; one whose wall gradients follow p differs, so the test can tell."""
     _say("restart with p")
-    kw = dict(CONFIG1, flow_control_method=2, perturbation_kind=1, seed=19, noise_kmax=4,
+    kw = dict(CHANNEL32, flow_control_method=2, perturbation_kind=1, seed=19, noise_kmax=4,
               perturbation_amplitude=0.05)
     src = orc.Oracle(**kw)
     src.init_fields()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, rel_l2
+from tests.gpu_util import CHANNEL32, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -32,10 +32,10 @@ def _pair(dns, **kw):
     return o, g
 
 
-@pytest.mark.parametrize("kw", [dict(CONFIG1),
-                                dict(CONFIG1, nx=48, ny=36, nz=129, lgl_derivative=1, flow_control_method=2,
+@pytest.mark.parametrize("kw", [dict(CHANNEL32),
+                                dict(CHANNEL32, nx=48, ny=36, nz=129, lgl_derivative=1, flow_control_method=2,
                                      perturbation_kind=1, seed=3, noise_kmax=5, perturbation_amplitude=0.05)],
-                         ids=["config1", "48x36x129"])
+                         ids=["channel32", "48x36x129"])
 def test_rk4_convection_against_oracle(dns, kw):
     _say(f"rk4 convection {kw['nx']}x{kw['ny']}x{kw['nz']}")
     o, g = _pair(dns, convection_scheme=1, **kw)
@@ -49,7 +49,7 @@ def test_rk4_convection_against_oracle(dns, kw):
 
 
 def test_rk4_stage_api_graph_and_fused_agree(dns):
-    kw = dict(CONFIG1, convection_scheme=1)
+    kw = dict(CHANNEL32, convection_scheme=1)
     _, a = _pair(dns, **kw)
     _, b = _pair(dns, **kw)
     _, c = _pair(dns, **kw)
@@ -65,7 +65,7 @@ def test_rk4_stage_api_graph_and_fused_agree(dns):
 
 
 def test_rk4_convection_mixed_precision(dns):
-    o, g = _pair(dns, convection_scheme=1, precision=dns.DNS_PREC_MIXED, **CONFIG1)
+    o, g = _pair(dns, convection_scheme=1, precision=dns.DNS_PREC_MIXED, **CHANNEL32)
     o.step(100)
     g.step(100)
     e = field_errors(g.get_fields(), o.get_fields())
@@ -78,7 +78,7 @@ def test_jacgs_pressure_against_oracle(dns, tol, bound):
     """jacgs per real column on the GPU against the oracle's jacgs; with a tight cgstol both
     reach the direct solution, with the reference's default 1e-6 they stop (or skip modes whose
     RHS is below it) at rounding-dependent points, so they agree to that looser level."""
-    kw = dict(CONFIG1, flow_control_method=2, perturbation_kind=1, seed=12, noise_kmax=4,
+    kw = dict(CHANNEL32, flow_control_method=2, perturbation_kind=1, seed=12, noise_kmax=4,
               perturbation_amplitude=0.05, pressure_solver=1, cgstol=tol)
     o, g = _pair(dns, **kw)
     for _ in range(5):
@@ -95,7 +95,7 @@ def test_jacgs_pressure_against_oracle(dns, tol, bound):
 
 
 def test_jacgs_and_rk4_together_and_in_a_batch(dns):
-    base = dict(CONFIG1, convection_scheme=1, pressure_solver=1, cgstol=1e-12)
+    base = dict(CHANNEL32, convection_scheme=1, pressure_solver=1, cgstol=1e-12)
     mem = [dict(re=180.0), dict(re=260.0, flow_control_method=2, perturbation_kind=1, seed=4,
                                 perturbation_amplitude=0.04)]
     batch = dns.Simulation(members=mem, **base)
@@ -112,10 +112,10 @@ def test_jacgs_and_rk4_together_and_in_a_batch(dns):
 
 def test_variants_rejected_where_not_built(dns):
     with pytest.raises(dns.DnsError):
-        dns.Simulation(**dict(CONFIG1, pressure_solver=1, use_weak_form_divergence=1))
+        dns.Simulation(**dict(CHANNEL32, pressure_solver=1, use_weak_form_divergence=1))
     with pytest.raises(dns.DnsError):
-        dns.Simulation(**dict(CONFIG1, pressure_solver=1, precision=dns.DNS_PREC_MIXED))
+        dns.Simulation(**dict(CHANNEL32, pressure_solver=1, precision=dns.DNS_PREC_MIXED))
     srv = dns.Server(hbm_budget_bytes=1 << 30)
     with pytest.raises(dns.DnsError):
-        srv.submit(5, **dict(CONFIG1, convection_scheme=1))
+        srv.submit(5, **dict(CHANNEL32, convection_scheme=1))
     srv.close()

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err, pair, rel_l2
+from tests.gpu_util import CHANNEL32, W_SELF_FP64, W_SELF_MIXED, check_parity, field_errors, max_err, pair, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -54,15 +54,15 @@ def test_weak_trace_matches_reference_stdout(dns, dealias):
 
 WEAK_CASES = {
     "trace32x32x17": (dict(TRACE, use_crank_nicolson=1, use_dealias_23_rule=1), 15),
-    "config1": (dict(CONFIG1, use_weak_form_divergence=1), 10),
-    "noise48x36x33": (dict(CONFIG1, nx=48, ny=36, nz=33, flow_control_method=2, perturbation_kind=1, seed=4,
+    "channel32": (dict(CHANNEL32, use_weak_form_divergence=1), 10),
+    "noise48x36x33": (dict(CHANNEL32, nx=48, ny=36, nz=33, flow_control_method=2, perturbation_kind=1, seed=4,
                            noise_kmax=5, perturbation_amplitude=0.05, use_dealias_23_rule=0,
                            use_weak_form_divergence=1), 10),
-    "exact64x32x65": (dict(CONFIG1, nx=64, ny=32, nz=65, lgl_derivative=1, use_crank_nicolson=0,
+    "exact64x32x65": (dict(CHANNEL32, nx=64, ny=32, nz=65, lgl_derivative=1, use_crank_nicolson=0,
                            perturbation_kind=1, seed=8, noise_kmax=6, perturbation_amplitude=0.05,
                            use_weak_form_divergence=1), 8),
     # nz = 129: the per-mode solves walk the packed mode space
-    "packed48x36x129": (dict(CONFIG1, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
+    "packed48x36x129": (dict(CHANNEL32, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
                              perturbation_kind=1, seed=9, noise_kmax=6, perturbation_amplitude=0.05,
                              use_weak_form_divergence=1), 5),
 }
@@ -87,7 +87,7 @@ def test_weak_pressure_stage_matches_oracle(dns):
     """phi of the stage API (separate launches, mode (0,0) in the Poisson launch) against the
     oracle's, in physical space: the reference transforms phi back with a c2r, which keeps only
     the Hermitian part of the solved coefficients."""
-    kw = dict(CONFIG1, use_weak_form_divergence=1)
+    kw = dict(CHANNEL32, use_weak_form_divergence=1)
     _say("weak form: pressure stage")
     o, g = pair(**kw)
     o.step(3)
@@ -105,7 +105,7 @@ def test_weak_pressure_stage_matches_oracle(dns):
 
 
 def test_weak_fused_step_equals_stages(dns):
-    kw = dict(CONFIG1, use_weak_form_divergence=1)
+    kw = dict(CHANNEL32, use_weak_form_divergence=1)
     _, g1 = pair(**kw)
     _, g2 = pair(**kw)
     for _ in range(4):
@@ -118,7 +118,7 @@ def test_weak_fused_step_equals_stages(dns):
 
 def test_weak_mixed_precision_short(dns):
     """Mixed precision with the weak-form RHS: 10 steps against the FP64 oracle."""
-    kw = dict(CONFIG1, use_weak_form_divergence=1)
+    kw = dict(CHANNEL32, use_weak_form_divergence=1)
     o = orc.Oracle(**kw)
     o.init_fields()
     g = dns.Simulation(precision=dns.DNS_PREC_MIXED, **kw)
@@ -133,12 +133,12 @@ def test_weak_mixed_precision_short(dns):
 def test_weak_members_in_batch_and_ensemble_equal_alone(dns):
     """A weak-form group next to a standard-form group of the same shape in an ensemble, and a
     batch of weak-form members: each member bitwise the same run alone."""
-    base = dict(CONFIG1, use_weak_form_divergence=1)
+    base = dict(CHANNEL32, use_weak_form_divergence=1)
     mem = [dict(re=180.0), dict(re=260.0, perturbation_kind=1, seed=5, perturbation_amplitude=0.03)]
     batch = dns.Simulation(members=mem, **base)
     batch.init_fields()
     batch.step(8, graph=True)
-    ens_members = [dict(base, **mem[0]), dict(CONFIG1, **mem[1]), dict(base, **mem[1])]
+    ens_members = [dict(base, **mem[0]), dict(CHANNEL32, **mem[1]), dict(base, **mem[1])]
     ens = dns.Ensemble(ens_members)
     assert ens.groups == 2
     ens.init_fields()

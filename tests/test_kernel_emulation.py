@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as orc
-from tests.gpu_util import CONFIG1, field_errors, max_err
+from tests.gpu_util import CHANNEL32, field_errors, max_err
 
 import f90_dns_amd
 
@@ -25,14 +25,14 @@ def _run(steps, **kw):
     return o, e
 
 
-def test_emulated_step_matches_oracle_config1():
-    o, e = _run(10, **CONFIG1)
+def test_emulated_step_matches_oracle_channel32():
+    o, e = _run(10, **CHANNEL32)
     errs = field_errors(e.get_fields(), o.get_fields())
     assert max_err(errs) < 1e-12, errs
 
 
 def test_emulated_step_pi_control_and_walls():
-    kw = dict(CONFIG1, nx=16, ny=16, nz=17, flow_control_method=2, perturbation_kind=1, seed=3, noise_kmax=3,
+    kw = dict(CHANNEL32, nx=16, ny=16, nz=17, flow_control_method=2, perturbation_kind=1, seed=3, noise_kmax=3,
               perturbation_amplitude=0.05, use_dealias_23_rule=0)
     o, e = _run(8, **kw)
     errs = field_errors(e.get_fields(), o.get_fields())
@@ -41,7 +41,7 @@ def test_emulated_step_pi_control_and_walls():
 
 
 def test_emulated_step_exact_d1_nz65():
-    kw = dict(CONFIG1, nx=16, ny=16, nz=65, lgl_derivative=1, flow_control_method=2, perturbation_kind=1, seed=9,
+    kw = dict(CHANNEL32, nx=16, ny=16, nz=65, lgl_derivative=1, flow_control_method=2, perturbation_kind=1, seed=9,
               noise_kmax=3, perturbation_amplitude=0.05)
     o, e = _run(4, **kw)
     errs = field_errors(e.get_fields()[:3], o.get_fields()[:3])

@@ -365,13 +365,13 @@ def test_laminar_bulk_velocity_is_two_thirds_of_centreline(nz):
 
 # --- the rounding floor of w -------------------------------------------------------------
 def test_w_rounding_floor_of_the_oracle_itself():
-    """w is ~1e-6 of u in config 1, so any reordering of the arithmetic moves it at the 1e-10
+    """w is ~1e-6 of u in the 32x32x33 channel, so any reordering of the arithmetic moves it at the 1e-10
     level of its own norm.  The oracle against itself with every plane transform factored in
     radix 2 instead of 4, odd factors first (an equivalent transform, other rounding): after 100
     steps u, v, p agree to ~1e-13 while w's own relative difference is ~1e-10."""
-    from tests.gpu_util import CONFIG1, field_errors
-    a = orc.Oracle(**CONFIG1); a.init_fields()
-    b = orc.Oracle(fft_variant=1, **CONFIG1); b.init_fields()
+    from tests.gpu_util import CHANNEL32, field_errors
+    a = orc.Oracle(**CHANNEL32); a.init_fields()
+    b = orc.Oracle(fft_variant=1, **CHANNEL32); b.init_fields()
     a.step(100); b.step(100)
     e = field_errors(b.get_fields(), a.get_fields())
     assert e["u"] < 1e-13 and e["v"] < 1e-12 and e["p"] < 1e-13, e
