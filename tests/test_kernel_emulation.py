@@ -46,3 +46,14 @@ def test_emulated_step_exact_d1_nz65():
     o, e = _run(4, **kw)
     errs = field_errors(e.get_fields()[:3], o.get_fields()[:3])
     assert max_err(errs) < 1e-10, errs
+
+
+@pytest.mark.parametrize("nz,exact", [(18, 0), (48, 1)])
+def test_emulated_step_even_nz(nz, exact):
+    """Even nz: the LGL nodes have no centre node, so the folds have no centre row and the
+    Helmholtz panel pairs (v row c, u row c+1) cover every node pair."""
+    kw = dict(CHANNEL32, nx=16, ny=12, nz=nz, lgl_derivative=exact, flow_control_method=2, perturbation_kind=1,
+              seed=5, noise_kmax=3, perturbation_amplitude=0.05)
+    o, e = _run(4, **kw)
+    errs = field_errors(e.get_fields(), o.get_fields())
+    assert max_err(errs) < 1e-12, errs
