@@ -73,6 +73,30 @@ def test_ensemble_member_equals_member_alone(dns, precision):
     ens.close()
 
 
+@pytest.mark.parametrize("precision,waves", [(0, None), (1, None), (0, (2, 2, 2)), (1, (4, 4, 4))])
+def test_even_nz_members_equal_members_alone(dns, precision, waves):
+    """Members of even nz (no centre node; one-, two- and four-wave contraction tiles) beside an
+    odd one: each is bitwise its run alone, also at other tile widths."""
+    from f90_dns_amd.workloads import BASE
+    shapes = [(32, 32, 18), (48, 36, 24), (32, 64, 48), (64, 32, 64), (48, 36, 128), (32, 32, 33)]
+    mem = []
+    for i, (nx, ny, nz) in enumerate(shapes):
+        re = 120.0 + 50.0 * i
+        mem.append(dict(BASE, nx=nx, ny=ny, nz=nz, re=re, target_pressure_gradient=2.0 / re, seed=31 + i,
+                        perturbation_amplitude=0.01 * (1.5 ** i), precision=precision, lgl_derivative=1,
+                        flow_control_method=2 if i % 2 else 1, target_bulk_velocity=2.0 / 3.0, controller_gain=0.15,
+                        controller_update_freq=7))
+    ens = dns.Ensemble(mem)
+    if waves:
+        ens.set_tile_waves(*waves)
+    ens.init_fields()
+    ens.step(10, graph=True)
+    ens.synchronize()
+    _say(f"even-nz ensemble ({ens.groups} groups), precision {precision}, waves {waves}: 10 steps")
+    _check_bitwise(dns, ens, mem, 10)
+    ens.close()
+
+
 @pytest.mark.parametrize("tuning,waves", [(dict(lanes=2), None), (dict(interleave=1), None),
                                           (dict(promote_fp64=(1, 1, 0)), None),
                                           (dict(lanes=2, promote_fp64=(2, 1, 0), interleave=1), None),
