@@ -242,7 +242,10 @@ FoldedA swizzle_fold(const std::vector<double>& M0, int rows0, const std::vector
     f.ks = (K + 3) / 4; f.ksp = (f.ks + 8) / 8 * 8;
     // a group 0 of 16 k + 1 rows keeps its last row out of the MFMA blocks: it is computed as a
     // dot product (stored after the blocks, 4 ksp entries, zero beyond K)
-    f.extra = (rows0 % 16 == 1) ? 1 : 0;
+    // (only where group 0 holds one row more than group 1, the centre row of an odd fold: the
+    // kernels unfold that row alone, as the centre node whose odd part is zero; an even fold's
+    // groups have equal rows and every row stays in the blocks)
+    f.extra = (rows0 % 16 == 1 && rows0 == rows1 + 1) ? 1 : 0;
     if (f.extra) f.rb0 = rows0 / 16;
     f.pb = 4 * ((std::max(f.rb0, f.rb1) + 3) / 4);
     f.data.assign(size_t(2 * f.pb) * f.ksp * 64 + (f.extra ? size_t(4) * f.ksp : 0), 0.0);

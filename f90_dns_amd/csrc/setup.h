@@ -50,9 +50,10 @@ SwizzledA swizzle_a(const std::vector<double>& rowmajor, int rows, int cols);
 // acts on its own K-row panel.  Block cb (group-0 blocks first, then group 1 from block pb),
 // k-step s: entry [(cb*ksp + s)*64 + lane] = M_g(16*(cb - first_g) + lane%16, 4s + lane/16);
 // ks = ceil(K/4) k-steps are used; ksp (a multiple of 8, >= ks + 1) are stored, the tail zero.
-// When rows0 = 16 k + 1 (the centre row or the extra even eigenvector of an odd column), that
-// last group-0 row is not in the blocks (rb0 = k) but stored dense after them (extra = 1,
-// 4 ksp entries) and computed as a dot product, so the row-block pairs stay k per group.
+// When rows0 = 16 k + 1 = rows1 + 1 (the centre row or the extra even eigenvector of an odd
+// column), that last group-0 row is not in the blocks (rb0 = k) but stored dense after them
+// (extra = 1, 4 ksp entries) and computed as a dot product, so the row-block pairs stay k per
+// group.  Even folds (rows0 = rows1, no centre row) keep every row in the blocks.
 // Both groups are padded with zero blocks to pb = 4 ceil(max(rb0, rb1) / 4) blocks, so each of
 // a block's 4 waves loads its pairs unconditionally; group 1 starts at block pb.
 struct FoldedA {

@@ -75,10 +75,11 @@ def test_ensemble_member_equals_member_alone(dns, precision):
 
 @pytest.mark.parametrize("precision,waves", [(0, None), (1, None), (0, (2, 2, 2)), (1, (4, 4, 4))])
 def test_even_nz_members_equal_members_alone(dns, precision, waves):
-    """Members of even nz (no centre node; one-, two- and four-wave contraction tiles) beside an
-    odd one: each is bitwise its run alone, also at other tile widths."""
+    """Members of even nz (no centre node; one- to four-wave contraction tiles, two row-block
+    pairs per wave at nz = 130) beside an odd one: each is bitwise its run alone, also at other
+    tile widths."""
     from f90_dns_amd.workloads import BASE
-    shapes = [(32, 32, 18), (48, 36, 24), (32, 64, 48), (64, 32, 64), (48, 36, 128), (32, 32, 33)]
+    shapes = [(32, 32, 18), (48, 36, 34), (32, 64, 48), (64, 32, 66), (48, 36, 130), (32, 32, 33), (48, 32, 128)]
     mem = []
     for i, (nx, ny, nz) in enumerate(shapes):
         re = 120.0 + 50.0 * i

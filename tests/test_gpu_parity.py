@@ -272,10 +272,28 @@ def test_largest_class_step_shapes(dns, nx, ny):
     check_parity(errs, 1e-10, W_SELF_FP64)
 
 
-@pytest.mark.parametrize("nz,exact", [(18, 0), (24, 0), (48, 1), (64, 1), (128, 1)])
+@pytest.mark.parametrize("nz", [81, 97, 130, 132, 144])
+def test_three_wave_and_two_pair_tiles(dns, nz):
+    """Folded operators of three row-block pairs (nz = 81, 97: three-wave blocks, one wave per
+    pair) and of five to eight (nz = 130 to 144: four waves with two pairs each; 144 is the
+    largest nz accepted): 10 FP64 steps against the oracle."""
+    _say(f"48x36x{nz} 10 steps")
+    kw = dict(CHANNEL32, nx=48, ny=36, nz=nz, lgl_derivative=1, flow_control_method=2, perturbation_kind=1,
+              seed=13, noise_kmax=5, perturbation_amplitude=0.05)
+    o, g = pair(**kw)
+    o.step(10); g.step(10)
+    errs = field_errors(g.get_fields(), o.get_fields())
+    _say(f"  48x36x{nz} after 10 steps {errs}")
+    check_parity(errs, 1e-10, W_SELF_FP64)
+
+
+@pytest.mark.parametrize("nz,exact", [(18, 0), (24, 0), (34, 1), (36, 1), (48, 1), (64, 1), (66, 1), (68, 1),
+                                      (128, 1)])
 def test_even_nz_steps(dns, nz, exact):
-    """Even nz (no centre node: the folded operators have no centre row; one-, two- and
-    four-wave contraction kernels): 10 FP64 steps against the oracle."""
+    """Even nz (no centre node: the folded operators have no centre row; one-, two-, three- and
+    four-wave contraction kernels), including halves of 16 k + 1 rows (nz = 34, 66; the
+    Helmholtz interior at nz = 36, 68), which once lost the odd part of their last row: 10 FP64
+    steps against the oracle."""
     _say(f"48x36x{nz} exact={exact} 10 steps")
     kw = dict(CHANNEL32, nx=48, ny=36, nz=nz, lgl_derivative=exact, flow_control_method=2, perturbation_kind=1,
               seed=12, noise_kmax=5, perturbation_amplitude=0.05)
