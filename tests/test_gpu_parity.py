@@ -23,8 +23,11 @@ def dns():
 
 
 # 192x192, 192x96 and 256x128 exceed the LDS-resident FP64 plane and take the two-pass transforms
+SMALL_AND_RADIX3 = [(8, 4), (12, 6), (18, 18), (24, 54), (54, 24), (162, 48)]
+
+
 @pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (96, 96), (128, 128), (128, 32), (96, 64), (48, 36),
-                                   (192, 192), (192, 96), (256, 128)])
+                                   (192, 192), (192, 96), (256, 128)] + SMALL_AND_RADIX3)
 def test_fft_forward_matches_numpy(dns, nx, ny):
     _say(f"fft forward {nx}x{ny}")
     rng = np.random.default_rng(nx + ny)
@@ -35,7 +38,7 @@ def test_fft_forward_matches_numpy(dns, nx, ny):
 
 
 @pytest.mark.parametrize("nx,ny", [(32, 32), (64, 64), (96, 96), (128, 128), (128, 32), (48, 36), (192, 192),
-                                   (192, 96), (256, 128)])
+                                   (192, 96), (256, 128)] + SMALL_AND_RADIX3)
 def test_fft_backward_non_hermitian_matches_oracle(dns, nx, ny):
     _say(f"fft backward {nx}x{ny}")
     rng = np.random.default_rng(7 + nx)
@@ -270,6 +273,29 @@ def test_largest_class_step_shapes(dns, nx, ny):
     errs = field_errors(g.get_fields(), o.get_fields())
     _say(f"  {nx}x{ny}x17 after 4 steps {errs}")
     check_parity(errs, 1e-10, W_SELF_FP64)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nx,ny,nz", [(8, 4, 5), (8, 6, 9), (12, 6, 9), (18, 18, 17), (24, 54, 17), (54, 24, 17),
+                                      (162, 48, 9)])
+def test_small_and_radix3_grids(dns, nx, ny, nz, precision):
+    """The smallest grid accepted (8 x 4 x 5), planes far smaller than a wavefront's share, and
+    plane lengths with several radix-3 stages (nx/2 = 9, 27, 81; ny = 27 x 2): 4 steps against
+    the FP64 oracle, FP64 at 1e-10 and mixed precision at 1e-6."""
+    _say(f"{nx}x{ny}x{nz} precision {precision} 4 steps")
+    kw = dict(CHANNEL32, nx=nx, ny=ny, nz=nz, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2,
+              perturbation_kind=1, seed=21, noise_kmax=2, perturbation_amplitude=0.05)
+    o = orc.Oracle(**kw)
+    o.init_fields()
+    g = dns.Simulation(precision=precision, **kw)
+    g.set_fields(*o.get_fields())
+    o.step(4); g.step(4)
+    errs = field_errors(g.get_fields(), o.get_fields())
+    _say(f"  {nx}x{ny}x{nz} after 4 steps {errs}")
+    if precision == 0:
+        check_parity(errs, 1e-10, W_SELF_FP64)
+    else:
+        check_parity(errs, 1e-6, W_SELF_MIXED)
 
 
 @pytest.mark.parametrize("nz", [81, 97, 130, 132, 144])
