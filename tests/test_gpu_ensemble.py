@@ -73,7 +73,8 @@ def test_ensemble_member_equals_member_alone(dns, precision):
     ens.close()
 
 
-@pytest.mark.parametrize("precision,waves", [(0, None), (1, None), (0, (2, 2, 2)), (1, (4, 4, 4))])
+@pytest.mark.parametrize("precision,waves", [(0, None), (1, None), (2, None), (0, (2, 2, 2)), (1, (4, 4, 4)),
+                                             (2, (2, 2, 2))])
 def test_even_nz_members_equal_members_alone(dns, precision, waves):
     """Members of even nz (no centre node; one- to four-wave contraction tiles, two row-block
     pairs per wave at nz = 130) beside an odd one: each is bitwise its run alone, also at other

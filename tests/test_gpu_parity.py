@@ -330,6 +330,24 @@ def test_even_nz_steps(dns, nz, exact):
     check_parity(errs, 1e-10, W_SELF_FP64)
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("nz", [18, 34, 64, 66, 128, 130])
+def test_even_nz_mixed_precision(dns, nz, precision):
+    """Mixed precision at even nz, with the vorticity's D1 in FP64 arithmetic (1) and on FP32
+    MFMA (2, whose result layout differs): 10 steps within 1e-6 of the FP64 oracle."""
+    _say(f"48x36x{nz} precision {precision} 10 steps")
+    kw = dict(CHANNEL32, nx=48, ny=36, nz=nz, lgl_derivative=1, flow_control_method=2, perturbation_kind=1,
+              seed=14, noise_kmax=5, perturbation_amplitude=0.05)
+    o = orc.Oracle(**kw)
+    o.init_fields()
+    g = dns.Simulation(precision=precision, **kw)
+    g.set_fields(*o.get_fields())
+    o.step(10); g.step(10)
+    errs = field_errors(g.get_fields(), o.get_fields())
+    _say(f"  48x36x{nz} precision {precision} after 10 steps {errs}")
+    check_parity(errs, 1e-6, W_SELF_MIXED)
+
+
 @pytest.mark.parametrize("kw", [dict(CHANNEL32),
                                 dict(CHANNEL32, nx=48, ny=36, nz=33, flow_control_method=2, perturbation_kind=1,
                                      seed=4, noise_kmax=5, perturbation_amplitude=0.05, use_dealias_23_rule=0),
