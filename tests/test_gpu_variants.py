@@ -34,8 +34,12 @@ def _pair(dns, **kw):
 
 @pytest.mark.parametrize("kw", [dict(CHANNEL32),
                                 dict(CHANNEL32, nx=48, ny=36, nz=129, lgl_derivative=1, flow_control_method=2,
-                                     perturbation_kind=1, seed=3, noise_kmax=5, perturbation_amplitude=0.05)],
-                         ids=["channel32", "48x36x129"])
+                                     perturbation_kind=1, seed=3, noise_kmax=5, perturbation_amplitude=0.05),
+                                dict(CHANNEL32, nx=48, ny=36, nz=34, lgl_derivative=1, flow_control_method=2,
+                                     perturbation_kind=1, seed=17, noise_kmax=5, perturbation_amplitude=0.05),
+                                dict(CHANNEL32, nx=48, ny=36, nz=140, lgl_derivative=1, flow_control_method=2,
+                                     perturbation_kind=1, seed=18, noise_kmax=5, perturbation_amplitude=0.05)],
+                         ids=["channel32", "48x36x129", "even48x36x34", "pairs48x36x140"])
 def test_rk4_convection_against_oracle(dns, kw):
     _say(f"rk4 convection {kw['nx']}x{kw['ny']}x{kw['nz']}")
     o, g = _pair(dns, convection_scheme=1, **kw)

@@ -65,6 +65,12 @@ WEAK_CASES = {
     "packed48x36x129": (dict(CHANNEL32, nx=48, ny=36, nz=129, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1,
                              perturbation_kind=1, seed=9, noise_kmax=6, perturbation_amplitude=0.05,
                              use_weak_form_divergence=1), 5),
+    # even nz (no centre node), and four-wave tiles with two row-block pairs per wave
+    "even48x36x66": (dict(CHANNEL32, nx=48, ny=36, nz=66, lgl_derivative=1, flow_control_method=2,
+                          perturbation_kind=1, seed=15, noise_kmax=5, perturbation_amplitude=0.05,
+                          use_weak_form_divergence=1), 8),
+    "pairs48x36x137": (dict(CHANNEL32, nx=48, ny=36, nz=137, lgl_derivative=1, perturbation_kind=1, seed=16,
+                            noise_kmax=5, perturbation_amplitude=0.05, use_weak_form_divergence=1), 5),
 }
 
 
