@@ -68,6 +68,27 @@ def test_rk4_stage_api_graph_and_fused_agree(dns):
         np.testing.assert_array_equal(z, x)
 
 
+@pytest.mark.parametrize("nz", [34, 66, 130])
+def test_stage_api_matches_oracle_at_even_nz(dns, nz):
+    """The reference's stage sequence (every plane transformed, the Poisson solve and the
+    correction as separate stages) at even nz, where the folds have no centre row, including
+    two row-block pairs per wave (nz = 130): 3 steps against the oracle and the fused step."""
+    kw = dict(CHANNEL32, nx=48, ny=36, nz=nz, lgl_derivative=1, flow_control_method=2, perturbation_kind=1, seed=19,
+              noise_kmax=5, perturbation_amplitude=0.05)
+    o, a = _pair(dns, **kw)
+    _, b = _pair(dns, **kw)
+    o.step(3)
+    a.step(3)
+    for _ in range(3):
+        b.begin_step(); b.compute_source_terms(); b.viscous_step(); b.solve_pressure()
+        b.pressure_correction(); b.apply_boundary_conditions()
+    e = field_errors(b.get_fields(), o.get_fields())
+    _say(f"stage api 48x36x{nz} after 3 steps {e}")
+    check_parity(e, 1e-10, W_SELF_FP64)
+    for x, y in zip(a.get_fields(), b.get_fields()):
+        assert rel_l2(y, x) < 1e-13
+
+
 def test_rk4_convection_mixed_precision(dns):
     o, g = _pair(dns, convection_scheme=1, precision=dns.DNS_PREC_MIXED, **CHANNEL32)
     o.step(100)
