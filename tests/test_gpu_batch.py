@@ -40,11 +40,14 @@ def _members(nz):
 SHAPES = [
     dict(nx=32, ny=32, nz=33),
     dict(nx=64, ny=96, nz=65, xlen=4 * np.pi, ylen=2 * np.pi, lgl_derivative=1),
+    # even nz (no centre node) and two row-block pairs per wave
+    dict(nx=48, ny=36, nz=66, lgl_derivative=1),
+    dict(nx=48, ny=36, nz=130, lgl_derivative=1),
 ]
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("shape", SHAPES, ids=["32x32x33", "64x96x65"])
+@pytest.mark.parametrize("shape", SHAPES, ids=["32x32x33", "64x96x65", "48x36x66", "48x36x130"])
 def test_member_alone_equals_member_in_batch(dns, shape, precision):
     base = {**CHANNEL32, **shape, "precision": precision}
     mem = _members(shape["nz"])
