@@ -277,14 +277,15 @@ def test_largest_class_step_shapes(dns, nx, ny):
 
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("nx,ny,nz", [(8, 4, 5), (8, 6, 9), (12, 6, 9), (18, 18, 17), (24, 54, 17), (54, 24, 17),
-                                      (162, 48, 9)])
+                                      (162, 48, 9), (8, 4, 129), (12, 6, 100), (18, 18, 144)])
 def test_small_and_radix3_grids(dns, nx, ny, nz, precision):
-    """The smallest grid accepted (8 x 4 x 5), planes far smaller than a wavefront's share, and
-    plane lengths with several radix-3 stages (nx/2 = 9, 27, 81; ny = 27 x 2): 4 steps against
-    the FP64 oracle, FP64 at 1e-10 and mixed precision at 1e-6."""
+    """The smallest grid accepted (8 x 4 x 5), planes far smaller than a wavefront's share,
+    plane lengths with several radix-3 stages (nx/2 = 9, 27, 81; ny = 27 x 2), and tiny planes
+    under the four-wave solves' packed mode space (nz = 100 to 144): 4 steps against the FP64
+    oracle, FP64 at 1e-10 and mixed precision at 1e-6."""
     _say(f"{nx}x{ny}x{nz} precision {precision} 4 steps")
     kw = dict(CHANNEL32, nx=nx, ny=ny, nz=nz, xlen=4 * np.pi, ylen=2 * np.pi, flow_control_method=2,
-              perturbation_kind=1, seed=21, noise_kmax=2, perturbation_amplitude=0.05)
+              perturbation_kind=1, seed=21, noise_kmax=2, perturbation_amplitude=0.05, lgl_derivative=int(nz > 33))
     o = orc.Oracle(**kw)
     o.init_fields()
     g = dns.Simulation(precision=precision, **kw)
