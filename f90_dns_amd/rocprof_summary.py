@@ -7,7 +7,10 @@ number of bytes each way).  The machine-readable lines `traffic <step-kernel> <b
 what bench.py reads for its roofline entry.
 
     python -m f90_dns_amd.rocprof_summary --trace T.db [--fetch F.db --write W.db] \\
-        --title "..." --command "..." > profiles/NAME.md
+        [--mfma M.db] --title "..." --command "..." > profiles/NAME.md
+
+MFMA utilisation: SQ_VALU_MFMA_BUSY_CYCLES per dispatch (summed over the SIMDs) from a --pmc run,
+divided by SIMDs x engine clock x the dispatch's mean duration from the kernel trace.
 """
 import argparse
 import collections
@@ -61,6 +64,9 @@ def main():
     ap.add_argument("--trace", required=True)
     ap.add_argument("--fetch")
     ap.add_argument("--write")
+    ap.add_argument("--mfma", help="--pmc SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CYCLES result database")
+    ap.add_argument("--simds", type=int, default=1024, help="SIMDs of the device (256 CUs x 4)")
+    ap.add_argument("--clock-ghz", type=float, default=2.4, help="peak engine clock")
     ap.add_argument("--title", default="kernel statistics")
     ap.add_argument("--command", default="")
     args = ap.parse_args()
@@ -105,6 +111,20 @@ def main():
         for l in lines:
             print(l)
         print("```")
+
+    if args.mfma:
+        mb = counter_table(args.mfma, "SQ_VALU_MFMA_BUSY_CYCLES")
+        print("\n## MFMA utilisation per dispatch\n")
+        print(f"SQ_VALU_MFMA_BUSY_CYCLES / ({args.simds} SIMDs x {args.clock_ghz} GHz x mean duration):\n")
+        print("| kernel | MFMA busy cycles (sum over SIMDs) | mean us | MFMA busy fraction |")
+        print("|---|---|---|---|")
+        for name, v in rows:
+            if name not in mb or "rocclr" in name or "k_copy" in name:
+                continue
+            busy = sum(mb[name]) / len(mb[name])
+            dur_us = sum(v) / len(v) / 1e3
+            frac = busy / (args.simds * args.clock_ghz * 1e3 * dur_us)
+            print(f"| `{name}` | {busy:.3e} | {dur_us:.1f} | {frac:.2f} |")
 
 
 if __name__ == "__main__":
